@@ -1,0 +1,173 @@
+"""Throughput of the IMRPhenomXAS hot path on one MI355X.
+
+A step is one pass of the hot path over one batch: the per-binary setup
+kernel and the per-(binary, frequency) evaluation kernel for 65,536 seeded
+binaries on linspace(20, 1024, 4017) Hz, FP32 evaluation with the
+inspiral's leading term split off (FP64 setup), complex64 outputs written
+to HBM. Inputs are resident on the device before the timed region.
+
+Printed (one JSON line): waveforms/s of that step, the same for the
+all-FP64 path and the double-float variant, the mismatch of the split path
+against the FP64 path on the device, the dominant kernel's HBM roofline
+(algorithmic bytes = 16 B per bin written, over the kernel's own
+event-timed duration), and the CPU baseline: the FP64 oracle timed on the
+host cores on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "XAS waveforms/s @4017 bins: FP32-split vs FP64 GPU (mismatch<=1e-6); fwd+grad/s"
+
+
+def draw(n, seed=1):
+    """The workload's binaries (B, 8): m1 ~ U[10, 80], q ~ U[0.125, 1],
+    chi ~ U[-0.9, 0.9], d_L ~ U[100, 2000] Mpc, inclination = arccos U[-1, 1],
+    phi_ref ~ U[0, 2 pi], f_ref = 20 Hz."""
+    rng = np.random.default_rng(seed)
+    m1 = rng.uniform(10.0, 80.0, n)
+    q = rng.uniform(0.125, 1.0, n)
+    c1 = rng.uniform(-0.9, 0.9, n)
+    c2 = rng.uniform(-0.9, 0.9, n)
+    dl = rng.uniform(100.0, 2000.0, n)
+    inc = np.arccos(rng.uniform(-1.0, 1.0, n))
+    phi = rng.uniform(0.0, 2 * np.pi, n)
+    return np.stack([m1, m1 * q, c1, c2, dl, inc, phi, np.full(n, 20.0)], axis=1)
+
+
+def timed(torch, fn, steps, warmup):
+    """Average ms of ``fn`` over ``steps`` calls after ``warmup``, by events
+    on the current stream."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+def cpu_baseline(n_binaries, freqs):
+    """The FP64 oracle on the host cores on ``n_binaries`` of the workload:
+    median of 3 timed runs after a warm-up."""
+    from oracle import oracle
+
+    cores = min(16, os.cpu_count() or 1)
+    params = draw(n_binaries, seed=2)
+    oracle.polarizations(params[:8], freqs, threads=cores)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.polarizations(params, freqs, threads=cores)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"value": n_binaries / t, "unit": "waveforms/s", "cores": cores, "kind": "port",
+            "sample": f"{n_binaries} binaries x {len(freqs)} bins, FP64 C++ oracle, OpenMP over binaries, "
+                      f"median of 3 ({t:.2f} s each)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--binaries", type=int, default=65536)
+    ap.add_argument("--bins", type=int, default=4017)
+    ap.add_argument("--group", type=int, default=0)
+    ap.add_argument("--cpu-binaries", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the binaries are independent, so the path shards by binary with no "
+              "exchange; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from waveformx_amd import xas
+
+    dev = torch.device("cuda", 0)
+    nb, n = args.binaries, args.bins
+    freqs64 = np.linspace(20.0, 1024.0, n)
+    params = torch.as_tensor(draw(nb), device=dev)
+    f32 = torch.as_tensor(freqs64, dtype=torch.float32, device=dev)
+    f64 = torch.as_tensor(freqs64, dtype=torch.float64, device=dev)
+    out32 = (torch.empty((nb, n), dtype=torch.complex64, device=dev),
+             torch.empty((nb, n), dtype=torch.complex64, device=dev))
+    out64 = (torch.empty((nb, n), dtype=torch.complex128, device=dev),
+             torch.empty((nb, n), dtype=torch.complex128, device=dev))
+    rec = {}
+
+    def step(precision, freqs, out):
+        def run():
+            records, _ = xas.setup(params, freqs[:1], 1e21, None, precision)
+            rec[precision] = records
+            xas.evaluate(records, freqs, precision, args.group, out=out)
+        return run
+
+    ms_split = timed(torch, step("split", f32, out32), args.steps, args.warmup)
+    ms_df = timed(torch, step("split-df", f32, out32), args.steps, args.warmup)
+    ms_fp64 = timed(torch, step("fp64", f64, out64), max(3, args.steps // 2), 1)
+    # the dominant kernel alone (the evaluation), on the stream it runs on
+    ms_eval = timed(torch, lambda: xas.evaluate(rec["split"], f32, "split", args.group, out=out32), args.steps,
+                    args.warmup)
+    ms_setup = timed(torch, lambda: xas.setup(params, f32[:1], 1e21, None, "split"), args.steps, args.warmup)
+
+    # accuracy of the split path against the FP64 path on the device
+    step("split", f32, out32)()
+    step("fp64", f64, out64)()
+    w = 1.0 / torch.as_tensor(((freqs64 / 100.0) ** -4 + 1.0 + (freqs64 / 100.0) ** 2), device=dev)
+    worst = 0.0
+    for a, b in ((out32[0], out64[0]), (out32[1], out64[1])):
+        a = a.to(torch.complex128)
+        ab = (a * b.conj()).real.mul(w).sum(1)
+        aa = a.abs().square().mul(w).sum(1)
+        bb = b.abs().square().mul(w).sum(1)
+        worst = max(worst, float((1.0 - ab / (aa * bb).sqrt()).max()))
+    torch.cuda.synchronize()
+
+    bytes_per_launch = 16.0 * nb * n
+    achieved = bytes_per_launch / (ms_eval * 1e-3) / 1e9
+    result = {
+        "metric": METRIC,
+        "value": nb / (ms_split * 1e-3),
+        "unit": "waveforms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_split,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (leading phase term f64)",
+        "data": "synthetic: seeded parameter draws",
+        "config": {"workload": f"IMRPhenomXAS {nb} binaries x {n} bins linspace(20, 1024) Hz, FP32-split forward "
+                               "(setup + evaluation)", "binaries": nb, "bins": n},
+        "fp64_value": nb / (ms_fp64 * 1e-3),
+        "fp64_ms_per_step": ms_fp64,
+        "split_over_fp64": ms_fp64 / ms_split,
+        "split_df_value": nb / (ms_df * 1e-3),
+        "eval_ms": ms_eval,
+        "setup_ms": ms_setup,
+        "max_weighted_mismatch_vs_fp64": worst,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(args.cpu_binaries, freqs64)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

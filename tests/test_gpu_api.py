@@ -1,0 +1,116 @@
+"""The drop-in API on the GPU: the torch function's input forms, dtypes,
+shapes and errors (the reference's tests/test_torch.py:119-145), and
+batch_polarizations' ok/errors (tests/test_xas.py:103-139)."""
+
+import numpy as np
+import pytest
+
+from conftest import compare
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+ALIGNED = [36.0, 29.0, 0.0, 0.0, 0.3, 0.0, 0.0, -0.4, 500.0, 0.7, 1.2, 25.0]
+FREQUENCIES = np.geomspace(20.0, 900.0, 300)
+
+
+def row_params(row):
+    row = np.asarray(row)
+    return row[..., [0, 1, 4, 7, 8, 9, 10, 11]]
+
+
+def test_single_and_batch_forms(cuda):
+    import torch
+
+    import waveformx_amd as wx
+    import waveformx_amd.torch as wt
+
+    f = wt.polarizations_function("IMRPhenomXAS")
+    x = torch.tensor(ALIGNED, dtype=torch.float64)
+    binary = wx.Binary(x[0], x[1], x[2:5], x[5:8], x[8], x[9], x[10], x[11])
+    hp, hc = f(binary, torch.tensor(FREQUENCIES))
+    assert hp.shape == (len(FREQUENCIES),) and hp.dtype == torch.complex128
+    assert hp.device.type == "cpu"  # results come back to the grid's device
+    op, oc = oracle.polarizations(row_params(ALIGNED)[None], FREQUENCIES)
+    assert compare(hp.numpy(), op[0])[0] < 1e-12
+    assert compare(hc.numpy(), oc[0])[0] < 1e-12
+
+    mapping = dict(mass_1=36.0, mass_2=29, spin_1=(0.0, 0.0, 0.3), spin_2=[0.0, 0.0, -0.4],
+                   luminosity_distance=500.0, inclination=0.7, phi_ref=1.2, f_ref=25.0)
+    hp2, _ = f(mapping, FREQUENCIES)
+    assert torch.equal(hp2, hp)
+
+    masses = torch.tensor([36.0, 40.0], dtype=torch.float64, device=cuda)
+    hp3, hc3 = f({**mapping, "mass_1": masses}, torch.tensor(FREQUENCIES, device=cuda))
+    assert hp3.shape == (2, len(FREQUENCIES)) and hp3.device.type == "cuda"
+    assert compare(hp3[0].cpu().numpy(), op[0])[0] < 1e-12
+
+    with pytest.raises(TypeError, match="unknown"):
+        f({**mapping, "mass": 1.0}, FREQUENCIES)
+    with pytest.raises(ValueError, match="batch"):
+        f({**mapping, "mass_1": masses, "mass_2": torch.ones(3, dtype=torch.float64)}, FREQUENCIES)
+    with pytest.raises(wx.UnsupportedError):
+        wt.polarizations_function("NRSur7dq4")
+    with pytest.raises(wx.UnknownApproximantError):
+        wt.polarizations_function("IMRPhenomZZZ")
+
+
+def test_float32_inputs_give_complex64(cuda):
+    import torch
+
+    import waveformx_amd.torch as wt
+
+    f = wt.polarizations_function("IMRPhenomXAS")
+    x = torch.tensor(ALIGNED, dtype=torch.float32, device=cuda)
+    grid = torch.tensor(FREQUENCIES, dtype=torch.float32, device=cuda)
+    hp, hc = f((x[0], x[1], x[2:5], x[5:8], x[8], x[9], x[10], x[11]), grid)
+    assert hp.dtype == torch.complex64
+    rounded = row_params(np.float32(ALIGNED).astype(np.float64))[None]
+    op, _ = oracle.polarizations(rounded, np.float32(FREQUENCIES).astype(np.float64))
+    assert compare(hp.cpu().numpy().astype(np.complex128), op[0])[1] < 1e-6
+
+
+def test_scaled_strain_and_reference_mf(cuda):
+    import torch
+
+    import waveformx_amd.torch as wt
+
+    grid = torch.tensor(FREQUENCIES, device=cuda)
+    x = torch.tensor(ALIGNED, dtype=torch.float64, device=cuda)
+    binary = (x[0], x[1], x[2:5], x[5:8], x[8], x[9], x[10], x[11])
+    scaled, _ = wt.polarizations_function("IMRPhenomXAS")(binary, grid)
+    raw, _ = wt.polarizations_function("IMRPhenomXAS", scaled_strain=False)(binary, grid)
+    assert torch.allclose(raw * 1e21, scaled, rtol=1e-15, atol=0)
+    # reference_mf: f_ref must be zero (NaN otherwise), then it is reference_mf / M
+    nan_hp, _ = wt.polarizations_function("IMRPhenomXAS", reference_mf=0.01)(binary, grid)
+    assert torch.isnan(nan_hp).all()
+    zero_ref = (x[0], x[1], x[2:5], x[5:8], x[8], x[9], x[10], torch.zeros((), dtype=torch.float64, device=cuda))
+    hp, _ = wt.polarizations_function("IMRPhenomXAS", reference_mf=0.01)(zero_ref, grid)
+    f_ref = 0.01 / (4.925490947641266978197229498498379006e-6 * 65.0)
+    p = row_params(ALIGNED).copy()
+    p[7] = f_ref
+    op, _ = oracle.polarizations(p[None], FREQUENCIES)
+    assert compare(hp.cpu().numpy(), op[0])[0] < 1e-11
+
+
+def test_batch_polarizations_errors(cuda):
+    import waveformx_amd as wx
+
+    f = np.geomspace(20.0, 1024.0, 700)
+    table = np.array([ALIGNED,
+                      [2000.0, 1.0] + ALIGNED[2:],                       # q > 1000: LAL refuses
+                      [36.0, 29.0, 0.1, 0.0, 0.3] + ALIGNED[5:],         # in-plane spin
+                      [-1.0, 29.0] + ALIGNED[2:],                        # negative mass
+                      [50.0, 8.0, 0.0, 0.0, 0.9, 0.0, 0.0, 0.3, 300.0, 2.1, 0.3, 30.0]])
+    batch = wx.batch_polarizations("IMRPhenomXAS", f, table)
+    assert list(batch.ok) == [True, False, False, False, True]
+    assert "1000" in str(batch.errors[1]) and "in-plane" in str(batch.errors[2])
+    assert "positive" in str(batch.errors[3])
+    hp = batch.hplus.cpu().numpy()
+    assert np.all(hp[1:4] == 0)
+    op, _ = oracle.polarizations(row_params(table[[0, 4]]), f)
+    assert compare(hp[0], op[0])[0] < 1e-12 and compare(hp[4], op[1])[0] < 1e-12
+    with pytest.raises(wx.InvalidParameterError):
+        wx.batch_polarizations("IMRPhenomXAS", f, table, strict=True)
+    chunked = wx.batch_polarizations("IMRPhenomXAS", f, table, chunk_size=2)
+    assert np.array_equal(chunked.hplus.cpu().numpy(), hp)

@@ -1,0 +1,29 @@
+"""waveformx_amd: IMRPhenomXAS frequency-domain polarizations on MI355X.
+
+A drop-in for the reference's PyTorch path (``waveformx.torch.
+polarizations_function``, ``waveformx.batch_polarizations``): the same
+:class:`Binary` and options in, the same ``(hplus, hcross)`` out, computed
+by hand-written HIP kernels for gfx950 (``libwfx_xas.so``, C ABI in
+``include/waveformx_amd.h``). There is no CPU path.
+"""
+
+from ._api import APPROXIMANTS, approximants, batch_polarizations, compute_device, ragged_polarizations
+from ._types import (BINARY_NAMES, COLUMNS, REFERENCE_MF, BatchPolarizations, Binary, FrequencyGrid,
+                     WaveformOptions, check_frequencies, parameter_table)
+from .exceptions import (InvalidParameterError, ModelError, UnknownApproximantError, UnsupportedError,
+                         WaveformError)
+
+__all__ = [
+    "APPROXIMANTS", "approximants", "batch_polarizations", "ragged_polarizations", "compute_device", "Binary",
+    "BINARY_NAMES", "COLUMNS", "REFERENCE_MF", "BatchPolarizations", "FrequencyGrid", "WaveformOptions",
+    "check_frequencies", "parameter_table", "WaveformError", "InvalidParameterError", "UnsupportedError",
+    "UnknownApproximantError", "ModelError", "polarizations_function",
+]
+
+
+def polarizations_function(approximant, options=None, **option_keywords):
+    """The PyTorch function of an approximant (see
+    :func:`waveformx_amd.torch.polarizations_function`)."""
+    from .torch import polarizations_function as f
+
+    return f(approximant, options, **option_keywords)

@@ -1,0 +1,270 @@
+"""Emit ``xas_fits.inc``: the IMRPhenomX (2,2) parameter-space fits as
+templated HIP device functions for the per-binary setup kernel.
+
+The fits are LALSimulation's calibration data (via waveformrs), which the
+reference holds as generated Python in
+``src/waveformx/models/xpnr/_generated/{utilities,qnm,inspiral,
+intermediate,ringdown}.py``; their literals and arithmetic order must be
+kept, so they are transcribed mechanically from that text (read as an AST,
+never imported). Two statement blocks of hand-written reference functions
+with long literal expressions are transcribed the same way
+(``amp22.py:65-69``, ``phase22.py:145-162``).
+
+Every function is ``template <class R>`` over the setup's scalar: double
+(the FP64 and split-precision paths), float (plain FP32) or a forward-mode
+dual (gradients). Literal-only subexpressions are evaluated in double and
+narrowed once to ``R``, as JAX does with Python floats (weak types) in a
+float32 evaluation. ``fma(a, b, c)`` is the reference's unfused ``a * b +
+c`` (``xpnr/_math.py:125-137``); the library is compiled with contraction
+off, so it stays unfused.
+
+The QNM fits are also re-expanded in powers of ``1 - a_f`` with numpy, the
+arithmetic of the reference's ``about_one`` (``xas/core.py:183-211``).
+
+Usage: ``python waveformx_amd/csrc/gen_fits.py [REFERENCE_ROOT]``.
+"""
+
+from __future__ import annotations
+
+import ast
+import sys
+from pathlib import Path
+
+import numpy as np
+
+FITS = {
+    "utilities": ("ut", ["sign", "chi_eff", "chi_pn_hat", "s_tot_r", "dchi", "f_isco", "f_meco", "final_mass_2017",
+                         "final_spin_2017", "linb", "psi4_to_strain"]),
+    "inspiral": ("ins", ["amp_22_v2", "amp_22_v3", "amp_22_v4", "pow_7_of", "amp_22_rho1", "amp_22_rho2",
+                         "amp_22_rho3", "amp_22_ansatz_exact", "amp_22_dansatz", "phase_22_v3", "phase_22_d13",
+                         "phase_22_d23", "phase_22_d43", "phase_22_ansatz_exact", "phase_22_ansatz_int_exact"]),
+    "intermediate": ("itm", ["amp_22_va", "amp_22_delta0", "amp_22_delta1", "amp_22_delta2", "amp_22_delta3",
+                             "amp_22_delta4", "phase_22_v2", "phase_22_v2m_rdv4", "phase_22_v3m_rdv4",
+                             "phase_22_d43", "phase_22_ansatz_exact", "phase_22_ansatz_int_exact"]),
+    "ringdown": ("rd", ["amp_22_gamma2", "amp_22_gamma3", "amp_22_v1", "amp_22_ansatz_exact", "amp_22_dansatz",
+                        "amp_22_peak_frequency", "phase_22_v4", "phase_22_d12", "phase_22_d24", "phase_22_d34",
+                        "phase_22_d54", "phase_22_ansatz_exact", "phase_22_ansatz_int_exact"]),
+}
+BLOCKS = [
+    ("amp22.py", "amp_coefficients", "amp_pn_coefficients",
+     ["pn_two_thirds", "pn_three_thirds", "pn_four_thirds", "pn_five_thirds", "pn_six_thirds"], set()),
+    ("phase22.py", "phase_coefficients", "taylorf2_coefficients",
+     ["phi2_ns", "phi3_ns", "phi3_s", "phi4_ns", "phi4_s", "phi5l_ns", "phi5l_s", "phi6_ns", "phi6_s", "phi6l_ns",
+      "phi7_ns", "phi7_s", "phi8_s", "phi8l_s"], {"phi8_common"}),
+]
+STRUCTS = {"powers": "Powers<R>", "amp": "AmpCoeffs<R>", "phase": "PhaseCoeffs<R>", "wf": "WaveformVars<R>"}
+CONSTANTS = {"PI": "kPi", "FRAC_PI_2": "(kPi / 2.0)", "FRAC_PI_4": "(kPi / 4.0)",
+             "EGAMMA": "kEulerGamma", "LN_2": "kLn2"}
+MATH = {"cbrt": "m_cbrt", "sqrt": "m_sqrt", "exp": "m_exp", "abs": "m_abs", "arctan": "m_atan", "power": "m_pow",
+        "minimum": "m_min", "where": "m_where"}
+HELPERS = {"fma": "m_fma", "fms": "m_fms", "fnms": "m_fnms", "sqrt_nonneg": "m_sqrt_nonneg"}
+
+
+class Writer:
+    """C++ text of one reference function body; ``static`` names hold
+    Python floats (literal-only values, computed in double)."""
+
+    def __init__(self, prefix, local_functions):
+        self.prefix = prefix
+        self.local = set(local_functions)
+        self.static = set()
+
+    def is_static(self, node):
+        for n in ast.walk(node):
+            if isinstance(n, ast.Name) and n.id not in CONSTANTS and n.id not in ("POWERS_OF_LALPI", "pi") \
+                    and n.id not in self.static and n.id != "jnp":
+                return False
+            if isinstance(n, ast.Call):
+                return False
+        return True
+
+    def double_text(self, node):
+        """A literal-only expression evaluated in double."""
+        if isinstance(node, ast.Constant):
+            text = repr(float(node.value))
+            return text if any(c in text for c in ".en") else text + ".0"
+        if isinstance(node, ast.Name):
+            return CONSTANTS.get(node.id, node.id)
+        if isinstance(node, ast.Attribute):
+            return f"lalpi_{node.attr}"
+        if isinstance(node, ast.UnaryOp) and isinstance(node.op, ast.USub):
+            return f"(-{self.double_text(node.operand)})"
+        if isinstance(node, ast.BinOp):
+            ops = {ast.Add: "+", ast.Sub: "-", ast.Mult: "*", ast.Div: "/"}
+            if isinstance(node.op, ast.Pow):
+                return f"pow({self.double_text(node.left)}, {self.double_text(node.right)})"
+            return f"({self.double_text(node.left)} {ops[type(node.op)]} {self.double_text(node.right)})"
+        raise NotImplementedError(ast.dump(node))
+
+    def text(self, node):
+        if self.is_static(node):
+            return f"R({self.double_text(node)})"
+        if isinstance(node, ast.Name):
+            return node.id
+        if isinstance(node, ast.Attribute):
+            return f"{self.text(node.value)}.{node.attr}"
+        if isinstance(node, ast.UnaryOp) and isinstance(node.op, ast.USub):
+            return f"(-{self.text(node.operand)})"
+        if isinstance(node, ast.BinOp):
+            ops = {ast.Add: "+", ast.Sub: "-", ast.Mult: "*", ast.Div: "/"}
+            if isinstance(node.op, ast.Pow):
+                return f"m_pow({self.text(node.left)}, {self.double_text(node.right)})"
+            return f"({self.text(node.left)} {ops[type(node.op)]} {self.text(node.right)})"
+        if isinstance(node, ast.Compare):
+            ops = {ast.Gt: ">", ast.Lt: "<", ast.GtE: ">=", ast.LtE: "<=", ast.Eq: "==", ast.NotEq: "!="}
+            return f"({self.text(node.left)} {ops[type(node.ops[0])]} {self.text(node.comparators[0])})"
+        if isinstance(node, ast.Call):
+            func, args = node.func, node.args
+            if isinstance(func, ast.Attribute):
+                name = MATH[func.attr]
+                if name == "m_pow":
+                    return f"m_pow({self.text(args[0])}, {self.double_text(args[1])})"
+            elif func.id in HELPERS:
+                name = HELPERS[func.id]
+            elif func.id in self.local:
+                name = f"{self.prefix}_{func.id}"
+            else:
+                raise NotImplementedError(ast.dump(node))
+            return f"{name}({', '.join(self.text(a) for a in args)})"
+        raise NotImplementedError(ast.dump(node))
+
+    def statement(self, stmt, declared, lines):
+        if isinstance(stmt, ast.Expr):
+            return
+        if isinstance(stmt, ast.Return):
+            lines.append(f"    return {self.text(stmt.value)};")
+            return
+        if isinstance(stmt, ast.AugAssign):
+            ops = {ast.Add: "+", ast.Sub: "-"}
+            name = stmt.target.id
+            lines.append(f"    {name} = ({name} {ops[type(stmt.op)]} {self.text(stmt.value)});")
+            return
+        (target,) = stmt.targets
+        name = target.id
+        if name not in declared and self.is_static(stmt.value):
+            self.static.add(name)
+            declared.add(name)
+            lines.append(f"    const double {name} = {self.double_text(stmt.value)};")
+            return
+        value = self.text(stmt.value)
+        if name in declared:
+            lines.append(f"    {name} = {value};")
+        else:
+            declared.add(name)
+            lines.append(f"    R {name} = {value};")
+
+    def function(self, fn, module):
+        self.static = set()
+        params = [f"const {STRUCTS[a.arg]}& {a.arg}" if a.arg in STRUCTS else f"R {a.arg}" for a in fn.args.args]
+        lines = [f"// {module}.py:{fn.lineno}",
+                 f"template <class R> WFX_HD inline R {self.prefix}_{fn.name}({', '.join(params)}) {{"]
+        declared = {a.arg for a in fn.args.args}
+        for stmt in fn.body:
+            self.statement(stmt, declared, lines)
+        lines.append("}")
+        return "\n".join(lines)
+
+
+def block(path, function, cname, outputs, helpers):
+    fn = next(n for n in ast.walk(ast.parse(path.read_text())) if isinstance(n, ast.FunctionDef) and n.name == function)
+    keep = set(outputs) | helpers
+    stmts = [s for s in fn.body if isinstance(s, ast.Assign) and isinstance(s.targets[0], ast.Name)
+             and s.targets[0].id in keep]
+    free = set()
+    for s in stmts:
+        free |= {n.id for n in ast.walk(s.value) if isinstance(n, ast.Name)}
+    free -= keep | set(HELPERS) | set(CONSTANTS) | {"pi", "jnp"}
+    inputs = sorted(free)
+    w = Writer("", set())
+    params = [f"R {n}" for n in inputs] + [f"R& {n}" for n in outputs]
+    lines = [f"// {path.name}:{fn.lineno} ({function}): the statements assigning {', '.join(outputs)}",
+             f"template <class R> WFX_HD inline void {cname}({', '.join(params)}) {{"]
+    declared = set(outputs)
+    for s in stmts:
+        w.statement(s, declared, lines)
+    lines.append("}")
+    return "\n".join(lines)
+
+
+def about_one(coefficients):
+    """The polynomial (coefficients from af^0) in powers of u = 1 - af, with
+    numpy's float64 arithmetic as the reference re-expands it."""
+    c = np.trim_zeros(np.asarray(coefficients, dtype=float), "b")
+    out = np.zeros(len(c))
+    for k, ck in enumerate(c):
+        out[:k + 1] += ck * np.polynomial.polynomial.polypow([1.0, -1.0], k)
+    return [float(x) for x in out]
+
+
+def qnm_tables(path):
+    tree = ast.parse(path.read_text())
+    lines = []
+    for fn in (n for n in tree.body if isinstance(n, ast.FunctionDef) and n.name in ("fring22", "fdamp22")):
+        for stmt in fn.body:
+            if isinstance(stmt, ast.Assign) and isinstance(stmt.value, ast.Call) and \
+                    getattr(stmt.value.func, "id", "") == "fma_sum":
+                const, coeffs, pows = stmt.value.args
+                table = {0: float(const.value)}
+                for c, x in zip(coeffs.elts, pows.elts):
+                    k = 1 if x.id == "af" else int(x.id[1:])
+                    table[k] = float(c.value) if isinstance(c, ast.Constant) else -float(c.operand.value)
+                poly = [table.get(k, 0.0) for k in range(max(table) + 1)]
+                u = about_one(poly)
+                name = f"{fn.name}_{stmt.targets[0].id}"
+                lines.append(f"// qnm.py:{stmt.lineno}: {name} in powers of u = 1 - a_f (xas/core.py:183-211)")
+                lines.append(f"constexpr int kQnm_{name}_n = {len(u)};")
+                lines.append(f"constexpr double kQnm_{name}[{len(u)}] = {{{', '.join(repr(x) for x in u)}}};")
+    return "\n".join(lines)
+
+
+def lalpi_powers():
+    """``POWERS_OF_LALPI`` (``xpnr/_math.py:190-260``): the useful powers of
+    pi in float64, with the same sequence of operations."""
+    x = np.float64(np.pi)
+    third = np.cbrt(x)
+    sixth = np.sqrt(third)
+    m_one = 1.0 / x
+    two_thirds = third * third
+    m_one_third = two_thirds * m_one
+    m_one_sixth = sixth * m_one_third
+    four_thirds = two_thirds * two_thirds
+    seven_thirds = four_thirds * x
+    m_two_thirds = m_one_third * m_one_third
+    m_four_thirds = m_two_thirds * m_two_thirds
+    return dict(itself=x, one_third=third, two_thirds=two_thirds, four_thirds=four_thirds,
+                five_thirds=four_thirds * third, seven_thirds=seven_thirds, eight_thirds=seven_thirds * third,
+                two=x * x, log=np.log(x), m_one_sixth=m_one_sixth,
+                m_five_thirds=m_four_thirds * m_one_third)
+
+
+def generate(root: Path) -> str:
+    base = root / "src/waveformx/models/xpnr"
+    out = ["// GENERATED by waveformx_amd/csrc/gen_fits.py from the reference's IMRPhenomX (2,2) fits",
+           "// (src/waveformx/models/xpnr/_generated/*.py; LALSimulation 6.2.0 data via waveformrs,",
+           "// GPL-3.0-or-later). Included by xas_setup.h inside namespace wfx. Do not edit.", ""]
+    out.append("// POWERS_OF_LALPI (xpnr/_math.py:249-260), float64")
+    for name, value in lalpi_powers().items():
+        out.append(f"constexpr double lalpi_{name} = {float(value)!r};")
+    out.append("")
+    for module, (prefix, wanted) in FITS.items():
+        tree = ast.parse((base / "_generated" / f"{module}.py").read_text())
+        functions = {n.name: n for n in tree.body if isinstance(n, ast.FunctionDef)}
+        w = Writer(prefix, functions)
+        for name in wanted:
+            out.append(w.function(functions[name], module))
+            out.append("")
+    out.append(qnm_tables(base / "_generated" / "qnm.py"))
+    out.append("")
+    for file, function, cname, outputs, helpers in BLOCKS:
+        out.append(block(base / file, function, cname, outputs, helpers))
+        out.append("")
+    return "\n".join(out)
+
+
+def main():
+    root = Path(sys.argv[1]) if len(sys.argv) > 1 else Path("/root/reference")
+    (Path(__file__).resolve().parent / "xas_fits.inc").write_text(generate(root))
+
+
+if __name__ == "__main__":
+    main()

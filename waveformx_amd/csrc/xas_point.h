@@ -1,0 +1,212 @@
+// The per-(binary, frequency) formula of IMRPhenomXAS (xas/core.py:423-507)
+// on one record of xas_layout.h: the phase (regions, leading term,
+// reduction to [-pi, pi]), the amplitude and the polarizations
+// h+ = kp A e^{i phi}, hx = i kc A e^{i phi}.
+//
+// Two arithmetics:
+//  * point_f64: FP64, the reference's order of operations, with the
+//    record's FP64 powers of f -- the parity path;
+//  * point_f32<Lead>: FP32 with Horner forms of the phase sums and the
+//    inspiral's leading term T0 f^(-5/3) - A and its reduction in FP64
+//    (kLeadF64), in double-float (kLeadDF), or plain FP32 (kLeadF32).
+#pragma once
+
+#include "xas_setup.h"
+
+namespace wfx {
+
+// ---------------------------------------------------------- FP64 (parity)
+
+WFX_HD inline void point_f64(const double* c, const FreqPowers<double>& fp, double& hp_re, double& hp_im,
+                             double& hc_re, double& hc_im) {
+    bool past;
+    const double rest = phase_rest(c, fp, past);
+    double phi = (past ? 0.0 : (c[kT0] + c[kT0Lo]) * fp.m53) + rest;
+    phi = phi - (c[kAnchor] + c[kAnchorLo]);
+    const double turns = rint(phi / kTwoPi);
+    phi = phi - turns * kTwoPi;
+
+    const double f = fp.f;
+    double amp;
+    if (f > c[kFAmpRd]) {
+        const double u = c[kAmpRdS] * (f - c[kShift]);
+        amp = c[kAmpRdN] * fp.m76 * exp(-c[kAmpRdGr] * u) / (u * u + c[kAmpRdG]);
+    } else if (f >= c[kFAmpIn]) {
+        double den = c[kAmpIntD + 4];
+        for (int j = 3; j >= 0; --j) den = den * f + c[kAmpIntD + j];
+        amp = c[kAmpIntN] / den;
+    } else {
+        double poly = c[kAmpIns + 9];
+        for (int k = 8; k >= 0; --k) poly = poly * fp.t + c[kAmpIns + k];
+        amp = fp.m76 * poly;
+    }
+    amp = f <= c[kFCut] ? amp : 0.0;
+    double s, co;
+    sincos(phi, &s, &co);
+    const double ap = c[kKp] * amp;
+    const double ac = c[kKc] * amp;
+    hp_re = ap * co;
+    hp_im = ap * s;
+    hc_re = -ac * s;
+    hc_im = ac * co;
+}
+
+// ---------------------------------------------------------- FP32 paths
+
+enum Lead : int { kLeadF64 = 0, kLeadDF = 1, kLeadF32 = 2 };
+
+// the powers of one frequency for the FP32 formula; f^(-5/3) in FP64 (and
+// as a double-float pair)
+struct PowF32 {
+    float f, t, m1, lf, m76;
+    double m53;
+    float m53_hi, m53_lo;
+};
+
+// FP32 powers of a frequency given in FP32; f^(-1/3) from 1/t is refined
+// by two Newton steps in FP64 before f^(-5/3) = (f^(-1/3))^5
+__device__ inline PowF32 pow_f32(float f) {
+    PowF32 p;
+    p.f = f;
+    p.lf = logf(f);
+    p.t = cbrtf(f);
+    p.m1 = 1.0f / f;
+    p.m76 = p.m1 / sqrtf(p.t);
+    const double fd = (double)f;
+    double r = (double)(1.0f / p.t);
+    r = r * (4.0 - fd * (r * r * r)) * (1.0 / 3.0);
+    r = r * (4.0 - fd * (r * r * r)) * (1.0 / 3.0);
+    const double r2 = r * r;
+    p.m53 = r2 * r2 * r;
+    p.m53_hi = (float)p.m53;
+    p.m53_lo = (float)(p.m53 - (double)p.m53_hi);
+    return p;
+}
+
+// double-float helpers (exact transformations: Knuth's TwoSum, Dekker's
+// TwoProduct through the fused multiply-add)
+struct F2 {
+    float hi, lo;
+};
+__device__ inline F2 two_sum(float a, float b) {
+    const float s = a + b;
+    const float bb = s - a;
+    return F2{s, (a - (s - bb)) + (b - bb)};
+}
+__device__ inline F2 fast_two_sum(float a, float b) {
+    const float s = a + b;
+    return F2{s, b - (s - a)};
+}
+__device__ inline F2 f2_mul(F2 a, F2 b) {
+    const float p = a.hi * b.hi;
+    const float e = fmaf(a.hi, b.hi, -p);
+    return fast_two_sum(p, fmaf(a.hi, b.lo, fmaf(a.lo, b.hi, e)));
+}
+__device__ inline F2 f2_add(F2 a, F2 b) {
+    const F2 s = two_sum(a.hi, b.hi);
+    return fast_two_sum(s.hi, s.lo + (a.lo + b.lo));
+}
+
+constexpr float kInvTwoPiF = 0.15915494309189535f;
+constexpr float kTwoPiHiF = 6.28318548202514648f;     // float(2 pi)
+constexpr float kTwoPiLoF = -1.7484555314695172e-07f;  // 2 pi - float(2 pi)
+constexpr double kInvTwoPi = 0.15915494309189535;
+
+// T0 and A of a FP32 record in FP64 (slots 60..63 hold them as doubles)
+__device__ inline double record_f64(const float* c, int slot) {
+    return *reinterpret_cast<const double*>(c + slot);
+}
+constexpr int kT0F64 = 60;
+constexpr int kAnchorF64 = 62;
+
+template <int L>
+__device__ inline void point_f32(const float* __restrict__ c, const PowF32& p, float2& hp, float2& hc) {
+    const float f = p.f;
+    const float t = p.t;
+    const float m1 = p.m1;
+    const bool past = f >= c[kFPhaseIn];
+    const bool ringdown = f > c[kFPhaseIm];
+    float rest;
+    if (!past) {
+        // sum_k c_k f^((k-3)/3) + log f (c + c' f^1/3 + c'' f)
+        float poly = c[kInsP2];
+        poly = fmaf(poly, t, c[kInsP53]);
+        poly = fmaf(poly, t, c[kInsP43]);
+        poly = fmaf(poly, t, c[kInsP1]);
+        poly = fmaf(poly, t, c[kInsP23]);
+        poly = fmaf(poly, t, c[kInsP13]);
+        poly = fmaf(poly, t, c[kInsConst]);
+        poly = fmaf(poly, t, c[kInsM13]);
+        poly = fmaf(poly, t, c[kInsM23]);
+        poly = fmaf(poly, t, c[kInsM1]);
+        const float lg = fmaf(fmaf(c[kInsP1L], t * t, c[kInsP13L]), t, c[kInsLog]);
+        rest = fmaf(poly, m1, lg * p.lf);
+    } else if (!ringdown) {
+        const float inv = fmaf(fmaf(c[kIntM3], m1, c[kIntM2]), m1, c[kIntM1]);
+        rest = fmaf(inv, m1, c[kIntConst]);
+        rest = fmaf(c[kIntP1], f, rest);
+        rest = fmaf(c[kIntLog], p.lf, rest);
+        rest = fmaf(c[kAtanA1], atanf(c[kAtanW1] * (f - c[kShift])), rest);
+    } else {
+        const float inv = fmaf(c[kRdM3], m1 * m1, c[kRdM1]);
+        rest = fmaf(inv, m1, c[kRdConst]);
+        rest = fmaf(c[kRdP1], f, rest);
+        rest = fmaf(c[kRdP23], t * t, rest);
+        rest = fmaf(c[kAtanA2], atanf(c[kAtanW2] * (f - c[kShift])), rest);
+    }
+
+    float phi;
+    if (L == kLeadF64) {
+        const double base = (double)rest - record_f64(c, kAnchorF64);
+        const double full = past ? base : fma(record_f64(c, kT0F64), p.m53, base);
+        const double turns = rint(full * kInvTwoPi);
+        phi = (float)fma(-turns, kTwoPi, full);
+    } else if (L == kLeadDF) {
+        F2 lead = past ? F2{0.0f, 0.0f} : f2_mul(F2{c[kT0], c[kT0Lo]}, F2{p.m53_hi, p.m53_lo});
+        F2 s = f2_add(lead, F2{-c[kAnchor], -c[kAnchorLo]});
+        s = f2_add(s, F2{rest, 0.0f});
+        const float turns = rintf(s.hi * kInvTwoPiF);
+        float r = fmaf(-turns, kTwoPiHiF, s.hi);
+        r = fmaf(-turns, kTwoPiLoF, r);
+        phi = r + s.lo;
+    } else {
+        const float base = rest - c[kAnchor];
+        const float full = past ? base : fmaf(c[kT0], p.m53_hi, base);
+        const float turns = rintf(full * kInvTwoPiF);
+        phi = fmaf(-turns, kTwoPiLoF, fmaf(-turns, kTwoPiHiF, full));
+    }
+
+    float amp;
+    if (f > c[kFAmpRd]) {
+        const float u = c[kAmpRdS] * (f - c[kShift]);
+        amp = c[kAmpRdN] * p.m76 * __expf(-c[kAmpRdGr] * u) / fmaf(u, u, c[kAmpRdG]);
+    } else if (f >= c[kFAmpIn]) {
+        float den = c[kAmpIntD + 4];
+        den = fmaf(den, f, c[kAmpIntD + 3]);
+        den = fmaf(den, f, c[kAmpIntD + 2]);
+        den = fmaf(den, f, c[kAmpIntD + 1]);
+        den = fmaf(den, f, c[kAmpIntD + 0]);
+        amp = c[kAmpIntN] / den;
+    } else {
+        float poly = c[kAmpIns + 9];
+        poly = fmaf(poly, t, c[kAmpIns + 8]);
+        poly = fmaf(poly, t, c[kAmpIns + 7]);
+        poly = fmaf(poly, t, c[kAmpIns + 6]);
+        poly = fmaf(poly, t, c[kAmpIns + 5]);
+        poly = fmaf(poly, t, c[kAmpIns + 4]);
+        poly = fmaf(poly, t, c[kAmpIns + 3]);
+        poly = fmaf(poly, t, c[kAmpIns + 2]);
+        poly = fmaf(poly, t, c[kAmpIns + 1]);
+        poly = fmaf(poly, t, c[kAmpIns + 0]);
+        amp = p.m76 * poly;
+    }
+    amp = f <= c[kFCut] ? amp : 0.0f;
+    float s, co;
+    __sincosf(phi, &s, &co);
+    const float ap = c[kKp] * amp;
+    const float ac = c[kKc] * amp;
+    hp = make_float2(ap * co, ap * s);
+    hc = make_float2(-ac * s, ac * co);
+}
+
+}  // namespace wfx
