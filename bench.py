@@ -58,6 +58,97 @@ def timed(torch, fn, steps, warmup):
     return start.elapsed_time(stop) / steps
 
 
+def bench_grad(torch, xas, dev, nb, n, steps, warmup):
+    """The fused forward + adjoint of L = sum_f (|h+ - d+|^2 + |hx - dx|^2) / S(f)
+    for nb binaries x n bins (FP32-split), d at parameters perturbed by 1%:
+    binaries/s of the whole gradient step (FP32 record setup, FP64 setup with
+    derivatives, fused kernel), and of the forward step alone."""
+    from waveformx_amd import adjoint
+
+    freqs = np.linspace(20.0, 1024.0, n)
+    p = draw(nb, seed=3)
+    rng = np.random.default_rng(4)
+    perturbed = p * (1 + 1e-2 * rng.normal(size=p.shape))
+    perturbed[:, 7] = p[:, 7]
+    params = torch.as_tensor(p, device=dev)
+    f32 = torch.as_tensor(freqs, dtype=torch.float32, device=dev)
+    d_plus, d_cross = xas.polarizations(torch.as_tensor(perturbed, device=dev), f32, 1e21, None, "split")
+    x = freqs / 100.0
+    w = torch.as_tensor(1.0 / (x ** -4 + 1.0 + x ** 2), dtype=torch.float32, device=dev)
+    out = (torch.empty((nb, n), dtype=torch.complex64, device=dev),
+           torch.empty((nb, n), dtype=torch.complex64, device=dev))
+
+    def forward():
+        records, _ = xas.setup(params, f32[:1], 1e21, None, "split")
+        xas.evaluate(records, f32, "split", out=out)
+
+    def gradient():
+        adjoint.loss_and_grad(params, f32, d_plus, d_cross, w, precision="split")
+
+    ms_fwd = timed(torch, forward, steps, warmup)
+    ms_grad = timed(torch, gradient, steps, warmup)
+    return {"fwd_grad_value": nb / (ms_grad * 1e-3), "fwd_grad_ms": ms_grad, "fwd_grad_binaries": nb,
+            "fwd_grad_over_fwd": ms_grad / ms_fwd}
+
+
+def ragged_grids(nb, seed=5):
+    """Binaries each on its own uniform grid: f_lower ~ U[10, 30] Hz,
+    duration T ~ log-U[4, 64] s (df = 1/T), f_max = min(2048, 0.3 / M),
+    n = floor((f_max - f_lower) / df + 0.5) + 1; masses m1 ~ U[5, 100],
+    q ~ U[0.125, 1], aligned spins ~ U[-0.9, 0.9]."""
+    rng = np.random.default_rng(seed)
+    m1 = rng.uniform(5.0, 100.0, nb)
+    m2 = m1 * rng.uniform(0.125, 1.0, nb)
+    p = draw(nb, seed=seed + 1)
+    p[:, 0], p[:, 1] = m1, m2
+    f_lower = rng.uniform(10.0, 30.0, nb)
+    df = 1.0 / np.exp(rng.uniform(np.log(4.0), np.log(64.0), nb))
+    f_max = np.minimum(2048.0, 0.3 / ((m1 + m2) * 4.925490947641267e-06))
+    n = (np.floor((f_max - f_lower) / df + 0.5) + 1).astype(np.int64)
+    p[:, 7] = f_lower
+    return p, [(a, b, int(c)) for a, b, c in zip(f_lower, df, n)]
+
+
+def bench_ragged(torch, xas, dev, nb, steps, warmup):
+    """nb binaries on their own grids (FP32-split): the graph-replayed
+    persistent launch (tuned tile and grid shape), against one launch per
+    binary and against the batch padded to the longest grid."""
+    from waveformx_amd.ragged import RaggedPlan
+
+    p, grids = ragged_grids(nb)
+    params = torch.as_tensor(p, device=dev)
+    plan = RaggedPlan(grids, precision="split")
+    ms_plan = timed(torch, lambda: plan(params), steps, warmup)
+    lengths = np.array([g[2] for g in grids])
+    # one launch per binary (the setup once for all)
+    per = torch.as_tensor(np.stack([np.zeros(nb, np.int64), lengths], 1), device=dev)
+    per_tiles = torch.as_tensor(np.stack([np.zeros(nb, np.int64), (lengths + 255) // 256], 1), device=dev)
+    starts = np.concatenate([[0], np.cumsum(lengths)])
+
+    blocks_per = np.maximum(np.minimum((lengths + 255) // 256, 2048), 1)
+
+    def per_binary():
+        records, _ = xas.setup(params, plan.f_min, 1e21, None, "split")
+        for b in range(nb):
+            xas.evaluate_ragged(records[b:b + 1], per[b], grid_f0=plan.grid_f0[b:b + 1],
+                                grid_df=plan.grid_df[b:b + 1], precision="split", tiles=per_tiles[b],
+                                blocks=int(blocks_per[b]), total=int(lengths[b]),
+                                out=(plan.hplus[starts[b]:starts[b + 1]], plan.hcross[starts[b]:starts[b + 1]]))
+
+    ms_per = timed(torch, per_binary, 2, 1)
+    # every binary padded to the longest grid (its own f_lower and df)
+    longest = int(lengths.max())
+    padded = RaggedPlan([(g[0], g[1], longest) for g in grids], precision="split", tune=False,
+                        bins_per_thread=plan.bins_per_thread, blocks=plan.blocks)
+    ms_pad = timed(torch, lambda: padded(params), max(2, steps // 4), 1)
+    total = int(lengths.sum())
+    del padded
+    return {"ragged_binaries": nb, "ragged_bins": total, "ragged_ms": ms_plan,
+            "ragged_value": nb / (ms_plan * 1e-3), "ragged_bins_per_s": total / (ms_plan * 1e-3),
+            "ragged_per_binary_launches_ms": ms_per, "ragged_padded_ms": ms_pad,
+            "ragged_padded_bins": nb * longest, "ragged_shape": [plan.bins_per_thread, plan.blocks]}
+
+
 def cpu_baseline(n_binaries, freqs):
     """The FP64 oracle on the host cores on ``n_binaries`` of the workload:
     median of 3 timed runs after a warm-up."""
@@ -85,8 +176,16 @@ def main():
     ap.add_argument("--binaries", type=int, default=65536)
     ap.add_argument("--bins", type=int, default=4017)
     ap.add_argument("--group", type=int, default=0)
-    ap.add_argument("--cpu-binaries", type=int, default=1024)
+    ap.add_argument("--cpu-binaries", type=int, default=32768)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grad-binaries", type=int, default=8192,
+                    help="binaries of the fused forward + adjoint measurement (0: skip)")
+    ap.add_argument("--ragged-binaries", type=int, default=16384,
+                    help="binaries of the ragged-grid measurement (0: skip)")
+    ap.add_argument("--only", choices=["split", "split-df", "fp64", "fp32"],
+                    help="run just this arithmetic's step (setup + evaluation) K times, for a profiler")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run a plain device copy of this many MiB K times (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the binaries are independent, so the path shards by binary with no "
@@ -98,11 +197,29 @@ def main():
     from waveformx_amd import xas
 
     dev = torch.device("cuda", 0)
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        ms = timed(torch, lambda: dst.copy_(src), args.steps, args.warmup)
+        print(json.dumps({"copy_mib": args.copy_mib, "ms": ms, "GB/s": 2 * src.numel() * 4 / ms / 1e6}), flush=True)
+        return
     nb, n = args.binaries, args.bins
     freqs64 = np.linspace(20.0, 1024.0, n)
     params = torch.as_tensor(draw(nb), device=dev)
     f32 = torch.as_tensor(freqs64, dtype=torch.float32, device=dev)
     f64 = torch.as_tensor(freqs64, dtype=torch.float64, device=dev)
+    if args.only:
+        grid = f64 if args.only == "fp64" else f32
+        ctype = torch.complex128 if args.only == "fp64" else torch.complex64
+        out = (torch.empty((nb, n), dtype=ctype, device=dev), torch.empty((nb, n), dtype=ctype, device=dev))
+
+        def one():
+            records, _ = xas.setup(params, grid[:1], 1e21, None, args.only)
+            xas.evaluate(records, grid, args.only, args.group, out=out)
+
+        ms = timed(torch, one, args.steps, args.warmup)
+        print(json.dumps({"only": args.only, "ms_per_step": ms, "waveforms/s": nb / ms * 1e3}), flush=True)
+        return
     out32 = (torch.empty((nb, n), dtype=torch.complex64, device=dev),
              torch.empty((nb, n), dtype=torch.complex64, device=dev))
     out64 = (torch.empty((nb, n), dtype=torch.complex128, device=dev),
@@ -164,6 +281,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }
+    del out32, out64, rec
+    torch.cuda.empty_cache()
+    if args.grad_binaries:
+        result.update(bench_grad(torch, xas, dev, args.grad_binaries, n, args.steps, args.warmup))
+    if args.ragged_binaries:
+        result.update(bench_ragged(torch, xas, dev, args.ragged_binaries, args.steps, args.warmup))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_binaries, freqs64)
     print(json.dumps(result), flush=True)

@@ -1,0 +1,120 @@
+/*
+ * waveformx_amd: the C ABI of the IMRPhenomXAS hot path on MI355X (gfx950),
+ * exported by waveformx_amd/libwfx_xas.so.
+ *
+ * The reference crosses from PyTorch into its model through DLPack and a
+ * jax.jit(jax.vmap(...)) of model.polarizations (src/waveformx/torch.py:
+ * 58-106, src/waveformx/_api.py:122-129); these entry points take that
+ * place: plain device pointers and sizes, caller-owned memory, work
+ * enqueued on the caller's HIP stream (hipStream_t passed as void*).
+ *
+ * Conventions.
+ *  - Every entry point returns 0 on success and -1 on an error that it
+ *    detects before launching (bad size, null pointer, unknown precision,
+ *    launch failure); wfx_last_error() then names it (per host thread).
+ *    Nothing throws across the ABI. Kernel execution is asynchronous.
+ *  - No entry point allocates or frees memory; buffers come from the caller.
+ *  - Parameters are packed as double params[B][8] = {mass_1, mass_2 (solar
+ *    masses), chi_1z, chi_2z, luminosity_distance (Mpc), inclination,
+ *    phi_ref (rad), f_ref (Hz, 0 = the first frequency)}; in-plane spins
+ *    do not enter (src/waveformx/models/xas/model.py:25-32).
+ *  - precision: 0 = FP64 throughout (double records, complex128 out);
+ *    1 = FP32 setup and evaluation (float records, complex64), no split;
+ *    2 = FP64 setup, FP32 evaluation with the inspiral's leading phase term
+ *    and the phase reduction in FP64 (float records, complex64);
+ *    3 = as 2 with that term in double-float FP32 arithmetic.
+ *  - Outputs are interleaved complex (re, im), row-major (B, N).
+ *  - LAL's failures make the binary's row NaN and set flags (bit 0: mass
+ *    ratio beyond 1000, 1: |spin| > 1, 2: final mass > 1, 3: |final spin|
+ *    > 1, 4: f_MECO > f_ISCO; src/waveformx/models/xpnr/internals.py:136-149).
+ *  - Calls on one stream are ordered; calls on different streams that share
+ *    an output buffer race, as any kernel would.
+ */
+#ifndef WAVEFORMX_AMD_H
+#define WAVEFORMX_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The message of the last failed call on this host thread. */
+const char* wfx_last_error(void);
+
+/* Slots of one binary's record (64) and its size in bytes for a precision
+ * (512 for FP64, 256 otherwise). The record replaces the reference's
+ * 86-entry table (src/waveformx/models/xas/core.py:68-101). */
+int wfx_xas_record_slots(void);
+int wfx_xas_record_bytes(int precision);
+
+/* Per-binary setup: records[B][64] (double for precision 0, else float) and
+ * flags[B] (may be NULL) of params[B][8] whose grids start at f_min
+ * (f_min_stride 0: one value for all, 1: one per binary). reference_mf:
+ * NaN for none, else f_ref <- reference_mf / M where f_ref was 0 (NaN
+ * otherwise). Replaces core.table (src/waveformx/models/xas/core.py:
+ * 257-379) with model.py:35-59's f_min and reference_mf handling. */
+int wfx_xas_setup(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
+                  double amp_scale, double reference_mf, int precision, void* records, uint32_t* flags,
+                  void* stream);
+
+/* The FP64 records and their derivatives jacobian[B][64][8] with respect to
+ * the 8 packed parameters (forward mode through the setup), for the
+ * adjoint. Replaces the forward half of the reference's jax.vjp
+ * (src/waveformx/torch.py:87-93). */
+int wfx_xas_setup_jvp(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
+                      double amp_scale, double reference_mf, double* records, double* jacobian, uint32_t* flags,
+                      void* stream);
+
+/* (h+, hx)[B][N] of records on one shared grid freqs[N] (float, or double
+ * if freqs_f64). group: binaries per block (0: default). Replaces
+ * core.frequency_powers + core.evaluate under jax.vmap
+ * (src/waveformx/models/xas/core.py:382-507; src/waveformx/torch.py:84-85). */
+int wfx_xas_eval(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
+                 int precision, int group, void* hplus, void* hcross, void* stream);
+
+/* tiles[B+1]: the prefix sum of the tiles (256 x bins_per_thread bins; 1, 2
+ * or 4) of each binary's range [offsets[b], offsets[b+1]), the work list of
+ * wfx_xas_eval_ragged. No counterpart in the reference (its batches share
+ * one grid). */
+int wfx_xas_tile_prefix(const int64_t* offsets, int64_t n_binaries, int bins_per_thread, int64_t* tiles,
+                        void* stream);
+
+/* (h+, hx) of binaries on their own grids, flat: binary b owns
+ * [offsets[b], offsets[b+1]) of freqs (float or double) and of the
+ * outputs, or, with freqs NULL, the uniform grid grid_f0[b] + k
+ * grid_df[b]. One persistent launch of `blocks` blocks (0: 8 per CU) over
+ * the tiles from wfx_xas_tile_prefix (same bins_per_thread). Generalises the shared-grid batching
+ * of src/waveformx/torch.py:84-85 (the MLX port's per-binary grids,
+ * src/waveformx/models/xas/mlx/_kernels.py:523-525, need equal lengths). */
+int wfx_xas_eval_ragged(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64,
+                        const double* grid_f0, const double* grid_df, const int64_t* offsets, const int64_t* tiles,
+                        int precision, int bins_per_thread, int blocks, void* hplus, void* hcross,
+                        void* stream);
+
+/* grads[B][8] (double) of a scalar loss from its cotangents cot_plus,
+ * cot_cross[B][N] (interleaved complex, dL/dRe + i dL/dIm, precision 0:
+ * double, else float) on the shared grid: the adjoint kernel recomputes
+ * each bin and contracts with jacobian (from wfx_xas_setup_jvp); records
+ * are those of the forward precision. Replaces the reverse half of the
+ * reference's jax.vjp backward (src/waveformx/torch.py:108-119). */
+int wfx_xas_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs, int freqs_f64,
+                    int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, double* grads,
+                    void* stream);
+
+/* Fused forward + adjoint of L_b = sum_f weights[f] (|h+ - d+|^2 +
+ * |hx - dx|^2) for data d (as the cotangents' layout) and weights[N]
+ * (double for precision 0, else float): loss[B] and grads[B][8], both
+ * double; nothing per bin is written. The consumer side of the path: the
+ * reference computes such losses by composing model.polarizations with
+ * jax.grad (src/waveformx/match.py:78-81 defines the weighted inner
+ * product). */
+int wfx_xas_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
+                         int freqs_f64, int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
+                         const void* weights, double* loss, double* grads, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WAVEFORMX_AMD_H */

@@ -1,0 +1,146 @@
+"""The hand-written adjoint against the oracle's forward-mode FP64
+derivatives and against central differences of the oracle: the FP64 path
+to 1e-8, the split FP32 path to the reference's float32 gradient bar
+``|g32 - g64| <= 1e-2 |g64| + 1e-3 max|g64|`` (tests/test_transforms_xas.py:58-63),
+the torch backward, and the fused noise-weighted loss."""
+
+import numpy as np
+import pytest
+
+from conftest import psd
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+GRID = np.linspace(20.0, 1024.0, 1024)
+
+
+def oracle_vjp(params, freqs, w):
+    """sum_f (w0 Re dh+ + w1 Im dh+ + w2 Re dhx + w3 Im dhx) per parameter."""
+    dhp, dhc = oracle.jacobian(params, freqs)
+    return (np.einsum("bf,bfk->bk", w[0], dhp.real) + np.einsum("bf,bfk->bk", w[1], dhp.imag)
+            + np.einsum("bf,bfk->bk", w[2], dhc.real) + np.einsum("bf,bfk->bk", w[3], dhc.imag))
+
+
+def device_vjp(cuda, params, freqs, w, precision):
+    import torch
+
+    from waveformx_amd import adjoint
+
+    ctype = torch.complex128 if precision == "fp64" else torch.complex64
+    rtype = torch.float64 if precision == "fp64" else torch.float32
+    p = torch.as_tensor(params, device=cuda)
+    f = torch.as_tensor(freqs, dtype=rtype, device=cuda)
+    gp = torch.as_tensor(w[0] + 1j * w[1], dtype=ctype, device=cuda)
+    gc = torch.as_tensor(w[2] + 1j * w[3], dtype=ctype, device=cuda)
+    grads, _ = adjoint.vjp(p, f, gp, gc, 1e21, None, precision)
+    return grads.cpu().numpy()
+
+
+def test_fp64_adjoint_matches_oracle(cuda):
+    params = oracle.draw(16, seed=4)
+    w = np.random.default_rng(5).normal(size=(4, 16, len(GRID)))
+    g = device_vjp(cuda, params, GRID, w, "fp64")
+    expected = oracle_vjp(params, GRID, w)
+    scale = np.max(np.abs(expected), axis=0)
+    err = np.abs(g - expected) / np.where(scale > 0, scale, 1.0)
+    print("fp64 adjoint max rel err per parameter", err.max(axis=0))
+    assert np.all(err < 1e-8), err.max(axis=0)
+
+
+def test_split_adjoint_within_float32_bar(cuda):
+    params = oracle.draw(16, seed=4)
+    rounded = params.astype(np.float32).astype(np.float64)
+    f = GRID.astype(np.float32).astype(np.float64)
+    w = np.random.default_rng(5).normal(size=(4, 16, len(GRID)))
+    g = device_vjp(cuda, params, f, w, "split")
+    expected = oracle_vjp(rounded, f, w)
+    bar = 1e-2 * np.abs(expected) + 1e-3 * np.max(np.abs(expected), axis=0)
+    assert np.all(np.abs(g - expected) <= bar), np.max(np.abs(g - expected) / bar, axis=0)
+
+
+def test_torch_backward(cuda):
+    """loss.backward() through the torch function reaches every field, as
+    the reference's tests/test_torch.py:58-96 (gradients vs the FP64
+    oracle, rtol 1e-10 here)."""
+    import torch
+
+    import waveformx_amd.torch as wt
+    from waveformx_amd import Binary
+
+    rows = np.array([[36.0, 29.0, 0.0, 0.0, 0.3, 0.0, 0.0, -0.4, 500.0, 0.7, 1.2, 25.0],
+                     [50.0, 8.0, 0.0, 0.0, 0.9, 0.0, 0.0, 0.3, 300.0, 2.1, 0.3, 30.0],
+                     [150.0, 100.0, 0.0, 0.0, -0.5, 0.0, 0.0, 0.95, 3000.0, 0.4, 2.0, 0.0]])
+    freqs = np.geomspace(20.0, 900.0, 300)
+    fields = [torch.tensor(rows[:, 0]), torch.tensor(rows[:, 1]), torch.tensor(rows[:, 2:5]),
+              torch.tensor(rows[:, 5:8]), torch.tensor(rows[:, 8]), torch.tensor(rows[:, 9]),
+              torch.tensor(rows[:, 10]), torch.tensor(rows[:, 11])]
+    fields = [x.to(cuda).requires_grad_(True) for x in fields]
+    f = wt.polarizations_function("IMRPhenomXAS")
+    hp, hc = f(Binary(*fields), torch.tensor(freqs, device=cuda))
+    w = np.random.default_rng(5).normal(size=(4, 3, len(freqs)))
+    wt_ = torch.tensor(w, device=cuda)
+    loss = (wt_[0] * hp.real + wt_[1] * hp.imag + wt_[2] * hc.real + wt_[3] * hc.imag).sum()
+    grads = torch.autograd.grad(loss, fields)
+    params = rows[:, [0, 1, 4, 7, 8, 9, 10, 11]]
+    expected = oracle_vjp(params, freqs, w)
+    got = np.stack([grads[0].cpu().numpy(), grads[1].cpu().numpy(), grads[2][:, 2].cpu().numpy(),
+                    grads[3][:, 2].cpu().numpy(), grads[4].cpu().numpy(), grads[5].cpu().numpy(),
+                    grads[6].cpu().numpy(), grads[7].cpu().numpy()], axis=1)
+    assert np.all(grads[2][:, :2].cpu().numpy() == 0)
+    scale = np.maximum(np.max(np.abs(expected), axis=0), 1e-30)
+    assert np.all(np.abs(got - expected) / scale < 1e-8)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_fused_loss_gradient(cuda, precision):
+    """L = sum_f (|h+ - d+|^2 + |hx - dx|^2) / S(f), d at perturbed
+    parameters: the fused kernel's loss and gradient against the FP64
+    oracle (analytic, and central differences step 1e-6 relative)."""
+    import torch
+
+    from waveformx_amd import adjoint
+
+    rng = np.random.default_rng(9)
+    params = oracle.draw(8, seed=6)
+    perturbed = params * (1 + 1e-2 * rng.normal(size=params.shape))
+    perturbed[:, 7] = params[:, 7]
+    if precision != "fp64":
+        params = params.astype(np.float32).astype(np.float64)
+    f = GRID if precision == "fp64" else GRID.astype(np.float32).astype(np.float64)
+    dp, dc = oracle.polarizations(perturbed, f)
+    wts = 1.0 / psd(f)
+
+    def oracle_loss(p):
+        hp, hc = oracle.polarizations(p, f)
+        return np.sum((np.abs(hp - dp) ** 2 + np.abs(hc - dc) ** 2) * wts, axis=1)
+
+    hp, hc = oracle.polarizations(params, f)
+    jp, jc = oracle.jacobian(params, f)
+    analytic = 2 * (np.einsum("bf,bfk->bk", (np.conj(hp - dp) * wts), jp).real
+                    + np.einsum("bf,bfk->bk", (np.conj(hc - dc) * wts), jc).real)
+    rtype = torch.float64 if precision == "fp64" else torch.float32
+    ctype = torch.complex128 if precision == "fp64" else torch.complex64
+    loss, grads = adjoint.loss_and_grad(torch.as_tensor(params, device=cuda),
+                                        torch.as_tensor(f, dtype=rtype, device=cuda),
+                                        torch.as_tensor(dp, dtype=ctype, device=cuda),
+                                        torch.as_tensor(dc, dtype=ctype, device=cuda),
+                                        torch.as_tensor(wts, dtype=rtype, device=cuda), precision=precision)
+    loss, grads = loss.cpu().numpy(), grads.cpu().numpy()
+    expected_loss = oracle_loss(params)
+    # central differences of the oracle, step 1e-6 max(|x|, 1)
+    fd = np.zeros_like(analytic)
+    for k in range(7):
+        h = 1e-6 * np.maximum(np.abs(params[:, k]), 1.0)
+        a, b = params.copy(), params.copy()
+        a[:, k] += h
+        b[:, k] -= h
+        fd[:, k] = (oracle_loss(a) - oracle_loss(b)) / (2 * h)
+    assert np.allclose(fd[:, :7], analytic[:, :7], rtol=1e-4, atol=1e-6 * np.abs(analytic[:, :7]).max())
+    if precision == "fp64":
+        assert np.allclose(loss, expected_loss, rtol=1e-10)
+        assert np.allclose(grads[:, :7], analytic[:, :7], rtol=1e-7, atol=1e-9 * np.abs(analytic).max())
+    else:
+        assert np.allclose(loss, expected_loss, rtol=1e-4)
+        bar = 1e-2 * np.abs(analytic[:, :7]) + 1e-3 * np.max(np.abs(analytic[:, :7]), axis=0)
+        assert np.all(np.abs(grads[:, :7] - analytic[:, :7]) <= bar)

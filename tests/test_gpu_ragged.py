@@ -1,0 +1,85 @@
+"""Ragged batches in one persistent launch, captured in a HIP graph: every
+replay (with eager work and synchronisations between replays, new
+parameters each time) against the eager evaluation and the oracle."""
+
+import numpy as np
+import pytest
+
+from conftest import compare
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def grids_for(params, rng):
+    grids = []
+    for p in params:
+        f_lower = rng.uniform(10.0, 30.0)
+        df = 1.0 / np.exp(rng.uniform(np.log(4.0), np.log(32.0)))
+        f_max = min(2048.0, 0.3 / ((p[0] + p[1]) * 4.925490947641267e-06))
+        grids.append((f_lower, df, int(np.floor((f_max - f_lower) / df + 0.5)) + 1))
+    return grids
+
+
+@pytest.mark.parametrize("precision", ["split", "fp64"])
+def test_graph_replays_match_eager_and_oracle(cuda, precision):
+    import torch
+
+    from waveformx_amd import xas
+    from waveformx_amd.ragged import RaggedPlan
+
+    rng = np.random.default_rng(21)
+    params = oracle.draw(48, seed=8, m1=(5.0, 100.0))
+    grids = grids_for(params, rng)
+    plan = RaggedPlan(grids, precision=precision, tune=False, bins_per_thread=2, blocks=512)
+    off = plan.offsets.cpu().numpy()
+    for replay in range(3):
+        p = params.copy()
+        p[:, 6] += 0.5 * replay  # new phases each replay
+        pt = torch.as_tensor(p, device=cuda)
+        hp, hc = plan(pt)
+        hp = hp.clone()
+        # eager work between replays
+        records, _ = xas.setup(pt, plan.f_min, 1e21, None, precision)
+        ep, ec = xas.evaluate_ragged(records, plan.offsets, grid_f0=plan.grid_f0, grid_df=plan.grid_df,
+                                     precision=precision, bins_per_thread=1)
+        torch.cuda.synchronize()
+        assert torch.equal(hp, ep), replay
+        hp_np = hp.cpu().numpy().astype(np.complex128)
+        for b in range(0, 48, 7):
+            f0, df, n = grids[b]
+            f = f0 + np.arange(n) * df
+            pb = p[b:b + 1]
+            if precision != "fp64":
+                f = f.astype(np.float32).astype(np.float64)
+                pb = pb.astype(np.float32).astype(np.float64)
+            op, _ = oracle.polarizations(pb, f)
+            err, mism = compare(hp_np[off[b]:off[b + 1]], op[0])
+            if precision == "fp64":
+                assert err < 1e-11, (b, err)
+            else:
+                assert mism < 1e-6, (b, mism)
+
+
+def test_tuner_and_shapes(cuda):
+    import torch
+
+    from waveformx_amd.ragged import RaggedPlan
+
+    rng = np.random.default_rng(3)
+    params = oracle.draw(64, seed=9, m1=(5.0, 100.0))
+    grids = grids_for(params, rng)
+    pt = torch.as_tensor(params, device=cuda)
+    reference = None
+    for bpt in (1, 2, 4):
+        for blocks in (256, 2048):
+            plan = RaggedPlan(grids, precision="split", tune=False, graph=False, bins_per_thread=bpt, blocks=blocks)
+            hp, _ = plan(pt)
+            if reference is None:
+                reference = hp.clone()
+            assert torch.equal(hp, reference), (bpt, blocks)
+    tuned = RaggedPlan(grids, precision="split", tune=True)
+    assert tuned.bins_per_thread in (1, 2, 4) and tuned.blocks > 0
+    hp, _ = tuned(pt)
+    torch.cuda.synchronize()
+    assert torch.equal(hp, reference)

@@ -1,0 +1,133 @@
+"""Host logic on the CPU: the value types and checks of the drop-in
+boundary (the reference's _binary.py, _options.py, _results.py semantics),
+the approximant registry's errors, and that the C-ABI library loads and
+exports every entry point include/waveformx_amd.h declares (no kernel is
+launched)."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import waveformx_amd as wx
+from waveformx_amd import _api, _types
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_header_symbols():
+    header = (ROOT / "include" / "waveformx_amd.h").read_text()
+    declared = set(re.findall(r"\b(wfx_\w+)\s*\(", header))
+    assert {"wfx_xas_setup", "wfx_xas_eval", "wfx_xas_adjoint", "wfx_last_error"} <= declared
+    from waveformx_amd import _lib
+
+    lib = _lib.lib()
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(_lib.SIGNATURES) == declared
+    assert lib.wfx_xas_record_slots() == 64
+    assert lib.wfx_xas_record_bytes(0) == 512 and lib.wfx_xas_record_bytes(2) == 256
+    # errors are reported, not raised, across the ABI
+    assert lib.wfx_xas_record_bytes(7) == -1
+    assert b"precision" in lib.wfx_last_error()
+    assert lib.wfx_xas_setup(None, 4, None, 0, 1.0, float("nan"), 0, None, None, None) == -1
+    assert b"null" in lib.wfx_last_error()
+    assert lib.wfx_xas_eval(None, -1, None, 0, 10, 0, 0, None, None, None) == -1
+    # empty batches are a no-op
+    assert lib.wfx_xas_setup(None, 0, None, 0, 1.0, float("nan"), 0, None, None, None) == 0
+
+
+def test_parameter_table_forms():
+    n = 3
+    mapping = {"mass_1": np.array([30.0, 20.0, 10.0]), "mass_2": np.array([20.0, 10.0, 5.0]),
+               "spin_1": np.zeros((n, 3)), "spin_2z": np.array([0.1, 0.2, 0.3])}
+    t = wx.parameter_table(mapping)
+    assert t.shape == (3, 12)
+    assert np.all(t[:, 8] == 1.0) and np.all(t[:, 11] == 0.0)
+    assert np.all(t[:, 7] == [0.1, 0.2, 0.3])
+    assert np.array_equal(wx.parameter_table(t), t)
+    with pytest.raises(TypeError, match="mass_2"):
+        wx.parameter_table({"mass_1": np.ones(3)})
+    with pytest.raises(TypeError, match="unknown"):
+        wx.parameter_table({**mapping, "spin": np.ones(3)})
+    with pytest.raises(ValueError, match="12 columns"):
+        wx.parameter_table(np.ones((3, 5)))
+
+
+def test_check_table_messages():
+    t = np.zeros((6, 12))
+    t[:, 0], t[:, 1], t[:, 8] = 30.0, 20.0, 400.0
+    t[1, 0] = -1.0
+    t[2, 2:5] = [0.1, 0.0, 0.2]
+    t[3, 5:8] = [0.0, 0.0, 1.5]
+    t[4, 8] = 0.0
+    t[5, 11] = -3.0
+    errors = _types.check_table(t)
+    assert sorted(errors) == [1, 2, 3, 4, 5]
+    assert "positive" in str(errors[1])
+    assert "in-plane" in str(errors[2])
+    assert "spin_2" in str(errors[3])
+    assert "luminosity_distance" in str(errors[4])
+    assert "f_ref" in str(errors[5])
+    assert all(isinstance(e, wx.InvalidParameterError) and isinstance(e, ValueError) for e in errors.values())
+
+
+def test_options():
+    o = wx.WaveformOptions()
+    assert o.scaled_strain and o.reference_mf is None
+    assert _types.strain_scale(o) == 1e21
+    assert _types.strain_scale(o.replace(scaled_strain=False)) == 1.0
+    with pytest.raises(wx.InvalidParameterError):
+        wx.WaveformOptions(reference_mf=0.02)
+    with pytest.raises(TypeError):
+        wx.WaveformOptions(scaled_strain=1)
+    with pytest.raises(TypeError, match="unexpected"):
+        _types.parse_options(None, {"bogus": 1})
+    assert _types.parse_options(None, {"reference_mf": 0.01}).reference_mf == 0.01
+    assert hash(o) == hash(wx.WaveformOptions())
+    assert wx.WaveformOptions(precession_version="MSA").precession_version == 223
+
+
+def test_frequency_grid():
+    g = wx.FrequencyGrid.uniform(20.0, 30.0, 0.25)
+    assert len(g) == 41 and g.frequencies[-1] == pytest.approx(30.0)
+    with pytest.raises(wx.InvalidParameterError):
+        wx.FrequencyGrid([20.0, 10.0])
+    with pytest.raises(wx.InvalidParameterError):
+        wx.FrequencyGrid([0.0, 10.0])
+    with pytest.raises(wx.InvalidParameterError):
+        wx.FrequencyGrid(np.ones((2, 2)))
+
+
+def test_registry():
+    assert wx.approximants() == ["IMRPhenomXAS"]
+    assert _api.model("IMRPhenomXAS") == "IMRPhenomXAS"
+    with pytest.raises(wx.UnknownApproximantError):
+        _api.model("IMRPhenomQQ")
+    with pytest.raises(wx.UnsupportedError):
+        _api.model("IMRPhenomXHM")
+    with pytest.raises(wx.UnsupportedError):
+        _api.model("NRSur7dq4")
+    with pytest.raises(ValueError):
+        wx.approximants("both")
+
+
+def test_torch_function_input_errors():
+    torch = pytest.importorskip("torch")
+    import waveformx_amd.torch as wt
+
+    f = wt.polarizations_function("IMRPhenomXAS")
+    with pytest.raises(TypeError, match="unknown"):
+        f({"mass_1": 1.0, "mass_2": 1.0, "spin_1": (0, 0, 0), "spin_2": (0, 0, 0), "mass": 2.0}, [20.0, 30.0])
+    with pytest.raises(TypeError, match="missing"):
+        f({"mass_1": 1.0}, [20.0, 30.0])
+    with pytest.raises(TypeError):
+        f(3.0, [20.0])
+    with pytest.raises(wx.UnsupportedError):
+        wt.polarizations_function("NRSur7dq4")
+    if not torch.cuda.is_available():
+        # there is no CPU path: a call without a HIP device raises
+        with pytest.raises(RuntimeError, match="HIP device"):
+            f(wx.Binary(30.0, 20.0, (0, 0, 0), (0, 0, 0)), torch.tensor([20.0, 30.0]))

@@ -1,0 +1,97 @@
+"""Gradients of IMRPhenomXAS on the GPU through the hand-written adjoint.
+
+:func:`vjp` is the backward pass of the torch function (the reference's
+``jax.vjp``, ``src/waveformx/torch.py:87-119``): the FP64 setup with
+forward-mode derivatives gives each binary's record and d record /
+d(m1, m2, chi1, chi2, d_L, inclination, phi_ref, f_ref); the adjoint kernel
+recomputes the formula per bin, reduces the cotangents into d L / d record
+and contracts with those derivatives. Nothing of the forward pass is stored.
+
+:func:`loss_and_grad` fuses the forward pass with a noise-weighted
+least-squares loss ``L = sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2)``: it reads
+the data and writes only each binary's loss and gradient.
+"""
+
+from __future__ import annotations
+
+from typing import Optional, Tuple
+
+import torch
+
+from . import xas
+from ._lib import call
+from .xas import _code, _ptr, _require_device, _stream
+
+__all__ = ["vjp", "adjoint", "loss_and_grad", "PARAMETERS"]
+
+PARAMETERS = ("mass_1", "mass_2", "chi_1", "chi_2", "luminosity_distance", "inclination", "phi_ref", "f_ref")
+
+
+def _records(params, freqs, amp_scale, reference_mf, precision):
+    records64, jac, _ = xas.setup_jvp(params, freqs[:1], amp_scale, reference_mf)
+    if precision == "fp64":
+        return records64, jac
+    records, _ = xas.setup(params, freqs[:1], amp_scale, reference_mf, precision)
+    return records, jac
+
+
+def adjoint(records: torch.Tensor, jac: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor,
+            cot_cross: torch.Tensor, precision: str) -> torch.Tensor:
+    """(B, 8) float64 gradients from the cotangents (B, N) (complex, the
+    torch convention dL/dRe + i dL/dIm) on the shared grid ``freqs``."""
+    for name, t in (("records", records), ("jac", jac), ("freqs", freqs), ("cot_plus", cot_plus),
+                    ("cot_cross", cot_cross)):
+        _require_device(t, name)
+    nb, n = records.shape[0], freqs.shape[0]
+    ctype = torch.complex128 if precision == "fp64" else torch.complex64
+    if cot_plus.shape != (nb, n) or cot_cross.shape != (nb, n) or cot_plus.dtype != ctype or \
+            cot_cross.dtype != ctype:
+        raise ValueError(f"cotangents must be ({nb}, {n}) {ctype}")
+    grads = torch.empty((nb, 8), dtype=torch.float64, device=records.device)
+    call("wfx_xas_adjoint", _ptr(records), _ptr(jac), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
+         _code(precision), _ptr(cot_plus), _ptr(cot_cross), _ptr(grads), _stream(records.device))
+    return grads
+
+
+def vjp(params: torch.Tensor, freqs: torch.Tensor, grad_plus: Optional[torch.Tensor],
+        grad_cross: Optional[torch.Tensor], amp_scale: float, reference_mf: Optional[float], precision: str,
+        wrt_grid: bool = False) -> Tuple[torch.Tensor, Optional[torch.Tensor]]:
+    """Gradients (B, 8) with respect to the packed parameters; the grid's
+    gradient is not computed (None)."""
+    nb, n = params.shape[0], freqs.shape[0]
+    ctype = torch.complex128 if precision == "fp64" else torch.complex64
+    zeros = None
+    if grad_plus is None or grad_cross is None:
+        zeros = torch.zeros((nb, n), dtype=ctype, device=params.device)
+    gp = zeros if grad_plus is None else grad_plus.to(ctype).reshape(nb, n).contiguous()
+    gc = zeros if grad_cross is None else grad_cross.to(ctype).reshape(nb, n).contiguous()
+    records, jac = _records(params, freqs, amp_scale, reference_mf, precision)
+    if wrt_grid:
+        raise NotImplementedError("the gradient with respect to the frequencies is not implemented")
+    return adjoint(records, jac, freqs, gp, gc, precision), None
+
+
+def loss_and_grad(params: torch.Tensor, freqs: torch.Tensor, data_plus: torch.Tensor, data_cross: torch.Tensor,
+                  weights: torch.Tensor, amp_scale: float = 1e21, reference_mf: Optional[float] = None,
+                  precision: str = "split", records: Optional[Tuple[torch.Tensor, torch.Tensor]] = None
+                  ) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Per-binary loss (B,) and gradient (B, 8), float64, of
+    ``sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2)`` in one fused pass."""
+    for name, t in (("params", params), ("freqs", freqs), ("data_plus", data_plus), ("data_cross", data_cross),
+                    ("weights", weights)):
+        _require_device(t, name)
+    nb, n = params.shape[0], freqs.shape[0]
+    rdtype = torch.float64 if precision == "fp64" else torch.float32
+    ctype = torch.complex128 if precision == "fp64" else torch.complex64
+    if data_plus.shape != (nb, n) or data_cross.shape != (nb, n) or data_plus.dtype != ctype or \
+            data_cross.dtype != ctype:
+        raise ValueError(f"data must be ({nb}, {n}) {ctype}")
+    if weights.shape != (n,) or weights.dtype != rdtype:
+        raise ValueError(f"weights must be ({n},) {rdtype}")
+    rec, jac = records if records is not None else _records(params, freqs, amp_scale, reference_mf, precision)
+    loss = torch.empty((nb,), dtype=torch.float64, device=params.device)
+    grads = torch.empty((nb, 8), dtype=torch.float64, device=params.device)
+    call("wfx_xas_loss_adjoint", _ptr(rec), _ptr(jac), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
+         _code(precision), _ptr(data_plus), _ptr(data_cross), _ptr(weights), _ptr(loss), _ptr(grads),
+         _stream(params.device))
+    return loss, grads

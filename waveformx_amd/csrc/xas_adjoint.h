@@ -1,0 +1,181 @@
+// The adjoint of the per-bin formula of IMRPhenomXAS with respect to the
+// binary's record: given the cotangents g+ = dL/dRe h+ + i dL/dIm h+ (and
+// gx), add dL/d(record slot) of one bin to acc[slot].
+//
+// With E = e^{i phi}, h+ = kp A E and hx = i kc A E:
+//   dL/dA   = Re P,  dL/dphi = -A Im P,
+//   P = conj(g+) kp E + conj(gx) i kc E,
+//   dL/dkp  = A Re(conj(g+) E),  dL/dkc = -A Im(conj(gx) E);
+// then through the phase sums (the basis values of each region), the
+// arctangent a atan(w (f - shift)), the leading term T0 f^(-5/3) - A, and
+// the three amplitude forms. The region bounds and the cutoff carry no
+// derivative (the reference's jnp.where selections). Intermediates are
+// recomputed, none stored.
+#pragma once
+
+#include "xas_point.h"
+
+namespace wfx {
+
+__device__ inline void m_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
+__device__ inline void m_sincos(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ inline float d_atan(float x) { return atanf(x); }
+__device__ inline double d_atan(double x) { return atan(x); }
+__device__ inline float d_exp(float x) { return expf(x); }
+__device__ inline double d_exp(double x) { return exp(x); }
+
+template <class T>
+struct BinPowers {
+    T f, t, m1, lf, m76;
+    double m53;
+};
+
+__device__ inline BinPowers<float> bin_powers(const PowF32& p) {
+    return BinPowers<float>{p.f, p.t, p.m1, p.lf, p.m76, p.m53};
+}
+__device__ inline BinPowers<double> bin_powers(const FreqPowers<double>& p) {
+    return BinPowers<double>{p.f, p.t, p.m1, p.log_f, p.m76, p.m53};
+}
+
+// h+, hx of one bin (the forward formula in T; the leading term and its
+// reduction in FP64)
+template <class T>
+__device__ inline T phase_and_amp(const T* __restrict__ c, const BinPowers<T>& p, bool& past, bool& ringdown,
+                                  T& z, T& amp) {
+    const T f = p.f, t = p.t, m1 = p.m1, lf = p.lf;
+    past = f >= c[kFPhaseIn];
+    ringdown = f > c[kFPhaseIm];
+    T rest;
+    z = T(0);
+    if (!past) {
+        const T t2 = t * t, mt1 = t2 * m1, mt2 = mt1 * mt1;
+        rest = c[kInsConst] + c[kInsM1] * m1 + c[kInsM23] * mt2 + c[kInsM13] * mt1 + c[kInsLog] * lf +
+               c[kInsP13] * t + c[kInsP13L] * (t * lf) + c[kInsP23] * t2 + c[kInsP1] * f + c[kInsP1L] * (f * lf) +
+               c[kInsP43] * (f * t) + c[kInsP53] * (f * t2) + c[kInsP2] * (f * f);
+    } else if (!ringdown) {
+        const T m2 = m1 * m1;
+        z = d_atan(c[kAtanW1] * (f - c[kShift]));
+        rest = c[kIntConst] + c[kIntP1] * f + c[kIntLog] * lf + c[kIntM1] * m1 + c[kIntM2] * m2 +
+               c[kIntM3] * (m2 * m1) + c[kAtanA1] * z;
+    } else {
+        const T m2 = m1 * m1;
+        z = d_atan(c[kAtanW2] * (f - c[kShift]));
+        rest = c[kRdConst] + c[kRdP1] * f + c[kRdP23] * (t * t) + c[kRdM1] * m1 + c[kRdM3] * (m2 * m1) +
+               c[kAtanA2] * z;
+    }
+    double full = (double)rest - ((double)c[kAnchor] + (double)c[kAnchorLo]);
+    if (!past) full = fma((double)c[kT0] + (double)c[kT0Lo], p.m53, full);
+    const double turns = rint(full * (1.0 / kTwoPi));
+    const T phi = (T)fma(-turns, kTwoPi, full);
+
+    if (f > c[kFAmpRd]) {
+        const T u = c[kAmpRdS] * (f - c[kShift]);
+        amp = c[kAmpRdN] * p.m76 * d_exp(-c[kAmpRdGr] * u) / (u * u + c[kAmpRdG]);
+    } else if (f >= c[kFAmpIn]) {
+        T den = c[kAmpIntD + 4];
+        for (int j = 3; j >= 0; --j) den = den * f + c[kAmpIntD + j];
+        amp = c[kAmpIntN] / den;
+    } else {
+        T poly = c[kAmpIns + 9];
+        for (int k = 8; k >= 0; --k) poly = poly * t + c[kAmpIns + k];
+        amp = p.m76 * poly;
+    }
+    if (!(f <= c[kFCut])) amp = T(0);
+    return phi;
+}
+
+template <class T>
+__device__ inline void adjoint_point(const T* __restrict__ c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re,
+                                     T gx_im, T* acc) {
+    bool past, ringdown;
+    T z, amp;
+    const T phi = phase_and_amp(c, p, past, ringdown, z, amp);
+    if (!(p.f <= c[kFCut])) return;  // zero above the cutoff, whatever the record
+    T s, co;
+    m_sincos(phi, &s, &co);
+    const T kp = c[kKp], kc = c[kKc];
+    const T re_p = kp * (gp_re * co + gp_im * s) + kc * (gx_im * co - gx_re * s);
+    const T im_p = kp * (gp_re * s - gp_im * co) + kc * (gx_re * co + gx_im * s);
+    const T g_amp = re_p;
+    const T g_phi = -amp * im_p;
+    acc[kKp] += amp * (gp_re * co + gp_im * s);
+    acc[kKc] += amp * (gx_im * co - gx_re * s);
+
+    const T f = p.f, t = p.t, m1 = p.m1, lf = p.lf;
+    // the phase
+    acc[kAnchor] -= g_phi;
+    if (!past) {
+        const T t2 = t * t, mt1 = t2 * m1, mt2 = mt1 * mt1;
+        acc[kT0] += g_phi * (T)p.m53;
+        acc[kInsConst] += g_phi;
+        acc[kInsM1] += g_phi * m1;
+        acc[kInsM23] += g_phi * mt2;
+        acc[kInsM13] += g_phi * mt1;
+        acc[kInsLog] += g_phi * lf;
+        acc[kInsP13] += g_phi * t;
+        acc[kInsP13L] += g_phi * (t * lf);
+        acc[kInsP23] += g_phi * t2;
+        acc[kInsP1] += g_phi * f;
+        acc[kInsP1L] += g_phi * (f * lf);
+        acc[kInsP43] += g_phi * (f * t);
+        acc[kInsP53] += g_phi * (f * t2);
+        acc[kInsP2] += g_phi * (f * f);
+    } else {
+        const T y = f - c[kShift];
+        const T a = ringdown ? c[kAtanA2] : c[kAtanA1];
+        const T w = ringdown ? c[kAtanW2] : c[kAtanW1];
+        const T wy = w * y;
+        const T datan = T(1) / (T(1) + wy * wy);
+        acc[kShift] -= g_phi * a * w * datan;
+        const T m2 = m1 * m1;
+        if (ringdown) {
+            acc[kAtanA2] += g_phi * z;
+            acc[kAtanW2] += g_phi * a * y * datan;
+            acc[kRdConst] += g_phi;
+            acc[kRdP1] += g_phi * f;
+            acc[kRdP23] += g_phi * (t * t);
+            acc[kRdM1] += g_phi * m1;
+            acc[kRdM3] += g_phi * (m2 * m1);
+        } else {
+            acc[kAtanA1] += g_phi * z;
+            acc[kAtanW1] += g_phi * a * y * datan;
+            acc[kIntConst] += g_phi;
+            acc[kIntP1] += g_phi * f;
+            acc[kIntLog] += g_phi * lf;
+            acc[kIntM1] += g_phi * m1;
+            acc[kIntM2] += g_phi * m2;
+            acc[kIntM3] += g_phi * (m2 * m1);
+        }
+    }
+    // the amplitude
+    if (f > c[kFAmpRd]) {
+        const T y = f - c[kShift];
+        const T u = c[kAmpRdS] * y;
+        const T q = u * u + c[kAmpRdG];
+        const T e = p.m76 * d_exp(-c[kAmpRdGr] * u) / q;  // dA/dn
+        acc[kAmpRdN] += g_amp * e;
+        acc[kAmpRdGr] -= g_amp * u * amp;
+        acc[kAmpRdG] -= g_amp * amp / q;
+        const T dadu = amp * (-c[kAmpRdGr] - T(2) * u / q);
+        acc[kAmpRdS] += g_amp * dadu * y;
+        acc[kShift] -= g_amp * dadu * c[kAmpRdS];
+    } else if (f >= c[kFAmpIn]) {
+        T den = c[kAmpIntD + 4];
+        for (int j = 3; j >= 0; --j) den = den * f + c[kAmpIntD + j];
+        const T inv = T(1) / den;
+        acc[kAmpIntN] += g_amp * inv;
+        T q = -g_amp * amp * inv;
+        for (int j = 0; j < 5; ++j) {
+            acc[kAmpIntD + j] += q;
+            q *= f;
+        }
+    } else {
+        T q = g_amp * p.m76;
+        for (int k = 0; k < 10; ++k) {
+            acc[kAmpIns + k] += q;
+            q *= t;
+        }
+    }
+}
+
+}  // namespace wfx

@@ -13,7 +13,7 @@
 //    are remapped so that one group's frequency tiles share an XCD's L2;
 //  * ragged: a persistent grid walks (binary, tile) work items from a
 //    prefix sum of tiles per binary; one thread per bin.
-#include "xas_point.h"
+#include "xas_adjoint.h"
 
 #include <atomic>
 #include <cstdio>
@@ -246,7 +246,9 @@ __device__ inline double ragged_freq(const F* freqs, const double* f0, const dou
     return freqs ? (double)freqs[i] : f0[b] + (double)k * df[b];
 }
 
-template <class F>
+// a work item is a tile of kBlock * BPT consecutive bins of one binary;
+// thread t takes bins t, t + kBlock, ... of the tile
+template <int BPT, class F>
 __global__ __launch_bounds__(kBlock) void eval_f64_ragged(const double* __restrict__ records, int64_t nb,
                                                           const F* __restrict__ freqs, const double* __restrict__ f0,
                                                           const double* __restrict__ df,
@@ -256,18 +258,22 @@ __global__ __launch_bounds__(kBlock) void eval_f64_ragged(const double* __restri
     const int64_t total = tiles[nb];
     for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
         const int64_t b = owner(tiles, nb, w);
-        const int64_t k = (w - tiles[b]) * kBlock + threadIdx.x;
-        const int64_t i = offsets[b] + k;
-        if (i >= offsets[b + 1]) continue;
-        const FreqPowers<double> fp = freq_powers<double>(ragged_freq(freqs, f0, df, b, i, k));
-        double a, bb, c, d;
-        point_f64(records + b * kSlots, fp, a, bb, c, d);
-        hp[i] = make_double2(a, bb);
-        hc[i] = make_double2(c, d);
+        const int64_t start = offsets[b], stop = offsets[b + 1];
+#pragma unroll
+        for (int j = 0; j < BPT; ++j) {
+            const int64_t k = (w - tiles[b]) * (kBlock * BPT) + j * kBlock + threadIdx.x;
+            const int64_t i = start + k;
+            if (i >= stop) break;
+            const FreqPowers<double> fp = freq_powers<double>(ragged_freq(freqs, f0, df, b, i, k));
+            double a, bb, c, d;
+            point_f64(records + b * kSlots, fp, a, bb, c, d);
+            hp[i] = make_double2(a, bb);
+            hc[i] = make_double2(c, d);
+        }
     }
 }
 
-template <int L, class F>
+template <int L, int BPT, class F>
 __global__ __launch_bounds__(kBlock) void eval_f32_ragged(const float* __restrict__ records, int64_t nb,
                                                           const F* __restrict__ freqs, const double* __restrict__ f0,
                                                           const double* __restrict__ df,
@@ -277,21 +283,25 @@ __global__ __launch_bounds__(kBlock) void eval_f32_ragged(const float* __restric
     const int64_t total = tiles[nb];
     for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
         const int64_t b = owner(tiles, nb, w);
-        const int64_t k = (w - tiles[b]) * kBlock + threadIdx.x;
-        const int64_t i = offsets[b] + k;
-        if (i >= offsets[b + 1]) continue;
-        const PowF32 p = pow_f32((float)ragged_freq(freqs, f0, df, b, i, k));
-        float2 a, c;
-        point_f32<L>(records + b * kSlots, p, a, c);
-        hp[i] = a;
-        hc[i] = c;
+        const int64_t start = offsets[b], stop = offsets[b + 1];
+#pragma unroll
+        for (int j = 0; j < BPT; ++j) {
+            const int64_t k = (w - tiles[b]) * (kBlock * BPT) + j * kBlock + threadIdx.x;
+            const int64_t i = start + k;
+            if (i >= stop) break;
+            const PowF32 p = pow_f32((float)ragged_freq(freqs, f0, df, b, i, k));
+            float2 a, c;
+            point_f32<L>(records + b * kSlots, p, a, c);
+            hp[i] = a;
+            hc[i] = c;
+        }
     }
 }
 
-// tiles[b] = sum_{j < b} ceil(len_j / kBlock): one block, a running prefix
+// tiles[b] = sum_{j < b} ceil(len_j / tile): one block, a running prefix
 // over chunks of 1024 binaries
 __global__ __launch_bounds__(1024) void tile_prefix_kernel(const int64_t* __restrict__ offsets, int64_t nb,
-                                                           int64_t* __restrict__ tiles) {
+                                                           int64_t tile, int64_t* __restrict__ tiles) {
     __shared__ int64_t scan[1024];
     __shared__ int64_t carry;
     if (threadIdx.x == 0) carry = 0;
@@ -300,7 +310,7 @@ __global__ __launch_bounds__(1024) void tile_prefix_kernel(const int64_t* __rest
         const int64_t b = base + threadIdx.x;
         int64_t len = b < nb ? offsets[b + 1] - offsets[b] : 0;
         if (len < 0) len = 0;
-        scan[threadIdx.x] = (len + kBlock - 1) / kBlock;
+        scan[threadIdx.x] = (len + tile - 1) / tile;
         __syncthreads();
         for (int s = 1; s < 1024; s <<= 1) {
             const int64_t v = threadIdx.x >= (unsigned)s ? scan[threadIdx.x - s] : 0;
@@ -314,6 +324,140 @@ __global__ __launch_bounds__(1024) void tile_prefix_kernel(const int64_t* __rest
         __syncthreads();
     }
     if (threadIdx.x == 0) tiles[0] = 0;
+}
+
+// ---------------------------------------------------------------- adjoint
+
+// dL/dtheta of each binary: one block per binary walks its frequencies,
+// each thread accumulating dL/d(record slot) of its bins (FP32 or FP64 as
+// the evaluation); the block then reduces the slots in FP64 and contracts
+// them with the record's derivatives jac (B, 64, 8) from the FP64 setup.
+// kLoss: the cotangents come from L = sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2)
+// (data d, weights w), whose per-binary value is written to loss.
+constexpr int kFromCotangents = 0;
+constexpr int kLoss = 1;
+
+template <class T>
+__device__ inline T wave_sum(T v) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <class T>
+__device__ inline BinPowers<T> adjoint_powers(double f);
+template <>
+__device__ inline BinPowers<float> adjoint_powers<float>(double f) {
+    return bin_powers(pow_f32((float)f));
+}
+template <>
+__device__ inline BinPowers<double> adjoint_powers<double>(double f) {
+    return bin_powers(freq_powers<double>(f));
+}
+
+template <class T, int Mode, class F>
+__global__ __launch_bounds__(kBlock) void adjoint_kernel(const T* __restrict__ records, const double* __restrict__ jac,
+                                                         int64_t nb, const F* __restrict__ freqs, int64_t n,
+                                                         const T* __restrict__ in_plus, const T* __restrict__ in_cross,
+                                                         const T* __restrict__ weights, double* __restrict__ grads,
+                                                         double* __restrict__ loss) {
+    const int64_t b = blockIdx.x;
+    if (b >= nb) return;
+    const T* c = records + b * kSlots;
+    T acc[kSlots];
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) acc[k] = T(0);
+    T loss_acc = T(0);
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) {
+        const BinPowers<T> p = adjoint_powers<T>((double)freqs[i]);
+        const int64_t o = 2 * (b * n + i);
+        T gp_re = in_plus[o], gp_im = in_plus[o + 1], gx_re = in_cross[o], gx_im = in_cross[o + 1];
+        if (Mode == kLoss) {
+            bool past, ringdown;
+            T z, amp;
+            const T phi = phase_and_amp(c, p, past, ringdown, z, amp);
+            T s, co;
+            m_sincos(phi, &s, &co);
+            const T w = weights[i];
+            const T ap = c[kKp] * amp, ac = c[kKc] * amp;
+            const T rp_re = ap * co - gp_re, rp_im = ap * s - gp_im;
+            const T rx_re = -ac * s - gx_re, rx_im = ac * co - gx_im;
+            loss_acc += w * (rp_re * rp_re + rp_im * rp_im + rx_re * rx_re + rx_im * rx_im);
+            const T two_w = T(2) * w;
+            gp_re = two_w * rp_re;
+            gp_im = two_w * rp_im;
+            gx_re = two_w * rx_re;
+            gx_im = two_w * rx_im;
+        }
+        adjoint_point(c, p, gp_re, gp_im, gx_re, gx_im, acc);
+    }
+    __shared__ double partial[kBlock / 64][kSlots + 1];
+    __shared__ double slots[kSlots];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) {
+        const double v = wave_sum((double)acc[k]);
+        if (lane == 0) partial[wave][k] = v;
+    }
+    if (Mode == kLoss) {
+        const double v = wave_sum((double)loss_acc);
+        if (lane == 0) partial[wave][kSlots] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kSlots) {
+        double v = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) v += partial[w][threadIdx.x];
+        // the FP32 record's T0 and A enter in FP64 through hi + lo
+        slots[threadIdx.x] = threadIdx.x == kT0Lo || threadIdx.x == kAnchorLo || threadIdx.x >= 60 ? 0.0 : v;
+    }
+    if (Mode == kLoss && threadIdx.x == kBlock - 1) {
+        double v = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) v += partial[w][kSlots];
+        loss[b] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const double* jb = jac + b * kSlots * 8;
+        double g = 0.0;
+        for (int k = 0; k < 60; ++k) g += slots[k] * jb[k * 8 + threadIdx.x];
+        grads[b * 8 + threadIdx.x] = g;
+    }
+}
+
+struct RaggedArgs {
+    const void* records;
+    int64_t nb;
+    const void* freqs;
+    const double* f0;
+    const double* df;
+    const int64_t* offsets;
+    const int64_t* tiles;
+    void* hp;
+    void* hc;
+    int blocks;
+    hipStream_t stream;
+};
+
+template <int BPT, class F>
+void launch_ragged(const RaggedArgs& a, int precision) {
+    const F* freqs = (const F*)a.freqs;
+    if (precision == kPrecF64) {
+        eval_f64_ragged<BPT, F><<<a.blocks, kBlock, 0, a.stream>>>((const double*)a.records, a.nb, freqs, a.f0, a.df,
+                                                                    a.offsets, a.tiles, (double2*)a.hp,
+                                                                    (double2*)a.hc);
+        return;
+    }
+    const float* rec = (const float*)a.records;
+    float2* hp = (float2*)a.hp;
+    float2* hc = (float2*)a.hc;
+    if (precision == kPrecSplit)
+        eval_f32_ragged<kLeadF64, BPT, F><<<a.blocks, kBlock, 0, a.stream>>>(rec, a.nb, freqs, a.f0, a.df, a.offsets,
+                                                                            a.tiles, hp, hc);
+    else if (precision == kPrecSplitDF)
+        eval_f32_ragged<kLeadDF, BPT, F><<<a.blocks, kBlock, 0, a.stream>>>(rec, a.nb, freqs, a.f0, a.df, a.offsets,
+                                                                           a.tiles, hp, hc);
+    else
+        eval_f32_ragged<kLeadF32, BPT, F><<<a.blocks, kBlock, 0, a.stream>>>(rec, a.nb, freqs, a.f0, a.df, a.offsets,
+                                                                            a.tiles, hp, hc);
 }
 
 int grid_blocks(int64_t nblocks, const char* what) {
@@ -431,56 +575,111 @@ int wfx_xas_eval(const void* records, int64_t n_binaries, const void* freqs, int
     return check_launch("wfx_xas_eval");
 }
 
-int wfx_xas_tile_prefix(const int64_t* offsets, int64_t n_binaries, int64_t* tiles, void* stream) {
+int wfx_xas_tile_prefix(const int64_t* offsets, int64_t n_binaries, int bins_per_thread, int64_t* tiles,
+                        void* stream) {
     if (n_binaries < 0) return fail("wfx_xas_tile_prefix: negative size");
     if (!offsets || !tiles) return fail("wfx_xas_tile_prefix: null pointer");
-    tile_prefix_kernel<<<1, 1024, 0, (hipStream_t)stream>>>(offsets, n_binaries, tiles);
+    if (bins_per_thread != 1 && bins_per_thread != 2 && bins_per_thread != 4)
+        return fail("wfx_xas_tile_prefix: bins_per_thread must be 1, 2 or 4");
+    tile_prefix_kernel<<<1, 1024, 0, (hipStream_t)stream>>>(offsets, n_binaries, (int64_t)kBlock * bins_per_thread,
+                                                              tiles);
     return check_launch("wfx_xas_tile_prefix");
 }
 
 int wfx_xas_eval_ragged(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64,
                         const double* grid_f0, const double* grid_df, const int64_t* offsets, const int64_t* tiles,
-                        int precision, int blocks, void* hplus, void* hcross, void* stream) {
+                        int precision, int bins_per_thread, int blocks, void* hplus, void* hcross, void* stream) {
     if (n_binaries < 0) return fail("wfx_xas_eval_ragged: negative size");
     if (n_binaries == 0) return 0;
     if (!records || !offsets || !tiles || !hplus || !hcross) return fail("wfx_xas_eval_ragged: null pointer");
     if (!freqs && (!grid_f0 || !grid_df)) return fail("wfx_xas_eval_ragged: give freqs or (grid_f0, grid_df)");
+    if (bins_per_thread != 1 && bins_per_thread != 2 && bins_per_thread != 4)
+        return fail("wfx_xas_eval_ragged: bins_per_thread must be 1, 2 or 4");
+    if (precision < kPrecF64 || precision > kPrecSplitDF) return fail("wfx_xas_eval_ragged: unknown precision");
     if (blocks <= 0) blocks = 8 * cu_count();
-    hipStream_t s = (hipStream_t)stream;
-#define WFX_RAGGED_F32(L)                                                                                      \
-    if (freqs_f64 || !freqs)                                                                                   \
-        eval_f32_ragged<L, double><<<blocks, kBlock, 0, s>>>((const float*)records, n_binaries,                \
-                                                             (const double*)freqs, grid_f0, grid_df, offsets,  \
-                                                             tiles, (float2*)hplus, (float2*)hcross);          \
-    else                                                                                                       \
-        eval_f32_ragged<L, float><<<blocks, kBlock, 0, s>>>((const float*)records, n_binaries,                 \
-                                                            (const float*)freqs, grid_f0, grid_df, offsets,    \
-                                                            tiles, (float2*)hplus, (float2*)hcross);
-    switch (precision) {
-        case kPrecF64:
-            if (freqs_f64 || !freqs)
-                eval_f64_ragged<double><<<blocks, kBlock, 0, s>>>((const double*)records, n_binaries,
-                                                                  (const double*)freqs, grid_f0, grid_df, offsets,
-                                                                  tiles, (double2*)hplus, (double2*)hcross);
-            else
-                eval_f64_ragged<float><<<blocks, kBlock, 0, s>>>((const double*)records, n_binaries,
-                                                                 (const float*)freqs, grid_f0, grid_df, offsets,
-                                                                 tiles, (double2*)hplus, (double2*)hcross);
+    const RaggedArgs a{records, n_binaries, freqs, grid_f0, grid_df, offsets, tiles, hplus, hcross, blocks,
+                       (hipStream_t)stream};
+    const bool f64 = freqs_f64 || !freqs;
+    switch (bins_per_thread) {
+        case 1:
+            f64 ? launch_ragged<1, double>(a, precision) : launch_ragged<1, float>(a, precision);
             break;
-        case kPrecSplit:
-            WFX_RAGGED_F32(kLeadF64)
-            break;
-        case kPrecSplitDF:
-            WFX_RAGGED_F32(kLeadDF)
-            break;
-        case kPrecF32:
-            WFX_RAGGED_F32(kLeadF32)
+        case 2:
+            f64 ? launch_ragged<2, double>(a, precision) : launch_ragged<2, float>(a, precision);
             break;
         default:
-            return fail("wfx_xas_eval_ragged: unknown precision");
+            f64 ? launch_ragged<4, double>(a, precision) : launch_ragged<4, float>(a, precision);
+            break;
     }
-#undef WFX_RAGGED_F32
     return check_launch("wfx_xas_eval_ragged");
+}
+
+int wfx_xas_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs, int freqs_f64,
+                    int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, double* grads,
+                    void* stream) {
+    if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xas_adjoint: negative size");
+    if (n_binaries == 0) return 0;
+    if (!records || !jacobian || !grads || (n_freqs && (!freqs || !cot_plus || !cot_cross)))
+        return fail("wfx_xas_adjoint: null pointer");
+    const int blocks = grid_blocks(n_binaries, "wfx_xas_adjoint: too many binaries");
+    if (blocks < 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == kPrecF64) {
+        if (freqs_f64)
+            adjoint_kernel<double, kFromCotangents, double><<<blocks, kBlock, 0, s>>>(
+                (const double*)records, jacobian, n_binaries, (const double*)freqs, n_freqs, (const double*)cot_plus,
+                (const double*)cot_cross, nullptr, grads, nullptr);
+        else
+            adjoint_kernel<double, kFromCotangents, float><<<blocks, kBlock, 0, s>>>(
+                (const double*)records, jacobian, n_binaries, (const float*)freqs, n_freqs, (const double*)cot_plus,
+                (const double*)cot_cross, nullptr, grads, nullptr);
+    } else if (precision == kPrecF32 || precision == kPrecSplit || precision == kPrecSplitDF) {
+        if (freqs_f64)
+            adjoint_kernel<float, kFromCotangents, double><<<blocks, kBlock, 0, s>>>(
+                (const float*)records, jacobian, n_binaries, (const double*)freqs, n_freqs, (const float*)cot_plus,
+                (const float*)cot_cross, nullptr, grads, nullptr);
+        else
+            adjoint_kernel<float, kFromCotangents, float><<<blocks, kBlock, 0, s>>>(
+                (const float*)records, jacobian, n_binaries, (const float*)freqs, n_freqs, (const float*)cot_plus,
+                (const float*)cot_cross, nullptr, grads, nullptr);
+    } else {
+        return fail("wfx_xas_adjoint: unknown precision");
+    }
+    return check_launch("wfx_xas_adjoint");
+}
+
+int wfx_xas_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
+                         int freqs_f64, int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
+                         const void* weights, double* loss, double* grads, void* stream) {
+    if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xas_loss_adjoint: negative size");
+    if (n_binaries == 0) return 0;
+    if (!records || !jacobian || !grads || !loss || (n_freqs && (!freqs || !data_plus || !data_cross || !weights)))
+        return fail("wfx_xas_loss_adjoint: null pointer");
+    const int blocks = grid_blocks(n_binaries, "wfx_xas_loss_adjoint: too many binaries");
+    if (blocks < 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == kPrecF64) {
+        if (freqs_f64)
+            adjoint_kernel<double, kLoss, double><<<blocks, kBlock, 0, s>>>(
+                (const double*)records, jacobian, n_binaries, (const double*)freqs, n_freqs,
+                (const double*)data_plus, (const double*)data_cross, (const double*)weights, grads, loss);
+        else
+            adjoint_kernel<double, kLoss, float><<<blocks, kBlock, 0, s>>>(
+                (const double*)records, jacobian, n_binaries, (const float*)freqs, n_freqs, (const double*)data_plus,
+                (const double*)data_cross, (const double*)weights, grads, loss);
+    } else if (precision == kPrecF32 || precision == kPrecSplit || precision == kPrecSplitDF) {
+        if (freqs_f64)
+            adjoint_kernel<float, kLoss, double><<<blocks, kBlock, 0, s>>>(
+                (const float*)records, jacobian, n_binaries, (const double*)freqs, n_freqs, (const float*)data_plus,
+                (const float*)data_cross, (const float*)weights, grads, loss);
+        else
+            adjoint_kernel<float, kLoss, float><<<blocks, kBlock, 0, s>>>(
+                (const float*)records, jacobian, n_binaries, (const float*)freqs, n_freqs, (const float*)data_plus,
+                (const float*)data_cross, (const float*)weights, grads, loss);
+    } else {
+        return fail("wfx_xas_loss_adjoint: unknown precision");
+    }
+    return check_launch("wfx_xas_loss_adjoint");
 }
 
 }  // extern "C"

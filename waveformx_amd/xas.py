@@ -132,23 +132,26 @@ def evaluate(records: torch.Tensor, freqs: torch.Tensor, precision: str = "split
     return hp, hc
 
 
-def tile_prefix(offsets: torch.Tensor) -> torch.Tensor:
-    """The prefix sum (B+1,) of 256-bin tiles per binary, for
-    :func:`evaluate_ragged`."""
+def tile_prefix(offsets: torch.Tensor, bins_per_thread: int = 1) -> torch.Tensor:
+    """The prefix sum (B+1,) of the (256 x bins_per_thread)-bin tiles per
+    binary, for :func:`evaluate_ragged`."""
     _require_device(offsets, "offsets")
     nb = offsets.shape[0] - 1
     tiles = torch.empty((nb + 1,), dtype=torch.int64, device=offsets.device)
-    call("wfx_xas_tile_prefix", _ptr(offsets), nb, _ptr(tiles), _stream(offsets.device))
+    call("wfx_xas_tile_prefix", _ptr(offsets), nb, int(bins_per_thread), _ptr(tiles), _stream(offsets.device))
     return tiles
 
 
 def evaluate_ragged(records: torch.Tensor, offsets: torch.Tensor, freqs: Optional[torch.Tensor] = None,
                     grid_f0: Optional[torch.Tensor] = None, grid_df: Optional[torch.Tensor] = None,
                     precision: str = "split", tiles: Optional[torch.Tensor] = None, blocks: int = 0,
-                    total: Optional[int] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+                    total: Optional[int] = None, bins_per_thread: int = 1,
+                    out: Optional[Tuple[torch.Tensor, torch.Tensor]] = None) -> Tuple[torch.Tensor, torch.Tensor]:
     """(h+, hx) flat (total,) of binaries on their own grids: binary b owns
     [offsets[b], offsets[b+1]) of the concatenated ``freqs``, or the uniform
-    grid ``grid_f0[b] + k grid_df[b]`` of that length."""
+    grid ``grid_f0[b] + k grid_df[b]`` of that length. ``tiles`` (from
+    :func:`tile_prefix` with the same ``bins_per_thread``) and ``total``
+    given, nothing synchronises with the host (graph capture)."""
     _require_device(records, "records")
     _require_device(offsets, "offsets")
     if offsets.dtype != torch.int64:
@@ -159,17 +162,28 @@ def evaluate_ragged(records: torch.Tensor, offsets: torch.Tensor, freqs: Optiona
     if freqs is None and (grid_f0 is None or grid_df is None):
         raise ValueError("give freqs or (grid_f0, grid_df)")
     if total is None:
-        total = int(offsets[-1].item()) if freqs is None else freqs.shape[0]
+        total = int(offsets[-1].item())
+        if freqs is not None and freqs.shape[0] != total:
+            raise ValueError("offsets[-1] must equal the number of frequencies")
+    if freqs is not None and freqs.shape[0] < total:
+        raise ValueError("fewer frequencies than offsets[-1]")
     if tiles is None:
-        tiles = tile_prefix(offsets)
+        tiles = tile_prefix(offsets, bins_per_thread)
     ctype = torch.complex128 if precision == "fp64" else torch.complex64
-    hp = torch.empty((total,), dtype=ctype, device=records.device)
-    hc = torch.empty((total,), dtype=ctype, device=records.device)
+    if out is None:
+        hp = torch.empty((total,), dtype=ctype, device=records.device)
+        hc = torch.empty((total,), dtype=ctype, device=records.device)
+    else:
+        hp, hc = out
+        if hp.shape[0] < total or hc.shape[0] < total or hp.dtype != ctype or hc.dtype != ctype:
+            raise ValueError(f"outputs must hold {total} {ctype}")
+    if grid_f0 is not None:
+        grid_f0 = grid_f0.to(torch.float64).contiguous()
+        grid_df = grid_df.to(torch.float64).contiguous()
     f64 = int(freqs is not None and freqs.dtype == torch.float64)
-    call("wfx_xas_eval_ragged", _ptr(records), nb, _ptr(freqs), f64,
-         _ptr(None if grid_f0 is None else grid_f0.to(torch.float64).contiguous()),
-         _ptr(None if grid_df is None else grid_df.to(torch.float64).contiguous()), _ptr(offsets), _ptr(tiles),
-         _code(precision), int(blocks), _ptr(hp), _ptr(hc), _stream(records.device))
+    call("wfx_xas_eval_ragged", _ptr(records), nb, _ptr(freqs), f64, _ptr(grid_f0), _ptr(grid_df), _ptr(offsets),
+         _ptr(tiles), _code(precision), int(bins_per_thread), int(blocks), _ptr(hp), _ptr(hc),
+         _stream(records.device))
     return hp, hc
 
 
