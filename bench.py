@@ -182,8 +182,9 @@ def main():
                     help="binaries of the fused forward + adjoint measurement (0: skip)")
     ap.add_argument("--ragged-binaries", type=int, default=16384,
                     help="binaries of the ragged-grid measurement (0: skip)")
-    ap.add_argument("--only", choices=["split", "split-df", "fp64", "fp32"],
+    ap.add_argument("--only", choices=["split", "split-df", "fp64", "fp32", "grad"],
                     help="run just this arithmetic's step (setup + evaluation) K times, for a profiler")
+    ap.add_argument("--sweep", action="store_true", help="time the evaluation's launch shapes and exit")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run a plain device copy of this many MiB K times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -202,12 +203,31 @@ def main():
         dst = torch.empty_like(src)
         ms = timed(torch, lambda: dst.copy_(src), args.steps, args.warmup)
         print(json.dumps({"copy_mib": args.copy_mib, "ms": ms, "GB/s": 2 * src.numel() * 4 / ms / 1e6}), flush=True)
+        ms = timed(torch, lambda: dst.fill_(1.5), args.steps, args.warmup)
+        print(json.dumps({"fill_mib": args.copy_mib, "ms": ms, "GB/s": src.numel() * 4 / ms / 1e6}), flush=True)
+        return
+    if args.only == "grad":
+        print(json.dumps(bench_grad(torch, xas, dev, args.grad_binaries, args.bins, args.steps, args.warmup)),
+              flush=True)
         return
     nb, n = args.binaries, args.bins
     freqs64 = np.linspace(20.0, 1024.0, n)
     params = torch.as_tensor(draw(nb), device=dev)
     f32 = torch.as_tensor(freqs64, dtype=torch.float32, device=dev)
     f64 = torch.as_tensor(freqs64, dtype=torch.float64, device=dev)
+    if args.sweep:
+        # launch shapes of the FP32-split evaluation: binaries per block x
+        # frequencies per thread
+        records, _ = xas.setup(params, f32[:1], 1e21, None, "split")
+        out = (torch.empty((nb, n), dtype=torch.complex64, device=dev),
+               torch.empty((nb, n), dtype=torch.complex64, device=dev))
+        for group in (4, 8, 16, 32):
+            for fpt in (1, 2, 4):
+                ms = timed(torch, lambda: xas.evaluate(records, f32, "split", group, out=out, freqs_per_thread=fpt),
+                           args.steps, args.warmup)
+                print(json.dumps({"group": group, "freqs_per_thread": fpt, "eval_ms": ms,
+                                  "GB/s": 16.0 * nb * n / ms / 1e6}), flush=True)
+        return
     if args.only:
         grid = f64 if args.only == "fp64" else f32
         ctype = torch.complex128 if args.only == "fp64" else torch.complex64

@@ -66,12 +66,20 @@ int wfx_xas_setup_jvp(const double* params, int64_t n_binaries, const double* f_
                       double amp_scale, double reference_mf, double* records, double* jacobian, uint32_t* flags,
                       void* stream);
 
+/* phase[B] = LAL's (2,2) phase of each binary at M f = mf[b] but for its
+ * constant, phase_22 / eta + linb M f, in FP64: what the reference-
+ * convention converters difference (src/waveformx/models/xas/reference.py:
+ * 29-38, src/waveformx/models/taylorf2/reference.py:46-127). */
+int wfx_xas_reference_phase(const double* params, int64_t n_binaries, const double* mf, double* phase,
+                            void* stream);
+
 /* (h+, hx)[B][N] of records on one shared grid freqs[N] (float, or double
- * if freqs_f64). group: binaries per block (0: default). Replaces
+ * if freqs_f64). group: binaries per block; freqs_per_thread: 1, 2 or 4
+ * frequencies per thread on the FP32 paths (0: defaults). Replaces
  * core.frequency_powers + core.evaluate under jax.vmap
  * (src/waveformx/models/xas/core.py:382-507; src/waveformx/torch.py:84-85). */
 int wfx_xas_eval(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
-                 int precision, int group, void* hplus, void* hcross, void* stream);
+                 int precision, int group, int freqs_per_thread, void* hplus, void* hcross, void* stream);
 
 /* tiles[B+1]: the prefix sum of the tiles (256 x bins_per_thread bins; 1, 2
  * or 4) of each binary's range [offsets[b], offsets[b+1]), the work list of
@@ -101,6 +109,14 @@ int wfx_xas_eval_ragged(const void* records, int64_t n_binaries, const void* fre
 int wfx_xas_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs, int freqs_f64,
                     int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, double* grads,
                     void* stream);
+
+/* dldf[N] += dL/df, the gradient with respect to the shared grid's
+ * frequencies (the first one's effect through f_min excluded, as the
+ * reference's stop_gradient, src/waveformx/models/xas/model.py:59), from
+ * the cotangents as wfx_xas_adjoint; the caller zeroes dldf (double).
+ * Replaces the grid part of the reference's vjp (src/waveformx/torch.py:116). */
+int wfx_xas_grid_adjoint(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
+                         int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* stream);
 
 /* Fused forward + adjoint of L_b = sum_f weights[f] (|h+ - d+|^2 +
  * |hx - dx|^2) for data d (as the cotangents' layout) and weights[N]

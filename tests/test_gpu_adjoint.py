@@ -93,6 +93,40 @@ def test_torch_backward(cuda):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_grid_gradient(cuda, precision):
+    """dL/df against central differences of the oracle's formula in each
+    frequency, the tables held at the first frequency (the reference's
+    stop_gradient on f_min, xas/model.py:59)."""
+    import torch
+
+    from waveformx_amd import adjoint
+
+    params = oracle.draw(6, seed=12)
+    f = np.geomspace(20.0, 1500.0, 700)
+    if precision != "fp64":
+        params = params.astype(np.float32).astype(np.float64)
+        f = f.astype(np.float32).astype(np.float64)
+    w = np.random.default_rng(2).normal(size=(4, 6, len(f)))
+    tables, _ = oracle.tables(params, f[0])
+    eps = 1e-6 * f
+    hp_a, hc_a = oracle.evaluate(tables, f + eps)
+    hp_b, hc_b = oracle.evaluate(tables, f - eps)
+    dhp, dhc = (hp_a - hp_b) / (2 * eps), (hc_a - hc_b) / (2 * eps)
+    expected = np.sum(w[0] * dhp.real + w[1] * dhp.imag + w[2] * dhc.real + w[3] * dhc.imag, axis=0)
+    ctype = torch.complex128 if precision == "fp64" else torch.complex64
+    rtype = torch.float64 if precision == "fp64" else torch.float32
+    p = torch.as_tensor(params, device=cuda)
+    ft = torch.as_tensor(f, dtype=rtype, device=cuda)
+    gp = torch.as_tensor(w[0] + 1j * w[1], dtype=ctype, device=cuda)
+    gc = torch.as_tensor(w[2] + 1j * w[3], dtype=ctype, device=cuda)
+    _, grid = adjoint.vjp(p, ft, gp, gc, 1e21, None, precision, wrt_grid=True)
+    grid = grid.cpu().numpy()
+    scale = np.max(np.abs(expected))
+    tol = 1e-6 if precision == "fp64" else 2e-3
+    assert np.max(np.abs(grid - expected)) / scale < tol, np.max(np.abs(grid - expected)) / scale
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
 def test_fused_loss_gradient(cuda, precision):
     """L = sum_f (|h+ - d+|^2 + |hx - dx|^2) / S(f), d at perturbed
     parameters: the fused kernel's loss and gradient against the FP64

@@ -20,10 +20,12 @@ SIGNATURES = {
     "wfx_xas_record_bytes": ([_I], _I),
     "wfx_xas_setup": ([_P, _I64, _P, _I64, _D, _D, _I, _P, _P, _P], _I),
     "wfx_xas_setup_jvp": ([_P, _I64, _P, _I64, _D, _D, _P, _P, _P, _P], _I),
-    "wfx_xas_eval": ([_P, _I64, _P, _I, _I64, _I, _I, _P, _P, _P], _I),
+    "wfx_xas_eval": ([_P, _I64, _P, _I, _I64, _I, _I, _I, _P, _P, _P], _I),
+    "wfx_xas_reference_phase": ([_P, _I64, _P, _P, _P], _I),
     "wfx_xas_tile_prefix": ([_P, _I64, _I, _P, _P], _I),
     "wfx_xas_eval_ragged": ([_P, _I64, _P, _I, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P], _I),
     "wfx_xas_adjoint": ([_P, _P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P], _I),
+    "wfx_xas_grid_adjoint": ([_P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P], _I),
     "wfx_xas_loss_adjoint": ([_P, _P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P, _P, _P], _I),
 }
 

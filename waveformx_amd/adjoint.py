@@ -53,11 +53,21 @@ def adjoint(records: torch.Tensor, jac: torch.Tensor, freqs: torch.Tensor, cot_p
     return grads
 
 
+def grid_adjoint(records: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor,
+                 cot_cross: torch.Tensor, precision: str) -> torch.Tensor:
+    """dL/df (N,) float64 of the shared grid from the cotangents (B, N)."""
+    nb, n = records.shape[0], freqs.shape[0]
+    dldf = torch.zeros((n,), dtype=torch.float64, device=records.device)
+    call("wfx_xas_grid_adjoint", _ptr(records), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
+         _code(precision), _ptr(cot_plus), _ptr(cot_cross), _ptr(dldf), _stream(records.device))
+    return dldf
+
+
 def vjp(params: torch.Tensor, freqs: torch.Tensor, grad_plus: Optional[torch.Tensor],
         grad_cross: Optional[torch.Tensor], amp_scale: float, reference_mf: Optional[float], precision: str,
         wrt_grid: bool = False) -> Tuple[torch.Tensor, Optional[torch.Tensor]]:
-    """Gradients (B, 8) with respect to the packed parameters; the grid's
-    gradient is not computed (None)."""
+    """Gradients (B, 8) with respect to the packed parameters, and (N,)
+    with respect to the grid if ``wrt_grid`` (else None)."""
     nb, n = params.shape[0], freqs.shape[0]
     ctype = torch.complex128 if precision == "fp64" else torch.complex64
     zeros = None
@@ -66,9 +76,8 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, grad_plus: Optional[torch.Ten
     gp = zeros if grad_plus is None else grad_plus.to(ctype).reshape(nb, n).contiguous()
     gc = zeros if grad_cross is None else grad_cross.to(ctype).reshape(nb, n).contiguous()
     records, jac = _records(params, freqs, amp_scale, reference_mf, precision)
-    if wrt_grid:
-        raise NotImplementedError("the gradient with respect to the frequencies is not implemented")
-    return adjoint(records, jac, freqs, gp, gc, precision), None
+    grid = grid_adjoint(records, freqs, gp, gc, precision) if wrt_grid else None
+    return adjoint(records, jac, freqs, gp, gc, precision), grid
 
 
 def loss_and_grad(params: torch.Tensor, freqs: torch.Tensor, data_plus: torch.Tensor, data_cross: torch.Tensor,

@@ -178,4 +178,66 @@ __device__ inline void adjoint_point(const T* __restrict__ c, const BinPowers<T>
     }
 }
 
+// dL/df of one bin (the frequency itself moving): dL/dA A'(f) + dL/dphi
+// phi'(f), with phi' the derivative of the region's phase (basis terms,
+// arctangent, leading term -5/3 T0 f^(-8/3)) and A' that of the region's
+// amplitude; zero above the cutoff.
+template <class T>
+__device__ inline T grid_adjoint_point(const T* __restrict__ c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re,
+                                       T gx_im) {
+    bool past, ringdown;
+    T z, amp;
+    const T phi = phase_and_amp(c, p, past, ringdown, z, amp);
+    if (!(p.f <= c[kFCut])) return T(0);
+    T s, co;
+    m_sincos(phi, &s, &co);
+    const T kp = c[kKp], kc = c[kKc];
+    const T g_amp = kp * (gp_re * co + gp_im * s) + kc * (gx_im * co - gx_re * s);
+    const T g_phi = -amp * (kp * (gp_re * s - gp_im * co) + kc * (gx_re * co + gx_im * s));
+    const T f = p.f, t = p.t, m1 = p.m1, lf = p.lf;
+    // d/df of f^a is a f^(a-1); of f^a log f, f^(a-1) (a log f + 1)
+    T dphi;
+    if (!past) {
+        const T t2 = t * t, mt1 = t2 * m1, mt2 = mt1 * mt1, m2 = m1 * m1;
+        const T third = T(1.0 / 3.0);
+        dphi = -c[kInsM1] * m2 - T(2) * third * c[kInsM23] * mt2 * m1 - third * c[kInsM13] * mt1 * m1 +
+               c[kInsLog] * m1 + third * c[kInsP13] * t * m1 + c[kInsP13L] * t * m1 * (third * lf + T(1)) +
+               T(2) * third * c[kInsP23] * t2 * m1 + c[kInsP1] + c[kInsP1L] * (lf + T(1)) +
+               T(4) * third * c[kInsP43] * t + T(5) * third * c[kInsP53] * t2 + T(2) * c[kInsP2] * f;
+        const double t0 = (double)c[kT0] + (double)c[kT0Lo];
+        dphi += (T)(-5.0 / 3.0 * t0 * p.m53 / (double)f);
+    } else {
+        const T y = f - c[kShift];
+        const T a = ringdown ? c[kAtanA2] : c[kAtanA1];
+        const T w = ringdown ? c[kAtanW2] : c[kAtanW1];
+        const T wy = w * y;
+        dphi = a * w / (T(1) + wy * wy);
+        const T m2 = m1 * m1;
+        if (ringdown)
+            dphi += c[kRdP1] + T(2.0 / 3.0) * c[kRdP23] * (t * t) * m1 - c[kRdM1] * m2 - T(3) * c[kRdM3] * m2 * m2;
+        else
+            dphi += c[kIntP1] + c[kIntLog] * m1 - c[kIntM1] * m2 - T(2) * c[kIntM2] * m2 * m1 -
+                    T(3) * c[kIntM3] * m2 * m2;
+    }
+    T damp;
+    if (f > c[kFAmpRd]) {
+        const T u = c[kAmpRdS] * (f - c[kShift]);
+        damp = amp * (T(-7.0 / 6.0) * m1 - c[kAmpRdGr] * c[kAmpRdS] -
+                      T(2) * u * c[kAmpRdS] / (u * u + c[kAmpRdG]));
+    } else if (f >= c[kFAmpIn]) {
+        T den = c[kAmpIntD + 4], dden = T(4) * c[kAmpIntD + 4];
+        for (int j = 3; j >= 0; --j) {
+            den = den * f + c[kAmpIntD + j];
+            if (j > 0) dden = dden * f + T(j) * c[kAmpIntD + j];
+        }
+        damp = -amp * dden / den;
+    } else {
+        // A = f^(-7/6) sum_k c_k t^k: A' = f^(-13/6) sum_k (k/3 - 7/6) c_k t^k
+        T poly = T(9.0 / 3.0 - 7.0 / 6.0) * c[kAmpIns + 9];
+        for (int k = 8; k >= 0; --k) poly = poly * t + T(k / 3.0 - 7.0 / 6.0) * c[kAmpIns + k];
+        damp = p.m76 * m1 * poly;
+    }
+    return g_amp * damp + g_phi * dphi;
+}
+
 }  // namespace wfx

@@ -171,6 +171,18 @@ __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict_
     }
 }
 
+// LAL's (2,2) phase of each binary at its M f = mf[b], FP64 (the
+// reference-convention converters)
+__global__ __launch_bounds__(64) void reference_phase_kernel(const double* __restrict__ params, int64_t nb,
+                                                             const double* __restrict__ mf,
+                                                             double* __restrict__ phase) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double p[kParams];
+    for (int k = 0; k < kParams; ++k) p[k] = params[b * kParams + k];
+    phase[b] = lal_phase_22<double>(p, mf[b]);
+}
+
 // ------------------------------------------------------------ evaluation
 
 constexpr int kBlock = 256;
@@ -203,8 +215,10 @@ __global__ __launch_bounds__(kBlock) void eval_f64_shared(const double* __restri
     }
 }
 
-// shared grid, FP32 paths
-template <int L, class F>
+// shared grid, FP32 paths: a block owns kBlock * FPT frequencies (thread t
+// takes t, t + kBlock, ...) and a group of binaries; each binary's record
+// is read once per iteration (scalar loads) and serves FPT bins per thread
+template <int L, int FPT, class F>
 __global__ __launch_bounds__(kBlock) void eval_f32_shared(const float* __restrict__ records, int64_t nb,
                                                           const F* __restrict__ freqs, int64_t n, int64_t ntiles,
                                                           int group, float2* __restrict__ hp,
@@ -212,16 +226,28 @@ __global__ __launch_bounds__(kBlock) void eval_f32_shared(const float* __restric
     const int64_t w = xcd_work_id(blockIdx.x, gridDim.x);
     const int64_t tile = w % ntiles;
     const int64_t b0 = (w / ntiles) * group;
-    const int64_t i = tile * kBlock + threadIdx.x;
-    const bool valid = i < n;
-    const PowF32 p = pow_f32((float)freqs[valid ? i : n - 1]);
+    const int64_t i0 = tile * (kBlock * FPT) + threadIdx.x;
+    PowF32 p[FPT];
+#pragma unroll
+    for (int j = 0; j < FPT; ++j) {
+        const int64_t i = i0 + j * kBlock;
+        p[j] = pow_f32((float)freqs[i < n ? i : n - 1]);
+    }
     const int64_t b1 = b0 + group < nb ? b0 + group : nb;
     for (int64_t b = b0; b < b1; ++b) {
-        float2 a, c;
-        point_f32<L>(records + b * kSlots, p, a, c);
-        if (valid) {
-            hp[b * n + i] = a;
-            hc[b * n + i] = c;
+        float c[kSlots];
+        const float* rec = records + b * kSlots;
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) c[k] = rec[k];
+#pragma unroll
+        for (int j = 0; j < FPT; ++j) {
+            const int64_t i = i0 + j * kBlock;
+            float2 a, x;
+            point_f32<L>(c, p[j], a, x);
+            if (i < n) {
+                hp[b * n + i] = a;
+                hc[b * n + i] = x;
+            }
         }
     }
 }
@@ -423,6 +449,67 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(const T* __restrict__ r
     }
 }
 
+// dL/df of each frequency of a shared grid: the evaluation's launch shape
+// (a thread per frequency, a loop over a group of binaries), the bins'
+// contributions summed in FP64 in a register, then added to dldf[i] (one
+// FP64 atomic per group and frequency)
+template <class T, class F>
+__global__ __launch_bounds__(kBlock) void grid_adjoint_kernel(const T* __restrict__ records, int64_t nb,
+                                                              const F* __restrict__ freqs, int64_t n, int64_t ntiles,
+                                                              int group, const T* __restrict__ cot_plus,
+                                                              const T* __restrict__ cot_cross,
+                                                              double* __restrict__ dldf) {
+    const int64_t w = blockIdx.x;
+    const int64_t tile = w % ntiles;
+    const int64_t b0 = (w / ntiles) * group;
+    const int64_t i = tile * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const BinPowers<T> p = adjoint_powers<T>((double)freqs[i]);
+    const int64_t b1 = b0 + group < nb ? b0 + group : nb;
+    double acc = 0.0;
+    for (int64_t b = b0; b < b1; ++b) {
+        const int64_t o = 2 * (b * n + i);
+        acc += (double)grid_adjoint_point(records + b * kSlots, p, cot_plus[o], cot_plus[o + 1], cot_cross[o],
+                                          cot_cross[o + 1]);
+    }
+    atomicAdd(dldf + i, acc);
+}
+
+constexpr int kDefaultFreqsPerThread = 2;
+
+struct SharedArgs {
+    const void* records;
+    int64_t nb;
+    const void* freqs;
+    int64_t n;
+    int64_t ntiles;
+    int group;
+    void* hp;
+    void* hc;
+    int blocks;
+    hipStream_t stream;
+};
+
+template <int L, int FPT, class F>
+void launch_f32_shared_t(const SharedArgs& a) {
+    eval_f32_shared<L, FPT, F><<<a.blocks, kBlock, 0, a.stream>>>((const float*)a.records, a.nb, (const F*)a.freqs,
+                                                                  a.n, a.ntiles, a.group, (float2*)a.hp,
+                                                                  (float2*)a.hc);
+}
+
+template <int L>
+void launch_f32_shared(const SharedArgs& a, int freqs_f64, int fpt) {
+    if (freqs_f64) {
+        if (fpt == 1) launch_f32_shared_t<L, 1, double>(a);
+        else if (fpt == 2) launch_f32_shared_t<L, 2, double>(a);
+        else launch_f32_shared_t<L, 4, double>(a);
+    } else {
+        if (fpt == 1) launch_f32_shared_t<L, 1, float>(a);
+        else if (fpt == 2) launch_f32_shared_t<L, 2, float>(a);
+        else launch_f32_shared_t<L, 4, float>(a);
+    }
+}
+
 struct RaggedArgs {
     const void* records;
     int64_t nb;
@@ -529,25 +616,34 @@ int wfx_xas_setup_jvp(const double* params, int64_t n_binaries, const double* f_
     return check_launch("wfx_xas_setup_jvp");
 }
 
+int wfx_xas_reference_phase(const double* params, int64_t n_binaries, const double* mf, double* phase,
+                            void* stream) {
+    if (n_binaries < 0) return fail("wfx_xas_reference_phase: negative size");
+    if (n_binaries == 0) return 0;
+    if (!params || !mf || !phase) return fail("wfx_xas_reference_phase: null pointer");
+    const int blocks = grid_blocks((n_binaries + 63) / 64, "wfx_xas_reference_phase: too many binaries");
+    if (blocks < 0) return -1;
+    reference_phase_kernel<<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, mf, phase);
+    return check_launch("wfx_xas_reference_phase");
+}
+
 int wfx_xas_eval(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
-                 int precision, int group, void* hplus, void* hcross, void* stream) {
+                 int precision, int group, int freqs_per_thread, void* hplus, void* hcross, void* stream) {
     if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xas_eval: negative size");
     if (n_binaries == 0 || n_freqs == 0) return 0;
     if (!records || !freqs || !hplus || !hcross) return fail("wfx_xas_eval: null pointer");
+    if (precision < kPrecF64 || precision > kPrecSplitDF) return fail("wfx_xas_eval: unknown precision");
+    if (freqs_per_thread < 0 || freqs_per_thread == 3 || freqs_per_thread > 4)
+        return fail("wfx_xas_eval: freqs_per_thread must be 1, 2 or 4 (0: default)");
+    if (precision == kPrecF64) freqs_per_thread = 1;
+    if (freqs_per_thread == 0) freqs_per_thread = kDefaultFreqsPerThread;
     if (group <= 0) group = precision == kPrecF64 ? 8 : 16;
-    const int64_t ntiles = (n_freqs + kBlock - 1) / kBlock;
+    const int64_t ntiles = (n_freqs + kBlock * freqs_per_thread - 1) / (kBlock * freqs_per_thread);
     const int64_t ngroups = (n_binaries + group - 1) / group;
     const int blocks = grid_blocks(ntiles * ngroups, "wfx_xas_eval: grid too large, raise the group size");
     if (blocks < 0) return -1;
     hipStream_t s = (hipStream_t)stream;
-#define WFX_F32_LAUNCH(L)                                                                                           \
-    if (freqs_f64)                                                                                                  \
-        eval_f32_shared<L, double><<<blocks, kBlock, 0, s>>>((const float*)records, n_binaries,                     \
-                                                             (const double*)freqs, n_freqs, ntiles, group,          \
-                                                             (float2*)hplus, (float2*)hcross);                      \
-    else                                                                                                            \
-        eval_f32_shared<L, float><<<blocks, kBlock, 0, s>>>((const float*)records, n_binaries, (const float*)freqs, \
-                                                            n_freqs, ntiles, group, (float2*)hplus, (float2*)hcross);
+    const SharedArgs a{records, n_binaries, freqs, n_freqs, ntiles, group, hplus, hcross, blocks, s};
     switch (precision) {
         case kPrecF64:
             if (freqs_f64)
@@ -560,18 +656,17 @@ int wfx_xas_eval(const void* records, int64_t n_binaries, const void* freqs, int
                                                                  (double2*)hplus, (double2*)hcross);
             break;
         case kPrecSplit:
-            WFX_F32_LAUNCH(kLeadF64)
+            launch_f32_shared<kLeadF64>(a, freqs_f64, freqs_per_thread);
             break;
         case kPrecSplitDF:
-            WFX_F32_LAUNCH(kLeadDF)
+            launch_f32_shared<kLeadDF>(a, freqs_f64, freqs_per_thread);
             break;
         case kPrecF32:
-            WFX_F32_LAUNCH(kLeadF32)
+            launch_f32_shared<kLeadF32>(a, freqs_f64, freqs_per_thread);
             break;
         default:
             return fail("wfx_xas_eval: unknown precision");
     }
-#undef WFX_F32_LAUNCH
     return check_launch("wfx_xas_eval");
 }
 
@@ -646,6 +741,40 @@ int wfx_xas_adjoint(const void* records, const double* jacobian, int64_t n_binar
         return fail("wfx_xas_adjoint: unknown precision");
     }
     return check_launch("wfx_xas_adjoint");
+}
+
+int wfx_xas_grid_adjoint(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
+                         int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* stream) {
+    if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xas_grid_adjoint: negative size");
+    if (n_binaries == 0 || n_freqs == 0) return 0;
+    if (!records || !freqs || !cot_plus || !cot_cross || !dldf) return fail("wfx_xas_grid_adjoint: null pointer");
+    const int group = 64;
+    const int64_t ntiles = (n_freqs + kBlock - 1) / kBlock;
+    const int blocks = grid_blocks(ntiles * ((n_binaries + group - 1) / group), "wfx_xas_grid_adjoint: too large");
+    if (blocks < 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == kPrecF64) {
+        if (freqs_f64)
+            grid_adjoint_kernel<double, double><<<blocks, kBlock, 0, s>>>(
+                (const double*)records, n_binaries, (const double*)freqs, n_freqs, ntiles, group,
+                (const double*)cot_plus, (const double*)cot_cross, dldf);
+        else
+            grid_adjoint_kernel<double, float><<<blocks, kBlock, 0, s>>>(
+                (const double*)records, n_binaries, (const float*)freqs, n_freqs, ntiles, group,
+                (const double*)cot_plus, (const double*)cot_cross, dldf);
+    } else if (precision == kPrecF32 || precision == kPrecSplit || precision == kPrecSplitDF) {
+        if (freqs_f64)
+            grid_adjoint_kernel<float, double><<<blocks, kBlock, 0, s>>>(
+                (const float*)records, n_binaries, (const double*)freqs, n_freqs, ntiles, group,
+                (const float*)cot_plus, (const float*)cot_cross, dldf);
+        else
+            grid_adjoint_kernel<float, float><<<blocks, kBlock, 0, s>>>(
+                (const float*)records, n_binaries, (const float*)freqs, n_freqs, ntiles, group,
+                (const float*)cot_plus, (const float*)cot_cross, dldf);
+    } else {
+        return fail("wfx_xas_grid_adjoint: unknown precision");
+    }
+    return check_launch("wfx_xas_grid_adjoint");
 }
 
 int wfx_xas_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,

@@ -458,6 +458,26 @@ WFX_HD inline R time_shift(const PhaseCoeffs<R>& ph, const WaveformVars<R>& wf) 
     return m_fma(R(-2.0 * kPi), R(500.0) + ut_psi4_to_strain(wf.eta, wf.s_tot_r, wf.dchi), linb - dphi);
 }
 
+// LAL's (2,2) phase at M f = mf but for its constant, phase_22 / eta +
+// linb mf (xas/reference.py:29-35, phase22.py:289-307): the quantity whose
+// change between two reference frequencies moves phi_ref
+template <class R>
+WFX_HD inline R lal_phase_22(const R* params, const R& mf) {
+    R p[kParams] = {params[kMass1], params[kMass2], params[kChi1], params[kChi2], R(1.0), R(0.0), R(0.0), R(1.0)};
+    WaveformVars<R> wf = waveform_vars(p, R(1.0), 1.0);
+    PhaseCoeffs<R> ph = phase_coeffs(wf);
+    R linb = time_shift(ph, wf);
+    Powers<R> pw = useful_powers(mf);
+    R phi;
+    if (mf < ph.f_phase_match_in)
+        phi = ins_phase_22_ansatz_int_exact(mf, pw, ph);
+    else if (mf >= ph.f_phase_match_im)
+        phi = m_fma(ph.c2_mrd, mf, rd_phase_22_ansatz_int_exact(mf, pw, ph) + ph.c1_mrd);
+    else
+        phi = m_fma(ph.c2_int, mf, itm_phase_22_ansatz_int_exact(mf, pw, ph) + ph.c1_int);
+    return phi / wf.eta + linb * mf;
+}
+
 // the powers of f the formula reads (xas/core.py:382-392); the leading
 // term's f^(-5/3) is always from the FP64 value
 template <class R>

@@ -113,9 +113,12 @@ def setup_jvp(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21
 
 
 def evaluate(records: torch.Tensor, freqs: torch.Tensor, precision: str = "split", group: int = 0,
-             out: Optional[Tuple[torch.Tensor, torch.Tensor]] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+             out: Optional[Tuple[torch.Tensor, torch.Tensor]] = None,
+             freqs_per_thread: int = 0) -> Tuple[torch.Tensor, torch.Tensor]:
     """(h+, hx) (B, N) on one shared grid ``freqs`` (N,) (float32 or
-    float64); complex128 for ``"fp64"``, else complex64."""
+    float64); complex128 for ``"fp64"``, else complex64. ``group``
+    (binaries per block) and ``freqs_per_thread`` (1, 2, 4) shape the
+    launch (0: defaults)."""
     _require_device(records, "records")
     _require_device(freqs, "freqs")
     if freqs.dtype not in (torch.float32, torch.float64):
@@ -127,8 +130,11 @@ def evaluate(records: torch.Tensor, freqs: torch.Tensor, precision: str = "split
         hc = torch.empty((nb, n), dtype=ctype, device=records.device)
     else:
         hp, hc = out
+    if out is not None and (hp.shape != (nb, n) or hc.shape != (nb, n) or hp.dtype != ctype or hc.dtype != ctype
+                            or not hp.is_contiguous() or not hc.is_contiguous()):
+        raise ValueError(f"outputs must be contiguous ({nb}, {n}) {ctype}")
     call("wfx_xas_eval", _ptr(records), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n, _code(precision),
-         int(group), _ptr(hp), _ptr(hc), _stream(records.device))
+         int(group), int(freqs_per_thread), _ptr(hp), _ptr(hc), _stream(records.device))
     return hp, hc
 
 
