@@ -42,6 +42,9 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
+/* The version of this ABI (7); a binding checks it before any call. */
+int wfx_abi_version(void);
+
 /* Slots of one binary's record (64) and its size in bytes for a precision
  * (512 for FP64, 256 otherwise). The record replaces the reference's
  * 86-entry table (src/waveformx/models/xas/core.py:68-101). */
@@ -100,15 +103,24 @@ int wfx_xas_eval_ragged(const void* records, int64_t n_binaries, const void* fre
                         int precision, int bins_per_thread, int blocks, void* hplus, void* hcross,
                         void* stream);
 
+/* A grid's table of frequency powers for the adjoint kernels:
+ * wfx_xas_power_table_bytes(precision) bytes per frequency (32 for the FP32
+ * paths, 48 for FP64) in the caller's buffer. No counterpart in the
+ * reference (XLA shares core.frequency_powers across a vmap itself). */
+int wfx_xas_power_table_bytes(int precision);
+int wfx_xas_power_table(const void* freqs, int freqs_f64, int64_t n_freqs, int precision, void* table, void* stream);
+
 /* grads[B][8] (double) of a scalar loss from its cotangents cot_plus,
  * cot_cross[B][N] (interleaved complex, dL/dRe + i dL/dIm, precision 0:
  * double, else float) on the shared grid: the adjoint kernel recomputes
  * each bin and contracts with jacobian (from wfx_xas_setup_jvp); records
- * are those of the forward precision. Replaces the reverse half of the
- * reference's jax.vjp backward (src/waveformx/torch.py:108-119). */
+ * are those of the forward precision; powers: the grid's table from
+ * wfx_xas_power_table, or NULL (then computed per bin). Replaces the
+ * reverse half of the reference's jax.vjp backward
+ * (src/waveformx/torch.py:108-119). */
 int wfx_xas_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs, int freqs_f64,
-                    int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, double* grads,
-                    void* stream);
+                    int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, const void* powers,
+                    double* grads, void* stream);
 
 /* dldf[N] += dL/df, the gradient with respect to the shared grid's
  * frequencies (the first one's effect through f_min excluded, as the
@@ -127,7 +139,7 @@ int wfx_xas_grid_adjoint(const void* records, int64_t n_binaries, const void* fr
  * product). */
 int wfx_xas_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
                          int freqs_f64, int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
-                         const void* weights, double* loss, double* grads, void* stream);
+                         const void* weights, const void* powers, double* loss, double* grads, void* stream);
 
 #ifdef __cplusplus
 }

@@ -14,8 +14,10 @@ LIBRARY = Path(__file__).resolve().parent / "libwfx_xas.so"
 _lib = None
 
 _P, _I64, _I, _D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+ABI_VERSION = 7
 SIGNATURES = {
     "wfx_last_error": ([], ctypes.c_char_p),
+    "wfx_abi_version": ([], _I),
     "wfx_xas_record_slots": ([], _I),
     "wfx_xas_record_bytes": ([_I], _I),
     "wfx_xas_setup": ([_P, _I64, _P, _I64, _D, _D, _I, _P, _P, _P], _I),
@@ -24,9 +26,11 @@ SIGNATURES = {
     "wfx_xas_reference_phase": ([_P, _I64, _P, _P, _P], _I),
     "wfx_xas_tile_prefix": ([_P, _I64, _I, _P, _P], _I),
     "wfx_xas_eval_ragged": ([_P, _I64, _P, _I, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P], _I),
-    "wfx_xas_adjoint": ([_P, _P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P], _I),
+    "wfx_xas_power_table_bytes": ([_I], _I),
+    "wfx_xas_power_table": ([_P, _I, _I64, _I, _P, _P], _I),
+    "wfx_xas_adjoint": ([_P, _P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P, _P], _I),
     "wfx_xas_grid_adjoint": ([_P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P], _I),
-    "wfx_xas_loss_adjoint": ([_P, _P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P, _P, _P], _I),
+    "wfx_xas_loss_adjoint": ([_P, _P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P, _P, _P, _P], _I),
 }
 
 
@@ -38,6 +42,11 @@ def lib() -> ctypes.CDLL:
             raise RuntimeError(f"{LIBRARY.name} is not built: run `python -m waveformx_amd.build` "
                                "(waveformx_amd has no CPU fallback)")
         handle = ctypes.CDLL(str(LIBRARY))
+        version = getattr(handle, "wfx_abi_version", None)
+        found = version() if version is not None else None
+        if found != ABI_VERSION:
+            raise RuntimeError(f"{LIBRARY.name} has ABI version {found}, this binding needs {ABI_VERSION}: rebuild "
+                               "it with `python -m waveformx_amd.build`")
         for name, (args, res) in SIGNATURES.items():
             fn = getattr(handle, name)
             fn.argtypes = args

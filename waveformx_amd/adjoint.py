@@ -35,10 +35,24 @@ def _records(params, freqs, amp_scale, reference_mf, precision):
     return records, jac
 
 
+def power_table(freqs: torch.Tensor, precision: str) -> torch.Tensor:
+    """The grid's table of frequency powers (bytes, on the device) that the
+    adjoint kernels read instead of recomputing them per bin."""
+    from ._lib import lib
+
+    _require_device(freqs, "freqs")
+    row = lib().wfx_xas_power_table_bytes(_code(precision))
+    table = torch.empty((freqs.shape[0], row), dtype=torch.uint8, device=freqs.device)
+    call("wfx_xas_power_table", _ptr(freqs), int(freqs.dtype == torch.float64), freqs.shape[0], _code(precision),
+         _ptr(table), _stream(freqs.device))
+    return table
+
+
 def adjoint(records: torch.Tensor, jac: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor,
-            cot_cross: torch.Tensor, precision: str) -> torch.Tensor:
+            cot_cross: torch.Tensor, precision: str, powers: Optional[torch.Tensor] = None) -> torch.Tensor:
     """(B, 8) float64 gradients from the cotangents (B, N) (complex, the
-    torch convention dL/dRe + i dL/dIm) on the shared grid ``freqs``."""
+    torch convention dL/dRe + i dL/dIm) on the shared grid ``freqs``
+    (``powers``: its :func:`power_table`, made here if not given)."""
     for name, t in (("records", records), ("jac", jac), ("freqs", freqs), ("cot_plus", cot_plus),
                     ("cot_cross", cot_cross)):
         _require_device(t, name)
@@ -48,8 +62,10 @@ def adjoint(records: torch.Tensor, jac: torch.Tensor, freqs: torch.Tensor, cot_p
             cot_cross.dtype != ctype:
         raise ValueError(f"cotangents must be ({nb}, {n}) {ctype}")
     grads = torch.empty((nb, 8), dtype=torch.float64, device=records.device)
+    if powers is None:
+        powers = power_table(freqs, precision)
     call("wfx_xas_adjoint", _ptr(records), _ptr(jac), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
-         _code(precision), _ptr(cot_plus), _ptr(cot_cross), _ptr(grads), _stream(records.device))
+         _code(precision), _ptr(cot_plus), _ptr(cot_cross), _ptr(powers), _ptr(grads), _stream(records.device))
     return grads
 
 
@@ -82,10 +98,12 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, grad_plus: Optional[torch.Ten
 
 def loss_and_grad(params: torch.Tensor, freqs: torch.Tensor, data_plus: torch.Tensor, data_cross: torch.Tensor,
                   weights: torch.Tensor, amp_scale: float = 1e21, reference_mf: Optional[float] = None,
-                  precision: str = "split", records: Optional[Tuple[torch.Tensor, torch.Tensor]] = None
-                  ) -> Tuple[torch.Tensor, torch.Tensor]:
+                  precision: str = "split", records: Optional[Tuple[torch.Tensor, torch.Tensor]] = None,
+                  powers: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, torch.Tensor]:
     """Per-binary loss (B,) and gradient (B, 8), float64, of
-    ``sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2)`` in one fused pass."""
+    ``sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2)`` in one fused pass
+    (``records``: (records, jacobian) if already set up; ``powers``: the
+    grid's :func:`power_table`, made here if not given)."""
     for name, t in (("params", params), ("freqs", freqs), ("data_plus", data_plus), ("data_cross", data_cross),
                     ("weights", weights)):
         _require_device(t, name)
@@ -100,7 +118,9 @@ def loss_and_grad(params: torch.Tensor, freqs: torch.Tensor, data_plus: torch.Te
     rec, jac = records if records is not None else _records(params, freqs, amp_scale, reference_mf, precision)
     loss = torch.empty((nb,), dtype=torch.float64, device=params.device)
     grads = torch.empty((nb, 8), dtype=torch.float64, device=params.device)
+    if powers is None:
+        powers = power_table(freqs, precision)
     call("wfx_xas_loss_adjoint", _ptr(rec), _ptr(jac), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
-         _code(precision), _ptr(data_plus), _ptr(data_cross), _ptr(weights), _ptr(loss), _ptr(grads),
+         _code(precision), _ptr(data_plus), _ptr(data_cross), _ptr(weights), _ptr(powers), _ptr(loss), _ptr(grads),
          _stream(params.device))
     return loss, grads

@@ -17,15 +17,28 @@
 
 namespace wfx {
 
-__device__ inline void m_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
+// the FP32 adjoint uses the forward kernel's fast forms (|phi| <= pi, so
+// the hardware sine and cosine need no range reduction); FP64 the library's
+__device__ inline void m_sincos(float x, float* s, float* c) { __sincosf(x, s, c); }
 __device__ inline void m_sincos(double x, double* s, double* c) { sincos(x, s, c); }
-__device__ inline float d_atan(float x) { return atanf(x); }
+__device__ inline float d_atan(float x) { return fast_atan(x); }
 __device__ inline double d_atan(double x) { return atan(x); }
-__device__ inline float d_exp(float x) { return expf(x); }
+__device__ inline float d_exp(float x) { return __expf(x); }
 __device__ inline double d_exp(double x) { return exp(x); }
+__device__ inline float d_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double d_rcp(double x) { return 1.0 / x; }
 
 template <class T>
 struct BinPowers {
+    T f, t, m1, lf, m76;
+    double m53;
+};
+
+// one frequency's powers as stored in a grid's table (32 B for float, 48 B
+// for double): the adjoint kernels, which walk every frequency once per
+// binary, read them instead of recomputing them per bin
+template <class T>
+struct PowerRow {
     T f, t, m1, lf, m76;
     double m53;
 };
@@ -70,11 +83,11 @@ __device__ inline T phase_and_amp(const T* __restrict__ c, const BinPowers<T>& p
 
     if (f > c[kFAmpRd]) {
         const T u = c[kAmpRdS] * (f - c[kShift]);
-        amp = c[kAmpRdN] * p.m76 * d_exp(-c[kAmpRdGr] * u) / (u * u + c[kAmpRdG]);
+        amp = c[kAmpRdN] * p.m76 * d_exp(-c[kAmpRdGr] * u) * d_rcp(u * u + c[kAmpRdG]);
     } else if (f >= c[kFAmpIn]) {
         T den = c[kAmpIntD + 4];
         for (int j = 3; j >= 0; --j) den = den * f + c[kAmpIntD + j];
-        amp = c[kAmpIntN] / den;
+        amp = c[kAmpIntN] * d_rcp(den);
     } else {
         T poly = c[kAmpIns + 9];
         for (int k = 8; k >= 0; --k) poly = poly * t + c[kAmpIns + k];
@@ -125,7 +138,7 @@ __device__ inline void adjoint_point(const T* __restrict__ c, const BinPowers<T>
         const T a = ringdown ? c[kAtanA2] : c[kAtanA1];
         const T w = ringdown ? c[kAtanW2] : c[kAtanW1];
         const T wy = w * y;
-        const T datan = T(1) / (T(1) + wy * wy);
+        const T datan = d_rcp(T(1) + wy * wy);
         acc[kShift] -= g_phi * a * w * datan;
         const T m2 = m1 * m1;
         if (ringdown) {
@@ -151,18 +164,18 @@ __device__ inline void adjoint_point(const T* __restrict__ c, const BinPowers<T>
     if (f > c[kFAmpRd]) {
         const T y = f - c[kShift];
         const T u = c[kAmpRdS] * y;
-        const T q = u * u + c[kAmpRdG];
-        const T e = p.m76 * d_exp(-c[kAmpRdGr] * u) / q;  // dA/dn
+        const T inv_q = d_rcp(u * u + c[kAmpRdG]);
+        const T e = p.m76 * d_exp(-c[kAmpRdGr] * u) * inv_q;  // dA/dn
         acc[kAmpRdN] += g_amp * e;
         acc[kAmpRdGr] -= g_amp * u * amp;
-        acc[kAmpRdG] -= g_amp * amp / q;
-        const T dadu = amp * (-c[kAmpRdGr] - T(2) * u / q);
+        acc[kAmpRdG] -= g_amp * amp * inv_q;
+        const T dadu = amp * (-c[kAmpRdGr] - T(2) * u * inv_q);
         acc[kAmpRdS] += g_amp * dadu * y;
         acc[kShift] -= g_amp * dadu * c[kAmpRdS];
     } else if (f >= c[kFAmpIn]) {
         T den = c[kAmpIntD + 4];
         for (int j = 3; j >= 0; --j) den = den * f + c[kAmpIntD + j];
-        const T inv = T(1) / den;
+        const T inv = d_rcp(den);
         acc[kAmpIntN] += g_amp * inv;
         T q = -g_amp * amp * inv;
         for (int j = 0; j < 5; ++j) {

@@ -380,12 +380,33 @@ __device__ inline BinPowers<double> adjoint_powers<double>(double f) {
     return bin_powers(freq_powers<double>(f));
 }
 
+// the powers of frequency i: from the grid's table if there is one
+template <class T, class F>
+__device__ inline BinPowers<T> powers_at(const F* __restrict__ freqs, const PowerRow<T>* __restrict__ table,
+                                         int64_t i) {
+    if (table) {
+        const PowerRow<T> r = table[i];
+        return BinPowers<T>{r.f, r.t, r.m1, r.lf, r.m76, r.m53};
+    }
+    return adjoint_powers<T>((double)freqs[i]);
+}
+
+template <class T, class F>
+__global__ __launch_bounds__(kBlock) void power_table_kernel(const F* __restrict__ freqs, int64_t n,
+                                                             PowerRow<T>* __restrict__ table) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const BinPowers<T> p = adjoint_powers<T>((double)freqs[i]);
+    table[i] = PowerRow<T>{p.f, p.t, p.m1, p.lf, p.m76, p.m53};
+}
+
 template <class T, int Mode, class F>
 __global__ __launch_bounds__(kBlock) void adjoint_kernel(const T* __restrict__ records, const double* __restrict__ jac,
                                                          int64_t nb, const F* __restrict__ freqs, int64_t n,
                                                          const T* __restrict__ in_plus, const T* __restrict__ in_cross,
                                                          const T* __restrict__ weights, double* __restrict__ grads,
-                                                         double* __restrict__ loss) {
+                                                         double* __restrict__ loss,
+                                                         const PowerRow<T>* __restrict__ table) {
     const int64_t b = blockIdx.x;
     if (b >= nb) return;
     const T* c = records + b * kSlots;
@@ -394,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(const T* __restrict__ r
     for (int k = 0; k < kSlots; ++k) acc[k] = T(0);
     T loss_acc = T(0);
     for (int64_t i = threadIdx.x; i < n; i += kBlock) {
-        const BinPowers<T> p = adjoint_powers<T>((double)freqs[i]);
+        const BinPowers<T> p = powers_at<T>(freqs, table, i);
         const int64_t o = 2 * (b * n + i);
         T gp_re = in_plus[o], gp_im = in_plus[o + 1], gx_re = in_cross[o], gx_im = in_cross[o + 1];
         if (Mode == kLoss) {
@@ -473,6 +494,42 @@ __global__ __launch_bounds__(kBlock) void grid_adjoint_kernel(const T* __restric
                                           cot_cross[o + 1]);
     }
     atomicAdd(dldf + i, acc);
+}
+
+struct AdjointArgs {
+    const void* records;
+    const double* jac;
+    int64_t nb;
+    const void* freqs;
+    int64_t n;
+    const void* in_plus;
+    const void* in_cross;
+    const void* weights;
+    double* grads;
+    double* loss;
+    const void* powers;
+    int blocks;
+    hipStream_t stream;
+};
+
+template <class T, int Mode, class F>
+void launch_adjoint_t(const AdjointArgs& a) {
+    adjoint_kernel<T, Mode, F><<<a.blocks, kBlock, 0, a.stream>>>(
+        (const T*)a.records, a.jac, a.nb, (const F*)a.freqs, a.n, (const T*)a.in_plus, (const T*)a.in_cross,
+        (const T*)a.weights, a.grads, a.loss, (const PowerRow<T>*)a.powers);
+}
+
+template <int Mode>
+bool launch_adjoint(const AdjointArgs& a, int precision, int freqs_f64) {
+    if (precision == kPrecF64) {
+        freqs_f64 ? launch_adjoint_t<double, Mode, double>(a) : launch_adjoint_t<double, Mode, float>(a);
+        return true;
+    }
+    if (precision == kPrecF32 || precision == kPrecSplit || precision == kPrecSplitDF) {
+        freqs_f64 ? launch_adjoint_t<float, Mode, double>(a) : launch_adjoint_t<float, Mode, float>(a);
+        return true;
+    }
+    return false;
 }
 
 constexpr int kDefaultFreqsPerThread = 2;
@@ -563,6 +620,9 @@ int grid_blocks(int64_t nblocks, const char* what) {
 extern "C" {
 
 const char* wfx_last_error(void) { return g_error.c_str(); }
+
+// bumped whenever an entry point's signature changes
+int wfx_abi_version(void) { return 7; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -709,37 +769,54 @@ int wfx_xas_eval_ragged(const void* records, int64_t n_binaries, const void* fre
     return check_launch("wfx_xas_eval_ragged");
 }
 
+int wfx_xas_power_table_bytes(int precision) {
+    if (precision == kPrecF64) return (int)sizeof(PowerRow<double>);
+    if (precision == kPrecF32 || precision == kPrecSplit || precision == kPrecSplitDF)
+        return (int)sizeof(PowerRow<float>);
+    return fail("wfx_xas_power_table_bytes: unknown precision");
+}
+
+int wfx_xas_power_table(const void* freqs, int freqs_f64, int64_t n_freqs, int precision, void* table,
+                        void* stream) {
+    if (n_freqs < 0) return fail("wfx_xas_power_table: negative size");
+    if (n_freqs == 0) return 0;
+    if (!freqs || !table) return fail("wfx_xas_power_table: null pointer");
+    const int blocks = grid_blocks((n_freqs + kBlock - 1) / kBlock, "wfx_xas_power_table: too many frequencies");
+    if (blocks < 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    const bool f64 = freqs_f64 != 0;
+    if (precision == kPrecF64) {
+        if (f64)
+            power_table_kernel<double, double><<<blocks, kBlock, 0, s>>>((const double*)freqs, n_freqs,
+                                                                         (PowerRow<double>*)table);
+        else
+            power_table_kernel<double, float><<<blocks, kBlock, 0, s>>>((const float*)freqs, n_freqs,
+                                                                        (PowerRow<double>*)table);
+    } else if (precision == kPrecF32 || precision == kPrecSplit || precision == kPrecSplitDF) {
+        if (f64)
+            power_table_kernel<float, double><<<blocks, kBlock, 0, s>>>((const double*)freqs, n_freqs,
+                                                                        (PowerRow<float>*)table);
+        else
+            power_table_kernel<float, float><<<blocks, kBlock, 0, s>>>((const float*)freqs, n_freqs,
+                                                                       (PowerRow<float>*)table);
+    } else {
+        return fail("wfx_xas_power_table: unknown precision");
+    }
+    return check_launch("wfx_xas_power_table");
+}
+
 int wfx_xas_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs, int freqs_f64,
-                    int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, double* grads,
-                    void* stream) {
+                    int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, const void* powers,
+                    double* grads, void* stream) {
     if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xas_adjoint: negative size");
     if (n_binaries == 0) return 0;
     if (!records || !jacobian || !grads || (n_freqs && (!freqs || !cot_plus || !cot_cross)))
         return fail("wfx_xas_adjoint: null pointer");
     const int blocks = grid_blocks(n_binaries, "wfx_xas_adjoint: too many binaries");
     if (blocks < 0) return -1;
-    hipStream_t s = (hipStream_t)stream;
-    if (precision == kPrecF64) {
-        if (freqs_f64)
-            adjoint_kernel<double, kFromCotangents, double><<<blocks, kBlock, 0, s>>>(
-                (const double*)records, jacobian, n_binaries, (const double*)freqs, n_freqs, (const double*)cot_plus,
-                (const double*)cot_cross, nullptr, grads, nullptr);
-        else
-            adjoint_kernel<double, kFromCotangents, float><<<blocks, kBlock, 0, s>>>(
-                (const double*)records, jacobian, n_binaries, (const float*)freqs, n_freqs, (const double*)cot_plus,
-                (const double*)cot_cross, nullptr, grads, nullptr);
-    } else if (precision == kPrecF32 || precision == kPrecSplit || precision == kPrecSplitDF) {
-        if (freqs_f64)
-            adjoint_kernel<float, kFromCotangents, double><<<blocks, kBlock, 0, s>>>(
-                (const float*)records, jacobian, n_binaries, (const double*)freqs, n_freqs, (const float*)cot_plus,
-                (const float*)cot_cross, nullptr, grads, nullptr);
-        else
-            adjoint_kernel<float, kFromCotangents, float><<<blocks, kBlock, 0, s>>>(
-                (const float*)records, jacobian, n_binaries, (const float*)freqs, n_freqs, (const float*)cot_plus,
-                (const float*)cot_cross, nullptr, grads, nullptr);
-    } else {
-        return fail("wfx_xas_adjoint: unknown precision");
-    }
+    const AdjointArgs a{records, jacobian, n_binaries, freqs, n_freqs, cot_plus, cot_cross, nullptr, grads, nullptr,
+                        powers, blocks, (hipStream_t)stream};
+    if (!launch_adjoint<kFromCotangents>(a, precision, freqs_f64)) return fail("wfx_xas_adjoint: unknown precision");
     return check_launch("wfx_xas_adjoint");
 }
 
@@ -779,35 +856,16 @@ int wfx_xas_grid_adjoint(const void* records, int64_t n_binaries, const void* fr
 
 int wfx_xas_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
                          int freqs_f64, int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
-                         const void* weights, double* loss, double* grads, void* stream) {
+                         const void* weights, const void* powers, double* loss, double* grads, void* stream) {
     if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xas_loss_adjoint: negative size");
     if (n_binaries == 0) return 0;
     if (!records || !jacobian || !grads || !loss || (n_freqs && (!freqs || !data_plus || !data_cross || !weights)))
         return fail("wfx_xas_loss_adjoint: null pointer");
     const int blocks = grid_blocks(n_binaries, "wfx_xas_loss_adjoint: too many binaries");
     if (blocks < 0) return -1;
-    hipStream_t s = (hipStream_t)stream;
-    if (precision == kPrecF64) {
-        if (freqs_f64)
-            adjoint_kernel<double, kLoss, double><<<blocks, kBlock, 0, s>>>(
-                (const double*)records, jacobian, n_binaries, (const double*)freqs, n_freqs,
-                (const double*)data_plus, (const double*)data_cross, (const double*)weights, grads, loss);
-        else
-            adjoint_kernel<double, kLoss, float><<<blocks, kBlock, 0, s>>>(
-                (const double*)records, jacobian, n_binaries, (const float*)freqs, n_freqs, (const double*)data_plus,
-                (const double*)data_cross, (const double*)weights, grads, loss);
-    } else if (precision == kPrecF32 || precision == kPrecSplit || precision == kPrecSplitDF) {
-        if (freqs_f64)
-            adjoint_kernel<float, kLoss, double><<<blocks, kBlock, 0, s>>>(
-                (const float*)records, jacobian, n_binaries, (const double*)freqs, n_freqs, (const float*)data_plus,
-                (const float*)data_cross, (const float*)weights, grads, loss);
-        else
-            adjoint_kernel<float, kLoss, float><<<blocks, kBlock, 0, s>>>(
-                (const float*)records, jacobian, n_binaries, (const float*)freqs, n_freqs, (const float*)data_plus,
-                (const float*)data_cross, (const float*)weights, grads, loss);
-    } else {
-        return fail("wfx_xas_loss_adjoint: unknown precision");
-    }
+    const AdjointArgs a{records, jacobian, n_binaries, freqs, n_freqs, data_plus, data_cross, weights, grads, loss,
+                        powers, blocks, (hipStream_t)stream};
+    if (!launch_adjoint<kLoss>(a, precision, freqs_f64)) return fail("wfx_xas_loss_adjoint: unknown precision");
     return check_launch("wfx_xas_loss_adjoint");
 }
 

@@ -96,7 +96,7 @@ def to_mass_scaled_reference(binary: Binary, reference_mf: float = REFERENCE_MF,
     if f_min is None:
         f = torch.where(f == 0.0, torch.full_like(f, float("nan")), f)
     else:
-        f = torch.where(f == 0.0, torch.as_tensor(f_min, dtype=torch.float64).expand_as(f), f)
+        f = torch.where(f == 0.0, torch.as_tensor(f_min, dtype=torch.float64, device=f.device).expand_as(f), f)
     f = torch.where(f > 0.0, f, torch.full_like(f, float("nan")))
     phi_from = phase(m1, m2, _chi(s1), _chi(s2), f * (m1 + m2) * MTSUN_SI, device)
     phi_to = phase(m1, m2, _chi(s1), _chi(s2), torch.full_like(m1, float(reference_mf)), device)
@@ -108,7 +108,7 @@ def to_physical_reference(binary: Binary, f_ref: Any, reference_mf: float = REFE
     ``reference_mf / M`` (its ``f_ref`` ignored) as a binary given at
     ``f_ref`` (Hz)."""
     (m1, m2, s1, s2, _, _, _, _), device = _fields(binary)
-    f = torch.as_tensor(f_ref, dtype=torch.float64).expand_as(m1)
+    f = torch.as_tensor(f_ref, dtype=torch.float64, device=m1.device).expand_as(m1)
     f = torch.where(f > 0.0, f, torch.full_like(f, float("nan")))
     phi_from = phase(m1, m2, _chi(s1), _chi(s2), torch.full_like(m1, float(reference_mf)), device)
     phi_to = phase(m1, m2, _chi(s1), _chi(s2), f * (m1 + m2) * MTSUN_SI, device)
