@@ -97,8 +97,12 @@ __device__ inline T phase_and_amp(const T* __restrict__ c, const BinPowers<T>& p
     return phi;
 }
 
+// Inlined before the optimiser simplifies it on its own: compiled as a
+// function, acc is opaque memory, and the compiler merges the regions' last
+// updates into one store through a selected address, which later keeps the
+// caller's accumulator array out of registers.
 template <class T>
-__device__ inline void adjoint_point(const T* __restrict__ c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re,
+__device__ __forceinline__ void adjoint_point(const T* __restrict__ c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re,
                                      T gx_im, T* acc) {
     bool past, ringdown;
     T z, amp;
@@ -133,6 +137,11 @@ __device__ inline void adjoint_point(const T* __restrict__ c, const BinPowers<T>
         acc[kInsP43] += g_phi * (f * t);
         acc[kInsP53] += g_phi * (f * t2);
         acc[kInsP2] += g_phi * (f * f);
+        // an empty statement that the compiler may not move or merge: the
+        // branches then end differently, and their last updates are not
+        // merged into one store through a selected slot address (which
+        // would keep the accumulators in scratch memory)
+        __asm__ volatile("");
     } else {
         const T y = f - c[kShift];
         const T a = ringdown ? c[kAtanA2] : c[kAtanA1];
@@ -140,25 +149,25 @@ __device__ inline void adjoint_point(const T* __restrict__ c, const BinPowers<T>
         const T wy = w * y;
         const T datan = d_rcp(T(1) + wy * wy);
         acc[kShift] -= g_phi * a * w * datan;
-        const T m2 = m1 * m1;
-        if (ringdown) {
-            acc[kAtanA2] += g_phi * z;
-            acc[kAtanW2] += g_phi * a * y * datan;
-            acc[kRdConst] += g_phi;
-            acc[kRdP1] += g_phi * f;
-            acc[kRdP23] += g_phi * (t * t);
-            acc[kRdM1] += g_phi * m1;
-            acc[kRdM3] += g_phi * (m2 * m1);
-        } else {
-            acc[kAtanA1] += g_phi * z;
-            acc[kAtanW1] += g_phi * a * y * datan;
-            acc[kIntConst] += g_phi;
-            acc[kIntP1] += g_phi * f;
-            acc[kIntLog] += g_phi * lf;
-            acc[kIntM1] += g_phi * m1;
-            acc[kIntM2] += g_phi * m2;
-            acc[kIntM3] += g_phi * (m2 * m1);
-        }
+        const T m2 = m1 * m1, m3 = m2 * m1;
+        // both regions' slots, the other one's with a zero cotangent (no
+        // pair of same-shaped branches for the compiler to merge)
+        const T g_rd = ringdown ? g_phi : T(0), g_int = ringdown ? T(0) : g_phi;
+        acc[kAtanA2] += g_rd * z;
+        acc[kAtanW2] += g_rd * a * y * datan;
+        acc[kRdConst] += g_rd;
+        acc[kRdP1] += g_rd * f;
+        acc[kRdP23] += g_rd * (t * t);
+        acc[kRdM1] += g_rd * m1;
+        acc[kRdM3] += g_rd * m3;
+        acc[kAtanA1] += g_int * z;
+        acc[kAtanW1] += g_int * a * y * datan;
+        acc[kIntConst] += g_int;
+        acc[kIntP1] += g_int * f;
+        acc[kIntLog] += g_int * lf;
+        acc[kIntM1] += g_int * m1;
+        acc[kIntM2] += g_int * m2;
+        acc[kIntM3] += g_int * m3;
     }
     // the amplitude
     if (f > c[kFAmpRd]) {

@@ -363,10 +363,106 @@ __global__ __launch_bounds__(1024) void tile_prefix_kernel(const int64_t* __rest
 constexpr int kFromCotangents = 0;
 constexpr int kLoss = 1;
 
+// Cross-lane moves of one wave for the reductions of the adjoint kernel.
+// swap32(a, b): v_permlane32_swap, lanes 32-63 of a trade places with
+// lanes 0-31 of b; swap16(a, b): v_permlane16_swap, the odd 16-lane rows of
+// a with the even rows of b. dpp<Ctrl>(x): x of the lane that the DPP
+// control names (row_ror:8 = 0x128, row_half_mirror = 0x141, quad_perm
+// [2,3,0,1] = 0x4e, [1,0,3,2] = 0xb1). Doubles move as two 32-bit halves.
+__device__ inline void swap32(float& a, float& b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+__device__ inline void swap16(float& a, float& b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+__device__ inline void swap32(double& a, double& b) {
+    const uint64_t ua = (uint64_t)__double_as_longlong(a), ub = (uint64_t)__double_as_longlong(b);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+    a = __hiloint2double((int)hi[0], (int)lo[0]);
+    b = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ inline void swap16(double& a, double& b) {
+    const uint64_t ua = (uint64_t)__double_as_longlong(a), ub = (uint64_t)__double_as_longlong(b);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ua, (unsigned)ub, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+    a = __hiloint2double((int)hi[0], (int)lo[0]);
+    b = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+template <int Ctrl>
+__device__ inline float dpp(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), Ctrl, 0xF, 0xF, false));
+}
+template <int Ctrl>
+__device__ inline double dpp(double x) {
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, Ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), Ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// One level of a reduce-scatter over the wave: of each pair (v[k], v[k+H]),
+// k < H, the lanes with bit Bit of their index clear keep the first and the
+// others the second, each adding its partner's copy (the partner differs in
+// that bit and agrees in the higher ones).
+template <int Bit, int H, class T>
+__device__ inline void scatter_level(T* v, int lane) {
+    if constexpr (Bit == 32) {
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            swap32(v[k], v[k + H]);
+            v[k] += v[k + H];
+        }
+    } else if constexpr (Bit == 16) {
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            swap16(v[k], v[k + H]);
+            v[k] += v[k + H];
+        }
+    } else {
+        constexpr int ctrl = Bit == 8 ? 0x128 : Bit == 4 ? 0x141 : Bit == 2 ? 0x4e : 0xb1;
+        const bool high = lane & Bit;
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            const T keep = high ? v[k + H] : v[k];
+            const T send = high ? v[k] : v[k + H];
+            v[k] = keep + dpp<ctrl>(send);
+        }
+    }
+}
+
+// v[0..63] per lane in, the sum over the wave's 64 lanes of v[lane] out
+// (in v[0]: each level halves the live values and lane L keeps index L).
+// Every lane of the wave must be active.
 template <class T>
-__device__ inline T wave_sum(T v) {
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
+__device__ inline T reduce_scatter64(T* v, int lane) {
+    scatter_level<32, 32>(v, lane);
+    scatter_level<16, 16>(v, lane);
+    scatter_level<8, 8>(v, lane);
+    scatter_level<4, 4>(v, lane);
+    scatter_level<2, 2>(v, lane);
+    scatter_level<1, 1>(v, lane);
+    return v[0];
+}
+
+// sum over the wave of s[lane] * jac_row[lane][d] for the 8 parameters d:
+// the value of parameter lane / 8, in every lane of its group of 8
+__device__ inline double contract8(double s, const double* __restrict__ jac_row, int lane) {
+    double p[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) p[d] = s * jac_row[d];
+    scatter_level<32, 4>(p, lane);
+    scatter_level<16, 2>(p, lane);
+    scatter_level<8, 1>(p, lane);
+    double g = p[0];
+    g += dpp<0x141>(g);
+    g += dpp<0x4e>(g);
+    g += dpp<0xb1>(g);
+    return g;
 }
 
 template <class T>
@@ -400,21 +496,34 @@ __global__ __launch_bounds__(kBlock) void power_table_kernel(const F* __restrict
     table[i] = PowerRow<T>{p.f, p.t, p.m1, p.lf, p.m76, p.m53};
 }
 
+// One wave per binary: lane l takes bins l, l + 64, ...; each lane sums its
+// bins' dL/d(record slot) in registers, one reduce-scatter over the wave
+// leaves slot l's total in lane l, and the wave contracts the totals with
+// the record's derivatives. The loss rides in the slot of T0's low part,
+// which carries no derivative of its own (T0 enters as hi + lo, accounted
+// on kT0).
+constexpr int kAdjointBlock = 64;
+constexpr int kLossSlot = kT0Lo;
+
 template <class T, int Mode, class F>
-__global__ __launch_bounds__(kBlock) void adjoint_kernel(const T* __restrict__ records, const double* __restrict__ jac,
-                                                         int64_t nb, const F* __restrict__ freqs, int64_t n,
-                                                         const T* __restrict__ in_plus, const T* __restrict__ in_cross,
-                                                         const T* __restrict__ weights, double* __restrict__ grads,
-                                                         double* __restrict__ loss,
-                                                         const PowerRow<T>* __restrict__ table) {
+__global__ __launch_bounds__(kAdjointBlock) void adjoint_kernel(const T* __restrict__ records,
+                                                                const double* __restrict__ jac, int64_t nb,
+                                                                const F* __restrict__ freqs, int64_t n,
+                                                                const T* __restrict__ in_plus,
+                                                                const T* __restrict__ in_cross,
+                                                                const T* __restrict__ weights,
+                                                                double* __restrict__ grads, double* __restrict__ loss,
+                                                                const PowerRow<T>* __restrict__ table) {
     const int64_t b = blockIdx.x;
     if (b >= nb) return;
-    const T* c = records + b * kSlots;
+    const int lane = threadIdx.x;
+    T c[kSlots];
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) c[k] = records[b * kSlots + k];
     T acc[kSlots];
 #pragma unroll
     for (int k = 0; k < kSlots; ++k) acc[k] = T(0);
-    T loss_acc = T(0);
-    for (int64_t i = threadIdx.x; i < n; i += kBlock) {
+    for (int64_t i = lane; i < n; i += kAdjointBlock) {
         const BinPowers<T> p = powers_at<T>(freqs, table, i);
         const int64_t o = 2 * (b * n + i);
         T gp_re = in_plus[o], gp_im = in_plus[o + 1], gx_re = in_cross[o], gx_im = in_cross[o + 1];
@@ -428,7 +537,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(const T* __restrict__ r
             const T ap = c[kKp] * amp, ac = c[kKc] * amp;
             const T rp_re = ap * co - gp_re, rp_im = ap * s - gp_im;
             const T rx_re = -ac * s - gx_re, rx_im = ac * co - gx_im;
-            loss_acc += w * (rp_re * rp_re + rp_im * rp_im + rx_re * rx_re + rx_im * rx_im);
+            acc[kLossSlot] += w * (rp_re * rp_re + rp_im * rp_im + rx_re * rx_re + rx_im * rx_im);
             const T two_w = T(2) * w;
             gp_re = two_w * rp_re;
             gp_im = two_w * rp_im;
@@ -437,37 +546,13 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(const T* __restrict__ r
         }
         adjoint_point(c, p, gp_re, gp_im, gx_re, gx_im, acc);
     }
-    __shared__ double partial[kBlock / 64][kSlots + 1];
-    __shared__ double slots[kSlots];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < kSlots; ++k) {
-        const double v = wave_sum((double)acc[k]);
-        if (lane == 0) partial[wave][k] = v;
-    }
-    if (Mode == kLoss) {
-        const double v = wave_sum((double)loss_acc);
-        if (lane == 0) partial[wave][kSlots] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < kSlots) {
-        double v = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w) v += partial[w][threadIdx.x];
-        // the FP32 record's T0 and A enter in FP64 through hi + lo
-        slots[threadIdx.x] = threadIdx.x == kT0Lo || threadIdx.x == kAnchorLo || threadIdx.x >= 60 ? 0.0 : v;
-    }
-    if (Mode == kLoss && threadIdx.x == kBlock - 1) {
-        double v = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w) v += partial[w][kSlots];
-        loss[b] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 8) {
-        const double* jb = jac + b * kSlots * 8;
-        double g = 0.0;
-        for (int k = 0; k < 60; ++k) g += slots[k] * jb[k * 8 + threadIdx.x];
-        grads[b * 8 + threadIdx.x] = g;
-    }
+    double s = (double)reduce_scatter64(acc, lane);
+    if (Mode == kLoss && lane == kLossSlot) loss[b] = s;
+    // slots without a derivative: T0's and A's low parts (the FP32 record's
+    // T0 and A enter in FP64 through hi + lo) and the FP64 copies 60..63
+    if (lane == kT0Lo || lane == kAnchorLo || lane >= 60) s = 0.0;
+    const double g = contract8(s, jac + (b * kSlots + lane) * 8, lane);
+    if ((lane & 7) == 0) grads[b * 8 + (lane >> 3)] = g;
 }
 
 // dL/df of each frequency of a shared grid: the evaluation's launch shape
@@ -514,7 +599,7 @@ struct AdjointArgs {
 
 template <class T, int Mode, class F>
 void launch_adjoint_t(const AdjointArgs& a) {
-    adjoint_kernel<T, Mode, F><<<a.blocks, kBlock, 0, a.stream>>>(
+    adjoint_kernel<T, Mode, F><<<a.blocks, kAdjointBlock, 0, a.stream>>>(
         (const T*)a.records, a.jac, a.nb, (const F*)a.freqs, a.n, (const T*)a.in_plus, (const T*)a.in_cross,
         (const T*)a.weights, a.grads, a.loss, (const PowerRow<T>*)a.powers);
 }
