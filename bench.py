@@ -61,8 +61,9 @@ def timed(torch, fn, steps, warmup):
 def bench_grad(torch, xas, dev, nb, n, steps, warmup):
     """The fused forward + adjoint of L = sum_f (|h+ - d+|^2 + |hx - dx|^2) / S(f)
     for nb binaries x n bins (FP32-split), d at parameters perturbed by 1%:
-    binaries/s of the whole gradient step (FP32 record setup, FP64 setup with
-    derivatives, fused kernel), and of the forward step alone."""
+    binaries/s of the whole gradient step (the FP64 setup with derivatives,
+    which also writes the FP32-split record; the grid's power table; the
+    fused kernel), and its time over the forward step's."""
     from waveformx_amd import adjoint
 
     freqs = np.linspace(20.0, 1024.0, n)
@@ -222,7 +223,7 @@ def main():
         out = (torch.empty((nb, n), dtype=torch.complex64, device=dev),
                torch.empty((nb, n), dtype=torch.complex64, device=dev))
         for group in (4, 8, 16, 32):
-            for fpt in (1, 2, 4):
+            for fpt in (2, 4, 8):
                 ms = timed(torch, lambda: xas.evaluate(records, f32, "split", group, out=out, freqs_per_thread=fpt),
                            args.steps, args.warmup)
                 print(json.dumps({"group": group, "freqs_per_thread": fpt, "eval_ms": ms,

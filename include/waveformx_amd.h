@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (7); a binding checks it before any call. */
+/* The version of this ABI (8); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -61,13 +61,15 @@ int wfx_xas_setup(const double* params, int64_t n_binaries, const double* f_min,
                   double amp_scale, double reference_mf, int precision, void* records, uint32_t* flags,
                   void* stream);
 
-/* The FP64 records and their derivatives jacobian[B][64][8] with respect to
- * the 8 packed parameters (forward mode through the setup), for the
- * adjoint. Replaces the forward half of the reference's jax.vjp
- * (src/waveformx/torch.py:87-93). */
+/* The records and their derivatives jacobian[B][64][8] (of the FP64
+ * record's slots) with respect to the 8 packed parameters (forward mode
+ * through the setup), for the adjoint. records: FP64 records as precision 0
+ * writes them, split_records: FP32 records as precision 2 and 3 write them;
+ * either may be NULL, not both. Replaces the forward half of the
+ * reference's jax.vjp (src/waveformx/torch.py:87-93). */
 int wfx_xas_setup_jvp(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
-                      double amp_scale, double reference_mf, double* records, double* jacobian, uint32_t* flags,
-                      void* stream);
+                      double amp_scale, double reference_mf, double* records, float* split_records,
+                      double* jacobian, uint32_t* flags, void* stream);
 
 /* phase[B] = LAL's (2,2) phase of each binary at M f = mf[b] but for its
  * constant, phase_22 / eta + linb M f, in FP64: what the reference-
@@ -77,10 +79,11 @@ int wfx_xas_reference_phase(const double* params, int64_t n_binaries, const doub
                             void* stream);
 
 /* (h+, hx)[B][N] of records on one shared grid freqs[N] (float, or double
- * if freqs_f64). group: binaries per block; freqs_per_thread: 1, 2 or 4
- * frequencies per thread on the FP32 paths (0: defaults). Replaces
- * core.frequency_powers + core.evaluate under jax.vmap
- * (src/waveformx/models/xas/core.py:382-507; src/waveformx/torch.py:84-85). */
+ * if freqs_f64). group: binaries per block; freqs_per_thread: 1, 2, 4 or 8
+ * frequencies per thread on the FP32 paths (0: defaults, 8 and 4); every
+ * shape writes the same bits. Replaces core.frequency_powers +
+ * core.evaluate under jax.vmap (src/waveformx/models/xas/core.py:382-507;
+ * src/waveformx/torch.py:84-85). */
 int wfx_xas_eval(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
                  int precision, int group, int freqs_per_thread, void* hplus, void* hcross, void* stream);
 

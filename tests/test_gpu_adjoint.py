@@ -178,3 +178,23 @@ def test_fused_loss_gradient(cuda, precision):
         assert np.allclose(loss, expected_loss, rtol=1e-4)
         bar = 1e-2 * np.abs(analytic[:, :7]) + 1e-3 * np.max(np.abs(analytic[:, :7]), axis=0)
         assert np.all(np.abs(grads[:, :7] - analytic[:, :7]) <= bar)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split", "split-df"])
+def test_jacobian_setup_writes_the_forward_records(cuda, precision):
+    """The record that the derivative setup writes alongside the Jacobian is
+    the forward setup's record, bit for bit (FP64, and the FP32 split record
+    with the anchor folded into the post-inspiral constants)."""
+    import torch
+
+    from waveformx_amd import xas
+
+    params = torch.as_tensor(oracle.draw(200, seed=8), device=cuda)
+    f_min = torch.tensor([20.0], dtype=torch.float64, device=cuda)
+    expected, flags = xas.setup(params, f_min, 1e21, None, precision)
+    records, jac, jflags = xas.setup_jvp(params, f_min, 1e21, None, precision)
+    assert records.dtype == expected.dtype
+    assert torch.equal(records.view(torch.int32 if precision != "fp64" else torch.int64),
+                       expected.view(torch.int32 if precision != "fp64" else torch.int64))
+    assert torch.equal(flags, jflags)
+    assert jac.shape == (200, 64, 8) and bool(torch.isfinite(jac[:, :60]).all())

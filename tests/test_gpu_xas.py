@@ -180,3 +180,20 @@ def test_ragged_matches_oracle(cuda, precision):
             assert compare(a, op[0])[1] < 1e-6
             assert compare(b, op[0])[1] < 1e-6
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("precision", ["split", "split-df", "fp32"])
+def test_launch_shapes_agree(xas, cuda, precision):
+    """Every launch shape of the FP32 evaluation writes the same bits (the
+    group and the frequencies per thread only move work between threads),
+    including groups and tiles that overhang the batch and the grid."""
+    import torch
+
+    params = torch.as_tensor(oracle.draw(37, seed=9), device=cuda)
+    f = torch.as_tensor(np.linspace(20.0, 1024.0, 1500), dtype=torch.float32, device=cuda)
+    records, _ = xas.setup(params, f[:1], 1e21, None, precision)
+    ref = [t.view(torch.int32).clone() for t in xas.evaluate(records, f, precision, 1, freqs_per_thread=1)]
+    for group, fpt in ((0, 0), (1, 2), (3, 4), (8, 8), (16, 1), (64, 4)):
+        hp, hc = xas.evaluate(records, f, precision, group, freqs_per_thread=fpt)
+        assert torch.equal(hp.view(torch.int32), ref[0]), (group, fpt)
+        assert torch.equal(hc.view(torch.int32), ref[1]), (group, fpt)

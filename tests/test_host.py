@@ -36,6 +36,8 @@ def test_library_exports_header_symbols():
     assert b"null" in lib.wfx_last_error()
     assert lib.wfx_xas_eval(None, -1, None, 0, 10, 0, 0, 0, None, None, None) == -1
     assert lib.wfx_xas_eval(None, 4, None, 0, 10, 2, 0, 3, None, None, None) == -1
+    assert lib.wfx_xas_setup_jvp(None, 4, None, 0, 1.0, float("nan"), None, None, None, None, None) == -1
+    assert b"null" in lib.wfx_last_error()
     # empty batches are a no-op
     assert lib.wfx_xas_setup(None, 0, None, 0, 1.0, float("nan"), 0, None, None, None) == 0
 

@@ -28,24 +28,17 @@ PARAMETERS = ("mass_1", "mass_2", "chi_1", "chi_2", "luminosity_distance", "incl
 
 
 def _records(params, freqs, amp_scale, reference_mf, precision):
-    records64, jac, _ = xas.setup_jvp(params, freqs[:1], amp_scale, reference_mf)
-    if precision == "fp64":
-        return records64, jac
+    """The forward precision's records and the FP64 records' derivatives
+    (one launch; plain FP32 records come from their own FP32 setup)."""
+    if precision in ("fp64", "split", "split-df"):
+        records, jac, _ = xas.setup_jvp(params, freqs[:1], amp_scale, reference_mf, precision)
+        return records, jac
+    _, jac, _ = xas.setup_jvp(params, freqs[:1], amp_scale, reference_mf, "split")
     records, _ = xas.setup(params, freqs[:1], amp_scale, reference_mf, precision)
     return records, jac
 
 
-def power_table(freqs: torch.Tensor, precision: str) -> torch.Tensor:
-    """The grid's table of frequency powers (bytes, on the device) that the
-    adjoint kernels read instead of recomputing them per bin."""
-    from ._lib import lib
-
-    _require_device(freqs, "freqs")
-    row = lib().wfx_xas_power_table_bytes(_code(precision))
-    table = torch.empty((freqs.shape[0], row), dtype=torch.uint8, device=freqs.device)
-    call("wfx_xas_power_table", _ptr(freqs), int(freqs.dtype == torch.float64), freqs.shape[0], _code(precision),
-         _ptr(table), _stream(freqs.device))
-    return table
+power_table = xas.power_table
 
 
 def adjoint(records: torch.Tensor, jac: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor,

@@ -13,6 +13,8 @@
 // recomputed, none stored.
 #pragma once
 
+#include <type_traits>
+
 #include "xas_point.h"
 
 namespace wfx {
@@ -76,10 +78,15 @@ __device__ inline T phase_and_amp(const T* __restrict__ c, const BinPowers<T>& p
         rest = c[kRdConst] + c[kRdP1] * f + c[kRdP23] * (t * t) + c[kRdM1] * m1 + c[kRdM3] * (m2 * m1) +
                c[kAtanA2] * z;
     }
-    double full = (double)rest - ((double)c[kAnchor] + (double)c[kAnchorLo]);
-    if (!past) full = fma((double)c[kT0] + (double)c[kT0Lo], p.m53, full);
-    const double turns = rint(full * (1.0 / kTwoPi));
-    const T phi = (T)fma(-turns, kTwoPi, full);
+    T phi;
+    if (std::is_same<T, float>::value && past) {
+        phi = (T)reduce_f32((float)rest);  // FP32 records hold -A in these constants (fold_anchor)
+    } else {
+        double full = (double)rest - ((double)c[kAnchor] + (double)c[kAnchorLo]);
+        if (!past) full = fma((double)c[kT0] + (double)c[kT0Lo], p.m53, full);
+        const double turns = rint(full * (1.0 / kTwoPi));
+        phi = (T)fma(-turns, kTwoPi, full);
+    }
 
     if (f > c[kFAmpRd]) {
         const T u = c[kAmpRdS] * (f - c[kShift]);

@@ -95,8 +95,25 @@ __global__ __launch_bounds__(64) void setup_f64_kernel(const double* __restrict_
     if (flags) flags[b] = fl;
 }
 
-// FP32 record from the FP64 setup (split precision): every slot rounded to
-// FP32; T0 and A also as hi + lo and as doubles in slots 60..63
+// FP32 record from FP64 slots: every slot rounded to FP32 but the
+// post-inspiral constants, which take -A modulo 2 pi first (fold_anchor);
+// T0 and A also as hi + lo and as doubles in slots 60..63 (low word first)
+__device__ inline void write_f32_record(const double* rec, float* __restrict__ out) {
+    for (int k = 0; k < 60; ++k) out[k] = (float)rec[k];
+    const double t0 = rec[kT0], anchor = rec[kAnchor];
+    out[kIntConst] = (float)fold_anchor(rec[kIntConst], anchor);
+    out[kRdConst] = (float)fold_anchor(rec[kRdConst], anchor);
+    out[kT0] = (float)t0;
+    out[kT0Lo] = (float)(t0 - (double)out[kT0]);
+    out[kAnchor] = (float)anchor;
+    out[kAnchorLo] = (float)(anchor - (double)out[kAnchor]);
+    out[kT0F64] = __int_as_float(__double2loint(t0));
+    out[kT0F64 + 1] = __int_as_float(__double2hiint(t0));
+    out[kAnchorF64] = __int_as_float(__double2loint(anchor));
+    out[kAnchorF64 + 1] = __int_as_float(__double2hiint(anchor));
+}
+
+// FP32 record of the FP64 setup (split precision)
 __global__ __launch_bounds__(64) void setup_split_kernel(const double* __restrict__ params, int64_t nb,
                                                          const double* __restrict__ f_min, int64_t f_min_stride,
                                                          double amp_scale, double reference_mf,
@@ -108,19 +125,12 @@ __global__ __launch_bounds__(64) void setup_split_kernel(const double* __restric
     with_reference_mf(p, reference_mf);
     double rec[kSlots];
     const unsigned fl = xas_record<double>(p, f_min[b * f_min_stride], amp_scale, rec);
-    float* out = records + b * kSlots;
-    for (int k = 0; k < 60; ++k) out[k] = (float)rec[k];
-    const double t0 = rec[kT0], anchor = rec[kAnchor];
-    out[kT0] = (float)t0;
-    out[kT0Lo] = (float)(t0 - (double)out[kT0]);
-    out[kAnchor] = (float)anchor;
-    out[kAnchorLo] = (float)(anchor - (double)out[kAnchor]);
-    *reinterpret_cast<double*>(out + kT0F64) = t0;
-    *reinterpret_cast<double*>(out + kAnchorF64) = anchor;
+    write_f32_record(rec, records + b * kSlots);
     if (flags) flags[b] = fl;
 }
 
-// plain FP32 record: the whole setup in FP32
+// plain FP32 record: the whole setup in FP32 (then written as above, T0 and
+// A exact in FP32, so their low parts are zero)
 __global__ __launch_bounds__(64) void setup_f32_kernel(const double* __restrict__ params, int64_t nb,
                                                        const double* __restrict__ f_min, int64_t f_min_stride,
                                                        double amp_scale, double reference_mf,
@@ -132,23 +142,21 @@ __global__ __launch_bounds__(64) void setup_f32_kernel(const double* __restrict_
     with_reference_mf(p, reference_mf);
     float rec[kSlots];
     const unsigned fl = xas_record<float>(p, (float)f_min[b * f_min_stride], amp_scale, rec);
-    rec[kT0Lo] = 0.0f;
-    rec[kAnchorLo] = 0.0f;
-    float* out = records + b * kSlots;
-    for (int k = 0; k < 60; ++k) out[k] = rec[k];
-    *reinterpret_cast<double*>(out + kT0F64) = (double)rec[kT0];
-    *reinterpret_cast<double*>(out + kAnchorF64) = (double)rec[kAnchor];
+    double wide[kSlots];
+    for (int k = 0; k < kSlots; ++k) wide[k] = (double)rec[k];
+    write_f32_record(wide, records + b * kSlots);
     if (flags) flags[b] = fl;
 }
 
-// the FP64 record and its derivatives with respect to the 8 parameters
-// (m1, m2, chi1, chi2, distance, inclination, phi_ref, f_ref): one thread
-// per (binary, direction); jac (B, 64, 8)
+// the record and its derivatives with respect to the 8 parameters (m1, m2,
+// chi1, chi2, distance, inclination, phi_ref, f_ref): one thread per
+// (binary, direction); jac (B, 64, 8); the direction-0 thread writes the
+// record, FP64 (records) and/or FP32 split (split_records), either NULL
 __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict__ params, int64_t nb,
                                                        const double* __restrict__ f_min, int64_t f_min_stride,
                                                        double amp_scale, double reference_mf,
-                                                       double* __restrict__ records, double* __restrict__ jac,
-                                                       uint32_t* __restrict__ flags) {
+                                                       double* __restrict__ records, float* __restrict__ split_records,
+                                                       double* __restrict__ jac, uint32_t* __restrict__ flags) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t b = gid >> 3;
     const int dir = (int)(gid & 7);
@@ -164,15 +172,17 @@ __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict_
     const unsigned fl = xas_record<D>(p, D(f_min[b * f_min_stride]), amp_scale, rec);
     for (int k = 0; k < kSlots; ++k) jac[(b * kSlots + k) * 8 + dir] = k < 60 ? rec[k].d[0] : 0.0;
     if (dir == 0) {
-        for (int k = 0; k < kSlots; ++k) records[b * kSlots + k] = rec[k].v;
-        records[b * kSlots + kT0Lo] = 0.0;
-        records[b * kSlots + kAnchorLo] = 0.0;
+        double v[kSlots];
+        for (int k = 0; k < kSlots; ++k) v[k] = rec[k].v;
+        v[kT0Lo] = 0.0;
+        v[kAnchorLo] = 0.0;
+        if (records)
+            for (int k = 0; k < kSlots; ++k) records[b * kSlots + k] = v[k];
+        if (split_records) write_f32_record(v, split_records + b * kSlots);
         if (flags) flags[b] = fl;
     }
 }
 
-// LAL's (2,2) phase of each binary at its M f = mf[b], FP64 (the
-// reference-convention converters)
 __global__ __launch_bounds__(64) void reference_phase_kernel(const double* __restrict__ params, int64_t nb,
                                                              const double* __restrict__ mf,
                                                              double* __restrict__ phase) {
@@ -217,7 +227,14 @@ __global__ __launch_bounds__(kBlock) void eval_f64_shared(const double* __restri
 
 // shared grid, FP32 paths: a block owns kBlock * FPT frequencies (thread t
 // takes t, t + kBlock, ...) and a group of binaries; each binary's record
-// is read once per iteration (scalar loads) and serves FPT bins per thread
+// is read once per iteration (scalar loads) and serves FPT bins per thread.
+//
+// The shape trades three costs, measured on MI355X at 65,536 x 4017: the
+// powers (amortised over the group), the record's scalar loads and their
+// latency (amortised over FPT bins per thread), and the write pattern (the
+// output streams best when the blocks running at one time on an XCD write a
+// narrow window of it; a block of G binaries writes G rows 32 KB apart).
+// 8 binaries x 4 frequencies per thread balances them.
 template <int L, int FPT, class F>
 __global__ __launch_bounds__(kBlock) void eval_f32_shared(const float* __restrict__ records, int64_t nb,
                                                           const F* __restrict__ freqs, int64_t n, int64_t ntiles,
@@ -617,7 +634,9 @@ bool launch_adjoint(const AdjointArgs& a, int precision, int freqs_f64) {
     return false;
 }
 
-constexpr int kDefaultFreqsPerThread = 2;
+// the FP32 shared-grid evaluation's default shape (see eval_f32_shared)
+constexpr int kDefaultGroup = 8;
+constexpr int kDefaultFreqsPerThread = 4;
 
 struct SharedArgs {
     const void* records;
@@ -644,11 +663,13 @@ void launch_f32_shared(const SharedArgs& a, int freqs_f64, int fpt) {
     if (freqs_f64) {
         if (fpt == 1) launch_f32_shared_t<L, 1, double>(a);
         else if (fpt == 2) launch_f32_shared_t<L, 2, double>(a);
-        else launch_f32_shared_t<L, 4, double>(a);
+        else if (fpt == 4) launch_f32_shared_t<L, 4, double>(a);
+        else launch_f32_shared_t<L, 8, double>(a);
     } else {
         if (fpt == 1) launch_f32_shared_t<L, 1, float>(a);
         else if (fpt == 2) launch_f32_shared_t<L, 2, float>(a);
-        else launch_f32_shared_t<L, 4, float>(a);
+        else if (fpt == 4) launch_f32_shared_t<L, 4, float>(a);
+        else launch_f32_shared_t<L, 8, float>(a);
     }
 }
 
@@ -707,7 +728,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 7; }
+int wfx_abi_version(void) { return 8; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -748,16 +769,17 @@ int wfx_xas_setup(const double* params, int64_t n_binaries, const double* f_min,
 }
 
 int wfx_xas_setup_jvp(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
-                      double amp_scale, double reference_mf, double* records, double* jacobian, uint32_t* flags,
-                      void* stream) {
+                      double amp_scale, double reference_mf, double* records, float* split_records, double* jacobian,
+                      uint32_t* flags, void* stream) {
     if (n_binaries < 0) return fail("wfx_xas_setup_jvp: negative number of binaries");
     if (n_binaries == 0) return 0;
-    if (!params || !f_min || !records || !jacobian) return fail("wfx_xas_setup_jvp: null pointer");
+    if (!params || !f_min || !jacobian) return fail("wfx_xas_setup_jvp: null pointer");
+    if (!records && !split_records) return fail("wfx_xas_setup_jvp: no record output");
     if (f_min_stride != 0 && f_min_stride != 1) return fail("wfx_xas_setup_jvp: f_min_stride must be 0 or 1");
     const int blocks = grid_blocks((8 * n_binaries + 63) / 64, "wfx_xas_setup_jvp: too many binaries");
     if (blocks < 0) return -1;
     setup_jvp_kernel<<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, f_min, f_min_stride, amp_scale,
-                                                             reference_mf, records, jacobian, flags);
+                                                             reference_mf, records, split_records, jacobian, flags);
     return check_launch("wfx_xas_setup_jvp");
 }
 
@@ -778,11 +800,12 @@ int wfx_xas_eval(const void* records, int64_t n_binaries, const void* freqs, int
     if (n_binaries == 0 || n_freqs == 0) return 0;
     if (!records || !freqs || !hplus || !hcross) return fail("wfx_xas_eval: null pointer");
     if (precision < kPrecF64 || precision > kPrecSplitDF) return fail("wfx_xas_eval: unknown precision");
-    if (freqs_per_thread < 0 || freqs_per_thread == 3 || freqs_per_thread > 4)
-        return fail("wfx_xas_eval: freqs_per_thread must be 1, 2 or 4 (0: default)");
+    if (freqs_per_thread != 0 && freqs_per_thread != 1 && freqs_per_thread != 2 && freqs_per_thread != 4 &&
+        freqs_per_thread != 8)
+        return fail("wfx_xas_eval: freqs_per_thread must be 1, 2, 4 or 8 (0: default)");
     if (precision == kPrecF64) freqs_per_thread = 1;
     if (freqs_per_thread == 0) freqs_per_thread = kDefaultFreqsPerThread;
-    if (group <= 0) group = precision == kPrecF64 ? 8 : 16;
+    if (group <= 0) group = kDefaultGroup;
     const int64_t ntiles = (n_freqs + kBlock * freqs_per_thread - 1) / (kBlock * freqs_per_thread);
     const int64_t ngroups = (n_binaries + group - 1) / group;
     const int blocks = grid_blocks(ntiles * ngroups, "wfx_xas_eval: grid too large, raise the group size");

@@ -137,12 +137,31 @@ constexpr float kTwoPiHiF = 6.28318548202514648f;     // float(2 pi)
 constexpr float kTwoPiLoF = -1.7484555314695172e-07f;  // 2 pi - float(2 pi)
 constexpr double kInvTwoPi = 0.15915494309189535;
 
-// T0 and A of a FP32 record in FP64 (slots 60..63 hold them as doubles)
+// T0 and A of a FP32 record in FP64 (slots 60..63 hold them as doubles,
+// low word first); assembled from the two words so that a record copied
+// to registers stays in registers
 __device__ inline double record_f64(const float* c, int slot) {
-    return *reinterpret_cast<const double*>(c + slot);
+    return __hiloint2double(__float_as_int(c[slot + 1]), __float_as_int(c[slot]));
 }
 constexpr int kT0F64 = 60;
 constexpr int kAnchorF64 = 62;
+
+// The FP32 records' post-inspiral constants (kIntConst, kRdConst) hold
+// their FP64 value minus the anchor A, reduced modulo 2 pi: past the
+// inspiral the phase is then the region's sum alone, of modest size,
+// reduced in FP32; the inspiral keeps T0 f^(-5/3) - A in the lead's
+// arithmetic. The derivative of such a constant is that of the constant
+// minus that of A (the reduction is a constant shift).
+WFX_HD inline double fold_anchor(double constant, double anchor) {
+    const double v = constant - anchor;
+    return v - kTwoPi * rint(v * (1.0 / kTwoPi));
+}
+
+// x reduced to [-pi, pi] in FP32 (Cody-Waite with a two-part 2 pi)
+__device__ inline float reduce_f32(float x) {
+    const float turns = rintf(x * kInvTwoPiF);
+    return fmaf(-turns, kTwoPiLoF, fmaf(-turns, kTwoPiHiF, x));
+}
 
 template <int L>
 __device__ inline void point_f32(const float* __restrict__ c, const PowF32& p, float2& hp, float2& hc) {
@@ -181,24 +200,19 @@ __device__ inline void point_f32(const float* __restrict__ c, const PowF32& p, f
     }
 
     float phi;
-    if (L == kLeadF64) {
-        const double base = (double)rest - record_f64(c, kAnchorF64);
-        const double full = past ? base : fma(record_f64(c, kT0F64), p.m53, base);
+    if (past) {
+        phi = reduce_f32(rest);  // the constants hold -A (fold_anchor)
+    } else if (L == kLeadF64) {
+        const double full = fma(record_f64(c, kT0F64), p.m53, (double)rest - record_f64(c, kAnchorF64));
         const double turns = rint(full * kInvTwoPi);
         phi = (float)fma(-turns, kTwoPi, full);
     } else if (L == kLeadDF) {
-        F2 lead = past ? F2{0.0f, 0.0f} : f2_mul(F2{c[kT0], c[kT0Lo]}, F2{p.m53_hi, p.m53_lo});
+        const F2 lead = f2_mul(F2{c[kT0], c[kT0Lo]}, F2{p.m53_hi, p.m53_lo});
         F2 s = f2_add(lead, F2{-c[kAnchor], -c[kAnchorLo]});
         s = f2_add(s, F2{rest, 0.0f});
-        const float turns = rintf(s.hi * kInvTwoPiF);
-        float r = fmaf(-turns, kTwoPiHiF, s.hi);
-        r = fmaf(-turns, kTwoPiLoF, r);
-        phi = r + s.lo;
+        phi = reduce_f32(s.hi) + s.lo;
     } else {
-        const float base = rest - c[kAnchor];
-        const float full = past ? base : fmaf(c[kT0], p.m53_hi, base);
-        const float turns = rintf(full * kInvTwoPiF);
-        phi = fmaf(-turns, kTwoPiLoF, fmaf(-turns, kTwoPiHiF, full));
+        phi = reduce_f32(fmaf(c[kT0], p.m53_hi, rest - c[kAnchor]));
     }
 
     float amp;

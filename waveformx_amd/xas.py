@@ -28,7 +28,7 @@ import torch
 from ._lib import call, lib
 
 __all__ = ["PRECISIONS", "RECORD_SLOTS", "setup", "setup_jvp", "evaluate", "evaluate_ragged", "tile_prefix",
-           "polarizations", "pack_params", "LAL_FLAGS"]
+           "power_table", "polarizations", "pack_params", "LAL_FLAGS"]
 
 PRECISIONS = {"fp64": 0, "fp32": 1, "split": 2, "split-df": 3}
 RECORD_SLOTS = 64
@@ -97,17 +97,24 @@ def setup(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21, re
 
 
 def setup_jvp(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21,
-              reference_mf: Optional[float] = None) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
-    """FP64 records (B, 64), their derivatives (B, 64, 8) with respect to
-    (m1, m2, chi1, chi2, distance, inclination, phi_ref, f_ref), and flags."""
+              reference_mf: Optional[float] = None,
+              precision: str = "fp64") -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Records (B, 64) of ``precision`` ("fp64": float64; "split" or
+    "split-df": the float32 records :func:`setup` writes for them), the
+    derivatives (B, 64, 8) of the FP64 record with respect to (m1, m2, chi1,
+    chi2, distance, inclination, phi_ref, f_ref), and flags."""
     _require_device(params, "params")
+    if precision not in ("fp64", "split", "split-df"):
+        raise ValueError(f"setup_jvp writes fp64 or split records, not {precision!r}")
     nb = params.shape[0]
     f_min, stride = _f_min(f_min, nb)
-    records = torch.empty((nb, RECORD_SLOTS), dtype=torch.float64, device=params.device)
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    records = torch.empty((nb, RECORD_SLOTS), dtype=dtype, device=params.device)
     jac = torch.empty((nb, RECORD_SLOTS, 8), dtype=torch.float64, device=params.device)
     flags = torch.empty((nb,), dtype=torch.int32, device=params.device)
+    wide, split = (_ptr(records), None) if precision == "fp64" else (None, _ptr(records))
     call("wfx_xas_setup_jvp", _ptr(params), nb, _ptr(f_min), stride, float(amp_scale),
-         float("nan") if reference_mf is None else float(reference_mf), _ptr(records), _ptr(jac), _ptr(flags),
+         float("nan") if reference_mf is None else float(reference_mf), wide, split, _ptr(jac), _ptr(flags),
          _stream(params.device))
     return records, jac, flags
 
@@ -117,8 +124,8 @@ def evaluate(records: torch.Tensor, freqs: torch.Tensor, precision: str = "split
              freqs_per_thread: int = 0) -> Tuple[torch.Tensor, torch.Tensor]:
     """(h+, hx) (B, N) on one shared grid ``freqs`` (N,) (float32 or
     float64); complex128 for ``"fp64"``, else complex64. ``group``
-    (binaries per block) and ``freqs_per_thread`` (1, 2, 4) shape the
-    launch (0: defaults)."""
+    (binaries per block) and ``freqs_per_thread`` (1, 2, 4, 8) shape the
+    launch (0: defaults); every shape writes the same bits."""
     _require_device(records, "records")
     _require_device(freqs, "freqs")
     if freqs.dtype not in (torch.float32, torch.float64):
@@ -191,6 +198,18 @@ def evaluate_ragged(records: torch.Tensor, offsets: torch.Tensor, freqs: Optiona
          _ptr(tiles), _code(precision), int(bins_per_thread), int(blocks), _ptr(hp), _ptr(hc),
          _stream(records.device))
     return hp, hc
+
+
+def power_table(freqs: torch.Tensor, precision: str) -> torch.Tensor:
+    """The grid's table of frequency powers (bytes, on the device) that the
+    adjoint kernels, which walk every frequency once per binary, read
+    instead of computing the powers per bin (the same values)."""
+    _require_device(freqs, "freqs")
+    row = lib().wfx_xas_power_table_bytes(_code(precision))
+    table = torch.empty((freqs.shape[0], row), dtype=torch.uint8, device=freqs.device)
+    call("wfx_xas_power_table", _ptr(freqs), int(freqs.dtype == torch.float64), freqs.shape[0], _code(precision),
+         _ptr(table), _stream(freqs.device))
+    return table
 
 
 def polarizations(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21,
