@@ -25,6 +25,12 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the FP32-split evaluation at 65,536 x 4017 in its
+# default launch shape, from two rocprofv3 --pmc passes over
+# `bench.py --only split` (summary under profiles/): WRITE_SIZE 4,122,000 KiB
+# x 1.000 + FETCH_SIZE 17,030 KiB x 2.000, the factors that a 1 GiB device
+# copy gives the two counters. Reported only for that workload and shape.
+EVAL_TRAFFIC = {(65536, 4017): 4.256e9}
 METRIC = "XAS waveforms/s @4017 bins: FP32-split vs FP64 GPU (mismatch<=1e-6); fwd+grad/s"
 
 
@@ -58,6 +64,33 @@ def timed(torch, fn, steps, warmup):
     return start.elapsed_time(stop) / steps
 
 
+def accuracy(torch, test, ref, w, rows=4096):
+    """The worst over h+ and hx and over binaries of: the noise-weighted
+    mismatch 1 - <a,b> / sqrt(<a,a><b,b>) (<a,b> = sum_f Re(a b*) w(f), no
+    time or phase maximisation), the phase error |wrap(arg a - arg b)| over
+    bins where b is not zero, and the relative amplitude error
+    ||a| - |b|| / |b| over bins where |b| > 1e-6 max|b| of the row; test
+    complex64, ref complex128, (B, N), in chunks of rows."""
+    worst = {"mismatch": 0.0, "max_phase_error_rad": 0.0, "max_rel_amplitude_error": 0.0}
+    for a_all, b_all in ((test[0], ref[0]), (test[1], ref[1])):
+        for start in range(0, a_all.shape[0], rows):
+            a = a_all[start:start + rows].to(torch.complex128)
+            b = b_all[start:start + rows]
+            ab = (a * b.conj()).real.mul(w).sum(1)
+            aa = a.abs().square().mul(w).sum(1)
+            bb = b.abs().square().mul(w).sum(1)
+            mism = 1.0 - ab / (aa * bb).sqrt()
+            babs = b.abs()
+            dphi = torch.where(babs > 0, (a * b.conj()).angle().abs(), torch.zeros_like(babs))
+            keep = babs > 1e-6 * babs.max(dim=1, keepdim=True).values
+            damp = torch.where(keep, (a.abs() - babs).abs() / torch.where(keep, babs, torch.ones_like(babs)),
+                               torch.zeros_like(babs))
+            worst["mismatch"] = max(worst["mismatch"], float(mism.max()))
+            worst["max_phase_error_rad"] = max(worst["max_phase_error_rad"], float(dphi.max()))
+            worst["max_rel_amplitude_error"] = max(worst["max_rel_amplitude_error"], float(damp.max()))
+    return worst
+
+
 def bench_grad(torch, xas, dev, nb, n, steps, warmup):
     """The fused forward + adjoint of L = sum_f (|h+ - d+|^2 + |hx - dx|^2) / S(f)
     for nb binaries x n bins (FP32-split), d at parameters perturbed by 1%:
@@ -88,8 +121,19 @@ def bench_grad(torch, xas, dev, nb, n, steps, warmup):
 
     ms_fwd = timed(torch, forward, steps, warmup)
     ms_grad = timed(torch, gradient, steps, warmup)
+    # the split gradient against the FP64 adjoint (which matches the FP64
+    # oracle's forward-mode gradient to 1e-8) at the same grid and data:
+    # per parameter, max over binaries of |g - g64| / max over binaries |g64|
+    loss, grads = adjoint.loss_and_grad(params, f32, d_plus, d_cross, w, precision="split")
+    loss64, grads64 = adjoint.loss_and_grad(params, f32.to(torch.float64), d_plus.to(torch.complex128),
+                                            d_cross.to(torch.complex128), w.to(torch.float64), precision="fp64")
+    err = ((grads - grads64).abs().amax(0) / grads64.abs().amax(0))[:7]
+    loss_err = float(((loss - loss64).abs() / loss64.abs()).max())
     return {"fwd_grad_value": nb / (ms_grad * 1e-3), "fwd_grad_ms": ms_grad, "fwd_grad_binaries": nb,
-            "fwd_grad_over_fwd": ms_grad / ms_fwd}
+            "fwd_grad_over_fwd": ms_grad / ms_fwd,
+            "grad_rel_err_vs_fp64": dict(zip(("mass_1", "mass_2", "chi_1", "chi_2", "luminosity_distance",
+                                              "inclination", "phi_ref"), map(float, err.cpu()))),
+            "loss_rel_err_vs_fp64": loss_err}
 
 
 def ragged_grids(nb, seed=5):
@@ -150,12 +194,13 @@ def bench_ragged(torch, xas, dev, nb, steps, warmup):
             "ragged_padded_bins": nb * longest, "ragged_shape": [plan.bins_per_thread, plan.blocks]}
 
 
-def cpu_baseline(n_binaries, freqs):
-    """The FP64 oracle on the host cores on ``n_binaries`` of the workload:
-    median of 3 timed runs after a warm-up."""
+def cpu_baseline(n_binaries, freqs, cores=None):
+    """The FP64 oracle on the host cores (16, the box's share, unless
+    ``cores``) on ``n_binaries`` of the workload: median of 3 timed runs
+    after a warm-up."""
     from oracle import oracle
 
-    cores = min(16, os.cpu_count() or 1)
+    cores = cores or min(16, os.cpu_count() or 1)
     params = draw(n_binaries, seed=2)
     oracle.polarizations(params[:8], freqs, threads=cores)
     times = []
@@ -262,17 +307,15 @@ def main():
                     args.warmup)
     ms_setup = timed(torch, lambda: xas.setup(params, f32[:1], 1e21, None, "split"), args.steps, args.warmup)
 
-    # accuracy of the split path against the FP64 path on the device
+    # accuracy of the split path against the FP64 path on the device (which
+    # matches the FP64 oracle to 1e-12): on the FP32-rounded grid the split
+    # path sees, and on the exact grid
     step("split", f32, out32)()
-    step("fp64", f64, out64)()
     w = 1.0 / torch.as_tensor(((freqs64 / 100.0) ** -4 + 1.0 + (freqs64 / 100.0) ** 2), device=dev)
-    worst = 0.0
-    for a, b in ((out32[0], out64[0]), (out32[1], out64[1])):
-        a = a.to(torch.complex128)
-        ab = (a * b.conj()).real.mul(w).sum(1)
-        aa = a.abs().square().mul(w).sum(1)
-        bb = b.abs().square().mul(w).sum(1)
-        worst = max(worst, float((1.0 - ab / (aa * bb).sqrt()).max()))
+    step("fp64", f32.to(torch.float64), out64)()
+    rounded = accuracy(torch, out32, out64, w)
+    step("fp64", f64, out64)()
+    exact = accuracy(torch, out32, out64, w)
     torch.cuda.synchronize()
 
     bytes_per_launch = 16.0 * nb * n
@@ -298,9 +341,12 @@ def main():
         "split_df_value": nb / (ms_df * 1e-3),
         "eval_ms": ms_eval,
         "setup_ms": ms_setup,
-        "max_weighted_mismatch_vs_fp64": worst,
+        "max_weighted_mismatch_vs_fp64": rounded["mismatch"],
+        "accuracy_vs_fp64_rounded_grid": rounded,
+        "accuracy_vs_fp64_exact_grid": exact,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": EVAL_TRAFFIC.get((nb, n)) if args.group == 0 else None},
     }
     del out32, out64, rec
     torch.cuda.empty_cache()
@@ -310,6 +356,7 @@ def main():
         result.update(bench_ragged(torch, xas, dev, args.ragged_binaries, args.steps, args.warmup))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_binaries, freqs64)
+        result["cpu_baseline_1core"] = cpu_baseline(max(64, args.cpu_binaries // 32), freqs64, cores=1)
     print(json.dumps(result), flush=True)
 
 
