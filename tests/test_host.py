@@ -42,6 +42,35 @@ def test_library_exports_header_symbols():
     assert lib.wfx_xas_setup(None, 0, None, 0, 1.0, float("nan"), 0, None, None, None) == 0
 
 
+def header_arities():
+    header = (ROOT / "include" / "waveformx_amd.h").read_text()
+    header = re.sub(r"/\*.*?\*/", "", header, flags=re.S)
+    arities = {}
+    for name, args in re.findall(r"\b(wfx_\w+)\s*\(([^)]*)\)\s*;", header):
+        args = args.strip()
+        arities[name] = 0 if args in ("", "void") else args.count(",") + 1
+    return arities
+
+
+def test_binding_and_stub_match_the_header():
+    """The ctypes binding and the maintainer's stub in INTEGRATION.md declare
+    as many arguments as the header does, and the library reports the
+    binding's ABI version."""
+    from waveformx_amd import _lib
+
+    arities = header_arities()
+    assert set(arities) == set(_lib.SIGNATURES)
+    for name, (args, _) in _lib.SIGNATURES.items():
+        assert len(args) == arities[name], name
+    assert _lib.lib().wfx_abi_version() == _lib.ABI_VERSION
+    stub = (ROOT / "INTEGRATION.md").read_text()
+    found = re.findall(r"_lib\.(wfx_\w+)\.argtypes = \[([^\]]*)\]", stub)
+    assert found
+    for name, args in found:
+        assert len([a for a in args.split(",") if a.strip()]) == arities[name], name
+    assert f"wfx_abi_version() == {_lib.ABI_VERSION}" in stub
+
+
 def test_parameter_table_forms():
     n = 3
     mapping = {"mass_1": np.array([30.0, 20.0, 10.0]), "mass_2": np.array([20.0, 10.0, 5.0]),
