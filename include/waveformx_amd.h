@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (8); a binding checks it before any call. */
+/* The version of this ABI (9); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -143,6 +143,31 @@ int wfx_xas_grid_adjoint(const void* records, int64_t n_binaries, const void* fr
 int wfx_xas_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
                          int freqs_f64, int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
                          const void* weights, const void* powers, double* loss, double* grads, void* stream);
+
+/* IMRPhenomD, on the same conventions. Its table is FP64 whatever the
+ * evaluation's arithmetic (src/waveformx/models/phenomd/model.py:33-50):
+ * precision 0, 2 or 3 only (1 fails). Records have the 64 slots of
+ * IMRPhenomXAS's (wfx_xas_record_bytes) in IMRPhenomD's layout. LAL's flag:
+ * bit 3, the final spin above one (phenomd/api.py:39-45); with
+ * reference_mf the given f_ref must be zero, else the row is NaN.
+ * wfx_phd_setup replaces phenomd core.table (core.py:274-377);
+ * wfx_phd_eval and wfx_phd_eval_ragged replace core.frequency_powers +
+ * core.evaluate under jax.vmap (core.py:395-485), with the launch shapes of
+ * wfx_xas_eval and wfx_xas_eval_ragged (tiles from wfx_xas_tile_prefix);
+ * every frequency is evaluated (no cutoff). wfx_phd_reference_phase:
+ * phase[b] = -Psi + t0 M f at M f = mf[b] but for its constant, for the
+ * reference-convention converters (phenomd/reference.py:18-26). */
+int wfx_phd_setup(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
+                  double amp_scale, double reference_mf, int precision, void* records, uint32_t* flags,
+                  void* stream);
+int wfx_phd_reference_phase(const double* params, int64_t n_binaries, const double* mf, double* phase,
+                            void* stream);
+int wfx_phd_eval(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
+                 int precision, int group, int freqs_per_thread, void* hplus, void* hcross, void* stream);
+int wfx_phd_eval_ragged(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64,
+                        const double* grid_f0, const double* grid_df, const int64_t* offsets, const int64_t* tiles,
+                        int precision, int bins_per_thread, int blocks, void* hplus, void* hcross,
+                        void* stream);
 
 #ifdef __cplusplus
 }

@@ -134,8 +134,9 @@ def test_frequency_grid():
 
 
 def test_registry():
-    assert wx.approximants() == ["IMRPhenomXAS"]
+    assert wx.approximants() == ["IMRPhenomD", "IMRPhenomXAS"]
     assert _api.model("IMRPhenomXAS") == "IMRPhenomXAS"
+    assert _api.model("IMRPhenomD") == "IMRPhenomD"
     with pytest.raises(wx.UnknownApproximantError):
         _api.model("IMRPhenomQQ")
     with pytest.raises(wx.UnsupportedError):

@@ -1,4 +1,5 @@
-"""waveformx_amd: IMRPhenomXAS frequency-domain polarizations on MI355X.
+"""waveformx_amd: IMRPhenomXAS (and IMRPhenomD) frequency-domain
+polarizations on MI355X.
 
 A drop-in for the reference's PyTorch path (``waveformx.torch.
 polarizations_function``, ``waveformx.batch_polarizations``): the same

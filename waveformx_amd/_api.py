@@ -3,8 +3,8 @@
 (``src/waveformx/_api.py:307-360``) and :func:`ragged_polarizations` (each
 binary on its own grid, which the reference does not have).
 
-IMRPhenomXAS is the one approximant built here; the reference's other
-names raise :class:`UnsupportedError`, unknown names
+IMRPhenomXAS and IMRPhenomD are built here; the reference's other names
+raise :class:`UnsupportedError`, unknown names
 :class:`UnknownApproximantError` (as ``models.get`` does).
 """
 
@@ -15,7 +15,7 @@ from typing import Any, Optional, Sequence, Tuple
 import numpy as np
 import torch
 
-from . import xas
+from . import phenomd, xas
 from ._types import (BatchPolarizations, FrequencyGrid, WaveformOptions, check_table, parameter_table,
                      parse_options, strain_scale)
 from .exceptions import InvalidParameterError, ModelError, UnknownApproximantError, UnsupportedError
@@ -25,7 +25,7 @@ __all__ = ["APPROXIMANTS", "approximants", "batch_polarizations", "ragged_polari
 #: approximant -> (domain, precessing, implemented here)
 APPROXIMANTS = {
     "TaylorF2": ("frequency", False, False),
-    "IMRPhenomD": ("frequency", False, False),
+    "IMRPhenomD": ("frequency", False, True),
     "IMRPhenomXAS": ("frequency", False, True),
     "IMRPhenomXHM": ("frequency", False, False),
     "IMRPhenomXPNR": ("frequency", True, False),
@@ -52,8 +52,16 @@ def model(approximant: str, domain: str = "frequency") -> str:
         what = "frequency-domain" if domain == "frequency" else "time-domain"
         raise UnsupportedError(f"{approximant} has no {what} waveforms")
     if not built:
-        raise UnsupportedError(f"{approximant} is not implemented in waveformx_amd (IMRPhenomXAS is)")
+        raise UnsupportedError(f"{approximant} is not implemented in waveformx_amd (IMRPhenomXAS and IMRPhenomD "
+                               "are)")
     return approximant
+
+
+def _module(approximant: str):
+    """The device module of a built approximant and its flag messages."""
+    if approximant == "IMRPhenomD":
+        return phenomd, phenomd.LAL_FLAGS
+    return xas, xas.LAL_FLAGS
 
 
 def compute_device(device: Optional[torch.device] = None) -> torch.device:
@@ -111,14 +119,15 @@ def batch_polarizations(approximant: str, grid: Any, parameters: Any, *, threads
     params = _table_params(table, dev)
     step = n if chunk_size is None else max(1, int(chunk_size))
     flags = torch.zeros((n,), dtype=torch.int32, device=dev)
+    module, lal_flags = _module(approximant)
     if nf:
         for start in range(0, n, step):
             stop = min(n, start + step)
-            records, flags[start:stop] = xas.setup(params[start:stop], freqs[:1], strain_scale(options),
-                                                   options.reference_mf, "fp64")
-            xas.evaluate(records, freqs, "fp64", out=(hplus[start:stop], hcross[start:stop]))
+            records, flags[start:stop] = module.setup(params[start:stop], freqs[:1], strain_scale(options),
+                                                      options.reference_mf, "fp64")
+            module.evaluate(records, freqs, "fp64", out=(hplus[start:stop], hcross[start:stop]))
     flags_host = flags.cpu().numpy()
-    for bit, message in xas.LAL_FLAGS:
+    for bit, message in lal_flags:
         for index in np.flatnonzero(flags_host & bit):
             errors.setdefault(int(index), InvalidParameterError(message))
     ok = np.ones(n, dtype=bool)
@@ -152,6 +161,7 @@ def ragged_polarizations(approximant: str, grids: Sequence[Any], parameters: Any
     if len(grids) != len(table):
         raise ValueError("give one grid per binary")
     params = _table_params(table, dev)
+    module, _ = _module(approximant)
     descriptors = all(isinstance(g, tuple) and len(g) == 3 for g in grids)
     if descriptors:
         f0 = np.array([float(g[0]) for g in grids])
@@ -162,9 +172,9 @@ def ragged_polarizations(approximant: str, grids: Sequence[Any], parameters: Any
         f0_t = torch.as_tensor(f0, device=dev)
         df_t = torch.as_tensor(df, device=dev)
         f_min = f0_t if precision in ("fp64", "split", "split-df") else f0_t.to(torch.float32).to(torch.float64)
-        records, _ = xas.setup(params, f_min, strain_scale(options), options.reference_mf, precision)
-        hp, hc = xas.evaluate_ragged(records, off, grid_f0=f0_t, grid_df=df_t, precision=precision,
-                                     total=int(offsets[-1]))
+        records, _ = module.setup(params, f_min, strain_scale(options), options.reference_mf, precision)
+        hp, hc = module.evaluate_ragged(records, off, grid_f0=f0_t, grid_df=df_t, precision=precision,
+                                        total=int(offsets[-1]))
         return hp, hc, off
     arrays = [np.asarray(g, dtype=float) for g in grids]
     offsets = np.concatenate([[0], np.cumsum([len(a) for a in arrays])]).astype(np.int64)
@@ -174,6 +184,6 @@ def ragged_polarizations(approximant: str, grids: Sequence[Any], parameters: Any
     starts = torch.as_tensor(offsets[:-1], device=dev).clamp(max=max(freqs.shape[0] - 1, 0))
     f_min = freqs[starts].to(torch.float64) if freqs.shape[0] else torch.zeros(len(table), device=dev,
                                                                                  dtype=torch.float64)
-    records, _ = xas.setup(params, f_min, strain_scale(options), options.reference_mf, precision)
-    hp, hc = xas.evaluate_ragged(records, off, freqs=freqs, precision=precision)
+    records, _ = module.setup(params, f_min, strain_scale(options), options.reference_mf, precision)
+    hp, hc = module.evaluate_ragged(records, off, freqs=freqs, precision=precision)
     return hp, hc, off

@@ -13,6 +13,7 @@
 //    are remapped so that one group's frequency tiles share an XCD's L2;
 //  * ragged: a persistent grid walks (binary, tile) work items from a
 //    prefix sum of tiles per binary; one thread per bin.
+#include "phenomd_point.h"
 #include "xas_adjoint.h"
 
 #include <atomic>
@@ -96,13 +97,15 @@ __global__ __launch_bounds__(64) void setup_f64_kernel(const double* __restrict_
 }
 
 // FP32 record from FP64 slots: every slot rounded to FP32 but the
-// post-inspiral constants, which take -A modulo 2 pi first (fold_anchor);
-// T0 and A also as hi + lo and as doubles in slots 60..63 (low word first)
-__device__ inline void write_f32_record(const double* rec, float* __restrict__ out) {
+// post-inspiral constants (slots const_a, const_b: IMRPhenomXAS's by
+// default), which take -A modulo 2 pi first (fold_anchor); T0 and A also
+// as hi + lo and as doubles in slots 60..63 (low word first)
+__device__ inline void write_f32_record(const double* rec, float* __restrict__ out, int const_a = kIntConst,
+                                        int const_b = kRdConst) {
     for (int k = 0; k < 60; ++k) out[k] = (float)rec[k];
     const double t0 = rec[kT0], anchor = rec[kAnchor];
-    out[kIntConst] = (float)fold_anchor(rec[kIntConst], anchor);
-    out[kRdConst] = (float)fold_anchor(rec[kRdConst], anchor);
+    out[const_a] = (float)fold_anchor(rec[const_a], anchor);
+    out[const_b] = (float)fold_anchor(rec[const_b], anchor);
     out[kT0] = (float)t0;
     out[kT0Lo] = (float)(t0 - (double)out[kT0]);
     out[kAnchor] = (float)anchor;
@@ -193,6 +196,39 @@ __global__ __launch_bounds__(64) void reference_phase_kernel(const double* __res
     phase[b] = lal_phase_22<double>(p, mf[b]);
 }
 
+// IMRPhenomD: the FP64 setup (phenomd_setup.h), written as the FP64 record
+// or as the FP32 split record (the anchor folded into the intermediate and
+// merger-ringdown constants); the table is FP64 whatever the evaluation's
+// arithmetic, as the reference's (phenomd/model.py:33-50)
+__global__ __launch_bounds__(64) void setup_pd_kernel(const double* __restrict__ params, int64_t nb,
+                                                      const double* __restrict__ f_min, int64_t f_min_stride,
+                                                      double amp_scale, double reference_mf,
+                                                      double* __restrict__ records64, float* __restrict__ records32,
+                                                      uint32_t* __restrict__ flags) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double p[kParams];
+    for (int k = 0; k < kParams; ++k) p[k] = params[b * kParams + k];
+    double rec[kSlots];
+    const unsigned fl = pd_record<double>(p, f_min[b * f_min_stride], amp_scale, reference_mf, rec);
+    rec[kT0Lo] = 0.0;
+    rec[kAnchorLo] = 0.0;
+    if (records64)
+        for (int k = 0; k < kSlots; ++k) records64[b * kSlots + k] = rec[k];
+    if (records32) write_f32_record(rec, records32 + b * kSlots, kPdIntConst, kPdMrdConst);
+    if (flags) flags[b] = fl;
+}
+
+__global__ __launch_bounds__(64) void reference_phase_pd_kernel(const double* __restrict__ params, int64_t nb,
+                                                                const double* __restrict__ mf,
+                                                                double* __restrict__ phase) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double p[kParams];
+    for (int k = 0; k < kParams; ++k) p[k] = params[b * kParams + k];
+    phase[b] = pd_reference_phase<double>(p, mf[b]);
+}
+
 // ------------------------------------------------------------ evaluation
 
 constexpr int kBlock = 256;
@@ -202,8 +238,39 @@ __device__ inline double load_freq(const F* freqs, int64_t i) {
     return (double)freqs[i];
 }
 
+// The models that share the evaluation kernels: a frequency's powers and
+// the per-bin formula on a record, in FP64 and in FP32 (Lead: how the
+// inspiral's leading term is carried).
+struct XasModel {
+    using Pow64 = FreqPowers<double>;
+    using Pow32 = PowF32;
+    __device__ static Pow64 powers64(double f) { return freq_powers<double>(f); }
+    __device__ static void point64(const double* c, const Pow64& p, double& a, double& b, double& x, double& y) {
+        point_f64(c, p, a, b, x, y);
+    }
+    __device__ static Pow32 powers32(float f) { return pow_f32(f); }
+    template <int L>
+    __device__ static void point32(const float* c, const Pow32& p, float2& hp, float2& hc) {
+        point_f32<L>(c, p, hp, hc);
+    }
+};
+
+struct PhenomDModel {
+    using Pow64 = PdPowers<double>;
+    using Pow32 = PdPowF32;
+    __device__ static Pow64 powers64(double f) { return pd_powers<double>(f); }
+    __device__ static void point64(const double* c, const Pow64& p, double& a, double& b, double& x, double& y) {
+        point_pd_f64(c, p, a, b, x, y);
+    }
+    __device__ static Pow32 powers32(float f) { return pd_pow_f32(f); }
+    template <int L>
+    __device__ static void point32(const float* c, const Pow32& p, float2& hp, float2& hc) {
+        point_pd_f32<L>(c, p, hp, hc);
+    }
+};
+
 // shared grid, FP64
-template <class F>
+template <class M, class F>
 __global__ __launch_bounds__(kBlock) void eval_f64_shared(const double* __restrict__ records, int64_t nb,
                                                           const F* __restrict__ freqs, int64_t n, int64_t ntiles,
                                                           int group, double2* __restrict__ hp,
@@ -213,11 +280,11 @@ __global__ __launch_bounds__(kBlock) void eval_f64_shared(const double* __restri
     const int64_t b0 = (w / ntiles) * group;
     const int64_t i = tile * kBlock + threadIdx.x;
     const bool valid = i < n;
-    const FreqPowers<double> fp = freq_powers<double>(load_freq(freqs, valid ? i : n - 1));
+    const typename M::Pow64 fp = M::powers64(load_freq(freqs, valid ? i : n - 1));
     const int64_t b1 = b0 + group < nb ? b0 + group : nb;
     for (int64_t b = b0; b < b1; ++b) {
         double a, bb, c, d;
-        point_f64(records + b * kSlots, fp, a, bb, c, d);
+        M::point64(records + b * kSlots, fp, a, bb, c, d);
         if (valid) {
             hp[b * n + i] = make_double2(a, bb);
             hc[b * n + i] = make_double2(c, d);
@@ -235,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void eval_f64_shared(const double* __restri
 // output streams best when the blocks running at one time on an XCD write a
 // narrow window of it; a block of G binaries writes G rows 32 KB apart).
 // 8 binaries x 4 frequencies per thread balances them.
-template <int L, int FPT, class F>
+template <class M, int L, int FPT, class F>
 __global__ __launch_bounds__(kBlock) void eval_f32_shared(const float* __restrict__ records, int64_t nb,
                                                           const F* __restrict__ freqs, int64_t n, int64_t ntiles,
                                                           int group, float2* __restrict__ hp,
@@ -244,11 +311,11 @@ __global__ __launch_bounds__(kBlock) void eval_f32_shared(const float* __restric
     const int64_t tile = w % ntiles;
     const int64_t b0 = (w / ntiles) * group;
     const int64_t i0 = tile * (kBlock * FPT) + threadIdx.x;
-    PowF32 p[FPT];
+    typename M::Pow32 p[FPT];
 #pragma unroll
     for (int j = 0; j < FPT; ++j) {
         const int64_t i = i0 + j * kBlock;
-        p[j] = pow_f32((float)freqs[i < n ? i : n - 1]);
+        p[j] = M::powers32((float)freqs[i < n ? i : n - 1]);
     }
     const int64_t b1 = b0 + group < nb ? b0 + group : nb;
     for (int64_t b = b0; b < b1; ++b) {
@@ -260,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void eval_f32_shared(const float* __restric
         for (int j = 0; j < FPT; ++j) {
             const int64_t i = i0 + j * kBlock;
             float2 a, x;
-            point_f32<L>(c, p[j], a, x);
+            M::template point32<L>(c, p[j], a, x);
             if (i < n) {
                 hp[b * n + i] = a;
                 hc[b * n + i] = x;
@@ -291,7 +358,7 @@ __device__ inline double ragged_freq(const F* freqs, const double* f0, const dou
 
 // a work item is a tile of kBlock * BPT consecutive bins of one binary;
 // thread t takes bins t, t + kBlock, ... of the tile
-template <int BPT, class F>
+template <class M, int BPT, class F>
 __global__ __launch_bounds__(kBlock) void eval_f64_ragged(const double* __restrict__ records, int64_t nb,
                                                           const F* __restrict__ freqs, const double* __restrict__ f0,
                                                           const double* __restrict__ df,
@@ -307,16 +374,16 @@ __global__ __launch_bounds__(kBlock) void eval_f64_ragged(const double* __restri
             const int64_t k = (w - tiles[b]) * (kBlock * BPT) + j * kBlock + threadIdx.x;
             const int64_t i = start + k;
             if (i >= stop) break;
-            const FreqPowers<double> fp = freq_powers<double>(ragged_freq(freqs, f0, df, b, i, k));
+            const typename M::Pow64 fp = M::powers64(ragged_freq(freqs, f0, df, b, i, k));
             double a, bb, c, d;
-            point_f64(records + b * kSlots, fp, a, bb, c, d);
+            M::point64(records + b * kSlots, fp, a, bb, c, d);
             hp[i] = make_double2(a, bb);
             hc[i] = make_double2(c, d);
         }
     }
 }
 
-template <int L, int BPT, class F>
+template <class M, int L, int BPT, class F>
 __global__ __launch_bounds__(kBlock) void eval_f32_ragged(const float* __restrict__ records, int64_t nb,
                                                           const F* __restrict__ freqs, const double* __restrict__ f0,
                                                           const double* __restrict__ df,
@@ -332,9 +399,9 @@ __global__ __launch_bounds__(kBlock) void eval_f32_ragged(const float* __restric
             const int64_t k = (w - tiles[b]) * (kBlock * BPT) + j * kBlock + threadIdx.x;
             const int64_t i = start + k;
             if (i >= stop) break;
-            const PowF32 p = pow_f32((float)ragged_freq(freqs, f0, df, b, i, k));
+            const typename M::Pow32 p = M::powers32((float)ragged_freq(freqs, f0, df, b, i, k));
             float2 a, c;
-            point_f32<L>(records + b * kSlots, p, a, c);
+            M::template point32<L>(records + b * kSlots, p, a, c);
             hp[i] = a;
             hc[i] = c;
         }
@@ -651,25 +718,24 @@ struct SharedArgs {
     hipStream_t stream;
 };
 
-template <int L, int FPT, class F>
+template <class M, int L, int FPT, class F>
 void launch_f32_shared_t(const SharedArgs& a) {
-    eval_f32_shared<L, FPT, F><<<a.blocks, kBlock, 0, a.stream>>>((const float*)a.records, a.nb, (const F*)a.freqs,
-                                                                  a.n, a.ntiles, a.group, (float2*)a.hp,
-                                                                  (float2*)a.hc);
+    eval_f32_shared<M, L, FPT, F><<<a.blocks, kBlock, 0, a.stream>>>(
+        (const float*)a.records, a.nb, (const F*)a.freqs, a.n, a.ntiles, a.group, (float2*)a.hp, (float2*)a.hc);
 }
 
-template <int L>
+template <class M, int L>
 void launch_f32_shared(const SharedArgs& a, int freqs_f64, int fpt) {
     if (freqs_f64) {
-        if (fpt == 1) launch_f32_shared_t<L, 1, double>(a);
-        else if (fpt == 2) launch_f32_shared_t<L, 2, double>(a);
-        else if (fpt == 4) launch_f32_shared_t<L, 4, double>(a);
-        else launch_f32_shared_t<L, 8, double>(a);
+        if (fpt == 1) launch_f32_shared_t<M, L, 1, double>(a);
+        else if (fpt == 2) launch_f32_shared_t<M, L, 2, double>(a);
+        else if (fpt == 4) launch_f32_shared_t<M, L, 4, double>(a);
+        else launch_f32_shared_t<M, L, 8, double>(a);
     } else {
-        if (fpt == 1) launch_f32_shared_t<L, 1, float>(a);
-        else if (fpt == 2) launch_f32_shared_t<L, 2, float>(a);
-        else if (fpt == 4) launch_f32_shared_t<L, 4, float>(a);
-        else launch_f32_shared_t<L, 8, float>(a);
+        if (fpt == 1) launch_f32_shared_t<M, L, 1, float>(a);
+        else if (fpt == 2) launch_f32_shared_t<M, L, 2, float>(a);
+        else if (fpt == 4) launch_f32_shared_t<M, L, 4, float>(a);
+        else launch_f32_shared_t<M, L, 8, float>(a);
     }
 }
 
@@ -687,27 +753,26 @@ struct RaggedArgs {
     hipStream_t stream;
 };
 
-template <int BPT, class F>
+template <class M, int BPT, class F>
 void launch_ragged(const RaggedArgs& a, int precision) {
     const F* freqs = (const F*)a.freqs;
     if (precision == kPrecF64) {
-        eval_f64_ragged<BPT, F><<<a.blocks, kBlock, 0, a.stream>>>((const double*)a.records, a.nb, freqs, a.f0, a.df,
-                                                                    a.offsets, a.tiles, (double2*)a.hp,
-                                                                    (double2*)a.hc);
+        eval_f64_ragged<M, BPT, F><<<a.blocks, kBlock, 0, a.stream>>>(
+            (const double*)a.records, a.nb, freqs, a.f0, a.df, a.offsets, a.tiles, (double2*)a.hp, (double2*)a.hc);
         return;
     }
     const float* rec = (const float*)a.records;
     float2* hp = (float2*)a.hp;
     float2* hc = (float2*)a.hc;
     if (precision == kPrecSplit)
-        eval_f32_ragged<kLeadF64, BPT, F><<<a.blocks, kBlock, 0, a.stream>>>(rec, a.nb, freqs, a.f0, a.df, a.offsets,
-                                                                            a.tiles, hp, hc);
+        eval_f32_ragged<M, kLeadF64, BPT, F><<<a.blocks, kBlock, 0, a.stream>>>(rec, a.nb, freqs, a.f0, a.df,
+                                                                               a.offsets, a.tiles, hp, hc);
     else if (precision == kPrecSplitDF)
-        eval_f32_ragged<kLeadDF, BPT, F><<<a.blocks, kBlock, 0, a.stream>>>(rec, a.nb, freqs, a.f0, a.df, a.offsets,
-                                                                           a.tiles, hp, hc);
+        eval_f32_ragged<M, kLeadDF, BPT, F><<<a.blocks, kBlock, 0, a.stream>>>(rec, a.nb, freqs, a.f0, a.df,
+                                                                              a.offsets, a.tiles, hp, hc);
     else
-        eval_f32_ragged<kLeadF32, BPT, F><<<a.blocks, kBlock, 0, a.stream>>>(rec, a.nb, freqs, a.f0, a.df, a.offsets,
-                                                                            a.tiles, hp, hc);
+        eval_f32_ragged<M, kLeadF32, BPT, F><<<a.blocks, kBlock, 0, a.stream>>>(rec, a.nb, freqs, a.f0, a.df,
+                                                                               a.offsets, a.tiles, hp, hc);
 }
 
 int grid_blocks(int64_t nblocks, const char* what) {
@@ -719,6 +784,83 @@ int grid_blocks(int64_t nblocks, const char* what) {
     return (int)nblocks;
 }
 
+// the shared-grid evaluation of model M behind its C entry point `name`
+template <class M>
+int eval_shared(const char* name, const void* records, int64_t n_binaries, const void* freqs, int freqs_f64,
+                int64_t n_freqs, int precision, int group, int freqs_per_thread, void* hplus, void* hcross,
+                void* stream) {
+    const std::string who(name);
+    if (n_binaries < 0 || n_freqs < 0) return fail((who + ": negative size").c_str());
+    if (n_binaries == 0 || n_freqs == 0) return 0;
+    if (!records || !freqs || !hplus || !hcross) return fail((who + ": null pointer").c_str());
+    if (precision < kPrecF64 || precision > kPrecSplitDF) return fail((who + ": unknown precision").c_str());
+    if (freqs_per_thread != 0 && freqs_per_thread != 1 && freqs_per_thread != 2 && freqs_per_thread != 4 &&
+        freqs_per_thread != 8)
+        return fail((who + ": freqs_per_thread must be 1, 2, 4 or 8 (0: default)").c_str());
+    if (precision == kPrecF64) freqs_per_thread = 1;
+    if (freqs_per_thread == 0) freqs_per_thread = kDefaultFreqsPerThread;
+    if (group <= 0) group = kDefaultGroup;
+    const int64_t ntiles = (n_freqs + kBlock * freqs_per_thread - 1) / (kBlock * freqs_per_thread);
+    const int64_t ngroups = (n_binaries + group - 1) / group;
+    const int blocks = grid_blocks(ntiles * ngroups, (who + ": grid too large, raise the group size").c_str());
+    if (blocks < 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    const SharedArgs a{records, n_binaries, freqs, n_freqs, ntiles, group, hplus, hcross, blocks, s};
+    switch (precision) {
+        case kPrecF64:
+            if (freqs_f64)
+                eval_f64_shared<M, double><<<blocks, kBlock, 0, s>>>((const double*)records, n_binaries,
+                                                                     (const double*)freqs, n_freqs, ntiles, group,
+                                                                     (double2*)hplus, (double2*)hcross);
+            else
+                eval_f64_shared<M, float><<<blocks, kBlock, 0, s>>>((const double*)records, n_binaries,
+                                                                    (const float*)freqs, n_freqs, ntiles, group,
+                                                                    (double2*)hplus, (double2*)hcross);
+            break;
+        case kPrecSplit:
+            launch_f32_shared<M, kLeadF64>(a, freqs_f64, freqs_per_thread);
+            break;
+        case kPrecSplitDF:
+            launch_f32_shared<M, kLeadDF>(a, freqs_f64, freqs_per_thread);
+            break;
+        default:
+            launch_f32_shared<M, kLeadF32>(a, freqs_f64, freqs_per_thread);
+            break;
+    }
+    return check_launch(name);
+}
+
+// the ragged-grid evaluation of model M behind its C entry point `name`
+template <class M>
+int eval_ragged(const char* name, const void* records, int64_t n_binaries, const void* freqs, int freqs_f64,
+                const double* grid_f0, const double* grid_df, const int64_t* offsets, const int64_t* tiles,
+                int precision, int bins_per_thread, int blocks, void* hplus, void* hcross, void* stream) {
+    const std::string who(name);
+    if (n_binaries < 0) return fail((who + ": negative size").c_str());
+    if (n_binaries == 0) return 0;
+    if (!records || !offsets || !tiles || !hplus || !hcross) return fail((who + ": null pointer").c_str());
+    if (!freqs && (!grid_f0 || !grid_df)) return fail((who + ": give freqs or (grid_f0, grid_df)").c_str());
+    if (bins_per_thread != 1 && bins_per_thread != 2 && bins_per_thread != 4)
+        return fail((who + ": bins_per_thread must be 1, 2 or 4").c_str());
+    if (precision < kPrecF64 || precision > kPrecSplitDF) return fail((who + ": unknown precision").c_str());
+    if (blocks <= 0) blocks = 8 * cu_count();
+    const RaggedArgs a{records, n_binaries, freqs, grid_f0, grid_df, offsets, tiles, hplus, hcross, blocks,
+                       (hipStream_t)stream};
+    const bool f64 = freqs_f64 || !freqs;
+    switch (bins_per_thread) {
+        case 1:
+            f64 ? launch_ragged<M, 1, double>(a, precision) : launch_ragged<M, 1, float>(a, precision);
+            break;
+        case 2:
+            f64 ? launch_ragged<M, 2, double>(a, precision) : launch_ragged<M, 2, float>(a, precision);
+            break;
+        default:
+            f64 ? launch_ragged<M, 4, double>(a, precision) : launch_ragged<M, 4, float>(a, precision);
+            break;
+    }
+    return check_launch(name);
+}
+
 }  // namespace
 
 // ================================================================= C ABI
@@ -728,7 +870,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 8; }
+int wfx_abi_version(void) { return 9; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -796,46 +938,8 @@ int wfx_xas_reference_phase(const double* params, int64_t n_binaries, const doub
 
 int wfx_xas_eval(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
                  int precision, int group, int freqs_per_thread, void* hplus, void* hcross, void* stream) {
-    if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xas_eval: negative size");
-    if (n_binaries == 0 || n_freqs == 0) return 0;
-    if (!records || !freqs || !hplus || !hcross) return fail("wfx_xas_eval: null pointer");
-    if (precision < kPrecF64 || precision > kPrecSplitDF) return fail("wfx_xas_eval: unknown precision");
-    if (freqs_per_thread != 0 && freqs_per_thread != 1 && freqs_per_thread != 2 && freqs_per_thread != 4 &&
-        freqs_per_thread != 8)
-        return fail("wfx_xas_eval: freqs_per_thread must be 1, 2, 4 or 8 (0: default)");
-    if (precision == kPrecF64) freqs_per_thread = 1;
-    if (freqs_per_thread == 0) freqs_per_thread = kDefaultFreqsPerThread;
-    if (group <= 0) group = kDefaultGroup;
-    const int64_t ntiles = (n_freqs + kBlock * freqs_per_thread - 1) / (kBlock * freqs_per_thread);
-    const int64_t ngroups = (n_binaries + group - 1) / group;
-    const int blocks = grid_blocks(ntiles * ngroups, "wfx_xas_eval: grid too large, raise the group size");
-    if (blocks < 0) return -1;
-    hipStream_t s = (hipStream_t)stream;
-    const SharedArgs a{records, n_binaries, freqs, n_freqs, ntiles, group, hplus, hcross, blocks, s};
-    switch (precision) {
-        case kPrecF64:
-            if (freqs_f64)
-                eval_f64_shared<double><<<blocks, kBlock, 0, s>>>((const double*)records, n_binaries,
-                                                                  (const double*)freqs, n_freqs, ntiles, group,
-                                                                  (double2*)hplus, (double2*)hcross);
-            else
-                eval_f64_shared<float><<<blocks, kBlock, 0, s>>>((const double*)records, n_binaries,
-                                                                 (const float*)freqs, n_freqs, ntiles, group,
-                                                                 (double2*)hplus, (double2*)hcross);
-            break;
-        case kPrecSplit:
-            launch_f32_shared<kLeadF64>(a, freqs_f64, freqs_per_thread);
-            break;
-        case kPrecSplitDF:
-            launch_f32_shared<kLeadDF>(a, freqs_f64, freqs_per_thread);
-            break;
-        case kPrecF32:
-            launch_f32_shared<kLeadF32>(a, freqs_f64, freqs_per_thread);
-            break;
-        default:
-            return fail("wfx_xas_eval: unknown precision");
-    }
-    return check_launch("wfx_xas_eval");
+    return eval_shared<XasModel>("wfx_xas_eval", records, n_binaries, freqs, freqs_f64, n_freqs, precision, group,
+                                 freqs_per_thread, hplus, hcross, stream);
 }
 
 int wfx_xas_tile_prefix(const int64_t* offsets, int64_t n_binaries, int bins_per_thread, int64_t* tiles,
@@ -852,29 +956,54 @@ int wfx_xas_tile_prefix(const int64_t* offsets, int64_t n_binaries, int bins_per
 int wfx_xas_eval_ragged(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64,
                         const double* grid_f0, const double* grid_df, const int64_t* offsets, const int64_t* tiles,
                         int precision, int bins_per_thread, int blocks, void* hplus, void* hcross, void* stream) {
-    if (n_binaries < 0) return fail("wfx_xas_eval_ragged: negative size");
+    return eval_ragged<XasModel>("wfx_xas_eval_ragged", records, n_binaries, freqs, freqs_f64, grid_f0, grid_df,
+                                 offsets, tiles, precision, bins_per_thread, blocks, hplus, hcross, stream);
+}
+
+// ------------------------------------------------------------- IMRPhenomD
+
+int wfx_phd_setup(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
+                  double amp_scale, double reference_mf, int precision, void* records, uint32_t* flags,
+                  void* stream) {
+    if (n_binaries < 0) return fail("wfx_phd_setup: negative number of binaries");
     if (n_binaries == 0) return 0;
-    if (!records || !offsets || !tiles || !hplus || !hcross) return fail("wfx_xas_eval_ragged: null pointer");
-    if (!freqs && (!grid_f0 || !grid_df)) return fail("wfx_xas_eval_ragged: give freqs or (grid_f0, grid_df)");
-    if (bins_per_thread != 1 && bins_per_thread != 2 && bins_per_thread != 4)
-        return fail("wfx_xas_eval_ragged: bins_per_thread must be 1, 2 or 4");
-    if (precision < kPrecF64 || precision > kPrecSplitDF) return fail("wfx_xas_eval_ragged: unknown precision");
-    if (blocks <= 0) blocks = 8 * cu_count();
-    const RaggedArgs a{records, n_binaries, freqs, grid_f0, grid_df, offsets, tiles, hplus, hcross, blocks,
-                       (hipStream_t)stream};
-    const bool f64 = freqs_f64 || !freqs;
-    switch (bins_per_thread) {
-        case 1:
-            f64 ? launch_ragged<1, double>(a, precision) : launch_ragged<1, float>(a, precision);
-            break;
-        case 2:
-            f64 ? launch_ragged<2, double>(a, precision) : launch_ragged<2, float>(a, precision);
-            break;
-        default:
-            f64 ? launch_ragged<4, double>(a, precision) : launch_ragged<4, float>(a, precision);
-            break;
-    }
-    return check_launch("wfx_xas_eval_ragged");
+    if (!params || !f_min || !records) return fail("wfx_phd_setup: null pointer");
+    if (f_min_stride != 0 && f_min_stride != 1) return fail("wfx_phd_setup: f_min_stride must be 0 or 1");
+    if (precision != kPrecF64 && precision != kPrecSplit && precision != kPrecSplitDF)
+        return fail("wfx_phd_setup: precision must be 0, 2 or 3 (the table of IMRPhenomD is FP64)");
+    const int blocks = grid_blocks((n_binaries + 63) / 64, "wfx_phd_setup: too many binaries");
+    if (blocks < 0) return -1;
+    const bool f64 = precision == kPrecF64;
+    setup_pd_kernel<<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, f_min, f_min_stride, amp_scale,
+                                                            reference_mf, f64 ? (double*)records : nullptr,
+                                                            f64 ? nullptr : (float*)records, flags);
+    return check_launch("wfx_phd_setup");
+}
+
+int wfx_phd_reference_phase(const double* params, int64_t n_binaries, const double* mf, double* phase,
+                            void* stream) {
+    if (n_binaries < 0) return fail("wfx_phd_reference_phase: negative size");
+    if (n_binaries == 0) return 0;
+    if (!params || !mf || !phase) return fail("wfx_phd_reference_phase: null pointer");
+    const int blocks = grid_blocks((n_binaries + 63) / 64, "wfx_phd_reference_phase: too many binaries");
+    if (blocks < 0) return -1;
+    reference_phase_pd_kernel<<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, mf, phase);
+    return check_launch("wfx_phd_reference_phase");
+}
+
+int wfx_phd_eval(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
+                 int precision, int group, int freqs_per_thread, void* hplus, void* hcross, void* stream) {
+    if (precision == kPrecF32) return fail("wfx_phd_eval: precision must be 0, 2 or 3 (no FP32 table)");
+    return eval_shared<PhenomDModel>("wfx_phd_eval", records, n_binaries, freqs, freqs_f64, n_freqs, precision, group,
+                                     freqs_per_thread, hplus, hcross, stream);
+}
+
+int wfx_phd_eval_ragged(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64,
+                        const double* grid_f0, const double* grid_df, const int64_t* offsets, const int64_t* tiles,
+                        int precision, int bins_per_thread, int blocks, void* hplus, void* hcross, void* stream) {
+    if (precision == kPrecF32) return fail("wfx_phd_eval_ragged: precision must be 0, 2 or 3 (no FP32 table)");
+    return eval_ragged<PhenomDModel>("wfx_phd_eval_ragged", records, n_binaries, freqs, freqs_f64, grid_f0, grid_df,
+                                     offsets, tiles, precision, bins_per_thread, blocks, hplus, hcross, stream);
 }
 
 int wfx_xas_power_table_bytes(int precision) {

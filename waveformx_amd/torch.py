@@ -13,17 +13,19 @@ N)`` on the grid's device, the strain times 1e21 unless
 
 What runs is the HIP path: float64 inputs run the FP64 kernels (complex128
 out), float32 inputs the split-precision FP32 kernels (complex64 out); the
-keyword ``arithmetic`` ("fp64", "split", "split-df", "fp32") overrides the
-choice. Tensors on the CPU are computed on the current HIP device and the
-results copied back.
+keyword ``arithmetic`` ("fp64", "split", "split-df", "fp32"; IMRPhenomD has
+no "fp32": its table is FP64) overrides the choice. Tensors on the CPU are
+computed on the current HIP device and the results copied back.
+Approximants: IMRPhenomXAS and IMRPhenomD.
 
-The backward pass (``torch.autograd.Function``, once differentiable, the
-cotangents conjugated as in ``torch.py:108-119``) runs the hand-written
-adjoint kernel: it recomputes the formula per bin, reduces the cotangents
-into derivatives of each binary's record, and contracts them with the
-record's forward-mode derivatives from the FP64 setup. Gradients reach
-every field (the in-plane spins get zero) and the grid. As in the
-reference, the function is opaque to ``torch.compile`` and ``torch.func``.
+The backward pass of IMRPhenomXAS (``torch.autograd.Function``, once
+differentiable, the cotangents conjugated as in ``torch.py:108-119``) runs
+the hand-written adjoint kernel: it recomputes the formula per bin, reduces
+the cotangents into derivatives of each binary's record, and contracts them
+with the record's forward-mode derivatives from the FP64 setup. Gradients
+reach every field (the in-plane spins get zero) and the grid. IMRPhenomD's
+backward raises ``NotImplementedError``. As in the reference, the function
+is opaque to ``torch.compile`` and ``torch.func``.
 """
 
 from __future__ import annotations
@@ -33,7 +35,7 @@ from typing import Any, Callable, Mapping, Optional
 import numpy as np
 import torch
 
-from . import _api, xas
+from . import _api, phenomd, xas
 from ._types import BINARY_NAMES, Binary, WaveformOptions, parse_options, strain_scale
 
 __all__ = ["polarizations_function"]
@@ -71,24 +73,26 @@ def _as_tensor(value: Any, dtype: torch.dtype, device: torch.device) -> torch.Te
     return torch.as_tensor(np.asarray(value, dtype=np.float64), dtype=dtype, device=device)
 
 
-def _arithmetic(dtype: torch.dtype, arithmetic: Optional[str]) -> str:
+def _arithmetic(approximant: str, dtype: torch.dtype, arithmetic: Optional[str]) -> str:
+    allowed = phenomd.PRECISIONS if approximant == "IMRPhenomD" else tuple(xas.PRECISIONS)
     if arithmetic is None:
         return "fp64" if dtype == torch.float64 else "split"
-    if arithmetic not in xas.PRECISIONS:
-        raise ValueError(f"arithmetic must be one of {sorted(xas.PRECISIONS)}")
+    if arithmetic not in allowed:
+        raise ValueError(f"arithmetic of {approximant} must be one of {sorted(allowed)}")
     return arithmetic
 
 
-class _XASFunction(torch.autograd.Function):
+class _ModelFunction(torch.autograd.Function):
     """(hplus, hcross) of the grid and the binary's fields (B,)-shaped, with
-    the adjoint kernel as the backward pass."""
+    the adjoint kernel as the backward pass (IMRPhenomXAS)."""
 
     @staticmethod
     def forward(ctx, key, grid, mass_1, mass_2, spin_1, spin_2, distance, inclination, phi_ref, f_ref):
-        amp_scale, reference_mf, arithmetic = key
+        approximant, amp_scale, reference_mf, arithmetic = key
         params = xas.pack_params(mass_1, mass_2, spin_1[:, 2], spin_2[:, 2], distance, inclination, phi_ref, f_ref)
         freqs = grid.contiguous()
-        hp, hc = xas.polarizations(params, freqs, amp_scale, reference_mf, arithmetic)
+        module = phenomd if approximant == "IMRPhenomD" else xas
+        hp, hc = module.polarizations(params, freqs, amp_scale, reference_mf, arithmetic)
         ctx.key = key
         ctx.save_for_backward(grid, params)
         return hp, hc
@@ -98,7 +102,9 @@ class _XASFunction(torch.autograd.Function):
     def backward(ctx, grad_hplus, grad_hcross):
         from . import adjoint
 
-        amp_scale, reference_mf, arithmetic = ctx.key
+        approximant, amp_scale, reference_mf, arithmetic = ctx.key
+        if approximant != "IMRPhenomXAS":
+            raise NotImplementedError(f"the gradient of {approximant} is not built (IMRPhenomXAS has one)")
         grid, params = ctx.saved_tensors
         wrt_grid = ctx.needs_input_grad[1]
         grads, grad_grid = adjoint.vjp(params, grid.contiguous(), grad_hplus, grad_hcross, amp_scale, reference_mf,
@@ -134,8 +140,8 @@ def _wrap(approximant: str, options: WaveformOptions, arithmetic: Optional[str])
         batched = bool(sizes)
         n = sizes.pop() if batched else 1
         fields = [torch.broadcast_to(f, (n, 3) if i in (2, 3) else (n,)) for i, f in enumerate(fields)]
-        key = (amp_scale, options.reference_mf, _arithmetic(dtype, arithmetic))
-        hp, hc = _XASFunction.apply(key, grid_t, *fields)
+        key = (approximant, amp_scale, options.reference_mf, _arithmetic(approximant, dtype, arithmetic))
+        hp, hc = _ModelFunction.apply(key, grid_t, *fields)
         if not batched:
             hp, hc = hp[0], hc[0]
         if device != run:
