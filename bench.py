@@ -306,6 +306,17 @@ def main():
     ms_eval = timed(torch, lambda: xas.evaluate(rec["split"], f32, "split", args.group, out=out32), args.steps,
                     args.warmup)
     ms_setup = timed(torch, lambda: xas.setup(params, f32[:1], 1e21, None, "split"), args.steps, args.warmup)
+    # IMRPhenomD on the same kernels, workload and buffers
+    from waveformx_amd import phenomd
+
+    def step_pd(precision, freqs, out):
+        def run():
+            records, _ = phenomd.setup(params, freqs[:1], 1e21, None, precision)
+            phenomd.evaluate(records, freqs, precision, args.group, out=out)
+        return run
+
+    ms_pd_split = timed(torch, step_pd("split", f32, out32), args.steps, args.warmup)
+    ms_pd_fp64 = timed(torch, step_pd("fp64", f64, out64), max(3, args.steps // 2), 1)
 
     # accuracy of the split path against the FP64 path on the device (which
     # matches the FP64 oracle to 1e-12): on the FP32-rounded grid the split
@@ -341,6 +352,8 @@ def main():
         "split_df_value": nb / (ms_df * 1e-3),
         "eval_ms": ms_eval,
         "setup_ms": ms_setup,
+        "phenomd_split_value": nb / (ms_pd_split * 1e-3),
+        "phenomd_fp64_value": nb / (ms_pd_fp64 * 1e-3),
         "max_weighted_mismatch_vs_fp64": rounded["mismatch"],
         "accuracy_vs_fp64_rounded_grid": rounded,
         "accuracy_vs_fp64_exact_grid": exact,

@@ -57,3 +57,33 @@ def test_zero_f_ref_and_batches(cuda):
         Binary(*(torch.as_tensor(np.asarray(v), dtype=torch.float32) for v in
                  Binary(36.0, 29.0, (0, 0, 0.3), (0, 0, -0.4), 400.0, 0.3, 1.2, 25.0))), 0.01)
     assert single32.phi_ref.dtype == torch.float32
+
+
+def test_phenomd_reference_invariance(cuda):
+    """IMRPhenomD (tests/test_phenomd.py:190-206): the converted binary under
+    reference_mf gives the same waveform; f_ref = 0 takes f_min; the phase
+    change is kept modulo 2 pi (phi_ref modulo pi)."""
+    import torch
+
+    import waveformx_amd.torch as wt
+    from test_oracle_phenomd import CASES as PD_CASES
+    from waveformx_amd import Binary, reference
+
+    to_scaled, to_physical = reference.converters("IMRPhenomD")
+    for case in PD_CASES[:7]:
+        m1, m2, c1, c2, distance, inclination, phi, f_min, f_ref = case
+        f = torch.tensor(np.geomspace(f_min, 512.0, 256), device=cuda)
+        binary = Binary(torch.tensor(m1, dtype=torch.float64, device=cuda), m2, (0.0, 0.0, c1), (0.0, 0.0, c2),
+                        distance, inclination, phi, f_ref)
+        hp, hc = wt.polarizations_function("IMRPhenomD")(binary, f)
+        scaled = to_scaled(binary, 0.01, f_min=f_min)
+        assert float(scaled.f_ref) == 0.0
+        hp2, hc2 = wt.polarizations_function("IMRPhenomD", reference_mf=0.01)(scaled, f)
+        for a, b in ((hp2, hp), (hc2, hc)):
+            err, _ = compare(a.cpu().numpy(), b.cpu().numpy())
+            assert err < 1e-10, (case, err)
+        back = to_physical(scaled, f_ref if f_ref > 0 else f_min, 0.01)
+        # phi_ref modulo pi
+        assert np.cos(2.0 * (float(back.phi_ref) - phi)) == pytest.approx(1.0, abs=1e-12)
+    nan = to_scaled(Binary(20.0, 45.0, (0, 0, 0.3), (0, 0, -0.4), 300.0, 2.9, 0.7, 0.0), 0.01)
+    assert np.isnan(float(nan.phi_ref))
