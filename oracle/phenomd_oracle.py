@@ -586,7 +586,7 @@ def reference_phase(m1, m2, chi1, chi2, mf):
 
 
 def draw(n, seed=0):
-    """The BASELINE-style parameter draw of the XAS oracle (m1 ~ U[10, 80],
+    """The benchmark workload's parameter draw, as the XAS oracle's (m1 ~ U[10, 80],
     q ~ U[0.125, 1], chi ~ U[-0.9, 0.9], d_L ~ U[100, 2000], cos i ~ U[-1,
     1], phi_ref ~ U[0, 2 pi], f_ref = 20)."""
     rng = np.random.default_rng(seed)
@@ -595,3 +595,30 @@ def draw(n, seed=0):
     return np.stack([m1, m1 * q, rng.uniform(-0.9, 0.9, n), rng.uniform(-0.9, 0.9, n),
                      rng.uniform(100.0, 2000.0, n), np.arccos(rng.uniform(-1.0, 1.0, n)),
                      rng.uniform(0.0, 2 * np.pi, n), np.full(n, 20.0)], axis=1)
+
+
+def jacobian(params, freqs, amp_scale=1e21, reference_mf=None, step=1e-5):
+    """(dh+/dparams, dhx/dparams) (B, N, 8) complex128 by fourth-order
+    central differences of :func:`polarizations` in each of the 8
+    parameters, step ``step * max(|x|, 1)``: the region bounds move with the
+    parameters, so the differences follow the formula of whichever region a
+    frequency falls in (away from a bound: the value and slope of the phase,
+    core.py:213-217, and of the amplitude, core.py:250-262, are continuous
+    across it). At the default step, the benchmark draw's vector-Jacobian
+    products agree with those at steps 3e-6 and 3e-5 to 5e-9 of their
+    largest value (rounding grows below, truncation above)."""
+    params = np.asarray(params, dtype=np.float64)
+    freqs = np.asarray(freqs, dtype=np.float64)
+    b, n = params.shape[0], freqs.shape[0]
+    dhp = np.zeros((b, n, 8), dtype=np.complex128)
+    dhc = np.zeros_like(dhp)
+    for k in range(8):
+        h = step * np.maximum(np.abs(params[:, k]), 1.0)
+        taps = []
+        for m in (-2, -1, 1, 2):
+            p = params.copy()
+            p[:, k] = p[:, k] + m * h
+            taps.append(polarizations(p, freqs, amp_scale, reference_mf))
+        for j, out in enumerate((dhp, dhc)):
+            out[:, :, k] = (taps[0][j] - 8.0 * taps[1][j] + 8.0 * taps[2][j] - taps[3][j]) / (12.0 * h[:, None])
+    return dhp, dhc

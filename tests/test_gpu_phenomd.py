@@ -181,7 +181,7 @@ def test_public_functions(cuda):
     hp = batch.hplus.cpu().numpy()
     assert np.all(row_error(hp[[0, 1, 2, 4]], op[[0, 1, 2, 4]]) < 1e-12)
     assert np.all(hp[3] == 0)
-    # the torch function: float64 -> FP64, float32 -> split; no gradient
+    # the torch function: float64 -> FP64, float32 -> split; with a gradient
     fn = wx.polarizations_function("IMRPhenomD")
     grid = torch.as_tensor(f, device=cuda)
     binary = wx.Binary(torch.tensor(params[0, 0], device=cuda, requires_grad=True), params[0, 1],
@@ -194,7 +194,7 @@ def test_public_functions(cuda):
     h32, _ = fn(binary32, grid.to(torch.float32))
     assert h32.dtype == torch.complex64
     assert compare(h32.cpu().numpy().astype(np.complex128), op[0])[1] < 1e-6
-    with pytest.raises(NotImplementedError):
-        h64.abs().sum().backward()
+    h64.abs().sum().backward()  # the adjoint (checked in test_gpu_phenomd_adjoint.py)
+    assert binary.mass_1.grad is not None and bool(torch.isfinite(binary.mass_1.grad))
     with pytest.raises(ValueError):
         wx.polarizations_function("IMRPhenomD", arithmetic="fp32")(binary, grid)

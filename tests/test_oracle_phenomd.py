@@ -250,3 +250,30 @@ def test_final_spin_failure_and_mass_scaled_reference():
     hp1, _ = pd.polarizations(p1, f)
     for i in range(4):
         assert compare(hp[i], hp1[i])[0] < 1e-9
+
+
+def test_finite_difference_jacobian():
+    """The oracle's difference Jacobian (the checker of the GPU gradients):
+    closed forms where the parameter enters the polarizations in one place
+    (h ~ 1/d_L; h+ ~ e^{i 2 phi_ref}; h+ ~ (1 + cos^2 i) / 2, hx ~ -cos i),
+    and, for the masses, spins and f_ref, agreement of steps 10x apart
+    (their truncation and rounding errors differ)."""
+    params = pd.draw(6, seed=21)
+    f = np.linspace(20.0, 1024.0, 300)
+    hp, hc = pd.polarizations(params, f)
+    dhp, dhc = pd.jacobian(params, f)
+    assert dhp.shape == (6, 300, 8) and dhc.shape == (6, 300, 8)
+    d, iota = params[:, 4:5], params[:, 5:6]
+    scale_p, scale_c = np.abs(hp).max(axis=1, keepdims=True), np.abs(hc).max(axis=1, keepdims=True)
+    assert np.all(np.abs(dhp[:, :, 4] + hp / d) <= 1e-8 * scale_p / d)
+    assert np.all(np.abs(dhc[:, :, 4] + hc / d) <= 1e-8 * scale_c / d)
+    assert np.all(np.abs(dhp[:, :, 6] - 2j * hp) <= 1e-8 * scale_p)
+    assert np.all(np.abs(dhc[:, :, 6] - 2j * hc) <= 1e-8 * scale_c)
+    kp, kc = 0.5 * (1 + np.cos(iota) ** 2), -np.cos(iota)
+    assert np.allclose(dhp[:, :, 5], hp * (-np.cos(iota) * np.sin(iota)) / kp, rtol=0, atol=1e-8 * scale_p.max())
+    assert np.allclose(dhc[:, :, 5], hc * np.sin(iota) / kc, rtol=0, atol=1e-8 * scale_c.max())
+    coarse = pd.jacobian(params, f, step=1e-4)
+    for k in (0, 1, 2, 3, 7):
+        for fine, wide in ((dhp, coarse[0]), (dhc, coarse[1])):
+            col_scale = np.abs(fine[:, :, k]).max()
+            assert np.abs(fine[:, :, k] - wide[:, :, k]).max() <= 1e-5 * col_scale, k

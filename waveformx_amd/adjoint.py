@@ -1,4 +1,5 @@
-"""Gradients of IMRPhenomXAS on the GPU through the hand-written adjoint.
+"""Gradients of IMRPhenomXAS and IMRPhenomD on the GPU through the
+hand-written adjoints.
 
 :func:`vjp` is the backward pass of the torch function (the reference's
 ``jax.vjp``, ``src/waveformx/torch.py:87-119``): the FP64 setup with
@@ -10,6 +11,11 @@ and contracts with those derivatives. Nothing of the forward pass is stored.
 :func:`loss_and_grad` fuses the forward pass with a noise-weighted
 least-squares loss ``L = sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2)``: it reads
 the data and writes only each binary's loss and gradient.
+
+Both models share the kernels' shape (one wave per binary, the same
+reduce-scatter and contraction); each has its own per-bin adjoint and its
+own entry points. The gradient with respect to the grid (``wrt_grid``) is
+IMRPhenomXAS's only.
 """
 
 from __future__ import annotations
@@ -18,22 +24,37 @@ from typing import Optional, Tuple
 
 import torch
 
-from . import xas
+from . import phenomd, xas
 from ._lib import call
 from .xas import _code, _ptr, _require_device, _stream
 
-__all__ = ["vjp", "adjoint", "loss_and_grad", "PARAMETERS"]
+__all__ = ["vjp", "adjoint", "loss_and_grad", "PARAMETERS", "MODELS"]
 
 PARAMETERS = ("mass_1", "mass_2", "chi_1", "chi_2", "luminosity_distance", "inclination", "phi_ref", "f_ref")
 
+#: approximant -> (derivative setup, adjoint, fused loss adjoint) entry points
+MODELS = {
+    "IMRPhenomXAS": ("wfx_xas_setup_jvp", "wfx_xas_adjoint", "wfx_xas_loss_adjoint"),
+    "IMRPhenomD": ("wfx_phd_setup_jvp", "wfx_phd_adjoint", "wfx_phd_loss_adjoint"),
+}
 
-def _records(params, freqs, amp_scale, reference_mf, precision):
+
+def _entries(approximant: str, precision: str) -> Tuple[str, str, str]:
+    if approximant not in MODELS:
+        raise ValueError(f"no adjoint for {approximant!r}; gradients exist for {sorted(MODELS)}")
+    if approximant == "IMRPhenomD" and precision not in phenomd.PRECISIONS:
+        raise ValueError(f"IMRPhenomD's arithmetic is one of {phenomd.PRECISIONS}, got {precision!r}")
+    return MODELS[approximant]
+
+
+def _records(params, freqs, amp_scale, reference_mf, precision, approximant="IMRPhenomXAS"):
     """The forward precision's records and the FP64 records' derivatives
     (one launch; plain FP32 records come from their own FP32 setup)."""
+    setup_entry = _entries(approximant, precision)[0]
     if precision in ("fp64", "split", "split-df"):
-        records, jac, _ = xas.setup_jvp(params, freqs[:1], amp_scale, reference_mf, precision)
+        records, jac, _ = xas.run_setup_jvp(setup_entry, params, freqs[:1], amp_scale, reference_mf, precision)
         return records, jac
-    _, jac, _ = xas.setup_jvp(params, freqs[:1], amp_scale, reference_mf, "split")
+    _, jac, _ = xas.run_setup_jvp(setup_entry, params, freqs[:1], amp_scale, reference_mf, "split")
     records, _ = xas.setup(params, freqs[:1], amp_scale, reference_mf, precision)
     return records, jac
 
@@ -42,10 +63,12 @@ power_table = xas.power_table
 
 
 def adjoint(records: torch.Tensor, jac: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor,
-            cot_cross: torch.Tensor, precision: str, powers: Optional[torch.Tensor] = None) -> torch.Tensor:
+            cot_cross: torch.Tensor, precision: str, powers: Optional[torch.Tensor] = None,
+            approximant: str = "IMRPhenomXAS") -> torch.Tensor:
     """(B, 8) float64 gradients from the cotangents (B, N) (complex, the
     torch convention dL/dRe + i dL/dIm) on the shared grid ``freqs``
     (``powers``: its :func:`power_table`, made here if not given)."""
+    entry = _entries(approximant, precision)[1]
     for name, t in (("records", records), ("jac", jac), ("freqs", freqs), ("cot_plus", cot_plus),
                     ("cot_cross", cot_cross)):
         _require_device(t, name)
@@ -57,14 +80,15 @@ def adjoint(records: torch.Tensor, jac: torch.Tensor, freqs: torch.Tensor, cot_p
     grads = torch.empty((nb, 8), dtype=torch.float64, device=records.device)
     if powers is None:
         powers = power_table(freqs, precision)
-    call("wfx_xas_adjoint", _ptr(records), _ptr(jac), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
+    call(entry, _ptr(records), _ptr(jac), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
          _code(precision), _ptr(cot_plus), _ptr(cot_cross), _ptr(powers), _ptr(grads), _stream(records.device))
     return grads
 
 
 def grid_adjoint(records: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor,
                  cot_cross: torch.Tensor, precision: str) -> torch.Tensor:
-    """dL/df (N,) float64 of the shared grid from the cotangents (B, N)."""
+    """dL/df (N,) float64 of the shared grid from the cotangents (B, N)
+    (IMRPhenomXAS records)."""
     nb, n = records.shape[0], freqs.shape[0]
     dldf = torch.zeros((n,), dtype=torch.float64, device=records.device)
     call("wfx_xas_grid_adjoint", _ptr(records), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
@@ -74,9 +98,13 @@ def grid_adjoint(records: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Ten
 
 def vjp(params: torch.Tensor, freqs: torch.Tensor, grad_plus: Optional[torch.Tensor],
         grad_cross: Optional[torch.Tensor], amp_scale: float, reference_mf: Optional[float], precision: str,
-        wrt_grid: bool = False) -> Tuple[torch.Tensor, Optional[torch.Tensor]]:
+        wrt_grid: bool = False, approximant: str = "IMRPhenomXAS") -> Tuple[torch.Tensor, Optional[torch.Tensor]]:
     """Gradients (B, 8) with respect to the packed parameters, and (N,)
     with respect to the grid if ``wrt_grid`` (else None)."""
+    _entries(approximant, precision)
+    if wrt_grid and approximant != "IMRPhenomXAS":
+        raise NotImplementedError(f"the gradient with respect to the frequencies is IMRPhenomXAS's only, "
+                                  f"not {approximant!r}'s")
     nb, n = params.shape[0], freqs.shape[0]
     ctype = torch.complex128 if precision == "fp64" else torch.complex64
     zeros = None
@@ -84,19 +112,21 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, grad_plus: Optional[torch.Ten
         zeros = torch.zeros((nb, n), dtype=ctype, device=params.device)
     gp = zeros if grad_plus is None else grad_plus.to(ctype).reshape(nb, n).contiguous()
     gc = zeros if grad_cross is None else grad_cross.to(ctype).reshape(nb, n).contiguous()
-    records, jac = _records(params, freqs, amp_scale, reference_mf, precision)
+    records, jac = _records(params, freqs, amp_scale, reference_mf, precision, approximant)
     grid = grid_adjoint(records, freqs, gp, gc, precision) if wrt_grid else None
-    return adjoint(records, jac, freqs, gp, gc, precision), grid
+    return adjoint(records, jac, freqs, gp, gc, precision, approximant=approximant), grid
 
 
 def loss_and_grad(params: torch.Tensor, freqs: torch.Tensor, data_plus: torch.Tensor, data_cross: torch.Tensor,
                   weights: torch.Tensor, amp_scale: float = 1e21, reference_mf: Optional[float] = None,
                   precision: str = "split", records: Optional[Tuple[torch.Tensor, torch.Tensor]] = None,
-                  powers: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+                  powers: Optional[torch.Tensor] = None,
+                  approximant: str = "IMRPhenomXAS") -> Tuple[torch.Tensor, torch.Tensor]:
     """Per-binary loss (B,) and gradient (B, 8), float64, of
     ``sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2)`` in one fused pass
     (``records``: (records, jacobian) if already set up; ``powers``: the
     grid's :func:`power_table`, made here if not given)."""
+    entry = _entries(approximant, precision)[2]
     for name, t in (("params", params), ("freqs", freqs), ("data_plus", data_plus), ("data_cross", data_cross),
                     ("weights", weights)):
         _require_device(t, name)
@@ -108,12 +138,13 @@ def loss_and_grad(params: torch.Tensor, freqs: torch.Tensor, data_plus: torch.Te
         raise ValueError(f"data must be ({nb}, {n}) {ctype}")
     if weights.shape != (n,) or weights.dtype != rdtype:
         raise ValueError(f"weights must be ({n},) {rdtype}")
-    rec, jac = records if records is not None else _records(params, freqs, amp_scale, reference_mf, precision)
+    rec, jac = records if records is not None else _records(params, freqs, amp_scale, reference_mf, precision,
+                                                            approximant)
     loss = torch.empty((nb,), dtype=torch.float64, device=params.device)
     grads = torch.empty((nb, 8), dtype=torch.float64, device=params.device)
     if powers is None:
         powers = power_table(freqs, precision)
-    call("wfx_xas_loss_adjoint", _ptr(rec), _ptr(jac), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
+    call(entry, _ptr(rec), _ptr(jac), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
          _code(precision), _ptr(data_plus), _ptr(data_cross), _ptr(weights), _ptr(powers), _ptr(loss), _ptr(grads),
          _stream(params.device))
     return loss, grads

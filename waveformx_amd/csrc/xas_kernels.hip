@@ -14,6 +14,7 @@
 //  * ragged: a persistent grid walks (binary, tile) work items from a
 //    prefix sum of tiles per binary; one thread per bin.
 #include "phenomd_point.h"
+#include "phenomd_adjoint.h"
 #include "xas_adjoint.h"
 
 #include <atomic>
@@ -151,10 +152,31 @@ __global__ __launch_bounds__(64) void setup_f32_kernel(const double* __restrict_
     if (flags) flags[b] = fl;
 }
 
+// The models' per-binary setups, for the derivative setup: the record of a
+// binary in any scalar (with reference_mf applied as the model does) and
+// the slots of the FP32 record that take the anchor (write_f32_record).
+struct XasRecord {
+    static constexpr int kFoldA = kIntConst, kFoldB = kRdConst;
+    template <class R>
+    __device__ static unsigned record(R* p, const R& f_min, double amp_scale, double reference_mf, R* rec) {
+        with_reference_mf(p, reference_mf);
+        return xas_record<R>(p, f_min, amp_scale, rec);
+    }
+};
+
+struct PhenomDRecord {
+    static constexpr int kFoldA = kPdIntConst, kFoldB = kPdMrdConst;
+    template <class R>
+    __device__ static unsigned record(R* p, const R& f_min, double amp_scale, double reference_mf, R* rec) {
+        return pd_record<R>(p, f_min, amp_scale, reference_mf, rec);
+    }
+};
+
 // the record and its derivatives with respect to the 8 parameters (m1, m2,
 // chi1, chi2, distance, inclination, phi_ref, f_ref): one thread per
 // (binary, direction); jac (B, 64, 8); the direction-0 thread writes the
 // record, FP64 (records) and/or FP32 split (split_records), either NULL
+template <class S>
 __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict__ params, int64_t nb,
                                                        const double* __restrict__ f_min, int64_t f_min_stride,
                                                        double amp_scale, double reference_mf,
@@ -170,9 +192,8 @@ __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict_
         p[k] = D(params[b * kParams + k]);
         p[k].d[0] = k == dir ? 1.0 : 0.0;
     }
-    with_reference_mf(p, reference_mf);
     D rec[kSlots];
-    const unsigned fl = xas_record<D>(p, D(f_min[b * f_min_stride]), amp_scale, rec);
+    const unsigned fl = S::template record<D>(p, D(f_min[b * f_min_stride]), amp_scale, reference_mf, rec);
     for (int k = 0; k < kSlots; ++k) jac[(b * kSlots + k) * 8 + dir] = k < 60 ? rec[k].d[0] : 0.0;
     if (dir == 0) {
         double v[kSlots];
@@ -181,7 +202,7 @@ __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict_
         v[kAnchorLo] = 0.0;
         if (records)
             for (int k = 0; k < kSlots; ++k) records[b * kSlots + k] = v[k];
-        if (split_records) write_f32_record(v, split_records + b * kSlots);
+        if (split_records) write_f32_record(v, split_records + b * kSlots, S::kFoldA, S::kFoldB);
         if (flags) flags[b] = fl;
     }
 }
@@ -589,7 +610,39 @@ __global__ __launch_bounds__(kBlock) void power_table_kernel(const F* __restrict
 constexpr int kAdjointBlock = 64;
 constexpr int kLossSlot = kT0Lo;
 
-template <class T, int Mode, class F>
+// The models' per-bin formula and its adjoint, for adjoint_kernel: forward
+// gives phi and A of one bin, adjoint adds the bin's dL/d(record slot).
+struct XasAdjoint {
+    static constexpr int kKpSlot = kKp, kKcSlot = kKc;
+    template <class T>
+    __device__ __forceinline__ static T forward(const T* c, const BinPowers<T>& p, T& amp) {
+        bool past, ringdown;
+        T z;
+        return phase_and_amp(c, p, past, ringdown, z, amp);
+    }
+    template <class T>
+    __device__ __forceinline__ static void adjoint(const T* c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re,
+                                                   T gx_im, T* acc) {
+        adjoint_point(c, p, gp_re, gp_im, gx_re, gx_im, acc);
+    }
+};
+
+struct PhenomDAdjoint {
+    static constexpr int kKpSlot = kPdKp, kKcSlot = kPdKc;
+    template <class T>
+    __device__ __forceinline__ static T forward(const T* c, const BinPowers<T>& p, T& amp) {
+        bool inspiral, mrd;
+        T z;
+        return pd_phase_and_amp(c, p, pd_f34(p.f), inspiral, mrd, z, amp);
+    }
+    template <class T>
+    __device__ __forceinline__ static void adjoint(const T* c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re,
+                                                   T gx_im, T* acc) {
+        pd_adjoint_point(c, p, gp_re, gp_im, gx_re, gx_im, acc);
+    }
+};
+
+template <class A, class T, int Mode, class F>
 __global__ __launch_bounds__(kAdjointBlock) void adjoint_kernel(const T* __restrict__ records,
                                                                 const double* __restrict__ jac, int64_t nb,
                                                                 const F* __restrict__ freqs, int64_t n,
@@ -612,13 +665,12 @@ __global__ __launch_bounds__(kAdjointBlock) void adjoint_kernel(const T* __restr
         const int64_t o = 2 * (b * n + i);
         T gp_re = in_plus[o], gp_im = in_plus[o + 1], gx_re = in_cross[o], gx_im = in_cross[o + 1];
         if (Mode == kLoss) {
-            bool past, ringdown;
-            T z, amp;
-            const T phi = phase_and_amp(c, p, past, ringdown, z, amp);
+            T amp;
+            const T phi = A::forward(c, p, amp);
             T s, co;
             m_sincos(phi, &s, &co);
             const T w = weights[i];
-            const T ap = c[kKp] * amp, ac = c[kKc] * amp;
+            const T ap = c[A::kKpSlot] * amp, ac = c[A::kKcSlot] * amp;
             const T rp_re = ap * co - gp_re, rp_im = ap * s - gp_im;
             const T rx_re = -ac * s - gx_re, rx_im = ac * co - gx_im;
             acc[kLossSlot] += w * (rp_re * rp_re + rp_im * rp_im + rx_re * rx_re + rx_im * rx_im);
@@ -628,7 +680,7 @@ __global__ __launch_bounds__(kAdjointBlock) void adjoint_kernel(const T* __restr
             gx_re = two_w * rx_re;
             gx_im = two_w * rx_im;
         }
-        adjoint_point(c, p, gp_re, gp_im, gx_re, gx_im, acc);
+        A::adjoint(c, p, gp_re, gp_im, gx_re, gx_im, acc);
     }
     double s = (double)reduce_scatter64(acc, lane);
     if (Mode == kLoss && lane == kLossSlot) loss[b] = s;
@@ -681,21 +733,21 @@ struct AdjointArgs {
     hipStream_t stream;
 };
 
-template <class T, int Mode, class F>
+template <class A, class T, int Mode, class F>
 void launch_adjoint_t(const AdjointArgs& a) {
-    adjoint_kernel<T, Mode, F><<<a.blocks, kAdjointBlock, 0, a.stream>>>(
+    adjoint_kernel<A, T, Mode, F><<<a.blocks, kAdjointBlock, 0, a.stream>>>(
         (const T*)a.records, a.jac, a.nb, (const F*)a.freqs, a.n, (const T*)a.in_plus, (const T*)a.in_cross,
         (const T*)a.weights, a.grads, a.loss, (const PowerRow<T>*)a.powers);
 }
 
-template <int Mode>
+template <class A, int Mode>
 bool launch_adjoint(const AdjointArgs& a, int precision, int freqs_f64) {
     if (precision == kPrecF64) {
-        freqs_f64 ? launch_adjoint_t<double, Mode, double>(a) : launch_adjoint_t<double, Mode, float>(a);
+        freqs_f64 ? launch_adjoint_t<A, double, Mode, double>(a) : launch_adjoint_t<A, double, Mode, float>(a);
         return true;
     }
     if (precision == kPrecF32 || precision == kPrecSplit || precision == kPrecSplitDF) {
-        freqs_f64 ? launch_adjoint_t<float, Mode, double>(a) : launch_adjoint_t<float, Mode, float>(a);
+        freqs_f64 ? launch_adjoint_t<A, float, Mode, double>(a) : launch_adjoint_t<A, float, Mode, float>(a);
         return true;
     }
     return false;
@@ -861,6 +913,47 @@ int eval_ragged(const char* name, const void* records, int64_t n_binaries, const
     return check_launch(name);
 }
 
+// the derivative setup of model S behind its C entry point `name`
+template <class S>
+int setup_jvp(const char* name, const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
+              double amp_scale, double reference_mf, double* records, float* split_records, double* jacobian,
+              uint32_t* flags, void* stream) {
+    const std::string who(name);
+    if (n_binaries < 0) return fail((who + ": negative number of binaries").c_str());
+    if (n_binaries == 0) return 0;
+    if (!params || !f_min || !jacobian) return fail((who + ": null pointer").c_str());
+    if (!records && !split_records) return fail((who + ": no record output").c_str());
+    if (f_min_stride != 0 && f_min_stride != 1) return fail((who + ": f_min_stride must be 0 or 1").c_str());
+    const int blocks = grid_blocks((8 * n_binaries + 63) / 64, (who + ": too many binaries").c_str());
+    if (blocks < 0) return -1;
+    setup_jvp_kernel<S><<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, f_min, f_min_stride, amp_scale,
+                                                                reference_mf, records, split_records, jacobian, flags);
+    return check_launch(name);
+}
+
+// the per-binary adjoint of model A behind its C entry point `name`: from
+// the cotangents (Mode kFromCotangents; in_plus/in_cross are the
+// cotangents) or from the noise-weighted loss against data (Mode kLoss;
+// in_plus/in_cross are the data, loss receives the loss)
+template <class A, int Mode>
+int adjoint(const char* name, const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
+            int freqs_f64, int64_t n_freqs, int precision, const void* in_plus, const void* in_cross,
+            const void* weights, const void* powers, double* loss, double* grads, void* stream) {
+    const std::string who(name);
+    if (n_binaries < 0 || n_freqs < 0) return fail((who + ": negative size").c_str());
+    if (n_binaries == 0) return 0;
+    const bool loss_mode = Mode == kLoss;
+    if (!records || !jacobian || !grads || (loss_mode && !loss) ||
+        (n_freqs && (!freqs || !in_plus || !in_cross || (loss_mode && !weights))))
+        return fail((who + ": null pointer").c_str());
+    const int blocks = grid_blocks(n_binaries, (who + ": too many binaries").c_str());
+    if (blocks < 0) return -1;
+    const AdjointArgs a{records, jacobian, n_binaries, freqs,  n_freqs, in_plus, in_cross, weights,
+                        grads,   loss,     powers,     blocks, (hipStream_t)stream};
+    if (!launch_adjoint<A, Mode>(a, precision, freqs_f64)) return fail((who + ": unknown precision").c_str());
+    return check_launch(name);
+}
+
 }  // namespace
 
 // ================================================================= C ABI
@@ -870,7 +963,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 9; }
+int wfx_abi_version(void) { return 10; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -913,16 +1006,8 @@ int wfx_xas_setup(const double* params, int64_t n_binaries, const double* f_min,
 int wfx_xas_setup_jvp(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
                       double amp_scale, double reference_mf, double* records, float* split_records, double* jacobian,
                       uint32_t* flags, void* stream) {
-    if (n_binaries < 0) return fail("wfx_xas_setup_jvp: negative number of binaries");
-    if (n_binaries == 0) return 0;
-    if (!params || !f_min || !jacobian) return fail("wfx_xas_setup_jvp: null pointer");
-    if (!records && !split_records) return fail("wfx_xas_setup_jvp: no record output");
-    if (f_min_stride != 0 && f_min_stride != 1) return fail("wfx_xas_setup_jvp: f_min_stride must be 0 or 1");
-    const int blocks = grid_blocks((8 * n_binaries + 63) / 64, "wfx_xas_setup_jvp: too many binaries");
-    if (blocks < 0) return -1;
-    setup_jvp_kernel<<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, f_min, f_min_stride, amp_scale,
-                                                             reference_mf, records, split_records, jacobian, flags);
-    return check_launch("wfx_xas_setup_jvp");
+    return setup_jvp<XasRecord>("wfx_xas_setup_jvp", params, n_binaries, f_min, f_min_stride, amp_scale,
+                                reference_mf, records, split_records, jacobian, flags, stream);
 }
 
 int wfx_xas_reference_phase(const double* params, int64_t n_binaries, const double* mf, double* phase,
@@ -1045,16 +1130,9 @@ int wfx_xas_power_table(const void* freqs, int freqs_f64, int64_t n_freqs, int p
 int wfx_xas_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs, int freqs_f64,
                     int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, const void* powers,
                     double* grads, void* stream) {
-    if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xas_adjoint: negative size");
-    if (n_binaries == 0) return 0;
-    if (!records || !jacobian || !grads || (n_freqs && (!freqs || !cot_plus || !cot_cross)))
-        return fail("wfx_xas_adjoint: null pointer");
-    const int blocks = grid_blocks(n_binaries, "wfx_xas_adjoint: too many binaries");
-    if (blocks < 0) return -1;
-    const AdjointArgs a{records, jacobian, n_binaries, freqs, n_freqs, cot_plus, cot_cross, nullptr, grads, nullptr,
-                        powers, blocks, (hipStream_t)stream};
-    if (!launch_adjoint<kFromCotangents>(a, precision, freqs_f64)) return fail("wfx_xas_adjoint: unknown precision");
-    return check_launch("wfx_xas_adjoint");
+    return adjoint<XasAdjoint, kFromCotangents>("wfx_xas_adjoint", records, jacobian, n_binaries, freqs, freqs_f64,
+                                                n_freqs, precision, cot_plus, cot_cross, nullptr, powers, nullptr,
+                                                grads, stream);
 }
 
 int wfx_xas_grid_adjoint(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
@@ -1094,16 +1172,34 @@ int wfx_xas_grid_adjoint(const void* records, int64_t n_binaries, const void* fr
 int wfx_xas_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
                          int freqs_f64, int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
                          const void* weights, const void* powers, double* loss, double* grads, void* stream) {
-    if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xas_loss_adjoint: negative size");
-    if (n_binaries == 0) return 0;
-    if (!records || !jacobian || !grads || !loss || (n_freqs && (!freqs || !data_plus || !data_cross || !weights)))
-        return fail("wfx_xas_loss_adjoint: null pointer");
-    const int blocks = grid_blocks(n_binaries, "wfx_xas_loss_adjoint: too many binaries");
-    if (blocks < 0) return -1;
-    const AdjointArgs a{records, jacobian, n_binaries, freqs, n_freqs, data_plus, data_cross, weights, grads, loss,
-                        powers, blocks, (hipStream_t)stream};
-    if (!launch_adjoint<kLoss>(a, precision, freqs_f64)) return fail("wfx_xas_loss_adjoint: unknown precision");
-    return check_launch("wfx_xas_loss_adjoint");
+    return adjoint<XasAdjoint, kLoss>("wfx_xas_loss_adjoint", records, jacobian, n_binaries, freqs, freqs_f64,
+                                      n_freqs, precision, data_plus, data_cross, weights, powers, loss, grads,
+                                      stream);
+}
+
+int wfx_phd_setup_jvp(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
+                      double amp_scale, double reference_mf, double* records, float* split_records, double* jacobian,
+                      uint32_t* flags, void* stream) {
+    return setup_jvp<PhenomDRecord>("wfx_phd_setup_jvp", params, n_binaries, f_min, f_min_stride, amp_scale,
+                                    reference_mf, records, split_records, jacobian, flags, stream);
+}
+
+int wfx_phd_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs, int freqs_f64,
+                    int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, const void* powers,
+                    double* grads, void* stream) {
+    if (precision == kPrecF32) return fail("wfx_phd_adjoint: IMRPhenomD has no FP32-lead records");
+    return adjoint<PhenomDAdjoint, kFromCotangents>("wfx_phd_adjoint", records, jacobian, n_binaries, freqs,
+                                                    freqs_f64, n_freqs, precision, cot_plus, cot_cross, nullptr,
+                                                    powers, nullptr, grads, stream);
+}
+
+int wfx_phd_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
+                         int freqs_f64, int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
+                         const void* weights, const void* powers, double* loss, double* grads, void* stream) {
+    if (precision == kPrecF32) return fail("wfx_phd_loss_adjoint: IMRPhenomD has no FP32-lead records");
+    return adjoint<PhenomDAdjoint, kLoss>("wfx_phd_loss_adjoint", records, jacobian, n_binaries, freqs, freqs_f64,
+                                          n_freqs, precision, data_plus, data_cross, weights, powers, loss, grads,
+                                          stream);
 }
 
 }  // extern "C"

@@ -22,9 +22,10 @@ from typing import Optional, Tuple
 import torch
 
 from ._lib import call
-from .xas import RECORD_SLOTS, _code, _f_min, _ptr, _require_device, _stream, tile_prefix
+from .xas import RECORD_SLOTS, _code, _f_min, _ptr, _require_device, _stream, run_setup_jvp, tile_prefix
 
-__all__ = ["PRECISIONS", "LAL_FLAGS", "setup", "evaluate", "evaluate_ragged", "polarizations", "reference_phase"]
+__all__ = ["PRECISIONS", "LAL_FLAGS", "setup", "setup_jvp", "evaluate", "evaluate_ragged", "polarizations",
+           "reference_phase"]
 
 PRECISIONS = ("fp64", "split", "split-df")
 #: the setup's flag bits -> the reference's message (phenomd/api.py:39-45)
@@ -54,6 +55,18 @@ def setup(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21, re
          float("nan") if reference_mf is None else float(reference_mf), _code(precision), _ptr(records),
          _ptr(flags), _stream(params.device))
     return records, flags
+
+
+def setup_jvp(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21,
+              reference_mf: Optional[float] = None,
+              precision: str = "fp64") -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Records (B, 64) of ``precision`` (as :func:`setup` writes them), the
+    derivatives (B, 64, 8) of the FP64 record with respect to (m1, m2, chi1,
+    chi2, distance, inclination, phi_ref, f_ref) (forward mode through the
+    FP64 table; the region bounds carry none, as the reference's
+    ``jnp.where`` selections), and the flags."""
+    _check_precision(precision)
+    return run_setup_jvp("wfx_phd_setup_jvp", params, f_min, amp_scale, reference_mf, precision)
 
 
 def evaluate(records: torch.Tensor, freqs: torch.Tensor, precision: str = "split", group: int = 0,

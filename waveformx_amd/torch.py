@@ -18,14 +18,15 @@ no "fp32": its table is FP64) overrides the choice. Tensors on the CPU are
 computed on the current HIP device and the results copied back.
 Approximants: IMRPhenomXAS and IMRPhenomD.
 
-The backward pass of IMRPhenomXAS (``torch.autograd.Function``, once
-differentiable, the cotangents conjugated as in ``torch.py:108-119``) runs
-the hand-written adjoint kernel: it recomputes the formula per bin, reduces
-the cotangents into derivatives of each binary's record, and contracts them
+The backward pass (``torch.autograd.Function``, once differentiable, the
+cotangents conjugated as in ``torch.py:108-119``) runs the model's
+hand-written adjoint kernel: it recomputes the formula per bin, reduces the
+cotangents into derivatives of each binary's record, and contracts them
 with the record's forward-mode derivatives from the FP64 setup. Gradients
-reach every field (the in-plane spins get zero) and the grid. IMRPhenomD's
-backward raises ``NotImplementedError``. As in the reference, the function
-is opaque to ``torch.compile`` and ``torch.func``.
+reach every field (the in-plane spins get zero); the grid's gradient is
+IMRPhenomXAS's only (IMRPhenomD raises ``NotImplementedError`` for a grid
+that requires one). As in the reference, the function is opaque to
+``torch.compile`` and ``torch.func``.
 """
 
 from __future__ import annotations
@@ -103,12 +104,10 @@ class _ModelFunction(torch.autograd.Function):
         from . import adjoint
 
         approximant, amp_scale, reference_mf, arithmetic = ctx.key
-        if approximant != "IMRPhenomXAS":
-            raise NotImplementedError(f"the gradient of {approximant} is not built (IMRPhenomXAS has one)")
         grid, params = ctx.saved_tensors
         wrt_grid = ctx.needs_input_grad[1]
         grads, grad_grid = adjoint.vjp(params, grid.contiguous(), grad_hplus, grad_hcross, amp_scale, reference_mf,
-                                       arithmetic, wrt_grid)
+                                       arithmetic, wrt_grid, approximant)
         dtype = grid.dtype
         g = grads.to(dtype)
         spin_1 = torch.zeros((g.shape[0], 3), dtype=dtype, device=g.device)

@@ -103,6 +103,13 @@ def setup_jvp(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21
     "split-df": the float32 records :func:`setup` writes for them), the
     derivatives (B, 64, 8) of the FP64 record with respect to (m1, m2, chi1,
     chi2, distance, inclination, phi_ref, f_ref), and flags."""
+    return run_setup_jvp("wfx_xas_setup_jvp", params, f_min, amp_scale, reference_mf, precision)
+
+
+def run_setup_jvp(entry: str, params: torch.Tensor, f_min: torch.Tensor, amp_scale: float,
+                  reference_mf: Optional[float], precision: str) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """:func:`setup_jvp` through the C entry point ``entry`` (a model's
+    derivative setup: they share one signature)."""
     _require_device(params, "params")
     if precision not in ("fp64", "split", "split-df"):
         raise ValueError(f"setup_jvp writes fp64 or split records, not {precision!r}")
@@ -113,7 +120,7 @@ def setup_jvp(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21
     jac = torch.empty((nb, RECORD_SLOTS, 8), dtype=torch.float64, device=params.device)
     flags = torch.empty((nb,), dtype=torch.int32, device=params.device)
     wide, split = (_ptr(records), None) if precision == "fp64" else (None, _ptr(records))
-    call("wfx_xas_setup_jvp", _ptr(params), nb, _ptr(f_min), stride, float(amp_scale),
+    call(entry, _ptr(params), nb, _ptr(f_min), stride, float(amp_scale),
          float("nan") if reference_mf is None else float(reference_mf), wide, split, _ptr(jac), _ptr(flags),
          _stream(params.device))
     return records, jac, flags
