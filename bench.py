@@ -27,10 +27,10 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the FP32-split evaluation at 65,536 x 4017 in its
 # default launch shape, from two rocprofv3 --pmc passes over
-# `bench.py --only split` (summary under profiles/): WRITE_SIZE 4,122,000 KiB
-# x 1.000 + FETCH_SIZE 17,030 KiB x 2.000, the factors that a 1 GiB device
+# `bench.py --only split` (summary under profiles/): WRITE_SIZE 4,116,078 KiB
+# x 1.000 + FETCH_SIZE 16,604 KiB x 2.000, the factors that a 1 GiB device
 # copy gives the two counters. Reported only for that workload and shape.
-EVAL_TRAFFIC = {(65536, 4017): 4.256e9}
+EVAL_TRAFFIC = {(65536, 4017): 4.249e9}
 METRIC = "XAS waveforms/s @4017 bins: FP32-split vs FP64 GPU (mismatch<=1e-6); fwd+grad/s"
 
 
@@ -267,7 +267,7 @@ def main():
         records, _ = xas.setup(params, f32[:1], 1e21, None, "split")
         out = (torch.empty((nb, n), dtype=torch.complex64, device=dev),
                torch.empty((nb, n), dtype=torch.complex64, device=dev))
-        for group in (4, 8, 16, 32):
+        for group in (2, 3, 4, 6, 8, 16):
             for fpt in (2, 4, 8):
                 ms = timed(torch, lambda: xas.evaluate(records, f32, "split", group, out=out, freqs_per_thread=fpt),
                            args.steps, args.warmup)

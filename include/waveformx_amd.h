@@ -79,8 +79,9 @@ int wfx_xas_reference_phase(const double* params, int64_t n_binaries, const doub
                             void* stream);
 
 /* (h+, hx)[B][N] of records on one shared grid freqs[N] (float, or double
- * if freqs_f64). group: binaries per block; freqs_per_thread: 1, 2, 4 or 8
- * frequencies per thread on the FP32 paths (0: defaults, 8 and 4); every
+ * if freqs_f64). group: binaries per block (0: default, 4 on the FP32
+ * paths, 8 for FP64); freqs_per_thread: 1, 2, 4 or 8 frequencies per thread
+ * on the FP32 paths (0: default, 4); every
  * shape writes the same bits. Replaces core.frequency_powers +
  * core.evaluate under jax.vmap (src/waveformx/models/xas/core.py:382-507;
  * src/waveformx/torch.py:84-85). */

@@ -319,10 +319,13 @@ __global__ __launch_bounds__(kBlock) void eval_f64_shared(const double* __restri
 //
 // The shape trades three costs, measured on MI355X at 65,536 x 4017: the
 // powers (amortised over the group), the record's scalar loads and their
-// latency (amortised over FPT bins per thread), and the write pattern (the
+// latency (amortised over FPT bins per thread), and the write pattern: the
 // output streams best when the blocks running at one time on an XCD write a
-// narrow window of it; a block of G binaries writes G rows 32 KB apart).
-// 8 binaries x 4 frequencies per thread balances them.
+// narrow, contiguous window of it. Group g therefore takes the binaries g,
+// g + ngroups, g + 2 ngroups, ... (ngroups = ceil(B / group)): the XCD's
+// blocks, which hold consecutive groups, write consecutive rows at each
+// step of their loops, where groups of consecutive binaries would write
+// rows `group` apart. 4 binaries x 4 frequencies per thread balances them.
 template <class M, int L, int FPT, class F>
 __global__ __launch_bounds__(kBlock) void eval_f32_shared(const float* __restrict__ records, int64_t nb,
                                                           const F* __restrict__ freqs, int64_t n, int64_t ntiles,
@@ -330,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void eval_f32_shared(const float* __restric
                                                           float2* __restrict__ hc) {
     const int64_t w = xcd_work_id(blockIdx.x, gridDim.x);
     const int64_t tile = w % ntiles;
-    const int64_t b0 = (w / ntiles) * group;
+    const int64_t g = w / ntiles;
     const int64_t i0 = tile * (kBlock * FPT) + threadIdx.x;
     typename M::Pow32 p[FPT];
 #pragma unroll
@@ -338,8 +341,11 @@ __global__ __launch_bounds__(kBlock) void eval_f32_shared(const float* __restric
         const int64_t i = i0 + j * kBlock;
         p[j] = M::powers32((float)freqs[i < n ? i : n - 1]);
     }
-    const int64_t b1 = b0 + group < nb ? b0 + group : nb;
-    for (int64_t b = b0; b < b1; ++b) {
+    // (a trip bound, not an exit inside the loop: the latter made the
+    // compiler load the record with vector loads)
+    const int64_t ngroups = (nb + group - 1) / group;
+    const int64_t b1 = g + (int64_t)group * ngroups < nb ? g + (int64_t)group * ngroups : nb;
+    for (int64_t b = g; b < b1; b += ngroups) {
         float c[kSlots];
         const float* rec = records + b * kSlots;
 #pragma unroll
@@ -754,7 +760,10 @@ bool launch_adjoint(const AdjointArgs& a, int precision, int freqs_f64) {
 }
 
 // the FP32 shared-grid evaluation's default shape (see eval_f32_shared)
-constexpr int kDefaultGroup = 8;
+constexpr int kDefaultGroup = 4;
+// the FP64 evaluation's (its frequency powers are FP64 transcendentals:
+// shared by more binaries)
+constexpr int kDefaultGroupF64 = 8;
 constexpr int kDefaultFreqsPerThread = 4;
 
 struct SharedArgs {
@@ -851,7 +860,7 @@ int eval_shared(const char* name, const void* records, int64_t n_binaries, const
         return fail((who + ": freqs_per_thread must be 1, 2, 4 or 8 (0: default)").c_str());
     if (precision == kPrecF64) freqs_per_thread = 1;
     if (freqs_per_thread == 0) freqs_per_thread = kDefaultFreqsPerThread;
-    if (group <= 0) group = kDefaultGroup;
+    if (group <= 0) group = precision == kPrecF64 ? kDefaultGroupF64 : kDefaultGroup;
     const int64_t ntiles = (n_freqs + kBlock * freqs_per_thread - 1) / (kBlock * freqs_per_thread);
     const int64_t ngroups = (n_binaries + group - 1) / group;
     const int blocks = grid_blocks(ntiles * ngroups, (who + ": grid too large, raise the group size").c_str());
