@@ -13,6 +13,8 @@
 // xas_fits.inc (transcribed from the reference's generated modules).
 #pragma once
 
+#include <type_traits>
+
 #include "wfx_math.h"
 #include "xas_layout.h"
 
@@ -224,8 +226,15 @@ WFX_HD inline AmpCoeffs<R> amp_coeffs(const WaveformVars<R>& wf) {
 // A x = b by Gaussian elimination with partial pivoting (largest |pivot|,
 // first on ties), the algorithm behind phase22.py:44-49's
 // jnp.linalg.solve; x overwrites b.
+//
+// For plain floating-point R the row exchange is written as selects over
+// the candidate rows, so that every index is a compile-time constant once
+// the loops unroll (an exchange through the runtime pivot index puts the
+// arrays in scratch memory). The arithmetic, and its result, is that of the
+// plain exchange, which dual numbers keep: their setup is bound by
+// register pressure, which the unrolled selects raise.
 template <class R, int N>
-WFX_HD inline void pivoted_solve(R (&a)[N][N], R (&b)[N]) {
+WFX_HD inline void pivoted_solve_exchange(R (&a)[N][N], R (&b)[N]) {
     for (int k = 0; k < N; ++k) {
         int piv = k;
         double big = fabs(value_of(a[k][k]));
@@ -257,6 +266,58 @@ WFX_HD inline void pivoted_solve(R (&a)[N][N], R (&b)[N]) {
         for (int j = i + 1; j < N; ++j) s = s - a[i][j] * b[j];
         b[i] = s / a[i][i];
     }
+}
+
+template <class R, int N>
+WFX_HD inline void pivoted_solve_select(R (&a)[N][N], R (&b)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        int piv = k;
+        double big = fabs(value_of(a[k][k]));
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) {
+            double m = fabs(value_of(a[i][k]));
+            if (m > big) {
+                big = m;
+                piv = i;
+            }
+        }
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) {
+            const bool swap = piv == i;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const R t = a[k][j];
+                a[k][j] = swap ? a[i][j] : t;
+                a[i][j] = swap ? t : a[i][j];
+            }
+            const R t = b[k];
+            b[k] = swap ? b[i] : t;
+            b[i] = swap ? t : b[i];
+        }
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) {
+            R l = a[i][k] / a[k][k];
+#pragma unroll
+            for (int j = k + 1; j < N; ++j) a[i][j] = a[i][j] - l * a[k][j];
+            b[i] = b[i] - l * b[k];
+        }
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        R s = b[i];
+#pragma unroll
+        for (int j = i + 1; j < N; ++j) s = s - a[i][j] * b[j];
+        b[i] = s / a[i][i];
+    }
+}
+
+template <class R, int N>
+WFX_HD inline void pivoted_solve(R (&a)[N][N], R (&b)[N]) {
+    if constexpr (std::is_floating_point<R>::value)
+        pivoted_solve_select(a, b);
+    else
+        pivoted_solve_exchange(a, b);
 }
 
 // phase22.py:52-263 then the connections of phase22.py:266-286
@@ -567,16 +628,41 @@ WFX_HD inline void fold_term(R* rec, int slot, int log_slot, const R& c, const R
 template <class R>
 WFX_HD inline unsigned xas_record(const R* params, const R& f_min, double amp_scale, R* rec) {
     WaveformVars<R> wf = waveform_vars(params, f_min, amp_scale);
-    AmpCoeffs<R> amp = amp_coeffs(wf);
+    R s = wf.m_sec;
+    R one = R(1.0);
+    for (int k = 0; k < kSlots; ++k) rec[k] = R(0.0);
+
+    // the amplitude first, so that its coefficients are dead before the
+    // phase's solves (each slot's arithmetic is independent of the order)
+    {
+        AmpCoeffs<R> amp = amp_coeffs(wf);
+        Powers<R> p = useful_powers(s);
+        R prefactor = wf.amp0 * wf.amp_norm;
+        R scale = prefactor * p.m_seven_sixths;
+        const R third[10] = {one, p.one_third, p.two_thirds, s, p.four_thirds, p.five_thirds, p.two,
+                             p.two * p.one_third, p.two * p.two_thirds, p.three};
+        const R ins[10] = {one, R(0.0), amp.pn_two_thirds, amp.pn_three_thirds, amp.pn_four_thirds,
+                           amp.pn_five_thirds, amp.pn_six_thirds, amp.rho1, amp.rho2, amp.rho3};
+        for (int k = 0; k < 10; ++k) rec[kAmpIns + k] = scale * ins[k] * third[k];
+        rec[kAmpIntN] = prefactor;
+        rec[kAmpIntD + 0] = amp.delta0 * one;
+        rec[kAmpIntD + 1] = amp.delta1 * s;
+        rec[kAmpIntD + 2] = amp.delta2 * p.two;
+        rec[kAmpIntD + 3] = amp.delta3 * p.three;
+        rec[kAmpIntD + 4] = amp.delta4 * p.four;
+        rec[kAmpRdN] = scale * amp.gamma_d13;
+        rec[kAmpRdGr] = amp.gamma_r;
+        rec[kAmpRdS] = s;
+        rec[kAmpRdG] = amp.gamma_d2;
+        rec[kFAmpIn] = amp.f_amp_match_in / s;
+        rec[kFAmpRd] = amp.f_amp_rd_min / s;
+    }
+
     PhaseCoeffs<R> ph = phase_coeffs(wf);
     R linb = time_shift(ph, wf);
     R inveta = R(1.0) / wf.eta;
-    R s = wf.m_sec;
     Powers<R> p = useful_powers(s);
     R norm = inveta * ph.phi_norm;
-    R one = R(1.0);
-
-    for (int k = 0; k < kSlots; ++k) rec[k] = R(0.0);
     // inspiral (the order of xas/core.py:306-310, the time shift last)
     fold_term(rec, kInsM1, -1, norm * ph.phi2, p.m_one, p.log);
     fold_term(rec, kInsM23, -1, norm * ph.phi3, p.m_two_thirds, p.log);
@@ -615,25 +701,6 @@ WFX_HD inline unsigned xas_record(const R* params, const R& f_min, double amp_sc
     rec[kFPhaseIn] = ph.f_phase_match_in / s;
     rec[kFPhaseIm] = ph.f_phase_match_im / s;
 
-    R prefactor = wf.amp0 * wf.amp_norm;
-    R scale = prefactor * p.m_seven_sixths;
-    const R third[10] = {one, p.one_third, p.two_thirds, s, p.four_thirds, p.five_thirds, p.two,
-                         p.two * p.one_third, p.two * p.two_thirds, p.three};
-    const R ins[10] = {one, R(0.0), amp.pn_two_thirds, amp.pn_three_thirds, amp.pn_four_thirds, amp.pn_five_thirds,
-                       amp.pn_six_thirds, amp.rho1, amp.rho2, amp.rho3};
-    for (int k = 0; k < 10; ++k) rec[kAmpIns + k] = scale * ins[k] * third[k];
-    rec[kAmpIntN] = prefactor;
-    rec[kAmpIntD + 0] = amp.delta0 * one;
-    rec[kAmpIntD + 1] = amp.delta1 * s;
-    rec[kAmpIntD + 2] = amp.delta2 * p.two;
-    rec[kAmpIntD + 3] = amp.delta3 * p.three;
-    rec[kAmpIntD + 4] = amp.delta4 * p.four;
-    rec[kAmpRdN] = scale * amp.gamma_d13;
-    rec[kAmpRdGr] = amp.gamma_r;
-    rec[kAmpRdS] = s;
-    rec[kAmpRdG] = amp.gamma_d2;
-    rec[kFAmpIn] = amp.f_amp_match_in / s;
-    rec[kFAmpRd] = amp.f_amp_rd_min / s;
     rec[kFCut] = wf.f_cut;
 
     const double nan = __builtin_nan("");
