@@ -194,24 +194,38 @@ def bench_ragged(torch, xas, dev, nb, steps, warmup):
             "ragged_padded_bins": nb * longest, "ragged_shape": [plan.bins_per_thread, plan.blocks]}
 
 
-def cpu_baseline(n_binaries, freqs, cores=None):
+def host_cpu():
+    """The host's CPU model and logical CPU count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
+def cpu_baseline(n_binaries, freqs, cores=None, repeats=7):
     """The FP64 oracle on the host cores (16, the box's share, unless
-    ``cores``) on ``n_binaries`` of the workload: median of 3 timed runs
-    after a warm-up."""
+    ``cores``) on ``n_binaries`` of the workload: median of ``repeats``
+    timed runs after a warm-up."""
     from oracle import oracle
 
     cores = cores or min(16, os.cpu_count() or 1)
     params = draw(n_binaries, seed=2)
     oracle.polarizations(params[:8], freqs, threads=cores)
     times = []
-    for _ in range(3):
+    for _ in range(repeats):
         t0 = time.perf_counter()
         oracle.polarizations(params, freqs, threads=cores)
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     return {"value": n_binaries / t, "unit": "waveforms/s", "cores": cores, "kind": "port",
             "sample": f"{n_binaries} binaries x {len(freqs)} bins, FP64 C++ oracle, OpenMP over binaries, "
-                      f"median of 3 ({t:.2f} s each)"}
+                      f"median of {repeats} ({t:.2f} s each)", "host": host_cpu()}
 
 
 def main():
@@ -301,6 +315,7 @@ def main():
 
     ms_split = timed(torch, step("split", f32, out32), args.steps, args.warmup)
     ms_df = timed(torch, step("split-df", f32, out32), args.steps, args.warmup)
+    ms_fp32 = timed(torch, step("fp32", f32, out32), args.steps, args.warmup)
     ms_fp64 = timed(torch, step("fp64", f64, out64), max(3, args.steps // 2), 1)
     # the dominant kernel alone (the evaluation), on the stream it runs on
     ms_eval = timed(torch, lambda: xas.evaluate(rec["split"], f32, "split", args.group, out=out32), args.steps,
@@ -320,11 +335,15 @@ def main():
 
     # accuracy of the split path against the FP64 path on the device (which
     # matches the FP64 oracle to 1e-12): on the FP32-rounded grid the split
-    # path sees, and on the exact grid
-    step("split", f32, out32)()
+    # path sees, and on the exact grid; then the plain FP32 path's on the
+    # rounded grid
     w = 1.0 / torch.as_tensor(((freqs64 / 100.0) ** -4 + 1.0 + (freqs64 / 100.0) ** 2), device=dev)
     step("fp64", f32.to(torch.float64), out64)()
+    step("split", f32, out32)()
     rounded = accuracy(torch, out32, out64, w)
+    step("fp32", f32, out32)()
+    rounded_fp32 = accuracy(torch, out32, out64, w)
+    step("split", f32, out32)()
     step("fp64", f64, out64)()
     exact = accuracy(torch, out32, out64, w)
     torch.cuda.synchronize()
@@ -350,6 +369,8 @@ def main():
         "fp64_ms_per_step": ms_fp64,
         "split_over_fp64": ms_fp64 / ms_split,
         "split_df_value": nb / (ms_df * 1e-3),
+        "fp32_value": nb / (ms_fp32 * 1e-3),
+        "fp32_accuracy_vs_fp64_rounded_grid": rounded_fp32,
         "eval_ms": ms_eval,
         "setup_ms": ms_setup,
         "phenomd_split_value": nb / (ms_pd_split * 1e-3),
@@ -370,6 +391,13 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_binaries, freqs64)
         result["cpu_baseline_1core"] = cpu_baseline(max(64, args.cpu_binaries // 32), freqs64, cores=1)
+    # literature, not measured here: the reference's JAX float64 jit(vmap) on
+    # a 24-CPU M2 Ultra, 4096 binaries x 1000 bins in 18.97 ms
+    # (report/data/xas.json:4132), and that figure scaled linearly to 4017 bins
+    result["reference_published_cpu"] = {"value": 4096 / 18.97e-3, "unit": "waveforms/s @1000 bins",
+                                         "scaled_to_bins": 4096 / 18.97e-3 * 1000 / n,
+                                         "hardware": "Apple M2 Ultra CPU (24), JAX float64 jit(vmap)",
+                                         "kind": "literature"}
     print(json.dumps(result), flush=True)
 
 
