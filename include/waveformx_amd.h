@@ -88,8 +88,8 @@ int wfx_xas_reference_phase(const double* params, int64_t n_binaries, const doub
 int wfx_xas_eval(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
                  int precision, int group, int freqs_per_thread, void* hplus, void* hcross, void* stream);
 
-/* tiles[B+1]: the prefix sum of the tiles (256 x bins_per_thread bins; 1, 2
- * or 4) of each binary's range [offsets[b], offsets[b+1]), the work list of
+/* tiles[B+1]: the prefix sum of the tiles (256 x bins_per_thread bins; 1, 2,
+ * 4, 8 or 16) of each binary's range [offsets[b], offsets[b+1]), the work list of
  * wfx_xas_eval_ragged. No counterpart in the reference (its batches share
  * one grid). */
 int wfx_xas_tile_prefix(const int64_t* offsets, int64_t n_binaries, int bins_per_thread, int64_t* tiles,

@@ -71,15 +71,41 @@ def test_tuner_and_shapes(cuda):
     grids = grids_for(params, rng)
     pt = torch.as_tensor(params, device=cuda)
     reference = None
-    for bpt in (1, 2, 4):
-        for blocks in (256, 2048):
+    for bpt in (1, 2, 4, 8, 16):
+        for blocks in (256, 2048, 16384):
             plan = RaggedPlan(grids, precision="split", tune=False, graph=False, bins_per_thread=bpt, blocks=blocks)
             hp, _ = plan(pt)
             if reference is None:
                 reference = hp.clone()
             assert torch.equal(hp, reference), (bpt, blocks)
     tuned = RaggedPlan(grids, precision="split", tune=True)
-    assert tuned.bins_per_thread in (1, 2, 4) and tuned.blocks > 0
+    assert tuned.bins_per_thread in (2, 4, 8, 16) and tuned.blocks > 0
     hp, _ = tuned(pt)
     torch.cuda.synchronize()
     assert torch.equal(hp, reference)
+
+
+@pytest.mark.parametrize("bpt", [1, 8])
+def test_many_binaries_with_empty_ranges_match_the_shared_grid(cuda, bpt):
+    """5,000 binaries on one grid, every seventh with an empty range: the
+    persistent launch's owner search (three rounds at this size) gives each
+    work item its binary, and the ragged evaluation writes the shared-grid
+    evaluation's bits."""
+    import torch
+
+    from waveformx_amd import xas
+
+    nb, n = 5000, 700
+    params = torch.as_tensor(oracle.draw(nb, seed=13), device=cuda)
+    f = torch.as_tensor(np.linspace(20.0, 900.0, n), dtype=torch.float32, device=cuda)
+    records, _ = xas.setup(params, f[:1], 1e21, None, "split")
+    shared_p, shared_c = xas.evaluate(records, f, "split")
+    lengths = np.where(np.arange(nb) % 7 == 3, 0, n)
+    offsets = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    keep = torch.as_tensor(lengths > 0, device=cuda)
+    flat = f.repeat(int(keep.sum()))
+    hp, hc = xas.evaluate_ragged(records, torch.as_tensor(offsets, device=cuda), freqs=flat, precision="split",
+                                 bins_per_thread=bpt, blocks=1024)
+    torch.cuda.synchronize()
+    assert torch.equal(hp.view(torch.int32), shared_p[keep].reshape(-1).view(torch.int32))
+    assert torch.equal(hc.view(torch.int32), shared_c[keep].reshape(-1).view(torch.int32))

@@ -364,15 +364,22 @@ __global__ __launch_bounds__(kBlock) void eval_f32_shared(const float* __restric
 }
 
 // ragged grids: work item w -> (binary, tile) through the prefix sum of
-// tiles per binary (tiles[b] .. tiles[b + 1])
+// tiles per binary (tiles[b] .. tiles[b + 1]): the largest b with tiles[b]
+// <= w, by a 64-ary search of the whole wave (the wave must be converged):
+// each round lane l tests the candidate lo + l step, the lanes that pass
+// form a prefix (tiles is non-decreasing), and the range shrinks to the last
+// passing candidate's step. Three rounds of one vector load for 2^18
+// binaries, where a binary search takes a chain of 18 dependent loads.
 __device__ inline int64_t owner(const int64_t* __restrict__ tiles, int64_t nb, int64_t w) {
     int64_t lo = 0, hi = nb;  // tiles[lo] <= w < tiles[hi]
+    const int lane = threadIdx.x & 63;
     while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (tiles[mid] <= w)
-            lo = mid;
-        else
-            hi = mid;
+        const int64_t step = (hi - lo + 63) >> 6;
+        const int64_t cand = lo + (int64_t)lane * step;
+        const bool pass = cand < hi && tiles[cand] <= w;  // lane 0 (cand = lo) passes
+        const int last = 63 - __clzll(__ballot(pass));
+        lo += (int64_t)last * step;
+        hi = lo + step < hi ? lo + step : hi;
     }
     return lo;
 }
@@ -421,6 +428,10 @@ __global__ __launch_bounds__(kBlock) void eval_f32_ragged(const float* __restric
     for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
         const int64_t b = owner(tiles, nb, w);
         const int64_t start = offsets[b], stop = offsets[b + 1];
+        float rc[kSlots];
+        const float* rec = records + b * kSlots;
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) rc[k] = rec[k];
 #pragma unroll
         for (int j = 0; j < BPT; ++j) {
             const int64_t k = (w - tiles[b]) * (kBlock * BPT) + j * kBlock + threadIdx.x;
@@ -428,7 +439,7 @@ __global__ __launch_bounds__(kBlock) void eval_f32_ragged(const float* __restric
             if (i >= stop) break;
             const typename M::Pow32 p = M::powers32((float)ragged_freq(freqs, f0, df, b, i, k));
             float2 a, c;
-            M::template point32<L>(records + b * kSlots, p, a, c);
+            M::template point32<L>(rc, p, a, c);
             hp[i] = a;
             hc[i] = c;
         }
@@ -901,8 +912,9 @@ int eval_ragged(const char* name, const void* records, int64_t n_binaries, const
     if (n_binaries == 0) return 0;
     if (!records || !offsets || !tiles || !hplus || !hcross) return fail((who + ": null pointer").c_str());
     if (!freqs && (!grid_f0 || !grid_df)) return fail((who + ": give freqs or (grid_f0, grid_df)").c_str());
-    if (bins_per_thread != 1 && bins_per_thread != 2 && bins_per_thread != 4)
-        return fail((who + ": bins_per_thread must be 1, 2 or 4").c_str());
+    if (bins_per_thread != 1 && bins_per_thread != 2 && bins_per_thread != 4 && bins_per_thread != 8 &&
+        bins_per_thread != 16)
+        return fail((who + ": bins_per_thread must be 1, 2, 4, 8 or 16").c_str());
     if (precision < kPrecF64 || precision > kPrecSplitDF) return fail((who + ": unknown precision").c_str());
     if (blocks <= 0) blocks = 8 * cu_count();
     const RaggedArgs a{records, n_binaries, freqs, grid_f0, grid_df, offsets, tiles, hplus, hcross, blocks,
@@ -915,8 +927,14 @@ int eval_ragged(const char* name, const void* records, int64_t n_binaries, const
         case 2:
             f64 ? launch_ragged<M, 2, double>(a, precision) : launch_ragged<M, 2, float>(a, precision);
             break;
-        default:
+        case 4:
             f64 ? launch_ragged<M, 4, double>(a, precision) : launch_ragged<M, 4, float>(a, precision);
+            break;
+        case 8:
+            f64 ? launch_ragged<M, 8, double>(a, precision) : launch_ragged<M, 8, float>(a, precision);
+            break;
+        default:
+            f64 ? launch_ragged<M, 16, double>(a, precision) : launch_ragged<M, 16, float>(a, precision);
             break;
     }
     return check_launch(name);
@@ -1040,8 +1058,9 @@ int wfx_xas_tile_prefix(const int64_t* offsets, int64_t n_binaries, int bins_per
                         void* stream) {
     if (n_binaries < 0) return fail("wfx_xas_tile_prefix: negative size");
     if (!offsets || !tiles) return fail("wfx_xas_tile_prefix: null pointer");
-    if (bins_per_thread != 1 && bins_per_thread != 2 && bins_per_thread != 4)
-        return fail("wfx_xas_tile_prefix: bins_per_thread must be 1, 2 or 4");
+    if (bins_per_thread != 1 && bins_per_thread != 2 && bins_per_thread != 4 && bins_per_thread != 8 &&
+        bins_per_thread != 16)
+        return fail("wfx_xas_tile_prefix: bins_per_thread must be 1, 2, 4, 8 or 16");
     tile_prefix_kernel<<<1, 1024, 0, (hipStream_t)stream>>>(offsets, n_binaries, (int64_t)kBlock * bins_per_thread,
                                                               tiles);
     return check_launch("wfx_xas_tile_prefix");

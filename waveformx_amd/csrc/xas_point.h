@@ -90,18 +90,24 @@ __device__ inline float fast_div(float a, float b) { return a * __builtin_amdgcn
 
 // FP32 powers of a frequency given in FP32; f^(-1/3) from 1/t is refined
 // by two Newton steps in FP64 before f^(-5/3) = (f^(-1/3))^5
+//
+// r = f^(-1/3) in FP64 by two Newton steps r <- r (4 - f r^3) / 3 from the
+// hardware exp2(-log2(f) / 3) (relative error ~1e-7, then ~1e-14, then
+// below FP64 rounding); f^(1/3) = f r^2 and 1/f = r^3 are rounded from it to
+// FP32 (no separate cube root or division), f^(-5/3) = r^5 stays FP64 for
+// the leading phase term.
 __device__ inline PowF32 pow_f32(float f) {
     PowF32 p;
     p.f = f;
     p.lf = logf(f);
-    p.t = cbrtf(f);
-    p.m1 = 1.0f / f;
-    p.m76 = p.m1 * __builtin_amdgcn_rsqf(p.t);
     const double fd = (double)f;
-    double r = (double)(1.0f / p.t);
+    double r = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(f) * (-1.0f / 3.0f));
     r = r * (4.0 - fd * (r * r * r)) * (1.0 / 3.0);
     r = r * (4.0 - fd * (r * r * r)) * (1.0 / 3.0);
     const double r2 = r * r;
+    p.t = (float)(fd * r2);
+    p.m1 = (float)(r2 * r);
+    p.m76 = p.m1 * __builtin_amdgcn_rsqf(p.t);
     p.m53 = r2 * r2 * r;
     p.m53_hi = (float)p.m53;
     p.m53_lo = (float)(p.m53 - (double)p.m53_hi);

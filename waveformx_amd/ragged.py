@@ -28,8 +28,8 @@ from . import xas
 
 __all__ = ["RaggedPlan", "uniform_grid_length"]
 
-BINS_PER_THREAD = (1, 2, 4)
-BLOCKS_PER_CU = (2, 4, 8, 16)
+BINS_PER_THREAD = (2, 4, 8, 16)
+BLOCKS_PER_CU = (4, 16, 32, 64)
 
 
 def uniform_grid_length(f_min: float, f_max: float, delta_f: float) -> int:
