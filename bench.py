@@ -27,10 +27,10 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the FP32-split evaluation at 65,536 x 4017 in its
 # default launch shape, from two rocprofv3 --pmc passes over
-# `bench.py --only split` (summary under profiles/): WRITE_SIZE 4,116,078 KiB
-# x 1.000 + FETCH_SIZE 16,604 KiB x 2.000, the factors that a 1 GiB device
+# `bench.py --only split` (summary under profiles/): WRITE_SIZE 4,118,276 KiB
+# x 1.000 + FETCH_SIZE 16,624 KiB x 2.000, the factors that a 1 GiB device
 # copy gives the two counters. Reported only for that workload and shape.
-EVAL_TRAFFIC = {(65536, 4017): 4.249e9}
+EVAL_TRAFFIC = {(65536, 4017): 4.251e9}
 METRIC = "XAS waveforms/s @4017 bins: FP32-split vs FP64 GPU (mismatch<=1e-6); fwd+grad/s"
 
 
@@ -242,7 +242,7 @@ def main():
                     help="binaries of the fused forward + adjoint measurement (0: skip)")
     ap.add_argument("--ragged-binaries", type=int, default=16384,
                     help="binaries of the ragged-grid measurement (0: skip)")
-    ap.add_argument("--only", choices=["split", "split-df", "fp64", "fp32", "grad"],
+    ap.add_argument("--only", choices=["split", "split-df", "fp64", "fp32", "grad", "ragged"],
                     help="run just this arithmetic's step (setup + evaluation) K times, for a profiler")
     ap.add_argument("--sweep", action="store_true", help="time the evaluation's launch shapes and exit")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -269,6 +269,17 @@ def main():
     if args.only == "grad":
         print(json.dumps(bench_grad(torch, xas, dev, args.grad_binaries, args.bins, args.steps, args.warmup)),
               flush=True)
+        return
+    if args.only == "ragged":
+        # the ragged workload's graph-replayed plan alone (tuned shape)
+        from waveformx_amd.ragged import RaggedPlan
+
+        p, grids = ragged_grids(args.ragged_binaries)
+        params = torch.as_tensor(p, device=dev)
+        plan = RaggedPlan(grids, precision="split")
+        ms = timed(torch, lambda: plan(params), args.steps, args.warmup)
+        print(json.dumps({"only": "ragged", "ms_per_step": ms, "bins": plan.total,
+                          "shape": [plan.bins_per_thread, plan.blocks]}), flush=True)
         return
     nb, n = args.binaries, args.bins
     freqs64 = np.linspace(20.0, 1024.0, n)
