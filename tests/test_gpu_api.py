@@ -114,3 +114,30 @@ def test_batch_polarizations_errors(cuda):
         wx.batch_polarizations("IMRPhenomXAS", f, table, strict=True)
     chunked = wx.batch_polarizations("IMRPhenomXAS", f, table, chunk_size=2)
     assert np.array_equal(chunked.hplus.cpu().numpy(), hp)
+
+
+def test_single_binary_polarizations(cuda):
+    """polarizations() and polarizations_on_grid() (src/waveformx/_api.py:
+    156-177): one binary from keyword arguments, FP64, against the oracle;
+    options as keywords; LAL's checks raised with the reference's messages."""
+    import torch
+
+    import waveformx_amd as wx
+
+    f = np.linspace(20.0, 1024.0, 2000)
+    kw = dict(mass_1=36.0, mass_2=29.0, spin_1=(0.0, 0.0, 0.3), spin_2=(0.0, 0.0, -0.4), luminosity_distance=500.0,
+              inclination=0.7, phi_ref=1.2, f_ref=25.0)
+    hp, hc = wx.polarizations("IMRPhenomXAS", f, **kw)
+    assert hp.dtype == torch.complex128 and hp.shape == (2000,) and hp.device.type == "cuda"
+    op, oc = oracle.polarizations(np.array([[36.0, 29.0, 0.3, -0.4, 500.0, 0.7, 1.2, 25.0]]), f)
+    assert compare(hp.cpu().numpy(), op[0])[0] < 1e-12 and compare(hc.cpu().numpy(), oc[0])[0] < 1e-12
+    gp, gc = wx.polarizations_on_grid("IMRPhenomXAS", wx.FrequencyGrid(f), **kw)
+    assert torch.equal(gp, hp) and torch.equal(gc, hc)
+    raw, _ = wx.polarizations("IMRPhenomXAS", f, scaled_strain=False, **kw)
+    assert torch.allclose(raw * 1e21, hp, rtol=1e-14, atol=0)
+    with pytest.raises(wx.InvalidParameterError, match="in-plane"):
+        wx.polarizations("IMRPhenomXAS", f, **dict(kw, spin_1=(0.1, 0.0, 0.3)))
+    with pytest.raises(wx.InvalidParameterError, match="positive"):
+        wx.polarizations("IMRPhenomXAS", f, **dict(kw, luminosity_distance=-1.0))
+    dp, _ = wx.polarizations("IMRPhenomD", f, **kw)
+    assert dp.dtype == torch.complex128 and bool(torch.isfinite(torch.view_as_real(dp)).all())

@@ -164,3 +164,23 @@ def test_torch_function_input_errors():
         # there is no CPU path: a call without a HIP device raises
         with pytest.raises(RuntimeError, match="HIP device"):
             f(wx.Binary(30.0, 20.0, (0, 0, 0), (0, 0, 0)), torch.tensor([20.0, 30.0]))
+
+
+def test_registry_queries_and_single_binary_argument_errors():
+    """is_precessing / is_frequency_domain over the reference's registry
+    (src/waveformx/_api.py:79-84), and polarizations()'s keyword errors,
+    raised on the host before any device work (_binary.py:82-106)."""
+    import waveformx_amd as wx
+
+    assert wx.is_precessing("IMRPhenomXPNR") and not wx.is_precessing("IMRPhenomXAS")
+    assert wx.is_frequency_domain("IMRPhenomD") and not wx.is_frequency_domain("NRSur7dq4")
+    with pytest.raises(wx.UnknownApproximantError):
+        wx.is_precessing("IMRPhenomXYZ")
+    with pytest.raises(TypeError, match="mass_2"):
+        wx.polarizations("IMRPhenomXAS", [20.0, 30.0], mass_1=30.0)
+    with pytest.raises(TypeError, match="spin_1"):
+        wx.polarizations("IMRPhenomXAS", [20.0, 30.0], mass_1=30.0, mass_2=20.0, spin_1=(0.0, 0.1))
+    with pytest.raises(TypeError, match="unexpected keyword"):
+        wx.polarizations("IMRPhenomXAS", [20.0, 30.0], mass_1=30.0, mass_2=20.0, colour="blue")
+    with pytest.raises(wx.UnsupportedError):
+        wx.polarizations("IMRPhenomXHM", [20.0, 30.0], mass_1=30.0, mass_2=20.0)

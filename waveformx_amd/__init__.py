@@ -8,14 +8,16 @@ by hand-written HIP kernels for gfx950 (``libwfx_xas.so``, C ABI in
 ``include/waveformx_amd.h``). There is no CPU path.
 """
 
-from ._api import APPROXIMANTS, approximants, batch_polarizations, compute_device, ragged_polarizations
+from ._api import (APPROXIMANTS, approximants, batch_polarizations, compute_device, is_frequency_domain,
+                   is_precessing, polarizations, polarizations_on_grid, ragged_polarizations)
 from ._types import (BINARY_NAMES, COLUMNS, REFERENCE_MF, BatchPolarizations, Binary, FrequencyGrid,
                      WaveformOptions, check_frequencies, parameter_table)
 from .exceptions import (InvalidParameterError, ModelError, UnknownApproximantError, UnsupportedError,
                          WaveformError)
 
 __all__ = [
-    "APPROXIMANTS", "approximants", "batch_polarizations", "ragged_polarizations", "compute_device", "Binary",
+    "APPROXIMANTS", "approximants", "is_precessing", "is_frequency_domain", "polarizations", "polarizations_on_grid",
+    "batch_polarizations", "ragged_polarizations", "compute_device", "Binary",
     "BINARY_NAMES", "COLUMNS", "REFERENCE_MF", "BatchPolarizations", "FrequencyGrid", "WaveformOptions",
     "check_frequencies", "parameter_table", "WaveformError", "InvalidParameterError", "UnsupportedError",
     "UnknownApproximantError", "ModelError", "polarizations_function",

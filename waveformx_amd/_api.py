@@ -1,7 +1,9 @@
 """The public functions of the drop-in boundary: the approximant registry
-(``src/waveformx/models/__init__.py:62-84``), :func:`batch_polarizations`
-(``src/waveformx/_api.py:307-360``) and :func:`ragged_polarizations` (each
-binary on its own grid, which the reference does not have).
+(``src/waveformx/models/__init__.py:62-84``, ``_api.py:65-84``),
+:func:`polarizations` and :func:`polarizations_on_grid` (one binary,
+``_api.py:156-177``), :func:`batch_polarizations` (``_api.py:307-360``)
+and :func:`ragged_polarizations` (each binary on its own grid, which the
+reference does not have).
 
 IMRPhenomXAS and IMRPhenomD are built here; the reference's other names
 raise :class:`UnsupportedError`, unknown names
@@ -10,17 +12,18 @@ raise :class:`UnsupportedError`, unknown names
 
 from __future__ import annotations
 
-from typing import Any, Optional, Sequence, Tuple
+from typing import Any, Mapping, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
 
 from . import phenomd, xas
-from ._types import (BatchPolarizations, FrequencyGrid, WaveformOptions, check_table, parameter_table,
-                     parse_options, strain_scale)
+from ._types import (BINARY_NAMES, BatchPolarizations, Binary, FrequencyGrid, WaveformOptions, check_frequencies,
+                     check_table, parameter_table, parse_options, strain_scale)
 from .exceptions import InvalidParameterError, ModelError, UnknownApproximantError, UnsupportedError
 
-__all__ = ["APPROXIMANTS", "approximants", "batch_polarizations", "ragged_polarizations", "compute_device"]
+__all__ = ["APPROXIMANTS", "approximants", "is_precessing", "is_frequency_domain", "polarizations",
+           "polarizations_on_grid", "batch_polarizations", "ragged_polarizations", "compute_device"]
 
 #: approximant -> (domain, precessing, implemented here)
 APPROXIMANTS = {
@@ -42,12 +45,27 @@ def approximants(domain: Optional[str] = None) -> list:
     return [n for n, (d, _, built) in APPROXIMANTS.items() if built and domain in (None, d)]
 
 
-def model(approximant: str, domain: str = "frequency") -> str:
-    """Check that ``approximant`` is built here for ``domain``."""
+def _entry(approximant: str) -> Tuple[str, bool, bool]:
     if not isinstance(approximant, str) or approximant not in APPROXIMANTS:
         raise UnknownApproximantError(f"unknown approximant {approximant!r}: waveformx implements "
                                       f"{', '.join(APPROXIMANTS)}")
-    d, _, built = APPROXIMANTS[approximant]
+    return APPROXIMANTS[approximant]
+
+
+def is_precessing(approximant: str) -> bool:
+    """Whether the approximant is precessing (``_api.py:79-80``; any name of
+    the registry, built here or not)."""
+    return _entry(approximant)[1]
+
+
+def is_frequency_domain(approximant: str) -> bool:
+    """Whether the approximant is a frequency-domain one (``_api.py:83-84``)."""
+    return _entry(approximant)[0] == "frequency"
+
+
+def model(approximant: str, domain: str = "frequency") -> str:
+    """Check that ``approximant`` is built here for ``domain``."""
+    d, _, built = _entry(approximant)
     if d != domain:
         what = "frequency-domain" if domain == "frequency" else "time-domain"
         raise UnsupportedError(f"{approximant} has no {what} waveforms")
@@ -87,6 +105,60 @@ def _reference_mf_errors(table: np.ndarray, options: WaveformOptions, errors: di
 def _table_params(table: np.ndarray, device: torch.device) -> torch.Tensor:
     cols = table[:, [0, 1, 4, 7, 8, 9, 10, 11]]
     return torch.as_tensor(np.ascontiguousarray(cols), dtype=torch.float64, device=device)
+
+
+def _binary_row(keywords: Mapping[str, Any]) -> np.ndarray:
+    """A one-row table of the binary keyword arguments, with the errors of
+    ``binary_from_keywords`` (``src/waveformx/_binary.py:82-106``)."""
+    for name in ("mass_1", "mass_2"):
+        if name not in keywords:
+            raise TypeError(f"missing required keyword argument {name!r}")
+
+    def real(name: str) -> float:
+        value = keywords.get(name, Binary._field_defaults.get(name))
+        try:
+            return float(value)
+        except (TypeError, ValueError):
+            raise TypeError(f"{name} must be a real number, got {type(value).__name__}") from None
+
+    def spin(name: str) -> list:
+        try:
+            value = np.asarray(keywords.get(name, (0.0, 0.0, 0.0)), dtype=float)
+        except (TypeError, ValueError):
+            raise TypeError(f"{name} must be a sequence of three real numbers (x, y, z)") from None
+        if value.shape != (3,):
+            raise TypeError(f"{name} must be a sequence of three real numbers (x, y, z)")
+        return list(value)
+
+    row = [real("mass_1"), real("mass_2"), *spin("spin_1"), *spin("spin_2"), real("luminosity_distance"),
+           real("inclination"), real("phi_ref"), real("f_ref")]
+    return np.asarray(row, dtype=float)[None, :]
+
+
+def polarizations(approximant: str, frequencies: Any, *, options: Optional[WaveformOptions] = None,
+                  device: Optional[torch.device] = None, **keywords) -> Tuple[torch.Tensor, torch.Tensor]:
+    """``(h_+, h_x)`` of one binary (keyword arguments ``mass_1``, ``mass_2``,
+    ``spin_1``, ... and options) at ``frequencies`` (Hz), FP64 on the GPU,
+    as complex128 device tensors (``src/waveformx/_api.py:156-170``). A
+    binary that fails LALSimulation's checks raises
+    :class:`InvalidParameterError` with the reference's message."""
+    model(approximant)
+    binary = {k: v for k, v in keywords.items() if k in BINARY_NAMES}
+    option_keywords = {k: v for k, v in keywords.items() if k not in BINARY_NAMES}
+    row = _binary_row(binary)
+    options = parse_options(options, option_keywords)
+    frequencies = check_frequencies(frequencies)
+    batch = batch_polarizations(approximant, frequencies, row, strict=True, options=options, device=device)
+    return batch.hplus[0], batch.hcross[0]
+
+
+def polarizations_on_grid(approximant: str, grid: Any, *, options: Optional[WaveformOptions] = None,
+                          device: Optional[torch.device] = None, **keywords) -> Tuple[torch.Tensor, torch.Tensor]:
+    """:func:`polarizations` on a :class:`FrequencyGrid` (or frequencies)
+    (``src/waveformx/_api.py:173-177``)."""
+    if not isinstance(grid, FrequencyGrid):
+        grid = FrequencyGrid(grid)
+    return polarizations(approximant, grid.frequencies, options=options, device=device, **keywords)
 
 
 def batch_polarizations(approximant: str, grid: Any, parameters: Any, *, threads: int = 1, strict: bool = False,
