@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (10); a binding checks it before any call. */
+/* The version of this ABI (11); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -174,15 +174,18 @@ int wfx_phd_eval_ragged(const void* records, int64_t n_binaries, const void* fre
  * wfx_xas_setup_jvp, wfx_xas_adjoint and wfx_xas_loss_adjoint (powers from
  * wfx_xas_power_table: both models read the same frequency powers); the
  * region bounds carry no derivative, as the reference's jnp.where
- * selections (phenomd/core.py:268-269, 426-440). Together they replace the
- * reference's jax.vjp backward for this model (src/waveformx/torch.py:108-119);
- * there is no gradient with respect to the grid. Precision 1 fails. */
+ * selections (phenomd/core.py:268-269, 426-440). Together with
+ * wfx_phd_grid_adjoint (dL/df of the shared grid, as wfx_xas_grid_adjoint)
+ * they replace the reference's jax.vjp backward for this model
+ * (src/waveformx/torch.py:108-119). Precision 1 fails. */
 int wfx_phd_setup_jvp(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
                       double amp_scale, double reference_mf, double* records, float* split_records, double* jacobian,
                       uint32_t* flags, void* stream);
 int wfx_phd_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs, int freqs_f64,
                     int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, const void* powers,
                     double* grads, void* stream);
+int wfx_phd_grid_adjoint(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
+                         int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* stream);
 int wfx_phd_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
                          int freqs_f64, int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
                          const void* weights, const void* powers, double* loss, double* grads, void* stream);

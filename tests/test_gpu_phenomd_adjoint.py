@@ -3,9 +3,9 @@ differences of its FP64 oracle (``phenomd_oracle.jacobian``, itself pinned
 by closed forms in test_oracle_phenomd.py): the FP64 path to 1e-7 of each
 parameter's largest gradient, the split FP32 path to the reference's
 float32 gradient bar ``|g32 - g64| <= 1e-2 |g64| + 1e-3 max|g64|``
-(tests/test_transforms_xas.py:58-63), the torch backward, the fused
-noise-weighted loss, the mass-scaled reference frequency, and the records
-the derivative setup writes."""
+(tests/test_transforms_xas.py:58-63), the torch backward (parameters and
+grid), the grid's gradient, the fused noise-weighted loss, the mass-scaled
+reference frequency, and the records the derivative setup writes."""
 
 import numpy as np
 import pytest
@@ -89,9 +89,8 @@ def test_empty_grid_gives_zero_gradients(cuda):
 
 
 def test_torch_backward(cuda):
-    """loss.backward() through the torch function reaches every field, as
-    the reference's tests/test_torch.py:58-96; a grid that requires a
-    gradient is refused (that gradient is IMRPhenomXAS's only)."""
+    """loss.backward() through the torch function reaches every field and
+    the grid, as the reference's tests/test_torch.py:58-96."""
     import torch
 
     import waveformx_amd.torch as wt
@@ -124,10 +123,54 @@ def test_torch_backward(cuda):
     err = relative_to_column(got, expected)
     assert np.all(err < 1e-7), err.max(axis=0)
 
+    # the grid's gradient through the torch function: that of the linear
+    # functional, against central differences in each frequency
     grid = torch.tensor(freqs, device=cuda, requires_grad=True)
-    hp, _ = f(Binary(*fields), grid)
-    with pytest.raises(NotImplementedError):
-        hp.abs().sum().backward()
+    hp, hc = f(Binary(*[x.detach() for x in fields]), grid)
+    loss = (wt_[0] * hp.real + wt_[1] * hp.imag + wt_[2] * hc.real + wt_[3] * hc.imag).sum()
+    (dldf,) = torch.autograd.grad(loss, grid)
+    expected = grid_differences(params, freqs, w)
+    scale = np.max(np.abs(expected))
+    assert np.max(np.abs(dldf.cpu().numpy() - expected)) / scale < 1e-6
+
+
+def grid_differences(params, f, w):
+    """sum over binaries of w . dh/df by central differences of the oracle's
+    formula in each frequency, the tables held at the first frequency (the
+    reference's stop_gradient on f_min)."""
+    fields = pd.table(params, f[0])
+    eps = 1e-6 * f
+    hp_a, hc_a = pd.evaluate(fields, f + eps)
+    hp_b, hc_b = pd.evaluate(fields, f - eps)
+    dhp, dhc = (hp_a - hp_b) / (2 * eps), (hc_a - hc_b) / (2 * eps)
+    return np.sum(w[0] * dhp.real + w[1] * dhp.imag + w[2] * dhc.real + w[3] * dhc.imag, axis=0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_grid_gradient(cuda, precision):
+    """dL/df of the shared grid (IMRPhenomD's grid adjoint) against central
+    differences of the oracle in each frequency."""
+    import torch
+
+    from waveformx_amd import adjoint
+
+    params = pd.draw(6, seed=12)
+    f = np.geomspace(20.0, 1500.0, 700)
+    if precision != "fp64":
+        params = params.astype(np.float32).astype(np.float64)
+        f = f.astype(np.float32).astype(np.float64)
+    w = np.random.default_rng(2).normal(size=(4, 6, len(f)))
+    expected = grid_differences(params, f, w)
+    ctype = torch.complex128 if precision == "fp64" else torch.complex64
+    rtype = torch.float64 if precision == "fp64" else torch.float32
+    gp = torch.as_tensor(w[0] + 1j * w[1], dtype=ctype, device=cuda)
+    gc = torch.as_tensor(w[2] + 1j * w[3], dtype=ctype, device=cuda)
+    _, grid = adjoint.vjp(torch.as_tensor(params, device=cuda), torch.as_tensor(f, dtype=rtype, device=cuda), gp, gc,
+                          1e21, None, precision, wrt_grid=True, approximant="IMRPhenomD")
+    grid = grid.cpu().numpy()
+    scale = np.max(np.abs(expected))
+    tol = 1e-6 if precision == "fp64" else 2e-3
+    assert np.max(np.abs(grid - expected)) / scale < tol, np.max(np.abs(grid - expected)) / scale
 
 
 @pytest.mark.parametrize("precision", ["fp64", "split"])

@@ -13,9 +13,8 @@ least-squares loss ``L = sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2)``: it reads
 the data and writes only each binary's loss and gradient.
 
 Both models share the kernels' shape (one wave per binary, the same
-reduce-scatter and contraction); each has its own per-bin adjoint and its
-own entry points. The gradient with respect to the grid (``wrt_grid``) is
-IMRPhenomXAS's only.
+reduce-scatter and contraction; one thread per frequency for the grid's
+gradient); each has its own per-bin adjoint and its own entry points.
 """
 
 from __future__ import annotations
@@ -32,14 +31,15 @@ __all__ = ["vjp", "adjoint", "loss_and_grad", "PARAMETERS", "MODELS"]
 
 PARAMETERS = ("mass_1", "mass_2", "chi_1", "chi_2", "luminosity_distance", "inclination", "phi_ref", "f_ref")
 
-#: approximant -> (derivative setup, adjoint, fused loss adjoint) entry points
+#: approximant -> (derivative setup, adjoint, fused loss adjoint, grid
+#: adjoint) entry points
 MODELS = {
-    "IMRPhenomXAS": ("wfx_xas_setup_jvp", "wfx_xas_adjoint", "wfx_xas_loss_adjoint"),
-    "IMRPhenomD": ("wfx_phd_setup_jvp", "wfx_phd_adjoint", "wfx_phd_loss_adjoint"),
+    "IMRPhenomXAS": ("wfx_xas_setup_jvp", "wfx_xas_adjoint", "wfx_xas_loss_adjoint", "wfx_xas_grid_adjoint"),
+    "IMRPhenomD": ("wfx_phd_setup_jvp", "wfx_phd_adjoint", "wfx_phd_loss_adjoint", "wfx_phd_grid_adjoint"),
 }
 
 
-def _entries(approximant: str, precision: str) -> Tuple[str, str, str]:
+def _entries(approximant: str, precision: str) -> Tuple[str, str, str, str]:
     if approximant not in MODELS:
         raise ValueError(f"no adjoint for {approximant!r}; gradients exist for {sorted(MODELS)}")
     if approximant == "IMRPhenomD" and precision not in phenomd.PRECISIONS:
@@ -86,12 +86,14 @@ def adjoint(records: torch.Tensor, jac: torch.Tensor, freqs: torch.Tensor, cot_p
 
 
 def grid_adjoint(records: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor,
-                 cot_cross: torch.Tensor, precision: str) -> torch.Tensor:
-    """dL/df (N,) float64 of the shared grid from the cotangents (B, N)
-    (IMRPhenomXAS records)."""
+                 cot_cross: torch.Tensor, precision: str, approximant: str = "IMRPhenomXAS") -> torch.Tensor:
+    """dL/df (N,) float64 of the shared grid from the cotangents (B, N), the
+    tables held (the first frequency's effect through the setup excluded, as
+    the reference's stop_gradient on f_min)."""
+    entry = _entries(approximant, precision)[3]
     nb, n = records.shape[0], freqs.shape[0]
     dldf = torch.zeros((n,), dtype=torch.float64, device=records.device)
-    call("wfx_xas_grid_adjoint", _ptr(records), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
+    call(entry, _ptr(records), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
          _code(precision), _ptr(cot_plus), _ptr(cot_cross), _ptr(dldf), _stream(records.device))
     return dldf
 
@@ -102,9 +104,6 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, grad_plus: Optional[torch.Ten
     """Gradients (B, 8) with respect to the packed parameters, and (N,)
     with respect to the grid if ``wrt_grid`` (else None)."""
     _entries(approximant, precision)
-    if wrt_grid and approximant != "IMRPhenomXAS":
-        raise NotImplementedError(f"the gradient with respect to the frequencies is IMRPhenomXAS's only, "
-                                  f"not {approximant!r}'s")
     nb, n = params.shape[0], freqs.shape[0]
     ctype = torch.complex128 if precision == "fp64" else torch.complex64
     zeros = None
@@ -113,7 +112,7 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, grad_plus: Optional[torch.Ten
     gp = zeros if grad_plus is None else grad_plus.to(ctype).reshape(nb, n).contiguous()
     gc = zeros if grad_cross is None else grad_cross.to(ctype).reshape(nb, n).contiguous()
     records, jac = _records(params, freqs, amp_scale, reference_mf, precision, approximant)
-    grid = grid_adjoint(records, freqs, gp, gc, precision) if wrt_grid else None
+    grid = grid_adjoint(records, freqs, gp, gc, precision, approximant) if wrt_grid else None
     return adjoint(records, jac, freqs, gp, gc, precision, approximant=approximant), grid
 
 

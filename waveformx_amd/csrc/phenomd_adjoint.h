@@ -153,4 +153,59 @@ __device__ __forceinline__ void pd_adjoint_point(const T* __restrict__ c, const 
     }
 }
 
+// dL/df of one bin (the frequency itself moving; the tables fixed): dL/dA
+// A'(f) + dL/dphi phi'(f), with phi' the derivative of the region's phase
+// (basis terms, the merger-ringdown's arctangent, the inspiral's leading
+// term -5/3 T0 f^(-8/3)) and A' that of the region's amplitude form.
+template <class T>
+__device__ inline T pd_grid_adjoint_point(const T* __restrict__ c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re,
+                                          T gx_im) {
+    const T f34 = pd_f34(p.f);
+    bool inspiral, mrd;
+    T z, amp;
+    const T phi = pd_phase_and_amp(c, p, f34, inspiral, mrd, z, amp);
+    T s, co;
+    m_sincos(phi, &s, &co);
+    const T kp = c[kPdKp], kc = c[kPdKc];
+    const T g_amp = kp * (gp_re * co + gp_im * s) + kc * (gx_im * co - gx_re * s);
+    const T g_phi = -amp * (kp * (gp_re * s - gp_im * co) + kc * (gx_re * co + gx_im * s));
+    const T f = p.f, t = p.t, m1 = p.m1, lf = p.lf;
+    const T m2 = m1 * m1;
+    // d/df of f^a is a f^(a-1); of f^a log f, f^(a-1) (a log f + 1)
+    T dphi;
+    if (inspiral) {
+        const T t2 = t * t, mt1 = t2 * m1;
+        const T third = T(1.0 / 3.0);
+        dphi = -c[kPdInsM1] * m2 - T(2) * third * c[kPdInsM23] * (mt1 * mt1) * m1 - third * c[kPdInsM13] * mt1 * m1 +
+               c[kPdInsLog] * m1 + third * c[kPdInsP13] * t * m1 + c[kPdInsP13L] * t * m1 * (third * lf + T(1)) +
+               T(2) * third * c[kPdInsP23] * t2 * m1 + c[kPdInsP1] + T(4) * third * c[kPdInsP43] * t +
+               T(5) * third * c[kPdInsP53] * t2 + T(2) * c[kPdInsP2] * f;
+        const double t0 = (double)c[kT0] + (double)c[kT0Lo];
+        dphi += (T)(-5.0 / 3.0 * t0 * p.m53 / (double)f);
+    } else if (mrd) {
+        const T wy = c[kPdAtanW] * (f - c[kPdAtanShift]);
+        dphi = c[kPdMrdP1] - c[kPdMrdM1] * m2 + T(0.75) * c[kPdMrdP34] * f34 * m1 +
+               c[kPdAtanA] * c[kPdAtanW] / (T(1) + wy * wy);
+    } else {
+        dphi = c[kPdIntP1] - T(3) * c[kPdIntM3] * m2 * m2 + c[kPdIntLog] * m1;
+    }
+    T damp;
+    if (f < c[kPdFAmpIns]) {
+        // A = f^(-7/6) sum_k c_k t^k: A' = f^(-13/6) sum_k (k/3 - 7/6) c_k t^k
+        T poly = T(9.0 / 3.0 - 7.0 / 6.0) * c[kPdAmpIns + 9];
+        for (int k = 8; k >= 0; --k) poly = poly * t + T(k / 3.0 - 7.0 / 6.0) * c[kPdAmpIns + k];
+        damp = p.m76 * m1 * poly;
+    } else if (f >= c[kPdFAmpMrd]) {
+        const T u = c[kPdAmpRdS] * (f - c[kPdAmpRdShift]);
+        damp = amp * (T(-7.0 / 6.0) * m1 - c[kPdAmpRdGr] * c[kPdAmpRdS] - T(2) * u * c[kPdAmpRdS] / (u * u + T(1)));
+    } else {
+        // A = f^(-7/6) sum_j d_j y^j, y = S f
+        const T y = c[kPdAmpIntS] * f;
+        T dpoly = T(4) * c[kPdAmpInt + 4];
+        for (int j = 3; j >= 1; --j) dpoly = dpoly * y + T(j) * c[kPdAmpInt + j];
+        damp = T(-7.0 / 6.0) * m1 * amp + p.m76 * c[kPdAmpIntS] * dpoly;
+    }
+    return g_amp * damp + g_phi * dphi;
+}
+
 }  // namespace wfx

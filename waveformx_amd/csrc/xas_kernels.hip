@@ -642,6 +642,10 @@ struct XasAdjoint {
                                                    T gx_im, T* acc) {
         adjoint_point(c, p, gp_re, gp_im, gx_re, gx_im, acc);
     }
+    template <class T>
+    __device__ __forceinline__ static T grid(const T* c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re, T gx_im) {
+        return grid_adjoint_point(c, p, gp_re, gp_im, gx_re, gx_im);
+    }
 };
 
 struct PhenomDAdjoint {
@@ -656,6 +660,10 @@ struct PhenomDAdjoint {
     __device__ __forceinline__ static void adjoint(const T* c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re,
                                                    T gx_im, T* acc) {
         pd_adjoint_point(c, p, gp_re, gp_im, gx_re, gx_im, acc);
+    }
+    template <class T>
+    __device__ __forceinline__ static T grid(const T* c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re, T gx_im) {
+        return pd_grid_adjoint_point(c, p, gp_re, gp_im, gx_re, gx_im);
     }
 };
 
@@ -712,7 +720,7 @@ __global__ __launch_bounds__(kAdjointBlock) void adjoint_kernel(const T* __restr
 // (a thread per frequency, a loop over a group of binaries), the bins'
 // contributions summed in FP64 in a register, then added to dldf[i] (one
 // FP64 atomic per group and frequency)
-template <class T, class F>
+template <class A, class T, class F>
 __global__ __launch_bounds__(kBlock) void grid_adjoint_kernel(const T* __restrict__ records, int64_t nb,
                                                               const F* __restrict__ freqs, int64_t n, int64_t ntiles,
                                                               int group, const T* __restrict__ cot_plus,
@@ -728,8 +736,8 @@ __global__ __launch_bounds__(kBlock) void grid_adjoint_kernel(const T* __restric
     double acc = 0.0;
     for (int64_t b = b0; b < b1; ++b) {
         const int64_t o = 2 * (b * n + i);
-        acc += (double)grid_adjoint_point(records + b * kSlots, p, cot_plus[o], cot_plus[o + 1], cot_cross[o],
-                                          cot_cross[o + 1]);
+        acc += (double)A::grid(records + b * kSlots, p, cot_plus[o], cot_plus[o + 1], cot_cross[o],
+                               cot_cross[o + 1]);
     }
     atomicAdd(dldf + i, acc);
 }
@@ -981,6 +989,44 @@ int adjoint(const char* name, const void* records, const double* jacobian, int64
     return check_launch(name);
 }
 
+// dL/df of a shared grid for model A behind its C entry point `name`
+template <class A>
+int grid_adjoint(const char* name, const void* records, int64_t n_binaries, const void* freqs, int freqs_f64,
+                 int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, double* dldf,
+                 void* stream) {
+    const std::string who(name);
+    if (n_binaries < 0 || n_freqs < 0) return fail((who + ": negative size").c_str());
+    if (n_binaries == 0 || n_freqs == 0) return 0;
+    if (!records || !freqs || !cot_plus || !cot_cross || !dldf) return fail((who + ": null pointer").c_str());
+    const int group = 64;
+    const int64_t ntiles = (n_freqs + kBlock - 1) / kBlock;
+    const int blocks = grid_blocks(ntiles * ((n_binaries + group - 1) / group), (who + ": too large").c_str());
+    if (blocks < 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == kPrecF64) {
+        if (freqs_f64)
+            grid_adjoint_kernel<A, double, double><<<blocks, kBlock, 0, s>>>(
+                (const double*)records, n_binaries, (const double*)freqs, n_freqs, ntiles, group,
+                (const double*)cot_plus, (const double*)cot_cross, dldf);
+        else
+            grid_adjoint_kernel<A, double, float><<<blocks, kBlock, 0, s>>>(
+                (const double*)records, n_binaries, (const float*)freqs, n_freqs, ntiles, group,
+                (const double*)cot_plus, (const double*)cot_cross, dldf);
+    } else if (precision == kPrecF32 || precision == kPrecSplit || precision == kPrecSplitDF) {
+        if (freqs_f64)
+            grid_adjoint_kernel<A, float, double><<<blocks, kBlock, 0, s>>>(
+                (const float*)records, n_binaries, (const double*)freqs, n_freqs, ntiles, group,
+                (const float*)cot_plus, (const float*)cot_cross, dldf);
+        else
+            grid_adjoint_kernel<A, float, float><<<blocks, kBlock, 0, s>>>(
+                (const float*)records, n_binaries, (const float*)freqs, n_freqs, ntiles, group,
+                (const float*)cot_plus, (const float*)cot_cross, dldf);
+    } else {
+        return fail((who + ": unknown precision").c_str());
+    }
+    return check_launch(name);
+}
+
 }  // namespace
 
 // ================================================================= C ABI
@@ -990,7 +1036,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 10; }
+int wfx_abi_version(void) { return 11; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -1165,36 +1211,8 @@ int wfx_xas_adjoint(const void* records, const double* jacobian, int64_t n_binar
 
 int wfx_xas_grid_adjoint(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
                          int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* stream) {
-    if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xas_grid_adjoint: negative size");
-    if (n_binaries == 0 || n_freqs == 0) return 0;
-    if (!records || !freqs || !cot_plus || !cot_cross || !dldf) return fail("wfx_xas_grid_adjoint: null pointer");
-    const int group = 64;
-    const int64_t ntiles = (n_freqs + kBlock - 1) / kBlock;
-    const int blocks = grid_blocks(ntiles * ((n_binaries + group - 1) / group), "wfx_xas_grid_adjoint: too large");
-    if (blocks < 0) return -1;
-    hipStream_t s = (hipStream_t)stream;
-    if (precision == kPrecF64) {
-        if (freqs_f64)
-            grid_adjoint_kernel<double, double><<<blocks, kBlock, 0, s>>>(
-                (const double*)records, n_binaries, (const double*)freqs, n_freqs, ntiles, group,
-                (const double*)cot_plus, (const double*)cot_cross, dldf);
-        else
-            grid_adjoint_kernel<double, float><<<blocks, kBlock, 0, s>>>(
-                (const double*)records, n_binaries, (const float*)freqs, n_freqs, ntiles, group,
-                (const double*)cot_plus, (const double*)cot_cross, dldf);
-    } else if (precision == kPrecF32 || precision == kPrecSplit || precision == kPrecSplitDF) {
-        if (freqs_f64)
-            grid_adjoint_kernel<float, double><<<blocks, kBlock, 0, s>>>(
-                (const float*)records, n_binaries, (const double*)freqs, n_freqs, ntiles, group,
-                (const float*)cot_plus, (const float*)cot_cross, dldf);
-        else
-            grid_adjoint_kernel<float, float><<<blocks, kBlock, 0, s>>>(
-                (const float*)records, n_binaries, (const float*)freqs, n_freqs, ntiles, group,
-                (const float*)cot_plus, (const float*)cot_cross, dldf);
-    } else {
-        return fail("wfx_xas_grid_adjoint: unknown precision");
-    }
-    return check_launch("wfx_xas_grid_adjoint");
+    return grid_adjoint<XasAdjoint>("wfx_xas_grid_adjoint", records, n_binaries, freqs, freqs_f64, n_freqs, precision,
+                                    cot_plus, cot_cross, dldf, stream);
 }
 
 int wfx_xas_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
@@ -1219,6 +1237,13 @@ int wfx_phd_adjoint(const void* records, const double* jacobian, int64_t n_binar
     return adjoint<PhenomDAdjoint, kFromCotangents>("wfx_phd_adjoint", records, jacobian, n_binaries, freqs,
                                                     freqs_f64, n_freqs, precision, cot_plus, cot_cross, nullptr,
                                                     powers, nullptr, grads, stream);
+}
+
+int wfx_phd_grid_adjoint(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
+                         int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* stream) {
+    if (precision == kPrecF32) return fail("wfx_phd_grid_adjoint: IMRPhenomD has no FP32-lead records");
+    return grid_adjoint<PhenomDAdjoint>("wfx_phd_grid_adjoint", records, n_binaries, freqs, freqs_f64, n_freqs,
+                                        precision, cot_plus, cot_cross, dldf, stream);
 }
 
 int wfx_phd_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,

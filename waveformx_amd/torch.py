@@ -23,10 +23,8 @@ cotangents conjugated as in ``torch.py:108-119``) runs the model's
 hand-written adjoint kernel: it recomputes the formula per bin, reduces the
 cotangents into derivatives of each binary's record, and contracts them
 with the record's forward-mode derivatives from the FP64 setup. Gradients
-reach every field (the in-plane spins get zero); the grid's gradient is
-IMRPhenomXAS's only (IMRPhenomD raises ``NotImplementedError`` for a grid
-that requires one). As in the reference, the function is opaque to
-``torch.compile`` and ``torch.func``.
+reach every field (the in-plane spins get zero) and the grid. As in the
+reference, the function is opaque to ``torch.compile`` and ``torch.func``.
 """
 
 from __future__ import annotations
