@@ -68,4 +68,5 @@ def call(name: str, *args) -> None:
     non-zero status."""
     status = getattr(lib(), name)(*args)
     if status != 0:
-        raise RuntimeError(f"{name}: {lib().wfx_last_error().decode()}")
+        message = lib().wfx_last_error().decode()
+        raise RuntimeError(message if message.startswith(name) else f"{name}: {message}")

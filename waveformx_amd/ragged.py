@@ -54,7 +54,7 @@ def _store_cache(cache: dict) -> None:
     try:
         path = _cache_path()
         path.parent.mkdir(parents=True, exist_ok=True)
-        tmp = path.with_suffix(".tmp")
+        tmp = path.with_name(f"{path.name}.{os.getpid()}.tmp")  # one per writer: concurrent tuners
         tmp.write_text(json.dumps(cache, indent=1, sort_keys=True))
         os.replace(tmp, path)
     except OSError:
