@@ -184,3 +184,15 @@ def test_registry_queries_and_single_binary_argument_errors():
         wx.polarizations("IMRPhenomXAS", [20.0, 30.0], mass_1=30.0, mass_2=20.0, colour="blue")
     with pytest.raises(wx.UnsupportedError):
         wx.polarizations("IMRPhenomXHM", [20.0, 30.0], mass_1=30.0, mass_2=20.0)
+    # parameter and grid errors come from the host checks, GPU or not
+    # (the reference's tests/test_api.py:64-80)
+    with pytest.raises(wx.InvalidParameterError, match="masses"):
+        wx.polarizations("IMRPhenomXAS", [20.0, 30.0], mass_1=-30.0, mass_2=20.0)
+    with pytest.raises(wx.InvalidParameterError, match="spin_1"):
+        wx.polarizations("IMRPhenomXAS", [20.0, 30.0], mass_1=30.0, mass_2=20.0, spin_1=(0.9, 0.9, 0.0))
+    with pytest.raises(wx.InvalidParameterError):
+        wx.polarizations("IMRPhenomXAS", [30.0, 20.0], mass_1=30.0, mass_2=20.0)
+    with pytest.raises(wx.InvalidParameterError, match="in-plane"):
+        wx.polarizations("IMRPhenomD", [20.0, 30.0], mass_1=30.0, mass_2=20.0, spin_2=(0.0, 0.2, 0.3))
+    with pytest.raises(ValueError):
+        wx.polarizations("SEOBNRv4_ROM", [20.0, 30.0], mass_1=1.4, mass_2=1.4)

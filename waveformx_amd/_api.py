@@ -147,7 +147,13 @@ def polarizations(approximant: str, frequencies: Any, *, options: Optional[Wavef
     option_keywords = {k: v for k, v in keywords.items() if k not in BINARY_NAMES}
     row = _binary_row(binary)
     options = parse_options(options, option_keywords)
+    # the parameter checks on the host, before any device work (check_binary,
+    # _binary.py:108-126, then the grid and the model's checks)
+    errors = check_table(row, approximant)
     frequencies = check_frequencies(frequencies)
+    _reference_mf_errors(row, options, errors)
+    if errors:
+        raise errors[0]
     batch = batch_polarizations(approximant, frequencies, row, strict=True, options=options, device=device)
     return batch.hplus[0], batch.hcross[0]
 
