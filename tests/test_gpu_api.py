@@ -141,3 +141,49 @@ def test_single_binary_polarizations(cuda):
         wx.polarizations("IMRPhenomXAS", f, **dict(kw, luminosity_distance=-1.0))
     dp, _ = wx.polarizations("IMRPhenomD", f, **kw)
     assert dp.dtype == torch.complex128 and bool(torch.isfinite(torch.view_as_real(dp)).all())
+
+
+@pytest.mark.parametrize("approximant", ["IMRPhenomXAS", "IMRPhenomD"])
+def test_gradcheck(cuda, approximant):
+    """torch.autograd.gradcheck of the torch function with respect to the
+    grid and every field (the reference's tests/test_torch.py:108-116): the
+    hand-written backward, conjugation convention included, against torch's
+    own finite differences. The grid's first frequency also sets the tables
+    (f_min), whose effect the gradient leaves out as the reference's
+    stop_gradient does: f_ref is given, so f_min enters nothing but the
+    grid."""
+    import torch
+
+    import waveformx_amd.torch as wt
+
+    f = wt.polarizations_function(approximant)
+    frequencies = torch.linspace(20.0, 200.0, 20, dtype=torch.float64, device=cuda, requires_grad=True)
+    row = np.asarray(ALIGNED)
+    values = [row[0], row[1], row[2:5], row[5:8], row[8], row[9], row[10], row[11]]
+    fields = [torch.tensor(v, dtype=torch.float64, device=cuda, requires_grad=True) for v in values]
+
+    def function(frequencies, *fields):
+        return f(tuple(fields), frequencies)
+
+    assert torch.autograd.gradcheck(function, (frequencies, *fields))
+
+
+def test_gradient_reaches_a_tensor_inside_a_spin(cuda):
+    """A tensor inside a spin sequence gets its gradient (the reference's
+    tests/test_torch.py:119-131)."""
+    import torch
+
+    import waveformx_amd.torch as wt
+
+    f = wt.polarizations_function("IMRPhenomXAS")
+    chi = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
+    binary = dict(mass_1=36.0, mass_2=29, spin_1=(0.0, 0.0, chi), spin_2=[0.0, 0.0, -0.4],
+                  luminosity_distance=500.0, inclination=0.7, phi_ref=1.2, f_ref=25.0)
+    hp, _ = f(binary, FREQUENCIES)
+    (grad,) = torch.autograd.grad(hp.abs().sum(), chi)
+    assert torch.isfinite(grad) and grad != 0
+    # against a central difference of the same function
+    h = 1e-6
+    up = f({**binary, "spin_1": (0.0, 0.0, 0.3 + h)}, FREQUENCIES)[0].abs().sum()
+    down = f({**binary, "spin_1": (0.0, 0.0, 0.3 - h)}, FREQUENCIES)[0].abs().sum()
+    assert abs(float(grad) - float(up - down) / (2 * h)) <= 1e-5 * abs(float(grad))
