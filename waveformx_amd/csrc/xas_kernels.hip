@@ -68,6 +68,21 @@ __device__ inline int64_t xcd_work_id(int64_t block, int64_t nblocks) {
     return (block & 7) * (full >> 3) + (block >> 3);
 }
 
+// the same for output-bound launches: the XCDs take turns at consecutive
+// chunks of `chunk` work ids (XCD x the x-th chunk of each round of 8), so
+// that the blocks running at one time on all XCDs work on one window of the
+// output, while each chunk's neighbouring work stays on one XCD; the last,
+// partial round splits evenly. A bijection of [0, nblocks).
+__device__ inline int64_t xcd_chunk_work_id(int64_t block, int64_t nblocks, int64_t chunk) {
+    const int64_t full = nblocks & ~int64_t(7);
+    if (block >= full) return block;
+    const int64_t xcd = block & 7, k = block >> 3, per = full >> 3;
+    const int64_t round = k / chunk, within = k % chunk;
+    const int64_t rounds = per / chunk;
+    if (round < rounds) return round * 8 * chunk + xcd * chunk + within;
+    return rounds * 8 * chunk + xcd * (per - rounds * chunk) + within;
+}
+
 // ------------------------------------------------------------------ setup
 
 template <class R>
@@ -313,6 +328,12 @@ __global__ __launch_bounds__(kBlock) void eval_f64_shared(const double* __restri
     }
 }
 
+// work ids per XCD chunk of the FP32 shared-grid evaluation (measured at
+// 65,536 x 4017, 4 binaries x 4 frequencies per thread: 0.741-0.747 ms,
+// chunks of 32 and 64 0.746-0.750 ms, one contiguous range per XCD
+// 0.768-0.774 ms)
+constexpr int64_t kWorkChunk = 128;
+
 // shared grid, FP32 paths: a block owns kBlock * FPT frequencies (thread t
 // takes t, t + kBlock, ...) and a group of binaries; each binary's record
 // is read once per iteration (scalar loads) and serves FPT bins per thread.
@@ -325,13 +346,15 @@ __global__ __launch_bounds__(kBlock) void eval_f64_shared(const double* __restri
 // g + ngroups, g + 2 ngroups, ... (ngroups = ceil(B / group)): the XCD's
 // blocks, which hold consecutive groups, write consecutive rows at each
 // step of their loops, where groups of consecutive binaries would write
-// rows `group` apart. 4 binaries x 4 frequencies per thread balances them.
+// rows `group` apart; and the XCDs take turns at chunks of work
+// (xcd_chunk_work_id), so that all of them write near one another. 4
+// binaries x 4 frequencies per thread balances the three costs.
 template <class M, int L, int FPT, class F>
 __global__ __launch_bounds__(kBlock) void eval_f32_shared(const float* __restrict__ records, int64_t nb,
                                                           const F* __restrict__ freqs, int64_t n, int64_t ntiles,
                                                           int group, float2* __restrict__ hp,
                                                           float2* __restrict__ hc) {
-    const int64_t w = xcd_work_id(blockIdx.x, gridDim.x);
+    const int64_t w = xcd_chunk_work_id(blockIdx.x, gridDim.x, kWorkChunk);
     const int64_t tile = w % ntiles;
     const int64_t g = w / ntiles;
     const int64_t i0 = tile * (kBlock * FPT) + threadIdx.x;
