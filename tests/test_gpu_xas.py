@@ -197,3 +197,41 @@ def test_launch_shapes_agree(xas, cuda, precision):
         hp, hc = xas.evaluate(records, f, precision, group, freqs_per_thread=fpt)
         assert torch.equal(hp.view(torch.int32), ref[0]), (group, fpt)
         assert torch.equal(hc.view(torch.int32), ref[1]), (group, fpt)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_degenerate_sizes(xas, cuda, precision):
+    """One binary at one frequency, two frequencies, a group larger than the
+    batch, and empty batches and grids (nothing launched, nothing written),
+    for both models; every shape against the oracle or the larger call."""
+    import torch
+
+    from oracle import phenomd_oracle as pd
+    from waveformx_amd import phenomd
+
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    params = oracle.draw(3, seed=31)
+    p = torch.as_tensor(params, device=cuda)
+    for module, reference in ((xas, oracle), (phenomd, pd)):
+        for n in (1, 2, 257):
+            f = np.linspace(25.0, 400.0, n)
+            ft = torch.as_tensor(f, dtype=dtype, device=cuda)
+            records, _ = module.setup(p, ft[:1], 1e21, None, precision)
+            full = module.evaluate(records, ft, precision)
+            for group in (1, 16):
+                hp, hc = module.evaluate(records, ft, precision, group)
+                assert torch.equal(hp, full[0]) and torch.equal(hc, full[1])
+            one, _ = module.evaluate(records[:1], ft, precision)
+            assert torch.equal(one[0], full[0][0])
+            op, _ = reference.polarizations(params, f if precision == "fp64" else f.astype(np.float32).astype(float))
+            got = full[0].cpu().numpy().astype(np.complex128)
+            if precision == "fp64":
+                assert np.max(np.abs(got - op)) <= 1e-12 * np.max(np.abs(op))
+            else:
+                assert np.max(np.abs(got - op)) <= 1e-3 * np.max(np.abs(op))
+        ft = torch.as_tensor([30.0, 40.0], dtype=dtype, device=cuda)
+        records, _ = module.setup(p, ft[:1], 1e21, None, precision)
+        empty_grid = module.evaluate(records, ft[:0], precision)
+        assert empty_grid[0].shape == (3, 0)
+        empty_batch = module.evaluate(records[:0], ft, precision)
+        assert empty_batch[0].shape == (0, 2)
