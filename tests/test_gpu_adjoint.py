@@ -198,3 +198,51 @@ def test_jacobian_setup_writes_the_forward_records(cuda, precision):
                        expected.view(torch.int32 if precision != "fp64" else torch.int64))
     assert torch.equal(flags, jflags)
     assert jac.shape == (200, 64, 8) and bool(torch.isfinite(jac[:, :60]).all())
+
+
+def oracle_vjp_differences(params, freqs, w, reference_mf, columns=range(7), step=1e-5):
+    """sum_f w . dh/dparam by fourth-order central differences of the FP64
+    oracle (step * max(|x|, 1))."""
+    out = np.zeros((len(params), 8))
+    for k in columns:
+        h = step * np.maximum(np.abs(params[:, k]), 1.0)
+        taps = []
+        for m in (-2, -1, 1, 2):
+            p = params.copy()
+            p[:, k] += m * h
+            taps.append(oracle.polarizations(p, freqs, reference_mf=reference_mf))
+        for j, (a, b) in enumerate(((0, 1), (2, 3))):
+            d = (taps[0][j] - 8 * taps[1][j] + 8 * taps[2][j] - taps[3][j]) / (12 * h[:, None])
+            out[:, k] += np.sum(w[a] * d.real + w[b] * d.imag, axis=1)
+    return out
+
+
+def test_reference_mf_gradient(cuda):
+    """The gradient with the mass-scaled reference frequency (f_ref zero:
+    f_ref = reference_mf / M, whose mass dependence the gradient carries),
+    against central differences of the oracle; a zero derivative with
+    respect to f_ref, also for a binary given with f_ref (NaN values), as the
+    reference's tests/test_transforms_xas.py:89-111."""
+    import torch
+
+    from waveformx_amd import adjoint
+
+    params = oracle.draw(6, seed=19)
+    params[:, 7] = 0.0
+    w = np.random.default_rng(7).normal(size=(4, 6, len(GRID)))
+    gp = torch.as_tensor(w[0] + 1j * w[1], device=cuda)
+    gc = torch.as_tensor(w[2] + 1j * w[3], device=cuda)
+    f = torch.as_tensor(GRID, device=cuda)
+    g, _ = adjoint.vjp(torch.as_tensor(params, device=cuda), f, gp, gc, 1e21, 0.01, "fp64")
+    g = g.cpu().numpy()
+    expected = oracle_vjp_differences(params, GRID, w, 0.01)
+    scale = np.max(np.abs(expected[:, :7]), axis=0)
+    err = np.abs(g[:, :7] - expected[:, :7]) / scale
+    assert np.all(err < 1e-6), err.max(axis=0)
+    assert np.all(g[:, 7] == 0.0)
+    bad = params.copy()
+    bad[:, 7] = 20.0
+    gb, _ = adjoint.vjp(torch.as_tensor(bad, device=cuda), f, gp, gc, 1e21, 0.01, "fp64")
+    assert np.all(gb.cpu().numpy()[:, 7] == 0.0)
+    hp, _ = oracle.polarizations(bad, GRID, reference_mf=0.01)
+    assert np.isnan(hp).all()

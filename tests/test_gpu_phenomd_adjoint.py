@@ -62,6 +62,11 @@ def test_fp64_adjoint_matches_differences(cuda, reference_mf):
     assert np.all(err < 1e-7), err.max(axis=0)
     if reference_mf is not None:
         assert np.all(g[:, 7] == 0.0)  # f_ref = 0 selects the mass-scaled reference
+        # given with f_ref (NaN values), still no derivative with respect to
+        # it (tests/test_transforms_phenomd.py:91-109 of the reference)
+        bad = params.copy()
+        bad[:, 7] = 20.0
+        assert np.all(device_vjp(cuda, bad, GRID, w, "fp64", reference_mf)[:, 7] == 0.0)
 
 
 def test_split_adjoint_within_float32_bar(cuda):

@@ -201,6 +201,12 @@ __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict_
     const int64_t b = gid >> 3;
     const int dir = (int)(gid & 7);
     if (b >= nb) return;
+    if (dir == kFRef && reference_mf == reference_mf) {
+        // with reference_mf the given f_ref only decides whether the binary
+        // is valid (zero, else NaN): it has no derivative (xas/model.py:35-45)
+        for (int k = 0; k < kSlots; ++k) jac[(b * kSlots + k) * 8 + dir] = 0.0;
+        return;
+    }
     using D = Dual<1>;
     D p[kParams];
     for (int k = 0; k < kParams; ++k) {
@@ -598,8 +604,11 @@ __device__ inline T reduce_scatter64(T* v, int lane) {
 // the value of parameter lane / 8, in every lane of its group of 8
 __device__ inline double contract8(double s, const double* __restrict__ jac_row, int lane) {
     double p[8];
+    // (an exactly zero derivative contributes nothing, whatever s: as the
+    // reference's derivatives through jnp.where, for a binary whose loss is
+    // NaN)
 #pragma unroll
-    for (int d = 0; d < 8; ++d) p[d] = s * jac_row[d];
+    for (int d = 0; d < 8; ++d) p[d] = jac_row[d] == 0.0 ? 0.0 : s * jac_row[d];
     scatter_level<32, 4>(p, lane);
     scatter_level<16, 2>(p, lane);
     scatter_level<8, 1>(p, lane);
