@@ -31,6 +31,13 @@ HBM_PEAK_GBS = 8000.0
 # x 1.000 + FETCH_SIZE 16,624 KiB x 2.000, the factors that a 1 GiB device
 # copy gives the two counters. Reported only for that workload and shape.
 EVAL_TRAFFIC = {(65536, 4017): 4.251e9}
+# VALU instructions per bin of the same launch (SQ_INSTS_VALU per 64 bins
+# from rocprofv3 --pmc, summary under profiles/; the oracle's op count of the
+# formula gives ~78 FP32 operation slots per bin, transcendentals at 4), and
+# the non-packed FP32 VALU issue rate: 256 CUs x 4 SIMDs x 64 lanes per 4
+# cycles at 2.4 GHz
+EVAL_VALU_PER_BIN = {(65536, 4017): 77.5}
+VALU_PEAK_TOPS = 256 * 4 * 64 / 4 * 2.4e9 / 1e12
 METRIC = "XAS waveforms/s @4017 bins: FP32-split vs FP64 GPU (mismatch<=1e-6); fwd+grad/s"
 
 
@@ -192,6 +199,17 @@ def bench_ragged(torch, xas, dev, nb, steps, warmup):
             "ragged_value": nb / (ms_plan * 1e-3), "ragged_bins_per_s": total / (ms_plan * 1e-3),
             "ragged_per_binary_launches_ms": ms_per, "ragged_padded_ms": ms_pad,
             "ragged_padded_bins": nb * longest, "ragged_shape": [plan.bins_per_thread, plan.blocks]}
+
+
+def valu_roofline(nb, n, ms_eval, group):
+    """The evaluation's VALU instructions per second against the issue rate,
+    for the workload and shape whose instruction count was measured."""
+    per_bin = EVAL_VALU_PER_BIN.get((nb, n)) if group == 0 else None
+    if per_bin is None:
+        return None
+    achieved = per_bin * nb * n / (ms_eval * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "T instructions/s",
+            "frac": achieved / VALU_PEAK_TOPS, "per_bin": per_bin}
 
 
 def host_cpu():
@@ -392,6 +410,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": EVAL_TRAFFIC.get((nb, n)) if args.group == 0 else None},
+        # the same launch against the VALU issue rate (co-limiting: the
+        # arithmetic runs under the output stream)
+        "roofline_valu": valu_roofline(nb, n, ms_eval, args.group),
     }
     del out32, out64, rec
     torch.cuda.empty_cache()
