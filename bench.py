@@ -98,12 +98,14 @@ def accuracy(torch, test, ref, w, rows=4096):
     return worst
 
 
-def bench_grad(torch, xas, dev, nb, n, steps, warmup):
+def bench_grad(torch, model, dev, nb, n, steps, warmup, approximant="IMRPhenomXAS", prefix=""):
     """The fused forward + adjoint of L = sum_f (|h+ - d+|^2 + |hx - dx|^2) / S(f)
     for nb binaries x n bins (FP32-split), d at parameters perturbed by 1%:
     binaries/s of the whole gradient step (the FP64 setup with derivatives,
     which also writes the FP32-split record; the grid's power table; the
-    fused kernel), and its time over the forward step's."""
+    fused kernel), and its time over the forward step's. ``model``: the
+    approximant's module (its setup, evaluate, polarizations); the keys of
+    the result start with ``prefix``."""
     from waveformx_amd import adjoint
 
     freqs = np.linspace(20.0, 1024.0, n)
@@ -113,34 +115,36 @@ def bench_grad(torch, xas, dev, nb, n, steps, warmup):
     perturbed[:, 7] = p[:, 7]
     params = torch.as_tensor(p, device=dev)
     f32 = torch.as_tensor(freqs, dtype=torch.float32, device=dev)
-    d_plus, d_cross = xas.polarizations(torch.as_tensor(perturbed, device=dev), f32, 1e21, None, "split")
+    d_plus, d_cross = model.polarizations(torch.as_tensor(perturbed, device=dev), f32, 1e21, None, "split")
     x = freqs / 100.0
     w = torch.as_tensor(1.0 / (x ** -4 + 1.0 + x ** 2), dtype=torch.float32, device=dev)
     out = (torch.empty((nb, n), dtype=torch.complex64, device=dev),
            torch.empty((nb, n), dtype=torch.complex64, device=dev))
 
     def forward():
-        records, _ = xas.setup(params, f32[:1], 1e21, None, "split")
-        xas.evaluate(records, f32, "split", out=out)
+        records, _ = model.setup(params, f32[:1], 1e21, None, "split")
+        model.evaluate(records, f32, "split", out=out)
 
     def gradient():
-        adjoint.loss_and_grad(params, f32, d_plus, d_cross, w, precision="split")
+        adjoint.loss_and_grad(params, f32, d_plus, d_cross, w, precision="split", approximant=approximant)
 
     ms_fwd = timed(torch, forward, steps, warmup)
     ms_grad = timed(torch, gradient, steps, warmup)
     # the split gradient against the FP64 adjoint (which matches the FP64
     # oracle's forward-mode gradient to 1e-8) at the same grid and data:
     # per parameter, max over binaries of |g - g64| / max over binaries |g64|
-    loss, grads = adjoint.loss_and_grad(params, f32, d_plus, d_cross, w, precision="split")
+    loss, grads = adjoint.loss_and_grad(params, f32, d_plus, d_cross, w, precision="split", approximant=approximant)
     loss64, grads64 = adjoint.loss_and_grad(params, f32.to(torch.float64), d_plus.to(torch.complex128),
-                                            d_cross.to(torch.complex128), w.to(torch.float64), precision="fp64")
+                                            d_cross.to(torch.complex128), w.to(torch.float64), precision="fp64",
+                                            approximant=approximant)
     err = ((grads - grads64).abs().amax(0) / grads64.abs().amax(0))[:7]
     loss_err = float(((loss - loss64).abs() / loss64.abs()).max())
-    return {"fwd_grad_value": nb / (ms_grad * 1e-3), "fwd_grad_ms": ms_grad, "fwd_grad_binaries": nb,
-            "fwd_grad_over_fwd": ms_grad / ms_fwd,
-            "grad_rel_err_vs_fp64": dict(zip(("mass_1", "mass_2", "chi_1", "chi_2", "luminosity_distance",
-                                              "inclination", "phi_ref"), map(float, err.cpu()))),
-            "loss_rel_err_vs_fp64": loss_err}
+    result = {"fwd_grad_value": nb / (ms_grad * 1e-3), "fwd_grad_ms": ms_grad, "fwd_grad_binaries": nb,
+              "fwd_grad_over_fwd": ms_grad / ms_fwd,
+              "grad_rel_err_vs_fp64": dict(zip(("mass_1", "mass_2", "chi_1", "chi_2", "luminosity_distance",
+                                                "inclination", "phi_ref"), map(float, err.cpu()))),
+              "loss_rel_err_vs_fp64": loss_err}
+    return {prefix + k: v for k, v in result.items()}
 
 
 def ragged_grids(nb, seed=5):
@@ -361,6 +365,10 @@ def main():
 
     ms_pd_split = timed(torch, step_pd("split", f32, out32), args.steps, args.warmup)
     ms_pd_fp64 = timed(torch, step_pd("fp64", f64, out64), max(3, args.steps // 2), 1)
+    pd_records, _ = phenomd.setup(params, f32[:1], 1e21, None, "split")
+    ms_pd_eval = timed(torch, lambda: phenomd.evaluate(pd_records, f32, "split", args.group, out=out32), args.steps,
+                       args.warmup)
+    del pd_records
 
     # accuracy of the split path against the FP64 path on the device (which
     # matches the FP64 oracle to 1e-12): on the FP32-rounded grid the split
@@ -404,6 +412,8 @@ def main():
         "setup_ms": ms_setup,
         "phenomd_split_value": nb / (ms_pd_split * 1e-3),
         "phenomd_fp64_value": nb / (ms_pd_fp64 * 1e-3),
+        "phenomd_eval_ms": ms_pd_eval,
+        "phenomd_roofline_frac": 16.0 * nb * n / (ms_pd_eval * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "max_weighted_mismatch_vs_fp64": rounded["mismatch"],
         "accuracy_vs_fp64_rounded_grid": rounded,
         "accuracy_vs_fp64_exact_grid": exact,
@@ -418,6 +428,8 @@ def main():
     torch.cuda.empty_cache()
     if args.grad_binaries:
         result.update(bench_grad(torch, xas, dev, args.grad_binaries, n, args.steps, args.warmup))
+        result.update(bench_grad(torch, phenomd, dev, args.grad_binaries, n, args.steps, args.warmup,
+                                 approximant="IMRPhenomD", prefix="phenomd_"))
     if args.ragged_binaries:
         result.update(bench_ragged(torch, xas, dev, args.ragged_binaries, args.steps, args.warmup))
     if not args.no_cpu:
