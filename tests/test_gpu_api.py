@@ -187,3 +187,27 @@ def test_gradient_reaches_a_tensor_inside_a_spin(cuda):
     up = f({**binary, "spin_1": (0.0, 0.0, 0.3 + h)}, FREQUENCIES)[0].abs().sum()
     down = f({**binary, "spin_1": (0.0, 0.0, 0.3 - h)}, FREQUENCIES)[0].abs().sum()
     assert abs(float(grad) - float(up - down) / (2 * h)) <= 1e-5 * abs(float(grad))
+
+
+def test_model_objects(cuda):
+    """models.get(...).validate raises LAL's errors from the setup kernel's
+    flags (xas/api.py:52-60, phenomd/api.py:33-41 of the reference), and
+    .polarizations is the pure function."""
+    import torch
+
+    import waveformx_amd as wx
+    from waveformx_amd import models
+
+    options = wx.WaveformOptions()
+    f = np.linspace(20.0, 512.0, 300)
+    xas_model, pd_model = models.get("IMRPhenomXAS"), models.get("IMRPhenomD")
+    good = wx.Binary(36.0, 29.0, (0.0, 0.0, 0.3), (0.0, 0.0, -0.4), 500.0, 0.7, 1.2, 25.0)
+    xas_model.validate(good, f, options)
+    pd_model.validate(good, f, options)
+    with pytest.raises(wx.InvalidParameterError, match="mass ratios beyond 1000"):
+        xas_model.validate(good._replace(mass_1=2000.0, mass_2=1.0), f, options)
+    with pytest.raises(wx.InvalidParameterError, match="final spin"):
+        pd_model.validate(good._replace(spin_1=(0.0, 0.0, 1.3), spin_2=(0.0, 0.0, 1.3)), [], options)
+    hp, hc = xas_model.polarizations(good, torch.as_tensor(f, device=cuda), options)
+    op, oc = oracle.polarizations(np.array([[36.0, 29.0, 0.3, -0.4, 500.0, 0.7, 1.2, 25.0]]), f)
+    assert compare(hp.cpu().numpy(), op[0])[0] < 1e-12 and compare(hc.cpu().numpy(), oc[0])[0] < 1e-12

@@ -196,3 +196,29 @@ def test_registry_queries_and_single_binary_argument_errors():
         wx.polarizations("IMRPhenomD", [20.0, 30.0], mass_1=30.0, mass_2=20.0, spin_2=(0.0, 0.2, 0.3))
     with pytest.raises(ValueError):
         wx.polarizations("SEOBNRv4_ROM", [20.0, 30.0], mass_1=1.4, mass_2=1.4)
+
+
+def test_model_contract_on_the_host():
+    """models.get and the MODEL objects' host-side checks
+    (src/waveformx/models/__init__.py:62-100, xas/api.py:42-48)."""
+    from waveformx_amd import models
+
+    xas_model = models.get("IMRPhenomXAS")
+    assert xas_model.domain == "frequency" and not xas_model.precessing
+    assert models.get("IMRPhenomD").approximant == "IMRPhenomD"
+    with pytest.raises(wx.UnknownApproximantError):
+        models.get("IMRPhenomXYZ")
+    with pytest.raises(wx.UnsupportedError):
+        models.get("NRSur7dq4")
+    options = wx.WaveformOptions()
+    with pytest.raises(wx.InvalidParameterError, match="in-plane"):
+        xas_model.validate(wx.Binary(30.0, 20.0, (0.1, 0.0, 0.0), (0.0, 0.0, 0.0)), [20.0, 30.0], options)
+    with pytest.raises(wx.InvalidParameterError, match="f_ref must be zero"):
+        xas_model.validate(wx.Binary(30.0, 20.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), f_ref=20.0), [20.0, 30.0],
+                           wx.WaveformOptions(reference_mf=0.01))
+    xas_model.validate(wx.Binary(30.0, 20.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)), [], options)  # no grid: no flags
+    with pytest.raises(wx.UnsupportedError):
+        xas_model.modes(None, None, options)
+    with pytest.raises(wx.InvalidParameterError, match="does not take"):
+        models.check_reference_mf(wx.Binary(30.0, 20.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)),
+                                  wx.WaveformOptions(reference_mf=0.01), "TaylorF2", supported=False)
