@@ -1,5 +1,6 @@
 """The HIP path at the benchmark's full size, 65,536 binaries x 4017 bins
-(linspace(20, 1024) Hz, the parameter draw of the oracle), through
+(linspace(20, 1024) Hz, the parameter draw of the oracle), both models,
+through
 properties that do not depend on the size: every row finite; the
 split-precision outputs within the reference's float32 bar (noise-weighted
 mismatch <= 1e-6, tests/test_xas.py:76-87) of the FP64 outputs on every
@@ -98,3 +99,51 @@ def test_doubling_the_distance_halves_every_output(batch, cuda, precision):
     far[:, 4] *= 2.0
     hp2, hc2 = xas.polarizations(far, f, precision=precision)
     assert torch.equal(hp2 * 2, hp) and torch.equal(hc2 * 2, hc)
+
+
+@pytest.fixture(scope="module")
+def phenomd_batch(cuda):
+    import torch
+
+    from oracle import phenomd_oracle as pd
+    from waveformx_amd import phenomd
+
+    params = pd.draw(NB, seed=2)
+    return phenomd, pd, params, torch.as_tensor(params, device=cuda), \
+        torch.as_tensor(GRID, dtype=torch.float32, device=cuda)
+
+
+def test_phenomd_at_full_size(phenomd_batch, cuda):
+    """IMRPhenomD at the same size: split within the float32 bar of FP64 on
+    every row, FP64 within 1e-12 of its oracle on rows across the batch,
+    launch shapes bit-identical, 1/d_L exact."""
+    import torch
+
+    phenomd, pd, params, p, f32 = phenomd_batch
+    hp32, hc32 = phenomd.polarizations(p, f32, precision="split")
+    f64r = f32.to(torch.float64)
+    hp64, hc64 = phenomd.polarizations(p, f64r, precision="fp64")
+    for t in (hp32, hc32, hp64, hc64):
+        assert bool(torch.isfinite(torch.view_as_real(t)).all())
+    w = torch.as_tensor(1.0 / psd(GRID.astype(np.float32).astype(np.float64)), device=cuda)
+    worst = max(float(row_mismatch(torch, hp32, hp64, w).max()), float(row_mismatch(torch, hc32, hc64, w).max()))
+    print("IMRPhenomD worst mismatch, split vs FP64, all rows:", worst)
+    assert worst <= 1e-6
+    del hp32, hc32
+    rows = np.unique(np.linspace(0, NB - 1, 48).astype(np.int64))
+    op, oc = pd.polarizations(params[rows], f64r.cpu().numpy())
+    idx = torch.as_tensor(rows, device=cuda)
+    for got, ref in ((hp64, op), (hc64, oc)):
+        g = got[idx].cpu().numpy()
+        err = np.max(np.abs(g - ref), axis=1) / np.max(np.abs(ref), axis=1)
+        assert np.all(err < 1e-12), err.max()
+    del hp64, hc64
+    records, _ = phenomd.setup(p, f32[:1], 1e21, None, "split")
+    a = phenomd.evaluate(records, f32, "split")
+    b = phenomd.evaluate(records, f32, "split", 8, freqs_per_thread=8)
+    assert torch.equal(a[0].view(torch.int32), b[0].view(torch.int32))
+    del b
+    far = p.clone()
+    far[:, 4] *= 2.0
+    hp2, _ = phenomd.polarizations(far, f32, precision="split")
+    assert torch.equal(hp2 * 2, a[0])
