@@ -4,7 +4,6 @@ the approximant registry's errors, and that the C-ABI library loads and
 exports every entry point include/waveformx_amd.h declares (no kernel is
 launched)."""
 
-import ctypes
 import re
 from pathlib import Path
 
