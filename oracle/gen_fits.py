@@ -19,8 +19,15 @@ Python's evaluation order (left to right), ``fma(a, b, c)`` as the unfused
 ``a * b + c`` the reference uses (``xpnr/_math.py:125-137``), and every
 float literal with ``repr`` (which round-trips the exact double).
 
+The IMRPhenomXHM fits (``.../_generated/hm_*.py``) go the same way into
+``oracle/xhm_fits.h``; the final-spin fits of the higher modes
+(``hm_qnm.py``) are written as their numerator and denominator
+coefficients, which the oracle re-expands in powers of ``1 - a_f`` as
+``xhm/reference.py:96-99`` does.
+
 Usage: ``python oracle/gen_fits.py [REFERENCE_ROOT]`` (default
-``/root/reference``); it rewrites ``oracle/xas_fits.h`` next to itself.
+``/root/reference``); it rewrites ``oracle/xas_fits.h`` and
+``oracle/xhm_fits.h`` next to itself.
 """
 
 from __future__ import annotations
@@ -244,10 +251,182 @@ def generate(reference_root: Path) -> str:
     return "\n".join(out) + "\n"
 
 
+# the IMRPhenomXHM (release 122022) fits: module -> (C prefix, functions
+# not emitted: the dispatchers by mode, which the oracle writes as switches)
+HM_MODULES = {
+    "hm_amp_fits_inspiral": ("hai", {"amp_fit"}),
+    "hm_amp_fits_intermediate": ("ham", {"amp_fit"}),
+    "hm_amp_fits_ringdown": ("har", set()),
+    "hm_intermediate": ("hpi", {"phase_fit"}),
+    "hm_inspiral": ("hpn", {"phase_lambda_fit"}),
+    "hm_ringdown": ("hpr", {"phase_32_fit"}),
+}
+HM_QNM = ["fring21", "fring33", "fring32", "fring44", "fdamp21", "fdamp33", "fdamp32", "fdamp44",
+          "re_l2m2lp2", "re_l2m2lp3", "re_l3m2lp2", "re_l3m2lp3", "im_l2m2lp2", "im_l2m2lp3", "im_l3m2lp2",
+          "im_l3m2lp3"]
+# parameters that are not the scalar R
+INT_ARGS = {"mode_tag"}
+
+
+class HmEmitter(Emitter):
+    """The emitter with ``if``/``elif`` on Python ints (``lambda_pn``'s
+    choice by mode) and ``math.log`` of constants."""
+
+    def expr(self, node):
+        if isinstance(node, ast.Call) and isinstance(node.func, ast.Attribute) and \
+                isinstance(node.func.value, ast.Name) and node.func.value.id == "math":
+            assert node.func.attr == "log"
+            return f"std::log({self.expr(node.args[0])})"
+        if isinstance(node, ast.Compare) and isinstance(node.left, ast.Name) and node.left.id in INT_ARGS:
+            (op,), (right,) = node.ops, node.comparators
+            assert isinstance(op, ast.Eq)
+            return f"({node.left.id} == {int(right.value)})"
+        return super().expr(node)
+
+    def statements(self, body, declared, indent):
+        lines = []
+        for stmt in body:
+            if isinstance(stmt, ast.Expr) and isinstance(stmt.value, ast.Constant):
+                continue
+            if isinstance(stmt, ast.Assign):
+                (target,) = stmt.targets
+                value = self.expr(stmt.value)
+                if target.id in declared:
+                    lines.append(f"{indent}{target.id} = {value};")
+                else:
+                    declared.add(target.id)
+                    lines.append(f"{indent}R {target.id} = {value};")
+            elif isinstance(stmt, ast.Return):
+                lines.append(f"{indent}return {self.expr(stmt.value)};")
+            elif isinstance(stmt, ast.If):
+                # (each branch assigns the same names: declare them first)
+                for name in sorted(_assigned(stmt) - declared):
+                    declared.add(name)
+                    lines.append(f"{indent}R {name};")
+                lines.append(f"{indent}if {self.expr(stmt.test)} {{")
+                lines += self.statements(stmt.body, declared, indent + "    ")
+                lines.append(f"{indent}}} else {{")
+                lines += self.statements(stmt.orelse, declared, indent + "    ")
+                lines.append(f"{indent}}}")
+            else:
+                raise NotImplementedError(ast.dump(stmt))
+        return lines
+
+    def function(self, fn):
+        params = [f"{'int' if a.arg in INT_ARGS else STRUCT_ARGS.get(a.arg, 'R')} {a.arg}" for a in fn.args.args]
+        lines = [f"template <class R> static inline R {self.prefix}_{fn.name}({', '.join(params)}) {{"]
+        lines += self.statements(fn.body, set(a.arg for a in fn.args.args), "    ")
+        lines.append("}")
+        return "\n".join(lines)
+
+
+def _assigned(node):
+    return {t.id for n in ast.walk(node) if isinstance(n, ast.Assign) for t in n.targets if isinstance(t, ast.Name)}
+
+
+class _Poly:
+    """Polynomial coefficients (ascending) from a fit's AST: ``fma(c, x, rest)``
+    chains in the powers ``af``, ``x2`` ... of the final spin."""
+
+    def __init__(self, names):
+        self.names = names
+
+    def of(self, node):
+        if isinstance(node, ast.Constant):
+            return {0: float(node.value)}
+        if isinstance(node, ast.UnaryOp) and isinstance(node.op, ast.USub):
+            return {k: -v for k, v in self.of(node.operand).items()}
+        if isinstance(node, ast.Name):
+            return {self.names[node.id]: 1.0}
+        if isinstance(node, ast.Call) and node.func.id == "fma":
+            a, b, c = (self.of(x) for x in node.args)
+            return self.add(self.mul(a, b), c)
+        if isinstance(node, ast.BinOp) and isinstance(node.op, ast.Mult):
+            return self.mul(self.of(node.left), self.of(node.right))
+        raise NotImplementedError(ast.dump(node))
+
+    @staticmethod
+    def mul(a, b):
+        out = {}
+        for i, x in a.items():
+            for j, y in b.items():
+                out[i + j] = out.get(i + j, 0.0) + x * y
+        return out
+
+    @staticmethod
+    def add(a, b):
+        out = dict(a)
+        for k, v in b.items():
+            out[k] = out.get(k, 0.0) + v
+        return out
+
+
+def qnm_ratio(fn):
+    """The numerator and denominator coefficients of a final-spin fit
+    ``num / den`` (``hm_qnm.py``), ascending powers of ``af``."""
+    names = {"af": 1}
+    ret = None
+    for stmt in fn.body:
+        if isinstance(stmt, ast.Assign):
+            name = stmt.targets[0].id
+            names[name] = int(name[1:])
+        elif isinstance(stmt, ast.Return):
+            ret = stmt.value
+    assert isinstance(ret, ast.BinOp) and isinstance(ret.op, ast.Div)
+    poly = _Poly(names)
+    out = []
+    for part in (ret.left, ret.right):
+        c = poly.of(part)
+        out.append([c.get(k, 0.0) for k in range(max(c) + 1)])
+    return out
+
+
+def generate_hm(reference_root: Path) -> str:
+    base = reference_root / "src/waveformx/models/xpnr/_generated"
+    out = [
+        "// GENERATED by oracle/gen_fits.py from the reference's",
+        "// src/waveformx/models/xpnr/_generated/hm_*.py (LALSimulation 6.2.0's IMRPhenomXHM fits via",
+        "// waveformrs; GPL-3.0-or-later data). Oracle only: generic over the scalar R, fma(a, b, c)",
+        "// unfused as a * b + c. Do not edit.",
+        "#pragma once",
+        "",
+    ]
+    qnm = ast.parse((base / "hm_qnm.py").read_text())
+    functions = {n.name: n for n in qnm.body if isinstance(n, ast.FunctionDef)}
+    out.append("// ---- hm_qnm.py: numerator and denominator in ascending powers of af ----")
+    for name in HM_QNM:
+        num, den = qnm_ratio(functions[name])
+        out.append(f"// hm_qnm.py:{functions[name].lineno}")
+        for part, c in (("num", num), ("den", den)):
+            out.append(f"static const double hmq_{name}_{part}[{len(c)}] = {{{', '.join(repr(x) for x in c)}}};")
+    out.append("")
+    for module, (prefix, skip) in HM_MODULES.items():
+        tree = ast.parse((base / f"{module}.py").read_text())
+        functions = {n.name: n for n in tree.body if isinstance(n, ast.FunctionDef)}
+        emitter = HmEmitter(prefix, functions)
+        out.append(f"// ---- {module}.py ----")
+        for name, fn in functions.items():
+            if name in skip:
+                continue
+            out.append(f"// {module}.py:{fn.lineno}")
+            out.append(emitter.function(fn))
+            out.append("")
+    # xhm/reference.py:189: the inverse of the pseudo-PN collocation system
+    # of the inspiral amplitude, (x^p) for x = 0.5, 0.75, 1 and p = 7/3, 8/3, 3
+    import numpy as np
+
+    inverse = np.linalg.inv(np.array([[x ** q for q in (7.0 / 3.0, 8.0 / 3.0, 3.0)] for x in (0.5, 0.75, 1.0)]))
+    rows = ", ".join("{" + ", ".join(repr(float(v)) for v in row) + "}" for row in inverse)
+    out.append("// xhm/reference.py:189 (numpy.linalg.inv in float64)")
+    out.append(f"static const double hm_inspiral_inverse[3][3] = {{{rows}}};")
+    return "\n".join(out) + "\n"
+
+
 def main():
     root = Path(sys.argv[1]) if len(sys.argv) > 1 else Path("/root/reference")
-    text = generate(root)
-    (Path(__file__).resolve().parent / "xas_fits.h").write_text(text)
+    here = Path(__file__).resolve().parent
+    (here / "xas_fits.h").write_text(generate(root))
+    (here / "xhm_fits.h").write_text(generate_hm(root))
 
 
 if __name__ == "__main__":

@@ -256,7 +256,7 @@ static const Powers<double>& lalpi() {
 template <class R>
 struct WfVars {
     R m1, m2, m_tot, q, eta, delta, eta2, eta3, chi1l, chi2l, chi1l2l, chi1l2, chi1l3, chi2l2, chi2l3, chi_eff,
-        chi_pn_hat, s_tot_r, dchi, f_ref, phi0, mf_ref, m_sec, f_min, f_max, f_cut, f_max_prime, distance, amp0,
+        chi_pn_hat, s_tot_r, dchi, dchi_half, f_ref, phi0, mf_ref, m_sec, f_min, f_max, f_cut, f_max_prime, distance, amp0,
         amp_norm, dphase0, m_final, a_final, f_ring, f_damp, f_meco, f_isco;
     bool error_mass_ratio, error_spin, error_final_mass, error_final_spin, error_meco;
 };
@@ -330,6 +330,7 @@ static WfVars<R> waveform_struct(R m1_in, R m2_in, R chi1l_in, R chi2l_in, R f_r
     wf.chi_pn_hat = ut_chi_pn_hat(eta, chi1l, chi2l);
     wf.s_tot_r = ut_s_tot_r(eta, chi1l, chi2l);
     wf.dchi = dchi;
+    wf.dchi_half = dchi * 0.5;  // internals.py:105
     wf.f_ref = f_ref;
     wf.phi0 = phi0;
     wf.mf_ref = f_ref * (MTSUN_SI_ * m_tot);
@@ -1348,3 +1349,6 @@ void oracle_xas_count(const double* tables, long nb, const double* freqs, long n
 }
 
 }  // extern "C"
+
+// IMRPhenomXHM (test infrastructure as the rest of this file)
+#include "xhm_oracle.h"

@@ -1,0 +1,103 @@
+"""numpy front end of the IMRPhenomXHM oracle (``xhm_oracle.h``, compiled
+into ``liboracle_xas.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke``
+and ``bench.py``'s CPU-baseline leg, as the checker. The product
+(``waveformx_amd``) never imports or links anything here.
+
+The C++ restatement follows the reference's LAL form of IMRPhenomXHM
+(release 122022): ``src/waveformx/models/xhm/reference.py`` and
+``xpnr/hm/{variables,inspiral,phase}.py``; see the header of
+``xhm_oracle.h`` for the file:line list.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import oracle
+
+#: The modes (l, m) in the order of the outputs: the (2,2), then LAL's
+#: higher modes (xhm/reference.py:55-57).
+MODES = ((2, 2), (2, 1), (3, 3), (3, 2), (4, 4))
+HIGHER_MODES = MODES[1:]
+INTERNALS = ("f_ring", "f_damp", "f_amp_match_in", "f_amp_match_im", "f_phase_match_in", "f_phase_match_im",
+             "c1_insp", "c_insp", "c1_rd", "c_rd", "deltaphi", "c0", "c1", "c2", "c3", "c4", "c_l", "amp0")
+
+_ready = False
+
+
+def lib():
+    global _ready
+    handle = oracle.lib()
+    if not _ready:
+        d = np.ctypeslib.ndpointer(np.float64, flags="C")
+        L, D, I, P = ctypes.c_long, ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+        handle.oracle_xhm_polarizations.argtypes = [d, L, d, L, D, d, d, P]
+        handle.oracle_xhm_mode_internals.argtypes = [d, D, I, d]
+        handle.oracle_xhm_mode_amplitude_phase.argtypes = [d, D, I, d, L, d, d, P]
+        handle.oracle_swsh.argtypes = [D, D, I, I, d]
+        _ready = True
+    return handle
+
+
+def draw(nb: int, seed: int = 4) -> np.ndarray:
+    """The parameters of the multimode workload, (nb, 8) = (m1, m2, chi1z,
+    chi2z, d_L, inclination, phi_ref, f_ref): m1 ~ U[10, 80], q ~ U[0.1, 1],
+    spins ~ U[-0.9, 0.9], d_L ~ U[100, 2000] Mpc, cos(inclination) ~
+    U[-1, 1], phi_ref ~ U[0, 2 pi], f_ref = 20 Hz."""
+    rng = np.random.default_rng(seed)
+    m1 = rng.uniform(10.0, 80.0, nb)
+    q = rng.uniform(0.1, 1.0, nb)
+    chi1 = rng.uniform(-0.9, 0.9, nb)
+    chi2 = rng.uniform(-0.9, 0.9, nb)
+    distance = rng.uniform(100.0, 2000.0, nb)
+    inclination = np.arccos(rng.uniform(-1.0, 1.0, nb))
+    phi = rng.uniform(0.0, 2.0 * np.pi, nb)
+    return np.stack([m1, q * m1, chi1, chi2, distance, inclination, phi, np.full(nb, 20.0)], axis=1)
+
+
+def polarizations(params, freqs, amp_scale: float = 1.0, modes: bool = False):
+    """``(h_+, h_x)`` (complex128, (nb, n)) of IMRPhenomXHM on the grid
+    ``freqs`` (f_min = freqs[0]) and, with ``modes``, the five modes
+    ``h_{l,-m}`` of :data:`MODES` as (5, nb, n)."""
+    params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+    freqs = np.ascontiguousarray(freqs, dtype=np.float64)
+    nb, n = params.shape[0], freqs.shape[0]
+    hp = np.empty((nb, n), np.complex128)
+    hc = np.empty((nb, n), np.complex128)
+    out = np.empty((5, nb, n), np.complex128) if modes else None
+    lib().oracle_xhm_polarizations(params, nb, freqs, n, amp_scale, hp.view(np.float64), hc.view(np.float64),
+                                   out.ctypes.data if modes else None)
+    return (hp, hc, out) if modes else (hp, hc)
+
+
+def mode_internals(params, f_min: float, k: int) -> dict:
+    """The set-up of higher mode ``k`` (0..3 for (2,1), (3,3), (3,2), (4,4))
+    of one binary: :data:`INTERNALS`."""
+    out = np.empty(len(INTERNALS))
+    lib().oracle_xhm_mode_internals(np.ascontiguousarray(params, dtype=np.float64), f_min, k, out)
+    return dict(zip(INTERNALS, out))
+
+
+def mode_amplitude_phase(params, f_min: float, k: int, mf, rotation: bool = False):
+    """Amplitude (without ``(-1)^l amp0``) and phase of higher mode ``k`` at
+    the mass-scaled frequencies ``mf``, and for the (3,2) with
+    ``rotation`` the rotated ringdown phase and its first two derivatives
+    in ``Mf`` (3, n)."""
+    mf = np.ascontiguousarray(mf, dtype=np.float64)
+    amp = np.empty_like(mf)
+    phase = np.empty_like(mf)
+    rot = np.empty((mf.size, 3)) if rotation else None
+    lib().oracle_xhm_mode_amplitude_phase(np.ascontiguousarray(params, dtype=np.float64), f_min, k, mf, mf.size,
+                                          amp, phase, rot.ctypes.data if rotation else None)
+    return (amp, phase, rot.T.copy()) if rotation else (amp, phase)
+
+
+def swsh(theta: float, phi: float, ell: int, m: int) -> complex:
+    """The spin-weighted spherical harmonic of spin weight -2."""
+    out = np.empty(2)
+    lib().oracle_swsh(theta, phi, ell, m, out)
+    return complex(out[0], out[1])
