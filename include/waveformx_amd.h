@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (11); a binding checks it before any call. */
+/* The version of this ABI (12); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -189,6 +189,35 @@ int wfx_phd_grid_adjoint(const void* records, int64_t n_binaries, const void* fr
 int wfx_phd_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
                          int freqs_f64, int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
                          const void* weights, const void* powers, double* loss, double* grads, void* stream);
+
+/* ------------------------------------------------------------ IMRPhenomXHM
+ *
+ * IMRPhenomXHM (release 122022, LAL's defaults: every mode, no
+ * multibanding), FP64. Replaces the reference's XHM model behind
+ * polarizations_function("IMRPhenomXHM") and batch_polarizations
+ * (src/waveformx/models/xhm/api.py:63-81, model.py; set-up
+ * xhm/reference.py:566-608, evaluation reference.py:620-677).
+ *
+ * wfx_xhm_record_slots / wfx_xhm_head_slots: the doubles of one
+ * higher-mode record and of one binary's head.
+ *
+ * wfx_xhm_setup: for params (B, 8) as wfx_xas_setup's, writes records22
+ * (B, 64) FP64 (the (2,2) mode: wfx_xas_setup's precision 0 record with
+ * reference_mf NaN), head (B, wfx_xhm_head_slots()) (the factors of the
+ * five modes (2,2), (2,1), (3,3), (3,2), (4,4) in the polarizations, NaN
+ * where LAL fails), modes (B, 4, wfx_xhm_record_slots()) (the higher
+ * modes) and flags (B,) (LAL's error bits, as wfx_xas_setup; may be null).
+ *
+ * wfx_xhm_eval: (h+, hx) (B, N) complex128 on the shared grid freqs (N,)
+ * (float64 if freqs_f64, else float32), and, if mode_out is not null, the
+ * modes h_{l,-m} (5, B, N) complex128 in the order above. */
+int wfx_xhm_record_slots(void);
+int wfx_xhm_head_slots(void);
+int wfx_xhm_setup(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
+                  double amp_scale, double* records22, double* head, double* modes, uint32_t* flags, void* stream);
+int wfx_xhm_eval(const double* records22, const double* head, const double* modes, int64_t n_binaries,
+                 const void* freqs, int freqs_f64, int64_t n_freqs, void* hplus, void* hcross, void* mode_out,
+                 void* stream);
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,7 @@ from pathlib import Path
 
 PACKAGE = Path(__file__).resolve().parent
 SOURCES = [PACKAGE / "csrc" / "xas_kernels.hip"]
-HEADERS = sorted((PACKAGE / "csrc").glob("*.h")) + [PACKAGE / "csrc" / "xas_fits.inc"]
+HEADERS = sorted((PACKAGE / "csrc").glob("*.h")) + sorted((PACKAGE / "csrc").glob("*.inc"))
 LIBRARY = PACKAGE / "libwfx_xas.so"
 ARCH = os.environ.get("WFX_OFFLOAD_ARCH", "gfx950")
 

@@ -261,9 +261,141 @@ def generate(root: Path) -> str:
     return "\n".join(out)
 
 
+# IMRPhenomXHM (release 122022): module -> (prefix, the dispatchers by mode
+# that the setup writes as switches)
+HM_FITS = {
+    "hm_amp_fits_inspiral": ("hai", {"amp_fit"}),
+    "hm_amp_fits_intermediate": ("ham", {"amp_fit"}),
+    "hm_amp_fits_ringdown": ("har", set()),
+    "hm_intermediate": ("hpi", {"phase_fit"}),
+    "hm_inspiral": ("hpn", {"phase_lambda_fit"}),
+    "hm_ringdown": ("hpr", {"phase_32_fit"}),
+}
+HM_QNM = ["fring21", "fring33", "fring32", "fring44", "fdamp21", "fdamp33", "fdamp32", "fdamp44",
+          "re_l2m2lp2", "re_l2m2lp3", "re_l3m2lp2", "re_l3m2lp3", "im_l2m2lp2", "im_l2m2lp3", "im_l3m2lp2",
+          "im_l3m2lp3"]
+
+
+class HmWriter(Writer):
+    """The writer with ``if``/``else`` on the mode tag (an int) and
+    ``math.log`` of literals (``lambda_pn``)."""
+
+    def text(self, node):
+        if isinstance(node, ast.Call) and isinstance(node.func, ast.Attribute) and \
+                getattr(node.func.value, "id", "") == "math":
+            return f"R(log({self.double_text(node.args[0])}))"
+        if isinstance(node, ast.Compare) and getattr(node.left, "id", "") == "mode_tag":
+            return f"(mode_tag == {int(node.comparators[0].value)})"
+        return super().text(node)
+
+    def statement(self, stmt, declared, lines, indent="    "):
+        if isinstance(stmt, ast.If):
+            for name in sorted({t.id for n in ast.walk(stmt) if isinstance(n, ast.Assign) for t in n.targets}
+                               - declared):
+                declared.add(name)
+                lines.append(f"{indent}R {name};")
+            lines.append(f"{indent}if {self.text(stmt.test)} {{")
+            for s in stmt.body:
+                self.statement(s, declared, lines, indent + "    ")
+            lines.append(f"{indent}}} else {{")
+            for s in stmt.orelse:
+                self.statement(s, declared, lines, indent + "    ")
+            lines.append(f"{indent}}}")
+            return
+        start = len(lines)
+        super().statement(stmt, declared, lines)
+        lines[start:] = [indent + line.lstrip() for line in lines[start:]]
+
+    def function(self, fn, module):
+        self.static = set()
+        params = [f"const {STRUCTS[a.arg]}& {a.arg}" if a.arg in STRUCTS else
+                  (f"int {a.arg}" if a.arg == "mode_tag" else f"R {a.arg}") for a in fn.args.args]
+        lines = [f"// {module}.py:{fn.lineno}",
+                 f"template <class R> WFX_HD inline R {self.prefix}_{fn.name}({', '.join(params)}) {{"]
+        declared = {a.arg for a in fn.args.args}
+        for stmt in fn.body:
+            self.statement(stmt, declared, lines)
+        lines.append("}")
+        return "\n".join(lines)
+
+
+def polynomial(node, powers):
+    """Ascending coefficients of a ``fma(c, x, rest)`` chain (or its
+    product with ``af``) in the final spin, ``powers`` naming ``af``, ``x2``
+    ... by degree."""
+    if isinstance(node, ast.Constant):
+        return {0: float(node.value)}
+    if isinstance(node, ast.UnaryOp):
+        return {k: -v for k, v in polynomial(node.operand, powers).items()}
+    if isinstance(node, ast.Name):
+        return {powers[node.id]: 1.0}
+    if isinstance(node, ast.Call):
+        a, b, c = (polynomial(x, powers) for x in node.args)
+        out = dict(c)
+        for i, x in a.items():
+            for j, y in b.items():
+                out[i + j] = out.get(i + j, 0.0) + x * y
+        return out
+    if isinstance(node, ast.BinOp) and isinstance(node.op, ast.Mult):
+        a, b = polynomial(node.left, powers), polynomial(node.right, powers)
+        out = {}
+        for i, x in a.items():
+            for j, y in b.items():
+                out[i + j] = out.get(i + j, 0.0) + x * y
+        return out
+    raise NotImplementedError(ast.dump(node))
+
+
+def hm_qnm_tables(path):
+    """The higher modes' final-spin fits (numerator / denominator) in powers
+    of u = 1 - a_f, as xhm/reference.py:96-99 evaluates them."""
+    tree = ast.parse(path.read_text())
+    functions = {n.name: n for n in tree.body if isinstance(n, ast.FunctionDef)}
+    lines = []
+    for name in HM_QNM:
+        fn = functions[name]
+        powers = {"af": 1}
+        for stmt in fn.body:
+            if isinstance(stmt, ast.Assign):
+                powers[stmt.targets[0].id] = int(stmt.targets[0].id[1:])
+            elif isinstance(stmt, ast.Return):
+                ratio = stmt.value
+        lines.append(f"// hm_qnm.py:{fn.lineno}: {name}, numerator and denominator in powers of u = 1 - a_f")
+        for part, node in (("num", ratio.left), ("den", ratio.right)):
+            c = polynomial(node, powers)
+            u = about_one([c.get(k, 0.0) for k in range(max(c) + 1)])
+            lines.append(f"constexpr double kHmq_{name}_{part}[{len(u)}] = {{{', '.join(repr(x) for x in u)}}};")
+    return "\n".join(lines)
+
+
+def generate_hm(root: Path) -> str:
+    base = root / "src/waveformx/models/xpnr/_generated"
+    out = ["// GENERATED by waveformx_amd/csrc/gen_fits.py from the reference's IMRPhenomXHM fits",
+           "// (src/waveformx/models/xpnr/_generated/hm_*.py; LALSimulation 6.2.0 data via waveformrs,",
+           "// GPL-3.0-or-later). Included by xhm_setup.h inside namespace wfx. Do not edit.", ""]
+    out.append(hm_qnm_tables(base / "hm_qnm.py"))
+    out.append("")
+    for module, (prefix, skip) in HM_FITS.items():
+        tree = ast.parse((base / f"{module}.py").read_text())
+        functions = {n.name: n for n in tree.body if isinstance(n, ast.FunctionDef)}
+        w = HmWriter(prefix, functions)
+        for name, fn in functions.items():
+            if name not in skip:
+                out.append(w.function(fn, module))
+                out.append("")
+    # xhm/reference.py:189: the pseudo-PN collocation system's inverse
+    inverse = np.linalg.inv(np.array([[x ** q for q in (7.0 / 3.0, 8.0 / 3.0, 3.0)] for x in (0.5, 0.75, 1.0)]))
+    rows = ", ".join("{" + ", ".join(repr(float(v)) for v in row) + "}" for row in inverse)
+    out.append("// xhm/reference.py:189 (numpy.linalg.inv in float64)")
+    out.append(f"constexpr double kHmInspiralInverse[3][3] = {{{rows}}};")
+    return "\n".join(out) + "\n"
+
+
 def main():
     root = Path(sys.argv[1]) if len(sys.argv) > 1 else Path("/root/reference")
-    (Path(__file__).resolve().parent / "xas_fits.inc").write_text(generate(root))
+    here = Path(__file__).resolve().parent
+    (here / "xas_fits.inc").write_text(generate(root))
+    (here / "xhm_fits.inc").write_text(generate_hm(root))
 
 
 if __name__ == "__main__":

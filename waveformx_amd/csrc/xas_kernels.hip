@@ -16,6 +16,7 @@
 #include "phenomd_point.h"
 #include "phenomd_adjoint.h"
 #include "xas_adjoint.h"
+#include "xhm_point.h"
 
 #include <atomic>
 #include <cstdio>
@@ -1059,6 +1060,64 @@ int grid_adjoint(const char* name, const void* records, int64_t n_binaries, cons
     return check_launch(name);
 }
 
+// ------------------------------------------------------------ IMRPhenomXHM
+
+// one FP64 thread per (binary, higher mode): the mode's record; the thread of
+// the first mode also writes the binary's head (the factors of the five
+// modes in the polarizations, NaN where LAL fails)
+__global__ __launch_bounds__(64) void xhm_setup_kernel(const double* __restrict__ params, int64_t nb,
+                                                       const double* __restrict__ f_min, int64_t f_min_stride,
+                                                       double amp_scale, double* __restrict__ head,
+                                                       double* __restrict__ modes) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb * kHmModes) return;
+    const int64_t b = t / kHmModes;
+    const int k = (int)(t % kHmModes);
+    const double* p = params + b * kParams;
+    const WaveformVars<double> wf = waveform_vars(p, f_min[b * f_min_stride], amp_scale);
+    const AmpCoeffs<double> amp22 = amp_coeffs(wf);
+    const PhaseCoeffs<double> ph22 = phase_coeffs(wf);
+    HmCoeffs c;
+    hm_setup(wf, amp22, ph22, k, c);
+    write_mode_record(c, wf, modes + (b * kHmModes + k) * kHmSlots);
+    if (k == 0) {
+        double* h = head + b * kHeadSlots;
+        for (int s = 0; s < kHeadSlots; ++s) h[s] = 0.0;
+        const double inclination = p[kInclination];
+        mode_factors(inclination, 2, 2, h + kHeadFactors);
+        for (int m = 0; m < kHmModes; ++m) mode_factors(inclination, kHmEll[m], kHmEmm[m], h + kHeadFactors + 4 * (m + 1));
+        if (wf.flags)
+            for (int s = 0; s < 20; ++s) h[kHeadFactors + s] = __builtin_nan("");
+    }
+}
+
+// one thread per (binary, frequency), 256-thread blocks, a block on 256
+// consecutive frequencies of one binary (its records wave-uniform)
+constexpr int kXhmBlock = 256;
+template <class F>
+__global__ __launch_bounds__(kXhmBlock) void xhm_eval_kernel(const double* __restrict__ rec22,
+                                                             const double* __restrict__ head,
+                                                             const double* __restrict__ modes, int64_t nb,
+                                                             const F* __restrict__ freqs, int64_t n, int64_t chunks,
+                                                             double2* __restrict__ hp, double2* __restrict__ hc,
+                                                             double2* __restrict__ mode_out) {
+    const int64_t b = blockIdx.x / chunks;
+    const int64_t i = (blockIdx.x % chunks) * kXhmBlock + threadIdx.x;
+    if (b >= nb || i >= n) return;
+    double h[5][2];
+    xhm_modes_f64(rec22 + b * kSlots, modes + b * kHmModes * kHmSlots, (double)freqs[i], h);
+    double p_re, p_im, c_re, c_im;
+    xhm_sum(head + b * kHeadSlots, h, p_re, p_im, c_re, c_im);
+    hp[b * n + i] = make_double2(p_re, p_im);
+    hc[b * n + i] = make_double2(c_re, c_im);
+    if (mode_out) {
+        const double nan = __builtin_nan("");
+        const bool bad = head[b * kHeadSlots] != head[b * kHeadSlots];
+        for (int k = 0; k < 5; ++k)
+            mode_out[(k * nb + b) * n + i] = bad ? make_double2(nan, nan) : make_double2(h[k][0], h[k][1]);
+    }
+}
+
 }  // namespace
 
 // ================================================================= C ABI
@@ -1068,7 +1127,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 11; }
+int wfx_abi_version(void) { return 12; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -1285,6 +1344,50 @@ int wfx_phd_loss_adjoint(const void* records, const double* jacobian, int64_t n_
     return adjoint<PhenomDAdjoint, kLoss>("wfx_phd_loss_adjoint", records, jacobian, n_binaries, freqs, freqs_f64,
                                           n_freqs, precision, data_plus, data_cross, weights, powers, loss, grads,
                                           stream);
+}
+
+
+int wfx_xhm_record_slots(void) { return kHmSlots; }
+
+int wfx_xhm_head_slots(void) { return kHeadSlots; }
+
+int wfx_xhm_setup(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
+                  double amp_scale, double* records22, double* head, double* modes, uint32_t* flags, void* stream) {
+    if (n_binaries < 0) return fail("wfx_xhm_setup: negative number of binaries");
+    if (n_binaries == 0) return 0;
+    if (!params || !f_min || !records22 || !head || !modes) return fail("wfx_xhm_setup: null pointer");
+    if (f_min_stride != 0 && f_min_stride != 1) return fail("wfx_xhm_setup: f_min_stride must be 0 or 1");
+    const int blocks22 = grid_blocks((n_binaries + 63) / 64, "wfx_xhm_setup: too many binaries");
+    const int blocks = grid_blocks((n_binaries * kHmModes + 63) / 64, "wfx_xhm_setup: too many binaries");
+    if (blocks22 < 0 || blocks < 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    setup_f64_kernel<<<blocks22, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale,
+                                              __builtin_nan(""), records22, flags);
+    if (check_launch("wfx_xhm_setup")) return -1;
+    xhm_setup_kernel<<<blocks, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, head, modes);
+    return check_launch("wfx_xhm_setup");
+}
+
+int wfx_xhm_eval(const double* records22, const double* head, const double* modes, int64_t n_binaries,
+                 const void* freqs, int freqs_f64, int64_t n_freqs, void* hplus, void* hcross, void* mode_out,
+                 void* stream) {
+    if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xhm_eval: negative size");
+    if (n_binaries == 0 || n_freqs == 0) return 0;
+    if (!records22 || !head || !modes || !freqs || !hplus || !hcross) return fail("wfx_xhm_eval: null pointer");
+    const int64_t chunks = (n_freqs + kXhmBlock - 1) / kXhmBlock;
+    if (n_binaries > INT64_MAX / chunks) return fail("wfx_xhm_eval: grid too large");
+    const int blocks = grid_blocks(n_binaries * chunks, "wfx_xhm_eval: grid too large");
+    if (blocks < 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (freqs_f64)
+        xhm_eval_kernel<double><<<blocks, kXhmBlock, 0, s>>>(records22, head, modes, n_binaries,
+                                                              (const double*)freqs, n_freqs, chunks,
+                                                              (double2*)hplus, (double2*)hcross, (double2*)mode_out);
+    else
+        xhm_eval_kernel<float><<<blocks, kXhmBlock, 0, s>>>(records22, head, modes, n_binaries,
+                                                             (const float*)freqs, n_freqs, chunks,
+                                                             (double2*)hplus, (double2*)hcross, (double2*)mode_out);
+    return check_launch("wfx_xhm_eval");
 }
 
 }  // extern "C"

@@ -17,17 +17,16 @@ namespace wfx {
 
 // ---------------------------------------------------------- FP64 (parity)
 
-WFX_HD inline void point_f64(const double* c, const FreqPowers<double>& fp, double& hp_re, double& hp_im,
-                             double& hc_re, double& hc_im) {
+// the amplitude and the phase (reduced to [-pi, pi]) of the (2,2) mode
+WFX_HD inline void amp_phase_f64(const double* c, const FreqPowers<double>& fp, double& amp, double& phi) {
     bool past;
     const double rest = phase_rest(c, fp, past);
-    double phi = (past ? 0.0 : (c[kT0] + c[kT0Lo]) * fp.m53) + rest;
+    phi = (past ? 0.0 : (c[kT0] + c[kT0Lo]) * fp.m53) + rest;
     phi = phi - (c[kAnchor] + c[kAnchorLo]);
     const double turns = rint(phi / kTwoPi);
     phi = phi - turns * kTwoPi;
 
     const double f = fp.f;
-    double amp;
     if (f > c[kFAmpRd]) {
         const double u = c[kAmpRdS] * (f - c[kShift]);
         amp = c[kAmpRdN] * fp.m76 * exp(-c[kAmpRdGr] * u) / (u * u + c[kAmpRdG]);
@@ -41,6 +40,12 @@ WFX_HD inline void point_f64(const double* c, const FreqPowers<double>& fp, doub
         amp = fp.m76 * poly;
     }
     amp = f <= c[kFCut] ? amp : 0.0;
+}
+
+WFX_HD inline void point_f64(const double* c, const FreqPowers<double>& fp, double& hp_re, double& hp_im,
+                             double& hc_re, double& hc_im) {
+    double amp, phi;
+    amp_phase_f64(c, fp, amp, phi);
     double s, co;
     sincos(phi, &s, &co);
     const double ap = c[kKp] * amp;

@@ -66,7 +66,7 @@ WFX_HD inline Powers<R> useful_powers(const R& x) {
 template <class R>
 struct WaveformVars {
     R eta, delta, eta2, eta3, chi1l, chi2l, chi1l2, chi1l3, chi2l2, chi_eff, chi_pn_hat, s_tot_r, dchi, f_ref, phi0,
-        m_sec, f_cut, amp0, amp_norm, dphase0, m_final, a_final, f_ring, f_damp, f_meco, f_isco;
+        m_sec, f_cut, amp0, amp_norm, dphase0, m_final, a_final, f_ring, f_damp, f_meco, f_isco, q, dchi_half;
     unsigned flags;
 };
 
@@ -132,6 +132,8 @@ WFX_HD inline WaveformVars<R> waveform_vars(const R* p, const R& f_min, double a
     wf.chi_pn_hat = ut_chi_pn_hat(eta, chi1l, chi2l);
     wf.s_tot_r = ut_s_tot_r(eta, chi1l, chi2l);
     wf.dchi = ut_dchi(chi1l, chi2l);
+    wf.dchi_half = wf.dchi * R(0.5);
+    wf.q = eta == R(0.25) ? R(1.0) : mass_ratio;
     wf.f_ref = f_ref;
     wf.phi0 = p[kPhiRef];
     wf.m_sec = m_sec;

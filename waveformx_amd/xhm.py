@@ -1,0 +1,95 @@
+"""IMRPhenomXHM (release 122022) on the GPU: device-tensor wrappers of its
+HIP kernels.
+
+* :func:`setup` -- per binary, the FP64 record of its (2,2) mode (that of
+  IMRPhenomXAS), a head (the factors of the five modes in the
+  polarizations, ``xhm/reference.py:646-653``) and one record per higher
+  mode, (2,1), (3,3), (3,2), (4,4) (``xhm/reference.py:580-608``: LAL's
+  set-up with exact derivatives where LAL differentiates numerically); one
+  FP64 thread per binary and mode;
+* :func:`evaluate` -- ``(h+, hx)`` (B, N) complex128 on a shared grid, and
+  optionally the five modes ``h_{l,-m}`` (5, B, N): each higher mode by
+  region in Mf, the (3,2) ringdown recycling the (2,2) mode
+  (``xhm/reference.py:620-677``);
+* :func:`polarizations` / :func:`modes` -- set-up at the grid's first
+  frequency, then the evaluation.
+
+The arithmetic is FP64 throughout (the reference's float64 path).
+"""
+
+from __future__ import annotations
+
+from typing import Optional, Tuple
+
+import torch
+
+from ._lib import call, lib
+from .xas import RECORD_SLOTS, _f_min, _ptr, _require_device, _stream
+
+__all__ = ["MODES", "LAL_FLAGS", "setup", "evaluate", "polarizations", "modes"]
+
+#: the modes (l, m) of the outputs, the (2,2) first (``xhm/reference.py:55-57``)
+MODES = ((2, 2), (2, 1), (3, 3), (3, 2), (4, 4))
+#: the setup's flag bits -> LAL's messages: those of the waveform variables
+#: (``xas/api.py:40-60``, checked for IMRPhenomXHM by ``xhm/api.py:67-72``)
+LAL_FLAGS = ((1, "IMRPhenomXHM is not valid at mass ratios beyond 1000"),
+             (4, "the fitted final mass is not physical"),
+             (8, "the fitted final spin exceeds one"),
+             (16, "f_MECO cannot be greater than f_ISCO"))
+
+
+def setup(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21
+          ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    """``(records22 (B, 64), head (B, H), modes (B, 4, S), flags (B,))``,
+    float64 (int32 flags), for binaries ``params`` (B, 8) at first frequency
+    ``f_min`` (one value or (B,))."""
+    _require_device(params, "params")
+    if params.ndim != 2 or params.shape[1] != 8 or params.dtype != torch.float64:
+        raise ValueError("params must be a (B, 8) float64 tensor")
+    nb = params.shape[0]
+    params = params.contiguous()
+    f_min, stride = _f_min(f_min, nb)
+    handle = lib()
+    dev = params.device
+    records = torch.empty((nb, RECORD_SLOTS), dtype=torch.float64, device=dev)
+    head = torch.empty((nb, handle.wfx_xhm_head_slots()), dtype=torch.float64, device=dev)
+    modes_ = torch.empty((nb, 4, handle.wfx_xhm_record_slots()), dtype=torch.float64, device=dev)
+    flags = torch.empty((nb,), dtype=torch.int32, device=dev)
+    call("wfx_xhm_setup", _ptr(params), nb, _ptr(f_min), stride, float(amp_scale), _ptr(records), _ptr(head),
+         _ptr(modes_), _ptr(flags), _stream(dev))
+    return records, head, modes_, flags
+
+
+def evaluate(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, freqs: torch.Tensor,
+             with_modes: bool = False):
+    """``(h+, hx)`` (B, N) complex128 on the shared grid ``freqs`` (N,)
+    (float32 or float64), and with ``with_modes`` the modes (5, B, N)."""
+    for t, name in ((records, "records"), (head, "head"), (modes_, "modes"), (freqs, "freqs")):
+        _require_device(t, name)
+    if freqs.dtype not in (torch.float32, torch.float64):
+        raise ValueError("freqs must be float32 or float64")
+    nb, n = records.shape[0], freqs.shape[0]
+    if head.shape[0] != nb or modes_.shape[0] != nb:
+        raise ValueError("records, head and modes must describe the same binaries")
+    freqs = freqs.contiguous()
+    dev = records.device
+    hp = torch.empty((nb, n), dtype=torch.complex128, device=dev)
+    hc = torch.empty((nb, n), dtype=torch.complex128, device=dev)
+    out = torch.empty((5, nb, n), dtype=torch.complex128, device=dev) if with_modes else None
+    call("wfx_xhm_eval", _ptr(records), _ptr(head), _ptr(modes_), nb, _ptr(freqs),
+         int(freqs.dtype == torch.float64), n, _ptr(hp), _ptr(hc), _ptr(out), _stream(dev))
+    return (hp, hc, out) if with_modes else (hp, hc)
+
+
+def polarizations(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21
+                  ) -> Tuple[torch.Tensor, torch.Tensor]:
+    """``(h+, hx)`` of a batch on a shared grid: the set-up at the grid's
+    first frequency, then the evaluation."""
+    records, head, modes_, _ = setup(params, freqs[:1], amp_scale)
+    return evaluate(records, head, modes_, freqs)
+
+
+def modes(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21) -> torch.Tensor:
+    """The five modes ``h_{l,-m}`` of :data:`MODES`, (5, B, N) complex128."""
+    records, head, modes_, _ = setup(params, freqs[:1], amp_scale)
+    return evaluate(records, head, modes_, freqs, with_modes=True)[2]
