@@ -201,9 +201,9 @@ int wfx_phd_loss_adjoint(const void* records, const double* jacobian, int64_t n_
  * wfx_xhm_record_slots / wfx_xhm_head_slots: the doubles of one
  * higher-mode record and of one binary's head.
  *
- * wfx_xhm_setup: for params (B, 8) as wfx_xas_setup's, writes records22
- * (B, 64) FP64 (the (2,2) mode: wfx_xas_setup's precision 0 record with
- * reference_mf NaN), head (B, wfx_xhm_head_slots()) (the factors of the
+ * wfx_xhm_setup: for params (B, 8) and reference_mf as wfx_xas_setup's
+ * (xhm/model.py:40-42), writes records22 (B, 64) FP64 (the (2,2) mode:
+ * wfx_xas_setup's precision 0 record), head (B, wfx_xhm_head_slots()) (the factors of the
  * five modes (2,2), (2,1), (3,3), (3,2), (4,4) in the polarizations, NaN
  * where LAL fails), modes (B, 4, wfx_xhm_record_slots()) (the higher
  * modes) and flags (B,) (LAL's error bits, as wfx_xas_setup; may be null).
@@ -214,7 +214,8 @@ int wfx_phd_loss_adjoint(const void* records, const double* jacobian, int64_t n_
 int wfx_xhm_record_slots(void);
 int wfx_xhm_head_slots(void);
 int wfx_xhm_setup(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
-                  double amp_scale, double* records22, double* head, double* modes, uint32_t* flags, void* stream);
+                  double amp_scale, double reference_mf, double* records22, double* head, double* modes,
+                  uint32_t* flags, void* stream);
 int wfx_xhm_eval(const double* records22, const double* head, const double* modes, int64_t n_binaries,
                  const void* freqs, int freqs_f64, int64_t n_freqs, void* hplus, void* hcross, void* mode_out,
                  void* stream);

@@ -87,3 +87,95 @@ def test_random_batch(cuda):
     f32 = f.astype(np.float32).astype(np.float64)
     hp64, _ = xhm.evaluate(records, head, modes_, torch.as_tensor(f32, device=cuda))
     assert torch.equal(hp32, hp64)
+
+
+def test_public_functions(cuda):
+    """IMRPhenomXHM through the reference's interface: the torch function
+    (float64 and float32 inputs, a batch axis), batch_polarizations with
+    failing rows, the mass-scaled reference, and the backward pass raising
+    (``torch.py:175-183``, ``_api.py:307-360``, ``xhm/model.py:40-42``)."""
+    import torch
+
+    import waveformx_amd as wx
+    import waveformx_amd.torch as wt
+
+    case = CASES[0]
+    params = np.array([[case[k] for k in (0, 1, 2, 3, 4, 5, 6, 8)]])
+    f = np.linspace(20.0, 1024.0, 2001)
+    op, oc = oxhm.polarizations(params, f, amp_scale=1e21)
+    binary = wx.Binary(case[0], case[1], (0.0, 0.0, case[2]), (0.0, 0.0, case[3]), case[4], case[5], case[6], case[8])
+    function = wt.polarizations_function("IMRPhenomXHM")
+    hp, hc = function(binary, torch.as_tensor(f, device=cuda))
+    assert hp.dtype == torch.complex128 and hp.shape == (f.size,)
+    assert relative_error(hp.cpu().numpy(), op[0]) <= BAR and relative_error(hc.cpu().numpy(), oc[0]) <= BAR
+    b32 = wx.Binary(*(torch.tensor(v, dtype=torch.float32, device=cuda) for v in binary))
+    hp32, _ = function(b32, torch.as_tensor(f, dtype=torch.float32, device=cuda))
+    assert hp32.dtype == torch.complex64
+    # a batch of two, the second binary heavier
+    m1 = torch.tensor([case[0], 60.0], dtype=torch.float64, device=cuda, requires_grad=True)
+    batch = wx.Binary(m1, case[1], (0.0, 0.0, case[2]), (0.0, 0.0, case[3]), case[4], case[5], case[6], case[8])
+    bp, _ = function(batch, torch.as_tensor(f, device=cuda))
+    assert bp.shape == (2, f.size) and torch.equal(bp[0], hp)
+    with pytest.raises(NotImplementedError):
+        bp.abs().sum().backward()
+    # batch_polarizations: a failing row (q > 1000) is zero and reported
+    table = wx.parameter_table({"mass_1": np.array([case[0], 2000.0]), "mass_2": np.array([case[1], 1.0]),
+                                "spin_1": np.array([[0, 0, case[2]], [0, 0, 0]]),
+                                "spin_2": np.array([[0, 0, case[3]], [0, 0, 0]]),
+                                "luminosity_distance": np.array([case[4], 100.0]),
+                                "inclination": np.array([case[5], 0.3]), "phi_ref": np.array([case[6], 0.0]),
+                                "f_ref": np.array([case[8], 20.0])})
+    result = wx.batch_polarizations("IMRPhenomXHM", f, table)
+    assert list(result.ok) == [True, False] and "1000" in str(result.errors[1])
+    assert torch.equal(result.hplus[0], hp) and not bool(result.hplus[1].any())
+    with pytest.raises(wx.InvalidParameterError, match="IMRPhenomXHM is not valid at mass ratios beyond 1000"):
+        wx.batch_polarizations("IMRPhenomXHM", f, table, strict=True)
+    # the mass-scaled reference: f_ref = reference_mf / M, NaN where f_ref != 0
+    scaled = params.copy()
+    scaled[0, 7] = 0.012 / (MTSUN * (case[0] + case[1]))
+    sp, _ = oxhm.polarizations(scaled, f, amp_scale=1e21)
+    zero_ref = wx.Binary(*binary[:7], 0.0)
+    rp, _ = wt.polarizations_function("IMRPhenomXHM", reference_mf=0.012)(zero_ref, torch.as_tensor(f, device=cuda))
+    assert relative_error(rp.cpu().numpy(), sp[0]) <= BAR
+    nan_p, _ = wt.polarizations_function("IMRPhenomXHM", reference_mf=0.012)(binary, torch.as_tensor(f, device=cuda))
+    assert bool(torch.isnan(nan_p).all())
+
+
+def test_modes_and_their_sum(cuda):
+    """``modes`` in ``SimInspiralChooseFDModes``'s two-sided conventions:
+    their sum with the harmonics at (inclination, pi/2) is the
+    polarizations (``tests/test_xhm.py:126-128``: to 1e-12), the negative
+    frequencies are (-1)^l conj(h_{l,-m}), and ``choose_fd_modes`` holds the
+    same modes on LAL's uniform grid."""
+    import torch
+
+    import waveformx_amd as wx
+
+    case = CASES[2]
+    keywords = dict(mass_1=case[0], mass_2=case[1], spin_1=(0.0, 0.0, case[2]), spin_2=(0.0, 0.0, case[3]),
+                    luminosity_distance=case[4], inclination=case[5], phi_ref=case[6], f_ref=case[8])
+    f = np.geomspace(case[7], 0.3 / ((case[0] + case[1]) * MTSUN), 700)
+    hp, hc = wx.polarizations("IMRPhenomXHM", f, **keywords)
+    fm = wx.modes("IMRPhenomXHM", f, **keywords)
+    assert fm.frequencies.shape == (2 * f.size,) and len(fm.modes) == 10
+    p2, c2 = fm.polarizations()
+    assert float((p2 - hp).abs().max()) <= 1e-12 * float(hp.abs().max())
+    assert float((c2 - hc).abs().max()) <= 1e-12 * float(hc.abs().max())
+    h21, h33 = fm[(2, -1)][f.size:], fm[(3, -3)][f.size:]
+    assert torch.equal(fm[(2, 1)][:f.size], torch.flip(torch.conj(h21), dims=(0,)))
+    assert torch.equal(fm[(3, 3)][:f.size], torch.flip(-torch.conj(h33), dims=(0,)))
+    params = np.array([[case[k] for k in (0, 1, 2, 3, 4, 5, 6, 8)]])
+    _, _, om = oxhm.polarizations(params, f, amp_scale=1e21, modes=True)
+    for k, (ell, m) in enumerate(oxhm.MODES):
+        assert relative_error(fm[(ell, -m)][f.size:].cpu().numpy(), om[k, 0]) <= BAR
+    # choose_fd_modes: the band [f_min, f_cut] of k delta_f, zero elsewhere
+    delta_f, f_min = 0.25, 20.0
+    chosen = wx.choose_fd_modes("IMRPhenomXHM", delta_f, f_min, 1024.0, **{**keywords, "f_ref": 25.0})
+    freqs = chosen.frequencies.cpu().numpy()
+    assert freqs[0] < 0 and np.count_nonzero(freqs == 0.0) == 1
+    h22 = chosen[(2, -2)].cpu().numpy()
+    positive = freqs >= 0
+    band = positive & (freqs >= int(f_min / delta_f) * delta_f) & (freqs <= 1024.0)
+    assert not np.any(h22[positive & ~band & (freqs < f_min)])
+    direct = wx.modes("IMRPhenomXHM", freqs[band], **{**keywords, "f_ref": 25.0})
+    assert relative_error(h22[band], direct[(2, -2)][np.count_nonzero(band):].cpu().numpy()) <= 1e-15

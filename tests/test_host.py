@@ -133,13 +133,14 @@ def test_frequency_grid():
 
 
 def test_registry():
-    assert wx.approximants() == ["IMRPhenomD", "IMRPhenomXAS"]
+    assert wx.approximants() == ["IMRPhenomD", "IMRPhenomXAS", "IMRPhenomXHM"]
     assert _api.model("IMRPhenomXAS") == "IMRPhenomXAS"
     assert _api.model("IMRPhenomD") == "IMRPhenomD"
+    assert _api.model("IMRPhenomXHM") == "IMRPhenomXHM"
     with pytest.raises(wx.UnknownApproximantError):
         _api.model("IMRPhenomQQ")
     with pytest.raises(wx.UnsupportedError):
-        _api.model("IMRPhenomXHM")
+        _api.model("IMRPhenomXPNR")
     with pytest.raises(wx.UnsupportedError):
         _api.model("NRSur7dq4")
     with pytest.raises(ValueError):
@@ -182,7 +183,7 @@ def test_registry_queries_and_single_binary_argument_errors():
     with pytest.raises(TypeError, match="unexpected keyword"):
         wx.polarizations("IMRPhenomXAS", [20.0, 30.0], mass_1=30.0, mass_2=20.0, colour="blue")
     with pytest.raises(wx.UnsupportedError):
-        wx.polarizations("IMRPhenomXHM", [20.0, 30.0], mass_1=30.0, mass_2=20.0)
+        wx.polarizations("IMRPhenomXPNR", [20.0, 30.0], mass_1=30.0, mass_2=20.0)
     # parameter and grid errors come from the host checks, GPU or not
     # (the reference's tests/test_api.py:64-80)
     with pytest.raises(wx.InvalidParameterError, match="masses"):
@@ -221,3 +222,31 @@ def test_model_contract_on_the_host():
     with pytest.raises(wx.InvalidParameterError, match="does not take"):
         models.check_reference_mf(wx.Binary(30.0, 20.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)),
                                   wx.WaveformOptions(reference_mf=0.01), "TaylorF2", supported=False)
+
+
+def test_xhm_host_surface():
+    """IMRPhenomXHM's model object, modes and harmonics on the host
+    (``models/xhm/api.py:54-126``, ``_harmonics.py:33-38``)."""
+    from waveformx_amd import _modes, models
+
+    model = models.get("IMRPhenomXHM")
+    assert model.domain == "frequency" and not model.precessing
+    options = wx.WaveformOptions()
+    with pytest.raises(wx.InvalidParameterError, match="IMRPhenomXHM is a non-precessing"):
+        model.validate(wx.Binary(30.0, 20.0, (0.1, 0.0, 0.0), (0.0, 0.0, 0.0)), [20.0, 30.0], options)
+    with pytest.raises(wx.UnsupportedError, match="every mode"):
+        model.validate(wx.Binary(30.0, 20.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)), [20.0, 30.0],
+                       wx.WaveformOptions(mode_array=((2, 2),)))
+    with pytest.raises(wx.UnsupportedError):
+        wx.modes("IMRPhenomD", [20.0, 30.0], mass_1=30.0, mass_2=20.0)
+    with pytest.raises(wx.InvalidParameterError, match="masses"):
+        wx.modes("IMRPhenomXHM", [20.0, 30.0], mass_1=-30.0, mass_2=20.0)
+    with pytest.raises(wx.InvalidParameterError, match="delta_f"):
+        wx.choose_fd_modes("IMRPhenomXHM", 0.0, 20.0, mass_1=30.0, mass_2=20.0)
+    with pytest.raises(wx.UnsupportedError):
+        wx.ragged_polarizations("IMRPhenomXHM", [np.array([20.0, 30.0])], np.ones((1, 12)))
+    theta, phi = 0.8, 0.4
+    c, s = np.cos(theta / 2), np.sin(theta / 2)
+    y44 = 3 * np.sqrt(7 / np.pi) * c ** 6 * s ** 2 * np.exp(4j * phi)
+    assert abs(_modes.spin_weighted_spherical_harmonic(theta, phi, 4, 4) - y44) < 1e-15
+

@@ -1,4 +1,4 @@
-"""waveformx_amd: IMRPhenomXAS (and IMRPhenomD) frequency-domain
+"""waveformx_amd: IMRPhenomXAS (and IMRPhenomD, IMRPhenomXHM) frequency-domain
 polarizations on MI355X.
 
 A drop-in for the reference's PyTorch path (``waveformx.torch.
@@ -10,6 +10,7 @@ by hand-written HIP kernels for gfx950 (``libwfx_xas.so``, C ABI in
 
 from ._api import (APPROXIMANTS, approximants, batch_polarizations, compute_device, is_frequency_domain,
                    is_precessing, polarizations, polarizations_on_grid, ragged_polarizations)
+from ._modes import FrequencyModes, choose_fd_modes, modes
 from ._types import (BINARY_NAMES, COLUMNS, REFERENCE_MF, BatchPolarizations, Binary, FrequencyGrid,
                      WaveformOptions, check_frequencies, parameter_table)
 from .exceptions import (InvalidParameterError, ModelError, UnknownApproximantError, UnsupportedError,
@@ -20,7 +21,8 @@ __all__ = [
     "batch_polarizations", "ragged_polarizations", "compute_device", "Binary",
     "BINARY_NAMES", "COLUMNS", "REFERENCE_MF", "BatchPolarizations", "FrequencyGrid", "WaveformOptions",
     "check_frequencies", "parameter_table", "WaveformError", "InvalidParameterError", "UnsupportedError",
-    "UnknownApproximantError", "ModelError", "polarizations_function",
+    "UnknownApproximantError", "ModelError", "polarizations_function", "modes", "choose_fd_modes",
+    "FrequencyModes",
 ]
 
 

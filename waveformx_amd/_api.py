@@ -5,8 +5,8 @@
 and :func:`ragged_polarizations` (each binary on its own grid, which the
 reference does not have).
 
-IMRPhenomXAS and IMRPhenomD are built here; the reference's other names
-raise :class:`UnsupportedError`, unknown names
+IMRPhenomXAS, IMRPhenomD and IMRPhenomXHM are built here; the
+reference's other names raise :class:`UnsupportedError`, unknown names
 :class:`UnknownApproximantError` (as ``models.get`` does).
 """
 
@@ -17,7 +17,7 @@ from typing import Any, Mapping, Optional, Sequence, Tuple
 import numpy as np
 import torch
 
-from . import phenomd, xas
+from . import phenomd, xas, xhm
 from ._types import (BINARY_NAMES, BatchPolarizations, Binary, FrequencyGrid, WaveformOptions, check_frequencies,
                      check_table, parameter_table, parse_options, strain_scale)
 from .exceptions import InvalidParameterError, ModelError, UnknownApproximantError, UnsupportedError
@@ -30,7 +30,7 @@ APPROXIMANTS = {
     "TaylorF2": ("frequency", False, False),
     "IMRPhenomD": ("frequency", False, True),
     "IMRPhenomXAS": ("frequency", False, True),
-    "IMRPhenomXHM": ("frequency", False, False),
+    "IMRPhenomXHM": ("frequency", False, True),
     "IMRPhenomXPNR": ("frequency", True, False),
     "IMRPhenomXPNR_NNLO": ("frequency", True, False),
     "IMRPhenomXPNR_MSA": ("frequency", True, False),
@@ -70,8 +70,8 @@ def model(approximant: str, domain: str = "frequency") -> str:
         what = "frequency-domain" if domain == "frequency" else "time-domain"
         raise UnsupportedError(f"{approximant} has no {what} waveforms")
     if not built:
-        raise UnsupportedError(f"{approximant} is not implemented in waveformx_amd (IMRPhenomXAS and IMRPhenomD "
-                               "are)")
+        raise UnsupportedError(f"{approximant} is not implemented in waveformx_amd (IMRPhenomXAS, IMRPhenomD and "
+                               "IMRPhenomXHM are)")
     return approximant
 
 
@@ -79,6 +79,8 @@ def _module(approximant: str):
     """The device module of a built approximant and its flag messages."""
     if approximant == "IMRPhenomD":
         return phenomd, phenomd.LAL_FLAGS
+    if approximant == "IMRPhenomXHM":
+        return xhm, xhm.LAL_FLAGS
     return xas, xas.LAL_FLAGS
 
 
@@ -201,6 +203,11 @@ def batch_polarizations(approximant: str, grid: Any, parameters: Any, *, threads
     if nf:
         for start in range(0, n, step):
             stop = min(n, start + step)
+            if approximant == "IMRPhenomXHM":
+                records, head, modes_, flags[start:stop] = xhm.setup(params[start:stop], freqs[:1],
+                                                                     strain_scale(options), options.reference_mf)
+                hplus[start:stop], hcross[start:stop] = xhm.evaluate(records, head, modes_, freqs)
+                continue
             records, flags[start:stop] = module.setup(params[start:stop], freqs[:1], strain_scale(options),
                                                       options.reference_mf, "fp64")
             module.evaluate(records, freqs, "fp64", out=(hplus[start:stop], hcross[start:stop]))
@@ -233,6 +240,8 @@ def ragged_polarizations(approximant: str, grids: Sequence[Any], parameters: Any
     ``hplus``, ``hcross`` and the (B+1,) int64 ``offsets`` of each binary's
     bins. No parameter checks (as the reference's pure functions)."""
     model(approximant)
+    if approximant == "IMRPhenomXHM":
+        raise UnsupportedError("ragged grids are built for IMRPhenomXAS and IMRPhenomD")
     options = parse_options(options, option_keywords)
     table = parameter_table(parameters)
     dev = compute_device(device)

@@ -41,7 +41,7 @@ SIGNATURES = {
     "wfx_phd_grid_adjoint": ([_P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P], _I),
     "wfx_xhm_record_slots": ([], _I),
     "wfx_xhm_head_slots": ([], _I),
-    "wfx_xhm_setup": ([_P, _I64, _P, _I64, _D, _P, _P, _P, _P, _P], _I),
+    "wfx_xhm_setup": ([_P, _I64, _P, _I64, _D, _D, _P, _P, _P, _P, _P], _I),
     "wfx_xhm_eval": ([_P, _P, _P, _I64, _P, _I, _I64, _P, _P, _P, _P], _I),
 }
 

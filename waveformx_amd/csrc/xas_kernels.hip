@@ -1067,13 +1067,15 @@ int grid_adjoint(const char* name, const void* records, int64_t n_binaries, cons
 // modes in the polarizations, NaN where LAL fails)
 __global__ __launch_bounds__(64) void xhm_setup_kernel(const double* __restrict__ params, int64_t nb,
                                                        const double* __restrict__ f_min, int64_t f_min_stride,
-                                                       double amp_scale, double* __restrict__ head,
-                                                       double* __restrict__ modes) {
+                                                       double amp_scale, double reference_mf,
+                                                       double* __restrict__ head, double* __restrict__ modes) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nb * kHmModes) return;
     const int64_t b = t / kHmModes;
     const int k = (int)(t % kHmModes);
-    const double* p = params + b * kParams;
+    double p[kParams];
+    for (int i = 0; i < kParams; ++i) p[i] = params[b * kParams + i];
+    with_reference_mf(p, reference_mf);
     const WaveformVars<double> wf = waveform_vars(p, f_min[b * f_min_stride], amp_scale);
     const AmpCoeffs<double> amp22 = amp_coeffs(wf);
     const PhaseCoeffs<double> ph22 = phase_coeffs(wf);
@@ -1352,7 +1354,8 @@ int wfx_xhm_record_slots(void) { return kHmSlots; }
 int wfx_xhm_head_slots(void) { return kHeadSlots; }
 
 int wfx_xhm_setup(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
-                  double amp_scale, double* records22, double* head, double* modes, uint32_t* flags, void* stream) {
+                  double amp_scale, double reference_mf, double* records22, double* head, double* modes,
+                  uint32_t* flags, void* stream) {
     if (n_binaries < 0) return fail("wfx_xhm_setup: negative number of binaries");
     if (n_binaries == 0) return 0;
     if (!params || !f_min || !records22 || !head || !modes) return fail("wfx_xhm_setup: null pointer");
@@ -1361,10 +1364,11 @@ int wfx_xhm_setup(const double* params, int64_t n_binaries, const double* f_min,
     const int blocks = grid_blocks((n_binaries * kHmModes + 63) / 64, "wfx_xhm_setup: too many binaries");
     if (blocks22 < 0 || blocks < 0) return -1;
     hipStream_t s = (hipStream_t)stream;
-    setup_f64_kernel<<<blocks22, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale,
-                                              __builtin_nan(""), records22, flags);
+    setup_f64_kernel<<<blocks22, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf,
+                                              records22, flags);
     if (check_launch("wfx_xhm_setup")) return -1;
-    xhm_setup_kernel<<<blocks, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, head, modes);
+    xhm_setup_kernel<<<blocks, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf, head,
+                                           modes);
     return check_launch("wfx_xhm_setup");
 }
 

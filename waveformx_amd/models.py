@@ -9,8 +9,9 @@
   ``phenomd/api.py:27-43``);
 * ``polarizations(binary, frequencies, options)``: the pure function (no
   checks; NaN rows where LAL fails), through the torch function;
-* ``modes`` / ``choose_fd_modes``: not implemented for these models, as in
-  the reference (:class:`UnsupportedError`).
+* ``modes`` / ``choose_fd_modes``: IMRPhenomXHM's in the conventions of
+  ``SimInspiralChooseFDModes``; not implemented for IMRPhenomXAS and
+  IMRPhenomD, as in the reference (:class:`UnsupportedError`).
 """
 
 from __future__ import annotations
@@ -24,7 +25,7 @@ from ._api import APPROXIMANTS, compute_device, model as _check_built
 from ._types import WaveformOptions, strain_scale
 from .exceptions import InvalidParameterError, UnsupportedError
 
-__all__ = ["APPROXIMANTS", "get", "check_reference_mf", "AlignedSpinModel"]
+__all__ = ["APPROXIMANTS", "get", "check_reference_mf", "AlignedSpinModel", "XHMModel"]
 
 
 def check_reference_mf(binary: Any, options: WaveformOptions, approximant: str, supported: bool = True) -> None:
@@ -105,7 +106,64 @@ class AlignedSpinModel:
         raise UnsupportedError(f"the modes of {self.approximant} are not implemented in waveformx")
 
 
+class XHMModel(AlignedSpinModel):
+    """The ``MODEL`` of IMRPhenomXHM (``models/xhm/api.py:54-126``):
+    ``validate`` as IMRPhenomXAS's with the name IMRPhenomXHM (and no mode
+    array), ``polarizations`` through the torch function, ``modes`` and
+    ``choose_fd_modes`` in ``SimInspiralChooseFDModes``'s conventions."""
+
+    def __init__(self):
+        super().__init__("IMRPhenomXHM")
+
+    def _module(self):
+        from . import xhm
+
+        return xhm, xhm.LAL_FLAGS
+
+    def validate(self, binary: Any, frequencies: Any, options: WaveformOptions) -> None:
+        if options.mode_array is not None:
+            raise UnsupportedError("IMRPhenomXHM computes every mode: select them from modes()")
+        check_reference_mf(binary, options, self.approximant)
+        spins = [_spin(binary.spin_1), _spin(binary.spin_2)]
+        if any(np.any(s[:2] != 0.0) for s in spins):
+            raise InvalidParameterError("non-zero in-plane spins were given, but IMRPhenomXHM is a non-precessing "
+                                        "approximant")
+        frequencies = np.asarray(frequencies, dtype=float)
+        if frequencies.size == 0:
+            return
+        from . import xhm
+
+        row = [_scalar(binary.mass_1), _scalar(binary.mass_2), spins[0][2], spins[1][2],
+               _scalar(binary.luminosity_distance), _scalar(binary.inclination), _scalar(binary.phi_ref),
+               _scalar(binary.f_ref)]
+        dev = compute_device()
+        params = torch.as_tensor([row], dtype=torch.float64, device=dev)
+        *_, flags = xhm.setup(params, torch.tensor([float(frequencies[0])], dtype=torch.float64, device=dev),
+                              strain_scale(options), options.reference_mf)
+        value = int(flags[0].item())
+        for bit, message in xhm.LAL_FLAGS:
+            if value & bit:
+                raise InvalidParameterError(message)
+
+    def modes(self, binary: Any, frequencies: Any, options: WaveformOptions):
+        from ._modes import modes
+
+        return modes(self.approximant, frequencies, options=options, **_keywords(binary))
+
+    def choose_fd_modes(self, binary: Any, delta_f: float, f_min: float, f_max: float, options: WaveformOptions):
+        from ._modes import choose_fd_modes
+
+        return choose_fd_modes(self.approximant, delta_f, f_min, f_max, options=options, **_keywords(binary))
+
+
+def _keywords(binary: Any) -> dict:
+    return dict(mass_1=_scalar(binary.mass_1), mass_2=_scalar(binary.mass_2), spin_1=tuple(_spin(binary.spin_1)),
+                spin_2=tuple(_spin(binary.spin_2)), luminosity_distance=_scalar(binary.luminosity_distance),
+                inclination=_scalar(binary.inclination), phi_ref=_scalar(binary.phi_ref), f_ref=_scalar(binary.f_ref))
+
+
 _MODELS = {name: AlignedSpinModel(name) for name in ("IMRPhenomXAS", "IMRPhenomD")}
+_MODELS["IMRPhenomXHM"] = XHMModel()
 
 
 def get(approximant: str) -> AlignedSpinModel:

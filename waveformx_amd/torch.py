@@ -16,7 +16,9 @@ out), float32 inputs the split-precision FP32 kernels (complex64 out); the
 keyword ``arithmetic`` ("fp64", "split", "split-df", "fp32"; IMRPhenomD has
 no "fp32": its table is FP64) overrides the choice. Tensors on the CPU are
 computed on the current HIP device and the results copied back.
-Approximants: IMRPhenomXAS and IMRPhenomD.
+Approximants: IMRPhenomXAS, IMRPhenomD and IMRPhenomXHM. IMRPhenomXHM runs
+its FP64 kernels whatever the inputs' precision (float32 inputs give the
+FP64 result as complex64), and has no backward pass here.
 
 The backward pass (``torch.autograd.Function``, once differentiable, the
 cotangents conjugated as in ``torch.py:108-119``) runs the model's
@@ -34,7 +36,7 @@ from typing import Any, Callable, Mapping, Optional
 import numpy as np
 import torch
 
-from . import _api, phenomd, xas
+from . import _api, phenomd, xas, xhm
 from ._types import BINARY_NAMES, Binary, WaveformOptions, parse_options, strain_scale
 
 __all__ = ["polarizations_function"]
@@ -73,7 +75,10 @@ def _as_tensor(value: Any, dtype: torch.dtype, device: torch.device) -> torch.Te
 
 
 def _arithmetic(approximant: str, dtype: torch.dtype, arithmetic: Optional[str]) -> str:
-    allowed = phenomd.PRECISIONS if approximant == "IMRPhenomD" else tuple(xas.PRECISIONS)
+    allowed = phenomd.PRECISIONS if approximant == "IMRPhenomD" else ("fp64",) if approximant == "IMRPhenomXHM" \
+        else tuple(xas.PRECISIONS)
+    if approximant == "IMRPhenomXHM" and arithmetic is None:
+        return "fp64"
     if arithmetic is None:
         return "fp64" if dtype == torch.float64 else "split"
     if arithmetic not in allowed:
@@ -90,8 +95,13 @@ class _ModelFunction(torch.autograd.Function):
         approximant, amp_scale, reference_mf, arithmetic = key
         params = xas.pack_params(mass_1, mass_2, spin_1[:, 2], spin_2[:, 2], distance, inclination, phi_ref, f_ref)
         freqs = grid.contiguous()
-        module = phenomd if approximant == "IMRPhenomD" else xas
-        hp, hc = module.polarizations(params, freqs, amp_scale, reference_mf, arithmetic)
+        if approximant == "IMRPhenomXHM":
+            hp, hc = xhm.polarizations(params, freqs, amp_scale, reference_mf)
+            if grid.dtype == torch.float32:
+                hp, hc = hp.to(torch.complex64), hc.to(torch.complex64)
+        else:
+            module = phenomd if approximant == "IMRPhenomD" else xas
+            hp, hc = module.polarizations(params, freqs, amp_scale, reference_mf, arithmetic)
         ctx.key = key
         ctx.save_for_backward(grid, params)
         return hp, hc
@@ -102,6 +112,8 @@ class _ModelFunction(torch.autograd.Function):
         from . import adjoint
 
         approximant, amp_scale, reference_mf, arithmetic = ctx.key
+        if approximant == "IMRPhenomXHM":
+            raise NotImplementedError("the gradient of IMRPhenomXHM is not implemented in waveformx_amd")
         grid, params = ctx.saved_tensors
         wrt_grid = ctx.needs_input_grad[1]
         grads, grad_grid = adjoint.vjp(params, grid.contiguous(), grad_hplus, grad_hcross, amp_scale, reference_mf,

@@ -38,11 +38,13 @@ LAL_FLAGS = ((1, "IMRPhenomXHM is not valid at mass ratios beyond 1000"),
              (16, "f_MECO cannot be greater than f_ISCO"))
 
 
-def setup(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21
+def setup(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21, reference_mf: Optional[float] = None
           ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
     """``(records22 (B, 64), head (B, H), modes (B, 4, S), flags (B,))``,
     float64 (int32 flags), for binaries ``params`` (B, 8) at first frequency
-    ``f_min`` (one value or (B,))."""
+    ``f_min`` (one value or (B,)); with ``reference_mf`` the binaries are
+    given at ``M f_ref = reference_mf`` (their f_ref zero, NaN rows
+    otherwise: ``xhm/model.py:40-42``)."""
     _require_device(params, "params")
     if params.ndim != 2 or params.shape[1] != 8 or params.dtype != torch.float64:
         raise ValueError("params must be a (B, 8) float64 tensor")
@@ -55,8 +57,9 @@ def setup(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21
     head = torch.empty((nb, handle.wfx_xhm_head_slots()), dtype=torch.float64, device=dev)
     modes_ = torch.empty((nb, 4, handle.wfx_xhm_record_slots()), dtype=torch.float64, device=dev)
     flags = torch.empty((nb,), dtype=torch.int32, device=dev)
-    call("wfx_xhm_setup", _ptr(params), nb, _ptr(f_min), stride, float(amp_scale), _ptr(records), _ptr(head),
-         _ptr(modes_), _ptr(flags), _stream(dev))
+    call("wfx_xhm_setup", _ptr(params), nb, _ptr(f_min), stride, float(amp_scale),
+         float("nan") if reference_mf is None else float(reference_mf), _ptr(records), _ptr(head), _ptr(modes_),
+         _ptr(flags), _stream(dev))
     return records, head, modes_, flags
 
 
@@ -81,15 +84,16 @@ def evaluate(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, fr
     return (hp, hc, out) if with_modes else (hp, hc)
 
 
-def polarizations(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21
-                  ) -> Tuple[torch.Tensor, torch.Tensor]:
+def polarizations(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21,
+                  reference_mf: Optional[float] = None) -> Tuple[torch.Tensor, torch.Tensor]:
     """``(h+, hx)`` of a batch on a shared grid: the set-up at the grid's
     first frequency, then the evaluation."""
-    records, head, modes_, _ = setup(params, freqs[:1], amp_scale)
+    records, head, modes_, _ = setup(params, freqs[:1], amp_scale, reference_mf)
     return evaluate(records, head, modes_, freqs)
 
 
-def modes(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21) -> torch.Tensor:
+def modes(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21,
+          reference_mf: Optional[float] = None) -> torch.Tensor:
     """The five modes ``h_{l,-m}`` of :data:`MODES`, (5, B, N) complex128."""
-    records, head, modes_, _ = setup(params, freqs[:1], amp_scale)
+    records, head, modes_, _ = setup(params, freqs[:1], amp_scale, reference_mf)
     return evaluate(records, head, modes_, freqs, with_modes=True)[2]
