@@ -39,6 +39,24 @@ EVAL_TRAFFIC = {(65536, 4017): 4.251e9}
 EVAL_VALU_PER_BIN = {(65536, 4017): 77.5}
 VALU_PEAK_TOPS = 256 * 4 * 64 / 4 * 2.4e9 / 1e12
 METRIC = "XAS waveforms/s @4017 bins: FP32-split vs FP64 GPU (mismatch<=1e-6); fwd+grad/s"
+# VALU instructions per bin of the IMRPhenomXHM FP64 evaluation at 4096 x
+# 8193 (SQ_INSTS_VALU per 64 bins from rocprofv3 --pmc, summary under
+# profiles/); FP64 FMA issues at the non-packed FP32 rate on this chip
+XHM_VALU_PER_BIN = {(4096, 8193): 1952.6}
+
+
+def draw_xhm(n, seed=4):
+    """The multimode workload's binaries (B, 8): as :func:`draw` with
+    q ~ U[0.1, 1]."""
+    rng = np.random.default_rng(seed)
+    m1 = rng.uniform(10.0, 80.0, n)
+    q = rng.uniform(0.1, 1.0, n)
+    c1 = rng.uniform(-0.9, 0.9, n)
+    c2 = rng.uniform(-0.9, 0.9, n)
+    dl = rng.uniform(100.0, 2000.0, n)
+    inc = np.arccos(rng.uniform(-1.0, 1.0, n))
+    phi = rng.uniform(0.0, 2 * np.pi, n)
+    return np.stack([m1, m1 * q, c1, c2, dl, inc, phi, np.full(n, 20.0)], axis=1)
 
 
 def draw(n, seed=1):
@@ -205,6 +223,62 @@ def bench_ragged(torch, xas, dev, nb, steps, warmup):
             "ragged_padded_bins": nb * longest, "ragged_shape": [plan.bins_per_thread, plan.blocks]}
 
 
+def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
+    """IMRPhenomXHM, the multimode workload: nb binaries on linspace(20,
+    2048, n) Hz, FP64 (the five modes summed into complex128 h+, hx): the
+    step (setup of the (2,2) record and the four higher-mode records, then
+    the evaluation) and the evaluation kernel alone, with its HBM and VALU
+    rooflines; the XHM oracle on the host cores on a bounded sample."""
+    from waveformx_amd import xhm
+
+    freqs = np.linspace(20.0, 2048.0, n)
+    params = torch.as_tensor(draw_xhm(nb), device=dev)
+    f64 = torch.as_tensor(freqs, device=dev)
+    out = (torch.empty((nb, n), dtype=torch.complex128, device=dev),
+           torch.empty((nb, n), dtype=torch.complex128, device=dev))
+    state = {}
+
+    def step():
+        state["r"] = xhm.setup(params, f64[:1])
+        xhm.evaluate(*state["r"][:3], f64, out=out)
+
+    ms = timed(torch, step, steps, warmup)
+    records, head, modes_, _ = state["r"]
+    ms_eval = timed(torch, lambda: xhm.evaluate(records, head, modes_, f64, out=out), steps, warmup)
+    ms_setup = timed(torch, lambda: xhm.setup(params, f64[:1]), steps, warmup)
+    bytes_per_launch = 32.0 * nb * n
+    achieved = bytes_per_launch / (ms_eval * 1e-3) / 1e9
+    result = {"xhm_value": nb / (ms * 1e-3), "xhm_unit": "waveforms/s", "xhm_ms_per_step": ms,
+              "xhm_eval_ms": ms_eval, "xhm_setup_ms": ms_setup,
+              "xhm_config": {"workload": f"IMRPhenomXHM {nb} binaries x {n} bins linspace(20, 2048) Hz, FP64, "
+                                         "modes (2,2), (2,1), (3,3), (3,2), (4,4)", "binaries": nb, "bins": n},
+              "xhm_roofline_hbm": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": achieved / HBM_PEAK_GBS}}
+    per_bin = XHM_VALU_PER_BIN.get((nb, n))
+    if per_bin is not None:
+        tops = per_bin * nb * n / (ms_eval * 1e-3) / 1e12
+        result["xhm_roofline"] = {"bound": "valu", "achieved": tops, "peak": VALU_PEAK_TOPS,
+                                  "unit": "T instructions/s", "frac": tops / VALU_PEAK_TOPS, "per_bin": per_bin}
+    del out, state
+    if cpu:
+        from oracle import xhm as oxhm
+
+        sample = 8192
+        cores = min(16, os.cpu_count() or 1)
+        p = draw_xhm(sample, seed=5)
+        oxhm.polarizations(p[:4], freqs, threads=cores)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oxhm.polarizations(p, freqs, threads=cores)
+            times.append(time.perf_counter() - t0)
+        t = float(np.median(times))
+        result["xhm_cpu_baseline"] = {"value": sample / t, "unit": "waveforms/s", "cores": cores, "kind": "port",
+                                      "sample": f"{sample} binaries x {n} bins, the FP64 C++ XHM oracle, OpenMP "
+                                                f"over binaries, median of 3 ({t:.2f} s each)"}
+    return result
+
+
 def valu_roofline(nb, n, ms_eval, group):
     """The evaluation's VALU instructions per second against the issue rate,
     for the workload and shape whose instruction count was measured."""
@@ -264,7 +338,10 @@ def main():
                     help="binaries of the fused forward + adjoint measurement (0: skip)")
     ap.add_argument("--ragged-binaries", type=int, default=16384,
                     help="binaries of the ragged-grid measurement (0: skip)")
-    ap.add_argument("--only", choices=["split", "split-df", "fp64", "fp32", "grad", "ragged"],
+    ap.add_argument("--xhm-binaries", type=int, default=4096,
+                    help="binaries of the IMRPhenomXHM leg (0: skip it)")
+    ap.add_argument("--xhm-bins", type=int, default=8193)
+    ap.add_argument("--only", choices=["split", "split-df", "fp64", "fp32", "grad", "ragged", "xhm"],
                     help="run just this arithmetic's step (setup + evaluation) K times, for a profiler")
     ap.add_argument("--sweep", action="store_true", help="time the evaluation's launch shapes and exit")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -291,6 +368,10 @@ def main():
     if args.only == "grad":
         print(json.dumps(bench_grad(torch, xas, dev, args.grad_binaries, args.bins, args.steps, args.warmup)),
               flush=True)
+        return
+    if args.only == "xhm":
+        print(json.dumps(bench_xhm(torch, dev, args.xhm_binaries, args.xhm_bins, args.steps, args.warmup,
+                                   cpu=False)), flush=True)
         return
     if args.only == "ragged":
         # the ragged workload's graph-replayed plan alone (tuned shape)
@@ -432,6 +513,9 @@ def main():
                                  approximant="IMRPhenomD", prefix="phenomd_"))
     if args.ragged_binaries:
         result.update(bench_ragged(torch, xas, dev, args.ragged_binaries, args.steps, args.warmup))
+    if args.xhm_binaries:
+        result.update(bench_xhm(torch, dev, args.xhm_binaries, args.xhm_bins, args.steps, args.warmup,
+                                cpu=not args.no_cpu))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_binaries, freqs64)
         result["cpu_baseline_1core"] = cpu_baseline(max(64, args.cpu_binaries // 32), freqs64, cores=1)

@@ -35,7 +35,7 @@ def lib():
     if not _ready:
         d = np.ctypeslib.ndpointer(np.float64, flags="C")
         L, D, I, P = ctypes.c_long, ctypes.c_double, ctypes.c_int, ctypes.c_void_p
-        handle.oracle_xhm_polarizations.argtypes = [d, L, d, L, D, d, d, P]
+        handle.oracle_xhm_polarizations.argtypes = [d, L, d, L, D, d, d, P, I]
         handle.oracle_xhm_mode_internals.argtypes = [d, D, I, d]
         handle.oracle_xhm_mode_amplitude_phase.argtypes = [d, D, I, d, L, d, d, P]
         handle.oracle_swsh.argtypes = [D, D, I, I, d]
@@ -59,10 +59,11 @@ def draw(nb: int, seed: int = 4) -> np.ndarray:
     return np.stack([m1, q * m1, chi1, chi2, distance, inclination, phi, np.full(nb, 20.0)], axis=1)
 
 
-def polarizations(params, freqs, amp_scale: float = 1.0, modes: bool = False):
+def polarizations(params, freqs, amp_scale: float = 1.0, modes: bool = False, threads: int = 0):
     """``(h_+, h_x)`` (complex128, (nb, n)) of IMRPhenomXHM on the grid
     ``freqs`` (f_min = freqs[0]) and, with ``modes``, the five modes
-    ``h_{l,-m}`` of :data:`MODES` as (5, nb, n)."""
+    ``h_{l,-m}`` of :data:`MODES` as (5, nb, n); ``threads`` OpenMP threads
+    (0: the default)."""
     params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
     freqs = np.ascontiguousarray(freqs, dtype=np.float64)
     nb, n = params.shape[0], freqs.shape[0]
@@ -70,7 +71,7 @@ def polarizations(params, freqs, amp_scale: float = 1.0, modes: bool = False):
     hc = np.empty((nb, n), np.complex128)
     out = np.empty((5, nb, n), np.complex128) if modes else None
     lib().oracle_xhm_polarizations(params, nb, freqs, n, amp_scale, hp.view(np.float64), hc.view(np.float64),
-                                   out.ctypes.data if modes else None)
+                                   out.ctypes.data if modes else None, threads)
     return (hp, hc, out) if modes else (hp, hc)
 
 

@@ -1226,12 +1226,14 @@ static void xhm_evaluate(const XhmSetup& s, double inclination, const double* fr
 extern "C" {
 
 // The polarizations (and, with modes non-null, the five modes h_{l,-m}
-// as (5, nb, n) complex) of nb binaries, params (nb, 8) = (m1, m2, chi1z,
+// as (5, nb, n) complex) of nb binaries (threads: OpenMP threads, 0 for
+// the default), params (nb, 8) = (m1, m2, chi1z,
 // chi2z, d_L, inclination, phi_ref, f_ref), on the grid freqs (n); f_min
 // = freqs[0]. hp, hc: (nb, n) complex as interleaved doubles.
 void oracle_xhm_polarizations(const double* params, long nb, const double* freqs, long n, double amp_scale,
-                              double* hp, double* hc, double* modes) {
-#pragma omp parallel for schedule(dynamic, 1)
+                              double* hp, double* hc, double* modes, int threads) {
+    const int nt = threads > 0 ? threads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
     for (long b = 0; b < nb; ++b) {
         XhmSetup s;
         xhm_setup(params + 8 * b, freqs[0], amp_scale, s);

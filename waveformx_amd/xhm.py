@@ -64,9 +64,10 @@ def setup(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21, re
 
 
 def evaluate(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, freqs: torch.Tensor,
-             with_modes: bool = False):
+             with_modes: bool = False, out: Optional[Tuple[torch.Tensor, torch.Tensor]] = None):
     """``(h+, hx)`` (B, N) complex128 on the shared grid ``freqs`` (N,)
-    (float32 or float64), and with ``with_modes`` the modes (5, B, N)."""
+    (float32 or float64), and with ``with_modes`` the modes (5, B, N);
+    ``out`` two contiguous (B, N) complex128 tensors to write into."""
     for t, name in ((records, "records"), (head, "head"), (modes_, "modes"), (freqs, "freqs")):
         _require_device(t, name)
     if freqs.dtype not in (torch.float32, torch.float64):
@@ -76,12 +77,18 @@ def evaluate(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, fr
         raise ValueError("records, head and modes must describe the same binaries")
     freqs = freqs.contiguous()
     dev = records.device
-    hp = torch.empty((nb, n), dtype=torch.complex128, device=dev)
-    hc = torch.empty((nb, n), dtype=torch.complex128, device=dev)
-    out = torch.empty((5, nb, n), dtype=torch.complex128, device=dev) if with_modes else None
+    if out is None:
+        hp = torch.empty((nb, n), dtype=torch.complex128, device=dev)
+        hc = torch.empty((nb, n), dtype=torch.complex128, device=dev)
+    else:
+        hp, hc = out
+        for t in out:
+            if t.shape != (nb, n) or t.dtype != torch.complex128 or not t.is_contiguous() or t.device != dev:
+                raise ValueError("out must be two contiguous (B, N) complex128 tensors on the records' device")
+    mode_out = torch.empty((5, nb, n), dtype=torch.complex128, device=dev) if with_modes else None
     call("wfx_xhm_eval", _ptr(records), _ptr(head), _ptr(modes_), nb, _ptr(freqs),
-         int(freqs.dtype == torch.float64), n, _ptr(hp), _ptr(hc), _ptr(out), _stream(dev))
-    return (hp, hc, out) if with_modes else (hp, hc)
+         int(freqs.dtype == torch.float64), n, _ptr(hp), _ptr(hc), _ptr(mode_out), _stream(dev))
+    return (hp, hc, mode_out) if with_modes else (hp, hc)
 
 
 def polarizations(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21,
