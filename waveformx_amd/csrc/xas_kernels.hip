@@ -1094,10 +1094,13 @@ __global__ __launch_bounds__(64) void xhm_setup_kernel(const double* __restrict_
 }
 
 // one thread per (binary, frequency), 256-thread blocks, a block on 256
-// consecutive frequencies of one binary (its records wave-uniform)
+// consecutive frequencies of one binary (its records wave-uniform). Three
+// waves per SIMD: the FP64 arithmetic is latency-bound at the two that 184
+// registers allow (168 registers and 20 B of spills: 2.44 ms against 3.48
+// ms at 4096 x 8193; four waves spill 140 B and take 2.83 ms)
 constexpr int kXhmBlock = 256;
 template <class F>
-__global__ __launch_bounds__(kXhmBlock) void xhm_eval_kernel(const double* __restrict__ rec22,
+__global__ __launch_bounds__(kXhmBlock) __attribute__((amdgpu_waves_per_eu(3))) void xhm_eval_kernel(const double* __restrict__ rec22,
                                                              const double* __restrict__ head,
                                                              const double* __restrict__ modes, int64_t nb,
                                                              const F* __restrict__ freqs, int64_t n, int64_t chunks,
