@@ -39,10 +39,11 @@ EVAL_TRAFFIC = {(65536, 4017): 4.251e9}
 EVAL_VALU_PER_BIN = {(65536, 4017): 77.5}
 VALU_PEAK_TOPS = 256 * 4 * 64 / 4 * 2.4e9 / 1e12
 METRIC = "XAS waveforms/s @4017 bins: FP32-split vs FP64 GPU (mismatch<=1e-6); fwd+grad/s"
-# VALU instructions per bin of the IMRPhenomXHM FP64 evaluation at 4096 x
-# 8193 (SQ_INSTS_VALU per 64 bins from rocprofv3 --pmc, summary under
-# profiles/); FP64 FMA issues at the non-packed FP32 rate on this chip
-XHM_VALU_PER_BIN = {(4096, 8193): 1952.6}
+# VALU instructions per bin of the IMRPhenomXHM evaluations at 4096 x 8193,
+# FP64 and split (SQ_INSTS_VALU x 64 / bins from rocprofv3 --pmc, summaries
+# under profiles/); FP64 FMA issues at the non-packed FP32 rate on this chip
+XHM_VALU_PER_BIN = {(4096, 8193): 1957.4}
+XHM_SPLIT_VALU_PER_BIN = {(4096, 8193): 629.9}
 
 
 def draw_xhm(n, seed=4):
@@ -225,10 +226,11 @@ def bench_ragged(torch, xas, dev, nb, steps, warmup):
 
 def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
     """IMRPhenomXHM, the multimode workload: nb binaries on linspace(20,
-    2048, n) Hz, FP64 (the five modes summed into complex128 h+, hx): the
-    step (setup of the (2,2) record and the four higher-mode records, then
-    the evaluation) and the evaluation kernel alone, with its HBM and VALU
-    rooflines; the XHM oracle on the host cores on a bounded sample."""
+    2048, n) Hz, the five modes summed into h+, hx, in FP64 (complex128) and
+    in the split arithmetic (complex64): each step (setup of the (2,2) record
+    and the four higher-mode records, then the evaluation), each evaluation
+    kernel alone with its HBM and VALU rooflines, the split path's mismatch
+    against FP64; the XHM oracle on the host cores on a bounded sample."""
     from waveformx_amd import xhm
 
     freqs = np.linspace(20.0, 2048.0, n)
@@ -254,6 +256,40 @@ def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
                                          "modes (2,2), (2,1), (3,3), (3,2), (4,4)", "binaries": nb, "bins": n},
               "xhm_roofline_hbm": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                    "frac": achieved / HBM_PEAK_GBS}}
+    # the split arithmetic (FP32, each mode's leading inspiral phase in FP64)
+    f32 = torch.as_tensor(freqs, dtype=torch.float32, device=dev)
+    out32 = (torch.empty((nb, n), dtype=torch.complex64, device=dev),
+             torch.empty((nb, n), dtype=torch.complex64, device=dev))
+
+    def step_split():
+        state["s"] = xhm.setup(params, f32[:1], precision="split")
+        xhm.evaluate(*state["s"][:3], f32, out=out32)
+
+    ms_split = timed(torch, step_split, steps, warmup)
+    srec = state["s"]
+    ms_split_eval = timed(torch, lambda: xhm.evaluate(*srec[:3], f32, out=out32), steps, warmup)
+    # its accuracy: noise-weighted mismatch against the FP64 path on the same float32 grid, every binary
+    f64r = f32.to(torch.float64)
+    xhm.evaluate(*xhm.setup(params, f64r[:1])[:3], f64r, out=out)
+    w = 1.0 / torch.as_tensor(((freqs / 100.0) ** -4 + 1.0 + (freqs / 100.0) ** 2), device=dev)
+    worst = 0.0
+    for a32, a64 in zip(out32, out):
+        x = a32.to(torch.complex128)
+        ab = (x * a64.conj()).real @ w
+        worst = max(worst, float((1.0 - ab / torch.sqrt(((x.abs() ** 2) @ w) * ((a64.abs() ** 2) @ w))).max()))
+    del out32
+    result.update({"xhm_split_value": nb / (ms_split * 1e-3), "xhm_split_ms_per_step": ms_split,
+                   "xhm_split_eval_ms": ms_split_eval, "xhm_split_over_fp64": ms / ms_split,
+                   "xhm_split_max_weighted_mismatch_vs_fp64": worst,
+                   "xhm_split_roofline_hbm": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                                              "achieved": 16.0 * nb * n / (ms_split_eval * 1e-3) / 1e9,
+                                              "frac": 16.0 * nb * n / (ms_split_eval * 1e-3) / 1e9 / HBM_PEAK_GBS}})
+    per_split = XHM_SPLIT_VALU_PER_BIN.get((nb, n))
+    if per_split is not None:
+        tops = per_split * nb * n / (ms_split_eval * 1e-3) / 1e12
+        result["xhm_split_roofline"] = {"bound": "valu", "achieved": tops, "peak": VALU_PEAK_TOPS,
+                                        "unit": "T instructions/s", "frac": tops / VALU_PEAK_TOPS,
+                                        "per_bin": per_split}
     per_bin = XHM_VALU_PER_BIN.get((nb, n))
     if per_bin is not None:
         tops = per_bin * nb * n / (ms_eval * 1e-3) / 1e12

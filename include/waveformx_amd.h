@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (12); a binding checks it before any call. */
+/* The version of this ABI (13); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -202,22 +202,28 @@ int wfx_phd_loss_adjoint(const void* records, const double* jacobian, int64_t n_
  * higher-mode record and of one binary's head.
  *
  * wfx_xhm_setup: for params (B, 8) and reference_mf as wfx_xas_setup's
- * (xhm/model.py:40-42), writes records22 (B, 64) FP64 (the (2,2) mode:
- * wfx_xas_setup's precision 0 record), head (B, wfx_xhm_head_slots()) (the factors of the
- * five modes (2,2), (2,1), (3,3), (3,2), (4,4) in the polarizations, NaN
- * where LAL fails), modes (B, 4, wfx_xhm_record_slots()) (the higher
- * modes) and flags (B,) (LAL's error bits, as wfx_xas_setup; may be null).
+ * (xhm/model.py:40-42) and precision 0 (FP64) or 2 (split), writes
+ * records22 (B, 64) (the (2,2) mode: wfx_xas_setup's record of that
+ * precision: double or float), head (B, wfx_xhm_head_slots()) (the factors
+ * of the five modes (2,2), (2,1), (3,3), (3,2), (4,4) in the polarizations,
+ * NaN where LAL fails), modes (B, 4, wfx_xhm_record_slots()) (the higher
+ * modes) -- head and modes double for FP64, float for split -- and flags
+ * (B,) (LAL's error bits, as wfx_xas_setup; may be null). The set-up is
+ * FP64 for both.
  *
- * wfx_xhm_eval: (h+, hx) (B, N) complex128 on the shared grid freqs (N,)
- * (float64 if freqs_f64, else float32), and, if mode_out is not null, the
- * modes h_{l,-m} (5, B, N) complex128 in the order above. */
+ * wfx_xhm_eval: (h+, hx) (B, N) on the shared grid freqs (N,) and, if
+ * mode_out is not null, the modes h_{l,-m} (5, B, N) in the order above:
+ * complex128 for precision 0 (freqs float64 if freqs_f64, else float32),
+ * complex64 for precision 2 (split: FP32 with each mode's leading inspiral
+ * phase in FP64, the reference's float32 design, xhm/core.py:13-21; freqs
+ * float32). */
 int wfx_xhm_record_slots(void);
 int wfx_xhm_head_slots(void);
 int wfx_xhm_setup(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
-                  double amp_scale, double reference_mf, double* records22, double* head, double* modes,
+                  double amp_scale, double reference_mf, int precision, void* records22, void* head, void* modes,
                   uint32_t* flags, void* stream);
-int wfx_xhm_eval(const double* records22, const double* head, const double* modes, int64_t n_binaries,
-                 const void* freqs, int freqs_f64, int64_t n_freqs, void* hplus, void* hcross, void* mode_out,
+int wfx_xhm_eval(const void* records22, const void* head, const void* modes, int64_t n_binaries, const void* freqs,
+                 int freqs_f64, int64_t n_freqs, int precision, void* hplus, void* hcross, void* mode_out,
                  void* stream);
 
 #ifdef __cplusplus

@@ -179,3 +179,53 @@ def test_modes_and_their_sum(cuda):
     assert not np.any(h22[positive & ~band & (freqs < f_min)])
     direct = wx.modes("IMRPhenomXHM", freqs[band], **{**keywords, "f_ref": 25.0})
     assert relative_error(h22[band], direct[(2, -2)][np.count_nonzero(band):].cpu().numpy()) <= 1e-15
+
+
+def mismatch(a, b, w=None):
+    """1 - Re<a, b> / sqrt(<a, a><b, b>), uniform or with weights w."""
+    w = np.ones(a.shape[-1]) if w is None else w
+    ab = np.sum((a * np.conj(b)).real * w)
+    return 1.0 - ab / np.sqrt(np.sum(np.abs(a) ** 2 * w) * np.sum(np.abs(b) ** 2 * w))
+
+
+def test_split_against_fp64(cuda):
+    """The split arithmetic (FP32, each mode's leading inspiral phase in
+    FP64) against the FP64 path on the same float32 grid, at the reference's
+    float32 bars (``tests/test_xhm.py:156-178``): mismatch <= 1e-6 for the
+    polarizations (uniform, and noise-weighted) and <= 2e-4 per mode, on the
+    reference's cases, a random batch and a binary neutron star on a
+    1/256 Hz grid."""
+    import torch
+
+    from conftest import psd
+    from waveformx_amd import xhm
+
+    batches = []
+    for case in CASES:
+        f = np.geomspace(case[7], min(0.3 / ((case[0] + case[1]) * MTSUN), 2048.0), 1000)
+        batches.append((np.array([[case[k] for k in (0, 1, 2, 3, 4, 5, 6, 8)]]), f))
+    batches.append((oxhm.draw(64, seed=7), np.linspace(20.0, 2048.0, 8193)))
+    bns = np.array([[1.42, 1.27, 0.05, -0.03, 100.0, 0.5, 1.0, 0.0]])
+    batches.append((bns, 20.0 + np.arange(2 ** 18) / 256.0))
+    worst = {"polarization": 0.0, "weighted": 0.0, "mode": 0.0}
+    for params, f in batches:
+        f32 = torch.as_tensor(f, dtype=torch.float32, device=cuda)
+        p = torch.as_tensor(params, device=cuda)
+        m32 = xhm.modes(p, f32, precision="split")
+        m64 = xhm.modes(p, f32.to(torch.float64), precision="fp64")
+        hp32, hc32 = xhm.polarizations(p, f32, precision="split")
+        hp64, hc64 = xhm.polarizations(p, f32.to(torch.float64), precision="fp64")
+        assert m32.dtype == torch.complex64 and hp32.dtype == torch.complex64
+        w = 1.0 / psd(f32.cpu().numpy().astype(np.float64))
+        for b in range(params.shape[0]):
+            for a32, a64 in ((hp32[b], hp64[b]), (hc32[b], hc64[b])):
+                a32 = a32.cpu().numpy().astype(np.complex128)
+                a64 = a64.cpu().numpy()
+                worst["polarization"] = max(worst["polarization"], mismatch(a32, a64))
+                worst["weighted"] = max(worst["weighted"], mismatch(a32, a64, w))
+            for k in range(5):
+                a64 = m64[k, b].cpu().numpy()
+                if np.any(a64):
+                    worst["mode"] = max(worst["mode"], mismatch(m32[k, b].cpu().numpy().astype(np.complex128), a64))
+    print("split vs FP64, worst mismatches:", worst)
+    assert worst["polarization"] <= 1e-6 and worst["weighted"] <= 1e-6 and worst["mode"] <= 2e-4

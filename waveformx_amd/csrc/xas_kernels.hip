@@ -1067,8 +1067,8 @@ int grid_adjoint(const char* name, const void* records, int64_t n_binaries, cons
 // modes in the polarizations, NaN where LAL fails)
 __global__ __launch_bounds__(64) void xhm_setup_kernel(const double* __restrict__ params, int64_t nb,
                                                        const double* __restrict__ f_min, int64_t f_min_stride,
-                                                       double amp_scale, double reference_mf,
-                                                       double* __restrict__ head, double* __restrict__ modes) {
+                                                       double amp_scale, double reference_mf, int split,
+                                                       void* __restrict__ head, void* __restrict__ modes) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nb * kHmModes) return;
     const int64_t b = t / kHmModes;
@@ -1081,15 +1081,25 @@ __global__ __launch_bounds__(64) void xhm_setup_kernel(const double* __restrict_
     const PhaseCoeffs<double> ph22 = phase_coeffs(wf);
     HmCoeffs c;
     hm_setup(wf, amp22, ph22, k, c);
-    write_mode_record(c, wf, modes + (b * kHmModes + k) * kHmSlots);
+    const int64_t at = (b * kHmModes + k) * kHmSlots;
+    if (split)
+        write_mode_record_split(c, wf, (float*)modes + at);
+    else
+        write_mode_record(c, wf, (double*)modes + at);
     if (k == 0) {
-        double* h = head + b * kHeadSlots;
+        double h[kHeadSlots];
         for (int s = 0; s < kHeadSlots; ++s) h[s] = 0.0;
         const double inclination = p[kInclination];
         mode_factors(inclination, 2, 2, h + kHeadFactors);
         for (int m = 0; m < kHmModes; ++m) mode_factors(inclination, kHmEll[m], kHmEmm[m], h + kHeadFactors + 4 * (m + 1));
         if (wf.flags)
             for (int s = 0; s < 20; ++s) h[kHeadFactors + s] = __builtin_nan("");
+        for (int s = 0; s < kHeadSlots; ++s) {
+            if (split)
+                ((float*)head)[b * kHeadSlots + s] = (float)h[s];
+            else
+                ((double*)head)[b * kHeadSlots + s] = h[s];
+        }
     }
 }
 
@@ -1123,6 +1133,32 @@ __global__ __launch_bounds__(kXhmBlock) __attribute__((amdgpu_waves_per_eu(3))) 
     }
 }
 
+// the split evaluation: the same shape, FP32 with the inspiral's leading
+// phase terms in FP64
+__global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* __restrict__ rec22,
+                                                                   const float* __restrict__ head,
+                                                                   const float* __restrict__ modes, int64_t nb,
+                                                                   const float* __restrict__ freqs, int64_t n,
+                                                                   int64_t chunks, float2* __restrict__ hp,
+                                                                   float2* __restrict__ hc,
+                                                                   float2* __restrict__ mode_out) {
+    const int64_t b = blockIdx.x / chunks;
+    const int64_t i = (blockIdx.x % chunks) * kXhmBlock + threadIdx.x;
+    if (b >= nb || i >= n) return;
+    float h[5][2];
+    xhm_modes_split(rec22 + b * kSlots, modes + b * kHmModes * kHmSlots, freqs[i], h);
+    float2 p, c;
+    xhm_sum_f32(head + b * kHeadSlots, h, p, c);
+    hp[b * n + i] = p;
+    hc[b * n + i] = c;
+    if (mode_out) {
+        const float nan = __builtin_nanf("");
+        const bool bad = head[b * kHeadSlots] != head[b * kHeadSlots];
+        for (int k = 0; k < 5; ++k)
+            mode_out[(k * nb + b) * n + i] = bad ? make_float2(nan, nan) : make_float2(h[k][0], h[k][1]);
+    }
+}
+
 }  // namespace
 
 // ================================================================= C ABI
@@ -1132,7 +1168,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 12; }
+int wfx_abi_version(void) { return 13; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -1357,43 +1393,59 @@ int wfx_xhm_record_slots(void) { return kHmSlots; }
 int wfx_xhm_head_slots(void) { return kHeadSlots; }
 
 int wfx_xhm_setup(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
-                  double amp_scale, double reference_mf, double* records22, double* head, double* modes,
+                  double amp_scale, double reference_mf, int precision, void* records22, void* head, void* modes,
                   uint32_t* flags, void* stream) {
     if (n_binaries < 0) return fail("wfx_xhm_setup: negative number of binaries");
     if (n_binaries == 0) return 0;
     if (!params || !f_min || !records22 || !head || !modes) return fail("wfx_xhm_setup: null pointer");
     if (f_min_stride != 0 && f_min_stride != 1) return fail("wfx_xhm_setup: f_min_stride must be 0 or 1");
+    if (precision != kPrecF64 && precision != kPrecSplit)
+        return fail("wfx_xhm_setup: precision must be 0 (FP64) or 2 (split)");
     const int blocks22 = grid_blocks((n_binaries + 63) / 64, "wfx_xhm_setup: too many binaries");
     const int blocks = grid_blocks((n_binaries * kHmModes + 63) / 64, "wfx_xhm_setup: too many binaries");
     if (blocks22 < 0 || blocks < 0) return -1;
     hipStream_t s = (hipStream_t)stream;
-    setup_f64_kernel<<<blocks22, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf,
-                                              records22, flags);
+    if (precision == kPrecF64)
+        setup_f64_kernel<<<blocks22, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf,
+                                                  (double*)records22, flags);
+    else
+        setup_split_kernel<<<blocks22, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf,
+                                                    (float*)records22, flags);
     if (check_launch("wfx_xhm_setup")) return -1;
-    xhm_setup_kernel<<<blocks, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf, head,
-                                           modes);
+    xhm_setup_kernel<<<blocks, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf,
+                                           precision == kPrecSplit, head, modes);
     return check_launch("wfx_xhm_setup");
 }
 
-int wfx_xhm_eval(const double* records22, const double* head, const double* modes, int64_t n_binaries,
-                 const void* freqs, int freqs_f64, int64_t n_freqs, void* hplus, void* hcross, void* mode_out,
+int wfx_xhm_eval(const void* records22, const void* head, const void* modes, int64_t n_binaries, const void* freqs,
+                 int freqs_f64, int64_t n_freqs, int precision, void* hplus, void* hcross, void* mode_out,
                  void* stream) {
     if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xhm_eval: negative size");
     if (n_binaries == 0 || n_freqs == 0) return 0;
     if (!records22 || !head || !modes || !freqs || !hplus || !hcross) return fail("wfx_xhm_eval: null pointer");
+    if (precision != kPrecF64 && precision != kPrecSplit)
+        return fail("wfx_xhm_eval: precision must be 0 (FP64) or 2 (split)");
+    if (precision == kPrecSplit && freqs_f64) return fail("wfx_xhm_eval: the split evaluation takes a float32 grid");
     const int64_t chunks = (n_freqs + kXhmBlock - 1) / kXhmBlock;
     if (n_binaries > INT64_MAX / chunks) return fail("wfx_xhm_eval: grid too large");
     const int blocks = grid_blocks(n_binaries * chunks, "wfx_xhm_eval: grid too large");
     if (blocks < 0) return -1;
     hipStream_t s = (hipStream_t)stream;
-    if (freqs_f64)
-        xhm_eval_kernel<double><<<blocks, kXhmBlock, 0, s>>>(records22, head, modes, n_binaries,
-                                                              (const double*)freqs, n_freqs, chunks,
-                                                              (double2*)hplus, (double2*)hcross, (double2*)mode_out);
+    if (precision == kPrecSplit)
+        xhm_eval_split_kernel<<<blocks, kXhmBlock, 0, s>>>((const float*)records22, (const float*)head,
+                                                           (const float*)modes, n_binaries, (const float*)freqs,
+                                                           n_freqs, chunks, (float2*)hplus, (float2*)hcross,
+                                                           (float2*)mode_out);
+    else if (freqs_f64)
+        xhm_eval_kernel<double><<<blocks, kXhmBlock, 0, s>>>((const double*)records22, (const double*)head,
+                                                              (const double*)modes, n_binaries, (const double*)freqs,
+                                                              n_freqs, chunks, (double2*)hplus, (double2*)hcross,
+                                                              (double2*)mode_out);
     else
-        xhm_eval_kernel<float><<<blocks, kXhmBlock, 0, s>>>(records22, head, modes, n_binaries,
-                                                             (const float*)freqs, n_freqs, chunks,
-                                                             (double2*)hplus, (double2*)hcross, (double2*)mode_out);
+        xhm_eval_kernel<float><<<blocks, kXhmBlock, 0, s>>>((const double*)records22, (const double*)head,
+                                                             (const double*)modes, n_binaries, (const float*)freqs,
+                                                             n_freqs, chunks, (double2*)hplus, (double2*)hcross,
+                                                             (double2*)mode_out);
     return check_launch("wfx_xhm_eval");
 }
 

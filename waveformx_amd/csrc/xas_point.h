@@ -174,8 +174,9 @@ __device__ inline float reduce_f32(float x) {
     return fmaf(-turns, kTwoPiLoF, fmaf(-turns, kTwoPiHiF, x));
 }
 
+// the amplitude and the phase (reduced to [-pi, pi]) of the (2,2) mode in FP32
 template <int L>
-__device__ inline void point_f32(const float* __restrict__ c, const PowF32& p, float2& hp, float2& hc) {
+__device__ inline void amp_phase_f32(const float* __restrict__ c, const PowF32& p, float& amp, float& phi) {
     const float f = p.f;
     const float t = p.t;
     const float m1 = p.m1;
@@ -210,7 +211,6 @@ __device__ inline void point_f32(const float* __restrict__ c, const PowF32& p, f
         rest = fmaf(c[kAtanA2], fast_atan(c[kAtanW2] * (f - c[kShift])), rest);
     }
 
-    float phi;
     if (past) {
         phi = reduce_f32(rest);  // the constants hold -A (fold_anchor)
     } else if (L == kLeadF64) {
@@ -226,7 +226,6 @@ __device__ inline void point_f32(const float* __restrict__ c, const PowF32& p, f
         phi = reduce_f32(fmaf(c[kT0], p.m53_hi, rest - c[kAnchor]));
     }
 
-    float amp;
     if (f > c[kFAmpRd]) {
         const float u = c[kAmpRdS] * (f - c[kShift]);
         amp = fast_div(c[kAmpRdN] * p.m76 * __expf(-c[kAmpRdGr] * u), fmaf(u, u, c[kAmpRdG]));
@@ -251,6 +250,12 @@ __device__ inline void point_f32(const float* __restrict__ c, const PowF32& p, f
         amp = p.m76 * poly;
     }
     amp = f <= c[kFCut] ? amp : 0.0f;
+}
+
+template <int L>
+__device__ inline void point_f32(const float* __restrict__ c, const PowF32& p, float2& hp, float2& hc) {
+    float amp, phi;
+    amp_phase_f32<L>(c, p, amp, phi);
     float s, co;
     __sincosf(phi, &s, &co);
     const float ap = c[kKp] * amp;

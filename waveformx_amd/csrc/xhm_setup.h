@@ -1105,6 +1105,50 @@ WFX_HD inline void write_mode_record(const HmCoeffs& c, const WaveformVars<doubl
     r[kHmMix + 3] = c.mix3_im;
 }
 
+// --------------------------------------------------------- split record
+
+// The FP32 record of a mode for the split evaluation: the FP64 record's
+// values rounded, except
+//  * the leading inspiral term phi_0 (M f)^(-5/3) = T0_lm f^(-5/3) and the
+//    inspiral's constant c_insp + dphi_lm, kept in FP64 (two float slots
+//    each): the evaluation sums them in FP64 and reduces modulo 2 pi before
+//    adding the rest of the inspiral phase in FP32;
+//  * the intermediate and ringdown constants (dphi_lm, c_rd + dphi_lm) and
+//    the spheroidal phase constant phi0_s reduced modulo 2 pi in FP64: past
+//    the inspiral the phase is an FP32 sum of terms of <~1e3 rad;
+//  * quotients folded as the evaluation uses them (the pseudo-PN
+//    coefficients over their cut-off powers, 1 / f_h, the Lorentzian's
+//    rd_0 f_damp, rd_1 / (f_damp rd_2), (f_damp rd_2)^2);
+//  * log M, M^(1/3) and 1 / M, which give the powers of M f from those of f.
+enum HmSplitSlot : int { kHmT0F64 = 112, kHmKInsF64 = 114, kHmLogM = 116, kHmCbrtM = 117, kHmInvM = 118 };
+
+WFX_HD inline double fold_two_pi(double x) { return x - kTwoPi * rint(x * (1.0 / kTwoPi)); }
+WFX_HD inline void store_f64(float* r, int slot, double v) { __builtin_memcpy(r + slot, &v, sizeof(double)); }
+
+WFX_HD inline void write_mode_record_split(const HmCoeffs& c, const WaveformVars<double>& wf, float* r) {
+    double d[kHmSlots];
+    write_mode_record(c, wf, d);
+    for (int s = 0; s < kHmSlots; ++s) r[s] = (float)d[s];
+    r[kHmPhi] = 0.0f;  // (the leading term is the FP64 one)
+    r[kHmIns + 0] = (float)(c.ins[0] / c.fcut73);
+    r[kHmIns + 1] = (float)(c.ins[1] / c.fcut83);
+    r[kHmIns + 2] = (float)(c.ins[2] / c.fcut3);
+    r[kHmInterH] = (float)(1.0 / c.inter_half);
+    const double dfd = c.f_damp * c.rd[2];
+    r[kHmRd + 0] = (float)(c.rd[0] * c.f_damp);
+    r[kHmRd + 1] = (float)(c.rd[1] / dfd);
+    r[kHmRd + 2] = (float)(dfd * dfd);
+    r[kHmDeltaPhi] = (float)fold_two_pi(c.deltaphi);
+    r[kHmCRd] = (float)fold_two_pi(c.c_rd + c.deltaphi);
+    r[kHmSph + 4] = (float)fold_two_pi(c.phi0_s);
+    const Powers<double> pm = useful_powers(wf.m_sec);
+    store_f64(r, kHmT0F64, c.phi[0] * pm.m_five_thirds);
+    store_f64(r, kHmKInsF64, c.c_insp + c.deltaphi);
+    r[kHmLogM] = (float)pm.log;
+    r[kHmCbrtM] = (float)pm.one_third;
+    r[kHmInvM] = (float)pm.m_one;
+}
+
 // ------------------------------------------------------------ harmonics
 
 // _harmonics.py:17-38 Wigner's d^l_{m' m}(beta) and {}_{-2}Y_{lm}

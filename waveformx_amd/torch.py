@@ -16,9 +16,8 @@ out), float32 inputs the split-precision FP32 kernels (complex64 out); the
 keyword ``arithmetic`` ("fp64", "split", "split-df", "fp32"; IMRPhenomD has
 no "fp32": its table is FP64) overrides the choice. Tensors on the CPU are
 computed on the current HIP device and the results copied back.
-Approximants: IMRPhenomXAS, IMRPhenomD and IMRPhenomXHM. IMRPhenomXHM runs
-its FP64 kernels whatever the inputs' precision (float32 inputs give the
-FP64 result as complex64), and has no backward pass here.
+Approximants: IMRPhenomXAS, IMRPhenomD and IMRPhenomXHM (arithmetics
+"fp64" and "split"; no backward pass here).
 
 The backward pass (``torch.autograd.Function``, once differentiable, the
 cotangents conjugated as in ``torch.py:108-119``) runs the model's
@@ -75,10 +74,8 @@ def _as_tensor(value: Any, dtype: torch.dtype, device: torch.device) -> torch.Te
 
 
 def _arithmetic(approximant: str, dtype: torch.dtype, arithmetic: Optional[str]) -> str:
-    allowed = phenomd.PRECISIONS if approximant == "IMRPhenomD" else ("fp64",) if approximant == "IMRPhenomXHM" \
-        else tuple(xas.PRECISIONS)
-    if approximant == "IMRPhenomXHM" and arithmetic is None:
-        return "fp64"
+    allowed = phenomd.PRECISIONS if approximant == "IMRPhenomD" else xhm.PRECISIONS \
+        if approximant == "IMRPhenomXHM" else tuple(xas.PRECISIONS)
     if arithmetic is None:
         return "fp64" if dtype == torch.float64 else "split"
     if arithmetic not in allowed:
@@ -96,8 +93,8 @@ class _ModelFunction(torch.autograd.Function):
         params = xas.pack_params(mass_1, mass_2, spin_1[:, 2], spin_2[:, 2], distance, inclination, phi_ref, f_ref)
         freqs = grid.contiguous()
         if approximant == "IMRPhenomXHM":
-            hp, hc = xhm.polarizations(params, freqs, amp_scale, reference_mf)
-            if grid.dtype == torch.float32:
+            hp, hc = xhm.polarizations(params, freqs, amp_scale, reference_mf, arithmetic)
+            if grid.dtype == torch.float32 and hp.dtype == torch.complex128:
                 hp, hc = hp.to(torch.complex64), hc.to(torch.complex64)
         else:
             module = phenomd if approximant == "IMRPhenomD" else xas

@@ -14,7 +14,11 @@ HIP kernels.
 * :func:`polarizations` / :func:`modes` -- set-up at the grid's first
   frequency, then the evaluation.
 
-The arithmetic is FP64 throughout (the reference's float64 path).
+Two arithmetics, both with an FP64 set-up: ``"fp64"`` (complex128 out; the
+reference's float64 path) and ``"split"`` (FP32 with each mode's leading
+inspiral phase term and constant in FP64 and the later regions' constants
+reduced modulo 2 pi: the reference's float32 design, ``xhm/core.py:13-21``;
+complex64 out, float32 grid).
 """
 
 from __future__ import annotations
@@ -26,10 +30,12 @@ import torch
 from ._lib import call, lib
 from .xas import RECORD_SLOTS, _f_min, _ptr, _require_device, _stream
 
-__all__ = ["MODES", "LAL_FLAGS", "setup", "evaluate", "polarizations", "modes"]
+__all__ = ["MODES", "LAL_FLAGS", "PRECISIONS", "setup", "evaluate", "polarizations", "modes"]
 
 #: the modes (l, m) of the outputs, the (2,2) first (``xhm/reference.py:55-57``)
 MODES = ((2, 2), (2, 1), (3, 3), (3, 2), (4, 4))
+PRECISIONS = ("fp64", "split")
+_CODES = {"fp64": 0, "split": 2}
 #: the setup's flag bits -> LAL's messages: those of the waveform variables
 #: (``xas/api.py:40-60``, checked for IMRPhenomXHM by ``xhm/api.py:67-72``)
 LAL_FLAGS = ((1, "IMRPhenomXHM is not valid at mass ratios beyond 1000"),
@@ -38,13 +44,16 @@ LAL_FLAGS = ((1, "IMRPhenomXHM is not valid at mass ratios beyond 1000"),
              (16, "f_MECO cannot be greater than f_ISCO"))
 
 
-def setup(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21, reference_mf: Optional[float] = None
-          ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+def setup(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21, reference_mf: Optional[float] = None,
+          precision: str = "fp64") -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
     """``(records22 (B, 64), head (B, H), modes (B, 4, S), flags (B,))``,
-    float64 (int32 flags), for binaries ``params`` (B, 8) at first frequency
-    ``f_min`` (one value or (B,)); with ``reference_mf`` the binaries are
-    given at ``M f_ref = reference_mf`` (their f_ref zero, NaN rows
-    otherwise: ``xhm/model.py:40-42``)."""
+    float64 for ``"fp64"``, float32 for ``"split"`` (int32 flags), for
+    binaries ``params`` (B, 8) at first frequency ``f_min`` (one value or
+    (B,)); with ``reference_mf`` the binaries are given at ``M f_ref =
+    reference_mf`` (their f_ref zero, NaN rows otherwise:
+    ``xhm/model.py:40-42``)."""
+    if precision not in PRECISIONS:
+        raise ValueError(f"IMRPhenomXHM's arithmetic is one of {PRECISIONS}, got {precision!r}")
     _require_device(params, "params")
     if params.ndim != 2 or params.shape[1] != 8 or params.dtype != torch.float64:
         raise ValueError("params must be a (B, 8) float64 tensor")
@@ -53,54 +62,66 @@ def setup(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21, re
     f_min, stride = _f_min(f_min, nb)
     handle = lib()
     dev = params.device
-    records = torch.empty((nb, RECORD_SLOTS), dtype=torch.float64, device=dev)
-    head = torch.empty((nb, handle.wfx_xhm_head_slots()), dtype=torch.float64, device=dev)
-    modes_ = torch.empty((nb, 4, handle.wfx_xhm_record_slots()), dtype=torch.float64, device=dev)
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    records = torch.empty((nb, RECORD_SLOTS), dtype=dtype, device=dev)
+    head = torch.empty((nb, handle.wfx_xhm_head_slots()), dtype=dtype, device=dev)
+    modes_ = torch.empty((nb, 4, handle.wfx_xhm_record_slots()), dtype=dtype, device=dev)
     flags = torch.empty((nb,), dtype=torch.int32, device=dev)
     call("wfx_xhm_setup", _ptr(params), nb, _ptr(f_min), stride, float(amp_scale),
-         float("nan") if reference_mf is None else float(reference_mf), _ptr(records), _ptr(head), _ptr(modes_),
-         _ptr(flags), _stream(dev))
+         float("nan") if reference_mf is None else float(reference_mf), _CODES[precision], _ptr(records),
+         _ptr(head), _ptr(modes_), _ptr(flags), _stream(dev))
     return records, head, modes_, flags
 
 
 def evaluate(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, freqs: torch.Tensor,
              with_modes: bool = False, out: Optional[Tuple[torch.Tensor, torch.Tensor]] = None):
-    """``(h+, hx)`` (B, N) complex128 on the shared grid ``freqs`` (N,)
-    (float32 or float64), and with ``with_modes`` the modes (5, B, N);
-    ``out`` two contiguous (B, N) complex128 tensors to write into."""
+    """``(h+, hx)`` (B, N) on the shared grid ``freqs`` (N,): complex128 from
+    FP64 records (float32 or float64 grid), complex64 from split records
+    (float32 grid); with ``with_modes`` also the modes (5, B, N); ``out``
+    two contiguous (B, N) tensors of that type to write into."""
     for t, name in ((records, "records"), (head, "head"), (modes_, "modes"), (freqs, "freqs")):
         _require_device(t, name)
     if freqs.dtype not in (torch.float32, torch.float64):
         raise ValueError("freqs must be float32 or float64")
+    split = records.dtype == torch.float32
+    if split and freqs.dtype != torch.float32:
+        raise ValueError("the split evaluation takes a float32 grid")
+    if head.dtype != records.dtype or modes_.dtype != records.dtype:
+        raise ValueError("records, head and modes must come from one setup")
     nb, n = records.shape[0], freqs.shape[0]
     if head.shape[0] != nb or modes_.shape[0] != nb:
         raise ValueError("records, head and modes must describe the same binaries")
     freqs = freqs.contiguous()
     dev = records.device
+    ctype = torch.complex64 if split else torch.complex128
     if out is None:
-        hp = torch.empty((nb, n), dtype=torch.complex128, device=dev)
-        hc = torch.empty((nb, n), dtype=torch.complex128, device=dev)
+        hp = torch.empty((nb, n), dtype=ctype, device=dev)
+        hc = torch.empty((nb, n), dtype=ctype, device=dev)
     else:
         hp, hc = out
         for t in out:
-            if t.shape != (nb, n) or t.dtype != torch.complex128 or not t.is_contiguous() or t.device != dev:
-                raise ValueError("out must be two contiguous (B, N) complex128 tensors on the records' device")
-    mode_out = torch.empty((5, nb, n), dtype=torch.complex128, device=dev) if with_modes else None
+            if t.shape != (nb, n) or t.dtype != ctype or not t.is_contiguous() or t.device != dev:
+                raise ValueError(f"out must be two contiguous (B, N) {ctype} tensors on the records' device")
+    mode_out = torch.empty((5, nb, n), dtype=ctype, device=dev) if with_modes else None
     call("wfx_xhm_eval", _ptr(records), _ptr(head), _ptr(modes_), nb, _ptr(freqs),
-         int(freqs.dtype == torch.float64), n, _ptr(hp), _ptr(hc), _ptr(mode_out), _stream(dev))
+         int(freqs.dtype == torch.float64), n, 2 if split else 0, _ptr(hp), _ptr(hc), _ptr(mode_out), _stream(dev))
     return (hp, hc, mode_out) if with_modes else (hp, hc)
 
 
 def polarizations(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21,
-                  reference_mf: Optional[float] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+                  reference_mf: Optional[float] = None, precision: str = "fp64") -> Tuple[torch.Tensor, torch.Tensor]:
     """``(h+, hx)`` of a batch on a shared grid: the set-up at the grid's
-    first frequency, then the evaluation."""
-    records, head, modes_, _ = setup(params, freqs[:1], amp_scale, reference_mf)
+    first frequency, then the evaluation (``"split"`` on the float32 grid)."""
+    if precision == "split":
+        freqs = freqs.to(torch.float32)
+    records, head, modes_, _ = setup(params, freqs[:1], amp_scale, reference_mf, precision)
     return evaluate(records, head, modes_, freqs)
 
 
 def modes(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21,
-          reference_mf: Optional[float] = None) -> torch.Tensor:
-    """The five modes ``h_{l,-m}`` of :data:`MODES`, (5, B, N) complex128."""
-    records, head, modes_, _ = setup(params, freqs[:1], amp_scale, reference_mf)
+          reference_mf: Optional[float] = None, precision: str = "fp64") -> torch.Tensor:
+    """The five modes ``h_{l,-m}`` of :data:`MODES`, (5, B, N)."""
+    if precision == "split":
+        freqs = freqs.to(torch.float32)
+    records, head, modes_, _ = setup(params, freqs[:1], amp_scale, reference_mf, precision)
     return evaluate(records, head, modes_, freqs, with_modes=True)[2]
