@@ -229,3 +229,34 @@ def test_split_against_fp64(cuda):
                     worst["mode"] = max(worst["mode"], mismatch(m32[k, b].cpu().numpy().astype(np.complex128), a64))
     print("split vs FP64, worst mismatches:", worst)
     assert worst["polarization"] <= 1e-6 and worst["weighted"] <= 1e-6 and worst["mode"] <= 2e-4
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[8] > 0][:5], ids=lambda c: f"{c[0]}+{c[1]}")
+def test_reference_invariance_of_the_modes(cuda, case):
+    """The reference's ``tests/test_xhm.py:181-197``: a binary given at f_ref and
+    the same binary converted by IMRPhenomXAS's converters to the mass-scaled
+    reference give the same modes under ``reference_mf``, the odd modes up
+    to a half-turn (the converters fix phi_ref modulo pi, the odd modes turn
+    by e^{i m delta})."""
+    import torch
+
+    import waveformx_amd as wx
+    from waveformx_amd import reference
+
+    m1, m2, c1, c2, distance, inclination, phi, f_min, f_ref = case
+    f = np.geomspace(f_min, min(0.3 / ((m1 + m2) * MTSUN), 1024.0), 400)
+    keywords = dict(mass_1=m1, mass_2=m2, spin_1=(0.0, 0.0, c1), spin_2=(0.0, 0.0, c2), luminosity_distance=distance,
+                    inclination=inclination, phi_ref=phi, f_ref=f_ref)
+    physical = wx.modes("IMRPhenomXHM", f, **keywords)
+    binary = wx.Binary(torch.tensor(m1, dtype=torch.float64), m2, (0.0, 0.0, c1), (0.0, 0.0, c2), distance,
+                       inclination, phi, f_ref)
+    scaled = reference.to_mass_scaled_reference(binary, 0.01)
+    assert float(scaled.f_ref) == 0.0
+    converted = wx.modes("IMRPhenomXHM", f, reference_mf=0.01,
+                         **{**keywords, "phi_ref": float(scaled.phi_ref), "f_ref": 0.0})
+    for (ell, m), h in physical.modes.items():
+        a, b = converted[(ell, m)].cpu().numpy(), h.cpu().numpy()
+        if not np.any(b):
+            continue
+        err = min(relative_error(a, b), relative_error(-a, b)) if m % 2 else relative_error(a, b)
+        assert err < 1e-9, ((ell, m), err)
