@@ -290,6 +290,18 @@ def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
         result["xhm_split_roofline"] = {"bound": "valu", "achieved": tops, "peak": VALU_PEAK_TOPS,
                                         "unit": "T instructions/s", "frac": tops / VALU_PEAK_TOPS,
                                         "per_bin": per_split}
+    # the gradient (xhm.vjp: differences of the FP64 path with its region selections held), a quarter of
+    # the binaries: forward + vector-Jacobian product of the noise-weighted residual
+    ng = max(1, nb // 4)
+    pg, fg = params[:ng].contiguous(), f64
+
+    def fwd_grad():
+        hp, hc = xhm.polarizations(pg, fg)
+        xhm.vjp(pg, fg, hp * w, hc * w)
+
+    ms_grad = timed(torch, fwd_grad, max(2, steps // 5), 1)
+    result.update({"xhm_fwd_grad_binaries": ng, "xhm_fwd_grad_ms": ms_grad,
+                   "xhm_fwd_grad_value": ng / (ms_grad * 1e-3)})
     per_bin = XHM_VALU_PER_BIN.get((nb, n))
     if per_bin is not None:
         tops = per_bin * nb * n / (ms_eval * 1e-3) / 1e12

@@ -102,3 +102,35 @@ def swsh(theta: float, phi: float, ell: int, m: int) -> complex:
     out = np.empty(2)
     lib().oracle_swsh(theta, phi, ell, m, out)
     return complex(out[0], out[1])
+
+
+def jacobian(params, freqs, amp_scale: float = 1.0, threads: int = 0):
+    """d(h_+, h_x)/d(m1, m2, chi1z, chi2z, d_L, inclination, phi_ref, f_ref),
+    each (nb, n, 8) complex128, by fourth-order central differences of
+    :func:`polarizations` (steps: masses 1e-6 relative, spins 2e-6,
+    d_L 1e-5 relative, angles 2e-3, f_ref 2e-6 relative; zero for f_ref = 0,
+    which takes f_min). The gradient checker of the GPU path's differences,
+    on grids away from the (2,2)'s cutoff and region bounds, where the
+    difference of the whole model is that of one smooth formula."""
+    params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+    nb, n = params.shape[0], len(freqs)
+    jp = np.zeros((nb, n, 8), np.complex128)
+    jc = np.zeros((nb, n, 8), np.complex128)
+    scale = np.abs(params)
+    steps = [1e-6 * scale[:, 0], 1e-6 * scale[:, 1], np.full(nb, 2e-6), np.full(nb, 2e-6), 1e-5 * scale[:, 4],
+             np.full(nb, 2e-3), np.full(nb, 2e-3), 2e-6 * scale[:, 7]]
+    for j, h in enumerate(steps):
+        if j == 7 and not np.any(params[:, 7] > 0):
+            continue
+        acc_p = np.zeros((nb, n), np.complex128)
+        acc_c = np.zeros((nb, n), np.complex128)
+        for offset, weight in ((-2, 1.0 / 12.0), (-1, -8.0 / 12.0), (1, 8.0 / 12.0), (2, -1.0 / 12.0)):
+            p = params.copy()
+            p[:, j] += offset * h
+            hp, hc = polarizations(p, freqs, amp_scale, threads=threads)
+            acc_p += weight * hp
+            acc_c += weight * hc
+        safe = np.where(h > 0, h, 1.0)[:, None]
+        jp[:, :, j] = np.where(h[:, None] > 0, acc_p / safe, 0.0)
+        jc[:, :, j] = np.where(h[:, None] > 0, acc_c / safe, 0.0)
+    return jp, jc

@@ -91,8 +91,8 @@ def test_random_batch(cuda):
 
 def test_public_functions(cuda):
     """IMRPhenomXHM through the reference's interface: the torch function
-    (float64 and float32 inputs, a batch axis), batch_polarizations with
-    failing rows, the mass-scaled reference, and the backward pass raising
+    (float64 and float32 inputs, a batch axis, a backward pass),
+    batch_polarizations with failing rows, the mass-scaled reference
     (``torch.py:175-183``, ``_api.py:307-360``, ``xhm/model.py:40-42``)."""
     import torch
 
@@ -116,8 +116,8 @@ def test_public_functions(cuda):
     batch = wx.Binary(m1, case[1], (0.0, 0.0, case[2]), (0.0, 0.0, case[3]), case[4], case[5], case[6], case[8])
     bp, _ = function(batch, torch.as_tensor(f, device=cuda))
     assert bp.shape == (2, f.size) and torch.equal(bp[0], hp)
-    with pytest.raises(NotImplementedError):
-        bp.abs().sum().backward()
+    bp.abs().sum().backward()
+    assert m1.grad.shape == (2,) and bool(torch.isfinite(m1.grad).all())
     # batch_polarizations: a failing row (q > 1000) is zero and reported
     table = wx.parameter_table({"mass_1": np.array([case[0], 2000.0]), "mass_2": np.array([case[1], 1.0]),
                                 "spin_1": np.array([[0, 0, case[2]], [0, 0, 0]]),
@@ -260,3 +260,85 @@ def test_reference_invariance_of_the_modes(cuda, case):
             continue
         err = min(relative_error(a, b), relative_error(-a, b)) if m % 2 else relative_error(a, b)
         assert err < 1e-9, ((ell, m), err)
+
+
+def test_degenerate_sizes_and_errors(cuda):
+    """Empty batches and grids, one frequency, a failing row (NaN in every
+    output and mode), and the argument checks of the device functions."""
+    import torch
+
+    from waveformx_amd import xhm
+
+    f = torch.linspace(20.0, 512.0, 300, dtype=torch.float64, device=cuda)
+    none = torch.zeros((0, 8), dtype=torch.float64, device=cuda)
+    for precision in ("fp64", "split"):
+        hp, hc = xhm.polarizations(none, f, precision=precision)
+        assert hp.shape == (0, 300) and hc.shape == (0, 300)
+    params = torch.as_tensor(oxhm.draw(3, seed=11), device=cuda)
+    records, head, modes_, flags = xhm.setup(params, f[:1])
+    hp, hc = xhm.evaluate(records, head, modes_, f[:0])
+    assert hp.shape == (3, 0)
+    one, _ = xhm.evaluate(records, head, modes_, f[:1])
+    full, _ = xhm.evaluate(records, head, modes_, f)
+    assert torch.equal(one[:, 0], full[:, 0])
+    bad = params.clone()
+    bad[1, 0], bad[1, 1] = 2000.0, 1.0  # q > 1000: LAL's error, NaN row
+    records, head, modes_, flags = xhm.setup(bad, f[:1])
+    assert int(flags[1].item()) & 1 and not bool(flags[[0, 2]].any())
+    hp, hc, m = xhm.evaluate(records, head, modes_, f, with_modes=True)
+    assert bool(torch.isnan(hp[1]).all()) and bool(torch.isnan(m[:, 1]).all())
+    assert bool(torch.isfinite(hp[[0, 2]]).all()) and bool(torch.isfinite(torch.view_as_real(m[:, [0, 2]])).all())
+    with pytest.raises(ValueError, match="float32 grid"):
+        s = xhm.setup(params, f[:1], precision="split")
+        xhm.evaluate(*s[:3], f)
+    with pytest.raises(ValueError, match="out"):
+        xhm.evaluate(records, head, modes_, f, out=(torch.empty((3, 300), dtype=torch.complex64, device=cuda),) * 2)
+    with pytest.raises(ValueError, match="arithmetic"):
+        xhm.setup(params, f[:1], precision="fp32")
+    with pytest.raises(ValueError, match="params"):
+        xhm.setup(params[:, :7].contiguous(), f[:1])
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[6]], ids=lambda c: f"{c[0]}+{c[1]}")
+def test_vjp_against_oracle_differences(cuda, case):
+    """The vector-Jacobian product of the FP64 path (xhm.vjp: differences of
+    the device path with its region selections held) against the oracle's
+    fourth-order differences, for a random cotangent, on a grid below the
+    (2,2)'s cutoff: within 1e-6 of each parameter's largest derivative; and
+    the torch backward equals it."""
+    import torch
+
+    import waveformx_amd as wx
+    import waveformx_amd.torch as wt
+    from waveformx_amd import xhm
+
+    params = np.array([[case[k] for k in (0, 1, 2, 3, 4, 5, 6, 8)]])
+    f = np.geomspace(case[7], 0.2 / ((case[0] + case[1]) * MTSUN), 500)
+    rng = np.random.default_rng(3)
+    op, oc = oxhm.polarizations(params, f, amp_scale=1e21)
+    scale = np.abs(op).max()
+    cot_p = (rng.normal(size=f.size) + 1j * rng.normal(size=f.size))[None] * scale
+    cot_c = (rng.normal(size=f.size) + 1j * rng.normal(size=f.size))[None] * scale
+    jp, jc = oxhm.jacobian(params, f, amp_scale=1e21)
+    expected = (np.einsum("bf,bfk->bk", np.conj(cot_p), jp) + np.einsum("bf,bfk->bk", np.conj(cot_c), jc)).real
+    got = xhm.vjp(torch.as_tensor(params, device=cuda), torch.as_tensor(f, device=cuda),
+                  torch.as_tensor(cot_p, device=cuda), torch.as_tensor(cot_c, device=cuda), 1e21).cpu().numpy()
+    scale_g = np.max(np.abs(expected), axis=0)
+    err = np.abs(got - expected) / np.where(scale_g > 0, scale_g, 1.0)
+    print("vjp vs oracle differences:", err.max(axis=0))
+    assert np.all(err <= 1e-6), err
+    # the torch backward: L = Re sum conj(cot) h
+    fields = [torch.tensor(case[k], dtype=torch.float64, device=cuda, requires_grad=True) for k in range(9)]
+    binary = wx.Binary(fields[0], fields[1], torch.stack([fields[2] * 0, fields[2] * 0, fields[2]]),
+                       torch.stack([fields[3] * 0, fields[3] * 0, fields[3]]), fields[4], fields[5], fields[6],
+                       fields[8])
+    hp, hc = wt.polarizations_function("IMRPhenomXHM")(binary, torch.as_tensor(f, device=cuda))
+    loss = (torch.conj(torch.as_tensor(cot_p[0], device=cuda)) * hp).real.sum() + \
+        (torch.conj(torch.as_tensor(cot_c[0], device=cuda)) * hc).real.sum()
+    loss.backward()
+    torch_grads = [float(fields[k].grad) for k in (0, 1, 2, 3, 4, 5, 6, 8)]
+    assert np.allclose(torch_grads, got[0], rtol=1e-12, atol=0.0)
+    with pytest.raises(NotImplementedError):
+        grid = torch.as_tensor(f, device=cuda).requires_grad_(True)
+        hp, _ = wt.polarizations_function("IMRPhenomXHM")(wx.Binary(30.0, 20.0, (0, 0, 0), (0, 0, 0)), grid)
+        hp.abs().sum().backward()

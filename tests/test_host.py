@@ -250,3 +250,26 @@ def test_xhm_host_surface():
     y44 = 3 * np.sqrt(7 / np.pi) * c ** 6 * s ** 2 * np.exp(4j * phi)
     assert abs(_modes.spin_weighted_spherical_harmonic(theta, phi, 4, 4) - y44) < 1e-15
 
+
+def test_xhm_selection_slots_match_the_headers():
+    """The record slots xhm.vjp holds fixed are the headers' region bounds
+    and branch flags (csrc/xhm_setup.h HmSlot, csrc/xas_layout.h XasSlot)."""
+    from waveformx_amd import xhm
+
+    csrc = ROOT / "waveformx_amd" / "csrc"
+
+    def slots(text):
+        out, last = {}, -1
+        body = re.sub(r"//.*", "", text)
+        for name, value in re.findall(r"\b(k[A-Z]\w*)\s*(?:=\s*(\d+))?\s*[,}]", body):
+            last = int(value) if value else last + 1
+            out[name] = last
+        return out
+
+    hm = slots((csrc / "xhm_setup.h").read_text().split("enum HmSlot : int {")[1].split("};")[0] + "}")
+    xas = slots((csrc / "xas_layout.h").read_text().split("enum XasSlot : int {")[1].split("};")[0] + "}")
+    assert xhm.MODE_SELECTION_SLOTS == tuple(hm[k] for k in ("kHmTag", "kHmOn", "kHmFalloff", "kHmMixing",
+                                                             "kHmFAmpIn", "kHmFAmpIm", "kHmFPhaseIn",
+                                                             "kHmFPhaseIm", "kHmFRdAux", "kHmNInter"))
+    assert xhm.XAS_SELECTION_SLOTS == tuple(xas[k] for k in ("kFPhaseIn", "kFPhaseIm", "kFAmpIn", "kFAmpRd", "kFCut"))
+

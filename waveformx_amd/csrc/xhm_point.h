@@ -19,7 +19,8 @@ WFX_HD inline void hm_amp_phase(const double* r, double mf, const Powers<double>
     double rot_re = 0.0, rot_im = 0.0;
     if (mixing && (mf >= r[kHmFAmpIm] || mf >= r[kHmFPhaseIm])) {
         // conj(-a322) h22 + conj(-a323) A_s e^{i phi_s} (reference.py:366-378)
-        const double amplm = rd_amp(mf, f_ring, f_damp, r[kHmFalloff], true, r + kHmRd, r + kHmRdAux, r[kHmFRdAux]);
+        const double amplm = rd_amp(mf, f_ring, f_damp, r[kHmFalloff], true, r + kHmRd, r + kHmRdAux, r[kHmFRdAux],
+                                    r[kHmFalloffAt]);
         const double philm = rd_phase_int(mf, p, f_ring, f_damp, true, 0.0, 0.0, 0.0, r[kHmSph], r[kHmSph + 1],
                                           r[kHmSph + 2], r[kHmSph + 3], r[kHmSph + 4]);
         const double m2r = r[kHmMix], m2i = r[kHmMix + 1], m3r = r[kHmMix + 2], m3i = r[kHmMix + 3];
@@ -37,7 +38,8 @@ WFX_HD inline void hm_amp_phase(const double* r, double mf, const Powers<double>
                          r[kHmFcut83], r[kHmFcut3], r + kHmIns);
     } else if (mf >= r[kHmFAmpIm]) {
         a = mixing ? m_hypot(rot_re, rot_im)
-                   : rd_amp(mf, f_ring, f_damp, r[kHmFalloff], false, r + kHmRd, r + kHmRdAux, r[kHmFRdAux]);
+                   : rd_amp(mf, f_ring, f_damp, r[kHmFalloff], false, r + kHmRd, r + kHmRdAux, r[kHmFRdAux],
+                            r[kHmFalloffAt]);
     } else {
         // reference.py:337-345, in u = (f - f_c) / f_h
         const double u = (mf - r[kHmInterC]) / r[kHmInterH];
@@ -134,7 +136,7 @@ __device__ inline void hm_amp_phase_split(const float* __restrict__ r, const MfP
             const float* x = r + kHmRdAux;
             amplm = fmaf(fmaf(fmaf(x[3], mf, x[2]), mf, x[1]), mf, x[0]);
         } else if (mf >= r[kHmFalloff]) {
-            amplm = r[kHmRd + 3] * __expf(-r[kHmRd + 4] * (mf - r[kHmFalloff]));
+            amplm = r[kHmRd + 3] * __expf(-r[kHmRd + 4] * (mf - r[kHmFalloffAt]));
         } else {
             const float dfr = mf - f_ring;
             amplm = fast_div(r[kHmRd], __expf(r[kHmRd + 1] * dfr) * fmaf(dfr, dfr, r[kHmRd + 2]));
@@ -166,7 +168,7 @@ __device__ inline void hm_amp_phase_split(const float* __restrict__ r, const MfP
         if (mixing) {
             a = __builtin_sqrtf(fmaf(rot_re, rot_re, rot_im * rot_im));
         } else if (mf >= r[kHmFalloff]) {
-            a = r[kHmRd + 3] * __expf(-r[kHmRd + 4] * (mf - r[kHmFalloff]));
+            a = r[kHmRd + 3] * __expf(-r[kHmRd + 4] * (mf - r[kHmFalloffAt]));
         } else {
             const float dfr = mf - f_ring;
             a = fast_div(r[kHmRd], __expf(r[kHmRd + 1] * dfr) * fmaf(dfr, dfr, r[kHmRd + 2]));

@@ -164,6 +164,7 @@ enum HmSlot : int {
     kHmCRd, kHmDeltaPhi,
     kHmSph = 96,    // 5: alpha0_s, alphaL_s, alpha2_s, alpha4_s, phi0_s
     kHmMix = 101,   // 4: -a322 (re, im), -a323 (re, im)
+    kHmFalloffAt = 119,  // the fall-off's position in its formula (kHmFalloff: its region's bound)
     kHmSlots = 128
 };
 // the head: (F+, Fx) of each mode (re, im), mode order (2,2), (2,1), (3,3),
@@ -436,11 +437,14 @@ WFX_HD inline R hm_dlorentzian(const R& mf, double f_ring, double fda, const dou
     denom = rd[2] * denom * denom * m_exp(dfr * rd[1] / (fda * rd[2]));
     return -numerator / denom;
 }
-// reference.py:223-236 the ringdown amplitude (fall-off; the (3,2)'s cubic)
+// reference.py:223-236 the ringdown amplitude (fall-off; the (3,2)'s cubic);
+// falloff_at is the fall-off's position in its formula, falloff the bound of
+// its region (the same value: the records keep them apart so that the bound
+// can be held fixed while the position varies, as a jnp.where selection)
 template <class R>
 WFX_HD inline R rd_amp(const R& mf, double f_ring, double f_damp, double falloff, bool mixing, const double* rd,
-                       const double* aux, double f_rd_aux) {
-    R out = mf >= falloff ? rd[3] * m_exp(-rd[4] * (mf - falloff)) : hm_lorentzian(mf, f_ring, f_damp, rd);
+                       const double* aux, double f_rd_aux, double falloff_at) {
+    R out = mf >= falloff ? rd[3] * m_exp(-rd[4] * (mf - falloff_at)) : hm_lorentzian(mf, f_ring, f_damp, rd);
     if (mixing) {
         const R cubic = m_fma(m_fma(m_fma(R(aux[3]), mf, R(aux[2])), mf, R(aux[1])), mf, R(aux[0]));
         out = mf < f_rd_aux ? cubic : out;
@@ -480,7 +484,8 @@ WFX_HD inline R rd_phase_int(const R& ff, const Powers<R>& p, double frd, double
 template <class R>
 WFX_HD inline void rotate(const R& mf, const Powers<R>& p, const HmCoeffs& c, const R& w_re, const R& w_im,
                           R& out_re, R& out_im) {
-    const R amplm = rd_amp(mf, c.f_ring, c.f_damp, c.f_amp_rd_falloff, true, c.rd, c.rd_aux, c.f_rd_aux);
+    const R amplm = rd_amp(mf, c.f_ring, c.f_damp, c.f_amp_rd_falloff, true, c.rd, c.rd_aux, c.f_rd_aux,
+                           c.f_amp_rd_falloff);
     const R philm = rd_phase_int(mf, p, c.f_ring, c.f_damp, true, 0.0, 0.0, 0.0, c.alpha0_s, c.alpha_l_s,
                                  c.alpha2_s, c.alpha4_s, c.phi0_s);
     const R a_re = w_re * c.mix2_re + w_im * c.mix2_im;
@@ -681,7 +686,8 @@ WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<double>& wf, const
         rd_value = h.v;
         rd_derivative = h.d;
     } else {
-        rd_value = rd_amp(f_rd, c.f_ring, c.f_damp, c.f_amp_rd_falloff, false, c.rd, c.rd_aux, c.f_rd_aux);
+        rd_value = rd_amp(f_rd, c.f_ring, c.f_damp, c.f_amp_rd_falloff, false, c.rd, c.rd_aux, c.f_rd_aux,
+                          c.f_amp_rd_falloff);
         rd_derivative = hm_dlorentzian(f_rd, c.f_ring, c.f_damp, c.rd);
     }
     for (int i = 0; i < n_colloc; ++i) {
@@ -1059,6 +1065,7 @@ WFX_HD inline void write_mode_record(const HmCoeffs& c, const WaveformVars<doubl
     r[kHmFRing] = c.f_ring;
     r[kHmFDamp] = c.f_damp;
     r[kHmFalloff] = c.f_amp_rd_falloff;
+    r[kHmFalloffAt] = c.f_amp_rd_falloff;
     r[kHmMixing] = c.mixing ? 1.0 : 0.0;
     r[kHmFAmpIn] = c.f_amp_match_in;
     r[kHmFAmpIm] = c.f_amp_match_im;

@@ -17,7 +17,8 @@ keyword ``arithmetic`` ("fp64", "split", "split-df", "fp32"; IMRPhenomD has
 no "fp32": its table is FP64) overrides the choice. Tensors on the CPU are
 computed on the current HIP device and the results copied back.
 Approximants: IMRPhenomXAS, IMRPhenomD and IMRPhenomXHM (arithmetics
-"fp64" and "split"; no backward pass here).
+"fp64" and "split"; its backward, to the binary's fields but not the grid,
+differentiates the FP64 path numerically: ``xhm.vjp``).
 
 The backward pass (``torch.autograd.Function``, once differentiable, the
 cotangents conjugated as in ``torch.py:108-119``) runs the model's
@@ -109,12 +110,16 @@ class _ModelFunction(torch.autograd.Function):
         from . import adjoint
 
         approximant, amp_scale, reference_mf, arithmetic = ctx.key
-        if approximant == "IMRPhenomXHM":
-            raise NotImplementedError("the gradient of IMRPhenomXHM is not implemented in waveformx_amd")
         grid, params = ctx.saved_tensors
         wrt_grid = ctx.needs_input_grad[1]
-        grads, grad_grid = adjoint.vjp(params, grid.contiguous(), grad_hplus, grad_hcross, amp_scale, reference_mf,
-                                       arithmetic, wrt_grid, approximant)
+        if approximant == "IMRPhenomXHM":
+            if wrt_grid:
+                raise NotImplementedError("the gradient of IMRPhenomXHM with respect to the frequencies is not "
+                                          "implemented in waveformx_amd")
+            grads, grad_grid = xhm.vjp(params, grid, grad_hplus, grad_hcross, amp_scale, reference_mf), None
+        else:
+            grads, grad_grid = adjoint.vjp(params, grid.contiguous(), grad_hplus, grad_hcross, amp_scale,
+                                           reference_mf, arithmetic, wrt_grid, approximant)
         dtype = grid.dtype
         g = grads.to(dtype)
         spin_1 = torch.zeros((g.shape[0], 3), dtype=dtype, device=g.device)

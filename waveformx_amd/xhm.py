@@ -125,3 +125,75 @@ def modes(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21,
         freqs = freqs.to(torch.float32)
     records, head, modes_, _ = setup(params, freqs[:1], amp_scale, reference_mf, precision)
     return evaluate(records, head, modes_, freqs, with_modes=True)[2]
+
+
+# -------------------------------------------------------------- gradient
+
+#: the record slots that select a region or a branch (xhm_setup.h HmSlot:
+#: kHmTag, kHmOn, kHmFalloff, kHmMixing, kHmFAmpIn, kHmFAmpIm, kHmFPhaseIn,
+#: kHmFPhaseIm, kHmFRdAux, kHmNInter; xas_layout.h: kFPhaseIn, kFPhaseIm,
+#: kFAmpIn, kFAmpRd, kFCut): held at their values at the parameters, as the
+#: reference's jnp.where selections carry no derivative
+MODE_SELECTION_SLOTS = (0, 1, 6, 7, 8, 9, 10, 11, 44, 55)
+XAS_SELECTION_SLOTS = (49, 50, 51, 52, 53)
+#: the fourth-order central-difference stencil (offset, weight)
+_STENCIL = ((-2, 1.0 / 12.0), (-1, -8.0 / 12.0), (1, 8.0 / 12.0), (2, -1.0 / 12.0))
+
+
+def _steps(params: torch.Tensor, reference_mf: Optional[float]) -> list:
+    """Per-parameter steps (B,) of the differences, or None for a parameter
+    whose derivative is zero (f_ref = 0 takes the grid's first frequency,
+    and with reference_mf f_ref carries none: xas/model.py:35-45)."""
+    m = params[:, :2].abs()
+    ones = torch.ones_like(params[:, 0])
+    steps = [3e-7 * m[:, 0], 3e-7 * m[:, 1], 1e-6 * ones, 1e-6 * ones, None, 1e-3 * ones, 1e-3 * ones]
+    f_ref = params[:, 7]
+    steps.append(None if reference_mf is not None or not bool((f_ref > 0).any()) else 1e-6 * f_ref.abs())
+    return steps
+
+
+def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_cross: torch.Tensor,
+        amp_scale: float = 1e21, reference_mf: Optional[float] = None) -> torch.Tensor:
+    """dL/d(m1, m2, chi1z, chi2z, d_L, inclination, phi_ref, f_ref) (B, 8)
+    float64 of the FP64 polarizations, for the cotangents dL/dRe h + i dL/dIm h
+    (B, N) of h+ and hx: Re sum_f conj(cot) dh/dtheta. The derivatives are
+    fourth-order central differences of the FP64 set-up and evaluation in
+    each parameter with every region selection (bounds, branches) held at the
+    parameters, the reference's jnp.where semantics, so that each difference
+    is of one smooth formula; relative accuracy ~1e-8. d_L's is exact (h is
+    proportional to 1/d_L). f_ref's is zero where f_ref is zero or with
+    reference_mf."""
+    _require_device(params, "params")
+    params = params.to(torch.float64).contiguous()
+    freqs = freqs.to(torch.float64).contiguous()
+    cot_plus = cot_plus.to(torch.complex128)
+    cot_cross = cot_cross.to(torch.complex128)
+    nb = params.shape[0]
+    grads = torch.zeros((nb, 8), dtype=torch.float64, device=params.device)
+    if nb == 0 or freqs.shape[0] == 0:
+        return grads
+    rec0, head0, modes0, _ = setup(params, freqs[:1], amp_scale, reference_mf)
+    hp0, hc0 = evaluate(rec0, head0, modes0, freqs)
+
+    def project(hp, hc):
+        return (torch.conj(cot_plus) * hp).real.sum(1) + (torch.conj(cot_cross) * hc).real.sum(1)
+
+    grads[:, 4] = -project(hp0, hc0) / params[:, 4]
+    del hp0, hc0
+    for j, step in enumerate(_steps(params, reference_mf)):
+        if step is None:
+            continue
+        acc = torch.zeros(nb, dtype=torch.float64, device=params.device)
+        for offset, weight in _STENCIL:
+            p = params.clone()
+            p[:, j] = p[:, j] + offset * step
+            rec, head, modes_, _ = setup(p, freqs[:1], amp_scale, reference_mf)
+            rec[:, XAS_SELECTION_SLOTS] = rec0[:, XAS_SELECTION_SLOTS]
+            modes_[:, :, MODE_SELECTION_SLOTS] = modes0[:, :, MODE_SELECTION_SLOTS]
+            hp, hc = evaluate(rec, head, modes_, freqs)
+            acc = acc + weight * project(hp, hc)
+            del hp, hc
+        grads[:, j] = acc / step
+    if reference_mf is None:
+        grads[:, 7] = torch.where(params[:, 7] > 0, grads[:, 7], torch.zeros_like(grads[:, 7]))
+    return grads
