@@ -39,6 +39,7 @@ def lib():
         handle.oracle_xhm_mode_internals.argtypes = [d, D, I, d]
         handle.oracle_xhm_mode_amplitude_phase.argtypes = [d, D, I, d, L, d, d, P]
         handle.oracle_swsh.argtypes = [D, D, I, I, d]
+        handle.oracle_xhm_collocation.argtypes = [d, D, I, d]
         _ready = True
     return handle
 
@@ -95,6 +96,17 @@ def mode_amplitude_phase(params, f_min: float, k: int, mf, rotation: bool = Fals
     lib().oracle_xhm_mode_amplitude_phase(np.ascontiguousarray(params, dtype=np.float64), f_min, k, mf, mf.size,
                                           amp, phase, rot.ctypes.data if rotation else None)
     return (amp, phase, rot.T.copy()) if rotation else (amp, phase)
+
+
+def collocation(params, f_min: float, k: int) -> dict:
+    """The collocation conditions of higher mode ``k``'s set-up: the phase
+    derivative's points (M f) and values, and the intermediate amplitude's
+    fitted points and amplitudes (without amp0)."""
+    out = np.empty(26)
+    lib().oracle_xhm_collocation(np.ascontiguousarray(params, dtype=np.float64), f_min, k, out)
+    n_p, n_a = int(out[0]), int(out[13])
+    return {"phase_f": out[1:1 + n_p], "phase_v": out[7:7 + n_p], "amp_f": out[14:14 + n_a],
+            "amp_v": out[20:20 + n_a]}
 
 
 def swsh(theta: float, phi: float, ell: int, m: int) -> complex:

@@ -196,6 +196,10 @@ struct HmMode {
     HmSpheroidal sph;
     HmMix22 mix22;
     double alpha2_fit, alpha_l_fit, amp0;
+    // the collocation conditions the solves met (for the checks): the phase
+    // derivative's points and values, the intermediate amplitude's fitted points
+    int n_colloc_phase, n_colloc_amp;
+    double colloc_phase_f[6], colloc_phase_v[6], colloc_amp_f[6], colloc_amp_v[6];
 };
 
 // reference.py:96-99 QNM: the fits of hm_qnm.py in powers of 1 - af
@@ -732,6 +736,8 @@ static void intermediate_amp_coefficients(HmMode& c, const WfVars<double>& wf) {
                 default: fit = ham_amp_44_int4(wf); break;
             }
             values[row++] = std::fabs(fit);
+            c.colloc_amp_f[c.n_colloc_amp] = f;
+            c.colloc_amp_v[c.n_colloc_amp++] = std::fabs(fit);
         }
     }
     if (n == 5) {
@@ -958,7 +964,10 @@ static void phase_coefficients(const WfVars<double>& wf, HmMode& c) {
             row(freqs[chosen[i]], r);
             for (int j = 0; j < 5; ++j) a[i][j] = r[j];
             b[i] = values[chosen[i]];
+            c.colloc_phase_f[i] = freqs[chosen[i]];
+            c.colloc_phase_v[i] = values[chosen[i]];
         }
+        c.n_colloc_phase = 5;
         lu_solve<double, 5>(a, b);
         for (int j = 0; j < 5; ++j) x[j] = b[j];
         x[5] = 0.0;
@@ -984,7 +993,10 @@ static void phase_coefficients(const WfVars<double>& wf, HmMode& c) {
             a[i][0] = 1.0, a[i][1] = lorentzian(freqs[i]), a[i][2] = v, a[i][3] = v * v, a[i][4] = v * v * v,
             a[i][5] = v * v * v * v;
             b[i] = values[i];
+            c.colloc_phase_f[i] = freqs[i];
+            c.colloc_phase_v[i] = values[i];
         }
+        c.n_colloc_phase = 5;
         double ff = freqs[5];
         double v = (1.0 / ff - g_c) / g_h;
         double dv = -1.0 / (ff * ff * g_h);
@@ -1054,6 +1066,7 @@ static void mode_setup(const WfVars<double>& wf, const AmpCoeffs<double>& amp22,
                  2.0 * wf.phi0 + M_PI_ / 4.0;
     c.alpha2_fit = hpr_phase_22_alpha2(wf);
     c.alpha_l_fit = hpr_phase_22_alpha_l(wf);
+    c.n_colloc_phase = c.n_colloc_amp = 0;
     c.sph = HmSpheroidal{0.0, 0.0, 0.0, 0.0, 0.0};
     std::memset(&c.ph, 0, sizeof(c.ph));
     if (hm.mixing) spheroidal_coefficients(wf, c);  // (sets the (3,2)'s time shift and phase offset)
@@ -1278,6 +1291,21 @@ void oracle_xhm_mode_amplitude_phase(const double* params, double f_min, int k, 
             rot[3 * i] = r.v, rot[3 * i + 1] = r.d, rot[3 * i + 2] = r.dd;
         }
     }
+}
+
+// The collocation conditions of mode k: out[0] = number of phase-derivative
+// points, out[1..6] their M f, out[7..12] the values; out[13] = number of the
+// intermediate amplitude's fitted points, out[14..19] their M f, out[20..25]
+// the fitted amplitudes (without amp0).
+void oracle_xhm_collocation(const double* params, double f_min, int k, double* out) {
+    XhmSetup s;
+    xhm_setup(params, f_min, 1.0, s);
+    const HmMode& c = s.modes[k];
+    for (int i = 0; i < 26; ++i) out[i] = 0.0;
+    out[0] = c.n_colloc_phase;
+    for (int i = 0; i < c.n_colloc_phase; ++i) out[1 + i] = c.colloc_phase_f[i], out[7 + i] = c.colloc_phase_v[i];
+    out[13] = c.n_colloc_amp;
+    for (int i = 0; i < c.n_colloc_amp; ++i) out[14 + i] = c.colloc_amp_f[i], out[20 + i] = c.colloc_amp_v[i];
 }
 
 // {}_{-2}Y_{lm}(theta, phi) as (re, im)

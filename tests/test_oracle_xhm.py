@@ -181,3 +181,37 @@ def test_exact_invariances(case):
         scale = np.max(np.abs(modes[k]))
         if scale:
             assert np.max(np.abs(turned[k] - expect)) <= 1e-9 * scale, (ell, m)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_collocation_conditions(case):
+    """The phase and amplitude pass through their collocation points: the
+    intermediate phase derivative (fourth-order differences of the phase in
+    M f) equals the collocation values at their frequencies, and the
+    intermediate amplitude the fitted amplitudes
+    (``xhm/reference.py:275-334,397-528``): a wrong row, term or derivative
+    in a system, or a point given the wrong value, breaks it where the
+    region joins do not."""
+    p = params(case)
+    eta = case[0] * case[1] / (case[0] + case[1]) ** 2
+    for k, mode in enumerate(xhm.HIGHER_MODES):
+        if case[0] == case[1] and case[2] == case[3] and mode in ((2, 1), (3, 3)):
+            continue
+        if mode == (3, 2) and eta < 0.05:
+            continue  # (the EMR (3,2) shifts its intermediate derivative to meet the ringdown's)
+        c = xhm.collocation(p, case[7], k)
+        assert len(c["phase_f"]) == 5
+        bounds = xhm.mode_internals(p, case[7], k)
+        checked = 0
+        for f, v in zip(c["phase_f"], c["phase_v"]):
+            if not bounds["f_phase_match_in"] <= f <= bounds["f_phase_match_im"]:
+                continue  # (a point past fRD - fdamp: there the model is the ringdown's formula)
+            checked += 1
+            h = 1e-6 * f
+            _, ph = xhm.mode_amplitude_phase(p, case[7], k, f + h * np.array([-2.0, -1.0, 1.0, 2.0]))
+            d = (ph[0] - 8 * ph[1] + 8 * ph[2] - ph[3]) / (12 * h)
+            assert d == pytest.approx(v, rel=1e-6, abs=1e-6 * abs(v) + 1e-3), (mode, f)
+        assert checked >= 3
+        for f, v in zip(c["amp_f"], c["amp_v"]):
+            a, _ = xhm.mode_amplitude_phase(p, case[7], k, np.array([f]))
+            assert a[0] == pytest.approx(v, rel=1e-10), (mode, f)
