@@ -383,6 +383,18 @@ def generate_hm(root: Path) -> str:
             if name not in skip:
                 out.append(w.function(fn, module))
                 out.append("")
+    # per mode (2,1), (3,3), (3,2), (4,4): the powers of m/2 that rescale the (2,2) inspiral
+    # (xhm/reference.py:415-427), log(m/2), and (2/m)^(-7/6) (xpnr/hm/inspiral.py:73,
+    # xhm/reference.py:350), with Python's float power and log as the reference evaluates them
+    import math
+
+    emms = (1, 3, 2, 4)
+    scale = ", ".join("{" + ", ".join(repr((m * 0.5) ** ((8 - i) / 3.0)) for i in range(13)) + "}" for m in emms)
+    out.append("// xhm/reference.py:415-427: (m/2)^((8 - i)/3), modes (2,1), (3,3), (3,2), (4,4)")
+    out.append(f"constexpr double kHmInspiralScale[4][13] = {{{scale}}};")
+    out.append(f"constexpr double kHmLogMOver2[4] = {{{', '.join(repr(math.log(m * 0.5)) for m in emms)}}};")
+    out.append("// xpnr/hm/inspiral.py:73: (2/m)^(-7/6)")
+    out.append(f"constexpr double kHmTwoOverMPow[4] = {{{', '.join(repr((2.0 / m) ** (-7.0 / 6.0)) for m in emms)}}};")
     # xhm/reference.py:189: the pseudo-PN collocation system's inverse
     inverse = np.linalg.inv(np.array([[x ** q for q in (7.0 / 3.0, 8.0 / 3.0, 3.0)] for x in (0.5, 0.75, 1.0)]))
     rows = ", ".join("{" + ", ".join(repr(float(v)) for v in row) + "}" for row in inverse)

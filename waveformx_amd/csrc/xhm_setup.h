@@ -265,7 +265,7 @@ WFX_HD inline void pn_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
         case 2: prefactor = sqrt_5_7 / 3.0; break;
         default: prefactor = 4.0 * sqrt(2.0) / 9.0 * sqrt_5_7; break;
     }
-    c.pn_global = pow(2.0 / c.emm, -7.0 / 6.0) * prefactor;
+    c.pn_global = kHmTwoOverMPow[c.mode_int] * prefactor;
     const double zero = 0.0 * eta;
     double* re = c.pn_re;
     double* im = c.pn_im;
@@ -883,10 +883,10 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<double>& wf, const PhaseCo
                                   ph22.sigma3, ph22.sigma4, zero};
     const double m_over_2 = emm * 0.5, two_over_m = 1.0 / m_over_2;
     const double fact = ph22.phi_norm / eta;
-    const double log_m_over_2 = log(m_over_2);
+    const double log_m_over_2 = kHmLogMOver2[c.mode_int];
     for (int i = 0; i < 13; ++i) {
         const double nl = non_log[i] * fact, lg = log_terms[i] * fact, ps = pseudo_pn[i] * fact;
-        const double scale = pow(m_over_2, (8 - i) / 3.0);
+        const double scale = kHmInspiralScale[c.mode_int][i];
         c.phi[i] = m_fma(-lg, log_m_over_2, nl + ps) * scale;
         c.phi_l[i] = lg * scale;
     }
@@ -1031,7 +1031,7 @@ WFX_HD inline void hm_setup(const WaveformVars<double>& wf, const AmpCoeffs<doub
     if (c.mixing) spheroidal_coeffs(wf, ph22, c);
     // amplitude_coefficients (reference.py:348-360)
     c.amp_norm = wf.amp_norm;
-    c.pn_dominant = wf.amp_norm * pow(2.0 / c.emm, -7.0 / 6.0);
+    c.pn_dominant = wf.amp_norm * kHmTwoOverMPow[c.mode_int];
     c.f_rd_aux = c.mixing ? c.f_ring - c.f_damp : 0.0;
     // IMRPhenomXHM_Amplitude_fcutInsp and _fcutRD (reference.py:130-154)
     {

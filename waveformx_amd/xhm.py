@@ -25,6 +25,7 @@ from __future__ import annotations
 
 from typing import Optional, Tuple
 
+import numpy as np
 import torch
 
 from ._lib import call, lib
@@ -142,14 +143,42 @@ _STENCIL = ((-2, 1.0 / 12.0), (-1, -8.0 / 12.0), (1, 8.0 / 12.0), (2, -1.0 / 12.
 
 def _steps(params: torch.Tensor, reference_mf: Optional[float]) -> list:
     """Per-parameter steps (B,) of the differences, or None for a parameter
-    whose derivative is zero (f_ref = 0 takes the grid's first frequency,
-    and with reference_mf f_ref carries none: xas/model.py:35-45)."""
+    taken otherwise: d_L, the inclination and phi_ref in closed form, f_ref
+    where its derivative is zero (f_ref = 0 takes the grid's first
+    frequency, and with reference_mf f_ref carries none: xas/model.py:35-45)."""
     m = params[:, :2].abs()
     ones = torch.ones_like(params[:, 0])
-    steps = [3e-7 * m[:, 0], 3e-7 * m[:, 1], 1e-6 * ones, 1e-6 * ones, None, 1e-3 * ones, 1e-3 * ones]
+    steps = [3e-7 * m[:, 0], 3e-7 * m[:, 1], 1e-6 * ones, 1e-6 * ones, None, None, None]
     f_ref = params[:, 7]
     steps.append(None if reference_mf is not None or not bool((f_ref > 0).any()) else 1e-6 * f_ref.abs())
     return steps
+
+
+def _factor_derivatives(inclination) -> Tuple:
+    """d(F+, Fx)_lm / d(inclination) (B, 5) complex128 of the five modes, by
+    fourth-order differences of the harmonics (xhm/reference.py:646-653),
+    vectorised over the binaries."""
+    from ._modes import spin_weighted_spherical_harmonic as swsh
+
+    def factors(theta):
+        out_p, out_c = [], []
+        for ell, emm in MODES:
+            sign = -1.0 if ell % 2 else 1.0
+            ym = swsh(theta, np.pi / 2.0, ell, -emm)
+            ystar = np.conj(swsh(theta, np.pi / 2.0, ell, emm))
+            out_p.append(0.5 * (ym + sign * ystar))
+            out_c.append(0.5j * (ym - sign * ystar))
+        return np.stack(out_p, axis=-1), np.stack(out_c, axis=-1)
+
+    theta = np.asarray(inclination, dtype=float)
+    step = 1e-3
+    dp = np.zeros((theta.size, 5), np.complex128)
+    dc = np.zeros((theta.size, 5), np.complex128)
+    for offset, weight in _STENCIL:
+        fp, fc = factors(theta + offset * step)
+        dp += weight * fp / step
+        dc += weight * fc / step
+    return dp, dc
 
 
 def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_cross: torch.Tensor,
@@ -160,9 +189,9 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_c
     fourth-order central differences of the FP64 set-up and evaluation in
     each parameter with every region selection (bounds, branches) held at the
     parameters, the reference's jnp.where semantics, so that each difference
-    is of one smooth formula; relative accuracy ~1e-8. d_L's is exact (h is
-    proportional to 1/d_L). f_ref's is zero where f_ref is zero or with
-    reference_mf."""
+    is of one smooth formula; relative accuracy ~1e-8. d_L's, phi_ref's and
+    the inclination's are closed forms on the modes. f_ref's is zero where
+    f_ref is zero or with reference_mf."""
     _require_device(params, "params")
     params = params.to(torch.float64).contiguous()
     freqs = freqs.to(torch.float64).contiguous()
@@ -173,13 +202,24 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_c
     if nb == 0 or freqs.shape[0] == 0:
         return grads
     rec0, head0, modes0, _ = setup(params, freqs[:1], amp_scale, reference_mf)
-    hp0, hc0 = evaluate(rec0, head0, modes0, freqs)
+    hp0, hc0, h = evaluate(rec0, head0, modes0, freqs, with_modes=True)
 
     def project(hp, hc):
         return (torch.conj(cot_plus) * hp).real.sum(1) + (torch.conj(cot_cross) * hc).real.sum(1)
 
+    # closed forms: h ~ 1/d_L; h_{l,-m} ~ e^{i m phi_ref} (each mode's phase holds m phi_ref: the (2,2)'s 2 phi_ref
+    # and its alignment m/2 phi_22); the inclination only in the factors (F+, Fx)_lm (mode_factors, differenced
+    # on the host: smooth functions of order one)
     grads[:, 4] = -project(hp0, hc0) / params[:, 4]
-    del hp0, hc0
+    factors = head0[:, :20].reshape(nb, 5, 4)
+    fp = torch.complex(factors[..., 0], factors[..., 1])
+    fc = torch.complex(factors[..., 2], factors[..., 3])
+    dfp, dfc = (torch.as_tensor(a, device=params.device) for a in _factor_derivatives(params[:, 5].cpu().numpy()))
+    for k, (_, m) in enumerate(MODES):
+        # (the projection is linear: mode by mode)
+        grads[:, 6] += project(1j * m * fp[:, k, None] * h[k], 1j * m * fc[:, k, None] * h[k])
+        grads[:, 5] += project(dfp[:, k, None] * h[k], dfc[:, k, None] * h[k])
+    del hp0, hc0, h
     for j, step in enumerate(_steps(params, reference_mf)):
         if step is None:
             continue
