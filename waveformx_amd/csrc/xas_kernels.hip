@@ -178,6 +178,44 @@ struct XasRecord {
         with_reference_mf(p, reference_mf);
         return xas_record<R>(p, f_min, amp_scale, rec);
     }
+    // The record's derivative with respect to parameter dir (d_L,
+    // inclination, phi_ref or f_ref) from its FP64 values v, into col[k *
+    // stride] (the slots it leaves alone are zero): d_L divides the
+    // amplitude's scale, the inclination sets the two polarization factors,
+    // phi_ref and f_ref move the phase constants by 2 phi_ref - phase(f_ref)
+    // and f_ref the anchor T0 f_ref^(-5/3) (xas/core.py:257-379)
+    __device__ static void closed_column(int dir, const double* p, double reference_mf, unsigned flags,
+                                         const double* v, double* col, int stride) {
+        using D = Dual<1>;
+        if (dir == kDistance) {
+            const double inv = -1.0 / p[kDistance];
+            for (int k = 0; k < 10; ++k) col[(kAmpIns + k) * stride] = v[kAmpIns + k] * inv;
+            col[kAmpIntN * stride] = v[kAmpIntN] * inv;
+            col[kAmpRdN * stride] = v[kAmpRdN] * inv;
+        } else if (dir == kInclination) {
+            D iota(p[kInclination]);
+            iota.d[0] = 1.0;
+            const D ci = m_cos(iota);
+            const D pfac = D(0.5) * (D(1.0) + ci * ci);
+            col[kKp * stride] = flags ? 0.0 : (D(-kYlm) * pfac).d[0];
+            col[kKc * stride] = flags ? 0.0 : (D(kYlm) * ci).d[0];
+        } else if (dir == kPhiRef) {
+            col[kInsConst * stride] = 2.0;
+            col[kIntConst * stride] = 2.0;
+            col[kRdConst * stride] = 2.0;
+        } else if (reference_mf != reference_mf && p[kFRef] != 0.0) {
+            // (with reference_mf, or f_ref = 0 meaning f_min, there is no
+            // f_ref to differentiate)
+            D f(p[kFRef]);
+            f.d[0] = 1.0;
+            bool past;
+            const double rest = phase_rest(v, freq_powers(f), past).d[0];
+            col[kInsConst * stride] = -rest;
+            col[kIntConst * stride] = -rest;
+            col[kRdConst * stride] = -rest;
+            col[kAnchor * stride] = past ? 0.0 : (D(v[kT0]) * m_exp(D(-5.0 / 3.0) * m_log(f))).d[0];
+        }
+    }
 };
 
 struct PhenomDRecord {
@@ -186,12 +224,48 @@ struct PhenomDRecord {
     __device__ static unsigned record(R* p, const R& f_min, double amp_scale, double reference_mf, R* rec) {
         return pd_record<R>(p, f_min, amp_scale, reference_mf, rec);
     }
+    // as XasRecord::closed_column, for IMRPhenomD's record
+    // (phenomd/core.py:274-377)
+    __device__ static void closed_column(int dir, const double* p, double reference_mf, unsigned flags,
+                                         const double* v, double* col, int stride) {
+        using D = Dual<1>;
+        const bool with_mf = reference_mf == reference_mf;
+        if (dir == kDistance) {
+            const double inv = -1.0 / p[kDistance];
+            for (int k = 0; k < 10; ++k) col[(kPdAmpIns + k) * stride] = v[kPdAmpIns + k] * inv;
+            for (int j = 0; j < 5; ++j) col[(kPdAmpInt + j) * stride] = v[kPdAmpInt + j] * inv;
+            col[kPdAmpRdN * stride] = v[kPdAmpRdN] * inv;
+        } else if (dir == kInclination) {
+            const bool error = flags != 0 || (with_mf && p[kFRef] != 0.0);
+            D iota(p[kInclination]);
+            iota.d[0] = 1.0;
+            const D cos_i = m_cos(iota);
+            col[kPdKp * stride] = error ? 0.0 : (D(0.5) * (D(1.0) + cos_i * cos_i)).d[0];
+            col[kPdKc * stride] = error ? 0.0 : (-cos_i).d[0];
+        } else if (dir == kPhiRef) {
+            col[kPdInsConst * stride] = 2.0;
+            col[kPdIntConst * stride] = 2.0;
+            col[kPdMrdConst * stride] = 2.0;
+        } else if (!with_mf && p[kFRef] != 0.0) {
+            D f(p[kFRef]);
+            f.d[0] = 1.0;
+            bool inspiral;
+            const double rest = pd_phase_rest(v, pd_powers(f), inspiral).d[0];
+            col[kPdInsConst * stride] = -rest;
+            col[kPdIntConst * stride] = -rest;
+            col[kPdMrdConst * stride] = -rest;
+            col[kAnchor * stride] = inspiral ? (D(v[kT0]) * m_exp(D(-5.0 / 3.0) * m_log(f))).d[0] : 0.0;
+        }
+    }
 };
 
 // the record and its derivatives with respect to the 8 parameters (m1, m2,
-// chi1, chi2, distance, inclination, phi_ref, f_ref): one thread per
-// (binary, direction); jac (B, 64, 8); the direction-0 thread writes the
-// record, FP64 (records) and/or FP32 split (split_records), either NULL
+// chi1, chi2, distance, inclination, phi_ref, f_ref), jac (B, 64, 8): one
+// thread per (binary, direction) for the masses and spins, through the
+// whole setup in dual numbers; the thread of direction j also writes the
+// column of parameter j + 4, which enters the record only through a few
+// slots (S::closed_column). The direction-0 thread writes the record, FP64
+// (records) and/or FP32 split (split_records), either NULL.
 template <class S>
 __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict__ params, int64_t nb,
                                                        const double* __restrict__ f_min, int64_t f_min_stride,
@@ -199,15 +273,9 @@ __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict_
                                                        double* __restrict__ records, float* __restrict__ split_records,
                                                        double* __restrict__ jac, uint32_t* __restrict__ flags) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t b = gid >> 3;
-    const int dir = (int)(gid & 7);
+    const int64_t b = gid >> 2;
+    const int dir = (int)(gid & 3);
     if (b >= nb) return;
-    if (dir == kFRef && reference_mf == reference_mf) {
-        // with reference_mf the given f_ref only decides whether the binary
-        // is valid (zero, else NaN): it has no derivative (xas/model.py:35-45)
-        for (int k = 0; k < kSlots; ++k) jac[(b * kSlots + k) * 8 + dir] = 0.0;
-        return;
-    }
     using D = Dual<1>;
     D p[kParams];
     for (int k = 0; k < kParams; ++k) {
@@ -216,10 +284,15 @@ __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict_
     }
     D rec[kSlots];
     const unsigned fl = S::template record<D>(p, D(f_min[b * f_min_stride]), amp_scale, reference_mf, rec);
-    for (int k = 0; k < kSlots; ++k) jac[(b * kSlots + k) * 8 + dir] = k < 60 ? rec[k].d[0] : 0.0;
+    double* out = jac + b * kSlots * 8;
+    for (int k = 0; k < kSlots; ++k) out[k * 8 + dir] = k < 60 ? rec[k].d[0] : 0.0;
+    double v[kSlots];
+    for (int k = 0; k < kSlots; ++k) v[k] = rec[k].v;
+    double pv[kParams];
+    for (int k = 0; k < kParams; ++k) pv[k] = params[b * kParams + k];
+    for (int k = 0; k < kSlots; ++k) out[k * 8 + dir + 4] = 0.0;
+    S::closed_column(dir + 4, pv, reference_mf, fl, v, out + dir + 4, 8);
     if (dir == 0) {
-        double v[kSlots];
-        for (int k = 0; k < kSlots; ++k) v[k] = rec[k].v;
         v[kT0Lo] = 0.0;
         v[kAnchorLo] = 0.0;
         if (records)
@@ -992,7 +1065,7 @@ int setup_jvp(const char* name, const double* params, int64_t n_binaries, const 
     if (!params || !f_min || !jacobian) return fail((who + ": null pointer").c_str());
     if (!records && !split_records) return fail((who + ": no record output").c_str());
     if (f_min_stride != 0 && f_min_stride != 1) return fail((who + ": f_min_stride must be 0 or 1").c_str());
-    const int blocks = grid_blocks((8 * n_binaries + 63) / 64, (who + ": too many binaries").c_str());
+    const int blocks = grid_blocks((4 * n_binaries + 63) / 64, (who + ": too many binaries").c_str());
     if (blocks < 0) return -1;
     setup_jvp_kernel<S><<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, f_min, f_min_stride, amp_scale,
                                                                 reference_mf, records, split_records, jacobian, flags);
