@@ -1135,20 +1135,50 @@ int grid_adjoint(const char* name, const void* records, int64_t n_binaries, cons
 
 // ------------------------------------------------------------ IMRPhenomXHM
 
-// one FP64 thread per (binary, higher mode): the mode's record; the thread of
-// the first mode also writes the binary's head (the factors of the five
-// modes in the polarizations, NaN where LAL fails)
+// one FP64 thread per (binary, group), five groups: the four higher modes'
+// records (group k < 4) and the (2,2)'s IMRPhenomXAS record with the flags
+// and the binary's head (the factors of the five modes in the
+// polarizations, NaN where LAL fails). The group is uniform over a block
+// (blocks [k P, (k + 1) P), P = ceil(nb / 64)), so a wave runs one mode's
+// branch of the setup rather than each in turn.
 __global__ __launch_bounds__(64) void xhm_setup_kernel(const double* __restrict__ params, int64_t nb,
                                                        const double* __restrict__ f_min, int64_t f_min_stride,
                                                        double amp_scale, double reference_mf, int split,
+                                                       void* __restrict__ records22, uint32_t* __restrict__ flags,
                                                        void* __restrict__ head, void* __restrict__ modes) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nb * kHmModes) return;
-    const int64_t b = t / kHmModes;
-    const int k = (int)(t % kHmModes);
+    const int64_t per_group = (nb + 63) / 64;
+    const int k = (int)(blockIdx.x / per_group);
+    const int64_t b = (int64_t)(blockIdx.x % per_group) * 64 + threadIdx.x;
+    if (k > kHmModes || b >= nb) return;
     double p[kParams];
     for (int i = 0; i < kParams; ++i) p[i] = params[b * kParams + i];
     with_reference_mf(p, reference_mf);
+    if (k == kHmModes) {
+        double rec[kSlots];
+        const unsigned fl = xas_record<double>(p, f_min[b * f_min_stride], amp_scale, rec);
+        if (split) {
+            write_f32_record(rec, (float*)records22 + b * kSlots);
+        } else {
+            rec[kT0Lo] = 0.0;
+            rec[kAnchorLo] = 0.0;
+            for (int s = 0; s < kSlots; ++s) ((double*)records22)[b * kSlots + s] = rec[s];
+        }
+        if (flags) flags[b] = fl;
+        double h[kHeadSlots];
+        for (int s = 0; s < kHeadSlots; ++s) h[s] = 0.0;
+        const double inclination = p[kInclination];
+        mode_factors(inclination, 2, 2, h + kHeadFactors);
+        for (int m = 0; m < kHmModes; ++m) mode_factors(inclination, kHmEll[m], kHmEmm[m], h + kHeadFactors + 4 * (m + 1));
+        if (fl)
+            for (int s = 0; s < 20; ++s) h[kHeadFactors + s] = __builtin_nan("");
+        for (int s = 0; s < kHeadSlots; ++s) {
+            if (split)
+                ((float*)head)[b * kHeadSlots + s] = (float)h[s];
+            else
+                ((double*)head)[b * kHeadSlots + s] = h[s];
+        }
+        return;
+    }
     const WaveformVars<double> wf = waveform_vars(p, f_min[b * f_min_stride], amp_scale);
     const AmpCoeffs<double> amp22 = amp_coeffs(wf);
     const PhaseCoeffs<double> ph22 = phase_coeffs(wf);
@@ -1159,21 +1189,6 @@ __global__ __launch_bounds__(64) void xhm_setup_kernel(const double* __restrict_
         write_mode_record_split(c, wf, (float*)modes + at);
     else
         write_mode_record(c, wf, (double*)modes + at);
-    if (k == 0) {
-        double h[kHeadSlots];
-        for (int s = 0; s < kHeadSlots; ++s) h[s] = 0.0;
-        const double inclination = p[kInclination];
-        mode_factors(inclination, 2, 2, h + kHeadFactors);
-        for (int m = 0; m < kHmModes; ++m) mode_factors(inclination, kHmEll[m], kHmEmm[m], h + kHeadFactors + 4 * (m + 1));
-        if (wf.flags)
-            for (int s = 0; s < 20; ++s) h[kHeadFactors + s] = __builtin_nan("");
-        for (int s = 0; s < kHeadSlots; ++s) {
-            if (split)
-                ((float*)head)[b * kHeadSlots + s] = (float)h[s];
-            else
-                ((double*)head)[b * kHeadSlots + s] = h[s];
-        }
-    }
 }
 
 // one thread per (binary, frequency), 256-thread blocks, a block on 256
@@ -1474,19 +1489,11 @@ int wfx_xhm_setup(const double* params, int64_t n_binaries, const double* f_min,
     if (f_min_stride != 0 && f_min_stride != 1) return fail("wfx_xhm_setup: f_min_stride must be 0 or 1");
     if (precision != kPrecF64 && precision != kPrecSplit)
         return fail("wfx_xhm_setup: precision must be 0 (FP64) or 2 (split)");
-    const int blocks22 = grid_blocks((n_binaries + 63) / 64, "wfx_xhm_setup: too many binaries");
-    const int blocks = grid_blocks((n_binaries * kHmModes + 63) / 64, "wfx_xhm_setup: too many binaries");
-    if (blocks22 < 0 || blocks < 0) return -1;
-    hipStream_t s = (hipStream_t)stream;
-    if (precision == kPrecF64)
-        setup_f64_kernel<<<blocks22, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf,
-                                                  (double*)records22, flags);
-    else
-        setup_split_kernel<<<blocks22, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf,
-                                                    (float*)records22, flags);
-    if (check_launch("wfx_xhm_setup")) return -1;
-    xhm_setup_kernel<<<blocks, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf,
-                                           precision == kPrecSplit, head, modes);
+    const int blocks = grid_blocks((kHmModes + 1) * ((n_binaries + 63) / 64), "wfx_xhm_setup: too many binaries");
+    if (blocks < 0) return -1;
+    xhm_setup_kernel<<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, f_min, f_min_stride, amp_scale,
+                                                             reference_mf, precision == kPrecSplit, records22, flags,
+                                                             head, modes);
     return check_launch("wfx_xhm_setup");
 }
 
