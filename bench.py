@@ -278,9 +278,22 @@ def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
         ab = (x * a64.conj()).real @ w
         worst = max(worst, float((1.0 - ab / torch.sqrt(((x.abs() ** 2) @ w) * ((a64.abs() ** 2) @ w))).max()))
     del out32
+    # and per mode, on the first 512 binaries (rows where both modes vanish skipped)
+    rows = min(nb, 512)
+    m32 = xhm.evaluate(*(t[:rows] for t in srec[:3]), f32, with_modes=True)[2]
+    m64 = xhm.evaluate(*(t[:rows] for t in xhm.setup(params[:rows], f64r[:1])[:3]), f64r, with_modes=True)[2]
+    per_mode = []
+    for a32, a64 in zip(m32, m64):
+        x = a32.to(torch.complex128)
+        n32, n64 = (x.abs() ** 2) @ w, (a64.abs() ** 2) @ w
+        keep = (n32 > 0) & (n64 > 0)
+        mm = 1.0 - ((x * a64.conj()).real @ w)[keep] / torch.sqrt(n32[keep] * n64[keep])
+        per_mode.append(float(mm.max()) if bool(keep.any()) else 0.0)
+    del m32, m64
     result.update({"xhm_split_value": nb / (ms_split * 1e-3), "xhm_split_ms_per_step": ms_split,
                    "xhm_split_eval_ms": ms_split_eval, "xhm_split_over_fp64": ms / ms_split,
                    "xhm_split_max_weighted_mismatch_vs_fp64": worst,
+                   "xhm_split_mode_mismatch_vs_fp64": dict(zip(("22", "21", "33", "32", "44"), per_mode)),
                    "xhm_split_roofline_hbm": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                                               "achieved": 16.0 * nb * n / (ms_split_eval * 1e-3) / 1e9,
                                               "frac": 16.0 * nb * n / (ms_split_eval * 1e-3) / 1e9 / HBM_PEAK_GBS}})
@@ -290,18 +303,23 @@ def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
         result["xhm_split_roofline"] = {"bound": "valu", "achieved": tops, "peak": VALU_PEAK_TOPS,
                                         "unit": "T instructions/s", "frac": tops / VALU_PEAK_TOPS,
                                         "per_bin": per_split}
-    # the gradient (xhm.vjp: differences of the FP64 path with its region selections held), a quarter of
-    # the binaries: forward + vector-Jacobian product of the noise-weighted residual
-    ng = max(1, nb // 4)
-    pg, fg = params[:ng].contiguous(), f64
+    # the gradient (xhm.vjp: differences of the FP64 path with its region selections held, each inner
+    # product fused into the evaluation) of L = 1/2 sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2), d at parameters
+    # perturbed by 1%: forward + vector-Jacobian product of the residual, every binary
+    rng = np.random.default_rng(7)
+    perturbed = params.cpu().numpy() * (1.0 + 1e-2 * rng.normal(size=tuple(params.shape)))
+    perturbed[:, 7] = params[:, 7].cpu().numpy()
+    d_plus, d_cross = xhm.polarizations(torch.as_tensor(perturbed, device=dev), f64)
 
     def fwd_grad():
-        hp, hc = xhm.polarizations(pg, fg)
-        xhm.vjp(pg, fg, hp * w, hc * w)
+        hp, hc = xhm.polarizations(params, f64)
+        state["g"] = xhm.vjp(params, f64, (hp - d_plus) * w, (hc - d_cross) * w)
 
     ms_grad = timed(torch, fwd_grad, max(2, steps // 5), 1)
-    result.update({"xhm_fwd_grad_binaries": ng, "xhm_fwd_grad_ms": ms_grad,
-                   "xhm_fwd_grad_value": ng / (ms_grad * 1e-3)})
+    result.update({"xhm_fwd_grad_binaries": nb, "xhm_fwd_grad_ms": ms_grad,
+                   "xhm_fwd_grad_value": nb / (ms_grad * 1e-3), "xhm_fwd_grad_over_fwd": ms_grad / ms,
+                   "xhm_grad_finite": bool(torch.isfinite(state["g"][:, :7]).all())})
+    del d_plus, d_cross
     per_bin = XHM_VALU_PER_BIN.get((nb, n))
     if per_bin is not None:
         tops = per_bin * nb * n / (ms_eval * 1e-3) / 1e12

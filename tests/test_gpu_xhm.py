@@ -342,3 +342,56 @@ def test_vjp_against_oracle_differences(cuda, case):
         grid = torch.as_tensor(f, device=cuda).requires_grad_(True)
         hp, _ = wt.polarizations_function("IMRPhenomXHM")(wx.Binary(30.0, 20.0, (0, 0, 0), (0, 0, 0)), grid)
         hp.abs().sum().backward()
+
+
+def test_fused_projection(cuda):
+    """xhm.project (the inner products of the gradient, taken in the
+    evaluation's launch) against the same inner products of the written
+    polarizations, with cotangent rows shared by binaries (row b % R) and a
+    grid that does not fill its last block; a failing binary gives NaN."""
+    import torch
+
+    from waveformx_amd import xhm
+
+    params = oxhm.draw(6, seed=11)
+    params[5, 2] = 1.5  # fails LAL's checks (the device flags it)
+    p = torch.as_tensor(params, device=cuda)
+    f = torch.linspace(20.0, 700.0, 300, dtype=torch.float64, device=cuda)
+    rec, head, modes_, _ = xhm.setup(p, f[:1])
+    hp, hc = xhm.evaluate(rec, head, modes_, f)
+    rng = np.random.default_rng(5)
+    for rows in (6, 2, 1):
+        cp = torch.as_tensor(rng.normal(size=(rows, 300)) + 1j * rng.normal(size=(rows, 300)), device=cuda) * 1e-21
+        cc = torch.as_tensor(rng.normal(size=(rows, 300)) + 1j * rng.normal(size=(rows, 300)), device=cuda) * 1e-21
+        index = torch.arange(6, device=cuda) % rows
+        expected = ((torch.conj(cp[index]) * hp).real.sum(1) + (torch.conj(cc[index]) * hc).real.sum(1)).cpu().numpy()
+        got = xhm.project(rec, head, modes_, f, cp, cc).cpu().numpy()
+        assert np.isnan(got[5]) and np.isnan(expected[5])
+        np.testing.assert_allclose(got[:5], expected[:5], rtol=1e-12, atol=1e-12 * np.abs(expected[:5]).max())
+    with pytest.raises(ValueError):
+        xhm.project(rec, head, modes_, f.float(), cp, cc)
+    assert xhm.project(rec[:0], head[:0], modes_[:0], f, cp, cc).shape == (0,)
+
+
+def test_vjp_of_a_batch_is_that_of_each_binary(cuda):
+    """The gradient of a batch (its differences set up and projected as one
+    batch of stencil points) equals each binary's own, bit for bit; f_ref = 0
+    has no f_ref derivative, a failing binary a NaN row."""
+    import torch
+
+    from waveformx_amd import xhm
+
+    params = oxhm.draw(5, seed=12)
+    params[1, 7] = 0.0
+    params[3, 2] = 1.5  # fails LAL's checks (the device flags it)
+    p = torch.as_tensor(params, device=cuda)
+    f = torch.as_tensor(np.geomspace(20.0, 300.0, 400), device=cuda)
+    rng = np.random.default_rng(6)
+    cp = torch.as_tensor(rng.normal(size=(5, 400)) + 1j * rng.normal(size=(5, 400)), device=cuda) * 1e-21
+    cc = torch.as_tensor(rng.normal(size=(5, 400)) + 1j * rng.normal(size=(5, 400)), device=cuda) * 1e-21
+    batch = xhm.vjp(p, f, cp, cc).cpu().numpy()
+    for b in range(5):
+        one = xhm.vjp(p[b:b + 1], f, cp[b:b + 1], cc[b:b + 1]).cpu().numpy()[0]
+        np.testing.assert_array_equal(batch[b], one)
+    assert batch[1, 7] == 0.0 and np.all(np.isfinite(batch[[0, 1, 2, 4]]))
+    assert np.all(np.isnan(batch[3, :7]))

@@ -109,6 +109,35 @@ def evaluate(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, fr
     return (hp, hc, mode_out) if with_modes else (hp, hc)
 
 
+def project(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, freqs: torch.Tensor,
+            cot_plus: torch.Tensor, cot_cross: torch.Tensor) -> torch.Tensor:
+    """``Re sum_f (conj(cot+) h+ + conj(cotx) hx)`` (B,) float64 of the FP64
+    polarizations of ``records, head, modes`` (B binaries, FP64 set-up) on
+    the float64 grid ``freqs`` (N,), computed without writing them (one
+    fused launch); binary b takes row ``b % R`` of the complex128 cotangents
+    (R, N)."""
+    for t, name in ((records, "records"), (head, "head"), (modes_, "modes"), (freqs, "freqs"),
+                    (cot_plus, "cot_plus"), (cot_cross, "cot_cross")):
+        _require_device(t, name)
+    if records.dtype != torch.float64 or head.dtype != torch.float64 or modes_.dtype != torch.float64:
+        raise ValueError("project takes the FP64 set-up")
+    if freqs.dtype != torch.float64:
+        raise ValueError("project takes a float64 grid")
+    nb, n = records.shape[0], freqs.shape[0]
+    if head.shape[0] != nb or modes_.shape[0] != nb:
+        raise ValueError("records, head and modes must describe the same binaries")
+    cot_plus = cot_plus.to(torch.complex128).contiguous()
+    cot_cross = cot_cross.to(torch.complex128).contiguous()
+    rows = cot_plus.shape[0]
+    if cot_plus.shape != (rows, n) or cot_cross.shape != (rows, n) or (rows == 0 and nb > 0):
+        raise ValueError("the cotangents must be two (R, N) arrays on the grid, R > 0")
+    chunks = (n + 255) // 256
+    partial = torch.zeros((nb, max(chunks, 1)), dtype=torch.float64, device=records.device)
+    call("wfx_xhm_project", _ptr(records), _ptr(head), _ptr(modes_), nb, _ptr(freqs.contiguous()), n,
+         _ptr(cot_plus), _ptr(cot_cross), rows, _ptr(partial), _stream(records.device))
+    return partial.sum(1)
+
+
 def polarizations(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21,
                   reference_mf: Optional[float] = None, precision: str = "fp64") -> Tuple[torch.Tensor, torch.Tensor]:
     """``(h+, hx)`` of a batch on a shared grid: the set-up at the grid's
@@ -190,8 +219,10 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_c
     each parameter with every region selection (bounds, branches) held at the
     parameters, the reference's jnp.where semantics, so that each difference
     is of one smooth formula; relative accuracy ~1e-8. d_L's, phi_ref's and
-    the inclination's are closed forms on the modes. f_ref's is zero where
-    f_ref is zero or with reference_mf."""
+    the inclination's are closed forms (the projection with the modes'
+    factors in the polarizations replaced). f_ref's is zero where f_ref is
+    zero or with reference_mf. Every inner product is taken in the
+    evaluation's launch (:func:`project`): no waveform is written."""
     _require_device(params, "params")
     params = params.to(torch.float64).contiguous()
     freqs = freqs.to(torch.float64).contiguous()
@@ -202,38 +233,55 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_c
     if nb == 0 or freqs.shape[0] == 0:
         return grads
     rec0, head0, modes0, _ = setup(params, freqs[:1], amp_scale, reference_mf)
-    hp0, hc0, h = evaluate(rec0, head0, modes0, freqs, with_modes=True)
+    dev = params.device
 
-    def project(hp, hc):
-        return (torch.conj(cot_plus) * hp).real.sum(1) + (torch.conj(cot_cross) * hc).real.sum(1)
-
-    # closed forms: h ~ 1/d_L; h_{l,-m} ~ e^{i m phi_ref} (each mode's phase holds m phi_ref: the (2,2)'s 2 phi_ref
-    # and its alignment m/2 phi_22); the inclination only in the factors (F+, Fx)_lm (mode_factors, differenced
-    # on the host: smooth functions of order one)
-    grads[:, 4] = -project(hp0, hc0) / params[:, 4]
+    # closed forms, each the projection of the set-up at the parameters with other factors (F+, Fx)_lm in
+    # the head: h ~ 1/d_L (the factors as they are); h_{l,-m} ~ e^{i m phi_ref} (each mode's phase holds
+    # m phi_ref: the (2,2)'s 2 phi_ref and its alignment m/2 phi_22), so i m (F+, Fx)_lm; the inclination
+    # enters the factors only (their derivatives differenced on the host: smooth functions of order one),
+    # NaN where LAL fails as the factors are
     factors = head0[:, :20].reshape(nb, 5, 4)
     fp = torch.complex(factors[..., 0], factors[..., 1])
     fc = torch.complex(factors[..., 2], factors[..., 3])
-    dfp, dfc = (torch.as_tensor(a, device=params.device) for a in _factor_derivatives(params[:, 5].cpu().numpy()))
-    for k, (_, m) in enumerate(MODES):
-        # (the projection is linear: mode by mode)
-        grads[:, 6] += project(1j * m * fp[:, k, None] * h[k], 1j * m * fc[:, k, None] * h[k])
-        grads[:, 5] += project(dfp[:, k, None] * h[k], dfc[:, k, None] * h[k])
-    del hp0, hc0, h
-    for j, step in enumerate(_steps(params, reference_mf)):
-        if step is None:
-            continue
-        acc = torch.zeros(nb, dtype=torch.float64, device=params.device)
-        for offset, weight in _STENCIL:
-            p = params.clone()
-            p[:, j] = p[:, j] + offset * step
-            rec, head, modes_, _ = setup(p, freqs[:1], amp_scale, reference_mf)
-            rec[:, XAS_SELECTION_SLOTS] = rec0[:, XAS_SELECTION_SLOTS]
-            modes_[:, :, MODE_SELECTION_SLOTS] = modes0[:, :, MODE_SELECTION_SLOTS]
-            hp, hc = evaluate(rec, head, modes_, freqs)
-            acc = acc + weight * project(hp, hc)
-            del hp, hc
-        grads[:, j] = acc / step
+    emm = torch.tensor([float(m) for _, m in MODES], dtype=torch.float64, device=dev)
+    dfp, dfc = (torch.as_tensor(a, device=dev) for a in _factor_derivatives(params[:, 5].cpu().numpy()))
+    failed = torch.isnan(fp)
+    dfp = torch.where(failed, fp, dfp)
+    dfc = torch.where(failed, fc, dfc)
+
+    def with_factors(f_plus, f_cross):
+        head = head0.clone()
+        head[:, :20] = torch.stack([f_plus.real, f_plus.imag, f_cross.real, f_cross.imag], -1).reshape(nb, 20)
+        return head
+
+    heads = torch.cat([head0, with_factors(1j * emm * fp, 1j * emm * fc), with_factors(dfp, dfc)])
+    closed = project(rec0.repeat(3, 1), heads, modes0.repeat(3, 1, 1), freqs, cot_plus, cot_cross).view(3, nb)
+    grads[:, 4] = -closed[0] / params[:, 4]
+    grads[:, 6] = closed[1]
+    grads[:, 5] = closed[2]
+
+    # the differences: every stencil point of every parameter as one batch of binaries (in slices of at
+    # most 65,536), set up at once with the selections held at the parameters, projected in one launch
+    steps = _steps(params, reference_mf)
+    points = [(j, offset, weight) for j, step in enumerate(steps) if step is not None for offset, weight in _STENCIL]
+    per_slice = max(1, 65536 // nb)
+    acc = torch.zeros((8, nb), dtype=torch.float64, device=dev)
+    for first in range(0, len(points), per_slice):
+        chunk = points[first:first + per_slice]
+        p = params.repeat(len(chunk), 1)
+        for s, (j, offset, _) in enumerate(chunk):
+            p[s * nb:(s + 1) * nb, j] += offset * steps[j]
+        rec, head, modes_, _ = setup(p, freqs[:1], amp_scale, reference_mf)
+        rec.view(len(chunk), nb, -1)[:, :, list(XAS_SELECTION_SLOTS)] = rec0[:, list(XAS_SELECTION_SLOTS)]
+        modes_.view(len(chunk), nb, 4, -1)[:, :, :, list(MODE_SELECTION_SLOTS)] = \
+            modes0[:, :, list(MODE_SELECTION_SLOTS)]
+        sums = project(rec, head, modes_, freqs, cot_plus, cot_cross).view(len(chunk), nb)
+        for s, (j, _, weight) in enumerate(chunk):
+            acc[j] += weight * sums[s]
+        del rec, head, modes_
+    for j, step in enumerate(steps):
+        if step is not None:
+            grads[:, j] = acc[j] / step
     if reference_mf is None:
         grads[:, 7] = torch.where(params[:, 7] > 0, grads[:, 7], torch.zeros_like(grads[:, 7]))
     return grads
