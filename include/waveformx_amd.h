@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (14); a binding checks it before any call. */
+/* The version of this ABI (15); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -228,15 +228,18 @@ int wfx_xhm_eval(const void* records22, const void* head, const void* modes, int
 
 /* wfx_xhm_project: the FP64 polarizations of wfx_xhm_eval (FP64 records of
  * wfx_xhm_setup, float64 grid) contracted with cotangents without writing
- * them: partial (B, C), C = ceil(N / 256), partial[b][c] = the sum over
- * frequencies 256 c .. 256 c + 255 of Re(conj(cot_plus) h+ + conj(cot_cross)
- * hx), the cotangents complex128 (n_cot, N), binary b taking row b % n_cot.
- * The inner products of the torch backward's vector-Jacobian product
- * (src/waveformx/torch.py:108-119) for IMRPhenomXHM, whose gradient is
- * differences of the set-up and evaluation (waveformx_amd/xhm.py). */
-int wfx_xhm_project(const double* records22, const double* head, const double* modes, int64_t n_binaries,
-                    const double* freqs, int64_t n_freqs, const void* cot_plus, const void* cot_cross, int64_t n_cot,
-                    double* partial, void* stream);
+ * them, for n_heads (1..4) heads (n_heads, B, wfx_xhm_head_slots()) sharing
+ * the records and modes (the modes' factors in the polarizations; the head
+ * of wfx_xhm_setup, or others): partial (n_heads, B, C), C = ceil(N / 256),
+ * partial[j][b][c] = the sum over frequencies 256 c .. 256 c + 255 of
+ * Re(conj(cot_plus) h+ + conj(cot_cross) hx) under head j, the cotangents
+ * complex128 (n_cot, N), binary b taking row b % n_cot. The inner products
+ * of the torch backward's vector-Jacobian product (src/waveformx/torch.py:
+ * 108-119) for IMRPhenomXHM, whose gradient is differences of the set-up
+ * and evaluation (waveformx_amd/xhm.py). */
+int wfx_xhm_project(const double* records22, const double* head, int n_heads, const double* modes,
+                    int64_t n_binaries, const double* freqs, int64_t n_freqs, const void* cot_plus,
+                    const void* cot_cross, int64_t n_cot, double* partial, void* stream);
 
 #ifdef __cplusplus
 }

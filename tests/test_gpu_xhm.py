@@ -368,6 +368,15 @@ def test_fused_projection(cuda):
         got = xhm.project(rec, head, modes_, f, cp, cc).cpu().numpy()
         assert np.isnan(got[5]) and np.isnan(expected[5])
         np.testing.assert_allclose(got[:5], expected[:5], rtol=1e-12, atol=1e-12 * np.abs(expected[:5]).max())
+    # several heads (other factors) over the same records and modes, in one launch: each as alone
+    other = head.clone()
+    other[:, :20] = torch.as_tensor(rng.normal(size=(6, 20)), device=cuda)
+    heads = torch.stack([head, other, head * 2.0])
+    many = xhm.project(rec, heads, modes_, f, cp, cc).cpu().numpy()
+    for j in range(3):
+        np.testing.assert_array_equal(many[j], xhm.project(rec, heads[j], modes_, f, cp, cc).cpu().numpy())
+    with pytest.raises(ValueError):
+        xhm.project(rec, torch.stack([head] * 5), modes_, f, cp, cc)
     with pytest.raises(ValueError):
         xhm.project(rec, head, modes_, f.float(), cp, cc)
     assert xhm.project(rec[:0], head[:0], modes_[:0], f, cp, cc).shape == (0,)

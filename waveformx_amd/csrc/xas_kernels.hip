@@ -1223,35 +1223,44 @@ __global__ __launch_bounds__(kXhmBlock) __attribute__((amdgpu_waves_per_eu(3))) 
 
 // the FP64 polarizations contracted with cotangents, nothing per bin
 // written (the gradient's differences and closed forms): the evaluation's
-// shape; partial[b * chunks + c] = the sum over block c's frequencies of
-// Re(conj(cot+) h+ + conj(cotx) hx), with the cotangent row b % n_cot. The
-// block's sum is in a fixed order (64-lane butterflies, then the waves in
-// turn), so the result is reproducible.
+// shape; for each of n_heads heads (head + j * nb * kHeadSlots: the modes'
+// factors in the polarizations), partial[(j * nb + b) * chunks + c] = the
+// sum over block c's frequencies of Re(conj(cot+) h+ + conj(cotx) hx), with
+// the cotangent row b % n_cot; the modes are computed once for all heads.
+// The block's sum is in a fixed order (64-lane butterflies, then the waves
+// in turn), so the result is reproducible.
+constexpr int kMaxProjectHeads = 4;
 __global__ __launch_bounds__(kXhmBlock) __attribute__((amdgpu_waves_per_eu(3))) void xhm_project_kernel(
-    const double* __restrict__ rec22, const double* __restrict__ head, const double* __restrict__ modes, int64_t nb,
-    const double* __restrict__ freqs, int64_t n, int64_t chunks, const double2* __restrict__ cot_plus,
+    const double* __restrict__ rec22, const double* __restrict__ head, int n_heads, const double* __restrict__ modes,
+    int64_t nb, const double* __restrict__ freqs, int64_t n, int64_t chunks, const double2* __restrict__ cot_plus,
     const double2* __restrict__ cot_cross, int64_t n_cot, double* __restrict__ partial) {
-    __shared__ double wave_sums[kXhmBlock / 64];
+    __shared__ double wave_sums[kMaxProjectHeads][kXhmBlock / 64];
     const int64_t b = blockIdx.x / chunks;
     const int64_t c = blockIdx.x % chunks;
     const int64_t i = c * kXhmBlock + threadIdx.x;
-    double v = 0.0;
+    double v[kMaxProjectHeads] = {0.0, 0.0, 0.0, 0.0};
     if (b < nb && i < n) {
         double h[5][2];
         xhm_modes_f64(rec22 + b * kSlots, modes + b * kHmModes * kHmSlots, freqs[i], h);
-        double p_re, p_im, c_re, c_im;
-        xhm_sum(head + b * kHeadSlots, h, p_re, p_im, c_re, c_im);
         const int64_t row = (b % n_cot) * n + i;
         const double2 cp = cot_plus[row], cc = cot_cross[row];
-        v = (cp.x * p_re + cp.y * p_im) + (cc.x * c_re + cc.y * c_im);
+        for (int j = 0; j < kMaxProjectHeads; ++j) {
+            if (j >= n_heads) break;
+            double p_re, p_im, c_re, c_im;
+            xhm_sum(head + (j * nb + b) * kHeadSlots, h, p_re, p_im, c_re, c_im);
+            v[j] = (cp.x * p_re + cp.y * p_im) + (cc.x * c_re + cc.y * c_im);
+        }
     }
-    for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, 64);
-    if ((threadIdx.x & 63) == 0) wave_sums[threadIdx.x >> 6] = v;
+    for (int j = 0; j < kMaxProjectHeads; ++j) {
+        for (int offset = 32; offset > 0; offset >>= 1) v[j] += __shfl_xor(v[j], offset, 64);
+        if ((threadIdx.x & 63) == 0) wave_sums[j][threadIdx.x >> 6] = v[j];
+    }
     __syncthreads();
-    if (threadIdx.x == 0 && b < nb) {
+    if (threadIdx.x < n_heads && b < nb) {
+        const int j = threadIdx.x;
         double total = 0.0;
-        for (int w = 0; w < kXhmBlock / 64; ++w) total += wave_sums[w];
-        partial[b * chunks + c] = total;
+        for (int w = 0; w < kXhmBlock / 64; ++w) total += wave_sums[j][w];
+        partial[(j * nb + b) * chunks + c] = total;
     }
 }
 
@@ -1290,7 +1299,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 14; }
+int wfx_abi_version(void) { return 15; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -1563,10 +1572,11 @@ int wfx_xhm_eval(const void* records22, const void* head, const void* modes, int
     return check_launch("wfx_xhm_eval");
 }
 
-int wfx_xhm_project(const double* records22, const double* head, const double* modes, int64_t n_binaries,
-                    const double* freqs, int64_t n_freqs, const void* cot_plus, const void* cot_cross, int64_t n_cot,
-                    double* partial, void* stream) {
+int wfx_xhm_project(const double* records22, const double* head, int n_heads, const double* modes,
+                    int64_t n_binaries, const double* freqs, int64_t n_freqs, const void* cot_plus,
+                    const void* cot_cross, int64_t n_cot, double* partial, void* stream) {
     if (n_binaries < 0 || n_freqs < 0 || n_cot < 0) return fail("wfx_xhm_project: negative size");
+    if (n_heads < 1 || n_heads > kMaxProjectHeads) return fail("wfx_xhm_project: n_heads must be 1 to 4");
     if (n_binaries == 0 || n_freqs == 0) return 0;
     if (n_cot == 0) return fail("wfx_xhm_project: no cotangent rows");
     if (!records22 || !head || !modes || !freqs || !cot_plus || !cot_cross || !partial)
@@ -1576,7 +1586,7 @@ int wfx_xhm_project(const double* records22, const double* head, const double* m
     const int blocks = grid_blocks(n_binaries * chunks, "wfx_xhm_project: grid too large");
     if (blocks < 0) return -1;
     xhm_project_kernel<<<blocks, kXhmBlock, 0, (hipStream_t)stream>>>(
-        records22, head, modes, n_binaries, freqs, n_freqs, chunks, (const double2*)cot_plus,
+        records22, head, n_heads, modes, n_binaries, freqs, n_freqs, chunks, (const double2*)cot_plus,
         (const double2*)cot_cross, n_cot, partial);
     return check_launch("wfx_xhm_project");
 }

@@ -115,7 +115,9 @@ def project(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, fre
     polarizations of ``records, head, modes`` (B binaries, FP64 set-up) on
     the float64 grid ``freqs`` (N,), computed without writing them (one
     fused launch); binary b takes row ``b % R`` of the complex128 cotangents
-    (R, N)."""
+    (R, N). ``head`` (H, B, S), H = 1..4 heads (the modes' factors in the
+    polarizations) over the same records and modes, gives (H, B) in the
+    same launch, the modes computed once."""
     for t, name in ((records, "records"), (head, "head"), (modes_, "modes"), (freqs, "freqs"),
                     (cot_plus, "cot_plus"), (cot_cross, "cot_cross")):
         _require_device(t, name)
@@ -123,19 +125,26 @@ def project(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, fre
         raise ValueError("project takes the FP64 set-up")
     if freqs.dtype != torch.float64:
         raise ValueError("project takes a float64 grid")
+    stacked = head.dim() == 3
+    heads = head if stacked else head[None]
+    n_heads = heads.shape[0]
+    if not 1 <= n_heads <= 4:
+        raise ValueError("project takes 1 to 4 heads")
     nb, n = records.shape[0], freqs.shape[0]
-    if head.shape[0] != nb or modes_.shape[0] != nb:
+    if heads.shape[1] != nb or modes_.shape[0] != nb:
         raise ValueError("records, head and modes must describe the same binaries")
+    heads = heads.contiguous()
     cot_plus = cot_plus.to(torch.complex128).contiguous()
     cot_cross = cot_cross.to(torch.complex128).contiguous()
     rows = cot_plus.shape[0]
     if cot_plus.shape != (rows, n) or cot_cross.shape != (rows, n) or (rows == 0 and nb > 0):
         raise ValueError("the cotangents must be two (R, N) arrays on the grid, R > 0")
     chunks = (n + 255) // 256
-    partial = torch.zeros((nb, max(chunks, 1)), dtype=torch.float64, device=records.device)
-    call("wfx_xhm_project", _ptr(records), _ptr(head), _ptr(modes_), nb, _ptr(freqs.contiguous()), n,
+    partial = torch.zeros((n_heads, nb, max(chunks, 1)), dtype=torch.float64, device=records.device)
+    call("wfx_xhm_project", _ptr(records), _ptr(heads), n_heads, _ptr(modes_), nb, _ptr(freqs.contiguous()), n,
          _ptr(cot_plus), _ptr(cot_cross), rows, _ptr(partial), _stream(records.device))
-    return partial.sum(1)
+    out = partial.sum(2)
+    return out if stacked else out[0]
 
 
 def polarizations(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21,
@@ -254,8 +263,8 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_c
         head[:, :20] = torch.stack([f_plus.real, f_plus.imag, f_cross.real, f_cross.imag], -1).reshape(nb, 20)
         return head
 
-    heads = torch.cat([head0, with_factors(1j * emm * fp, 1j * emm * fc), with_factors(dfp, dfc)])
-    closed = project(rec0.repeat(3, 1), heads, modes0.repeat(3, 1, 1), freqs, cot_plus, cot_cross).view(3, nb)
+    heads = torch.stack([head0, with_factors(1j * emm * fp, 1j * emm * fc), with_factors(dfp, dfc)])
+    closed = project(rec0, heads, modes0, freqs, cot_plus, cot_cross)
     grads[:, 4] = -closed[0] / params[:, 4]
     grads[:, 6] = closed[1]
     grads[:, 5] = closed[2]
