@@ -28,14 +28,15 @@ struct Powers {
         three, four, five, m_two, m_three, m_four, m_five, seven_sixths, m_seven_sixths, log, sqrt;
 };
 
+// the powers of x from x^(1/3), 1/x and log x
 template <class R>
-WFX_HD inline Powers<R> useful_powers(const R& x) {
+WFX_HD inline Powers<R> powers_from(const R& x, const R& one_third, const R& m_one, const R& log) {
     Powers<R> p;
     p.itself = x;
-    p.one_third = m_cbrt(x);
+    p.one_third = one_third;
     p.one_sixth = m_sqrt(p.one_third);
-    p.m_one = R(1.0) / x;
-    p.log = m_log(x);
+    p.m_one = m_one;
+    p.log = log;
     p.two_thirds = p.one_third * p.one_third;
     p.m_one_third = p.two_thirds * p.m_one;
     p.m_one_sixth = p.one_sixth * p.m_one_third;
@@ -60,6 +61,11 @@ WFX_HD inline Powers<R> useful_powers(const R& x) {
     p.m_seven_sixths = p.m_one_sixth * p.m_one;
     p.sqrt = p.one_sixth * p.one_third;
     return p;
+}
+
+template <class R>
+WFX_HD inline Powers<R> useful_powers(const R& x) {
+    return powers_from(x, m_cbrt(x), R(1.0) / x, m_log(x));
 }
 
 // IMRPhenomXSetWaveformVariables' fields used by the (2,2) mode

@@ -76,8 +76,9 @@ WFX_HD inline void xhm_modes_f64(const double* rec22, const double* modes, doubl
     sincos(phi22, &s22, &c22);
     h[0][0] = a22 * c22;
     h[0][1] = a22 * s22;
+    // the powers of M f from those of f and the binary's M
     const double mf = modes[kHmMsec] * f;
-    const Powers<double> p = useful_powers(mf);
+    const Powers<double> p = powers_from(mf, fp.t * modes[kHmCbrtM], fp.m1 * modes[kHmInvM], fp.log_f + modes[kHmLogM]);
     for (int k = 0; k < kHmModes; ++k) {
         const double* r = modes + k * kHmSlots;
         const double amp0 = fabs(r[kHmAmp0]);

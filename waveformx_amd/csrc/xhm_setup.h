@@ -1056,6 +1056,11 @@ WFX_HD inline void hm_setup(const WaveformVars<double>& wf, const AmpCoeffs<doub
 }
 
 // the record of a mode
+// log M, M^(1/3) and 1 / M (both records), which give the powers of M f
+// from those of f; the split record's FP64 leading term and inspiral
+// constant (two floats each)
+enum HmMassSlot : int { kHmT0F64 = 112, kHmKInsF64 = 114, kHmLogM = 116, kHmCbrtM = 117, kHmInvM = 118 };
+
 WFX_HD inline void write_mode_record(const HmCoeffs& c, const WaveformVars<double>& wf, double* r) {
     for (int s = 0; s < kHmSlots; ++s) r[s] = 0.0;
     r[kHmTag] = c.tag;
@@ -1066,6 +1071,10 @@ WFX_HD inline void write_mode_record(const HmCoeffs& c, const WaveformVars<doubl
     r[kHmFDamp] = c.f_damp;
     r[kHmFalloff] = c.f_amp_rd_falloff;
     r[kHmFalloffAt] = c.f_amp_rd_falloff;
+    const Powers<double> pm = useful_powers(wf.m_sec);
+    r[kHmLogM] = pm.log;
+    r[kHmCbrtM] = pm.one_third;
+    r[kHmInvM] = pm.m_one;
     r[kHmMixing] = c.mixing ? 1.0 : 0.0;
     r[kHmFAmpIn] = c.f_amp_match_in;
     r[kHmFAmpIm] = c.f_amp_match_im;
@@ -1125,9 +1134,7 @@ WFX_HD inline void write_mode_record(const HmCoeffs& c, const WaveformVars<doubl
 //    the inspiral the phase is an FP32 sum of terms of <~1e3 rad;
 //  * quotients folded as the evaluation uses them (the pseudo-PN
 //    coefficients over their cut-off powers, 1 / f_h, the Lorentzian's
-//    rd_0 f_damp, rd_1 / (f_damp rd_2), (f_damp rd_2)^2);
-//  * log M, M^(1/3) and 1 / M, which give the powers of M f from those of f.
-enum HmSplitSlot : int { kHmT0F64 = 112, kHmKInsF64 = 114, kHmLogM = 116, kHmCbrtM = 117, kHmInvM = 118 };
+//    rd_0 f_damp, rd_1 / (f_damp rd_2), (f_damp rd_2)^2).
 
 WFX_HD inline double fold_two_pi(double x) { return x - kTwoPi * rint(x * (1.0 / kTwoPi)); }
 WFX_HD inline void store_f64(float* r, int slot, double v) { __builtin_memcpy(r + slot, &v, sizeof(double)); }
@@ -1151,9 +1158,6 @@ WFX_HD inline void write_mode_record_split(const HmCoeffs& c, const WaveformVars
     const Powers<double> pm = useful_powers(wf.m_sec);
     store_f64(r, kHmT0F64, c.phi[0] * pm.m_five_thirds);
     store_f64(r, kHmKInsF64, c.c_insp + c.deltaphi);
-    r[kHmLogM] = (float)pm.log;
-    r[kHmCbrtM] = (float)pm.one_third;
-    r[kHmInvM] = (float)pm.m_one;
 }
 
 // ------------------------------------------------------------ harmonics
