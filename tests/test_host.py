@@ -272,4 +272,5 @@ def test_xhm_selection_slots_match_the_headers():
                                                              "kHmFAmpIn", "kHmFAmpIm", "kHmFPhaseIn",
                                                              "kHmFPhaseIm", "kHmFRdAux", "kHmNInter"))
     assert xhm.XAS_SELECTION_SLOTS == tuple(xas[k] for k in ("kFPhaseIn", "kFPhaseIm", "kFAmpIn", "kFAmpRd", "kFCut"))
+    assert (xhm._XAS_INS_CONST, xhm._XAS_ANCHOR) == (xas["kInsConst"], xas["kAnchor"])
 

@@ -174,22 +174,19 @@ def modes(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21,
 #: kFAmpIn, kFAmpRd, kFCut): held at their values at the parameters, as the
 #: reference's jnp.where selections carry no derivative
 MODE_SELECTION_SLOTS = (0, 1, 6, 7, 8, 9, 10, 11, 44, 55)
+#: the IMRPhenomXAS record's inspiral phase constant and anchor (csrc/xas_layout.h)
+_XAS_INS_CONST, _XAS_ANCHOR = 0, 58
 XAS_SELECTION_SLOTS = (49, 50, 51, 52, 53)
 #: the fourth-order central-difference stencil (offset, weight)
 _STENCIL = ((-2, 1.0 / 12.0), (-1, -8.0 / 12.0), (1, 8.0 / 12.0), (2, -1.0 / 12.0))
 
 
-def _steps(params: torch.Tensor, reference_mf: Optional[float]) -> list:
+def _steps(params: torch.Tensor) -> list:
     """Per-parameter steps (B,) of the differences, or None for a parameter
-    taken otherwise: d_L, the inclination and phi_ref in closed form, f_ref
-    where its derivative is zero (f_ref = 0 takes the grid's first
-    frequency, and with reference_mf f_ref carries none: xas/model.py:35-45)."""
+    taken in closed form (d_L, the inclination, phi_ref and f_ref)."""
     m = params[:, :2].abs()
     ones = torch.ones_like(params[:, 0])
-    steps = [3e-7 * m[:, 0], 3e-7 * m[:, 1], 1e-6 * ones, 1e-6 * ones, None, None, None]
-    f_ref = params[:, 7]
-    steps.append(None if reference_mf is not None or not bool((f_ref > 0).any()) else 1e-6 * f_ref.abs())
-    return steps
+    return [3e-7 * m[:, 0], 3e-7 * m[:, 1], 1e-6 * ones, 1e-6 * ones, None, None, None, None]
 
 
 def _factor_derivatives(inclination) -> Tuple:
@@ -223,15 +220,17 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_c
         amp_scale: float = 1e21, reference_mf: Optional[float] = None) -> torch.Tensor:
     """dL/d(m1, m2, chi1z, chi2z, d_L, inclination, phi_ref, f_ref) (B, 8)
     float64 of the FP64 polarizations, for the cotangents dL/dRe h + i dL/dIm h
-    (B, N) of h+ and hx: Re sum_f conj(cot) dh/dtheta. The derivatives are
-    fourth-order central differences of the FP64 set-up and evaluation in
-    each parameter with every region selection (bounds, branches) held at the
-    parameters, the reference's jnp.where semantics, so that each difference
-    is of one smooth formula; relative accuracy ~1e-8. d_L's, phi_ref's and
-    the inclination's are closed forms (the projection with the modes'
-    factors in the polarizations replaced). f_ref's is zero where f_ref is
-    zero or with reference_mf. Every inner product is taken in the
-    evaluation's launch (:func:`project`): no waveform is written."""
+    (B, N) of h+ and hx: Re sum_f conj(cot) dh/dtheta. The masses' and
+    spins' derivatives are fourth-order central differences of the FP64
+    set-up and evaluation with every region selection (bounds, branches)
+    held at the parameters, the reference's jnp.where semantics, so that
+    each difference is of one smooth formula; relative accuracy ~1e-8.
+    d_L's, phi_ref's, the inclination's and f_ref's are closed forms (the
+    projection with the modes' factors in the polarizations replaced: f_ref
+    turns each mode h_{l,-m} by m/2 times the (2,2)'s constant phase); f_ref's
+    is zero where f_ref is zero or with reference_mf. Every inner product is
+    taken in the evaluation's launch (:func:`project`): no waveform is
+    written."""
     _require_device(params, "params")
     params = params.to(torch.float64).contiguous()
     freqs = freqs.to(torch.float64).contiguous()
@@ -263,15 +262,27 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_c
         head[:, :20] = torch.stack([f_plus.real, f_plus.imag, f_cross.real, f_cross.imag], -1).reshape(nb, 20)
         return head
 
-    heads = torch.stack([head0, with_factors(1j * emm * fp, 1j * emm * fc), with_factors(dfp, dfc)])
+    # f_ref turns every mode by a constant: h_{l,-m} by m/2 times the (2,2)'s phase constant, whose derivative
+    # is that of the (2,2) record's phase constants less its anchor's (the XAS set-up's Jacobian); none where
+    # f_ref is zero (it takes f_min) or with the mass-scaled reference
+    if reference_mf is None:
+        from . import xas
+
+        jac = xas.setup_jvp(params, freqs[:1], amp_scale, None, "fp64")[1]
+        turn = (jac[:, _XAS_INS_CONST, 7] - jac[:, _XAS_ANCHOR, 7])[:, None]
+    else:
+        turn = torch.zeros((nb, 1), dtype=torch.float64, device=dev)
+    heads = torch.stack([head0, with_factors(1j * emm * fp, 1j * emm * fc), with_factors(dfp, dfc),
+                         with_factors(0.5j * emm * turn * fp, 0.5j * emm * turn * fc)])
     closed = project(rec0, heads, modes0, freqs, cot_plus, cot_cross)
     grads[:, 4] = -closed[0] / params[:, 4]
     grads[:, 6] = closed[1]
     grads[:, 5] = closed[2]
+    grads[:, 7] = closed[3]
 
     # the differences: every stencil point of every parameter as one batch of binaries (in slices of at
     # most 65,536), set up at once with the selections held at the parameters, projected in one launch
-    steps = _steps(params, reference_mf)
+    steps = _steps(params)
     points = [(j, offset, weight) for j, step in enumerate(steps) if step is not None for offset, weight in _STENCIL]
     per_slice = max(1, 65536 // nb)
     acc = torch.zeros((8, nb), dtype=torch.float64, device=dev)
@@ -291,6 +302,4 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_c
     for j, step in enumerate(steps):
         if step is not None:
             grads[:, j] = acc[j] / step
-    if reference_mf is None:
-        grads[:, 7] = torch.where(params[:, 7] > 0, grads[:, 7], torch.zeros_like(grads[:, 7]))
     return grads
