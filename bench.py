@@ -42,7 +42,7 @@ METRIC = "XAS waveforms/s @4017 bins: FP32-split vs FP64 GPU (mismatch<=1e-6); f
 # VALU instructions per bin of the IMRPhenomXHM evaluations at 4096 x 8193,
 # FP64 and split (SQ_INSTS_VALU x 64 / bins from rocprofv3 --pmc, summaries
 # under profiles/); FP64 FMA issues at the non-packed FP32 rate on this chip
-XHM_VALU_PER_BIN = {(4096, 8193): 1957.4}
+XHM_VALU_PER_BIN = {(4096, 8193): 1822.4}
 XHM_SPLIT_VALU_PER_BIN = {(4096, 8193): 629.9}
 
 
