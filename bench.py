@@ -305,20 +305,29 @@ def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
                                         "per_bin": per_split}
     # the gradient (xhm.vjp: differences of the FP64 path with its region selections held, each inner
     # product fused into the evaluation) of L = 1/2 sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2), d at parameters
-    # perturbed by 1%: forward + vector-Jacobian product of the residual, every binary
+    # perturbed by 1%: the split forward + vector-Jacobian product of the residual, every binary
     rng = np.random.default_rng(7)
     perturbed = params.cpu().numpy() * (1.0 + 1e-2 * rng.normal(size=tuple(params.shape)))
     perturbed[:, 7] = params[:, 7].cpu().numpy()
     d_plus, d_cross = xhm.polarizations(torch.as_tensor(perturbed, device=dev), f64)
 
     def fwd_grad():
-        hp, hc = xhm.polarizations(params, f64)
-        state["g"] = xhm.vjp(params, f64, (hp - d_plus) * w, (hc - d_cross) * w)
+        # the FP32-split forward, its residual's vector-Jacobian product through the FP64 path
+        hp, hc = xhm.polarizations(params, f32, precision="split")
+        state["g"] = xhm.vjp(params, f64, (hp.to(torch.complex128) - d_plus) * w, (hc.to(torch.complex128) - d_cross) * w)
 
     ms_grad = timed(torch, fwd_grad, max(2, steps // 5), 1)
+    # the gradient with the FP64 forward's residual instead, for the split forward's effect on it
+    hp, hc = xhm.polarizations(params, f64)
+    g64 = xhm.vjp(params, f64, (hp - d_plus) * w, (hc - d_cross) * w)
+    del hp, hc
+    g_err = ((state["g"] - g64).abs().amax(0) / g64.abs().amax(0))[:7]
     result.update({"xhm_fwd_grad_binaries": nb, "xhm_fwd_grad_ms": ms_grad,
-                   "xhm_fwd_grad_value": nb / (ms_grad * 1e-3), "xhm_fwd_grad_over_fwd": ms_grad / ms,
-                   "xhm_grad_finite": bool(torch.isfinite(state["g"][:, :7]).all())})
+                   "xhm_fwd_grad_value": nb / (ms_grad * 1e-3), "xhm_fwd_grad_over_split_fwd": ms_grad / ms_split,
+                   "xhm_grad_finite": bool(torch.isfinite(state["g"][:, :7]).all()),
+                   "xhm_grad_rel_err_split_vs_fp64_forward": dict(zip(
+                       ("mass_1", "mass_2", "chi_1", "chi_2", "luminosity_distance", "inclination", "phi_ref"),
+                       map(float, g_err.cpu())))})
     del d_plus, d_cross
     per_bin = XHM_VALU_PER_BIN.get((nb, n))
     if per_bin is not None:
