@@ -404,3 +404,36 @@ def test_vjp_of_a_batch_is_that_of_each_binary(cuda):
         np.testing.assert_array_equal(batch[b], one)
     assert batch[1, 7] == 0.0 and np.all(np.isfinite(batch[[0, 1, 2, 4]]))
     assert np.all(np.isnan(batch[3, :7]))
+
+
+def test_loss_and_grad(cuda):
+    """adjoint.loss_and_grad for IMRPhenomXHM: the loss of the forward pass
+    in each arithmetic, and the gradient xhm.vjp gives for the cotangents
+    2 w (h - d) of that forward pass; its argument checks."""
+    import torch
+
+    from waveformx_amd import adjoint, xhm
+
+    params = oxhm.draw(4, seed=13)
+    p = torch.as_tensor(params, device=cuda)
+    d_params = torch.as_tensor(params * (1.0 + 1e-2 * np.random.default_rng(8).normal(size=params.shape)),
+                               device=cuda)
+    d_params[:, 7] = p[:, 7]
+    freqs = np.linspace(20.0, 600.0, 700)
+    psd = (freqs / 100.0) ** -4 + 1.0 + (freqs / 100.0) ** 2
+    for precision, rdtype in (("fp64", torch.float64), ("split", torch.float32)):
+        f = torch.as_tensor(freqs, dtype=rdtype, device=cuda)
+        w = torch.as_tensor(1.0 / psd, dtype=rdtype, device=cuda)
+        dp, dc = xhm.polarizations(d_params, f, precision=precision)
+        loss, grads = adjoint.loss_and_grad(p, f, dp, dc, w, precision=precision, approximant="IMRPhenomXHM")
+        hp, hc = xhm.polarizations(p, f, precision=precision)
+        expected = ((hp - dp).abs() ** 2 + (hc - dc).abs() ** 2) @ w.to(hp.real.dtype)
+        np.testing.assert_allclose(loss.cpu().numpy(), expected.double().cpu().numpy(), rtol=1e-5 if precision == "split" else 1e-12)
+        w64 = w.double()
+        ref = xhm.vjp(p, f, 2.0 * w64 * (hp - dp).to(torch.complex128), 2.0 * w64 * (hc - dc).to(torch.complex128))
+        np.testing.assert_array_equal(grads.cpu().numpy(), ref.cpu().numpy())
+        assert bool(torch.isfinite(grads[:, :7]).all()) and bool((loss > 0).all())
+    with pytest.raises(ValueError):
+        adjoint.loss_and_grad(p, f, dp, dc, w, precision="fp32", approximant="IMRPhenomXHM")
+    with pytest.raises(ValueError):
+        adjoint.loss_and_grad(p, f, dp, dc, w, precision="split", powers=w, approximant="IMRPhenomXHM")

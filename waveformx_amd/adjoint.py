@@ -10,7 +10,10 @@ and contracts with those derivatives. Nothing of the forward pass is stored.
 
 :func:`loss_and_grad` fuses the forward pass with a noise-weighted
 least-squares loss ``L = sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2)``: it reads
-the data and writes only each binary's loss and gradient.
+the data and writes only each binary's loss and gradient. For IMRPhenomXHM
+it runs the forward pass, takes the loss in torch and the gradient from
+:func:`waveformx_amd.xhm.vjp` (the differences of the FP64 path, each inner
+product fused into the evaluation).
 
 Both models share the kernels' shape (one wave per binary, the same
 reduce-scatter and contraction; one thread per frequency for the grid's
@@ -124,8 +127,18 @@ def loss_and_grad(params: torch.Tensor, freqs: torch.Tensor, data_plus: torch.Te
     """Per-binary loss (B,) and gradient (B, 8), float64, of
     ``sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2)`` in one fused pass
     (``records``: (records, jacobian) if already set up; ``powers``: the
-    grid's :func:`power_table`, made here if not given)."""
-    entry = _entries(approximant, precision)[2]
+    grid's :func:`power_table`, made here if not given; neither for
+    IMRPhenomXHM)."""
+    xhm_model = approximant == "IMRPhenomXHM"
+    if xhm_model:
+        from . import xhm
+
+        if precision not in xhm.PRECISIONS:
+            raise ValueError(f"IMRPhenomXHM's arithmetic is one of {xhm.PRECISIONS}, got {precision!r}")
+        if records is not None or powers is not None:
+            raise ValueError("IMRPhenomXHM's loss takes neither records nor powers")
+    else:
+        entry = _entries(approximant, precision)[2]
     for name, t in (("params", params), ("freqs", freqs), ("data_plus", data_plus), ("data_cross", data_cross),
                     ("weights", weights)):
         _require_device(t, name)
@@ -137,6 +150,14 @@ def loss_and_grad(params: torch.Tensor, freqs: torch.Tensor, data_plus: torch.Te
         raise ValueError(f"data must be ({nb}, {n}) {ctype}")
     if weights.shape != (n,) or weights.dtype != rdtype:
         raise ValueError(f"weights must be ({n},) {rdtype}")
+    if xhm_model:
+        hp, hc = xhm.polarizations(params, freqs, amp_scale, reference_mf, precision)
+        res_plus, res_cross = hp - data_plus, hc - data_cross
+        loss = ((res_plus.abs() ** 2 + res_cross.abs() ** 2) * weights).sum(1).to(torch.float64)
+        w = weights.to(torch.float64)
+        grads = xhm.vjp(params, freqs, 2.0 * w * res_plus.to(torch.complex128),
+                        2.0 * w * res_cross.to(torch.complex128), amp_scale, reference_mf)
+        return loss, grads
     rec, jac = records if records is not None else _records(params, freqs, amp_scale, reference_mf, precision,
                                                             approximant)
     loss = torch.empty((nb,), dtype=torch.float64, device=params.device)
