@@ -433,6 +433,14 @@ def test_loss_and_grad(cuda):
         ref = xhm.vjp(p, f, 2.0 * w64 * (hp - dp).to(torch.complex128), 2.0 * w64 * (hc - dc).to(torch.complex128))
         np.testing.assert_array_equal(grads.cpu().numpy(), ref.cpu().numpy())
         assert bool(torch.isfinite(grads[:, :7]).all()) and bool((loss > 0).all())
+    # adjoint.vjp routes to xhm.vjp
+    f64 = f.double()
+    cot = 2.0 * w.double() * (hp - dp).to(torch.complex128)
+    got, grid = adjoint.vjp(p, f64, cot, None, 1e21, None, "split", approximant="IMRPhenomXHM")
+    assert grid is None
+    np.testing.assert_array_equal(got.cpu().numpy(), xhm.vjp(p, f64, cot, torch.zeros_like(cot)).cpu().numpy())
+    with pytest.raises(NotImplementedError):
+        adjoint.vjp(p, f64, cot, None, 1e21, None, "split", wrt_grid=True, approximant="IMRPhenomXHM")
     with pytest.raises(ValueError):
         adjoint.loss_and_grad(p, f, dp, dc, w, precision="fp32", approximant="IMRPhenomXHM")
     with pytest.raises(ValueError):

@@ -105,7 +105,19 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, grad_plus: Optional[torch.Ten
         grad_cross: Optional[torch.Tensor], amp_scale: float, reference_mf: Optional[float], precision: str,
         wrt_grid: bool = False, approximant: str = "IMRPhenomXAS") -> Tuple[torch.Tensor, Optional[torch.Tensor]]:
     """Gradients (B, 8) with respect to the packed parameters, and (N,)
-    with respect to the grid if ``wrt_grid`` (else None)."""
+    with respect to the grid if ``wrt_grid`` (else None). IMRPhenomXHM's is
+    :func:`waveformx_amd.xhm.vjp` (the FP64 path's, whatever ``precision``;
+    no grid gradient)."""
+    if approximant == "IMRPhenomXHM":
+        from . import xhm
+
+        if wrt_grid:
+            raise NotImplementedError("IMRPhenomXHM has no gradient with respect to the grid")
+        nb, n = params.shape[0], freqs.shape[0]
+        zeros = torch.zeros((nb, n), dtype=torch.complex128, device=params.device)
+        gp = zeros if grad_plus is None else grad_plus.reshape(nb, n)
+        gc = zeros if grad_cross is None else grad_cross.reshape(nb, n)
+        return xhm.vjp(params, freqs, gp, gc, amp_scale, reference_mf), None
     _entries(approximant, precision)
     nb, n = params.shape[0], freqs.shape[0]
     ctype = torch.complex128 if precision == "fp64" else torch.complex64
