@@ -1,5 +1,6 @@
 """Gradients of IMRPhenomXAS and IMRPhenomD on the GPU through the
-hand-written adjoints.
+hand-written adjoints (IMRPhenomXHM's, through :mod:`waveformx_amd.xhm`'s
+differences, are routed here too).
 
 :func:`vjp` is the backward pass of the torch function (the reference's
 ``jax.vjp``, ``src/waveformx/torch.py:87-119``): the FP64 setup with
