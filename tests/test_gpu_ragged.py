@@ -109,3 +109,28 @@ def test_many_binaries_with_empty_ranges_match_the_shared_grid(cuda, bpt):
     torch.cuda.synchronize()
     assert torch.equal(hp.view(torch.int32), shared_p[keep].reshape(-1).view(torch.int32))
     assert torch.equal(hc.view(torch.int32), shared_c[keep].reshape(-1).view(torch.int32))
+
+
+@pytest.mark.parametrize("approximant", ["IMRPhenomXAS", "IMRPhenomD"])
+def test_bad_descriptors_and_offsets_raise_before_any_launch(cuda, approximant):
+    """A negative length, a non-positive f0 or df, or decreasing offsets
+    raise ValueError on the host: no kernel ever sees them."""
+    import torch
+
+    import waveformx_amd as wx
+    from waveformx_amd import phenomd, xas
+
+    params = oracle.draw(3, seed=2)
+    table = {"mass_1": params[:, 0], "mass_2": params[:, 1], "spin_1": np.zeros((3, 3)), "spin_2": np.zeros((3, 3))}
+    for grids in ([(20.0, 0.25, 100), (20.0, 0.25, -50), (20.0, 0.25, 10)],
+                  [(20.0, 0.0, 100), (20.0, 0.25, 50), (20.0, 0.25, 10)],
+                  [(0.0, 0.25, 100), (20.0, 0.25, 50), (20.0, 0.25, 10)]):
+        with pytest.raises(ValueError, match="f0 > 0, df > 0 and n >= 0"):
+            wx.ragged_polarizations(approximant, grids, table)
+    module = xas if approximant == "IMRPhenomXAS" else phenomd
+    pt = torch.as_tensor(params, device=cuda)
+    f = torch.full((1,), 20.0, dtype=torch.float64, device=cuda)
+    records, _ = module.setup(pt, f, 1e21, None, "split")
+    offsets = torch.as_tensor([0, 100, 50, 60], dtype=torch.int64, device=cuda)
+    with pytest.raises(ValueError, match="start at 0 and never decrease"):
+        module.evaluate_ragged(records, offsets, freqs=torch.full((60,), 30.0, device=cuda), precision="split")

@@ -20,6 +20,7 @@ import torch
 from . import phenomd, xas, xhm
 from ._types import (BINARY_NAMES, BatchPolarizations, Binary, FrequencyGrid, WaveformOptions, check_frequencies,
                      check_table, parameter_table, parse_options, strain_scale)
+from .ragged import check_descriptors
 from .exceptions import InvalidParameterError, ModelError, UnknownApproximantError, UnsupportedError
 
 __all__ = ["APPROXIMANTS", "approximants", "is_precessing", "is_frequency_domain", "polarizations",
@@ -254,6 +255,7 @@ def ragged_polarizations(approximant: str, grids: Sequence[Any], parameters: Any
         f0 = np.array([float(g[0]) for g in grids])
         df = np.array([float(g[1]) for g in grids])
         lengths = np.array([int(g[2]) for g in grids], dtype=np.int64)
+        check_descriptors(f0, df, lengths)
         offsets = np.concatenate([[0], np.cumsum(lengths)])
         off = torch.as_tensor(offsets, device=dev)
         f0_t = torch.as_tensor(f0, device=dev)

@@ -22,7 +22,8 @@ from typing import Optional, Tuple
 import torch
 
 from ._lib import call
-from .xas import RECORD_SLOTS, _code, _f_min, _ptr, _require_device, _stream, run_setup_jvp, tile_prefix
+from .xas import (RECORD_SLOTS, _code, _f_min, _ptr, _require_device, _stream, checked_total, run_setup_jvp,
+                  tile_prefix)
 
 __all__ = ["PRECISIONS", "LAL_FLAGS", "setup", "setup_jvp", "evaluate", "evaluate_ragged", "polarizations",
            "reference_phase"]
@@ -110,7 +111,7 @@ def evaluate_ragged(records: torch.Tensor, offsets: torch.Tensor, freqs: Optiona
     if freqs is None and (grid_f0 is None or grid_df is None):
         raise ValueError("give freqs or (grid_f0, grid_df)")
     if total is None:
-        total = int(offsets[-1].item())
+        total = checked_total(offsets)
     if freqs is not None and freqs.shape[0] < total:
         raise ValueError("fewer frequencies than offsets[-1]")
     if tiles is None:

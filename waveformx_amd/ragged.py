@@ -38,6 +38,13 @@ def uniform_grid_length(f_min: float, f_max: float, delta_f: float) -> int:
     return int(np.floor((f_max - f_min) / delta_f + 0.5)) + 1
 
 
+def check_descriptors(f0: np.ndarray, df: np.ndarray, lengths: np.ndarray) -> None:
+    """Reject uniform-grid descriptors ``(f0, df, n)`` that no grid has:
+    ``f0 > 0`` and ``df > 0`` (NaN fails both) and ``n >= 0``."""
+    if not (np.all(f0 > 0) and np.all(df > 0) and np.all(lengths >= 0)):
+        raise ValueError("grids need f0 > 0, df > 0 and n >= 0")
+
+
 def _cache_path() -> Path:
     root = os.environ.get("WFX_CACHE_DIR") or os.path.join(os.path.expanduser("~"), ".cache", "waveformx_amd")
     return Path(root) / "ragged_tuning.json"
@@ -82,8 +89,7 @@ class RaggedPlan:
         f0 = np.array([float(g[0]) for g in grids])
         df = np.array([float(g[1]) for g in grids])
         lengths = np.array([int(g[2]) for g in grids], dtype=np.int64)
-        if np.any(lengths < 0) or np.any(df <= 0) or np.any(f0 <= 0):
-            raise ValueError("grids need f0 > 0, df > 0 and n >= 0")
+        check_descriptors(f0, df, lengths)
         self.n_binaries = len(grids)
         offsets = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
         self.total = int(offsets[-1])

@@ -152,6 +152,16 @@ def evaluate(records: torch.Tensor, freqs: torch.Tensor, precision: str = "split
     return hp, hc
 
 
+def checked_total(offsets: torch.Tensor) -> int:
+    """``offsets[-1]``, read on the host after checking that the offsets
+    start at 0 and never decrease (a decreasing pair would hand a binary
+    bins past the end of the outputs)."""
+    host = offsets.cpu()
+    if int(host[0]) != 0 or bool((host[1:] < host[:-1]).any()):
+        raise ValueError("offsets must start at 0 and never decrease")
+    return int(host[-1])
+
+
 def tile_prefix(offsets: torch.Tensor, bins_per_thread: int = 1) -> torch.Tensor:
     """The prefix sum (B+1,) of the (256 x bins_per_thread)-bin tiles per
     binary, for :func:`evaluate_ragged`."""
@@ -182,7 +192,7 @@ def evaluate_ragged(records: torch.Tensor, offsets: torch.Tensor, freqs: Optiona
     if freqs is None and (grid_f0 is None or grid_df is None):
         raise ValueError("give freqs or (grid_f0, grid_df)")
     if total is None:
-        total = int(offsets[-1].item())
+        total = checked_total(offsets)
         if freqs is not None and freqs.shape[0] != total:
             raise ValueError("offsets[-1] must equal the number of frequencies")
     if freqs is not None and freqs.shape[0] < total:
