@@ -274,3 +274,24 @@ def test_xhm_selection_slots_match_the_headers():
     assert xhm.XAS_SELECTION_SLOTS == tuple(xas[k] for k in ("kFPhaseIn", "kFPhaseIm", "kFAmpIn", "kFAmpRd", "kFCut"))
     assert (xhm._XAS_INS_CONST, xhm._XAS_ANCHOR) == (xas["kInsConst"], xas["kAnchor"])
 
+
+
+def test_ragged_descriptor_and_offset_checks():
+    """Uniform-grid descriptors need f0 > 0, df > 0 and n >= 0, and offsets
+    read on the host must start at 0 and never decrease: lengths
+    [100, -50, 10] would give offsets [0, 100, 50, 60] and hand binary 0
+    bins past the end of a 60-bin output."""
+    import torch
+
+    from waveformx_amd import ragged, xas
+
+    ok = np.array([20.0, 10.0]), np.array([0.25, 0.5]), np.array([100, 0])
+    ragged.check_descriptors(*ok)
+    for f0, df, n in ((ok[0], ok[1], np.array([100, -50])), (np.array([0.0, 10.0]), ok[1], ok[2]),
+                      (ok[0], np.array([0.25, 0.0]), ok[2]), (ok[0], np.array([np.nan, 0.5]), ok[2])):
+        with pytest.raises(ValueError, match="f0 > 0, df > 0 and n >= 0"):
+            ragged.check_descriptors(f0, df, n)
+    assert xas.checked_total(torch.tensor([0, 100, 100, 160])) == 160
+    for bad in ([0, 100, 50, 60], [5, 10, 20]):
+        with pytest.raises(ValueError, match="start at 0 and never decrease"):
+            xas.checked_total(torch.tensor(bad))
