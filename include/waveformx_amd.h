@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (15); a binding checks it before any call. */
+/* The version of this ABI (16); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -144,6 +144,25 @@ int wfx_xas_grid_adjoint(const void* records, int64_t n_binaries, const void* fr
 int wfx_xas_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
                          int freqs_f64, int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
                          const void* weights, const void* powers, double* loss, double* grads, void* stream);
+
+/* wfx_xas_loss_grad_static: loss[B] and grads[B][8] as wfx_xas_loss_adjoint
+ * gives them, straight from params[B][8] (with f_min, amp_scale and
+ * reference_mf as wfx_xas_setup_jvp), for precision 0, 2 or 3: the same
+ * derivative setup, power table and fused loss adjoint, with the records,
+ * their derivatives and the table in a static device array of 48 MiB
+ * instead of caller buffers. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, serving and
+ * sampling loops that forbid hipMalloc). It allocates nothing; it needs
+ * B x 4608 + N x 48 bytes (FP64) or B x 4352 + N x 32 (split), each part
+ * rounded up to 256, within 48 MiB (8192 binaries x 4017 bins fit), and
+ * fails otherwise. Each GPU holds one such array until the process ends,
+ * shared by every call on it: calls that overlap on one GPU (two streams,
+ * two host threads) race. Every call writes all of the array it reads, so
+ * nothing needs clearing between calls. */
+int wfx_xas_loss_grad_static(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
+                             double amp_scale, double reference_mf, const void* freqs, int freqs_f64,
+                             int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
+                             const void* weights, double* loss, double* grads, void* stream);
 
 /* IMRPhenomD, on the same conventions. Its table is FP64 whatever the
  * evaluation's arithmetic (src/waveformx/models/phenomd/model.py:33-50):

@@ -14,7 +14,7 @@ LIBRARY = Path(__file__).resolve().parent / "libwfx_xas.so"
 _lib = None
 
 _P, _I64, _I, _D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
-ABI_VERSION = 15
+ABI_VERSION = 16
 SIGNATURES = {
     "wfx_last_error": ([], ctypes.c_char_p),
     "wfx_abi_version": ([], _I),
@@ -31,6 +31,7 @@ SIGNATURES = {
     "wfx_xas_adjoint": ([_P, _P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P, _P], _I),
     "wfx_xas_grid_adjoint": ([_P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P], _I),
     "wfx_xas_loss_adjoint": ([_P, _P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P, _P, _P, _P], _I),
+    "wfx_xas_loss_grad_static": ([_P, _I64, _P, _I64, _D, _D, _P, _I, _I64, _I, _P, _P, _P, _P, _P, _P], _I),
     "wfx_phd_setup": ([_P, _I64, _P, _I64, _D, _D, _I, _P, _P, _P], _I),
     "wfx_phd_reference_phase": ([_P, _I64, _P, _P, _P], _I),
     "wfx_phd_eval": ([_P, _I64, _P, _I, _I64, _I, _I, _I, _P, _P, _P], _I),

@@ -181,3 +181,43 @@ def loss_and_grad(params: torch.Tensor, freqs: torch.Tensor, data_plus: torch.Te
          _code(precision), _ptr(data_plus), _ptr(data_cross), _ptr(weights), _ptr(powers), _ptr(loss), _ptr(grads),
          _stream(params.device))
     return loss, grads
+
+
+#: the bytes of the static scratch behind :func:`loss_and_grad_static`
+STATIC_SCRATCH_BYTES = 48 << 20
+
+
+def loss_and_grad_static(params: torch.Tensor, freqs: torch.Tensor, data_plus: torch.Tensor,
+                         data_cross: torch.Tensor, weights: torch.Tensor, amp_scale: float = 1e21,
+                         reference_mf: Optional[float] = None, precision: str = "split",
+                         out: Optional[Tuple[torch.Tensor, torch.Tensor]] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+    """IMRPhenomXAS's :func:`loss_and_grad` through
+    ``wfx_xas_loss_grad_static``: the library allocates nothing (its records,
+    derivatives and power table live in a 48 MiB static device array per
+    GPU, shared by every call on it), for callers that cannot allocate.
+    ``out``: (loss (B,), grads (B, 8)) float64 to write into. Calls that
+    overlap on one GPU race."""
+    for name, t in (("params", params), ("freqs", freqs), ("data_plus", data_plus), ("data_cross", data_cross),
+                    ("weights", weights)):
+        _require_device(t, name)
+    if precision not in ("fp64", "split", "split-df"):
+        raise ValueError(f"the static loss runs fp64, split or split-df, not {precision!r}")
+    nb, n = params.shape[0], freqs.shape[0]
+    rdtype = torch.float64 if precision == "fp64" else torch.float32
+    ctype = torch.complex128 if precision == "fp64" else torch.complex64
+    if data_plus.shape != (nb, n) or data_cross.shape != (nb, n) or data_plus.dtype != ctype or \
+            data_cross.dtype != ctype:
+        raise ValueError(f"data must be ({nb}, {n}) {ctype}")
+    if weights.shape != (n,) or weights.dtype != rdtype:
+        raise ValueError(f"weights must be ({n},) {rdtype}")
+    if out is None:
+        loss = torch.empty((nb,), dtype=torch.float64, device=params.device)
+        grads = torch.empty((nb, 8), dtype=torch.float64, device=params.device)
+    else:
+        loss, grads = out
+    f_min = freqs[:1].to(torch.float64)
+    call("wfx_xas_loss_grad_static", _ptr(params), nb, _ptr(f_min), 0, float(amp_scale),
+         float("nan") if reference_mf is None else float(reference_mf), _ptr(freqs),
+         int(freqs.dtype == torch.float64), n, _code(precision), _ptr(data_plus), _ptr(data_cross), _ptr(weights),
+         _ptr(loss), _ptr(grads), _stream(params.device))
+    return loss, grads

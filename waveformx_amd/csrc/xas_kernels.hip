@@ -1292,6 +1292,45 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
 
 }  // namespace
 
+// ------------------------------------------ the scratch-free fused loss
+
+// The device scratch of wfx_xas_loss_grad_static: one call's records, their
+// derivatives and the grid's power table, so that the call allocates
+// nothing. Each GPU the library runs on holds its own copy until the
+// process ends; every call on a GPU shares it, so calls that overlap on one
+// GPU race. Host code never passes the array's name to a kernel (on the
+// host it is a stand-in, not a device address): it passes the address that
+// hipGetSymbolAddress gives for the current device.
+constexpr size_t kStaticScratchBytes = size_t(48) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices];
+
+// the scratch's address on the current device, looked up once per device
+int scratch_address(const char* who, unsigned char** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail((std::string(who) + ": no current HIP device").c_str());
+    if (dev < 0 || dev >= kMaxDevices) return fail((std::string(who) + ": device index beyond 63").c_str());
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !ptr)
+            return fail((std::string(who) + ": hipGetSymbolAddress(scratch): " + hipGetErrorString(err)).c_str());
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = (unsigned char*)g_scratch_ptr[dev];
+    return 0;
+}
+
+constexpr size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+}  // namespace
+
 // ================================================================= C ABI
 
 extern "C" {
@@ -1299,7 +1338,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 15; }
+int wfx_abi_version(void) { return 16; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -1484,6 +1523,42 @@ int wfx_xas_loss_adjoint(const void* records, const double* jacobian, int64_t n_
     return adjoint<XasAdjoint, kLoss>("wfx_xas_loss_adjoint", records, jacobian, n_binaries, freqs, freqs_f64,
                                       n_freqs, precision, data_plus, data_cross, weights, powers, loss, grads,
                                       stream);
+}
+
+int wfx_xas_loss_grad_static(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
+                             double amp_scale, double reference_mf, const void* freqs, int freqs_f64,
+                             int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
+                             const void* weights, double* loss, double* grads, void* stream) {
+    const char* who = "wfx_xas_loss_grad_static";
+    if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xas_loss_grad_static: negative size");
+    if (n_binaries == 0) return 0;
+    if (precision != kPrecF64 && precision != kPrecSplit && precision != kPrecSplitDF)
+        return fail("wfx_xas_loss_grad_static: precision must be 0, 2 or 3");
+    if (!params || !f_min || !loss || !grads || (n_freqs && (!freqs || !data_plus || !data_cross || !weights)))
+        return fail("wfx_xas_loss_grad_static: null pointer");
+    const bool f64 = precision == kPrecF64;
+    // the scratch's three parts: records, derivatives, power table
+    const size_t record_bytes = align256((size_t)n_binaries * kSlots * (f64 ? 8 : 4));
+    const size_t jac_bytes = align256((size_t)n_binaries * kSlots * 8 * sizeof(double));
+    const size_t row = f64 ? sizeof(PowerRow<double>) : sizeof(PowerRow<float>);
+    const size_t power_bytes = align256((size_t)n_freqs * row);
+    if (n_binaries > (int64_t)(kStaticScratchBytes / (kSlots * 8 * sizeof(double))) ||
+        n_freqs > (int64_t)(kStaticScratchBytes / row) ||
+        record_bytes + jac_bytes + power_bytes > kStaticScratchBytes)
+        return fail("wfx_xas_loss_grad_static: B (4096 + 512) + N 48 bytes (FP64; B (4096 + 256) + N 32 "
+                    "split) exceed the 48 MiB scratch: give fewer binaries per call");
+    unsigned char* base = nullptr;
+    if (scratch_address(who, &base) != 0) return -1;
+    void* records = base;
+    double* jacobian = (double*)(base + record_bytes);
+    void* powers = base + record_bytes + jac_bytes;
+    if (setup_jvp<XasRecord>(who, params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf,
+                             f64 ? (double*)records : nullptr, f64 ? nullptr : (float*)records, jacobian, nullptr,
+                             stream) != 0)
+        return -1;
+    if (n_freqs > 0 && wfx_xas_power_table(freqs, freqs_f64, n_freqs, precision, powers, stream) != 0) return -1;
+    return adjoint<XasAdjoint, kLoss>(who, records, jacobian, n_binaries, freqs, freqs_f64, n_freqs, precision,
+                                      data_plus, data_cross, weights, powers, loss, grads, stream);
 }
 
 int wfx_phd_setup_jvp(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
