@@ -55,7 +55,18 @@ def lib():
         _lib.oracle_solve.argtypes = [I, d, d, d]
         _lib.oracle_xas_internals.argtypes = [d, D, D, d]
         _lib.oracle_xas_count.argtypes = [d, L, d, L, ll]
+        _lib.oracle_libm.argtypes = [I, d, d, L]
     return _lib
+
+
+def libm(name, x):
+    """The product's shared-source FP64 ``exp``, ``log`` or ``cbrt``
+    (``waveformx_amd/csrc/wfx_libm.h``, compiled into this library) on an
+    array: the functions the leading phase term reads on both sides."""
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    y = np.empty_like(x)
+    lib().oracle_libm(("exp", "log", "cbrt").index(name), x.reshape(-1), y.reshape(-1), x.size)
+    return y
 
 
 def _params(params):

@@ -17,6 +17,12 @@ with two fixed choices where the reference leaves it to JAX:
 * the 5 x 5 amplitude system by Gaussian elimination with partial pivoting,
   unfused, row by row (``solve``; LAPACK's ``getrf``/``getrs`` choose the
   same pivots), checked against ``numpy.linalg.solve``.
+
+The frequency powers and the leading term ``T0 f^(-5/3) - A`` (up to ~1e4
+rad) take exp, log and cbrt from the shared-source functions that the FP64
+kernels use too (``oracle.libm``, pinned against long double in
+``tests/test_libm.py``): numpy's and the device's own functions differ by an
+ulp, ~1e-12 rad there.
 """
 
 from __future__ import annotations
@@ -25,6 +31,8 @@ import json
 from pathlib import Path
 
 import numpy as np
+
+from .oracle import libm
 
 DATA = json.loads((Path(__file__).resolve().parent / "phenomd_data.json").read_text())
 COEFFICIENTS = {k: np.asarray(v, dtype=np.float64) for k, v in DATA["coefficients"].items()}
@@ -418,11 +426,11 @@ def frequency_powers(f):
     """f, f^(1/3), 1/f, log f, f^(-7/6), f^(3/4), f^(-5/3) (core.py:395-407)."""
     f = np.asarray(f, dtype=np.float64)
     m1 = 1.0 / f
-    log_f = np.log(f)
-    t = np.exp(log_f / 3.0)
+    log_f = libm("log", f)
+    t = libm("exp", log_f / 3.0)
     t = t * (4.0 - t * t * t * m1) / 3.0
     root = np.sqrt(f)
-    return f, t, m1, log_f, m1 / np.sqrt(t), root * np.sqrt(root), np.exp(-5.0 / 3.0 * np.log(f))
+    return f, t, m1, log_f, m1 / np.sqrt(t), root * np.sqrt(root), libm("exp", -5.0 / 3.0 * libm("log", f))
 
 
 def phase_basis(fp):
@@ -495,8 +503,9 @@ def table(params, f_min, amp_scale=1e21, reference_mf=None):
         f_ref64 = reference_mf / (MTSUN_SI * (m1 + m2))
     rows[:, :, CONST] = rows[:, :, CONST] + (2.0 * phi_ref - rest)[:, None]
 
-    t0 = K0 * np.cbrt(m1 + m2) / (m1 * m2)
-    anchor = np.where(inspiral, t0 * np.exp(-5.0 / 3.0 * np.log(f_ref64)), 0.0)
+    t0 = K0 * libm("cbrt", m1 + m2) / (m1 * m2)
+    with np.errstate(invalid="ignore"):
+        anchor = np.where(inspiral, t0 * libm("exp", -5.0 / 3.0 * libm("log", f_ref64)), 0.0)
 
     prefactor = K_AMP * amp_scale * np.sqrt(m1 * m2) * np.exp(-np.log(total) / 6.0) / distance
     amp_ins = np.stack([prefactor * x * u_pow[k] for k, x in enumerate(c.rho)], -1)

@@ -31,6 +31,8 @@
 #include <algorithm>
 #include <limits>
 
+#include "../waveformx_amd/csrc/wfx_libm.h"
+
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -162,6 +164,26 @@ static inline Counted o_atan(const Counted& x) { g_tally.atan_++; return Counted
 // sin and cos of one angle count once (one sincos)
 static inline Counted o_cos(const Counted& x) { g_tally.sincos++; return Counted(std::cos(x.v)); }
 static inline Counted o_sin(const Counted& x) { return Counted(std::sin(x.v)); }
+
+// The leading phase term T0 f^(-5/3) - A and the frequency powers take exp,
+// log and cbrt from the product's shared-source elementary functions
+// (wfx_libm.h, built here by g++ as on the device by hipcc, both with
+// contraction off): the reference's leading term reaches ~1e4 rad, where the
+// host's and the device's own libraries differ by ~1e-12 rad, so only a
+// common source gives the FP64 kernels bit-comparable inputs. Those three
+// functions are pinned against long double in tests/test_libm.py.
+static inline double o_lead_exp(double x) { return wfx_libm::exp(x); }
+static inline double o_lead_log(double x) { return wfx_libm::log(x); }
+static inline double o_lead_cbrt(double x) { return wfx_libm::cbrt(x); }
+static inline Dual o_lead_exp(const Dual& x) { double e = wfx_libm::exp(x.v); return mk(e, x, e); }
+static inline Dual o_lead_log(const Dual& x) { return mk(wfx_libm::log(x.v), x, 1.0 / x.v); }
+static inline Dual o_lead_cbrt(const Dual& x) {
+    double c = wfx_libm::cbrt(x.v);
+    return mk(c, x, c / (3.0 * x.v));
+}
+static inline Counted o_lead_exp(const Counted& x) { g_tally.exp_++; return Counted(wfx_libm::exp(x.v)); }
+static inline Counted o_lead_log(const Counted& x) { g_tally.log_++; return Counted(wfx_libm::log(x.v)); }
+static inline Counted o_lead_cbrt(const Counted& x) { g_tally.cbrt_++; return Counted(wfx_libm::cbrt(x.v)); }
 
 template <class A, class B, class C>
 static inline auto o_fma(const A& a, const B& b, const C& c) { return a * b + c; }
@@ -817,15 +839,15 @@ template <class R>
 static FreqPowers<R> frequency_powers(const R& f) {
     FreqPowers<R> fp;
     R m1 = 1.0 / f;
-    R log_f = o_log(f);
-    R t = o_exp(log_f / 3.0);
+    R log_f = o_lead_log(f);
+    R t = o_lead_exp(log_f / 3.0);
     t = t * (4.0 - t * t * t * m1) / 3.0;
     fp.f = f;
     fp.t = t;
     fp.m1 = m1;
     fp.log_f = log_f;
     fp.m76 = m1 / o_sqrt(t);
-    fp.m53 = o_exp(-5.0 / 3.0 * o_log(f));
+    fp.m53 = o_lead_exp(-5.0 / 3.0 * o_lead_log(f));
     return fp;
 }
 
@@ -1052,7 +1074,7 @@ static unsigned table_of(const R* params, const R& f_min, double amp_scale, R* t
     // the leading term in float64 (core.py:342-344)
     R m1d = params[0], m2d = params[1];
     const double K0 = -3.0 / 128.0 * std::pow(M_PI_ * MTSUN_SI_, -5.0 / 3.0);
-    R t0 = K0 * o_cbrt(m1d + m2d) / (m1d * m2d);
+    R t0 = K0 * o_lead_cbrt(m1d + m2d) / (m1d * m2d);
     // the reference phase (core.py:351-358): phi(f_ref) = 2 phi0 + pi / 4
     for (int r = 0; r < 3; ++r)
         for (int k = 0; k < NBASIS; ++k) tab[T_PHASE + 16 * r + k] = rows[r][k];
@@ -1061,7 +1083,7 @@ static unsigned table_of(const R* params, const R& f_min, double amp_scale, R* t
     R rest = phase_rest(tab, frequency_powers(f_ref), past);
     R shift_constant = 2.0 * wf.phi0 + M_PI_ / 4.0 - rest;
     for (int r = 0; r < 3; ++r) tab[T_PHASE + 16 * r + B_ONE] = tab[T_PHASE + 16 * r + B_ONE] + shift_constant;
-    R anchor = past ? R(0.0) : t0 * o_exp(-5.0 / 3.0 * o_log(f_ref));
+    R anchor = past ? R(0.0) : t0 * o_lead_exp(-5.0 / 3.0 * o_lead_log(f_ref));
     tab[T_BIG] = t0;
     tab[T_BIG + 1] = anchor;
     tab[T_BIG_LO] = R(0.0);
@@ -1136,6 +1158,14 @@ static void lal_form(const double* params, const double* freqs, long n, double a
 extern "C" {
 
 int oracle_table_size() { return TABLE_SIZE; }
+
+// the shared-source elementary functions on an array: which 0 = exp, 1 = log,
+// 2 = cbrt (for their accuracy tests and the numpy IMRPhenomD oracle's
+// leading term)
+void oracle_libm(int which, const double* x, double* y, long n) {
+    for (long i = 0; i < n; ++i)
+        y[i] = which == 0 ? wfx_libm::exp(x[i]) : which == 1 ? wfx_libm::log(x[i]) : wfx_libm::cbrt(x[i]);
+}
 
 // tables (B, 86) and LAL flags (B,) of params (B, 8), f_min (B,)
 void oracle_xas_tables(const double* params, const double* f_min, long nb, double amp_scale, double reference_mf,

@@ -57,12 +57,10 @@ def test_fp64_reference_cases(phd, cuda, case):
     p = case_params(case)[None]
     hp, hc = run(phd, cuda, p, f, "fp64")
     op, oc = pd.polarizations(p, f)
-    m = case[0] + case[1]
-    lead = abs(pd.K0 * np.cbrt(m) / (case[0] * case[1]) * f[0] ** (-5.0 / 3.0))
-    tol = max(1e-12, 8 * np.finfo(float).eps * lead)
     for a, b in ((hp, op), (hc, oc)):
         err = compare(a[0], b[0])[0]
-        assert err < tol, (err, tol)
+        print("fp64 case error", err)
+        assert err < 1e-12, err
 
 
 @pytest.mark.parametrize("precision", ["split", "split-df"])

@@ -56,7 +56,7 @@ def test_graph_replays_match_eager_and_oracle(cuda, precision):
             op, _ = oracle.polarizations(pb, f)
             err, mism = compare(hp_np[off[b]:off[b + 1]], op[0])
             if precision == "fp64":
-                assert err < 1e-11, (b, err)
+                assert err < 1e-12, (b, err)
             else:
                 assert mism < 1e-6, (b, mism)
 

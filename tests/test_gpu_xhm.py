@@ -19,6 +19,7 @@ otherwise follow the same LAL form on both sides."""
 import numpy as np
 import pytest
 
+from oracle import oracle
 from oracle import xhm as oxhm
 
 pytestmark = pytest.mark.gpu
@@ -35,7 +36,26 @@ CASES = [
     (100.0, 1.0, 0.5, 0.0, 300.0, 1.1, 1.7, 10.0, 0.0),
     (30.0, 30.0, 0.2, 0.2, 400.0, 0.4, 0.0, 20.0, 20.0),
 ]
-BAR = 2e-10
+ULP = np.finfo(float).eps
+# the (2,2) mode is IMRPhenomXAS's table form on the device and LAL's form in
+# the XHM oracle: the two arrangements agree to 5e-11 (tests/test_oracle.py)
+TABLE_VS_LAL = 5e-11
+
+
+def mode_bars(params, f_min):
+    """Each mode's FP64 bar against the oracle: 1e-12, or 128 ulp of its
+    leading inspiral phase |T0| f_min^(-5/3) (m/2)^(8/3) where that is
+    larger. The LAL form sums ~13 inspiral terms of up to that size, which
+    the device takes from the frequency's powers times the binary's powers
+    of M and the oracle computes in M f directly: measured 9-40 ulp, up to
+    183 ulp where higher-order terms that cancel outgrow the leading one
+    (q = 100's (2,1), (250 Msun)'s (3,2))."""
+    t0 = abs(oracle.internals(np.asarray(params, dtype=float), f_min)["T0"])
+    return [max(1e-12, 128 * ULP * t0 * f_min ** (-5.0 / 3.0) * (m / 2.0) ** (8.0 / 3.0)) for _, m in oxhm.MODES]
+
+
+def polarization_bar(params, f_min):
+    return max(TABLE_VS_LAL, max(mode_bars(params, f_min)))
 
 
 def relative_error(got, ref):
@@ -55,12 +75,24 @@ def test_reference_cases(cuda, case):
     hp, hc, modes = xhm.polarizations(torch.as_tensor(params, device=cuda), torch.as_tensor(f, device=cuda)) + \
         (xhm.modes(torch.as_tensor(params, device=cuda), torch.as_tensor(f, device=cuda)),)
     op, oc, om = oxhm.polarizations(params, f, amp_scale=1e21, modes=True)
-    errors = {"h+": relative_error(hp.cpu().numpy()[0], op[0]), "hx": relative_error(hc.cpu().numpy()[0], oc[0])}
+    bar = polarization_bar(params[0], f[0])
+    for got, ref in ((hp, op), (hc, oc)):
+        error = relative_error(got.cpu().numpy()[0], ref[0])
+        print("polarization", error, bar)
+        assert error <= bar, (error, bar)
     got_modes = modes.cpu().numpy()
-    for k, mode in enumerate(oxhm.MODES):
-        errors[mode] = relative_error(got_modes[k, 0], om[k, 0])
-    print(errors)
-    assert all(e <= BAR for e in errors.values()), errors
+    for k, (mode, bar) in enumerate(zip(oxhm.MODES, mode_bars(params[0], f[0]))):
+        error = relative_error(got_modes[k, 0], om[k, 0])
+        print(mode, error, bar)
+        assert error <= bar, (mode, error, bar)
+    # the (2,2) mode against IMRPhenomXAS's oracle, the same table form: its
+    # leading term is the same source on both sides, so 1e-12 but where the
+    # other inspiral terms reach ~1e3 rad (q = 100), the (2,2)'s mode bar
+    xp, _ = oracle.polarizations(params, f)
+    kp = -0.5 * (1.0 + np.cos(params[0, 5]) ** 2) * 2.0 * np.sqrt(5.0 / (64.0 * np.pi))
+    error = relative_error(got_modes[0, 0], xp[0] / kp)
+    print("(2,2) vs IMRPhenomXAS's oracle", error)
+    assert error <= max(1e-12, mode_bars(params[0], f[0])[0] if case[0] / case[1] > 20 else 1e-12)
 
 
 def test_random_batch(cuda):
@@ -78,10 +110,13 @@ def test_random_batch(cuda):
     hp, hc, modes = hp.cpu().numpy(), hc.cpu().numpy(), modes.cpu().numpy()
     worst = 0.0
     for b in range(params.shape[0]):
-        for got, ref in [(hp[b], op[b]), (hc[b], oc[b])] + [(modes[k, b], om[k, b]) for k in range(5)]:
-            worst = max(worst, relative_error(got, ref))
-    print("worst relative error over the batch:", worst)
-    assert worst <= BAR
+        bars = [polarization_bar(params[b], f[0])] * 2 + mode_bars(params[b], f[0])
+        refs = [(hp[b], op[b]), (hc[b], oc[b])] + [(modes[k, b], om[k, b]) for k in range(5)]
+        for (got, ref), bar in zip(refs, bars):
+            error = relative_error(got, ref)
+            worst = max(worst, error / bar)
+            assert error <= bar, (b, error, bar)
+    print("worst error over the batch, in units of its bar:", worst)
     # a float32 grid is the same grid, read as float32
     hp32, _ = xhm.evaluate(records, head, modes_, torch.as_tensor(f, dtype=torch.float32, device=cuda))
     f32 = f.astype(np.float32).astype(np.float64)
@@ -107,7 +142,8 @@ def test_public_functions(cuda):
     function = wt.polarizations_function("IMRPhenomXHM")
     hp, hc = function(binary, torch.as_tensor(f, device=cuda))
     assert hp.dtype == torch.complex128 and hp.shape == (f.size,)
-    assert relative_error(hp.cpu().numpy(), op[0]) <= BAR and relative_error(hc.cpu().numpy(), oc[0]) <= BAR
+    bar = polarization_bar(params[0], f[0])
+    assert relative_error(hp.cpu().numpy(), op[0]) <= bar and relative_error(hc.cpu().numpy(), oc[0]) <= bar
     b32 = wx.Binary(*(torch.tensor(v, dtype=torch.float32, device=cuda) for v in binary))
     hp32, _ = function(b32, torch.as_tensor(f, dtype=torch.float32, device=cuda))
     assert hp32.dtype == torch.complex64
@@ -136,7 +172,7 @@ def test_public_functions(cuda):
     sp, _ = oxhm.polarizations(scaled, f, amp_scale=1e21)
     zero_ref = wx.Binary(*binary[:7], 0.0)
     rp, _ = wt.polarizations_function("IMRPhenomXHM", reference_mf=0.012)(zero_ref, torch.as_tensor(f, device=cuda))
-    assert relative_error(rp.cpu().numpy(), sp[0]) <= BAR
+    assert relative_error(rp.cpu().numpy(), sp[0]) <= polarization_bar(scaled[0], f[0])
     nan_p, _ = wt.polarizations_function("IMRPhenomXHM", reference_mf=0.012)(binary, torch.as_tensor(f, device=cuda))
     assert bool(torch.isnan(nan_p).all())
 
@@ -166,8 +202,8 @@ def test_modes_and_their_sum(cuda):
     assert torch.equal(fm[(3, 3)][:f.size], torch.flip(-torch.conj(h33), dims=(0,)))
     params = np.array([[case[k] for k in (0, 1, 2, 3, 4, 5, 6, 8)]])
     _, _, om = oxhm.polarizations(params, f, amp_scale=1e21, modes=True)
-    for k, (ell, m) in enumerate(oxhm.MODES):
-        assert relative_error(fm[(ell, -m)][f.size:].cpu().numpy(), om[k, 0]) <= BAR
+    for k, ((ell, m), bar) in enumerate(zip(oxhm.MODES, mode_bars(params[0], f[0]))):
+        assert relative_error(fm[(ell, -m)][f.size:].cpu().numpy(), om[k, 0]) <= bar
     # choose_fd_modes: the band [f_min, f_cut] of k delta_f, zero elsewhere
     delta_f, f_min = 0.25, 20.0
     chosen = wx.choose_fd_modes("IMRPhenomXHM", delta_f, f_min, 1024.0, **{**keywords, "f_ref": 25.0})

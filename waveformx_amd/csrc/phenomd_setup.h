@@ -423,13 +423,13 @@ WFX_HD inline PdPowers<R> pd_powers(const R& f) {
     PdPowers<R> fp;
     fp.f = f;
     fp.m1 = R(1.0) / f;
-    fp.log_f = m_log(f);
-    R t = m_exp(fp.log_f / R(3.0));
+    fp.log_f = lead_log(f);
+    R t = lead_exp(fp.log_f / R(3.0));
     fp.t = t * (R(4.0) - t * t * t * fp.m1) / R(3.0);
     const R root = m_sqrt(f);
     fp.m76 = fp.m1 / m_sqrt(fp.t);
     fp.f34 = root * m_sqrt(root);
-    fp.m53 = exp(-5.0 / 3.0 * log(value_of(f)));
+    fp.m53 = lead_exp(-5.0 / 3.0 * lead_log(value_of(f)));
     return fp;
 }
 
@@ -563,9 +563,9 @@ WFX_HD inline unsigned pd_record(const R* params, const R& f_min, double amp_sca
     rec[kPdMrdConst] = rec[kPdMrdConst] + shift;
 
     // the leading term (as IMRPhenomXAS's, xas/core.py:342-363)
-    const R t0 = R(kPdK0) * m_cbrt(m1 + m2) / (m1 * m2);
+    const R t0 = R(kPdK0) * lead_cbrt(m1 + m2) / (m1 * m2);
     rec[kT0] = t0;
-    rec[kAnchor] = inspiral ? t0 * m_exp(R(-5.0 / 3.0) * m_log(f_ref64)) : R(0.0);
+    rec[kAnchor] = inspiral ? t0 * lead_exp(R(-5.0 / 3.0) * lead_log(f_ref64)) : R(0.0);
 
     // the amplitude
     const R prefactor = R(kPdKAmp) * R(amp_scale) * m_sqrt(m1 * m2) * m_exp(-m_log(total) / R(6.0)) / distance;

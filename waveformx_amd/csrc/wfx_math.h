@@ -12,6 +12,8 @@
 #include <cmath>
 #include <cstdint>
 
+#include "wfx_libm.h"
+
 #define WFX_HD __host__ __device__
 #define WFX_CONST __device__ __constant__ static const
 
@@ -175,6 +177,30 @@ WFX_HD inline Dual<N> m_abs(const Dual<N>& x) {
 template <int N>
 WFX_HD inline Dual<N> m_pow(const Dual<N>& x, double y) {
     return chain(pow(x.v, y), x, y * pow(x.v, y - 1.0));
+}
+
+// The phase's leading term T0 f^(-5/3) - A and the frequency powers read the
+// shared-source exp, log and cbrt of wfx_libm.h in FP64 (the same bits on the
+// host and the device); the FP32 setup keeps the FP32 library functions.
+WFX_HD inline double lead_exp(double x) { return wfx_libm::exp(x); }
+WFX_HD inline double lead_log(double x) { return wfx_libm::log(x); }
+WFX_HD inline double lead_cbrt(double x) { return wfx_libm::cbrt(x); }
+WFX_HD inline float lead_exp(float x) { return expf(x); }
+WFX_HD inline float lead_log(float x) { return logf(x); }
+WFX_HD inline float lead_cbrt(float x) { return cbrtf(x); }
+template <int N>
+WFX_HD inline Dual<N> lead_exp(const Dual<N>& x) {
+    const double e = wfx_libm::exp(x.v);
+    return chain(e, x, e);
+}
+template <int N>
+WFX_HD inline Dual<N> lead_log(const Dual<N>& x) {
+    return chain(wfx_libm::log(x.v), x, 1.0 / x.v);
+}
+template <int N>
+WFX_HD inline Dual<N> lead_cbrt(const Dual<N>& x) {
+    const double c = wfx_libm::cbrt(x.v);
+    return chain(c, x, c / (3.0 * x.v));
 }
 
 template <class R>

@@ -560,11 +560,11 @@ WFX_HD inline FreqPowers<R> freq_powers(const R& f) {
     FreqPowers<R> fp;
     fp.f = f;
     fp.m1 = R(1.0) / f;
-    fp.log_f = m_log(f);
-    R t = m_exp(fp.log_f / R(3.0));
+    fp.log_f = lead_log(f);
+    R t = lead_exp(fp.log_f / R(3.0));
     fp.t = t * (R(4.0) - t * t * t * fp.m1) / R(3.0);
     fp.m76 = fp.m1 / m_sqrt(fp.t);
-    fp.m53 = exp(-5.0 / 3.0 * log(value_of(f)));
+    fp.m53 = lead_exp(-5.0 / 3.0 * lead_log(value_of(f)));
     return fp;
 }
 
@@ -728,9 +728,9 @@ WFX_HD inline unsigned xas_record(const R* params, const R& f_min, double amp_sc
     // T0 = K0 M^(1/3) / (m1 m2) and A = T0 f_ref^(-5/3), zero past the
     // inspiral (xas/core.py:342-344,359-363)
     const double K0 = -2439189.9493546346;  // -3/128 (pi MTSUN_SI)^(-5/3)
-    R t0 = R(K0) * m_cbrt(params[kMass1] + params[kMass2]) / (params[kMass1] * params[kMass2]);
+    R t0 = R(K0) * lead_cbrt(params[kMass1] + params[kMass2]) / (params[kMass1] * params[kMass2]);
     rec[kT0] = t0;
-    rec[kAnchor] = past ? R(0.0) : t0 * m_exp(R(-5.0 / 3.0) * m_log(wf.f_ref));
+    rec[kAnchor] = past ? R(0.0) : t0 * lead_exp(R(-5.0 / 3.0) * lead_log(wf.f_ref));
     return wf.flags;
 }
 
