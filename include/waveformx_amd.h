@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (16); a binding checks it before any call. */
+/* The version of this ABI (17); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -77,6 +77,14 @@ int wfx_xas_setup_jvp(const double* params, int64_t n_binaries, const double* f_
  * 29-38, src/waveformx/models/taylorf2/reference.py:46-127). */
 int wfx_xas_reference_phase(const double* params, int64_t n_binaries, const double* mf, double* phase,
                             void* stream);
+
+/* wfx_xas_reference_phase with its derivatives grad[B][5] = d phase /
+ * d(mass_1, mass_2, chi_1z, chi_2z, mf) (forward mode through the same FP64
+ * code); phase may be NULL. The derivatives of the reference's converters,
+ * which it takes with jax.grad (src/waveformx/models/xas/reference.py:29-38,
+ * tests/test_xas.py:174-175). */
+int wfx_xas_reference_phase_grad(const double* params, int64_t n_binaries, const double* mf, double* phase,
+                                 double* grad, void* stream);
 
 /* (h+, hx)[B][N] of records on one shared grid freqs[N] (float, or double
  * if freqs_f64). group: binaries per block (0: default, 4 on the FP32
@@ -182,6 +190,10 @@ int wfx_phd_setup(const double* params, int64_t n_binaries, const double* f_min,
                   void* stream);
 int wfx_phd_reference_phase(const double* params, int64_t n_binaries, const double* mf, double* phase,
                             void* stream);
+/* IMRPhenomD's reference phase with its derivatives, as
+ * wfx_xas_reference_phase_grad (phenomd/reference.py:18-26). */
+int wfx_phd_reference_phase_grad(const double* params, int64_t n_binaries, const double* mf, double* phase,
+                                 double* grad, void* stream);
 int wfx_phd_eval(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
                  int precision, int group, int freqs_per_thread, void* hplus, void* hcross, void* stream);
 int wfx_phd_eval_ragged(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64,

@@ -15,6 +15,9 @@ from ._types import (BINARY_NAMES, COLUMNS, REFERENCE_MF, BatchPolarizations, Bi
                      WaveformOptions, check_frequencies, parameter_table)
 from .exceptions import (InvalidParameterError, ModelError, UnknownApproximantError, UnsupportedError,
                          WaveformError)
+from .reference import REFERENCE_MODULES, reference_module
+from .reference import package_to_mass_scaled_reference as to_mass_scaled_reference
+from .reference import package_to_physical_reference as to_physical_reference
 
 __all__ = [
     "APPROXIMANTS", "approximants", "is_precessing", "is_frequency_domain", "polarizations", "polarizations_on_grid",
@@ -22,7 +25,7 @@ __all__ = [
     "BINARY_NAMES", "COLUMNS", "REFERENCE_MF", "BatchPolarizations", "FrequencyGrid", "WaveformOptions",
     "check_frequencies", "parameter_table", "WaveformError", "InvalidParameterError", "UnsupportedError",
     "UnknownApproximantError", "ModelError", "polarizations_function", "modes", "choose_fd_modes",
-    "FrequencyModes",
+    "FrequencyModes", "REFERENCE_MODULES", "reference_module", "to_mass_scaled_reference", "to_physical_reference",
 ]
 
 

@@ -14,7 +14,7 @@ LIBRARY = Path(__file__).resolve().parent / "libwfx_xas.so"
 _lib = None
 
 _P, _I64, _I, _D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
-ABI_VERSION = 16
+ABI_VERSION = 17
 SIGNATURES = {
     "wfx_last_error": ([], ctypes.c_char_p),
     "wfx_abi_version": ([], _I),
@@ -24,6 +24,7 @@ SIGNATURES = {
     "wfx_xas_setup_jvp": ([_P, _I64, _P, _I64, _D, _D, _P, _P, _P, _P, _P], _I),
     "wfx_xas_eval": ([_P, _I64, _P, _I, _I64, _I, _I, _I, _P, _P, _P], _I),
     "wfx_xas_reference_phase": ([_P, _I64, _P, _P, _P], _I),
+    "wfx_xas_reference_phase_grad": ([_P, _I64, _P, _P, _P, _P], _I),
     "wfx_xas_tile_prefix": ([_P, _I64, _I, _P, _P], _I),
     "wfx_xas_eval_ragged": ([_P, _I64, _P, _I, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P], _I),
     "wfx_xas_power_table_bytes": ([_I], _I),
@@ -34,6 +35,7 @@ SIGNATURES = {
     "wfx_xas_loss_grad_static": ([_P, _I64, _P, _I64, _D, _D, _P, _I, _I64, _I, _P, _P, _P, _P, _P, _P], _I),
     "wfx_phd_setup": ([_P, _I64, _P, _I64, _D, _D, _I, _P, _P, _P], _I),
     "wfx_phd_reference_phase": ([_P, _I64, _P, _P, _P], _I),
+    "wfx_phd_reference_phase_grad": ([_P, _I64, _P, _P, _P, _P], _I),
     "wfx_phd_eval": ([_P, _I64, _P, _I, _I64, _I, _I, _I, _P, _P, _P], _I),
     "wfx_phd_eval_ragged": ([_P, _I64, _P, _I, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P], _I),
     "wfx_phd_setup_jvp": ([_P, _I64, _P, _I64, _D, _D, _P, _P, _P, _P, _P], _I),

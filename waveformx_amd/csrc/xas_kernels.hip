@@ -345,6 +345,31 @@ __global__ __launch_bounds__(64) void reference_phase_pd_kernel(const double* __
     phase[b] = pd_reference_phase<double>(p, mf[b]);
 }
 
+// the reference phase and its derivatives grad[B][5] with respect to (m1,
+// m2, chi1, chi2, mf): one thread per (binary, direction) in dual numbers;
+// Model 0 is IMRPhenomXAS's lal_phase_22, 1 IMRPhenomD's pd_reference_phase
+template <int Model>
+__global__ __launch_bounds__(64) void reference_phase_grad_kernel(const double* __restrict__ params, int64_t nb,
+                                                                  const double* __restrict__ mf,
+                                                                  double* __restrict__ phase,
+                                                                  double* __restrict__ grad) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t b = gid / 5;
+    const int dir = (int)(gid % 5);
+    if (b >= nb) return;
+    using D = Dual<1>;
+    D p[kParams];
+    for (int k = 0; k < kParams; ++k) {
+        p[k] = D(params[b * kParams + k]);
+        p[k].d[0] = k == dir ? 1.0 : 0.0;
+    }
+    D x(mf[b]);
+    x.d[0] = dir == 4 ? 1.0 : 0.0;
+    const D v = Model == 0 ? lal_phase_22<D>(p, x) : pd_reference_phase<D>(p, x);
+    grad[b * 5 + dir] = v.d[0];
+    if (dir == 0 && phase) phase[b] = v.v;
+}
+
 // ------------------------------------------------------------ evaluation
 
 constexpr int kBlock = 256;
@@ -1290,6 +1315,20 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
     }
 }
 
+// the reference phase with its derivatives, for model Model behind `name`
+template <int Model>
+int reference_phase_grad(const char* name, const double* params, int64_t n_binaries, const double* mf,
+                         double* phase, double* grad, void* stream) {
+    const std::string who(name);
+    if (n_binaries < 0) return fail((who + ": negative size").c_str());
+    if (n_binaries == 0) return 0;
+    if (!params || !mf || !grad) return fail((who + ": null pointer").c_str());
+    const int blocks = grid_blocks((5 * n_binaries + 63) / 64, (who + ": too many binaries").c_str());
+    if (blocks < 0) return -1;
+    reference_phase_grad_kernel<Model><<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, mf, phase, grad);
+    return check_launch(name);
+}
+
 }  // namespace
 
 // ------------------------------------------ the scratch-free fused loss
@@ -1338,7 +1377,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 16; }
+int wfx_abi_version(void) { return 17; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -1396,6 +1435,11 @@ int wfx_xas_reference_phase(const double* params, int64_t n_binaries, const doub
     return check_launch("wfx_xas_reference_phase");
 }
 
+int wfx_xas_reference_phase_grad(const double* params, int64_t n_binaries, const double* mf, double* phase,
+                                 double* grad, void* stream) {
+    return reference_phase_grad<0>("wfx_xas_reference_phase_grad", params, n_binaries, mf, phase, grad, stream);
+}
+
 int wfx_xas_eval(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
                  int precision, int group, int freqs_per_thread, void* hplus, void* hcross, void* stream) {
     return eval_shared<XasModel>("wfx_xas_eval", records, n_binaries, freqs, freqs_f64, n_freqs, precision, group,
@@ -1450,6 +1494,11 @@ int wfx_phd_reference_phase(const double* params, int64_t n_binaries, const doub
     if (blocks < 0) return -1;
     reference_phase_pd_kernel<<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, mf, phase);
     return check_launch("wfx_phd_reference_phase");
+}
+
+int wfx_phd_reference_phase_grad(const double* params, int64_t n_binaries, const double* mf, double* phase,
+                                 double* grad, void* stream) {
+    return reference_phase_grad<1>("wfx_phd_reference_phase_grad", params, n_binaries, mf, phase, grad, stream);
 }
 
 int wfx_phd_eval(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
