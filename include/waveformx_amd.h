@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (17); a binding checks it before any call. */
+/* The version of this ABI (19); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -256,6 +256,44 @@ int wfx_xhm_setup(const double* params, int64_t n_binaries, const double* f_min,
 int wfx_xhm_eval(const void* records22, const void* head, const void* modes, int64_t n_binaries, const void* freqs,
                  int freqs_f64, int64_t n_freqs, int precision, void* hplus, void* hcross, void* mode_out,
                  void* stream);
+
+/* wfx_xhm_setup_jvp: wfx_xhm_setup's head and higher-mode records (same
+ * arguments, same values) with their derivatives, forward mode through the
+ * same FP64 set-up: dhead (B, wfx_xhm_head_slots()) the head's derivative in
+ * the inclination, dmodes (B, 4 modes, wfx_xhm_record_slots(), 4 directions)
+ * the records' derivatives in (mass_1, mass_2, chi_1z, chi_2z) (for
+ * precision 2, of the split records' slots, float; the FP64 pairs' in their
+ * first word). The (2,2) record and its derivatives come from
+ * wfx_xas_setup_jvp (the same record). The forward half of the reference's
+ * jax.vjp for IMRPhenomXHM (src/waveformx/torch.py:87-93; the MLX port's
+ * tables_jvp, src/waveformx/models/xhm/mlx/_kernels.py:551-603). */
+int wfx_xhm_setup_jvp(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
+                      double amp_scale, double reference_mf, int precision, void* head, void* dhead, void* modes,
+                      void* dmodes, void* stream);
+
+/* wfx_xhm_adjoint: the gradient's projections of the split polarizations
+ * (precision 2) on the float grid freqs (N,), from the set-up of
+ * wfx_xhm_setup_jvp: records22 (B, 64) the (2,2)'s split record, t22 (B, 64,
+ * 4) float its slots' derivatives in (mass_1, mass_2, chi_1z, chi_2z) as
+ * the split evaluation reads them (wfx_xas_setup_jvp's Jacobian with the
+ * anchor folded into the post-inspiral constants, T0's and A's in slots 60
+ * and 62), head, dhead, modes, dmodes as wfx_xhm_setup_jvp writes them.
+ * With weights NULL, in_plus/in_cross are the cotangents (B, N) complex64
+ * (dL/dRe + i dL/dIm); else the data, with weights (N,) float, of the loss
+ * L = sum_f weights (|h+ - d+|^2 + |hx - dx|^2). partial (B, C, 8) double,
+ * C = ceil(N / 256), the sums over frequencies 256 c .. 256 c + 255 (in a
+ * fixed order) of: Re conj(cot) dh/d(mass_1, mass_2, chi_1z, chi_2z)
+ * (0..3); Re sum_k conj(G_k) h_k (4; the distance's derivative is minus it
+ * over d_L); Re conj(cot) dF/d(iota) h (5); Re sum_k m_k conj(G_k) i h_k
+ * (6; phi_ref's derivative, and half of it times the (2,2)'s phase turn per
+ * unit f_ref is f_ref's); the loss (7); G_k = conj(F+_k) cot+ + conj(Fx_k)
+ * cotx is mode k's cotangent. Forward mode per bin, nothing per bin
+ * written: the reverse half of the reference's jax.vjp for IMRPhenomXHM
+ * (src/waveformx/torch.py:108-119; its GPU port's cotangent_product,
+ * src/waveformx/models/xhm/mlx/_kernels.py:300-441). */
+int wfx_xhm_adjoint(const void* records22, const void* t22, const void* head, const void* dhead, const void* modes,
+                    const void* dmodes, int64_t n_binaries, const void* freqs, int64_t n_freqs, int precision,
+                    const void* in_plus, const void* in_cross, const void* weights, double* partial, void* stream);
 
 /* wfx_xhm_project: the FP64 polarizations of wfx_xhm_eval (FP64 records of
  * wfx_xhm_setup, float64 grid) contracted with cotangents without writing

@@ -442,10 +442,18 @@ def test_vjp_of_a_batch_is_that_of_each_binary(cuda):
     assert np.all(np.isnan(batch[3, :7]))
 
 
+def float32_bar(ref):
+    """The reference's float32 gradient bar (tests/test_xas.py:157-171):
+    1e-2 |g| + 1e-3 max |g| per parameter."""
+    return 1e-2 * np.abs(ref) + 1e-3 * np.abs(ref).max(axis=0)
+
+
 def test_loss_and_grad(cuda):
     """adjoint.loss_and_grad for IMRPhenomXHM: the loss of the forward pass
-    in each arithmetic, and the gradient xhm.vjp gives for the cotangents
-    2 w (h - d) of that forward pass; its argument checks."""
+    in each arithmetic; FP64's gradient is xhm.vjp's for the cotangents
+    2 w (h - d) of that forward pass, bit for bit, the split one (exact, the
+    loss fused) within the reference's float32 bar of it; adjoint.vjp routes
+    each arithmetic; the argument checks."""
     import torch
 
     from waveformx_amd import adjoint, xhm
@@ -461,26 +469,109 @@ def test_loss_and_grad(cuda):
         f = torch.as_tensor(freqs, dtype=rdtype, device=cuda)
         w = torch.as_tensor(1.0 / psd, dtype=rdtype, device=cuda)
         dp, dc = xhm.polarizations(d_params, f, precision=precision)
+        print("loss_and_grad", precision, flush=True)
         loss, grads = adjoint.loss_and_grad(p, f, dp, dc, w, precision=precision, approximant="IMRPhenomXHM")
         hp, hc = xhm.polarizations(p, f, precision=precision)
         expected = ((hp - dp).abs() ** 2 + (hc - dc).abs() ** 2) @ w.to(hp.real.dtype)
-        np.testing.assert_allclose(loss.cpu().numpy(), expected.double().cpu().numpy(), rtol=1e-5 if precision == "split" else 1e-12)
+        np.testing.assert_allclose(loss.cpu().numpy(), expected.double().cpu().numpy(),
+                                   rtol=1e-5 if precision == "split" else 1e-12)
         w64 = w.double()
-        ref = xhm.vjp(p, f, 2.0 * w64 * (hp - dp).to(torch.complex128), 2.0 * w64 * (hc - dc).to(torch.complex128))
-        np.testing.assert_array_equal(grads.cpu().numpy(), ref.cpu().numpy())
+        ref = xhm.vjp(p, f.double(), 2.0 * w64 * (hp - dp).to(torch.complex128),
+                      2.0 * w64 * (hc - dc).to(torch.complex128)).cpu().numpy()
+        if precision == "fp64":
+            np.testing.assert_array_equal(grads.cpu().numpy(), ref)
+        else:
+            excess = np.abs(grads.cpu().numpy() - ref) / float32_bar(ref)
+            print("split loss gradient, worst fraction of the float32 bar", excess.max(), flush=True)
+            assert np.all(excess <= 1.0), excess
         assert bool(torch.isfinite(grads[:, :7]).all()) and bool((loss > 0).all())
-    # adjoint.vjp routes to xhm.vjp
-    f64 = f.double()
-    cot = 2.0 * w.double() * (hp - dp).to(torch.complex128)
-    got, grid = adjoint.vjp(p, f64, cot, None, 1e21, None, "split", approximant="IMRPhenomXHM")
+    # adjoint.vjp routes "split" to the exact adjoint and "fp64" to xhm.vjp
+    cot = (2.0 * w * (hp - dp)).to(torch.complex64)
+    got, grid = adjoint.vjp(p, f, cot, None, 1e21, None, "split", approximant="IMRPhenomXHM")
     assert grid is None
-    np.testing.assert_array_equal(got.cpu().numpy(), xhm.vjp(p, f64, cot, torch.zeros_like(cot)).cpu().numpy())
+    np.testing.assert_array_equal(got.cpu().numpy(),
+                                  xhm.adjoint_split(p, f, cot, torch.zeros_like(cot))[0].cpu().numpy())
+    f64 = f.double()
+    cot64 = cot.to(torch.complex128)
+    got, _ = adjoint.vjp(p, f64, cot64, None, 1e21, None, "fp64", approximant="IMRPhenomXHM")
+    np.testing.assert_array_equal(got.cpu().numpy(), xhm.vjp(p, f64, cot64, torch.zeros_like(cot64)).cpu().numpy())
     with pytest.raises(NotImplementedError):
-        adjoint.vjp(p, f64, cot, None, 1e21, None, "split", wrt_grid=True, approximant="IMRPhenomXHM")
+        adjoint.vjp(p, f64, cot64, None, 1e21, None, "split", wrt_grid=True, approximant="IMRPhenomXHM")
+    with pytest.raises(ValueError):
+        adjoint.vjp(p, f64, cot64, None, 1e21, None, "fp32", approximant="IMRPhenomXHM")
     with pytest.raises(ValueError):
         adjoint.loss_and_grad(p, f, dp, dc, w, precision="fp32", approximant="IMRPhenomXHM")
     with pytest.raises(ValueError):
         adjoint.loss_and_grad(p, f, dp, dc, w, precision="split", powers=w, approximant="IMRPhenomXHM")
+
+
+def test_split_gradient(cuda):
+    """The exact split gradient (xhm.adjoint_split: the dual-number set-up
+    and the split evaluation differentiated per bin) for random cotangents
+    on a float32 grid over every region of the modes: within the
+    reference's float32 bar of the FP64 path's (xhm.vjp, differences at the
+    same float32 frequencies) on a random batch and edge rows (equal masses
+    and spins, m1 < m2, a spin of 0.99, q = 18, f_ref = 0 and the
+    mass-scaled reference); the swapped binary's gradient is the permuted
+    one bit for bit; a binary that fails LAL's checks has a NaN row; a batch
+    equals each binary alone; the torch backward of the float32 function is
+    this gradient."""
+    import torch
+
+    import waveformx_amd as wx
+    import waveformx_amd.torch as wt
+    from waveformx_amd import xhm
+
+    params = oxhm.draw(12, seed=21)
+    edge = np.array([[30.0, 30.0, 0.2, 0.2, 400.0, 0.4, 0.0, 20.0],
+                     [20.0, 45.0, 0.3, -0.4, 300.0, 2.9, 0.7, 20.0],
+                     [45.0, 20.0, -0.4, 0.3, 300.0, 2.9, 0.7, 20.0],
+                     [50.0, 12.0, 0.99, 0.3, 300.0, 1.1, 1.7, 20.0],
+                     [90.0, 5.0, 0.5, 0.0, 300.0, 1.1, 1.7, 0.0],
+                     [30.0, 20.0, 1.5, 0.0, 300.0, 1.1, 1.7, 20.0]])
+    params = np.concatenate([params, edge])
+    nb = params.shape[0]
+    p = torch.as_tensor(params, device=cuda)
+    f = torch.linspace(20.0, 1024.0, 2049, dtype=torch.float32, device=cuda)
+    rng = np.random.default_rng(22)
+    cp = torch.as_tensor(rng.normal(size=(nb, 2049)) + 1j * rng.normal(size=(nb, 2049)), dtype=torch.complex64,
+                         device=cuda)
+    cc = torch.as_tensor(rng.normal(size=(nb, 2049)) + 1j * rng.normal(size=(nb, 2049)), dtype=torch.complex64,
+                         device=cuda)
+    cp[nb - 5], cc[nb - 5] = cp[nb - 4], cc[nb - 4]  # the swapped pair sees the same cotangents
+    for reference_mf in (None, 0.01):
+        print("split gradient, reference_mf", reference_mf, flush=True)
+        pr = p.clone()
+        if reference_mf is not None:
+            pr[:, 7] = 0.0  # given at the mass-scaled reference
+        got = xhm.adjoint_split(pr, f, cp, cc, reference_mf=reference_mf)[0].cpu().numpy()
+        ref = xhm.vjp(pr, f.double(), cp.to(torch.complex128), cc.to(torch.complex128),
+                      reference_mf=reference_mf).cpu().numpy()
+        ok = np.arange(nb) != nb - 1
+        assert np.all(np.isnan(got[~ok, :7])) and np.all(np.isfinite(got[ok]))
+        diff, bar = np.abs(got[ok] - ref[ok]), float32_bar(ref[ok])
+        excess = np.where(bar > 0, diff / np.where(bar > 0, bar, 1.0), np.where(diff == 0, 0.0, np.inf))
+        print("worst fraction of the float32 bar per parameter", excess.max(axis=0), flush=True)
+        assert np.all(excess <= 1.0), np.unravel_index(excess.argmax(), excess.shape)
+        swapped, plain = got[nb - 5], got[nb - 4]
+        np.testing.assert_array_equal(swapped[[1, 0, 3, 2, 4, 5, 6, 7]], plain)
+        assert got[nb - 2, 7] == 0.0 and (reference_mf is None or np.all(got[:, 7][ok] == 0.0))
+    for b in (0, nb - 4):
+        one = xhm.adjoint_split(p[b:b + 1], f, cp[b:b + 1], cc[b:b + 1])[0].cpu().numpy()[0]
+        np.testing.assert_array_equal(one, xhm.adjoint_split(p, f, cp, cc)[0].cpu().numpy()[b])
+    # the torch backward of the float32 function (split arithmetic), at the float32-rounded parameters
+    b = 3
+    fields = [torch.tensor(params[b, k], dtype=torch.float32, device=cuda, requires_grad=True) for k in range(8)]
+    zero = fields[2] * 0
+    binary = wx.Binary(fields[0], fields[1], torch.stack([zero, zero, fields[2]]),
+                       torch.stack([zero, zero, fields[3]]), fields[4], fields[5], fields[6], fields[7])
+    hp, hc = wt.polarizations_function("IMRPhenomXHM")(binary, f)
+    loss = (torch.conj(cp[b]) * hp).real.sum() + (torch.conj(cc[b]) * hc).real.sum()
+    loss.backward()
+    torch_grads = np.array([float(x.grad) for x in fields])
+    p32 = p[b:b + 1].float().double()
+    expected = xhm.adjoint_split(p32, f, cp[b:b + 1], cc[b:b + 1])[0].cpu().numpy()[0]
+    np.testing.assert_allclose(torch_grads, expected.astype(np.float32), rtol=1e-6)
 
 
 def test_against_lalsimulation_aligned_xpnr(cuda):

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "wfx_libm.h"
 
@@ -207,6 +208,39 @@ template <class R>
 WFX_HD inline R m_fma(const R& a, const R& b, const R& c) {
     return a * b + c;
 }
+// a * b + c of mixed scalars (a constant times a traced value, say): the
+// same unfused operations as m_fma above, which is chosen when all three
+// are of one type
+template <class A, class B, class C>
+WFX_HD inline auto m_fma(const A& a, const B& b, const C& c) -> decltype(a * b + c) {
+    return a * b + c;
+}
+WFX_HD inline double m_tanh(double x) { return tanh(x); }
+WFX_HD inline float m_tanh(float x) { return tanhf(x); }
+template <int N>
+WFX_HD inline Dual<N> m_tanh(const Dual<N>& x) {
+    const double t = tanh(x.v);
+    return chain(t, x, 1.0 - t * t);
+}
+template <int N>
+WFX_HD inline Dual<N> m_atan2(const Dual<N>& y, const Dual<N>& x) {
+    const double r2 = x.v * x.v + y.v * y.v;
+    Dual<N> out(atan2(y.v, x.v));
+    for (int i = 0; i < N; ++i) out.d[i] = (x.v * y.d[i] - y.v * x.d[i]) / r2;
+    return out;
+}
+// the first derivative direction of a scalar (zero for a plain number)
+WFX_HD inline double tangent_of(double) { return 0.0; }
+template <int N>
+WFX_HD inline double tangent_of(const Dual<N>& x) {
+    return x.d[0];
+}
+// a parameter type left out of template deduction (deduced from the others)
+template <class T>
+struct NoDeduce {
+    using type = T;
+};
+
 template <class R>
 WFX_HD inline R m_fms(const R& a, const R& b, const R& c) {
     return a * b - c;

@@ -16,6 +16,7 @@
 #include "phenomd_point.h"
 #include "phenomd_adjoint.h"
 #include "xas_adjoint.h"
+#include "xhm_adjoint.h"
 #include "xhm_point.h"
 
 #include <atomic>
@@ -1207,13 +1208,87 @@ __global__ __launch_bounds__(64) void xhm_setup_kernel(const double* __restrict_
     const WaveformVars<double> wf = waveform_vars(p, f_min[b * f_min_stride], amp_scale);
     const AmpCoeffs<double> amp22 = amp_coeffs(wf);
     const PhaseCoeffs<double> ph22 = phase_coeffs(wf);
-    HmCoeffs c;
+    HmCoeffs<double> c;
     hm_setup(wf, amp22, ph22, k, c);
     const int64_t at = (b * kHmModes + k) * kHmSlots;
     if (split)
         write_mode_record_split(c, wf, (float*)modes + at);
     else
         write_mode_record(c, wf, (double*)modes + at);
+}
+
+// The higher modes' records with their derivatives in the masses and
+// spins, dual numbers through the whole set-up: one thread per (binary,
+// mode, direction), the (mode, direction) uniform over a block (blocks
+// [g P, (g + 1) P), g = 4 k + direction, P = ceil(nb / 64)); the 17th group
+// writes the head and its derivative in the inclination (one thread per
+// binary). Records as wfx_xhm_setup's (FP64, or split with the tangents of
+// the split slots); dmodes (B, 4 modes, kHmSlots, 4 directions), double or
+// float as the records. The (2,2) and its derivatives come from the
+// IMRPhenomXAS derivative setup (the same record).
+__global__ __launch_bounds__(64) void xhm_setup_jvp_kernel(const double* __restrict__ params, int64_t nb,
+                                                           const double* __restrict__ f_min, int64_t f_min_stride,
+                                                           double amp_scale, double reference_mf, int split,
+                                                           void* __restrict__ head, void* __restrict__ dhead,
+                                                           void* __restrict__ modes, void* __restrict__ dmodes) {
+    const int64_t per_group = (nb + 63) / 64;
+    const int g = (int)(blockIdx.x / per_group);
+    const int64_t b = (int64_t)(blockIdx.x % per_group) * 64 + threadIdx.x;
+    if (g > 4 * kHmModes || b >= nb) return;
+    if (g == 4 * kHmModes) {
+        double p[kParams];
+        for (int i = 0; i < kParams; ++i) p[i] = params[b * kParams + i];
+        with_reference_mf(p, reference_mf);
+        const unsigned fl = waveform_vars(p, f_min[b * f_min_stride], amp_scale).flags;
+        using D = Dual<1>;
+        D inclination(p[kInclination]);
+        inclination.d[0] = 1.0;
+        D h[kHeadSlots];
+        for (int s = 0; s < kHeadSlots; ++s) h[s] = D(0.0);
+        mode_factors(inclination, 2, 2, h + kHeadFactors);
+        for (int m = 0; m < kHmModes; ++m)
+            mode_factors(inclination, kHmEll[m], kHmEmm[m], h + kHeadFactors + 4 * (m + 1));
+        for (int s = 0; s < kHeadSlots; ++s) {
+            // a binary that fails LAL's checks: NaN factors and derivatives
+            const double v = (fl && s < 20) ? __builtin_nan("") : h[s].v;
+            const double dv = (fl && s < 20) ? __builtin_nan("") : h[s].d[0];
+            if (split) {
+                ((float*)head)[b * kHeadSlots + s] = (float)v;
+                ((float*)dhead)[b * kHeadSlots + s] = (float)dv;
+            } else {
+                ((double*)head)[b * kHeadSlots + s] = v;
+                ((double*)dhead)[b * kHeadSlots + s] = dv;
+            }
+        }
+        return;
+    }
+    const int k = g >> 2, dir = g & 3;
+    using D = Dual<1>;
+    D p[kParams];
+    for (int i = 0; i < kParams; ++i) {
+        p[i] = D(params[b * kParams + i]);
+        p[i].d[0] = i == dir ? 1.0 : 0.0;
+    }
+    with_reference_mf(p, reference_mf);
+    const WaveformVars<D> wf = waveform_vars(p, D(f_min[b * f_min_stride]), amp_scale);
+    const AmpCoeffs<D> amp22 = amp_coeffs(wf);
+    const PhaseCoeffs<D> ph22 = phase_coeffs(wf);
+    HmCoeffs<D> c;
+    hm_setup(wf, amp22, ph22, k, c);
+    const int64_t at = (b * kHmModes + k) * kHmSlots;
+    if (split) {
+        float values[kHmSlots], tangents[kHmSlots];
+        write_mode_record_split(c, wf, values, tangents);
+        if (dir == 0)
+            for (int s = 0; s < kHmSlots; ++s) ((float*)modes)[at + s] = values[s];
+        for (int s = 0; s < kHmSlots; ++s) ((float*)dmodes)[4 * (at + s) + dir] = tangents[s];
+    } else {
+        D rec[kHmSlots];
+        write_mode_record(c, wf, rec);
+        if (dir == 0)
+            for (int s = 0; s < kHmSlots; ++s) ((double*)modes)[at + s] = rec[s].v;
+        for (int s = 0; s < kHmSlots; ++s) ((double*)dmodes)[4 * (at + s) + dir] = rec[s].d[0];
+    }
 }
 
 // one thread per (binary, frequency), 256-thread blocks, a block on 256
@@ -1315,6 +1390,70 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
     }
 }
 
+// The gradient of IMRPhenomXHM's polarizations, split arithmetic: the
+// evaluation's shape (a thread per bin, a block on 256 frequencies of one
+// binary), each bin's modes and their derivatives in (m1, m2, chi1z, chi2z)
+// in forward mode (xhm_adjoint.h), contracted with the cotangents
+// (Mode kFromCotangents) or with the residuals of the noise-weighted loss
+// against data (Mode kLoss: in_plus/in_cross are the data; the loss rides in
+// slot 7). partial[(b * chunks + c) * 8 + j], summed over block c's
+// frequencies in a fixed order (64-lane butterflies, then the waves in turn):
+//   j = 0..3  Re sum conj(cot) dh/d(m1, m2, chi1z, chi2z)
+//   j = 4     Re sum conj(G_k) h_k            (d_L: -it / d_L)
+//   j = 5     Re sum conj(cot) dF/d(iota) h   (the inclination)
+//   j = 6     Re sum m_k conj(G_k) i h_k      (phi_ref; f_ref: it / 2 times the
+//             (2,2)'s turn of the phase constant per unit f_ref)
+//   j = 7     the loss
+// with G_k = conj(F+_k) cot+ + conj(Fx_k) cotx the cotangent of mode k.
+constexpr int kXhmGradSlots = 8;
+template <int Mode>
+__global__ __launch_bounds__(kXhmBlock) void xhm_adjoint_split_kernel(
+    const float* __restrict__ rec22, const float* __restrict__ t22, const float* __restrict__ head,
+    const float* __restrict__ dhead, const float* __restrict__ modes, const float* __restrict__ tmodes, int64_t nb,
+    const float* __restrict__ freqs, int64_t n, int64_t chunks, const float2* __restrict__ in_plus,
+    const float2* __restrict__ in_cross, const float* __restrict__ weights, double* __restrict__ partial) {
+    __shared__ double wave_sums[kXhmGradSlots][kXhmBlock / 64];
+    const int64_t b = blockIdx.x / chunks;
+    const int64_t c = blockIdx.x % chunks;
+    if (b >= nb) return;
+    const float* r22 = rec22 + b * kSlots;
+    const float* m = modes + b * kHmModes * kHmSlots;
+    const float* hd = head + b * kHeadSlots;
+    double v[kXhmGradSlots];
+    for (int j = 0; j < kXhmGradSlots; ++j) v[j] = 0.0;
+    const int64_t i = c * kXhmBlock + threadIdx.x;
+    if (i < n) {
+        const float f = freqs[i];
+        float2 cp = in_plus[b * n + i], cc = in_cross[b * n + i];
+        if (Mode == kLoss) {
+            // the residual from the forward pass's values
+            float h[5][2];
+            xhm_modes_split(r22, m, f, h);
+            float2 hp, hc;
+            xhm_sum_f32(hd, h, hp, hc);
+            const float w = weights[i];
+            const float rp_re = hp.x - cp.x, rp_im = hp.y - cp.y, rx_re = hc.x - cc.x, rx_im = hc.y - cc.y;
+            v[7] = (double)(w * (rp_re * rp_re + rp_im * rp_im + rx_re * rx_re + rx_im * rx_im));
+            const float two_w = 2.0f * w;
+            cp = make_float2(two_w * rp_re, two_w * rp_im);
+            cc = make_float2(two_w * rx_re, two_w * rx_im);
+        }
+        xhm_bin_gradient(r22, t22 + b * 4 * kSlots, m, tmodes + b * 4 * kHmModes * kHmSlots, hd,
+                         dhead + b * kHeadSlots, f, cp, cc, v);
+    }
+    for (int j = 0; j < kXhmGradSlots; ++j) {
+        for (int offset = 32; offset > 0; offset >>= 1) v[j] += __shfl_xor(v[j], offset, 64);
+        if ((threadIdx.x & 63) == 0) wave_sums[j][threadIdx.x >> 6] = v[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < kXhmGradSlots) {
+        const int j = threadIdx.x;
+        double total = 0.0;
+        for (int w = 0; w < kXhmBlock / 64; ++w) total += wave_sums[j][w];
+        partial[(b * chunks + c) * kXhmGradSlots + j] = total;
+    }
+}
+
 // the reference phase with its derivatives, for model Model behind `name`
 template <int Model>
 int reference_phase_grad(const char* name, const double* params, int64_t n_binaries, const double* mf,
@@ -1377,7 +1516,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 17; }
+int wfx_abi_version(void) { return 19; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -1662,6 +1801,49 @@ int wfx_xhm_setup(const double* params, int64_t n_binaries, const double* f_min,
                                                              reference_mf, precision == kPrecSplit, records22, flags,
                                                              head, modes);
     return check_launch("wfx_xhm_setup");
+}
+
+int wfx_xhm_setup_jvp(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
+                      double amp_scale, double reference_mf, int precision, void* head, void* dhead, void* modes,
+                      void* dmodes, void* stream) {
+    if (n_binaries < 0) return fail("wfx_xhm_setup_jvp: negative number of binaries");
+    if (n_binaries == 0) return 0;
+    if (!params || !f_min || !head || !dhead || !modes || !dmodes) return fail("wfx_xhm_setup_jvp: null pointer");
+    if (f_min_stride != 0 && f_min_stride != 1) return fail("wfx_xhm_setup_jvp: f_min_stride must be 0 or 1");
+    if (precision != kPrecF64 && precision != kPrecSplit)
+        return fail("wfx_xhm_setup_jvp: precision must be 0 (FP64) or 2 (split)");
+    const int blocks =
+        grid_blocks((4 * kHmModes + 1) * ((n_binaries + 63) / 64), "wfx_xhm_setup_jvp: too many binaries");
+    if (blocks < 0) return -1;
+    xhm_setup_jvp_kernel<<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, f_min, f_min_stride, amp_scale,
+                                                                 reference_mf, precision == kPrecSplit, head, dhead,
+                                                                 modes, dmodes);
+    return check_launch("wfx_xhm_setup_jvp");
+}
+
+int wfx_xhm_adjoint(const void* records22, const void* t22, const void* head, const void* dhead, const void* modes,
+                    const void* dmodes, int64_t n_binaries, const void* freqs, int64_t n_freqs, int precision,
+                    const void* in_plus, const void* in_cross, const void* weights, double* partial, void* stream) {
+    if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xhm_adjoint: negative size");
+    if (n_binaries == 0 || n_freqs == 0) return 0;
+    if (!records22 || !t22 || !head || !dhead || !modes || !dmodes || !freqs || !in_plus || !in_cross || !partial)
+        return fail("wfx_xhm_adjoint: null pointer");
+    if (precision != kPrecSplit) return fail("wfx_xhm_adjoint: precision must be 2 (split)");
+    const int64_t chunks = (n_freqs + kXhmBlock - 1) / kXhmBlock;
+    const int blocks = grid_blocks(n_binaries * chunks, "wfx_xhm_adjoint: too large");
+    if (blocks < 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (weights)
+        xhm_adjoint_split_kernel<kLoss><<<blocks, kXhmBlock, 0, s>>>(
+            (const float*)records22, (const float*)t22, (const float*)head, (const float*)dhead, (const float*)modes,
+            (const float*)dmodes, n_binaries, (const float*)freqs, n_freqs, chunks, (const float2*)in_plus,
+            (const float2*)in_cross, (const float*)weights, partial);
+    else
+        xhm_adjoint_split_kernel<kFromCotangents><<<blocks, kXhmBlock, 0, s>>>(
+            (const float*)records22, (const float*)t22, (const float*)head, (const float*)dhead, (const float*)modes,
+            (const float*)dmodes, n_binaries, (const float*)freqs, n_freqs, chunks, (const float2*)in_plus,
+            (const float2*)in_cross, nullptr, partial);
+    return check_launch("wfx_xhm_adjoint");
 }
 
 int wfx_xhm_eval(const void* records22, const void* head, const void* modes, int64_t n_binaries, const void* freqs,

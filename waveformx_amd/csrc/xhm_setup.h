@@ -22,38 +22,77 @@ namespace wfx {
 
 // ------------------------------------------------------- Taylor scalar
 
+// value, first and second derivative in one variable, over a scalar R
+// (double, or a dual number for the set-up's derivatives in the
+// parameters)
+template <class R>
 struct Jet2 {
-    double v, d, dd;
+    R v, d, dd;
     WFX_HD Jet2() : v(0.0), d(0.0), dd(0.0) {}
-    WFX_HD Jet2(double x) : v(x), d(0.0), dd(0.0) {}
-    WFX_HD Jet2(double x, double dx, double ddx) : v(x), d(dx), dd(ddx) {}
+    WFX_HD Jet2(const R& x) : v(x), d(0.0), dd(0.0) {}
+    WFX_HD Jet2(const R& x, const R& dx, const R& ddx) : v(x), d(dx), dd(ddx) {}
 };
-WFX_HD inline Jet2 jet_chain(const Jet2& x, double f, double f1, double f2) {
-    return Jet2(f, f1 * x.d, f2 * x.d * x.d + f1 * x.dd);
+template <class T>
+struct IsJet {
+    static constexpr bool value = false;
+};
+template <class R>
+struct IsJet<Jet2<R>> {
+    static constexpr bool value = true;
+};
+template <class R>
+WFX_HD inline Jet2<R> jet_chain(const Jet2<R>& x, const R& f, const R& f1, const R& f2) {
+    return Jet2<R>(f, f1 * x.d, f2 * x.d * x.d + f1 * x.dd);
 }
-WFX_HD inline Jet2 operator+(const Jet2& a, const Jet2& b) { return Jet2(a.v + b.v, a.d + b.d, a.dd + b.dd); }
-WFX_HD inline Jet2 operator-(const Jet2& a, const Jet2& b) { return Jet2(a.v - b.v, a.d - b.d, a.dd - b.dd); }
-WFX_HD inline Jet2 operator-(const Jet2& a) { return Jet2(-a.v, -a.d, -a.dd); }
-WFX_HD inline Jet2 operator*(const Jet2& a, const Jet2& b) {
-    return Jet2(a.v * b.v, a.d * b.v + a.v * b.d, a.dd * b.v + 2.0 * a.d * b.d + a.v * b.dd);
+template <class R>
+WFX_HD inline Jet2<R> operator+(const Jet2<R>& a, const Jet2<R>& b) {
+    return Jet2<R>(a.v + b.v, a.d + b.d, a.dd + b.dd);
 }
-WFX_HD inline Jet2 operator/(const Jet2& a, const Jet2& b) {
-    const double q = a.v / b.v;
-    const double dq = (a.d - q * b.d) / b.v;
-    return Jet2(q, dq, (a.dd - 2.0 * dq * b.d - q * b.dd) / b.v);
+template <class R>
+WFX_HD inline Jet2<R> operator-(const Jet2<R>& a, const Jet2<R>& b) {
+    return Jet2<R>(a.v - b.v, a.d - b.d, a.dd - b.dd);
 }
-#define WFX_JET_MIXED(op)                                                                  \
-    WFX_HD inline Jet2 operator op(const Jet2& a, double b) { return a op Jet2(b); }       \
-    WFX_HD inline Jet2 operator op(double a, const Jet2& b) { return Jet2(a) op b; }
+template <class R>
+WFX_HD inline Jet2<R> operator-(const Jet2<R>& a) {
+    return Jet2<R>(-a.v, -a.d, -a.dd);
+}
+template <class R>
+WFX_HD inline Jet2<R> operator*(const Jet2<R>& a, const Jet2<R>& b) {
+    return Jet2<R>(a.v * b.v, a.d * b.v + a.v * b.d, a.dd * b.v + 2.0 * a.d * b.d + a.v * b.dd);
+}
+template <class R>
+WFX_HD inline Jet2<R> operator/(const Jet2<R>& a, const Jet2<R>& b) {
+    const R q = a.v / b.v;
+    const R dq = (a.d - q * b.d) / b.v;
+    return Jet2<R>(q, dq, (a.dd - 2.0 * dq * b.d - q * b.dd) / b.v);
+}
+#define WFX_JET_MIXED(op)                                                                              \
+    template <class R, class S, typename std::enable_if<!IsJet<S>::value, int>::type = 0>              \
+    WFX_HD inline Jet2<R> operator op(const Jet2<R>& a, const S& b) {                                 \
+        return a op Jet2<R>(R(b));                                                                     \
+    }                                                                                                  \
+    template <class R, class S, typename std::enable_if<!IsJet<S>::value, int>::type = 0>              \
+    WFX_HD inline Jet2<R> operator op(const S& a, const Jet2<R>& b) {                                 \
+        return Jet2<R>(R(a)) op b;                                                                     \
+    }
 WFX_JET_MIXED(+)
 WFX_JET_MIXED(-)
 WFX_JET_MIXED(*)
 WFX_JET_MIXED(/)
 #undef WFX_JET_MIXED
-#define WFX_JET_CMP(op)                                                                    \
-    WFX_HD inline bool operator op(const Jet2& a, const Jet2& b) { return a.v op b.v; }    \
-    WFX_HD inline bool operator op(const Jet2& a, double b) { return a.v op b; }           \
-    WFX_HD inline bool operator op(double a, const Jet2& b) { return a op b.v; }
+#define WFX_JET_CMP(op)                                                                                \
+    template <class R>                                                                                 \
+    WFX_HD inline bool operator op(const Jet2<R>& a, const Jet2<R>& b) {                               \
+        return a.v op b.v;                                                                             \
+    }                                                                                                  \
+    template <class R, class S, typename std::enable_if<!IsJet<S>::value, int>::type = 0>              \
+    WFX_HD inline bool operator op(const Jet2<R>& a, const S& b) {                                     \
+        return a.v op b;                                                                               \
+    }                                                                                                  \
+    template <class R, class S, typename std::enable_if<!IsJet<S>::value, int>::type = 0>              \
+    WFX_HD inline bool operator op(const S& a, const Jet2<R>& b) {                                     \
+        return a op b.v;                                                                               \
+    }
 WFX_JET_CMP(<)
 WFX_JET_CMP(>)
 WFX_JET_CMP(<=)
@@ -61,29 +100,49 @@ WFX_JET_CMP(>=)
 WFX_JET_CMP(==)
 WFX_JET_CMP(!=)
 #undef WFX_JET_CMP
-WFX_HD inline double value_of(const Jet2& x) { return x.v; }
-WFX_HD inline Jet2 m_exp(const Jet2& x) {
-    const double e = exp(x.v);
+template <class R>
+WFX_HD inline double value_of(const Jet2<R>& x) {
+    return value_of(x.v);
+}
+template <class R>
+WFX_HD inline Jet2<R> m_exp(const Jet2<R>& x) {
+    const R e = m_exp(x.v);
     return jet_chain(x, e, e, e);
 }
-WFX_HD inline Jet2 m_log(const Jet2& x) { return jet_chain(x, log(x.v), 1.0 / x.v, -1.0 / (x.v * x.v)); }
-WFX_HD inline Jet2 m_sqrt(const Jet2& x) {
-    const double s = sqrt(x.v);
-    return jet_chain(x, s, 0.5 / s, -0.25 / (x.v * s));
+template <class R>
+WFX_HD inline Jet2<R> m_log(const Jet2<R>& x) {
+    return jet_chain(x, m_log(x.v), R(1.0) / x.v, R(-1.0) / (x.v * x.v));
 }
-WFX_HD inline Jet2 m_cbrt(const Jet2& x) {
-    const double c = cbrt(x.v);
+template <class R>
+WFX_HD inline Jet2<R> m_sqrt(const Jet2<R>& x) {
+    const R s = m_sqrt(x.v);
+    return jet_chain(x, s, R(0.5) / s, R(-0.25) / (x.v * s));
+}
+template <class R>
+WFX_HD inline Jet2<R> m_cbrt(const Jet2<R>& x) {
+    const R c = m_cbrt(x.v);
     return jet_chain(x, c, c / (3.0 * x.v), -2.0 * c / (9.0 * x.v * x.v));
 }
-WFX_HD inline Jet2 m_atan(const Jet2& x) {
-    const double r = 1.0 / (1.0 + x.v * x.v);
-    return jet_chain(x, atan(x.v), r, -2.0 * x.v * r * r);
+template <class R>
+WFX_HD inline Jet2<R> m_atan(const Jet2<R>& x) {
+    const R r = R(1.0) / (R(1.0) + x.v * x.v);
+    return jet_chain(x, m_atan(x.v), r, -2.0 * x.v * r * r);
 }
-WFX_HD inline Jet2 m_cos(const Jet2& x) { return jet_chain(x, cos(x.v), -sin(x.v), -cos(x.v)); }
-WFX_HD inline Jet2 m_sin(const Jet2& x) { return jet_chain(x, sin(x.v), cos(x.v), -sin(x.v)); }
-WFX_HD inline Jet2 m_abs(const Jet2& x) { return x.v < 0.0 ? -x : x; }
-WFX_HD inline Jet2 m_pow(const Jet2& x, double y) {
-    return jet_chain(x, pow(x.v, y), y * pow(x.v, y - 1.0), y * (y - 1.0) * pow(x.v, y - 2.0));
+template <class R>
+WFX_HD inline Jet2<R> m_cos(const Jet2<R>& x) {
+    return jet_chain(x, m_cos(x.v), -m_sin(x.v), -m_cos(x.v));
+}
+template <class R>
+WFX_HD inline Jet2<R> m_sin(const Jet2<R>& x) {
+    return jet_chain(x, m_sin(x.v), m_cos(x.v), -m_sin(x.v));
+}
+template <class R>
+WFX_HD inline Jet2<R> m_abs(const Jet2<R>& x) {
+    return x.v < 0.0 ? -x : x;
+}
+template <class R>
+WFX_HD inline Jet2<R> m_pow(const Jet2<R>& x, double y) {
+    return jet_chain(x, m_pow(x.v, y), y * m_pow(x.v, y - 1.0), y * (y - 1.0) * m_pow(x.v, y - 2.0));
 }
 
 // jnp.hypot: the larger modulus times sqrt(1 + (smaller / larger)^2)
@@ -94,18 +153,28 @@ WFX_HD inline double m_hypot(double x, double y) {
     const double r = lo / hi;
     return hi * sqrt(1.0 + r * r);
 }
-WFX_HD inline Jet2 m_hypot(const Jet2& x, const Jet2& y) {
-    const double h = m_hypot(x.v, y.v);
-    const double s = x.v * x.d + y.v * y.d;
-    return Jet2(h, s / h, (x.d * x.d + x.v * x.dd + y.d * y.d + y.v * y.dd) / h - s * s / (h * h * h));
+template <int N>
+WFX_HD inline Dual<N> m_hypot(const Dual<N>& x, const Dual<N>& y) {
+    const Dual<N> a = m_abs(x), b = m_abs(y);
+    const Dual<N> hi = a > b ? a : b, lo = a > b ? b : a;
+    if (hi == 0.0) return hi;
+    const Dual<N> r = lo / hi;
+    return hi * m_sqrt(1.0 + r * r);
+}
+template <class R>
+WFX_HD inline Jet2<R> m_hypot(const Jet2<R>& x, const Jet2<R>& y) {
+    const R h = m_hypot(x.v, y.v);
+    const R s = x.v * x.d + y.v * y.d;
+    return Jet2<R>(h, s / h, (x.d * x.d + x.v * x.dd + y.d * y.d + y.v * y.dd) / h - s * s / (h * h * h));
 }
 WFX_HD inline double m_atan2(double y, double x) { return atan2(y, x); }
-WFX_HD inline Jet2 m_atan2(const Jet2& y, const Jet2& x) {
-    const double r2 = x.v * x.v + y.v * y.v;
-    const double n = x.v * y.d - y.v * x.d;
-    const double n1 = x.v * y.dd - y.v * x.dd;
-    const double r21 = 2.0 * (x.v * x.d + y.v * y.d);
-    return Jet2(atan2(y.v, x.v), n / r2, (n1 * r2 - n * r21) / (r2 * r2));
+template <class R>
+WFX_HD inline Jet2<R> m_atan2(const Jet2<R>& y, const Jet2<R>& x) {
+    const R r2 = x.v * x.v + y.v * y.v;
+    const R n = x.v * y.d - y.v * x.d;
+    const R n1 = x.v * y.dd - y.v * x.dd;
+    const R r21 = 2.0 * (x.v * x.d + y.v * y.d);
+    return Jet2<R>(m_atan2(y.v, x.v), n / r2, (n1 * r2 - n * r21) / (r2 * r2));
 }
 
 // lax.integer_pow (the reference's `x ** n` for a traced x, an int n):
@@ -126,16 +195,16 @@ WFX_HD inline R int_pow(const R& x, int n) {
     return acc;
 }
 
-// a coefficient struct of doubles as the same struct of Jet2 constants
-template <template <class> class S>
-WFX_HD inline S<Jet2> lift_jet(const S<double>& s) {
-    static_assert(sizeof(S<double>) % sizeof(double) == 0, "a struct of doubles");
-    constexpr int n = sizeof(S<double>) / sizeof(double);
-    static_assert(sizeof(S<Jet2>) == n * sizeof(Jet2), "a struct of Jet2");
-    S<Jet2> out;
-    const double* a = reinterpret_cast<const double*>(&s);
-    Jet2* b = reinterpret_cast<Jet2*>(&out);
-    for (int i = 0; i < n; ++i) b[i] = Jet2(a[i]);
+// a coefficient struct of R as the same struct of Jet2<R> constants
+template <template <class> class S, class R>
+WFX_HD inline S<Jet2<R>> lift_jet(const S<R>& s) {
+    static_assert(sizeof(S<R>) % sizeof(R) == 0, "a struct of R");
+    constexpr int n = sizeof(S<R>) / sizeof(R);
+    static_assert(sizeof(S<Jet2<R>>) == n * sizeof(Jet2<R>), "a struct of Jet2<R>");
+    S<Jet2<R>> out;
+    const R* a = reinterpret_cast<const R*>(&s);
+    Jet2<R>* b = reinterpret_cast<Jet2<R>*>(&out);
+    for (int i = 0; i < n; ++i) b[i] = Jet2<R>(a[i]);
     return out;
 }
 
@@ -194,26 +263,27 @@ WFX_HD inline R dphase_22(const R& mf, const Powers<R>& p, const PhaseCoeffs<R>&
 
 // ------------------------------------------------------------ the mode
 
+template <class R>
 struct HmCoeffs {
     int ell, emm, tag, mode_int;
     bool mixing, amp_zero;
-    double chi_s, chi_a, f_meco_lm, f_ring, f_damp, delta_t, f_amp_rd_falloff;
-    double mix2_re, mix2_im, mix3_re, mix3_im;
+    R chi_s, chi_a, f_meco_lm, f_ring, f_damp, delta_t, f_amp_rd_falloff;
+    R mix2_re, mix2_im, mix3_re, mix3_im;
     // amplitude
-    double amp_norm, pn_dominant, f_rd_aux, f_amp_match_in, f_amp_match_im, pn_re[7], pn_im[7], pn_global,
-        fcut73, fcut83, fcut3, ins[3], rd[5], rd_aux[4], inter_center, inter_half, inter[8];
+    R amp_norm, pn_dominant, f_rd_aux, f_amp_match_in, f_amp_match_im, pn_re[7], pn_im[7], pn_global, fcut73,
+        fcut83, fcut3, ins[3], rd[5], rd_aux[4], inter_center, inter_half, inter[8];
     int n_inter;
     // phase
-    double phi[13], phi_l[13], alpha0, alpha2, alpha_l, c0, c1, c2, c3, c4, c_l, c1_insp, c_insp, c1_rd, c_rd,
-        deltaphi, f_phase_match_in, f_phase_match_im;
-    double alpha0_s, alpha_l_s, alpha2_s, alpha4_s, phi0_s;
+    R phi[13], phi_l[13], alpha0, alpha2, alpha_l, c0, c1, c2, c3, c4, c_l, c1_insp, c_insp, c1_rd, c_rd, deltaphi,
+        f_phase_match_in, f_phase_match_im;
+    R alpha0_s, alpha_l_s, alpha2_s, alpha4_s, phi0_s;
     // the (2,2) data of the (3,2) rotation (reference.py:625-633)
-    double f_ring22, f_damp22, inveta, timeshift, phiref22;
+    R f_ring22, f_damp22, inveta, timeshift, phiref22;
 };
 
-template <int N>
-WFX_HD inline double hm_horner(const double (&c)[N], double u) {
-    double total = c[N - 1];
+template <int N, class R>
+WFX_HD inline R hm_horner(const double (&c)[N], const R& u) {
+    R total = R(c[N - 1]);
     for (int k = N - 2; k >= 0; --k) total = total * u + c[k];
     return total;
 }
@@ -221,7 +291,8 @@ WFX_HD inline double hm_horner(const double (&c)[N], double u) {
 
 // variables.py:118-169 with reference.py:102-122 (QNM in 1 - a_f, the
 // 122022 flags)
-WFX_HD inline void mode_vars(const WaveformVars<double>& wf, int k, HmCoeffs& c) {
+template <class R>
+WFX_HD inline void mode_vars(const WaveformVars<R>& wf, int k, HmCoeffs<R>& c) {
     c.ell = kHmEll[k];
     c.emm = kHmEmm[k];
     c.tag = 10 * c.ell + c.emm;
@@ -231,8 +302,8 @@ WFX_HD inline void mode_vars(const WaveformVars<double>& wf, int k, HmCoeffs& c)
     c.chi_s = (wf.chi1l + wf.chi2l) * 0.5;
     c.chi_a = (wf.chi1l - wf.chi2l) * 0.5;
     c.f_meco_lm = wf.f_meco * (double)c.emm * 0.5;
-    const double af = wf.a_final;
-    double fr, fd;
+    const R af = wf.a_final;
+    R fr, fd;
     switch (c.tag) {
         case 21: fr = WFX_HMQ(fring21, af), fd = WFX_HMQ(fdamp21, af); break;
         case 33: fr = WFX_HMQ(fring33, af), fd = WFX_HMQ(fdamp33, af); break;
@@ -243,7 +314,7 @@ WFX_HD inline void mode_vars(const WaveformVars<double>& wf, int k, HmCoeffs& c)
     c.f_damp = fd / wf.m_final;
     c.delta_t = -2.0 * kPi * (500.0 + ut_psi4_to_strain(wf.eta, wf.s_tot_r, wf.dchi));
     c.f_amp_rd_falloff = m_fma(2.0, c.f_damp, c.f_ring);
-    c.mix2_re = c.mix2_im = c.mix3_re = c.mix3_im = 0.0;
+    c.mix2_re = c.mix2_im = c.mix3_re = c.mix3_im = R(0.0);
     if (c.mixing) {
         c.mix2_re = -WFX_HMQ(re_l3m2lp2, af);
         c.mix2_im = -WFX_HMQ(im_l3m2lp2, af);
@@ -254,8 +325,9 @@ WFX_HD inline void mode_vars(const WaveformVars<double>& wf, int k, HmCoeffs& c)
 
 // inspiral.py:49-215 IMRPhenomXHM_GetPNAmplitudeCoefficients (LAL's
 // complex PN coefficients as (re, im) pairs, the reference's order)
-WFX_HD inline void pn_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
-    const double chi_a = c.chi_a, chi_s = c.chi_s, eta = wf.eta, delta = wf.delta;
+template <class R>
+WFX_HD inline void pn_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
+    const R chi_a = c.chi_a, chi_s = c.chi_s, eta = wf.eta, delta = wf.delta;
     const double pi = lalpi_itself;
     const double sqrt_5_7 = sqrt(5.0 / 7.0);
     double prefactor;
@@ -266,18 +338,18 @@ WFX_HD inline void pn_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
         default: prefactor = 4.0 * sqrt(2.0) / 9.0 * sqrt_5_7; break;
     }
     c.pn_global = kHmTwoOverMPow[c.mode_int] * prefactor;
-    const double zero = 0.0 * eta;
-    double* re = c.pn_re;
-    double* im = c.pn_im;
+    const R zero = 0.0 * eta;
+    R* re = c.pn_re;
+    R* im = c.pn_im;
     for (int k = 0; k < 7; ++k) re[k] = im[k] = zero;
     if (c.tag == 21) {
         const Powers<double> p2 = useful_powers(2.0);
-        double r = 3427.0 * chi_a;
+        R r = 3427.0 * chi_a;
         r = r + 3427.0 * chi_s * delta;
         r = r - 8404.0 * chi_a * eta;
         r = r - 3860.0 * chi_s * delta * eta;
         r = r - 1344.0 * delta * pi;
-        double i = -(672.0 * delta);
+        R i = -(672.0 * delta);
         i = i - 672.0 * delta * log(16.0);
         double scale = lalpi_four_thirds * p2.four_thirds;
         re[4] = r / 1344.0 * scale;
@@ -290,11 +362,11 @@ WFX_HD inline void pn_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
         r = r + 20176128.0 * delta * eta * pi;
         r = r + 61725888.0 * chi_s * delta * chi_a * chi_a;
         r = r - 81285120.0 * chi_s * delta * eta * chi_a * chi_a;
-        r = r + 20575296.0 * pow(chi_a, 3.0);
-        r = r - 81285120.0 * eta * pow(chi_a, 3.0);
+        r = r + 20575296.0 * m_pow(chi_a, 3.0);
+        r = r - 81285120.0 * eta * m_pow(chi_a, 3.0);
         r = r + 61725888.0 * chi_a * chi_s * chi_s;
         r = r - 165618432.0 * chi_a * eta * chi_s * chi_s;
-        r = r + 20575296.0 * delta * pow(chi_s, 3.0);
+        r = r + 20575296.0 * delta * m_pow(chi_s, 3.0);
         r = r - 1016064.0 * delta * eta * chi_s * chi_s * chi_s;
         r = r + 128873808.0 * chi_a * eta * eta;
         r = r - 3859632.0 * chi_s * delta * eta * eta;
@@ -305,7 +377,7 @@ WFX_HD inline void pn_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
         scale = lalpi_two * p2.two;
         re[6] = r / 5.419008e6 * scale;
         im[6] = i / 5.419008e6 * scale;
-        const double five = m_fma(21535920.0 * delta * eta, eta, m_fma(39626496.0 * delta * eta * chi_s, chi_s, m_fma(
+        const R five = m_fma(21535920.0 * delta * eta, eta, m_fma(39626496.0 * delta * eta * chi_s, chi_s, m_fma(
             -(77982912.0 * delta * chi_s), chi_s, m_fma(81285120.0 * delta * eta * chi_a, chi_a, m_fma(
                 -(77982912.0 * delta * chi_a), chi_a, m_fma(24385536.0 * chi_s * delta, pi, m_fma(
                     24385536.0 * chi_a, pi, m_fma(-(23670792.0 * delta), eta, m_fma(
@@ -317,12 +389,12 @@ WFX_HD inline void pn_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
     } else if (c.tag == 33) {
         const Powers<double> p2 = useful_powers(2.0 / 3.0);
         const double ln15 = log(1.5);
-        double r = 325.0 * chi_a;
+        R r = 325.0 * chi_a;
         r = r + 325.0 * chi_s * delta;
         r = r - 1120.0 * chi_a * eta;
         r = r - 80.0 * chi_s * delta * eta;
         r = r + 120.0 * delta * pi;
-        double i = -(504.0 * delta);
+        R i = -(504.0 * delta);
         i = i + 720.0 * delta * ln15;
         double scale = lalpi_four_thirds * p2.four_thirds;
         re[4] = r / 120.0 * scale;
@@ -342,7 +414,7 @@ WFX_HD inline void pn_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
         scale = lalpi_two * p2.two;
         re[6] = r / 2.17728e6 * scale;
         im[6] = i / 2.17728e6 * scale;
-        const double five = m_fma(2966264784.0 * delta * eta, eta, m_fma(55883520.0 * delta * eta * chi_s, chi_s, m_fma(
+        const R five = m_fma(2966264784.0 * delta * eta, eta, m_fma(55883520.0 * delta * eta * chi_s, chi_s, m_fma(
             -(1131641280.0 * delta * chi_s), chi_s, m_fma(4470681600.0 * delta * eta * chi_a, chi_a, m_fma(
                 -(1131641280.0 * delta * chi_a), chi_a, m_fma(-(5926068792.0 * delta), eta, m_fma(
                     9053130240.0 * chi_a * chi_s, eta, m_fma(-2263282560.0 * chi_a, chi_s, -(1077664867.0 * delta)))))))));
@@ -350,17 +422,17 @@ WFX_HD inline void pn_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
         re[3] = m_fma(-1945.0, delta, 2268.0 * delta * eta) / 672.0 * lalpi_itself * p2.itself;
         re[5] = five / 4.4706816e8 * lalpi_five_thirds * p2.five_thirds;
     } else if (c.tag == 32) {
-        double r = 0.0 - 3955.0 * chi_s;
+        R r = 0.0 - 3955.0 * chi_s;
         r = r - 3955.0 * chi_a * delta;
         r = r + 10810.0 * chi_s * eta;
         r = r + 11865.0 * chi_a * delta * eta;
         r = r - 12600.0 * chi_s * eta * eta;
-        double i = 2520.0 + zero;
+        R i = 2520.0 + zero;
         i = i - 11088.0 * eta;
         const double scale = lalpi_five_thirds;
         re[5] = r / 840.0 * scale;
         im[5] = i / 840.0 * scale;
-        const double six = m_fma(5907445488.0, pow(eta, 3.0), m_fma(5830513920.0 * chi_s * chi_s * eta, eta, m_fma(
+        const R six = m_fma(5907445488.0, m_pow(eta, 3.0), m_fma(5830513920.0 * chi_s * chi_s * eta, eta, m_fma(
             13412044800.0 * chi_a * chi_a * eta, eta, m_fma(-(13202119896.0 * eta), eta, m_fma(
                 -(11870591040.0 * eta * chi_s), chi_s, m_fma(1131641280.0 * chi_s, chi_s, m_fma(
                     -(7865605440.0 * eta * chi_a), chi_a, m_fma(1131641280.0 * chi_a, chi_a, m_fma(
@@ -372,21 +444,21 @@ WFX_HD inline void pn_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
         re[6] = six / 4.4706816e8 * lalpi_two;
     } else {
         const Powers<double> p2 = useful_powers(0.5);
-        double r = -0.0 + 565.0 * chi_s;
+        R r = -0.0 + 565.0 * chi_s;
         r = r + 565.0 * chi_a * delta;
         r = r - 2075.0 * chi_s * eta;
         r = r - 1695.0 * chi_a * delta * eta;
         r = r + 240.0 * pi;
         r = r - 720.0 * eta * pi;
         r = r + 1140.0 * chi_s * eta * eta;
-        double i = -1008.0 + zero;
+        R i = -1008.0 + zero;
         i = i + 3579.0 * eta;
         i = i + 960.0 * kLn2;
         i = i - 2880.0 * eta * kLn2;
         const double scale = lalpi_five_thirds * p2.five_thirds;
         re[5] = r / 120.0 * scale;
         im[5] = i / 120.0 * scale;
-        const double six = m_fma(-805075876080.0, pow(eta, 3.0), m_fma(-(10897286400.0 * chi_s * chi_s * eta), eta, m_fma(
+        const R six = m_fma(-805075876080.0, m_pow(eta, 3.0), m_fma(-(10897286400.0 * chi_s * chi_s * eta), eta, m_fma(
             -(871782912000.0 * chi_a * chi_a * eta), eta, m_fma(2271682065240.0 * eta, eta, m_fma(
                 224302478400.0 * eta * chi_s, chi_s, m_fma(-(73556683200.0 * chi_s), chi_s, m_fma(
                     511264353600.0 * eta * chi_a, chi_a, m_fma(-(73556683200.0 * chi_a), chi_a, m_fma(
@@ -401,10 +473,11 @@ WFX_HD inline void pn_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
 
 // ------------------------------------------------------------ amplitude
 
-// reference.py:157-176 the PN and inspiral amplitude ansatzes
-template <class R>
-WFX_HD inline R pn_amp(const Powers<R>& p, const double* pn_re, const double* pn_im, double pn_global,
-                       double amp_norm) {
+// reference.py:157-176 the PN and inspiral amplitude ansatzes; R the
+// frequency's scalar, C the coefficients'
+template <class R, class C>
+WFX_HD inline R pn_amp(const Powers<R>& p, const C* pn_re, const C* pn_im, const typename NoDeduce<C>::type& pn_global,
+                       const typename NoDeduce<C>::type& amp_norm) {
     const R terms[6] = {p.one_third, p.two_thirds, p.itself, p.four_thirds, p.five_thirds, p.two};
     R re = R(pn_re[0]), im = R(pn_im[0]);
     for (int k = 0; k < 6; ++k) {
@@ -413,26 +486,29 @@ WFX_HD inline R pn_amp(const Powers<R>& p, const double* pn_re, const double* pn
     }
     return m_hypot(re, im) * (pn_global * p.m_seven_sixths * amp_norm);
 }
-template <class R>
-WFX_HD inline R inspiral_amp(const Powers<R>& p, const double* pn_re, const double* pn_im, double pn_global,
-                             double amp_norm, double pn_dominant, double fcut73, double fcut83, double fcut3,
-                             const double* ins) {
+template <class R, class C>
+WFX_HD inline R inspiral_amp(const Powers<R>& p, const C* pn_re, const C* pn_im,
+                             const typename NoDeduce<C>::type& pn_global, const typename NoDeduce<C>::type& amp_norm,
+                             const typename NoDeduce<C>::type& pn_dominant, const typename NoDeduce<C>::type& fcut73,
+                             const typename NoDeduce<C>::type& fcut83, const typename NoDeduce<C>::type& fcut3,
+                             const C* ins) {
     R pseudo = m_fma(p.seven_thirds / fcut73, R(ins[0]), p.eight_thirds / fcut83 * ins[1]);
     pseudo = m_fma(p.three / fcut3, R(ins[2]), pseudo);
     return pn_amp(p, pn_re, pn_im, pn_global, amp_norm) + pseudo * p.m_seven_sixths * pn_dominant;
 }
 
 // reference.py:203-220 the Lorentzian and its derivative
-template <class R>
-WFX_HD inline R hm_lorentzian(const R& mf, double f_ring, double f_damp, const double* rd) {
+template <class R, class C>
+WFX_HD inline R hm_lorentzian(const R& mf, const C& f_ring, const typename NoDeduce<C>::type& f_damp, const C* rd) {
     const R dfr = mf - f_ring;
-    const double dfd = f_damp * rd[2];
+    const C dfd = f_damp * rd[2];
     return rd[0] * f_damp / (m_exp(rd[1] / dfd * dfr) * m_fma(dfr, dfr, R(dfd * dfd)));
 }
-template <class R>
-WFX_HD inline R hm_dlorentzian(const R& mf, double f_ring, double fda, const double* rd) {
+template <class R, class C>
+WFX_HD inline R hm_dlorentzian(const R& mf, const C& f_ring, const typename NoDeduce<C>::type& fda, const C* rd) {
     const R dfr = mf - f_ring;
-    const R numerator = rd[0] * m_fma(R(fda * fda * rd[1] * rd[2]), R(rd[2]), m_fma(dfr * dfr, R(rd[1]), 2.0 * fda * dfr * rd[2]));
+    const R numerator =
+        rd[0] * m_fma(R(fda * fda * rd[1] * rd[2]), R(rd[2]), m_fma(dfr * dfr, R(rd[1]), 2.0 * fda * dfr * rd[2]));
     R denom = m_fma(dfr, dfr, R(fda * fda * rd[2] * rd[2]));
     denom = rd[2] * denom * denom * m_exp(dfr * rd[1] / (fda * rd[2]));
     return -numerator / denom;
@@ -441,10 +517,11 @@ WFX_HD inline R hm_dlorentzian(const R& mf, double f_ring, double fda, const dou
 // falloff_at is the fall-off's position in its formula, falloff the bound of
 // its region (the same value: the records keep them apart so that the bound
 // can be held fixed while the position varies, as a jnp.where selection)
-template <class R>
-WFX_HD inline R rd_amp(const R& mf, double f_ring, double f_damp, double falloff, bool mixing, const double* rd,
-                       const double* aux, double f_rd_aux, double falloff_at) {
-    R out = mf >= falloff ? rd[3] * m_exp(-rd[4] * (mf - falloff_at)) : hm_lorentzian(mf, f_ring, f_damp, rd);
+template <class R, class C>
+WFX_HD inline R rd_amp(const R& mf, const C& f_ring, const typename NoDeduce<C>::type& f_damp,
+                       const typename NoDeduce<C>::type& falloff, bool mixing, const C* rd, const C* aux,
+                       const typename NoDeduce<C>::type& f_rd_aux, const typename NoDeduce<C>::type& falloff_at) {
+    R out = mf >= falloff ? R(rd[3] * m_exp(-rd[4] * (mf - falloff_at))) : hm_lorentzian(mf, f_ring, f_damp, rd);
     if (mixing) {
         const R cubic = m_fma(m_fma(m_fma(R(aux[3]), mf, R(aux[2])), mf, R(aux[1])), mf, R(aux[0]));
         out = mf < f_rd_aux ? cubic : out;
@@ -456,19 +533,24 @@ WFX_HD inline R rd_amp(const R& mf, double f_ring, double f_damp, double falloff
 
 // phase.py:58-80 the ringdown phase ansatz and its integral: the rescaled
 // (2,2) one or, with mixing, the spheroidal one
-template <class R>
-WFX_HD inline R rd_phase(const R& ff, const Powers<R>& p, double frd, double fda, bool mixing, double alpha0,
-                         double alpha2, double alpha_l, double alpha0_s, double alpha_l_s, double alpha2_s,
-                         double alpha4_s) {
+template <class R, class C>
+WFX_HD inline R rd_phase(const R& ff, const Powers<R>& p, const C& frd, const typename NoDeduce<C>::type& fda,
+                         bool mixing, const typename NoDeduce<C>::type& alpha0,
+                         const typename NoDeduce<C>::type& alpha2, const typename NoDeduce<C>::type& alpha_l,
+                         const typename NoDeduce<C>::type& alpha0_s, const typename NoDeduce<C>::type& alpha_l_s,
+                         const typename NoDeduce<C>::type& alpha2_s, const typename NoDeduce<C>::type& alpha4_s) {
     const R den = m_fma(R(fda), R(fda), (ff - frd) * (ff - frd));
     if (!mixing) return m_fma(R(frd * frd * alpha2), p.m_two, R(alpha0)) + (alpha_l * fda) / den;
     const R d = m_fma(R(alpha2_s), p.m_two, R(alpha0_s));
     return m_fma(R(alpha4_s), p.m_four, d) + (alpha_l_s * fda) / den;
 }
-template <class R>
-WFX_HD inline R rd_phase_int(const R& ff, const Powers<R>& p, double frd, double fda, bool mixing, double alpha0,
-                             double alpha2, double alpha_l, double alpha0_s, double alpha_l_s, double alpha2_s,
-                             double alpha4_s, double phi0_s) {
+template <class R, class C>
+WFX_HD inline R rd_phase_int(const R& ff, const Powers<R>& p, const C& frd, const typename NoDeduce<C>::type& fda,
+                             bool mixing, const typename NoDeduce<C>::type& alpha0,
+                             const typename NoDeduce<C>::type& alpha2, const typename NoDeduce<C>::type& alpha_l,
+                             const typename NoDeduce<C>::type& alpha0_s, const typename NoDeduce<C>::type& alpha_l_s,
+                             const typename NoDeduce<C>::type& alpha2_s, const typename NoDeduce<C>::type& alpha4_s,
+                             const typename NoDeduce<C>::type& phi0_s) {
     const R invf = p.m_one;
     const R arctan = m_atan((ff - frd) / fda);
     if (!mixing) {
@@ -481,12 +563,12 @@ WFX_HD inline R rd_phase_int(const R& ff, const Powers<R>& p, double frd, double
     return m_fma(R(alpha_l_s), arctan, phase_rd);
 }
 // reference.py:366-378 conj(a322) w22 + conj(a323) A_s e^{i phi_s}
-template <class R>
-WFX_HD inline void rotate(const R& mf, const Powers<R>& p, const HmCoeffs& c, const R& w_re, const R& w_im,
+template <class R, class C>
+WFX_HD inline void rotate(const R& mf, const Powers<R>& p, const HmCoeffs<C>& c, const R& w_re, const R& w_im,
                           R& out_re, R& out_im) {
     const R amplm = rd_amp(mf, c.f_ring, c.f_damp, c.f_amp_rd_falloff, true, c.rd, c.rd_aux, c.f_rd_aux,
                            c.f_amp_rd_falloff);
-    const R philm = rd_phase_int(mf, p, c.f_ring, c.f_damp, true, 0.0, 0.0, 0.0, c.alpha0_s, c.alpha_l_s,
+    const R philm = rd_phase_int(mf, p, c.f_ring, c.f_damp, true, C(0.0), C(0.0), C(0.0), c.alpha0_s, c.alpha_l_s,
                                  c.alpha2_s, c.alpha4_s, c.phi0_s);
     const R a_re = w_re * c.mix2_re + w_im * c.mix2_im;
     const R a_im = w_im * c.mix2_re - w_re * c.mix2_im;
@@ -497,8 +579,8 @@ WFX_HD inline void rotate(const R& mf, const Powers<R>& p, const HmCoeffs& c, co
     out_im = a_im + (b_re * sn + b_im * cs);
 }
 // reference.py:381-388 SpheroidalToSpherical with the (2,2) ringdown ansatz
-template <class R>
-WFX_HD inline void spheroidal_to_spherical(const R& mf, const HmCoeffs& c, const AmpCoeffs<R>& amp22,
+template <class R, class C>
+WFX_HD inline void spheroidal_to_spherical(const R& mf, const HmCoeffs<C>& c, const AmpCoeffs<R>& amp22,
                                            const PhaseCoeffs<R>& ph22, R& re, R& im) {
     const Powers<R> p = useful_powers(mf);
     const R a22 = rd_amp_22_ansatz_exact(mf, R(c.f_ring22), R(c.f_damp22), amp22) * c.amp_norm * p.m_seven_sixths;
@@ -508,54 +590,56 @@ WFX_HD inline void spheroidal_to_spherical(const R& mf, const HmCoeffs& c, const
 
 // phase.py:155-193 GetSpheroidalCoefficients (and the (2,2) time shift and
 // phase offset of the (3,2))
-WFX_HD inline void spheroidal_coeffs(const WaveformVars<double>& wf, const PhaseCoeffs<double>& ph22, HmCoeffs& c) {
-    const double fringlm = c.f_ring, fdamplm = c.f_damp;
-    const double freqs[4] = {wf.f_ring, m_fma(-1.5, fdamplm, fringlm), m_fma(-0.5, fdamplm, fringlm),
-                             m_fma(0.5, fdamplm, fringlm)};
-    double a[4][4], x[4] = {hpr_phase_32_p1(wf), hpr_phase_32_p2(wf), hpr_phase_32_p3(wf), hpr_phase_32_p4(wf)};
+template <class R>
+WFX_HD inline void spheroidal_coeffs(const WaveformVars<R>& wf, const PhaseCoeffs<R>& ph22, HmCoeffs<R>& c) {
+    const R fringlm = c.f_ring, fdamplm = c.f_damp;
+    const R freqs[4] = {wf.f_ring, m_fma(-1.5, fdamplm, fringlm), m_fma(-0.5, fdamplm, fringlm),
+                        m_fma(0.5, fdamplm, fringlm)};
+    R a[4][4], x[4] = {hpr_phase_32_p1(wf), hpr_phase_32_p2(wf), hpr_phase_32_p3(wf), hpr_phase_32_p4(wf)};
     for (int i = 0; i < 4; ++i) {
-        const double ff = freqs[i];
-        const double ffm1 = 1.0 / ff, ffm2 = ffm1 * ffm1;
+        const R ff = freqs[i];
+        const R ffm1 = 1.0 / ff, ffm2 = ffm1 * ffm1;
         a[i][0] = 1.0;
         a[i][1] = fdamplm / (fdamplm * fdamplm + (ff - fringlm) * (ff - fringlm));
         a[i][2] = ffm2;
         a[i][3] = ffm2 * ffm2;
     }
-    pivoted_solve<double, 4>(a, x);
-    c.alpha0_s = x[0], c.alpha_l_s = x[1], c.alpha2_s = x[2], c.alpha4_s = x[3], c.phi0_s = 0.0;
-    double fref_rd = wf.f_ring + wf.f_damp;
-    Powers<double> p_ref = useful_powers(fref_rd);
-    const double tshift = hpr_phase_32_spheroidal_time_shift(wf);
-    const double timeshift = time_shift(ph22, wf);
-    const double dphi22_ref = m_fma(1.0 / wf.eta, dphase_22(fref_rd, p_ref, ph22), timeshift);
+    pivoted_solve<R, 4>(a, x);
+    c.alpha0_s = x[0], c.alpha_l_s = x[1], c.alpha2_s = x[2], c.alpha4_s = x[3], c.phi0_s = R(0.0);
+    R fref_rd = wf.f_ring + wf.f_damp;
+    Powers<R> p_ref = useful_powers(fref_rd);
+    const R tshift = hpr_phase_32_spheroidal_time_shift(wf);
+    const R timeshift = time_shift(ph22, wf);
+    const R dphi22_ref = m_fma(1.0 / wf.eta, dphase_22(fref_rd, p_ref, ph22), timeshift);
     c.alpha0_s = c.alpha0_s + dphi22_ref + tshift -
-                 rd_phase(fref_rd, p_ref, fringlm, fdamplm, true, 0.0, 0.0, 0.0, c.alpha0_s, c.alpha_l_s, c.alpha2_s,
-                          c.alpha4_s);
+                 rd_phase(fref_rd, p_ref, fringlm, fdamplm, true, R(0.0), R(0.0), R(0.0), c.alpha0_s, c.alpha_l_s,
+                          c.alpha2_s, c.alpha4_s);
     const double phaseshift = 0.0;
     fref_rd = wf.f_ring;
     p_ref = useful_powers(fref_rd);
-    const double mf_ref = wf.f_ref * wf.m_sec;
-    const double phase_22_mf_ref = phase_22(mf_ref, useful_powers(mf_ref), ph22);
-    const double phase_22_fref = phase_22(fref_rd, p_ref, ph22);
-    const double phiref22 =
+    const R mf_ref = wf.f_ref * wf.m_sec;
+    const R phase_22_mf_ref = phase_22(mf_ref, useful_powers(mf_ref), ph22);
+    const R phase_22_fref = phase_22(fref_rd, p_ref, ph22);
+    const R phiref22 =
         m_fma(-1.0 / wf.eta, phase_22_mf_ref, -(timeshift * mf_ref)) - phaseshift + 2.0 * wf.phi0 + kPi / 4.0;
-    const double phi22_ref = m_fma(1.0 / wf.eta, phase_22_fref, timeshift * fref_rd) + phaseshift + phiref22;
-    const double phishift = hpr_phase_32_spheroidal_phase_shift(wf);
+    const R phi22_ref = m_fma(1.0 / wf.eta, phase_22_fref, timeshift * fref_rd) + phaseshift + phiref22;
+    const R phishift = hpr_phase_32_spheroidal_phase_shift(wf);
     c.phi0_s = phi22_ref -
-               rd_phase_int(fref_rd, p_ref, fringlm, fdamplm, true, 0.0, 0.0, 0.0, c.alpha0_s, c.alpha_l_s,
-                            c.alpha2_s, c.alpha4_s, 0.0) +
+               rd_phase_int(fref_rd, p_ref, fringlm, fdamplm, true, R(0.0), R(0.0), R(0.0), c.alpha0_s, c.alpha_l_s,
+                            c.alpha2_s, c.alpha4_s, R(0.0)) +
                phishift;
     c.timeshift = timeshift;
     c.phiref22 = phiref22;
 }
 
 // reference.py:179-197 the pseudo-PN coefficients (three collocation points)
-WFX_HD inline void inspiral_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
-    const double fcut = c.f_amp_match_in;
-    const double freqs[3] = {0.5 * fcut, 0.75 * fcut, fcut};
-    double values[3];
+template <class R>
+WFX_HD inline void inspiral_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
+    const R fcut = c.f_amp_match_in;
+    const R freqs[3] = {0.5 * fcut, 0.75 * fcut, fcut};
+    R values[3];
     for (int i = 0; i < 3; ++i) {
-        double v;
+        R v;
         switch (c.tag * 10 + i) {
             case 210: v = hai_amp_21_iv1(wf); break;
             case 211: v = hai_amp_21_iv2(wf); break;
@@ -570,20 +654,20 @@ WFX_HD inline void inspiral_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs&
             case 441: v = hai_amp_44_iv2(wf); break;
             default: v = hai_amp_44_iv3(wf); break;
         }
-        values[i] = fabs(v);
+        values[i] = m_abs(v);
     }
-    const Powers<double> finsp = useful_powers(fcut);
+    const Powers<R> finsp = useful_powers(fcut);
     c.fcut73 = finsp.seven_thirds;
     c.fcut83 = finsp.eight_thirds;
     c.fcut3 = finsp.three;
-    double v[3];
+    R v[3];
     for (int i = 0; i < 3; ++i) {
-        const Powers<double> f = useful_powers(freqs[i]);
-        const double pn = pn_amp(f, c.pn_re, c.pn_im, c.pn_global, c.amp_norm);
+        const Powers<R> f = useful_powers(freqs[i]);
+        const R pn = pn_amp(f, c.pn_re, c.pn_im, c.pn_global, c.amp_norm);
         v[i] = (values[i] - pn) / c.pn_dominant / f.m_seven_sixths;
     }
     for (int j = 0; j < 3; ++j) {
-        double s = 0.0;
+        R s = R(0.0);
         for (int i = 0; i < 3; ++i) s = s + kHmInspiralInverse[j][i] * v[i];
         c.ins[j] = s;
     }
@@ -591,15 +675,16 @@ WFX_HD inline void inspiral_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs&
 
 // reference.py:239-272 the ringdown amplitude coefficients (and the (3,2)'s
 // auxiliary cubic)
-WFX_HD inline void rd_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
-    const double fdamp = c.f_damp;
-    double rd[3];
+template <class R>
+WFX_HD inline void rd_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
+    const R fdamp = c.f_damp;
+    R rd[3];
     if (c.tag == 32) {
-        rd[0] = fabs(har_amp_32_alambda(wf));
+        rd[0] = m_abs(har_amp_32_alambda(wf));
         rd[1] = har_amp_32_lambda(wf);
         rd[2] = har_amp_32_sigma(wf);
     } else {
-        double c1, c2, c3;
+        R c1, c2, c3;
         if (c.tag == 21) {
             c1 = har_amp_21_rdcp1(wf), c2 = har_amp_21_rdcp2(wf), c3 = har_amp_21_rdcp3(wf);
         } else if (c.tag == 33) {
@@ -607,82 +692,84 @@ WFX_HD inline void rd_amp_coeffs(const WaveformVars<double>& wf, HmCoeffs& c) {
         } else {
             c1 = har_amp_44_rdcp1(wf), c2 = har_amp_44_rdcp2(wf), c3 = har_amp_44_rdcp3(wf);
         }
-        const double rdcp1 = fabs(c1), rdcp2 = fabs(c2);
-        double rdcp3 = fabs(c3);
-        rdcp3 = rdcp3 >= rdcp2 * rdcp2 / rdcp1 ? 0.5 * rdcp2 * rdcp2 / rdcp1 : rdcp3;
-        rdcp3 = rdcp3 > rdcp2 ? 0.5 * rdcp2 : rdcp3;
+        const R rdcp1 = m_abs(c1), rdcp2 = m_abs(c2);
+        R rdcp3 = m_abs(c3);
+        rdcp3 = rdcp3 >= rdcp2 * rdcp2 / rdcp1 ? R(0.5 * rdcp2 * rdcp2 / rdcp1) : rdcp3;
+        rdcp3 = rdcp3 > rdcp2 ? R(0.5 * rdcp2) : rdcp3;
         rdcp3 = (rdcp1 < rdcp2 && rdcp3 > rdcp1) ? rdcp1 : rdcp3;
-        double deno = sqrt(rdcp1 / rdcp3) - rdcp1 / rdcp2;
-        deno = deno <= 0.0 ? 1e-16 : deno;
-        const double a_lambda = rdcp1 * fdamp / deno;
-        const double sigma = sqrt(a_lambda / (rdcp2 * fdamp));
+        R deno = m_sqrt(rdcp1 / rdcp3) - rdcp1 / rdcp2;
+        deno = deno <= 0.0 ? R(1e-16) : deno;
+        const R a_lambda = rdcp1 * fdamp / deno;
+        const R sigma = m_sqrt(a_lambda / (rdcp2 * fdamp));
         rd[0] = a_lambda;
-        rd[1] = 0.5 * sigma * log(rdcp1 / rdcp3);
+        rd[1] = 0.5 * sigma * m_log(rdcp1 / rdcp3);
         rd[2] = sigma;
     }
-    const double falloff = c.f_amp_rd_falloff;
-    const double value = hm_lorentzian(falloff, c.f_ring, c.f_damp, rd);
+    const R falloff = c.f_amp_rd_falloff;
+    const R value = hm_lorentzian(falloff, c.f_ring, c.f_damp, rd);
     c.rd[0] = rd[0];
     c.rd[1] = rd[1];
     c.rd[2] = rd[2];
     c.rd[3] = value;
     c.rd[4] = -1.0 * hm_dlorentzian(falloff, c.f_ring, c.f_damp, rd) / value;
-    for (int j = 0; j < 4; ++j) c.rd_aux[j] = 0.0;
+    for (int j = 0; j < 4; ++j) c.rd_aux[j] = R(0.0);
     if (c.mixing) {
-        double values[4] = {fabs(har_amp_32_rdaux1(wf)), fabs(har_amp_32_rdaux2(wf)),
-                            hm_lorentzian(c.f_rd_aux, c.f_ring, c.f_damp, c.rd),
-                            hm_dlorentzian(c.f_rd_aux, c.f_ring, c.f_damp, c.rd)};
-        const double freqs[3] = {c.f_amp_match_im, 0.5 * (c.f_amp_match_im + c.f_rd_aux), c.f_rd_aux};
-        double a[4][4];
+        R values[4] = {m_abs(har_amp_32_rdaux1(wf)), m_abs(har_amp_32_rdaux2(wf)),
+                       hm_lorentzian(c.f_rd_aux, c.f_ring, c.f_damp, c.rd),
+                       hm_dlorentzian(c.f_rd_aux, c.f_ring, c.f_damp, c.rd)};
+        const R freqs[3] = {c.f_amp_match_im, 0.5 * (c.f_amp_match_im + c.f_rd_aux), c.f_rd_aux};
+        R a[4][4];
         for (int i = 0; i < 3; ++i) {
-            const double f = freqs[i];
+            const R f = freqs[i];
             a[i][0] = 1.0, a[i][1] = f, a[i][2] = f * f, a[i][3] = f * f * f;
         }
-        const double f = c.f_rd_aux;
+        const R f = c.f_rd_aux;
         a[3][0] = 0.0, a[3][1] = 1.0, a[3][2] = 2.0 * f, a[3][3] = 3.0 * f * f;
-        pivoted_solve<double, 4>(a, values);
+        pivoted_solve<R, 4>(a, values);
         for (int j = 0; j < 4; ++j) c.rd_aux[j] = values[j];
     }
 }
 
 // reference.py:275-334 the intermediate amplitude (a polynomial in u times
 // f^(-7/6) through the collocation points)
-WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<double>& wf, const AmpCoeffs<Jet2>& amp22j,
-                                           const PhaseCoeffs<Jet2>& ph22j, HmCoeffs& c) {
+template <class R>
+WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<R>& wf, const AmpCoeffs<Jet2<R>>& amp22j,
+                                           const PhaseCoeffs<Jet2<R>>& ph22j, HmCoeffs<R>& c) {
     // IMRPhenomXHMIntermediateAmpVersion (reference.py:59-60): 0 skipped,
     // 1 value, 2 value and derivative
     const int v21[6] = {1, 1, 0, 1, 0, 2}, vhm[6] = {2, 1, 1, 1, 1, 2};
     const int* version = c.tag == 21 ? v21 : vhm;
     const int n_colloc = 6;
-    const double deltaf = (c.f_amp_match_im - c.f_amp_match_in) / (n_colloc - 1);
+    const R deltaf = (c.f_amp_match_im - c.f_amp_match_in) / (double)(n_colloc - 1);
     const int n = c.tag == 21 ? 5 : 8;
     const double seven_sixths = 7.0 / 6.0;
-    const double center = 0.5 * (c.f_amp_match_im + c.f_amp_match_in);
-    const double half = 0.5 * (c.f_amp_match_im - c.f_amp_match_in);
+    const R center = 0.5 * (c.f_amp_match_im + c.f_amp_match_in);
+    const R half = 0.5 * (c.f_amp_match_im - c.f_amp_match_in);
     c.inter_center = center;
     c.inter_half = half;
-    double a[8][8], values[8];
+    R a[8][8], values[8];
     int row = 0;
-    auto value_row = [&](double f) {
-        const double m76 = pow(f, -seven_sixths);
-        const double u = (f - center) / half;
+    auto value_row = [&](const R& f) {
+        const R m76 = m_pow(f, -seven_sixths);
+        const R u = (f - center) / half;
         for (int j = 0; j < n; ++j) a[row][j] = m76 * int_pow(u, j);
     };
-    auto derivative_row = [&](double f) {
-        const double m76 = pow(f, -seven_sixths);
-        const double u = (f - center) / half;
+    auto derivative_row = [&](const R& f) {
+        const R m76 = m_pow(f, -seven_sixths);
+        const R u = (f - center) / half;
         for (int j = 0; j < n; ++j)
-            a[row][j] = m76 * (-seven_sixths / f * int_pow(u, j) + (j ? (double)j * int_pow(u, j - 1) / half : 0.0));
+            a[row][j] =
+                m76 * (-seven_sixths / f * int_pow(u, j) + (j ? R((double)j * int_pow(u, j - 1) / half) : R(0.0)));
     };
-    const double f_in = c.f_amp_match_in;
-    const Jet2 ins = inspiral_amp(useful_powers(Jet2(f_in, 1.0, 0.0)), c.pn_re, c.pn_im, c.pn_global, c.amp_norm,
-                                  c.pn_dominant, c.fcut73, c.fcut83, c.fcut3, c.ins);
-    const double f_rd = c.f_amp_match_im;
-    double rd_value, rd_derivative;
+    const R f_in = c.f_amp_match_in;
+    const Jet2<R> ins = inspiral_amp(useful_powers(Jet2<R>(f_in, R(1.0), R(0.0))), c.pn_re, c.pn_im, c.pn_global,
+                                     c.amp_norm, c.pn_dominant, c.fcut73, c.fcut83, c.fcut3, c.ins);
+    const R f_rd = c.f_amp_match_im;
+    R rd_value, rd_derivative;
     if (c.mixing) {
-        Jet2 re, im;
-        spheroidal_to_spherical(Jet2(f_rd, 1.0, 0.0), c, amp22j, ph22j, re, im);
-        const Jet2 h = m_hypot(re, im);
+        Jet2<R> re, im;
+        spheroidal_to_spherical(Jet2<R>(f_rd, R(1.0), R(0.0)), c, amp22j, ph22j, re, im);
+        const Jet2<R> h = m_hypot(re, im);
         rd_value = h.v;
         rd_derivative = h.d;
     } else {
@@ -692,7 +779,7 @@ WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<double>& wf, const
     }
     for (int i = 0; i < n_colloc; ++i) {
         const int digit = version[i];
-        const double f = m_fma(deltaf, (double)i, f_in);
+        const R f = m_fma(deltaf, (double)i, f_in);
         if (digit == 0) continue;
         if (i == 0) {
             value_row(f);
@@ -710,7 +797,7 @@ WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<double>& wf, const
             }
         } else {
             value_row(f);
-            double fit;
+            R fit;
             switch (c.tag * 10 + (i - 1)) {
                 case 210: fit = ham_amp_21_int1(wf); break;
                 case 211: fit = ham_amp_21_int2(wf); break;
@@ -729,23 +816,23 @@ WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<double>& wf, const
                 case 442: fit = ham_amp_44_int3(wf); break;
                 default: fit = ham_amp_44_int4(wf); break;
             }
-            values[row++] = fabs(fit);
+            values[row++] = m_abs(fit);
         }
     }
     if (n == 5) {
-        double a5[5][5], b5[5];
+        R a5[5][5], b5[5];
         for (int i = 0; i < 5; ++i) {
             b5[i] = values[i];
             for (int j = 0; j < 5; ++j) a5[i][j] = a[i][j];
         }
-        pivoted_solve<double, 5>(a5, b5);
+        pivoted_solve<R, 5>(a5, b5);
         for (int j = 0; j < 5; ++j) c.inter[j] = b5[j];
     } else {
-        pivoted_solve<double, 8>(a, values);
+        pivoted_solve<R, 8>(a, values);
         for (int j = 0; j < 8; ++j) c.inter[j] = values[j];
     }
     c.n_inter = n;
-    for (int j = n; j < 8; ++j) c.inter[j] = 0.0;
+    for (int j = n; j < 8; ++j) c.inter[j] = R(0.0);
 }
 
 // -------------------------------------------------------------- phase
@@ -754,44 +841,49 @@ WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<double>& wf, const
 // derivative
 WFX_CONST double kHmInspiralExponents[13] = {-5.0 / 3.0, -4.0 / 3.0, -1.0, -2.0 / 3.0, -1.0 / 3.0, 0.0, 1.0 / 3.0,
                                              2.0 / 3.0, 1.0, 4.0 / 3.0, 5.0 / 3.0, 2.0, 7.0 / 3.0};
-WFX_HD inline double inspiral_phase_int(double mf, const Powers<double>& p, const double* phi, const double* phi_l) {
-    const double freqs[13] = {p.m_five_thirds, p.m_four_thirds, p.m_one, p.m_two_thirds, p.m_one_third, 1.0,
-                              p.one_third, p.two_thirds, mf, p.four_thirds, p.five_thirds, p.two, p.seven_thirds};
-    double philm = 0.0;
+template <class R, class C>
+WFX_HD inline R inspiral_phase_int(const R& mf, const Powers<R>& p, const C* phi, const C* phi_l) {
+    const R freqs[13] = {p.m_five_thirds, p.m_four_thirds, p.m_one, p.m_two_thirds, p.m_one_third, R(1.0),
+                         p.one_third, p.two_thirds, mf, p.four_thirds, p.five_thirds, p.two, p.seven_thirds};
+    R philm = R(0.0);
     for (int i = 0; i < 13; ++i) philm = m_fma(m_fma(phi_l[i], p.log, phi[i]), freqs[i], philm);
     return philm;
 }
-WFX_HD inline double inspiral_phase(double mf, const Powers<double>& p, const double* phi, const double* phi_l) {
-    const double freqs[13] = {p.m_eight_thirds, p.m_seven_thirds, p.m_two, p.m_five_thirds, p.m_four_thirds,
-                              p.m_one, p.m_two_thirds, p.m_one_third, 1.0, p.one_third, p.two_thirds, mf,
-                              p.four_thirds};
-    double dphilm = 0.0;
+template <class R, class C>
+WFX_HD inline R inspiral_phase(const R& mf, const Powers<R>& p, const C* phi, const C* phi_l) {
+    const R freqs[13] = {p.m_eight_thirds, p.m_seven_thirds, p.m_two, p.m_five_thirds, p.m_four_thirds,
+                         p.m_one, p.m_two_thirds, p.m_one_third, R(1.0), p.one_third, p.two_thirds, mf,
+                         p.four_thirds};
+    R dphilm = R(0.0);
     for (int i = 0; i < 13; ++i) {
-        const double coefficient = m_fma(phi_l[i], p.log, phi[i]);
+        const R coefficient = m_fma(phi_l[i], p.log, phi[i]);
         dphilm = m_fma(m_fma(coefficient, kHmInspiralExponents[i], phi_l[i]), freqs[i], dphilm);
     }
     return dphilm;
 }
 // phase.py:83-111 the intermediate phase derivative and phase
-WFX_HD inline double inter_phase(double ff, const Powers<double>& p, const HmCoeffs& c) {
-    const double lorentzian = (c.c_l * c.f_damp) / m_fma(c.f_damp, c.f_damp, (ff - c.f_ring) * (ff - c.f_ring));
-    double dphase = m_fma(c.c1, p.m_one, c.c0);
+template <class R>
+WFX_HD inline R inter_phase(const R& ff, const Powers<R>& p, const HmCoeffs<R>& c) {
+    const R lorentzian = (c.c_l * c.f_damp) / m_fma(c.f_damp, c.f_damp, (ff - c.f_ring) * (ff - c.f_ring));
+    R dphase = m_fma(c.c1, p.m_one, c.c0);
     dphase = m_fma(c.c2, p.m_two, dphase);
     dphase = m_fma(c.c4, p.m_four, dphase);
     if (c.tag == 32) dphase = m_fma(c.c3, p.m_three, dphase);
     return dphase + lorentzian;
 }
-WFX_HD inline double inter_phase_int(double ff, const Powers<double>& p, const HmCoeffs& c) {
-    double phase_ir = m_fma(c.c0, ff, c.c1 * p.log);
+template <class R>
+WFX_HD inline R inter_phase_int(const R& ff, const Powers<R>& p, const HmCoeffs<R>& c) {
+    R phase_ir = m_fma(c.c0, ff, c.c1 * p.log);
     phase_ir = m_fma(-c.c2, p.m_one, phase_ir);
     phase_ir = m_fma(-(1.0 / 3.0 * c.c4), p.m_three, phase_ir);
     if (c.tag == 32) phase_ir = m_fma(-(0.5 * c.c3), p.m_two, phase_ir);
-    return m_fma(c.c_l, atan((ff - c.f_ring) / c.f_damp), phase_ir);
+    return m_fma(c.c_l, m_atan((ff - c.f_ring) / c.f_damp), phase_ir);
 }
 
-// phase.py:205-219 IMRPhenomXHM_PN21AmpSign
-WFX_HD inline bool pn21_amp_sign(double ff, const WaveformVars<double>& wf) {
-    const double eta = wf.eta, chi1 = wf.chi1l, chi2 = wf.chi2l;
+// phase.py:205-219 IMRPhenomXHM_PN21AmpSign (a selection: on the values)
+template <class R>
+WFX_HD inline bool pn21_amp_sign(double ff, const WaveformVars<R>& wf) {
+    const double eta = value_of(wf.eta), chi1 = value_of(wf.chi1l), chi2 = value_of(wf.chi2l);
     const double delta = sqrt(m_fma(-4.0, eta, 1.0));
     const double sqrt5 = sqrt(5.0);
     double output = (-16.0 * delta * eta * ff * pow(kPi, 1.5)) / (3.0 * sqrt5);
@@ -803,7 +895,8 @@ WFX_HD inline bool pn21_amp_sign(double ff, const WaveformVars<double>& wf) {
 }
 
 // phase.py:222-229 the intermediate collocation values and lambda
-WFX_HD inline double phase_fit(const WaveformVars<double>& wf, int tag, int i) {
+template <class R>
+WFX_HD inline R phase_fit(const WaveformVars<R>& wf, int tag, int i) {
     switch (tag * 10 + i) {
         case 210: return hpi_phase_21_p1(wf);
         case 211: return hpi_phase_21_p2(wf);
@@ -834,23 +927,25 @@ WFX_HD inline double phase_fit(const WaveformVars<double>& wf, int tag, int i) {
 
 // reference.py:397-563 IMRPhenomXHM_GetPhaseCoefficients (122022, exact
 // derivatives of the rotated (3,2) ringdown phase)
-WFX_HD inline void phase_coeffs_hm(const WaveformVars<double>& wf, const PhaseCoeffs<double>& ph22,
-                                   const AmpCoeffs<Jet2>& amp22j, const PhaseCoeffs<Jet2>& ph22j, HmCoeffs& c) {
+template <class R>
+WFX_HD inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<R>& ph22,
+                                   const AmpCoeffs<Jet2<R>>& amp22j, const PhaseCoeffs<Jet2<R>>& ph22j,
+                                   HmCoeffs<R>& c) {
     const int emm = c.emm, tag = c.tag;
-    const double eta = wf.eta, eta_m1 = 1.0 / eta;
-    const double fring = c.f_ring, fdamp = c.f_damp;
-    const double fcut = m_fma(0.001, 0.25 / wf.eta - 1.0, 1.0) * (emm * 1.0) * 0.5 * wf.f_meco;  // GetfcutInsp
+    const R eta = wf.eta, eta_m1 = 1.0 / eta;
+    const R fring = c.f_ring, fdamp = c.f_damp;
+    const R fcut = m_fma(0.001, 0.25 / wf.eta - 1.0, 1.0) * (emm * 1.0) * 0.5 * wf.f_meco;  // GetfcutInsp
     const double sqrt3 = sqrt(3.0);
-    double freqs[6], f_phase_match_im;
+    R freqs[6], f_phase_match_im;
     if (tag == 32) {
-        const double f_end = m_fma(-0.5, wf.f_damp, wf.f_ring);
+        const R f_end = m_fma(-0.5, wf.f_damp, wf.f_ring);
         freqs[0] = fcut;
         freqs[1] = m_fma(sqrt3, fcut - f_end, 2.0 * (fcut + f_end)) / 4.0;
         freqs[2] = m_fma(3.0, fcut, f_end) / 4.0;
         freqs[3] = (fcut + f_end) / 2.0;
         freqs[4] = f_end;
         freqs[5] = f_end;
-        f_phase_match_im = (eta < 0.01 && wf.chi1l < 0.0) ? f_end * m_fma(-0.25, wf.chi1l, 1.2) : f_end;
+        f_phase_match_im = (eta < 0.01 && wf.chi1l < 0.0) ? R(f_end * m_fma(-0.25, wf.chi1l, 1.2)) : f_end;
     } else {
         freqs[0] = fcut;
         freqs[1] = m_fma(sqrt3, fcut - fring, 2.0 * (fcut + fring)) / 4.0;
@@ -860,56 +955,56 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<double>& wf, const PhaseCo
         freqs[5] = m_fma(7.0, fring, fcut) / 8.0;
         f_phase_match_im = fring - fdamp;
     }
-    const double f_phase_match_in = c.f_meco_lm;
-    double values[6];
+    const R f_phase_match_in = c.f_meco_lm;
+    R values[6];
     for (int i = 0; i < 6; ++i) values[i] = phase_fit(wf, tag, i) + c.delta_t;
-    double lambda_fit;
+    R lambda_fit;
     switch (tag) {
         case 21: lambda_fit = hpn_phase_21_lambda(wf); break;
         case 33: lambda_fit = hpn_phase_33_lambda(wf); break;
         case 32: lambda_fit = hpn_phase_32_lambda(wf); break;
         default: lambda_fit = hpn_phase_44_lambda(wf); break;
     }
-    const double lambda_pn = wf.eta > 0.01 ? hpn_lambda_pn(wf.eta, tag) : lambda_fit;
-    const double fcut_rd = f_phase_match_im, fcut_insp = f_phase_match_in;
+    const R lambda_pn = wf.eta > 0.01 ? hpn_lambda_pn(wf.eta, tag) : lambda_fit;
+    const R fcut_rd = f_phase_match_im, fcut_insp = f_phase_match_in;
 
     // inspiral: the (2,2) coefficients rescaled, phi_lm(f) ~ m/2 phi_22(2 f / m)
-    const double zero = 0.0;
-    const double non_log[13] = {ph22.phi0, zero, ph22.phi2, ph22.phi3, ph22.phi4, zero, ph22.phi6,
-                                ph22.phi7, ph22.phi8, zero, zero, zero, zero};
-    const double log_terms[13] = {zero, zero, zero, zero, zero, ph22.phi5l, ph22.phi6l, zero, ph22.phi8l, zero,
-                                  zero, zero, zero};
-    const double pseudo_pn[13] = {zero, zero, zero, zero, zero, zero, zero, zero, ph22.sigma1, ph22.sigma2,
-                                  ph22.sigma3, ph22.sigma4, zero};
+    const R zero = R(0.0);
+    const R non_log[13] = {ph22.phi0, zero, ph22.phi2, ph22.phi3, ph22.phi4, zero, ph22.phi6,
+                           ph22.phi7, ph22.phi8, zero, zero, zero, zero};
+    const R log_terms[13] = {zero, zero, zero, zero, zero, ph22.phi5l, ph22.phi6l, zero, ph22.phi8l, zero,
+                             zero, zero, zero};
+    const R pseudo_pn[13] = {zero, zero, zero, zero, zero, zero, zero, zero, ph22.sigma1, ph22.sigma2,
+                             ph22.sigma3, ph22.sigma4, zero};
     const double m_over_2 = emm * 0.5, two_over_m = 1.0 / m_over_2;
-    const double fact = ph22.phi_norm / eta;
+    const R fact = ph22.phi_norm / eta;
     const double log_m_over_2 = kHmLogMOver2[c.mode_int];
     for (int i = 0; i < 13; ++i) {
-        const double nl = non_log[i] * fact, lg = log_terms[i] * fact, ps = pseudo_pn[i] * fact;
+        const R nl = non_log[i] * fact, lg = log_terms[i] * fact, ps = pseudo_pn[i] * fact;
         const double scale = kHmInspiralScale[c.mode_int][i];
         c.phi[i] = m_fma(-lg, log_m_over_2, nl + ps) * scale;
         c.phi_l[i] = lg * scale;
     }
     c.phi[8] = c.phi[8] + lambda_pn;
-    c.alpha0 = c.alpha2 = c.alpha_l = 0.0;
-    c.c0 = c.c1 = c.c2 = c.c3 = c.c4 = c.c_l = 0.0;
+    c.alpha0 = c.alpha2 = c.alpha_l = R(0.0);
+    c.c0 = c.c1 = c.c2 = c.c3 = c.c4 = c.c_l = R(0.0);
 
     if (tag == 21) {
         // for high spins the (2,1) inspiral extends into the intermediate region
-        PhaseCoeffs<double> unconnected = ph22;
-        unconnected.c1_int = unconnected.c2_int = unconnected.c1_mrd = unconnected.c2_mrd = 0.0;
-        double insp[3];
+        PhaseCoeffs<R> unconnected = ph22;
+        unconnected.c1_int = unconnected.c2_int = unconnected.c1_mrd = unconnected.c2_mrd = R(0.0);
+        R insp[3];
         for (int k = 0; k < 3; ++k) {
-            const double ff = two_over_m * freqs[k];
+            const R ff = two_over_m * freqs[k];
             insp[k] = 1.0 / eta * dphase_22(ff, useful_powers(ff), unconnected);
         }
-        const double diff12 = insp[0] - insp[1], diff23 = insp[1] - insp[2];
+        const R diff12 = insp[0] - insp[1], diff23 = insp[1] - insp[2];
         const bool high = wf.s_tot_r >= 0.8;
-        const double v1 = values[2] + diff23;
+        const R v1 = values[2] + diff23;
         values[1] = high ? v1 : values[1];
-        values[0] = high ? v1 + diff12 : values[0];
+        values[0] = high ? R(v1 + diff12) : values[0];
     }
-    double x[6], phi0_rd, dphi0_rd;
+    R x[6], phi0_rd, dphi0_rd;
     if (tag != 32) {
         bool cond_a = eta < 0.05;
         if (emm == c.ell) cond_a = cond_a || wf.s_tot_r >= 0.8;
@@ -920,14 +1015,15 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<double>& wf, const PhaseCo
         const double wlm = emm == c.ell ? 2.0 : emm / 3.0;
         c.alpha2 = 1.0 / (fring * fring) * wlm * hpr_phase_22_alpha2(wf);
         c.alpha_l = eta_m1 * hpr_phase_22_alpha_l(wf);
-        const Powers<double> p_rd = useful_powers(fcut_rd);
-        phi0_rd = rd_phase_int(fcut_rd, p_rd, fring, fdamp, false, c.alpha0, c.alpha2, c.alpha_l, 0.0, 0.0, 0.0, 0.0,
-                               0.0);
-        dphi0_rd = rd_phase(fcut_rd, p_rd, fring, fdamp, false, c.alpha0, c.alpha2, c.alpha_l, 0.0, 0.0, 0.0, 0.0);
-        double a[5][5], b[5];
+        const Powers<R> p_rd = useful_powers(fcut_rd);
+        phi0_rd = rd_phase_int(fcut_rd, p_rd, fring, fdamp, false, c.alpha0, c.alpha2, c.alpha_l, R(0.0), R(0.0),
+                               R(0.0), R(0.0), R(0.0));
+        dphi0_rd = rd_phase(fcut_rd, p_rd, fring, fdamp, false, c.alpha0, c.alpha2, c.alpha_l, R(0.0), R(0.0), R(0.0),
+                            R(0.0));
+        R a[5][5], b[5];
         for (int i = 0; i < 5; ++i) {
-            const double ff = freqs[chosen[i]];
-            const double ffm1 = 1.0 / ff, ffm2 = ffm1 * ffm1;
+            const R ff = freqs[chosen[i]];
+            const R ffm1 = 1.0 / ff, ffm2 = ffm1 * ffm1;
             a[i][0] = 1.0;
             a[i][1] = fdamp / (fdamp * fdamp + (ff - fring) * (ff - fring));
             a[i][2] = ffm1;
@@ -935,14 +1031,14 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<double>& wf, const PhaseCo
             a[i][4] = ffm2 * ffm2;
             b[i] = values[chosen[i]];
         }
-        pivoted_solve<double, 5>(a, b);
+        pivoted_solve<R, 5>(a, b);
         for (int j = 0; j < 5; ++j) x[j] = b[j];
-        x[5] = 0.0;
+        x[5] = R(0.0);
     } else {
         // the rotated ringdown phase with its first and second derivatives
-        Jet2 re, im;
-        spheroidal_to_spherical(Jet2(fcut_rd, 1.0, 0.0), c, amp22j, ph22j, re, im);
-        const Jet2 rotated = m_atan2(im, re);
+        Jet2<R> re, im;
+        spheroidal_to_spherical(Jet2<R>(fcut_rd, R(1.0), R(0.0)), c, amp22j, ph22j, re, im);
+        const Jet2<R> rotated = m_atan2(im, re);
         phi0_rd = rotated.v;
         dphi0_rd = rotated.d;
         const bool comparable = eta > 0.05;
@@ -951,12 +1047,12 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<double>& wf, const PhaseCo
         freqs[5] = fcut_rd;
         values[5] = rotated.dd;
         // the quartic in v = (1/f - g_c) / g_h, converted to powers of 1/f
-        const double g_c = 0.5 * (1.0 / fcut_insp + 1.0 / fcut_rd);
-        const double g_h = 0.5 * (1.0 / fcut_insp - 1.0 / fcut_rd);
-        double a[6][6], b[6];
+        const R g_c = 0.5 * (1.0 / fcut_insp + 1.0 / fcut_rd);
+        const R g_h = 0.5 * (1.0 / fcut_insp - 1.0 / fcut_rd);
+        R a[6][6], b[6];
         for (int i = 0; i < 5; ++i) {
-            const double ff = freqs[i];
-            const double v = (1.0 / ff - g_c) / g_h;
+            const R ff = freqs[i];
+            const R v = (1.0 / ff - g_c) / g_h;
             a[i][0] = 1.0;
             a[i][1] = fdamp / (fdamp * fdamp + (ff - fring) * (ff - fring));
             a[i][2] = v;
@@ -965,10 +1061,10 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<double>& wf, const PhaseCo
             a[i][5] = v * v * v * v;
             b[i] = values[i];
         }
-        const double ff = freqs[5];
-        const double v = (1.0 / ff - g_c) / g_h;
-        const double dv = -1.0 / (ff * ff * g_h);
-        const double den = fdamp * fdamp + (ff - fring) * (ff - fring);
+        const R ff = freqs[5];
+        const R v = (1.0 / ff - g_c) / g_h;
+        const R dv = -1.0 / (ff * ff * g_h);
+        const R den = fdamp * fdamp + (ff - fring) * (ff - fring);
         a[5][0] = 0.0;
         a[5][1] = -2.0 * fdamp * (ff - fring) / (den * den);
         a[5][2] = dv;
@@ -976,13 +1072,13 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<double>& wf, const PhaseCo
         a[5][4] = 3.0 * v * v * dv;
         a[5][5] = 4.0 * v * v * v * dv;
         b[5] = values[5];
-        pivoted_solve<double, 6>(a, b);
-        const double quartic[5] = {b[0], b[2], b[3], b[4], b[5]};
+        pivoted_solve<R, 6>(a, b);
+        const R quartic[5] = {b[0], b[2], b[3], b[4], b[5]};
         const double binom[5][5] = {{1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0}, {1, 3, 3, 1, 0},
                                     {1, 4, 6, 4, 1}};
-        double pc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        R pc[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
         for (int k = 0; k < 5; ++k) {
-            const double scale = quartic[k] / int_pow(g_h, k);
+            const R scale = quartic[k] / int_pow(g_h, k);
             for (int j = 0; j <= k; ++j) pc[j] = pc[j] + scale * binom[k][j] * int_pow(-g_c, k - j);
         }
         x[0] = pc[0], x[1] = b[1], x[2] = pc[1], x[3] = pc[2], x[4] = pc[4], x[5] = pc[3];
@@ -990,25 +1086,25 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<double>& wf, const PhaseCo
     c.c0 = x[0], c.c_l = x[1], c.c1 = x[2], c.c2 = x[3], c.c4 = x[4];
     if (tag == 32) {
         c.c3 = x[5];
-        const Powers<double> p_rd = useful_powers(fcut_rd);
-        const double glued = c.c0 + dphi0_rd - inter_phase(fcut_rd, p_rd, c);
+        const Powers<R> p_rd = useful_powers(fcut_rd);
+        const R glued = c.c0 + dphi0_rd - inter_phase(fcut_rd, p_rd, c);
         c.c0 = eta < 0.05 ? glued : c.c0;
     }
-    const Powers<double> p_in = useful_powers(fcut_insp);
+    const Powers<R> p_in = useful_powers(fcut_insp);
     c.c1_insp = inter_phase(fcut_insp, p_in, c) - inspiral_phase(fcut_insp, p_in, c.phi, c.phi_l);
     c.c_insp = m_fma(-c.c1_insp, fcut_insp, inter_phase_int(fcut_insp, p_in, c)) -
                inspiral_phase_int(fcut_insp, p_in, c.phi, c.phi_l);
-    const Powers<double> p_rd = useful_powers(fcut_rd);
+    const Powers<R> p_rd = useful_powers(fcut_rd);
     c.c1_rd = inter_phase(fcut_rd, p_rd, c) - dphi0_rd;
     c.c_rd = m_fma(-c.c1_rd, fcut_rd, inter_phase_int(fcut_rd, p_rd, c)) - phi0_rd;
 
     // align the mode with the (2,2) at low frequency (Eq. 4.13 of arXiv:2001.10914)
-    const double falign = eta > 0.05 ? 0.6 * m_over_2 * wf.f_meco : m_over_2 * wf.f_meco;
-    const Powers<double> p_falign = useful_powers(falign);
-    const double x22 = two_over_m * falign;
-    const double phi22 = phase_22(x22, useful_powers(x22), ph22) / eta + c.phiref22;
-    const double phi_lm = m_fma(c.c1_insp, falign, inspiral_phase_int(falign, p_falign, c.phi, c.phi_l)) + c.c_insp;
-    double deltaphi = m_fma(-(3.0 * (kPi / 4.0)), 1.0 - m_over_2, m_fma(m_over_2, phi22, c.timeshift * falign)) - phi_lm;
+    const R falign = eta > 0.05 ? R(0.6 * m_over_2 * wf.f_meco) : R(m_over_2 * wf.f_meco);
+    const Powers<R> p_falign = useful_powers(falign);
+    const R x22 = two_over_m * falign;
+    const R phi22 = phase_22(x22, useful_powers(x22), ph22) / eta + c.phiref22;
+    const R phi_lm = m_fma(c.c1_insp, falign, inspiral_phase_int(falign, p_falign, c.phi, c.phi_l)) + c.c_insp;
+    R deltaphi = m_fma(-(3.0 * (kPi / 4.0)), 1.0 - m_over_2, m_fma(m_over_2, phi22, c.timeshift * falign)) - phi_lm;
     if (tag == 21) deltaphi = deltaphi + (pn21_amp_sign(0.008, wf) ? kPi : 0.0);
     c.deltaphi = deltaphi;
     c.f_phase_match_in = f_phase_match_in;
@@ -1016,41 +1112,43 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<double>& wf, const PhaseCo
 }
 
 // reference.py:580-608 mode_setup: the coefficients of higher mode k
-WFX_HD inline void hm_setup(const WaveformVars<double>& wf, const AmpCoeffs<double>& amp22,
-                            const PhaseCoeffs<double>& ph22, int k, HmCoeffs& c) {
+template <class R>
+WFX_HD inline void hm_setup(const WaveformVars<R>& wf, const AmpCoeffs<R>& amp22, const PhaseCoeffs<R>& ph22,
+                            int k, HmCoeffs<R>& c) {
     mode_vars(wf, k, c);
     c.f_ring22 = wf.f_ring;
     c.f_damp22 = wf.f_damp;
     c.inveta = 1.0 / wf.eta;
-    const double timeshift = time_shift(ph22, wf);
-    const double mf_ref = wf.f_ref * wf.m_sec;
+    const R timeshift = time_shift(ph22, wf);
+    const R mf_ref = wf.f_ref * wf.m_sec;
     c.timeshift = timeshift;
     c.phiref22 = m_fma(-1.0 / wf.eta, phase_22(mf_ref, useful_powers(mf_ref), ph22), -(timeshift * mf_ref)) +
                  2.0 * wf.phi0 + kPi / 4.0;
-    c.alpha0_s = c.alpha_l_s = c.alpha2_s = c.alpha4_s = c.phi0_s = 0.0;
+    c.alpha0_s = c.alpha_l_s = c.alpha2_s = c.alpha4_s = c.phi0_s = R(0.0);
     if (c.mixing) spheroidal_coeffs(wf, ph22, c);
     // amplitude_coefficients (reference.py:348-360)
     c.amp_norm = wf.amp_norm;
     c.pn_dominant = wf.amp_norm * kHmTwoOverMPow[c.mode_int];
-    c.f_rd_aux = c.mixing ? c.f_ring - c.f_damp : 0.0;
+    c.f_rd_aux = c.mixing ? R(c.f_ring - c.f_damp) : R(0.0);
     // IMRPhenomXHM_Amplitude_fcutInsp and _fcutRD (reference.py:130-154)
     {
-        const double emm = (double)c.emm, eta = wf.eta, chi1 = wf.chi1l;
-        const double fcut_emr =
+        const double emm = (double)c.emm;
+        const R eta = wf.eta, chi1 = wf.chi1l;
+        const R fcut_emr =
             1.25 * emm *
             (m_fma(0.000316707064291237, chi1 * chi1, m_fma(-0.0000858396080377194, chi1, 0.011671068725758493)) *
              m_fma(6.2873167352395125, eta, 0.8447212540381764)) /
             m_fma(-0.9977728883419751, chi1, 1.2857082764038923);
         const double transition_eta = 0.0192234, sharpness = 0.004;
-        const double funcs = m_fma(0.5, tanh((eta - transition_eta) / sharpness), 0.5);
+        const R funcs = m_fma(0.5, m_tanh((eta - transition_eta) / sharpness), 0.5);
         c.f_amp_match_in = wf.q < 20.0 ? c.f_meco_lm : m_fma(funcs, c.f_meco_lm, (1.0 - funcs) * fcut_emr);
-        c.f_amp_match_im = c.mixing ? m_fma(-0.5, wf.f_damp, wf.f_ring) : c.f_ring - c.f_damp;
+        c.f_amp_match_im = c.mixing ? m_fma(-0.5, wf.f_damp, wf.f_ring) : R(c.f_ring - c.f_damp);
     }
     pn_amp_coeffs(wf, c);
     inspiral_amp_coeffs(wf, c);
     rd_amp_coeffs(wf, c);
-    const AmpCoeffs<Jet2> amp22j = lift_jet(amp22);
-    const PhaseCoeffs<Jet2> ph22j = lift_jet(ph22);
+    const AmpCoeffs<Jet2<R>> amp22j = lift_jet(amp22);
+    const PhaseCoeffs<Jet2<R>> ph22j = lift_jet(ph22);
     intermediate_amp_coeffs(wf, amp22j, ph22j, c);
     phase_coeffs_hm(wf, ph22, amp22j, ph22j, c);
 }
@@ -1061,21 +1159,22 @@ WFX_HD inline void hm_setup(const WaveformVars<double>& wf, const AmpCoeffs<doub
 // constant (two floats each)
 enum HmMassSlot : int { kHmT0F64 = 112, kHmKInsF64 = 114, kHmLogM = 116, kHmCbrtM = 117, kHmInvM = 118 };
 
-WFX_HD inline void write_mode_record(const HmCoeffs& c, const WaveformVars<double>& wf, double* r) {
-    for (int s = 0; s < kHmSlots; ++s) r[s] = 0.0;
-    r[kHmTag] = c.tag;
-    r[kHmOn] = c.amp_zero ? 0.0 : 1.0;
+template <class R>
+WFX_HD inline void write_mode_record(const HmCoeffs<R>& c, const WaveformVars<R>& wf, R* r) {
+    for (int s = 0; s < kHmSlots; ++s) r[s] = R(0.0);
+    r[kHmTag] = R((double)c.tag);
+    r[kHmOn] = R(c.amp_zero ? 0.0 : 1.0);
     r[kHmAmp0] = (c.ell % 2 ? -1.0 : 1.0) * wf.amp0;
     r[kHmMsec] = wf.m_sec;
     r[kHmFRing] = c.f_ring;
     r[kHmFDamp] = c.f_damp;
     r[kHmFalloff] = c.f_amp_rd_falloff;
     r[kHmFalloffAt] = c.f_amp_rd_falloff;
-    const Powers<double> pm = useful_powers(wf.m_sec);
+    const Powers<R> pm = useful_powers(wf.m_sec);
     r[kHmLogM] = pm.log;
     r[kHmCbrtM] = pm.one_third;
     r[kHmInvM] = pm.m_one;
-    r[kHmMixing] = c.mixing ? 1.0 : 0.0;
+    r[kHmMixing] = R(c.mixing ? 1.0 : 0.0);
     r[kHmFAmpIn] = c.f_amp_match_in;
     r[kHmFAmpIm] = c.f_amp_match_im;
     r[kHmFPhaseIn] = c.f_phase_match_in;
@@ -1094,7 +1193,7 @@ WFX_HD inline void write_mode_record(const HmCoeffs& c, const WaveformVars<doubl
     r[kHmInterC] = c.inter_center;
     r[kHmInterH] = c.inter_half;
     for (int k = 0; k < 8; ++k) r[kHmInter + k] = c.inter[k];
-    r[kHmNInter] = c.n_inter;
+    r[kHmNInter] = R((double)c.n_inter);
     for (int k = 0; k < 13; ++k) r[kHmPhi + k] = c.phi[k], r[kHmPhiL + k] = c.phi_l[k];
     r[kHmAlpha0] = c.alpha0;
     r[kHmAlpha2] = c.alpha2;
@@ -1135,42 +1234,61 @@ WFX_HD inline void write_mode_record(const HmCoeffs& c, const WaveformVars<doubl
 //  * quotients folded as the evaluation uses them (the pseudo-PN
 //    coefficients over their cut-off powers, 1 / f_h, the Lorentzian's
 //    rd_0 f_damp, rd_1 / (f_damp rd_2), (f_damp rd_2)^2).
+// With dual numbers, tangent (if not null) receives each slot's derivative
+// in the dual's first direction, as float (the FP64 pairs' in their first
+// word, zero in the second; the reductions modulo 2 pi carry none).
 
-WFX_HD inline double fold_two_pi(double x) { return x - kTwoPi * rint(x * (1.0 / kTwoPi)); }
+template <class R>
+WFX_HD inline R fold_two_pi(const R& x) {
+    return x - kTwoPi * rint(value_of(x) * (1.0 / kTwoPi));
+}
 WFX_HD inline void store_f64(float* r, int slot, double v) { __builtin_memcpy(r + slot, &v, sizeof(double)); }
 
-WFX_HD inline void write_mode_record_split(const HmCoeffs& c, const WaveformVars<double>& wf, float* r) {
-    double d[kHmSlots];
+template <class R>
+WFX_HD inline void write_mode_record_split(const HmCoeffs<R>& c, const WaveformVars<R>& wf, float* r,
+                                           float* tangent = nullptr) {
+    R d[kHmSlots];
     write_mode_record(c, wf, d);
-    for (int s = 0; s < kHmSlots; ++s) r[s] = (float)d[s];
-    r[kHmPhi] = 0.0f;  // (the leading term is the FP64 one)
-    r[kHmIns + 0] = (float)(c.ins[0] / c.fcut73);
-    r[kHmIns + 1] = (float)(c.ins[1] / c.fcut83);
-    r[kHmIns + 2] = (float)(c.ins[2] / c.fcut3);
-    r[kHmInterH] = (float)(1.0 / c.inter_half);
-    const double dfd = c.f_damp * c.rd[2];
-    r[kHmRd + 0] = (float)(c.rd[0] * c.f_damp);
-    r[kHmRd + 1] = (float)(c.rd[1] / dfd);
-    r[kHmRd + 2] = (float)(dfd * dfd);
-    r[kHmDeltaPhi] = (float)fold_two_pi(c.deltaphi);
-    r[kHmCRd] = (float)fold_two_pi(c.c_rd + c.deltaphi);
-    r[kHmSph + 4] = (float)fold_two_pi(c.phi0_s);
-    const Powers<double> pm = useful_powers(wf.m_sec);
-    store_f64(r, kHmT0F64, c.phi[0] * pm.m_five_thirds);
-    store_f64(r, kHmKInsF64, c.c_insp + c.deltaphi);
+    d[kHmPhi] = R(0.0);  // (the leading term is the FP64 one)
+    d[kHmIns + 0] = c.ins[0] / c.fcut73;
+    d[kHmIns + 1] = c.ins[1] / c.fcut83;
+    d[kHmIns + 2] = c.ins[2] / c.fcut3;
+    d[kHmInterH] = 1.0 / c.inter_half;
+    const R dfd = c.f_damp * c.rd[2];
+    d[kHmRd + 0] = c.rd[0] * c.f_damp;
+    d[kHmRd + 1] = c.rd[1] / dfd;
+    d[kHmRd + 2] = dfd * dfd;
+    d[kHmDeltaPhi] = fold_two_pi(c.deltaphi);
+    d[kHmCRd] = fold_two_pi(c.c_rd + c.deltaphi);
+    d[kHmSph + 4] = fold_two_pi(c.phi0_s);
+    const Powers<R> pm = useful_powers(wf.m_sec);
+    const R t0 = c.phi[0] * pm.m_five_thirds;
+    const R k_ins = c.c_insp + c.deltaphi;
+    for (int s = 0; s < kHmSlots; ++s) r[s] = (float)value_of(d[s]);
+    store_f64(r, kHmT0F64, value_of(t0));
+    store_f64(r, kHmKInsF64, value_of(k_ins));
+    if (tangent) {
+        for (int s = 0; s < kHmSlots; ++s) tangent[s] = (float)tangent_of(d[s]);
+        tangent[kHmT0F64] = (float)tangent_of(t0);
+        tangent[kHmT0F64 + 1] = 0.0f;
+        tangent[kHmKInsF64] = (float)tangent_of(k_ins);
+        tangent[kHmKInsF64 + 1] = 0.0f;
+    }
 }
 
 // ------------------------------------------------------------ harmonics
 
-// _harmonics.py:17-38 Wigner's d^l_{m' m}(beta) and {}_{-2}Y_{lm}
+// _harmonics.py:17-38 Wigner's d^l_{m' m}(beta) and {}_{-2}Y_{lm} (generic
+// in beta's scalar, for the inclination's derivative)
 WFX_HD inline double factorial_d(int n) {
     double f = 1.0;
     for (int k = 2; k <= n; ++k) f *= k;
     return f;
 }
-WFX_HD inline double wigner_d(int ell, int mp, int m, double beta) {
-    const double cos_half = cos(beta / 2), sin_half = sin(beta / 2);
-    double total = 0.0;
+template <class R>
+WFX_HD inline R wigner_d(int ell, int mp, int m, const R& beta) {
+    const R cos_half = m_cos(beta / 2), sin_half = m_sin(beta / 2);
+    R total = R(0.0);
     const double prefactor = sqrt(factorial_d(ell + mp) * factorial_d(ell - mp) * factorial_d(ell + m) *
                                   factorial_d(ell - m));
     const int k0 = m - mp > 0 ? m - mp : 0;
@@ -1183,22 +1301,24 @@ WFX_HD inline double wigner_d(int ell, int mp, int m, double beta) {
     }
     return total;
 }
-WFX_HD inline void swsh(double theta, double phi, int ell, int m, double& re, double& im) {
-    const double d = sqrt((2 * ell + 1) / (4 * kPi)) * wigner_d(ell, m, 2, theta);
+template <class R>
+WFX_HD inline void swsh(const R& theta, double phi, int ell, int m, R& re, R& im) {
+    const R d = sqrt((2 * ell + 1) / (4 * kPi)) * wigner_d(ell, m, 2, theta);
     const double arg = m * phi;
     re = d * cos(arg);
     im = d * sin(arg);
 }
 // reference.py:646-653 (F+, Fx) of h_{l,-m}: 0.5 (Y_{l,-m} + (-1)^l conj Y_{lm}),
 // 0.5 i (Y_{l,-m} - (-1)^l conj Y_{lm}), at phi = pi / 2
-WFX_HD inline void mode_factors(double inclination, int ell, int emm, double* out) {
+template <class R>
+WFX_HD inline void mode_factors(const R& inclination, int ell, int emm, R* out) {
     const double sign = ell % 2 ? -1.0 : 1.0;
-    double ym_re, ym_im, y_re, y_im;
+    R ym_re, ym_im, y_re, y_im;
     swsh(inclination, kPi / 2.0, ell, -emm, ym_re, ym_im);
     swsh(inclination, kPi / 2.0, ell, emm, y_re, y_im);
-    const double ys_re = y_re, ys_im = -y_im;  // conj
-    const double sum_re = ym_re + sign * ys_re, sum_im = ym_im + sign * ys_im;
-    const double dif_re = ym_re - sign * ys_re, dif_im = ym_im - sign * ys_im;
+    const R ys_re = y_re, ys_im = -y_im;  // conj
+    const R sum_re = ym_re + sign * ys_re, sum_im = ym_im + sign * ys_im;
+    const R dif_re = ym_re - sign * ys_re, dif_im = ym_im - sign * ys_im;
     out[0] = 0.5 * sum_re;
     out[1] = 0.5 * sum_im;
     // (0 + 0.5 i) (dif_re + i dif_im), complex multiplication

@@ -112,14 +112,11 @@ class _ModelFunction(torch.autograd.Function):
         approximant, amp_scale, reference_mf, arithmetic = ctx.key
         grid, params = ctx.saved_tensors
         wrt_grid = ctx.needs_input_grad[1]
-        if approximant == "IMRPhenomXHM":
-            if wrt_grid:
-                raise NotImplementedError("the gradient of IMRPhenomXHM with respect to the frequencies is not "
-                                          "implemented in waveformx_amd")
-            grads, grad_grid = xhm.vjp(params, grid, grad_hplus, grad_hcross, amp_scale, reference_mf), None
-        else:
-            grads, grad_grid = adjoint.vjp(params, grid.contiguous(), grad_hplus, grad_hcross, amp_scale,
-                                           reference_mf, arithmetic, wrt_grid, approximant)
+        if approximant == "IMRPhenomXHM" and wrt_grid:
+            raise NotImplementedError("the gradient of IMRPhenomXHM with respect to the frequencies is not "
+                                      "implemented in waveformx_amd")
+        grads, grad_grid = adjoint.vjp(params, grid.contiguous(), grad_hplus, grad_hcross, amp_scale,
+                                       reference_mf, arithmetic, wrt_grid, approximant)
         dtype = grid.dtype
         g = grads.to(dtype)
         spin_1 = torch.zeros((g.shape[0], 3), dtype=dtype, device=g.device)
