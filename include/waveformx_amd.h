@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (19); a binding checks it before any call. */
+/* The version of this ABI (20); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -271,29 +271,31 @@ int wfx_xhm_setup_jvp(const double* params, int64_t n_binaries, const double* f_
                       double amp_scale, double reference_mf, int precision, void* head, void* dhead, void* modes,
                       void* dmodes, void* stream);
 
-/* wfx_xhm_adjoint: the gradient's projections of the split polarizations
- * (precision 2) on the float grid freqs (N,), from the set-up of
- * wfx_xhm_setup_jvp: records22 (B, 64) the (2,2)'s split record, t22 (B, 64,
- * 4) float its slots' derivatives in (mass_1, mass_2, chi_1z, chi_2z) as
- * the split evaluation reads them (wfx_xas_setup_jvp's Jacobian with the
- * anchor folded into the post-inspiral constants, T0's and A's in slots 60
- * and 62), head, dhead, modes, dmodes as wfx_xhm_setup_jvp writes them.
- * With weights NULL, in_plus/in_cross are the cotangents (B, N) complex64
- * (dL/dRe + i dL/dIm); else the data, with weights (N,) float, of the loss
- * L = sum_f weights (|h+ - d+|^2 + |hx - dx|^2). partial (B, C, 8) double,
- * C = ceil(N / 256), the sums over frequencies 256 c .. 256 c + 255 (in a
- * fixed order) of: Re conj(cot) dh/d(mass_1, mass_2, chi_1z, chi_2z)
- * (0..3); Re sum_k conj(G_k) h_k (4; the distance's derivative is minus it
- * over d_L); Re conj(cot) dF/d(iota) h (5); Re sum_k m_k conj(G_k) i h_k
- * (6; phi_ref's derivative, and half of it times the (2,2)'s phase turn per
- * unit f_ref is f_ref's); the loss (7); G_k = conj(F+_k) cot+ + conj(Fx_k)
- * cotx is mode k's cotangent. Forward mode per bin, nothing per bin
- * written: the reverse half of the reference's jax.vjp for IMRPhenomXHM
- * (src/waveformx/torch.py:108-119; its GPU port's cotangent_product,
- * src/waveformx/models/xhm/mlx/_kernels.py:300-441). */
-int wfx_xhm_adjoint(const void* records22, const void* t22, const void* head, const void* dhead, const void* modes,
-                    const void* dmodes, int64_t n_binaries, const void* freqs, int64_t n_freqs, int precision,
-                    const void* in_plus, const void* in_cross, const void* weights, double* partial, void* stream);
+/* wfx_xhm_adjoint: grads (B, 8) double, the gradient in (mass_1, mass_2,
+ * chi_1z, chi_2z, luminosity_distance, inclination, phi_ref, f_ref) of the
+ * split polarizations (precision 2) on the float grid freqs (N,), for the
+ * binaries params (B, 8) set up by wfx_xas_setup_jvp (records22 (B, 64) the
+ * (2,2)'s split record, jacobian22 (B, 64, 8) its FP64 derivatives) and
+ * wfx_xhm_setup_jvp (head, dhead, modes, dmodes, precision 2). With weights
+ * NULL, in_plus/in_cross are the cotangents (B, N) complex64 (dL/dRe +
+ * i dL/dIm) and loss is not written; else they are the data, weights (N,)
+ * float, and loss (B,) double receives L = sum_f weights (|h+ - d+|^2 +
+ * |hx - dx|^2), the gradient being L's. Reverse mode: one wave per span of
+ * frequencies and mode of a binary recomputes the mode, sums each bin's
+ * dL/d(record slot) in registers and contracts the totals with the
+ * records' derivatives once; nothing per bin is kept but the (3,2)'s
+ * cotangent of the (2,2) it recycles (and, for the loss, the residuals).
+ * workspace: device memory of at least wfx_xhm_adjoint_workspace(B, N,
+ * weights != NULL) bytes, 256-aligned (B x N x 8 bytes, 3x that for the
+ * loss, and little more). Each sum runs in a fixed order: the result is
+ * reproducible. The reverse half of the reference's jax.vjp for
+ * IMRPhenomXHM (src/waveformx/torch.py:108-119; its GPU port's
+ * cotangent_product, src/waveformx/models/xhm/mlx/_kernels.py:300-441). */
+int64_t wfx_xhm_adjoint_workspace(int64_t n_binaries, int64_t n_freqs, int loss);
+int wfx_xhm_adjoint(const double* params, const void* records22, const double* jacobian22, const void* head,
+                    const void* dhead, const void* modes, const void* dmodes, int64_t n_binaries, const void* freqs,
+                    int64_t n_freqs, int precision, const void* in_plus, const void* in_cross, const void* weights,
+                    void* workspace, int64_t workspace_bytes, double* loss, double* grads, void* stream);
 
 /* wfx_xhm_project: the FP64 polarizations of wfx_xhm_eval (FP64 records of
  * wfx_xhm_setup, float64 grid) contracted with cotangents without writing

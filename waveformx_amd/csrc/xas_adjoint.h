@@ -104,20 +104,25 @@ __device__ inline T phase_and_amp(const T* __restrict__ c, const BinPowers<T>& p
     return phi;
 }
 
-// Inlined before the optimiser simplifies it on its own: compiled as a
-// function, acc is opaque memory, and the compiler merges the regions' last
-// updates into one store through a selected address, which later keeps the
-// caller's accumulator array out of registers.
+// One bin's dL/d(slot), added to acc, with the polarization factors kp, kc
+// given (adjoint_point: the record's; IMRPhenomXHM's (2,2) takes kp = 1,
+// kc = 0, the cotangent of the mode itself), the bin's amplitude and its
+// phase's sine and cosine returned (zero above the cutoff). Inlined before
+// the optimiser simplifies it on its own: compiled as a function, acc is
+// opaque memory, and the compiler merges the regions' last updates into one
+// store through a selected address, which later keeps the caller's
+// accumulator array out of registers.
 template <class T>
-__device__ __forceinline__ void adjoint_point(const T* __restrict__ c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re,
-                                     T gx_im, T* acc) {
+__device__ __forceinline__ void adjoint_point_k(const T* __restrict__ c, const BinPowers<T>& p, T kp, T kc, T gp_re,
+                                                T gp_im, T gx_re, T gx_im, T* acc, T& amp, T& s, T& co) {
     bool past, ringdown;
-    T z, amp;
+    T z;
     const T phi = phase_and_amp(c, p, past, ringdown, z, amp);
-    if (!(p.f <= c[kFCut])) return;  // zero above the cutoff, whatever the record
-    T s, co;
+    if (!(p.f <= c[kFCut])) {  // zero above the cutoff, whatever the record
+        amp = s = co = T(0);
+        return;
+    }
     m_sincos(phi, &s, &co);
-    const T kp = c[kKp], kc = c[kKc];
     const T re_p = kp * (gp_re * co + gp_im * s) + kc * (gx_im * co - gx_re * s);
     const T im_p = kp * (gp_re * s - gp_im * co) + kc * (gx_re * co + gx_im * s);
     const T g_amp = re_p;
@@ -205,6 +210,13 @@ __device__ __forceinline__ void adjoint_point(const T* __restrict__ c, const Bin
             q *= t;
         }
     }
+}
+
+template <class T>
+__device__ __forceinline__ void adjoint_point(const T* __restrict__ c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re,
+                                              T gx_im, T* acc) {
+    T amp, s, co;
+    adjoint_point_k(c, p, c[kKp], c[kKc], gp_re, gp_im, gx_re, gx_im, acc, amp, s, co);
 }
 
 // dL/df of one bin (the frequency itself moving): dL/dA A'(f) + dL/dphi

@@ -1390,67 +1390,301 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
     }
 }
 
-// The gradient of IMRPhenomXHM's polarizations, split arithmetic: the
-// evaluation's shape (a thread per bin, a block on 256 frequencies of one
-// binary), each bin's modes and their derivatives in (m1, m2, chi1z, chi2z)
-// in forward mode (xhm_adjoint.h), contracted with the cotangents
-// (Mode kFromCotangents) or with the residuals of the noise-weighted loss
-// against data (Mode kLoss: in_plus/in_cross are the data; the loss rides in
-// slot 7). partial[(b * chunks + c) * 8 + j], summed over block c's
-// frequencies in a fixed order (64-lane butterflies, then the waves in turn):
-//   j = 0..3  Re sum conj(cot) dh/d(m1, m2, chi1z, chi2z)
-//   j = 4     Re sum conj(G_k) h_k            (d_L: -it / d_L)
-//   j = 5     Re sum conj(cot) dF/d(iota) h   (the inclination)
-//   j = 6     Re sum m_k conj(G_k) i h_k      (phi_ref; f_ref: it / 2 times the
-//             (2,2)'s turn of the phase constant per unit f_ref)
-//   j = 7     the loss
-// with G_k = conj(F+_k) cot+ + conj(Fx_k) cotx the cotangent of mode k.
+// The gradient of IMRPhenomXHM's split polarizations, reverse mode: each
+// binary's grid is cut into `chunks` spans of `span` frequencies (a multiple
+// of 64), and one wave takes one span of one mode of one binary, lane l the
+// span's frequencies l, l + 64, ...: the mode's forward values recomputed,
+// each bin's dL/d(record slot) summed per lane in registers (the (2,2): the
+// IMRPhenomXAS adjoint's 64 slots; a higher mode: its 86 slots with a
+// derivative), one reduce-scatter over the wave, and the slots' totals
+// contracted once with the record's derivatives. The (3,2)'s ringdown
+// recycles the (2,2): its wave writes the cotangent it passes to the (2,2)
+// per bin (gh22), which the (2,2)'s wave, launched after, adds to its own.
+// Per wave, partial[((b * 5 + row) * chunks + c) * 8 + j] (row 0 the (2,2),
+// 1..4 the higher modes): Re conj(cot) dh/d(mass_1, mass_2, chi_1z, chi_2z)
+// (0..3); Re conj(G) h (4); Re conj(cot) dF/d(iota) h (5); Re m conj(G) i h
+// (6), G = conj(F+) cot+ + conj(Fx) cotx the mode's cotangent; 0 (7).
 constexpr int kXhmGradSlots = 8;
-template <int Mode>
-__global__ __launch_bounds__(kXhmBlock) void xhm_adjoint_split_kernel(
-    const float* __restrict__ rec22, const float* __restrict__ t22, const float* __restrict__ head,
-    const float* __restrict__ dhead, const float* __restrict__ modes, const float* __restrict__ tmodes, int64_t nb,
-    const float* __restrict__ freqs, int64_t n, int64_t chunks, const float2* __restrict__ in_plus,
-    const float2* __restrict__ in_cross, const float* __restrict__ weights, double* __restrict__ partial) {
-    __shared__ double wave_sums[kXhmGradSlots][kXhmBlock / 64];
+constexpr int kXhmGradRows = 5;
+// at least this many higher-mode waves per launch (the chip holds ~3,000
+// at the kernels' occupancy)
+constexpr int64_t kXhmGradWaves = 12288;
+
+struct XhmGradPlan {
+    int64_t chunks, span, loss_chunks;
+    int64_t table, gh22, partial, cot_plus, cot_cross, loss_partial, bytes;
+};
+
+inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
+
+// the spans and the workspace's layout (each part 256-aligned) of B
+// binaries on N frequencies; bytes < 0 where the sizes overflow
+inline XhmGradPlan xhm_grad_plan(int64_t nb, int64_t n, bool loss) {
+    XhmGradPlan g{};
+    const int64_t max_chunks = (n + 63) / 64;
+    int64_t chunks = (kXhmGradWaves + 4 * nb - 1) / (4 * nb);
+    chunks = chunks < 1 ? 1 : chunks > max_chunks ? max_chunks : chunks;
+    g.span = ((n + chunks - 1) / chunks + 63) / 64 * 64;
+    g.chunks = (n + g.span - 1) / g.span;
+    g.loss_chunks = (n + kXhmBlock - 1) / kXhmBlock;
+    if (nb > (INT64_MAX / 64) / n) {
+        g.bytes = -1;
+        return g;
+    }
+    int64_t off = 0;
+    g.table = off;
+    off += align256(n * (int64_t)sizeof(PowerRow<float>));
+    g.gh22 = off;
+    off += align256(nb * n * 8);
+    g.partial = off;
+    off += align256(nb * kXhmGradRows * g.chunks * kXhmGradSlots * 8);
+    if (loss) {
+        g.cot_plus = off;
+        off += align256(nb * n * 8);
+        g.cot_cross = off;
+        off += align256(nb * n * 8);
+        g.loss_partial = off;
+        off += align256(nb * g.loss_chunks * 8);
+    }
+    g.bytes = off;
+    return g;
+}
+
+// a double summed over the wave (every lane active), in every lane
+__device__ inline double wave_sum(double x) {
+    for (int offset = 32; offset > 0; offset >>= 1) x += __shfl_xor(x, offset, 64);
+    return x;
+}
+
+// the sums over the wave of d[0..3] and the closed forms, written by lane 0
+__device__ inline void xhm_grad_store(double* d, float v4, float v5, float v6, int lane, double* out) {
+    double v[kXhmGradSlots] = {d[0], d[1], d[2], d[3], (double)v4, (double)v5, (double)v6, 0.0};
+#pragma unroll
+    for (int j = 0; j < 7; ++j) v[j] = wave_sum(v[j]);
+    if (lane == 0)
+        for (int j = 0; j < kXhmGradSlots; ++j) out[j] = v[j];
+}
+
+// mode h's cotangent from the polarizations' (F+, Fx the mode's factors)
+__device__ inline void mode_cotangent(const float* fac, float2 cp, float2 cc, float& g_re, float& g_im) {
+    g_re = fac[0] * cp.x + fac[1] * cp.y + fac[2] * cc.x + fac[3] * cc.y;
+    g_im = fac[0] * cp.y - fac[1] * cp.x + fac[2] * cc.y - fac[3] * cc.x;
+}
+
+// the inclination's closed form: Re(conj(cot+) dF+ h + conj(cotx) dFx h)
+__device__ inline float inclination_term(const float* dfac, float2 cp, float2 cc, float h_re, float h_im) {
+    const float dp_re = dfac[0] * h_re - dfac[1] * h_im, dp_im = dfac[0] * h_im + dfac[1] * h_re;
+    const float dc_re = dfac[2] * h_re - dfac[3] * h_im, dc_im = dfac[2] * h_im + dfac[3] * h_re;
+    return cp.x * dp_re + cp.y * dp_im + cc.x * dc_re + cc.y * dc_im;
+}
+
+// the cotangents 2 w (h - d) of the loss and its value per block (the
+// evaluation's shape; loss_partial[b * chunks + c], summed in a fixed order)
+__global__ __launch_bounds__(kXhmBlock) void xhm_residual_split_kernel(
+    const float* __restrict__ rec22, const float* __restrict__ head, const float* __restrict__ modes, int64_t nb,
+    const float* __restrict__ freqs, int64_t n, int64_t chunks, const float2* __restrict__ data_plus,
+    const float2* __restrict__ data_cross, const float* __restrict__ weights, float2* __restrict__ cot_plus,
+    float2* __restrict__ cot_cross, double* __restrict__ loss_partial) {
+    __shared__ double wave_sums[kXhmBlock / 64];
+    const int64_t b = blockIdx.x / chunks;
+    const int64_t c = blockIdx.x % chunks;
+    const int64_t i = c * kXhmBlock + threadIdx.x;
+    double l = 0.0;
+    if (b < nb && i < n) {
+        float h[5][2];
+        xhm_modes_split(rec22 + b * kSlots, modes + b * kHmModes * kHmSlots, freqs[i], h);
+        float2 hp, hc;
+        xhm_sum_f32(head + b * kHeadSlots, h, hp, hc);
+        const float2 dp = data_plus[b * n + i], dc = data_cross[b * n + i];
+        const float w = weights[i];
+        const float rp_re = hp.x - dp.x, rp_im = hp.y - dp.y, rx_re = hc.x - dc.x, rx_im = hc.y - dc.y;
+        l = (double)(w * (rp_re * rp_re + rp_im * rp_im + rx_re * rx_re + rx_im * rx_im));
+        const float two_w = 2.0f * w;
+        cot_plus[b * n + i] = make_float2(two_w * rp_re, two_w * rp_im);
+        cot_cross[b * n + i] = make_float2(two_w * rx_re, two_w * rx_im);
+    }
+    l = wave_sum(l);
+    if ((threadIdx.x & 63) == 0) wave_sums[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0 && b < nb) {
+        double total = 0.0;
+        for (int w = 0; w < kXhmBlock / 64; ++w) total += wave_sums[w];
+        loss_partial[b * chunks + c] = total;
+    }
+}
+
+// one wave per (binary, span, higher mode k): the mode's adjoint
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void xhm_hm_adjoint_kernel(
+    const float* __restrict__ rec22, const float* __restrict__ head, const float* __restrict__ dhead,
+    const float* __restrict__ modes, const float* __restrict__ dmodes, int64_t nb,
+    const PowerRow<float>* __restrict__ table, int64_t n, int64_t chunks, int64_t span,
+    const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross, float2* __restrict__ gh22,
+    double* __restrict__ partial) {
+    const int64_t x = blockIdx.x;
+    const int k = (int)(x % kHmModes);
+    const int64_t c = (x / kHmModes) % chunks;
+    const int64_t b = x / kHmModes / chunks;
+    if (b >= nb) return;
+    const int lane = threadIdx.x;
+    // the records in LDS: the loop's many record reads then cost no scalar
+    // loads and waits (its registers cannot hold them all)
+    __shared__ float s_mode[kHmSlots], s_22[kSlots];
+    for (int j = lane; j < kHmSlots; j += 64) s_mode[j] = modes[(b * kHmModes + k) * kHmSlots + j];
+    s_22[lane] = rec22[b * kSlots + lane];
+    __syncthreads();
+    const float* r = s_mode;
+    const float* r22 = s_22;
+    const float* fac = head + b * kHeadSlots + kHeadFactors + 4 * (k + 1);
+    const float* dfac = dhead + b * kHeadSlots + kHeadFactors + 4 * (k + 1);
+    const float emm = (float)kHmEmm[k];
+    const bool on = r[kHmOn] != 0.0f;
+    const bool mixing = r[kHmMixing] != 0.0f;
+    const bool writes = k == kHmMixMode;
+    const float amp0 = r[kHmAmp0];
+    const float inv_amp0 = __builtin_amdgcn_rcpf(fabsf(amp0));
+    const float sign_inv = amp0 < 0.0f ? -inv_amp0 : inv_amp0;
+    const int64_t i0 = c * span;
+    const int64_t i1 = i0 + span < n ? i0 + span : n;
+    float acc[kHmAcc];
+#pragma unroll
+    for (int j = 0; j < kHmAcc; ++j) acc[j] = 0.0f;
+    float v4 = 0.0f, v5 = 0.0f, v6 = 0.0f;
+    for (int64_t i = i0 + lane; i < i1; i += 64) {
+        float gw_re = 0.0f, gw_im = 0.0f;
+        if (on) {
+            const PowerRow<float> row = table[i];
+            const BinPowers<float> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
+            const float2 cp = cot_plus[b * n + i], cc = cot_cross[b * n + i];
+            float g_re, g_im;
+            mode_cotangent(fac, cp, cc, g_re, g_im);
+            float w_re = 0.0f, w_im = 0.0f;
+            const float mf = r[kHmMsec] * p.f;
+            if (mixing && (mf >= r[kHmFAmpIm] || mf >= r[kHmFPhaseIm])) {
+                // the (2,2) over |amp0|, as xhm_modes_split computes it
+                PowF32 q;
+                q.f = p.f;
+                q.t = p.t;
+                q.m1 = p.m1;
+                q.lf = p.lf;
+                q.m76 = p.m76;
+                q.m53 = p.m53;
+                q.m53_hi = q.m53_lo = 0.0f;
+                float a22, phi22, s22, c22;
+                amp_phase_f32<kLeadF64>(r22, q, a22, phi22);
+                __sincosf(phi22, &s22, &c22);
+                w_re = a22 * c22 * inv_amp0;
+                w_im = a22 * s22 * inv_amp0;
+            }
+            float h_re, h_im;
+            hm_adjoint_split(r, p, w_re, w_im, g_re, g_im, acc, h_re, h_im, gw_re, gw_im);
+            v4 += g_re * h_re + g_im * h_im;
+            v6 += emm * (g_im * h_re - g_re * h_im);
+            v5 += inclination_term(dfac, cp, cc, h_re, h_im);
+            // w's dependence on amp0 (w = h_22 / |amp0|), for the mixing mode
+            if (mixing) acc[aAmp0] -= (gw_re * w_re + gw_im * w_im) * sign_inv;
+        }
+        if (writes) gh22[b * n + i] = make_float2(gw_re * inv_amp0, gw_im * inv_amp0);
+    }
+    if (!on) {
+        // a mode that vanishes (the odd modes of equal masses): nothing,
+        // whatever its record's derivatives (which may be NaN there)
+        if (lane < kXhmGradSlots) partial[((b * kXhmGradRows + 1 + k) * chunks + c) * kXhmGradSlots + lane] = 0.0;
+        return;
+    }
+    // lane j: the totals of accumulators j and 64 + j
+    float hi[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) hi[j] = 64 + j < kHmAcc ? acc[64 + j] : 0.0f;
+    const double lo_total = (double)reduce_scatter64(acc, lane);
+    const double hi_total = (double)reduce_scatter64(hi, lane);
+    const float* t = dmodes + (b * kHmModes + k) * kHmSlots * 4;
+    double d[4];
+    {
+        const float4 t0 = *reinterpret_cast<const float4*>(t + 4 * kHmAccSlot[lane]);
+        const float tl[4] = {t0.x, t0.y, t0.z, t0.w};
+        for (int j = 0; j < 4; ++j) d[j] = tl[j] == 0.0f ? 0.0 : lo_total * (double)tl[j];
+    }
+    if (64 + lane < kHmAcc) {
+        const float4 t1 = *reinterpret_cast<const float4*>(t + 4 * kHmAccSlot[64 + lane]);
+        const float th[4] = {t1.x, t1.y, t1.z, t1.w};
+        for (int j = 0; j < 4; ++j) d[j] += th[j] == 0.0f ? 0.0 : hi_total * (double)th[j];
+    }
+    xhm_grad_store(d, v4, v5, v6, lane, partial + ((b * kXhmGradRows + 1 + k) * chunks + c) * kXhmGradSlots);
+}
+
+// one wave per (binary, span): the (2,2)'s adjoint (the IMRPhenomXAS
+// record's, with the mode's own cotangent plus the one the (3,2) passes)
+__global__ __launch_bounds__(64) void xhm_22_adjoint_kernel(
+    const float* __restrict__ rec22, const double* __restrict__ jac22, const float* __restrict__ head,
+    const float* __restrict__ dhead, int64_t nb, const PowerRow<float>* __restrict__ table, int64_t n,
+    int64_t chunks, int64_t span, const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross,
+    const float2* __restrict__ gh22, double* __restrict__ partial) {
     const int64_t b = blockIdx.x / chunks;
     const int64_t c = blockIdx.x % chunks;
     if (b >= nb) return;
-    const float* r22 = rec22 + b * kSlots;
-    const float* m = modes + b * kHmModes * kHmSlots;
-    const float* hd = head + b * kHeadSlots;
-    double v[kXhmGradSlots];
-    for (int j = 0; j < kXhmGradSlots; ++j) v[j] = 0.0;
-    const int64_t i = c * kXhmBlock + threadIdx.x;
-    if (i < n) {
-        const float f = freqs[i];
-        float2 cp = in_plus[b * n + i], cc = in_cross[b * n + i];
-        if (Mode == kLoss) {
-            // the residual from the forward pass's values
-            float h[5][2];
-            xhm_modes_split(r22, m, f, h);
-            float2 hp, hc;
-            xhm_sum_f32(hd, h, hp, hc);
-            const float w = weights[i];
-            const float rp_re = hp.x - cp.x, rp_im = hp.y - cp.y, rx_re = hc.x - cc.x, rx_im = hc.y - cc.y;
-            v[7] = (double)(w * (rp_re * rp_re + rp_im * rp_im + rx_re * rx_re + rx_im * rx_im));
-            const float two_w = 2.0f * w;
-            cp = make_float2(two_w * rp_re, two_w * rp_im);
-            cc = make_float2(two_w * rx_re, two_w * rx_im);
-        }
-        xhm_bin_gradient(r22, t22 + b * 4 * kSlots, m, tmodes + b * 4 * kHmModes * kHmSlots, hd,
-                         dhead + b * kHeadSlots, f, cp, cc, v);
+    const int lane = threadIdx.x;
+    float rec[kSlots];
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) rec[k] = rec22[b * kSlots + k];
+    float acc[kSlots];
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
+    const float* fac = head + b * kHeadSlots + kHeadFactors;
+    const float* dfac = dhead + b * kHeadSlots + kHeadFactors;
+    const int64_t i0 = c * span;
+    const int64_t i1 = i0 + span < n ? i0 + span : n;
+    float v4 = 0.0f, v5 = 0.0f, v6 = 0.0f;
+    for (int64_t i = i0 + lane; i < i1; i += 64) {
+        const PowerRow<float> row = table[i];
+        const BinPowers<float> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
+        const float2 cp = cot_plus[b * n + i], cc = cot_cross[b * n + i];
+        float g_re, g_im;
+        mode_cotangent(fac, cp, cc, g_re, g_im);
+        const float2 extra = gh22[b * n + i];
+        float amp, s, co;
+        adjoint_point_k(rec, p, 1.0f, 0.0f, g_re + extra.x, g_im + extra.y, 0.0f, 0.0f, acc, amp, s, co);
+        const float h_re = amp * co, h_im = amp * s;
+        v4 += g_re * h_re + g_im * h_im;
+        v6 += 2.0f * (g_im * h_re - g_re * h_im);
+        v5 += inclination_term(dfac, cp, cc, h_re, h_im);
     }
-    for (int j = 0; j < kXhmGradSlots; ++j) {
-        for (int offset = 32; offset > 0; offset >>= 1) v[j] += __shfl_xor(v[j], offset, 64);
-        if ((threadIdx.x & 63) == 0) wave_sums[j][threadIdx.x >> 6] = v[j];
-    }
-    __syncthreads();
-    if (threadIdx.x < kXhmGradSlots) {
-        const int j = threadIdx.x;
-        double total = 0.0;
-        for (int w = 0; w < kXhmBlock / 64; ++w) total += wave_sums[j][w];
-        partial[(b * chunks + c) * kXhmGradSlots + j] = total;
+    double total = (double)reduce_scatter64(acc, lane);
+    // slots without a derivative here: the factors (the mode's are the
+    // head's), T0's and A's low parts and the FP64 copies 60..63
+    if (lane == kKp || lane == kKc || lane == kT0Lo || lane == kAnchorLo || lane >= 60) total = 0.0;
+    const double* jrow = jac22 + (b * kSlots + lane) * 8;
+    double d[4];
+    for (int j = 0; j < 4; ++j) d[j] = jrow[j] == 0.0 ? 0.0 : total * jrow[j];
+    xhm_grad_store(d, v4, v5, v6, lane, partial + (b * kXhmGradRows * chunks + c) * kXhmGradSlots);
+}
+
+// each binary's gradient (and loss) from the waves' sums, in a fixed order:
+// d_L's derivative -Re conj(G) h / d_L, f_ref's half phi_ref's times the
+// (2,2)'s phase turn per unit f_ref (its record's inspiral constant's
+// derivative less its anchor's: zero where f_ref is zero or mass-scaled)
+__global__ __launch_bounds__(64) void xhm_adjoint_finish_kernel(const double* __restrict__ params,
+                                                                const double* __restrict__ jac22, int64_t nb,
+                                                                const double* __restrict__ partial, int64_t chunks,
+                                                                const double* __restrict__ loss_partial,
+                                                                int64_t loss_chunks, double* __restrict__ loss,
+                                                                double* __restrict__ grads) {
+    const int64_t b = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (b >= nb) return;
+    double s[kXhmGradSlots];
+    for (int j = 0; j < kXhmGradSlots; ++j) s[j] = 0.0;
+    for (int64_t rc = 0; rc < kXhmGradRows * chunks; ++rc)
+        for (int j = 0; j < kXhmGradSlots; ++j) s[j] += partial[(b * kXhmGradRows * chunks + rc) * kXhmGradSlots + j];
+    double* g = grads + b * 8;
+    for (int j = 0; j < 4; ++j) g[j] = s[j];
+    g[4] = -s[4] / params[b * kParams + kDistance];
+    g[5] = s[5];
+    g[6] = s[6];
+    const double turn = jac22[(b * kSlots + kInsConst) * 8 + kFRef] - jac22[(b * kSlots + kAnchor) * 8 + kFRef];
+    g[7] = 0.5 * s[6] * turn;
+    if (loss) {
+        double l = 0.0;
+        for (int64_t c = 0; c < loss_chunks; ++c) l += loss_partial[b * loss_chunks + c];
+        loss[b] = l;
     }
 }
 
@@ -1516,7 +1750,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 19; }
+int wfx_abi_version(void) { return 20; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -1821,28 +2055,67 @@ int wfx_xhm_setup_jvp(const double* params, int64_t n_binaries, const double* f_
     return check_launch("wfx_xhm_setup_jvp");
 }
 
-int wfx_xhm_adjoint(const void* records22, const void* t22, const void* head, const void* dhead, const void* modes,
-                    const void* dmodes, int64_t n_binaries, const void* freqs, int64_t n_freqs, int precision,
-                    const void* in_plus, const void* in_cross, const void* weights, double* partial, void* stream) {
+int64_t wfx_xhm_adjoint_workspace(int64_t n_binaries, int64_t n_freqs, int loss) {
+    if (n_binaries <= 0 || n_freqs <= 0) return 0;
+    return xhm_grad_plan(n_binaries, n_freqs, loss != 0).bytes;
+}
+
+int wfx_xhm_adjoint(const double* params, const void* records22, const double* jacobian22, const void* head,
+                    const void* dhead, const void* modes, const void* dmodes, int64_t n_binaries, const void* freqs,
+                    int64_t n_freqs, int precision, const void* in_plus, const void* in_cross, const void* weights,
+                    void* workspace, int64_t workspace_bytes, double* loss, double* grads, void* stream) {
     if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xhm_adjoint: negative size");
-    if (n_binaries == 0 || n_freqs == 0) return 0;
-    if (!records22 || !t22 || !head || !dhead || !modes || !dmodes || !freqs || !in_plus || !in_cross || !partial)
+    if (n_binaries == 0) return 0;
+    if (!params || !records22 || !jacobian22 || !head || !dhead || !modes || !dmodes || !grads)
         return fail("wfx_xhm_adjoint: null pointer");
     if (precision != kPrecSplit) return fail("wfx_xhm_adjoint: precision must be 2 (split)");
-    const int64_t chunks = (n_freqs + kXhmBlock - 1) / kXhmBlock;
-    const int blocks = grid_blocks(n_binaries * chunks, "wfx_xhm_adjoint: too large");
-    if (blocks < 0) return -1;
+    if (weights && !loss) return fail("wfx_xhm_adjoint: the loss needs its output");
     hipStream_t s = (hipStream_t)stream;
-    if (weights)
-        xhm_adjoint_split_kernel<kLoss><<<blocks, kXhmBlock, 0, s>>>(
-            (const float*)records22, (const float*)t22, (const float*)head, (const float*)dhead, (const float*)modes,
-            (const float*)dmodes, n_binaries, (const float*)freqs, n_freqs, chunks, (const float2*)in_plus,
-            (const float2*)in_cross, (const float*)weights, partial);
-    else
-        xhm_adjoint_split_kernel<kFromCotangents><<<blocks, kXhmBlock, 0, s>>>(
-            (const float*)records22, (const float*)t22, (const float*)head, (const float*)dhead, (const float*)modes,
-            (const float*)dmodes, n_binaries, (const float*)freqs, n_freqs, chunks, (const float2*)in_plus,
-            (const float2*)in_cross, nullptr, partial);
+    if (n_freqs == 0) {
+        // no frequency: a zero gradient and loss (a kernel, not a memset
+        // node, should the caller capture the call in a graph)
+        const int blocks = grid_blocks((n_binaries + 63) / 64, "wfx_xhm_adjoint: grid too large");
+        if (blocks < 0) return -1;
+        xhm_adjoint_finish_kernel<<<blocks, 64, 0, s>>>(params, jacobian22, n_binaries, nullptr, 0, nullptr, 0,
+                                                        weights ? loss : nullptr, grads);
+        return check_launch("wfx_xhm_adjoint");
+    }
+    if (!freqs || !in_plus || !in_cross) return fail("wfx_xhm_adjoint: null pointer");
+    const XhmGradPlan g = xhm_grad_plan(n_binaries, n_freqs, weights != nullptr);
+    if (g.bytes < 0) return fail("wfx_xhm_adjoint: sizes too large");
+    if (!workspace || workspace_bytes < g.bytes || (uintptr_t)workspace % 256)
+        return fail("wfx_xhm_adjoint: the workspace must be 256-aligned and hold wfx_xhm_adjoint_workspace() bytes");
+    char* ws = (char*)workspace;
+    PowerRow<float>* table = (PowerRow<float>*)(ws + g.table);
+    float2* gh22 = (float2*)(ws + g.gh22);
+    double* partial = (double*)(ws + g.partial);
+    const float2* cot_plus = (const float2*)in_plus;
+    const float2* cot_cross = (const float2*)in_cross;
+    const int table_blocks = grid_blocks((n_freqs + kBlock - 1) / kBlock, "wfx_xhm_adjoint: grid too large");
+    const int hm_blocks = grid_blocks(n_binaries * g.chunks * kHmModes, "wfx_xhm_adjoint: grid too large");
+    const int blocks22 = grid_blocks(n_binaries * g.chunks, "wfx_xhm_adjoint: grid too large");
+    const int loss_blocks = grid_blocks(n_binaries * g.loss_chunks, "wfx_xhm_adjoint: grid too large");
+    if (table_blocks < 0 || hm_blocks < 0 || blocks22 < 0 || loss_blocks < 0) return -1;
+    power_table_kernel<float, float><<<table_blocks, kBlock, 0, s>>>((const float*)freqs, n_freqs, table);
+    if (weights) {
+        float2* res_plus = (float2*)(ws + g.cot_plus);
+        float2* res_cross = (float2*)(ws + g.cot_cross);
+        xhm_residual_split_kernel<<<loss_blocks, kXhmBlock, 0, s>>>(
+            (const float*)records22, (const float*)head, (const float*)modes, n_binaries, (const float*)freqs,
+            n_freqs, g.loss_chunks, cot_plus, cot_cross, (const float*)weights, res_plus, res_cross,
+            (double*)(ws + g.loss_partial));
+        cot_plus = res_plus;
+        cot_cross = res_cross;
+    }
+    xhm_hm_adjoint_kernel<<<hm_blocks, 64, 0, s>>>((const float*)records22, (const float*)head, (const float*)dhead,
+                                                   (const float*)modes, (const float*)dmodes, n_binaries, table,
+                                                   n_freqs, g.chunks, g.span, cot_plus, cot_cross, gh22, partial);
+    xhm_22_adjoint_kernel<<<blocks22, 64, 0, s>>>((const float*)records22, jacobian22, (const float*)head,
+                                                  (const float*)dhead, n_binaries, table, n_freqs, g.chunks, g.span,
+                                                  cot_plus, cot_cross, gh22, partial);
+    xhm_adjoint_finish_kernel<<<(int)((n_binaries + 63) / 64), 64, 0, s>>>(
+        params, jacobian22, n_binaries, partial, g.chunks, weights ? (const double*)(ws + g.loss_partial) : nullptr,
+        g.loss_chunks, weights ? loss : nullptr, grads);
     return check_launch("wfx_xhm_adjoint");
 }
 

@@ -1,345 +1,448 @@
-// The gradient of IMRPhenomXHM per (binary, frequency), split arithmetic:
-// the five modes and their derivatives in the four traced parameters (m1,
-// m2, chi1z, chi2z), forward mode through the split evaluation of
-// xhm_point.h (the same FP32 operations in the same order for the values,
-// so that a fused loss sees the forward pass's bits), from the records and
-// their derivatives of the dual-number set-up (wfx_xhm_setup_jvp,
-// wfx_xas_setup_jvp). The four parameters that enter in closed form (d_L,
-// the inclination, phi_ref, f_ref) are projections the kernel also takes.
+// The adjoint of IMRPhenomXHM's split evaluation (xhm_point.h) with respect
+// to a higher mode's record: given the cotangent g = dL/dRe h + i dL/dIm h
+// of the mode h_{l,-m} at one bin, add dL/d(record slot) of that bin to the
+// mode's accumulators (HmAcc: the slots that carry a derivative, numbered
+// densely so that a wave keeps them in registers over its bins). The
+// forward values are recomputed with the forward kernel's operations; the
+// adjoint follows each region's formula back to its coefficients and to the
+// powers of M f (through the binary's M slots), the (3,2)'s mixing back to
+// the (2,2) mode it recycles (returned as that mode's cotangent). The
+// region bounds carry no derivative, as the reference's jnp.where
+// selections (xhm/reference.py:620-643).
 //
-// The reference's GPU port differentiates the same way in part (the (2,2)
-// and the factors in forward mode, src/waveformx/models/xhm/mlx/_kernels.py:
-// 300-441); here every mode is in forward mode with four directions, which
-// needs no per-slot accumulators.
+// The reference differentiates this evaluation with jax's reverse mode
+// through its kernels (src/waveformx/torch.py:108-119; the MLX port's
+// hand-written cotangent_product, src/waveformx/models/xhm/mlx/_kernels.py:
+// 300-441); here each mode's bins are reduced into its slots' totals per
+// wave, and the totals contracted once with the records' derivatives.
 #pragma once
 
+#include "xas_adjoint.h"
 #include "xhm_point.h"
 
 namespace wfx {
 
-// ------------------------------------------------------ tangent scalar
-
-// an FP32 value with its derivatives in four directions
-struct DF {
-    float v;
-    float d[4];
+// the accumulators of a higher mode's adjoint, one per record slot with a
+// derivative (kHmAccSlot maps them to the slots)
+enum HmAcc : int {
+    aAmp0 = 0, aMsec = 1, aFRing = 2, aFDamp = 3, aFalloffAt = 4,
+    aPnRe = 5,          // 7
+    aPnIm = 12,         // 7
+    aPnGlobal = 19, aAmpNorm = 20, aPnDominant = 21,
+    aIns = 22,          // 3
+    aRd = 25,           // 5
+    aRdAux = 30,        // 4
+    aInterC = 34, aInterH = 35,
+    aInter = 36,        // 8
+    aPhi = 44,          // 12: phi_1 .. phi_12
+    aPhiL5 = 56, aPhiL6 = 57, aPhiL8 = 58,
+    aAlpha0 = 59, aAlpha2 = 60, aAlphaL = 61, aC0 = 62, aC1 = 63, aC2 = 64, aC3 = 65, aC4 = 66, aCL = 67,
+    aC1Insp = 68, aC1Rd = 69, aCRd = 70, aDeltaPhi = 71,
+    aSph = 72,          // 5
+    aMix = 77,          // 4
+    aT0 = 81, aKIns = 82, aLogM = 83, aCbrtM = 84, aInvM = 85,
+    kHmAcc = 86
 };
-__device__ inline DF df_const(float x) { return DF{x, {0.0f, 0.0f, 0.0f, 0.0f}}; }
-__device__ inline DF operator+(const DF& a, const DF& b) {
-    return DF{a.v + b.v, {a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2], a.d[3] + b.d[3]}};
-}
-__device__ inline DF operator-(const DF& a, const DF& b) {
-    return DF{a.v - b.v, {a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2], a.d[3] - b.d[3]}};
-}
-__device__ inline DF operator-(const DF& a) { return DF{-a.v, {-a.d[0], -a.d[1], -a.d[2], -a.d[3]}}; }
-__device__ inline DF operator*(const DF& a, const DF& b) {
-    DF r;
-    r.v = a.v * b.v;
-    for (int i = 0; i < 4; ++i) r.d[i] = fmaf(a.d[i], b.v, a.v * b.d[i]);
-    return r;
-}
-__device__ inline DF operator*(const DF& a, float b) { return DF{a.v * b, {a.d[0] * b, a.d[1] * b, a.d[2] * b, a.d[3] * b}}; }
-__device__ inline DF operator*(float a, const DF& b) { return b * a; }
-__device__ inline DF operator+(const DF& a, float b) { return DF{a.v + b, {a.d[0], a.d[1], a.d[2], a.d[3]}}; }
-__device__ inline DF operator-(const DF& a, float b) { return DF{a.v - b, {a.d[0], a.d[1], a.d[2], a.d[3]}}; }
-__device__ inline DF operator-(float a, const DF& b) { return DF{a - b.v, {-b.d[0], -b.d[1], -b.d[2], -b.d[3]}}; }
-// fmaf(a, b, c) on the values, the product rule on the derivatives
-__device__ inline DF dfma(const DF& a, const DF& b, const DF& c) {
-    DF r;
-    r.v = fmaf(a.v, b.v, c.v);
-    for (int i = 0; i < 4; ++i) r.d[i] = fmaf(a.d[i], b.v, fmaf(a.v, b.d[i], c.d[i]));
-    return r;
-}
-__device__ inline DF dfma(const DF& a, float b, const DF& c) {
-    DF r;
-    r.v = fmaf(a.v, b, c.v);
-    for (int i = 0; i < 4; ++i) r.d[i] = fmaf(a.d[i], b, c.d[i]);
-    return r;
-}
-// the value's function f and its derivative f1 on the derivatives
-__device__ inline DF df_chain(float f, const DF& x, float f1) {
-    return DF{f, {x.d[0] * f1, x.d[1] * f1, x.d[2] * f1, x.d[3] * f1}};
-}
-// a / b as fast_div (the hardware reciprocal)
-__device__ inline DF ddiv(const DF& a, const DF& b) {
-    const float r = __builtin_amdgcn_rcpf(b.v);
-    const float q = a.v * r;
-    DF out;
-    out.v = q;
-    for (int i = 0; i < 4; ++i) out.d[i] = (a.d[i] - q * b.d[i]) * r;
-    return out;
-}
-__device__ inline DF dexp(const DF& x) {
-    const float e = __expf(x.v);
-    return df_chain(e, x, e);
-}
-__device__ inline DF dsqrt(const DF& x) {
-    const float s = __builtin_sqrtf(x.v);
-    return df_chain(s, x, s > 0.0f ? 0.5f * __builtin_amdgcn_rcpf(s) : 0.0f);
-}
-__device__ inline DF datan(const DF& x) {
-    return df_chain(fast_atan(x.v), x, __builtin_amdgcn_rcpf(fmaf(x.v, x.v, 1.0f)));
-}
-// the derivative (x dy - y dx) / (x^2 + y^2) scaled by max(|x|, |y|) first,
-// so that a mode whose amplitude is near FP32's smallest normal (far in
-// its ringdown) keeps a finite phase derivative; at the origin (the
-// amplitude zero: its phase moves nothing) it is zero
-__device__ inline DF datan2(const DF& y, const DF& x) {
-    const float s = fmaxf(fabsf(x.v), fabsf(y.v));
-    const float inv_s = s > 0.0f ? __builtin_amdgcn_rcpf(s) : 0.0f;
-    const float xs = x.v * inv_s, ys = y.v * inv_s;
-    const float r = s > 0.0f ? __builtin_amdgcn_rcpf(s * fmaf(xs, xs, ys * ys)) : 0.0f;
-    DF out;
-    out.v = atan2f(y.v, x.v);
-    for (int i = 0; i < 4; ++i) out.d[i] = (xs * y.d[i] - ys * x.d[i]) * r;
-    return out;
-}
-// (sin, cos) of an angle whose value is reduced first (the reduction is a
-// constant shift: no derivative)
-__device__ inline void dsincos(const DF& x, float reduced, DF& sn, DF& cs) {
-    float s, c;
-    __sincosf(reduced, &s, &c);
-    sn = df_chain(s, x, c);
-    cs = df_chain(c, x, -s);
-}
 
-// slot s of a record r with its derivatives t[4 s + k], k = 0..3 (a slot's
-// four derivatives are contiguous: one 16-byte load)
-__device__ inline DF ld(const float* __restrict__ r, const float* __restrict__ t, int s) {
-    const float4 d = *reinterpret_cast<const float4*>(t + 4 * s);
-    return DF{r[s], {d.x, d.y, d.z, d.w}};
-}
-
-// ----------------------------------------------------------- the (2,2)
-
-// amp_phase_f32<kLeadF64> with derivatives: c the (2,2)'s split record, t
-// its slots' derivatives (kSlots x 4: the post-inspiral constants' include
-// -A's, T0's and A's FP64 copies' sit in the first word)
-__device__ inline void amp_phase22_t(const float* __restrict__ c, const float* __restrict__ t, const PowF32& p,
-                                     DF& amp, DF& phi) {
-    auto C = [&](int s) { return ld(c, t, s); };
-    const float f = p.f;
-    const float tt = p.t;
-    const float m1 = p.m1;
-    const bool past = f >= c[kFPhaseIn];
-    const bool ringdown = f > c[kFPhaseIm];
-    DF rest;
-    if (!past) {
-        DF poly = C(kInsP2);
-        poly = dfma(poly, tt, C(kInsP53));
-        poly = dfma(poly, tt, C(kInsP43));
-        poly = dfma(poly, tt, C(kInsP1));
-        poly = dfma(poly, tt, C(kInsP23));
-        poly = dfma(poly, tt, C(kInsP13));
-        poly = dfma(poly, tt, C(kInsConst));
-        poly = dfma(poly, tt, C(kInsM13));
-        poly = dfma(poly, tt, C(kInsM23));
-        poly = dfma(poly, tt, C(kInsM1));
-        const DF lg = dfma(dfma(C(kInsP1L), tt * tt, C(kInsP13L)), tt, C(kInsLog));
-        rest = dfma(poly, m1, lg * p.lf);
-    } else if (!ringdown) {
-        const DF inv = dfma(dfma(C(kIntM3), m1, C(kIntM2)), m1, C(kIntM1));
-        rest = dfma(inv, m1, C(kIntConst));
-        rest = dfma(C(kIntP1), f, rest);
-        rest = dfma(C(kIntLog), p.lf, rest);
-        rest = dfma(C(kAtanA1), datan(C(kAtanW1) * (df_const(f) - C(kShift))), rest);
-    } else {
-        const DF inv = dfma(C(kRdM3), m1 * m1, C(kRdM1));
-        rest = dfma(inv, m1, C(kRdConst));
-        rest = dfma(C(kRdP1), f, rest);
-        rest = dfma(C(kRdP23), tt * tt, rest);
-        rest = dfma(C(kAtanA2), datan(C(kAtanW2) * (df_const(f) - C(kShift))), rest);
-    }
-    if (past) {
-        phi = rest;
-        phi.v = reduce_f32(rest.v);  // the constants hold -A (fold_anchor)
-    } else {
-        const double full = fma(record_f64(c, kT0F64), p.m53, (double)rest.v - record_f64(c, kAnchorF64));
-        const double turns = rint(full * kInvTwoPi);
-        phi.v = (float)fma(-turns, kTwoPi, full);
-        const float m53 = (float)p.m53;
-        for (int i = 0; i < 4; ++i)
-            phi.d[i] = fmaf(t[4 * kT0F64 + i], m53, rest.d[i] - t[4 * kAnchorF64 + i]);
-    }
-    if (f > c[kFAmpRd]) {
-        const DF u = C(kAmpRdS) * (df_const(f) - C(kShift));
-        amp = ddiv(C(kAmpRdN) * p.m76 * dexp(-C(kAmpRdGr) * u), dfma(u, u, C(kAmpRdG)));
-    } else if (f >= c[kFAmpIn]) {
-        DF den = C(kAmpIntD + 4);
-        den = dfma(den, f, C(kAmpIntD + 3));
-        den = dfma(den, f, C(kAmpIntD + 2));
-        den = dfma(den, f, C(kAmpIntD + 1));
-        den = dfma(den, f, C(kAmpIntD + 0));
-        amp = ddiv(C(kAmpIntN), den);
-    } else {
-        DF poly = C(kAmpIns + 9);
-        for (int k = 8; k >= 0; --k) poly = dfma(poly, tt, C(kAmpIns + k));
-        amp = p.m76 * poly;
-    }
-    if (!(f <= c[kFCut])) amp = df_const(0.0f);
-}
-
-// --------------------------------------------------------- higher modes
-
-// the powers of M f from those of f and the binary's M (their derivatives
-// through M's slots)
-struct MfPowDF {
-    DF mf, t, m1, lg, m76;
+WFX_CONST int kHmAccSlot[kHmAcc] = {
+    kHmAmp0, kHmMsec, kHmFRing, kHmFDamp, kHmFalloffAt,
+    kHmPnRe, kHmPnRe + 1, kHmPnRe + 2, kHmPnRe + 3, kHmPnRe + 4, kHmPnRe + 5, kHmPnRe + 6,
+    kHmPnIm, kHmPnIm + 1, kHmPnIm + 2, kHmPnIm + 3, kHmPnIm + 4, kHmPnIm + 5, kHmPnIm + 6,
+    kHmPnGlobal, kHmAmpNorm, kHmPnDominant,
+    kHmIns, kHmIns + 1, kHmIns + 2,
+    kHmRd, kHmRd + 1, kHmRd + 2, kHmRd + 3, kHmRd + 4,
+    kHmRdAux, kHmRdAux + 1, kHmRdAux + 2, kHmRdAux + 3,
+    kHmInterC, kHmInterH,
+    kHmInter, kHmInter + 1, kHmInter + 2, kHmInter + 3, kHmInter + 4, kHmInter + 5, kHmInter + 6, kHmInter + 7,
+    kHmPhi + 1, kHmPhi + 2, kHmPhi + 3, kHmPhi + 4, kHmPhi + 5, kHmPhi + 6, kHmPhi + 7, kHmPhi + 8, kHmPhi + 9,
+    kHmPhi + 10, kHmPhi + 11, kHmPhi + 12,
+    kHmPhiL + 5, kHmPhiL + 6, kHmPhiL + 8,
+    kHmAlpha0, kHmAlpha2, kHmAlphaL, kHmC0, kHmC1, kHmC2, kHmC3, kHmC4, kHmCL,
+    kHmC1Insp, kHmC1Rd, kHmCRd, kHmDeltaPhi,
+    kHmSph, kHmSph + 1, kHmSph + 2, kHmSph + 3, kHmSph + 4,
+    kHmMix, kHmMix + 1, kHmMix + 2, kHmMix + 3,
+    kHmT0F64, kHmKInsF64, kHmLogM, kHmCbrtM, kHmInvM,
 };
-__device__ inline MfPowDF mf_pow_t(const float* __restrict__ r, const float* __restrict__ t, const PowF32& p) {
-    MfPowDF q;
-    q.mf = ld(r, t, kHmMsec) * p.f;
-    q.t = ld(r, t, kHmCbrtM) * p.t;
-    q.m1 = ld(r, t, kHmInvM) * p.m1;
-    q.lg = ld(r, t, kHmLogM) + p.lf;
-    // m1 t^(-1/2), value as mf_pow_f32's
-    const float rs = __builtin_amdgcn_rsqf(q.t.v);
-    q.m76.v = q.m1.v * rs;
-    const float drs = -0.5f * rs * rs * rs;
-    for (int i = 0; i < 4; ++i) q.m76.d[i] = fmaf(q.m1.d[i], rs, q.m1.v * drs * q.t.d[i]);
-    return q;
+
+// the higher mode (index into the four records) whose ringdown recycles the
+// (2,2): the (3,2), the only record with kHmMixing set
+constexpr int kHmMixMode = 2;
+
+// the cotangents of one bin's powers of M f: of M f, t = (M f)^(1/3),
+// (M f)^-1, log(M f) and (M f)^(-7/6)
+struct MfGrad {
+    float mf, t, m1, lg, m76;
+};
+
+// z = (M f - f_ring) / f_damp's adjoint (the arctangents of the
+// intermediate and ringdown phases), for its cotangent gz
+__device__ __forceinline__ void atan_z_adjoint(float gz, float z, float inv_damp, float* acc, MfGrad& g) {
+    g.mf += gz * inv_damp;
+    acc[aFRing] -= gz * inv_damp;
+    acc[aFDamp] -= gz * z * inv_damp;
 }
 
-// hm_amp_phase_split with derivatives (the same operations on the values)
-__device__ inline void hm_amp_phase_split_t(const float* __restrict__ r, const float* __restrict__ t,
-                                            const MfPowDF& q, double m53, const DF& w_re, const DF& w_im, DF& a,
-                                            DF& phi) {
-    auto R = [&](int s) { return ld(r, t, s); };
+// a Lorentzian ringdown amplitude rd0 / (exp(rd1 d) (d^2 + rd2)), d = M f -
+// f_ring (its value a), for its cotangent ga
+__device__ __forceinline__ void lorentzian_adjoint(const float* __restrict__ r, float mf, float a, float ga,
+                                                   float* acc, MfGrad& g) {
+    const float dfr = mf - r[kHmFRing];
+    const float e = __expf(r[kHmRd + 1] * dfr);
+    const float q = fmaf(dfr, dfr, r[kHmRd + 2]);
+    const float inv_q = __builtin_amdgcn_rcpf(q);
+    acc[aRd] += ga * __builtin_amdgcn_rcpf(e * q);
+    acc[aRd + 1] -= ga * a * dfr;
+    acc[aRd + 2] -= ga * a * inv_q;
+    const float gd = -ga * a * fmaf(2.0f * dfr, inv_q, r[kHmRd + 1]);
+    g.mf += gd;
+    acc[aFRing] -= gd;
+}
+
+// the fall-off rd3 exp(-rd4 (M f - f_at)) (its value a), cotangent ga
+__device__ __forceinline__ void falloff_adjoint(const float* __restrict__ r, float mf, float a, float ga, float* acc,
+                                                MfGrad& g) {
+    const float x = mf - r[kHmFalloffAt];
+    acc[aRd + 3] += ga * __expf(-r[kHmRd + 4] * x);
+    acc[aRd + 4] -= ga * a * x;
+    const float gx = -ga * a * r[kHmRd + 4];
+    g.mf += gx;
+    acc[aFalloffAt] -= gx;
+}
+
+// One bin of higher mode r (its kHmOn set) at the powers p of f: the mode's
+// value h = amp0 a e^{i phi} (the forward's operations), dL/d(slot) added to
+// acc for the cotangent (g_re, g_im) of h, and the cotangent (gw_re, gw_im)
+// of the mixing's input w = h_22 / |amp0| (zero where it does not mix).
+// Inlined, so that every accumulator index is a constant and acc stays in
+// registers; an empty volatile statement ends each region's branch, so that
+// their last updates are not merged into one store through a selected
+// accumulator (which would move acc to scratch memory).
+__device__ __forceinline__ void hm_adjoint_split(const float* __restrict__ r, const BinPowers<float>& p, float w_re,
+                                                 float w_im, float g_re, float g_im, float* acc, float& h_re,
+                                                 float& h_im, float& gw_re, float& gw_im) {
+    // the powers of M f (mf_pow_f32)
+    const float mf = r[kHmMsec] * p.f;
+    const float t = p.t * r[kHmCbrtM];
+    const float m1 = p.m1 * r[kHmInvM];
+    const float lg = p.lf + r[kHmLogM];
+    const float rs = __builtin_amdgcn_rsqf(t);
+    const float m76 = m1 * rs;
     const bool mixing = r[kHmMixing] != 0.0f;
-    const DF mf = q.mf, tt = q.t, m1 = q.m1;
-    const DF f_ring = R(kHmFRing), f_damp = R(kHmFDamp);
-    DF rot_re = df_const(0.0f), rot_im = df_const(0.0f);
-    if (mixing && (mf.v >= r[kHmFAmpIm] || mf.v >= r[kHmFPhaseIm])) {
-        DF amplm;
-        if (mf.v < r[kHmFRdAux]) {
-            amplm = dfma(dfma(dfma(R(kHmRdAux + 3), mf, R(kHmRdAux + 2)), mf, R(kHmRdAux + 1)), mf, R(kHmRdAux));
-        } else if (mf.v >= r[kHmFalloff]) {
-            amplm = R(kHmRd + 3) * dexp(-R(kHmRd + 4) * (mf - R(kHmFalloffAt)));
-        } else {
-            const DF dfr = mf - f_ring;
-            amplm = ddiv(R(kHmRd), dexp(R(kHmRd + 1) * dfr) * dfma(dfr, dfr, R(kHmRd + 2)));
-        }
-        DF philm = dfma(R(kHmSph), mf, R(kHmSph + 4));
-        philm = dfma(-R(kHmSph + 2), m1, philm);
-        philm = dfma(-((1.0f / 3.0f) * R(kHmSph + 3)), m1 * m1 * m1, philm);
-        philm = dfma(R(kHmSph + 1), datan(ddiv(mf - f_ring, f_damp)), philm);
-        DF sn, cs;
-        dsincos(philm, reduce_f32(philm.v), sn, cs);
-        const DF m2r = R(kHmMix), m2i = R(kHmMix + 1), m3r = R(kHmMix + 2), m3i = R(kHmMix + 3);
-        const DF b_re = amplm * m3r, b_im = -(m3i * amplm);
-        rot_re = dfma(w_re, m2r, w_im * m2i) + dfma(b_re, cs, -b_im * sn);
-        rot_im = dfma(w_im, m2r, -w_re * m2i) + dfma(b_re, sn, b_im * cs);
+    const float f_ring = r[kHmFRing], f_damp = r[kHmFDamp];
+    // the mixing's rotation conj(-a322) w + conj(-a323) A_s e^{i phi_s}
+    const bool rotating = mixing && (mf >= r[kHmFAmpIm] || mf >= r[kHmFPhaseIm]);
+    // atan((M f - f_ring) / f_damp), of the post-inspiral phases and the
+    // rotation only
+    float z = 0.0f, atan_z = 0.0f, inv_damp = 0.0f, datan_z = 0.0f;
+    if (rotating || !(mf < r[kHmFPhaseIn])) {
+        z = fast_div(mf - f_ring, f_damp);
+        atan_z = fast_atan(z);
+        inv_damp = __builtin_amdgcn_rcpf(f_damp);
+        datan_z = __builtin_amdgcn_rcpf(fmaf(z, z, 1.0f));
     }
-    if (mf.v < r[kHmFAmpIn]) {
-        DF re = R(kHmPnRe + 6), im = R(kHmPnIm + 6);
+    float rot_re = 0.0f, rot_im = 0.0f, amplm = 0.0f, sn_s = 0.0f, cs_s = 0.0f;
+    if (rotating) {
+        if (mf < r[kHmFRdAux]) {
+            const float* x = r + kHmRdAux;
+            amplm = fmaf(fmaf(fmaf(x[3], mf, x[2]), mf, x[1]), mf, x[0]);
+        } else if (mf >= r[kHmFalloff]) {
+            amplm = r[kHmRd + 3] * __expf(-r[kHmRd + 4] * (mf - r[kHmFalloffAt]));
+        } else {
+            const float dfr = mf - f_ring;
+            amplm = fast_div(r[kHmRd], __expf(r[kHmRd + 1] * dfr) * fmaf(dfr, dfr, r[kHmRd + 2]));
+        }
+        const float* sp = r + kHmSph;
+        float philm = fmaf(sp[0], mf, sp[4]);
+        philm = fmaf(-sp[2], m1, philm);
+        philm = fmaf(-(1.0f / 3.0f) * sp[3], m1 * m1 * m1, philm);
+        philm = fmaf(sp[1], atan_z, philm);
+        __sincosf(reduce_f32(philm), &sn_s, &cs_s);
+        const float m2r = r[kHmMix], m2i = r[kHmMix + 1], m3r = r[kHmMix + 2], m3i = r[kHmMix + 3];
+        const float b_re = amplm * m3r, b_im = -(m3i * amplm);
+        rot_re = fmaf(w_re, m2r, w_im * m2i) + fmaf(b_re, cs_s, -b_im * sn_s);
+        rot_im = fmaf(w_im, m2r, -w_re * m2i) + fmaf(b_re, sn_s, b_im * cs_s);
+    }
+
+    // the amplitude (hm_amp_phase_split's regions)
+    const bool amp_ins = mf < r[kHmFAmpIn];
+    const bool amp_rd = !amp_ins && mf >= r[kHmFAmpIm];
+    const bool falloff = mf >= r[kHmFalloff];
+    const int n_inter = (int)r[kHmNInter];
+    float a;
+    float pn_re = 0.0f, pn_im = 0.0f, mag = 0.0f, pseudo = 0.0f, ins_poly = 0.0f, t7 = 0.0f, u = 0.0f, inter = 0.0f;
+    if (amp_ins) {
+        pn_re = r[kHmPnRe + 6];
+        pn_im = r[kHmPnIm + 6];
         for (int k = 5; k >= 0; --k) {
-            re = dfma(re, tt, R(kHmPnRe + k));
-            im = dfma(im, tt, R(kHmPnIm + k));
+            pn_re = fmaf(pn_re, t, r[kHmPnRe + k]);
+            pn_im = fmaf(pn_im, t, r[kHmPnIm + k]);
         }
-        const DF t2 = tt * tt, t7 = t2 * t2 * t2 * tt;
-        const DF pseudo = t7 * dfma(dfma(R(kHmIns + 2), tt, R(kHmIns + 1)), tt, R(kHmIns));
-        a = q.m76 * dfma(dsqrt(dfma(re, re, im * im)), R(kHmPnGlobal) * R(kHmAmpNorm), pseudo * R(kHmPnDominant));
-    } else if (mf.v >= r[kHmFAmpIm]) {
+        const float t2 = t * t;
+        t7 = t2 * t2 * t2 * t;
+        ins_poly = fmaf(fmaf(r[kHmIns + 2], t, r[kHmIns + 1]), t, r[kHmIns]);
+        pseudo = t7 * ins_poly;
+        mag = __builtin_sqrtf(fmaf(pn_re, pn_re, pn_im * pn_im));
+        a = m76 * fmaf(mag, r[kHmPnGlobal] * r[kHmAmpNorm], pseudo * r[kHmPnDominant]);
+    } else if (amp_rd) {
         if (mixing) {
-            a = dsqrt(dfma(rot_re, rot_re, rot_im * rot_im));
-        } else if (mf.v >= r[kHmFalloff]) {
-            a = R(kHmRd + 3) * dexp(-R(kHmRd + 4) * (mf - R(kHmFalloffAt)));
+            a = __builtin_sqrtf(fmaf(rot_re, rot_re, rot_im * rot_im));
+        } else if (falloff) {
+            a = r[kHmRd + 3] * __expf(-r[kHmRd + 4] * (mf - r[kHmFalloffAt]));
         } else {
-            const DF dfr = mf - f_ring;
-            a = ddiv(R(kHmRd), dexp(R(kHmRd + 1) * dfr) * dfma(dfr, dfr, R(kHmRd + 2)));
+            const float dfr = mf - f_ring;
+            a = fast_div(r[kHmRd], __expf(r[kHmRd + 1] * dfr) * fmaf(dfr, dfr, r[kHmRd + 2]));
         }
     } else {
-        const DF u = (mf - R(kHmInterC)) * R(kHmInterH);
-        const int n = (int)r[kHmNInter];
-        DF result = R(kHmInter + n - 1);
-        for (int j = n - 2; j >= 0; --j) result = dfma(result, u, R(kHmInter + j));
-        a = result * q.m76;
+        u = (mf - r[kHmInterC]) * r[kHmInterH];
+        // the forward's Horner sum from coefficient n - 1 down
+#pragma unroll
+        for (int j = 7; j >= 0; --j)
+            if (j < n_inter) inter = fmaf(inter, u, r[kHmInter + j]);
+        a = inter * m76;
     }
-    if (a.v < 0.0f) a = df_const(1.0e-15f);
-    if (mf.v < r[kHmFPhaseIn]) {
-        DF poly = R(kHmPhi + 12);
-        for (int i = 11; i >= 1; --i) poly = dfma(poly, tt, R(kHmPhi + i));
-        const DF m43 = m1 * ddiv(df_const(1.0f), tt);
-        DF rest = poly * m43;
-        rest = dfma(q.lg, dfma(dfma(R(kHmPhiL + 8), tt * tt, df_const(0.0f)), tt, dfma(R(kHmPhiL + 6), tt, R(kHmPhiL + 5))),
-                    rest);
-        rest = dfma(R(kHmC1Insp), mf, rest);
-        const double full = fma(record_f64(r, kHmT0F64), m53, record_f64(r, kHmKInsF64));
+    const bool clamped = a < 0.0f;
+    if (clamped) a = 1.0e-15f;
+
+    // the phase
+    const bool ph_ins = mf < r[kHmFPhaseIn];
+    const bool ph_rd = !ph_ins && mf >= r[kHmFPhaseIm];
+    float phi, poly = 0.0f, inv_t = 0.0f, m43 = 0.0f, logs = 0.0f;
+    if (ph_ins) {
+        poly = r[kHmPhi + 12];
+        for (int i = 11; i >= 1; --i) poly = fmaf(poly, t, r[kHmPhi + i]);
+        inv_t = fast_div(1.0f, t);
+        m43 = m1 * inv_t;
+        float rest = poly * m43;
+        logs = fmaf(fmaf(r[kHmPhiL + 8], t * t, 0.0f), t, fmaf(r[kHmPhiL + 6], t, r[kHmPhiL + 5]));
+        rest = fmaf(lg, logs, rest);
+        rest = fmaf(r[kHmC1Insp], mf, rest);
+        const double full = fma(record_f64(r, kHmT0F64), p.m53, record_f64(r, kHmKInsF64));
         const double turns = rint(full * kInvTwoPi);
-        phi.v = (float)fma(-turns, kTwoPi, full) + rest.v;
-        const float m53f = (float)m53;
-        for (int i = 0; i < 4; ++i)
-            phi.d[i] = fmaf(t[4 * kHmT0F64 + i], m53f, t[4 * kHmKInsF64 + i] + rest.d[i]);
-    } else if (mf.v >= r[kHmFPhaseIm]) {
-        DF rest;
+        phi = (float)fma(-turns, kTwoPi, full) + rest;
+    } else if (ph_rd) {
+        float rest;
         if (mixing) {
-            rest = dfma(R(kHmC1Rd), mf, datan2(rot_im, rot_re));
+            rest = fmaf(r[kHmC1Rd], mf, atan2f(rot_im, rot_re));
         } else {
-            rest = dfma(R(kHmC1Rd) + R(kHmAlpha0), mf, -(f_ring * f_ring * R(kHmAlpha2)) * m1);
-            rest = dfma(R(kHmAlphaL), datan(ddiv(mf - f_ring, f_damp)), rest);
+            rest = fmaf(r[kHmC1Rd] + r[kHmAlpha0], mf, -(f_ring * f_ring * r[kHmAlpha2]) * m1);
+            rest = fmaf(r[kHmAlphaL], atan_z, rest);
         }
-        phi = rest + R(kHmCRd);
+        phi = rest + r[kHmCRd];
     } else {
-        DF rest = dfma(R(kHmC0), mf, R(kHmC1) * q.lg);
-        rest = dfma(-R(kHmC2), m1, rest);
-        const DF m12 = m1 * m1;
-        rest = dfma(-((1.0f / 3.0f) * R(kHmC4)), m12 * m1, rest);
-        if (mixing) rest = dfma(-(0.5f * R(kHmC3)), m12, rest);
-        rest = dfma(R(kHmCL), datan(ddiv(mf - f_ring, f_damp)), rest);
-        phi = rest + R(kHmDeltaPhi);
+        float rest = fmaf(r[kHmC0], mf, r[kHmC1] * lg);
+        rest = fmaf(-r[kHmC2], m1, rest);
+        const float m12 = m1 * m1;
+        rest = fmaf(-(1.0f / 3.0f) * r[kHmC4], m12 * m1, rest);
+        if (mixing) rest = fmaf(-0.5f * r[kHmC3], m12, rest);
+        rest = fmaf(r[kHmCL], atan_z, rest);
+        phi = rest + r[kHmDeltaPhi];
     }
-    phi.v = reduce_f32(phi.v);
-}
+    phi = reduce_f32(phi);
+    float sn, cs;
+    __sincosf(phi, &sn, &cs);
+    const float amp0 = r[kHmAmp0];
+    const float val = amp0 * a;
+    h_re = val * cs;
+    h_im = val * sn;
 
-// Mode k's contribution to the gradient's projections v (the slots of
-// xhm_adjoint_split_kernel) from its value and derivatives h (re, im), its
-// factors f = (F+, Fx) and their inclination derivatives df, the cotangents
-// (cp, cc) and m its azimuthal number
-__device__ inline void accumulate_mode(const DF& h_re, const DF& h_im, const float* f, const float* df, float emm,
-                                       float2 cp, float2 cc, double* v) {
-    const float g_re = f[0] * cp.x + f[1] * cp.y + f[2] * cc.x + f[3] * cc.y;
-    const float g_im = f[0] * cp.y - f[1] * cp.x + f[2] * cc.y - f[3] * cc.x;
-    for (int d = 0; d < 4; ++d) v[d] += (double)(g_re * h_re.d[d] + g_im * h_im.d[d]);
-    const float re = h_re.v, im = h_im.v;
-    v[4] += (double)(g_re * re + g_im * im);
-    v[6] += (double)(emm * (g_im * re - g_re * im));
-    const float dp_re = df[0] * re - df[1] * im, dp_im = df[0] * im + df[1] * re;
-    const float dc_re = df[2] * re - df[3] * im, dc_im = df[2] * im + df[3] * re;
-    v[5] += (double)(cp.x * dp_re + cp.y * dp_im + cc.x * dc_re + cc.y * dc_im);
-}
-
-// one bin's five modes, in forward mode one at a time, accumulated into v
-// as they come (xhm_modes_split's values): rec22, t22 the (2,2)'s split
-// record and derivatives (kSlots x 4), modes, tmodes the higher modes'
-// (4 x kHmSlots, 4 x kHmSlots x 4), fac, dfac the head and its derivative
-__device__ inline void xhm_bin_gradient(const float* __restrict__ rec22, const float* __restrict__ t22,
-                                        const float* __restrict__ modes, const float* __restrict__ tmodes,
-                                        const float* __restrict__ fac, const float* __restrict__ dfac, float f,
-                                        float2 cp, float2 cc, double* v) {
-    const PowF32 p = pow_f32(f);
-    DF a22, phi22;
-    amp_phase22_t(rec22, t22, p, a22, phi22);
-    DF s22, c22;
-    dsincos(phi22, phi22.v, s22, c22);
-    const DF h0_re = a22 * c22, h0_im = a22 * s22;
-    accumulate_mode(h0_re, h0_im, fac + kHeadFactors, dfac + kHeadFactors, 2.0f, cp, cc, v);
-    const MfPowDF q = mf_pow_t(modes, tmodes, p);
-#pragma unroll 1
-    for (int k = 0; k < kHmModes; ++k) {
-        const float* r = modes + k * kHmSlots;
-        const float* t = tmodes + 4 * k * kHmSlots;
-        if (r[kHmOn] == 0.0f) continue;
-        const DF amp0 = ld(r, t, kHmAmp0);
-        const float inv = __builtin_amdgcn_rcpf(fabsf(amp0.v));
-        const DF inv_amp0 = df_chain(inv, amp0, -(amp0.v < 0.0f ? -1.0f : 1.0f) * inv * inv);
-        DF a, phi;
-        hm_amp_phase_split_t(r, t, q, p.m53, h0_re * inv_amp0, h0_im * inv_amp0, a, phi);
-        const DF val = amp0 * a;
-        DF sn, cs;
-        dsincos(phi, phi.v, sn, cs);
-        accumulate_mode(val * cs, val * sn, fac + kHeadFactors + 4 * (k + 1), dfac + kHeadFactors + 4 * (k + 1),
-                        (float)kHmEmm[k], cp, cc, v);
+    // ---------------------------------------------------------- reverse
+    const float g_val = fmaf(g_re, cs, g_im * sn);
+    const float g_phi = val * fmaf(g_im, cs, -g_re * sn);
+    acc[aAmp0] += g_val * a;
+    const float g_a = clamped ? 0.0f : g_val * amp0;
+    MfGrad g{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float g_rot_re = 0.0f, g_rot_im = 0.0f;
+    // where both the amplitude and the phase take the rotation, h = amp0
+    // rot e^{i (c1_rd M f + c_rd)} exactly (a = |rot|, phi = arg rot +
+    // c1_rd M f + c_rd): rot's cotangent then needs no division by |rot|,
+    // which the deep ringdown takes below FP32's smallest normal
+    const bool mix_rd = mixing && amp_rd && ph_rd;
+    if (mix_rd) {
+        float e_s, e_c;
+        __sincosf(reduce_f32(fmaf(r[kHmC1Rd], mf, r[kHmCRd])), &e_s, &e_c);
+        g_rot_re = amp0 * fmaf(g_re, e_c, g_im * e_s);
+        g_rot_im = amp0 * fmaf(g_im, e_c, -g_re * e_s);
     }
+
+    // the phase's coefficients
+    if (ph_ins) {
+        acc[aT0] += g_phi * (float)p.m53;
+        acc[aKIns] += g_phi;
+        acc[aC1Insp] += g_phi * mf;
+        g.mf += g_phi * r[kHmC1Insp];
+        g.lg += g_phi * logs;
+        const float g_logs = g_phi * lg;
+        const float t2 = t * t;
+        acc[aPhiL5] += g_logs;
+        acc[aPhiL6] += g_logs * t;
+        acc[aPhiL8] += g_logs * t2 * t;
+        g.t += g_logs * fmaf(3.0f * r[kHmPhiL + 8], t2, r[kHmPhiL + 6]);
+        const float g_poly = g_phi * m43;
+        const float g_m43 = g_phi * poly;
+        g.m1 += g_m43 * inv_t;
+        g.t -= g_m43 * m43 * inv_t;
+        float pw = 1.0f, dpoly = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 12; ++j) {
+            acc[aPhi + j] += g_poly * pw;
+            if (j < 11) dpoly = fmaf((float)(j + 1) * r[kHmPhi + j + 2], pw, dpoly);
+            pw *= t;
+        }
+        g.t += g_poly * dpoly;
+        __asm__ volatile("");
+    } else if (ph_rd) {
+        acc[aC1Rd] += g_phi * mf;
+        acc[aCRd] += g_phi;
+        if (mixing) {
+            g.mf += g_phi * r[kHmC1Rd];
+            if (!mix_rd) {
+                // d atan2(y, x) = (x dy - y dx) / (x^2 + y^2), scaled by
+                // max(|x|, |y|) first; zero below FP32's smallest normal
+                const float s = fmaxf(fabsf(rot_re), fabsf(rot_im));
+                const bool normal = s >= 1.17549435e-38f;
+                const float inv_s = normal ? __builtin_amdgcn_rcpf(s) : 0.0f;
+                const float xs = rot_re * inv_s, ys = rot_im * inv_s;
+                const float rr = normal ? __builtin_amdgcn_rcpf(s * fmaf(xs, xs, ys * ys)) : 0.0f;
+                g_rot_re -= g_phi * ys * rr;
+                g_rot_im += g_phi * xs * rr;
+            }
+        } else {
+            acc[aAlpha0] += g_phi * mf;
+            g.mf += g_phi * (r[kHmC1Rd] + r[kHmAlpha0]);
+            const float a2 = r[kHmAlpha2];
+            acc[aAlpha2] -= g_phi * f_ring * f_ring * m1;
+            acc[aFRing] -= g_phi * 2.0f * f_ring * a2 * m1;
+            g.m1 -= g_phi * f_ring * f_ring * a2;
+            acc[aAlphaL] += g_phi * atan_z;
+            atan_z_adjoint(g_phi * r[kHmAlphaL] * datan_z, z, inv_damp, acc, g);
+        }
+        __asm__ volatile("");
+    } else {
+        acc[aC0] += g_phi * mf;
+        g.mf += g_phi * r[kHmC0];
+        acc[aC1] += g_phi * lg;
+        g.lg += g_phi * r[kHmC1];
+        acc[aC2] -= g_phi * m1;
+        g.m1 -= g_phi * r[kHmC2];
+        const float m12 = m1 * m1;
+        acc[aC4] -= g_phi * m12 * m1 * (1.0f / 3.0f);
+        g.m1 -= g_phi * r[kHmC4] * m12;
+        if (mixing) {
+            acc[aC3] -= 0.5f * g_phi * m12;
+            g.m1 -= g_phi * r[kHmC3] * m1;
+        }
+        acc[aCL] += g_phi * atan_z;
+        atan_z_adjoint(g_phi * r[kHmCL] * datan_z, z, inv_damp, acc, g);
+        acc[aDeltaPhi] += g_phi;
+    }
+
+    // the amplitude's coefficients
+    if (amp_ins) {
+        const float glob = r[kHmPnGlobal] * r[kHmAmpNorm], dom = r[kHmPnDominant];
+        g.m76 += g_a * fmaf(mag, glob, pseudo * dom);
+        const float g_in = g_a * m76;
+        acc[aPnGlobal] += g_in * mag * r[kHmAmpNorm];
+        acc[aAmpNorm] += g_in * mag * r[kHmPnGlobal];
+        acc[aPnDominant] += g_in * pseudo;
+        const float g_mag = g_in * glob;
+        const float inv_mag = mag > 0.0f ? __builtin_amdgcn_rcpf(mag) : 0.0f;
+        const float g_pre = g_mag * pn_re * inv_mag, g_pim = g_mag * pn_im * inv_mag;
+        float pw = 1.0f, d_re = 0.0f, d_im = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            acc[aPnRe + k] += g_pre * pw;
+            acc[aPnIm + k] += g_pim * pw;
+            if (k < 6) {
+                d_re = fmaf((float)(k + 1) * r[kHmPnRe + k + 1], pw, d_re);
+                d_im = fmaf((float)(k + 1) * r[kHmPnIm + k + 1], pw, d_im);
+            }
+            pw *= t;
+        }
+        g.t += fmaf(g_pre, d_re, g_pim * d_im);
+        const float g_ps = g_in * dom;
+        acc[aIns] += g_ps * t7;
+        acc[aIns + 1] += g_ps * t7 * t;
+        acc[aIns + 2] += g_ps * t7 * t * t;
+        const float t6 = t7 * fast_div(1.0f, t);
+        g.t += g_ps * fmaf(7.0f * t6, ins_poly, t7 * fmaf(2.0f * r[kHmIns + 2], t, r[kHmIns + 1]));
+        __asm__ volatile("");
+    } else if (amp_rd) {
+        if (mixing) {
+            if (!mix_rd) {
+                const float inv_a = a >= 1.17549435e-38f ? __builtin_amdgcn_rcpf(a) : 0.0f;
+                g_rot_re += g_a * rot_re * inv_a;
+                g_rot_im += g_a * rot_im * inv_a;
+            }
+        } else if (falloff) {
+            falloff_adjoint(r, mf, a, g_a, acc, g);
+        } else {
+            lorentzian_adjoint(r, mf, a, g_a, acc, g);
+        }
+        __asm__ volatile("");
+    } else {
+        g.m76 += g_a * inter;
+        const float g_res = g_a * m76;
+        float pw = 1.0f, d_u = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (j < n_inter) acc[aInter + j] += g_res * pw;
+            if (j + 1 < n_inter) d_u = fmaf((float)(j + 1) * r[kHmInter + j + 1], pw, d_u);
+            pw *= u;
+        }
+        const float g_u = g_res * d_u;
+        acc[aInterH] += g_u * (mf - r[kHmInterC]);
+        const float g_x = g_u * r[kHmInterH];
+        g.mf += g_x;
+        acc[aInterC] -= g_x;
+    }
+
+    // the mixing's rotation, back to w, the spheroidal phase and amplitude
+    gw_re = 0.0f;
+    gw_im = 0.0f;
+    if (rotating) {
+        const float m2r = r[kHmMix], m2i = r[kHmMix + 1], m3r = r[kHmMix + 2], m3i = r[kHmMix + 3];
+        acc[aMix] += fmaf(g_rot_re, w_re, g_rot_im * w_im);
+        acc[aMix + 1] += fmaf(g_rot_re, w_im, -g_rot_im * w_re);
+        gw_re = fmaf(g_rot_re, m2r, -g_rot_im * m2i);
+        gw_im = fmaf(g_rot_re, m2i, g_rot_im * m2r);
+        const float b_re = amplm * m3r, b_im = -(m3i * amplm);
+        const float g_bre = fmaf(g_rot_re, cs_s, g_rot_im * sn_s);
+        const float g_bim = fmaf(-g_rot_re, sn_s, g_rot_im * cs_s);
+        const float g_phl = g_rot_re * (-b_re * sn_s - b_im * cs_s) + g_rot_im * fmaf(b_re, cs_s, -b_im * sn_s);
+        acc[aMix + 2] += g_bre * amplm;
+        acc[aMix + 3] -= g_bim * amplm;
+        const float g_aml = fmaf(g_bre, m3r, -g_bim * m3i);
+        const float* sp = r + kHmSph;
+        acc[aSph] += g_phl * mf;
+        g.mf += g_phl * sp[0];
+        acc[aSph + 4] += g_phl;
+        acc[aSph + 2] -= g_phl * m1;
+        g.m1 -= g_phl * sp[2];
+        acc[aSph + 3] -= g_phl * m1 * m1 * m1 * (1.0f / 3.0f);
+        g.m1 -= g_phl * sp[3] * m1 * m1;
+        acc[aSph + 1] += g_phl * atan_z;
+        atan_z_adjoint(g_phl * sp[1] * datan_z, z, inv_damp, acc, g);
+        if (mf < r[kHmFRdAux]) {
+            const float* x = r + kHmRdAux;
+            acc[aRdAux] += g_aml;
+            acc[aRdAux + 1] += g_aml * mf;
+            acc[aRdAux + 2] += g_aml * mf * mf;
+            acc[aRdAux + 3] += g_aml * mf * mf * mf;
+            g.mf += g_aml * fmaf(fmaf(3.0f * x[3], mf, 2.0f * x[2]), mf, x[1]);
+        } else if (mf >= r[kHmFalloff]) {
+            falloff_adjoint(r, mf, amplm, g_aml, acc, g);
+        } else {
+            lorentzian_adjoint(r, mf, amplm, g_aml, acc, g);
+        }
+    }
+
+    // the powers of M f, back to the binary's M slots
+    g.m1 += g.m76 * rs;
+    g.t -= 0.5f * g.m76 * m76 * fast_div(1.0f, t);
+    acc[aMsec] += g.mf * p.f;
+    acc[aCbrtM] += g.t * p.t;
+    acc[aInvM] += g.m1 * p.m1;
+    acc[aLogM] += g.lg;
 }
 
 }  // namespace wfx

@@ -344,50 +344,6 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_c
 
 # ------------------------------------------------ the exact gradient
 
-#: the IMRPhenomXAS record's post-inspiral constants, T0, A and their FP64
-#: copies in the split record (csrc/xas_layout.h, csrc/xas_point.h)
-_XAS_INT_CONST, _XAS_RD_CONST, _XAS_T0, _XAS_T0_LO, _XAS_ANCHOR_LO = 13, 19, 56, 57, 59
-_XAS_T0_F64, _XAS_ANCHOR_F64 = 60, 62
-
-
-def tangents22(jac22: torch.Tensor) -> torch.Tensor:
-    """(B, 64, 4) float32: the (2,2) split record's slots' derivatives in
-    (m1, m2, chi1z, chi2z) as the split evaluation reads them, from the FP64
-    record's Jacobian (B, 64, 8): the post-inspiral constants hold -A
-    (fold_anchor), T0 and A sit in their FP64 copies' slots."""
-    t = jac22[:, :, :4].clone()
-    anchor = t[:, _XAS_ANCHOR].clone()
-    t[:, _XAS_INT_CONST] -= anchor
-    t[:, _XAS_RD_CONST] -= anchor
-    t[:, _XAS_T0_LO] = 0.0
-    t[:, _XAS_ANCHOR_LO] = 0.0
-    t[:, _XAS_T0_F64] = t[:, _XAS_T0]
-    t[:, _XAS_T0_F64 + 1] = 0.0
-    t[:, _XAS_ANCHOR_F64] = anchor
-    t[:, _XAS_ANCHOR_F64 + 1] = 0.0
-    return t.to(torch.float32).contiguous()
-
-
-def _f_ref_turn(jac22: torch.Tensor, reference_mf: Optional[float]) -> torch.Tensor:
-    """d(the (2,2)'s phase constant)/d f_ref (B,): its record's inspiral
-    constant's derivative less its anchor's; none with the mass-scaled
-    reference (nor where f_ref is zero: the Jacobian's column is zero)."""
-    if reference_mf is not None:
-        return torch.zeros_like(jac22[:, 0, 0])
-    return jac22[:, _XAS_INS_CONST, 7] - jac22[:, _XAS_ANCHOR, 7]
-
-
-def _gradient_from(partial: torch.Tensor, params: torch.Tensor, turn: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
-    s = partial.sum(1)
-    grads = torch.empty((params.shape[0], 8), dtype=torch.float64, device=params.device)
-    grads[:, :4] = s[:, :4]
-    grads[:, 4] = -s[:, 4] / params[:, 4]
-    grads[:, 5] = s[:, 5]
-    grads[:, 6] = s[:, 6]
-    grads[:, 7] = 0.5 * s[:, 6] * turn
-    return grads, s[:, 7]
-
-
 def adjoint_split(params: torch.Tensor, freqs: torch.Tensor, in_plus: torch.Tensor, in_cross: torch.Tensor,
                   weights: Optional[torch.Tensor] = None, amp_scale: float = 1e21,
                   reference_mf: Optional[float] = None) -> Tuple[torch.Tensor, Optional[torch.Tensor]]:
@@ -395,8 +351,9 @@ def adjoint_split(params: torch.Tensor, freqs: torch.Tensor, in_plus: torch.Tens
     grid ``freqs``: with ``weights`` None, ``in_plus``/``in_cross`` are the
     cotangents (B, N) (dL/dRe + i dL/dIm) and the loss is None; else they are
     the data and the loss (B,) is ``sum_f w (|h+ - d+|^2 + |hx - dx|^2)``.
-    One dual-number set-up (the records and their derivatives) and one
-    forward-mode pass over the bins: nothing per bin is stored."""
+    One dual-number set-up (the records and their derivatives) and
+    ``wfx_xhm_adjoint``: reverse mode per mode and span of frequencies, each
+    record's slots' totals contracted once with their derivatives."""
     _require_device(params, "params")
     params = params.to(torch.float64).contiguous()
     f32 = freqs.to(torch.float32).contiguous()
@@ -411,11 +368,12 @@ def adjoint_split(params: torch.Tensor, freqs: torch.Tensor, in_plus: torch.Tens
         zeros = torch.zeros((nb, 8), dtype=torch.float64, device=params.device)
         return zeros, (None if weights is None else torch.zeros((nb,), dtype=torch.float64, device=params.device))
     j = setup_jvp(params, f32[:1], amp_scale, reference_mf, "split")
-    t22 = tangents22(j["jac22"])
-    chunks = (n + 255) // 256
-    partial = torch.empty((nb, chunks, 8), dtype=torch.float64, device=params.device)
-    call("wfx_xhm_adjoint", _ptr(j["records22"]), _ptr(t22), _ptr(j["head"]), _ptr(j["dhead"]), _ptr(j["modes"]),
-         _ptr(j["dmodes"]), nb, _ptr(f32), n, 2, _ptr(in_plus), _ptr(in_cross), _ptr(weights), _ptr(partial),
-         _stream(params.device))
-    grads, loss = _gradient_from(partial, params, _f_ref_turn(j["jac22"], reference_mf))
+    dev = params.device
+    workspace = torch.empty((lib().wfx_xhm_adjoint_workspace(nb, n, int(weights is not None)),), dtype=torch.uint8,
+                            device=dev)
+    grads = torch.empty((nb, 8), dtype=torch.float64, device=dev)
+    loss = None if weights is None else torch.empty((nb,), dtype=torch.float64, device=dev)
+    call("wfx_xhm_adjoint", _ptr(params), _ptr(j["records22"]), _ptr(j["jac22"]), _ptr(j["head"]), _ptr(j["dhead"]),
+         _ptr(j["modes"]), _ptr(j["dmodes"]), nb, _ptr(f32), n, 2, _ptr(in_plus), _ptr(in_cross), _ptr(weights),
+         _ptr(workspace), workspace.numel(), _ptr(loss), _ptr(grads), _stream(dev))
     return grads, (None if weights is None else loss)
