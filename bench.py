@@ -303,31 +303,40 @@ def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
         result["xhm_split_roofline"] = {"bound": "valu", "achieved": tops, "peak": VALU_PEAK_TOPS,
                                         "unit": "T instructions/s", "frac": tops / VALU_PEAK_TOPS,
                                         "per_bin": per_split}
-    # the gradient (xhm.vjp: differences of the FP64 path with its region selections held, each inner
-    # product fused into the evaluation) of L = 1/2 sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2), d at parameters
-    # perturbed by 1%: the split forward + vector-Jacobian product of the residual, every binary
+    # the gradient: the fused split loss and gradient (adjoint.loss_and_grad: the dual-number set-up, the
+    # residuals, each mode's reverse-mode adjoint) of L = sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2), d at
+    # parameters perturbed by 1%, every binary; against the FP64 path's gradient of the same loss (xhm.vjp:
+    # differences with the region selections held) on the same float32 grid
+    from waveformx_amd import adjoint
+
     rng = np.random.default_rng(7)
     perturbed = params.cpu().numpy() * (1.0 + 1e-2 * rng.normal(size=tuple(params.shape)))
     perturbed[:, 7] = params[:, 7].cpu().numpy()
-    d_plus, d_cross = xhm.polarizations(torch.as_tensor(perturbed, device=dev), f64)
+    d_plus, d_cross = xhm.polarizations(torch.as_tensor(perturbed, device=dev), f64r)
+    d_plus32, d_cross32 = d_plus.to(torch.complex64), d_cross.to(torch.complex64)
+    w32 = w.to(torch.float32)
 
     def fwd_grad():
-        # the FP32-split forward, its residual's vector-Jacobian product through the FP64 path
-        hp, hc = xhm.polarizations(params, f32, precision="split")
-        state["g"] = xhm.vjp(params, f64, (hp.to(torch.complex128) - d_plus) * w, (hc.to(torch.complex128) - d_cross) * w)
+        state["g"] = adjoint.loss_and_grad(params, f32, d_plus32, d_cross32, w32, precision="split",
+                                           approximant="IMRPhenomXHM")
 
-    ms_grad = timed(torch, fwd_grad, max(2, steps // 5), 1)
-    # the gradient with the FP64 forward's residual instead, for the split forward's effect on it
-    hp, hc = xhm.polarizations(params, f64)
-    g64 = xhm.vjp(params, f64, (hp - d_plus) * w, (hc - d_cross) * w)
+    ms_grad = timed(torch, fwd_grad, steps, warmup)
+    loss32, g32 = state["g"]
+    hp, hc = xhm.polarizations(params, f64r)
+    w64 = w32.to(torch.float64)
+    g64 = xhm.vjp(params, f64r, 2.0 * w64 * (hp - d_plus), 2.0 * w64 * (hc - d_cross))
+    loss64 = ((hp - d_plus).abs() ** 2 + (hc - d_cross).abs() ** 2) @ w64
     del hp, hc
-    g_err = ((state["g"] - g64).abs().amax(0) / g64.abs().amax(0))[:7]
+    g_err = ((g32 - g64).abs().amax(0) / g64.abs().amax(0))[:7]
+    bar = 1e-2 * g64.abs() + 1e-3 * g64.abs().amax(0)
     result.update({"xhm_fwd_grad_binaries": nb, "xhm_fwd_grad_ms": ms_grad,
                    "xhm_fwd_grad_value": nb / (ms_grad * 1e-3), "xhm_fwd_grad_over_split_fwd": ms_grad / ms_split,
-                   "xhm_grad_finite": bool(torch.isfinite(state["g"][:, :7]).all()),
-                   "xhm_grad_rel_err_split_vs_fp64_forward": dict(zip(
+                   "xhm_grad_finite": bool(torch.isfinite(g32[:, :7]).all()),
+                   "xhm_grad_rel_err_split_vs_fp64": dict(zip(
                        ("mass_1", "mass_2", "chi_1", "chi_2", "luminosity_distance", "inclination", "phi_ref"),
-                       map(float, g_err.cpu())))})
+                       map(float, g_err.cpu()))),
+                   "xhm_grad_worst_fraction_of_float32_bar": float(((g32 - g64).abs() / bar)[:, :7].max()),
+                   "xhm_loss_rel_err_split_vs_fp64": float(((loss32 - loss64).abs() / loss64).max())})
     del d_plus, d_cross
     per_bin = XHM_VALU_PER_BIN.get((nb, n))
     if per_bin is not None:

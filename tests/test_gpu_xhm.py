@@ -507,13 +507,15 @@ def test_loss_and_grad(cuda):
 
 def test_split_gradient(cuda):
     """The exact split gradient (xhm.adjoint_split: the dual-number set-up
-    and the split evaluation differentiated per bin) for random cotangents
-    on a float32 grid over every region of the modes: within the
+    and the reverse-mode adjoint of the split evaluation) for random
+    cotangents on a float32 grid over every region of the modes: within the
     reference's float32 bar of the FP64 path's (xhm.vjp, differences at the
     same float32 frequencies) on a random batch and edge rows (equal masses
     and spins, m1 < m2, a spin of 0.99, q = 18, f_ref = 0 and the
-    mass-scaled reference); the swapped binary's gradient is the permuted
-    one bit for bit; a binary that fails LAL's checks has a NaN row; a batch
+    mass-scaled reference), and at the boundaries where the differences are
+    one-sided (masses 1e-5 apart with unequal spins: the labels' swap; a
+    spin of 1 - 1e-7; q = 1000); the swapped binary's gradient is the permuted one
+    bit for bit; a binary that fails LAL's checks has a NaN row; a batch
     equals each binary alone; the torch backward of the float32 function is
     this gradient."""
     import torch
@@ -524,13 +526,21 @@ def test_split_gradient(cuda):
 
     params = oxhm.draw(12, seed=21)
     edge = np.array([[30.0, 30.0, 0.2, 0.2, 400.0, 0.4, 0.0, 20.0],
-                     [20.0, 45.0, 0.3, -0.4, 300.0, 2.9, 0.7, 20.0],
-                     [45.0, 20.0, -0.4, 0.3, 300.0, 2.9, 0.7, 20.0],
+                     [20.0, 45.0, 0.3, -0.4, 300.0, 2.9, 0.7, 20.0],   # swapped, then
+                     [45.0, 20.0, -0.4, 0.3, 300.0, 2.9, 0.7, 20.0],   # in order
                      [50.0, 12.0, 0.99, 0.3, 300.0, 1.1, 1.7, 20.0],
-                     [90.0, 5.0, 0.5, 0.0, 300.0, 1.1, 1.7, 0.0],
-                     [30.0, 20.0, 1.5, 0.0, 300.0, 1.1, 1.7, 20.0]])
+                     [90.0, 5.0, 0.5, 0.0, 300.0, 1.1, 1.7, 0.0],      # f_ref = 0
+                     # at the boundaries that the differences take one-sided: masses 1e-5 apart with
+                     # unequal spins (the labels' swap within the central stencil's reach; at exactly
+                     # equal masses the exact gradient takes |m1 - m2|'s derivative as zero, jax's
+                     # convention, which no difference sees), a spin at its bound, q = 1000
+                     [30.0, 30.0 - 1e-5, 0.5, -0.3, 400.0, 0.9, 0.3, 20.0],
+                     [40.0, 20.0, 1.0 - 1e-7, 0.2, 400.0, 0.9, 0.3, 20.0],
+                     [200.0, 0.2, 0.3, 0.0, 100.0, 0.9, 0.3, 20.0],
+                     [30.0, 20.0, 1.5, 0.0, 300.0, 1.1, 1.7, 20.0]])   # fails LAL's checks
     params = np.concatenate([params, edge])
     nb = params.shape[0]
+    swapped, ordered, no_f_ref = 13, 14, 16
     p = torch.as_tensor(params, device=cuda)
     f = torch.linspace(20.0, 1024.0, 2049, dtype=torch.float32, device=cuda)
     rng = np.random.default_rng(22)
@@ -538,7 +548,7 @@ def test_split_gradient(cuda):
                          device=cuda)
     cc = torch.as_tensor(rng.normal(size=(nb, 2049)) + 1j * rng.normal(size=(nb, 2049)), dtype=torch.complex64,
                          device=cuda)
-    cp[nb - 5], cc[nb - 5] = cp[nb - 4], cc[nb - 4]  # the swapped pair sees the same cotangents
+    cp[swapped], cc[swapped] = cp[ordered], cc[ordered]  # the swapped pair sees the same cotangents
     for reference_mf in (None, 0.01):
         print("split gradient, reference_mf", reference_mf, flush=True)
         pr = p.clone()
@@ -553,10 +563,9 @@ def test_split_gradient(cuda):
         excess = np.where(bar > 0, diff / np.where(bar > 0, bar, 1.0), np.where(diff == 0, 0.0, np.inf))
         print("worst fraction of the float32 bar per parameter", excess.max(axis=0), flush=True)
         assert np.all(excess <= 1.0), np.unravel_index(excess.argmax(), excess.shape)
-        swapped, plain = got[nb - 5], got[nb - 4]
-        np.testing.assert_array_equal(swapped[[1, 0, 3, 2, 4, 5, 6, 7]], plain)
-        assert got[nb - 2, 7] == 0.0 and (reference_mf is None or np.all(got[:, 7][ok] == 0.0))
-    for b in (0, nb - 4):
+        np.testing.assert_array_equal(got[swapped][[1, 0, 3, 2, 4, 5, 6, 7]], got[ordered])
+        assert got[no_f_ref, 7] == 0.0 and (reference_mf is None or np.all(got[:, 7][ok] == 0.0))
+    for b in (0, ordered):
         one = xhm.adjoint_split(p[b:b + 1], f, cp[b:b + 1], cc[b:b + 1])[0].cpu().numpy()[0]
         np.testing.assert_array_equal(one, xhm.adjoint_split(p, f, cp, cc)[0].cpu().numpy()[b])
     # the torch backward of the float32 function (split arithmetic), at the float32-rounded parameters

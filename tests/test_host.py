@@ -295,3 +295,33 @@ def test_ragged_descriptor_and_offset_checks():
     for bad in ([0, 100, 50, 60], [5, 10, 20]):
         with pytest.raises(ValueError, match="start at 0 and never decrease"):
             xas.checked_total(torch.tensor(bad))
+
+
+def test_xhm_difference_stencils_stay_on_their_side():
+    """The FP64 XHM gradient's differences: central away from the
+    boundaries, one-sided towards the parameters' side where the central
+    stencil would cross the set-up's swap of the labels (m1 = m2), the spin
+    bound |chi| = 1 or the mass-ratio bound q = 1000 (xhm._directions); the
+    one-sided stencil differentiates a quartic exactly."""
+    import torch
+
+    from waveformx_amd import xhm
+
+    p = torch.tensor([[30.0, 20.0, 0.1, 0.2, 400.0, 0.4, 0.0, 20.0],
+                      [30.0, 30.0, 0.5, -0.3, 400.0, 0.4, 0.0, 20.0],
+                      [30.0, 30.0 + 1e-6, 0.5, -0.3, 400.0, 0.4, 0.0, 20.0],
+                      [40.0, 20.0, 1.0 - 1e-7, -1.0 + 1e-7, 400.0, 0.4, 0.0, 20.0],
+                      [1000.0, 1.0, 0.0, 0.0, 400.0, 0.4, 0.0, 20.0],
+                      [1.0, 1000.0, 0.0, 0.0, 400.0, 0.4, 0.0, 20.0]], dtype=torch.float64)
+    d = xhm._directions(p, xhm._steps(p))
+    expected = {0: [0, 1, -1, 0, -1, 1], 1: [0, -1, 1, 0, 1, -1], 2: [0, 0, 0, -1, 0, 0], 3: [0, 0, 0, 1, 0, 0]}
+    for j, values in expected.items():
+        assert d[j].tolist() == values, (j, d[j].tolist())
+    # the stencils: exact on polynomials of degree 4
+    x0, h = 0.7, 1e-2
+    poly = lambda x: 3 * x ** 4 - 2 * x ** 3 + x - 5  # noqa: E731
+    exact = 12 * x0 ** 3 - 6 * x0 ** 2 + 1
+    for side in (1, -1):
+        est = xhm._FORWARD_BASE * side * poly(x0) + sum(side * w * poly(x0 + side * o * h) for o, w in xhm._FORWARD)
+        assert abs(est / h - exact) < 1e-9
+    assert abs(sum(w * poly(x0 + o * h) for o, w in xhm._STENCIL) / h - exact) < 1e-9

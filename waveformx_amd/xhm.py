@@ -218,12 +218,53 @@ XAS_SELECTION_SLOTS = (49, 50, 51, 52, 53)
 _STENCIL = ((-2, 1.0 / 12.0), (-1, -8.0 / 12.0), (1, 8.0 / 12.0), (2, -1.0 / 12.0))
 
 
+#: the fourth-order one-sided stencils (offsets 1..4 and the base point's
+#: weight), for a parameter whose central stencil would cross a boundary
+_FORWARD = ((1, 48.0 / 12.0), (2, -36.0 / 12.0), (3, 16.0 / 12.0), (4, -3.0 / 12.0))
+_FORWARD_BASE = -25.0 / 12.0
+#: the mass ratio beyond which LAL's checks fail the binary (xas/api.py:40-60)
+_MAX_MASS_RATIO = 1000.0
+
+
 def _steps(params: torch.Tensor) -> list:
     """Per-parameter steps (B,) of the differences, or None for a parameter
     taken in closed form (d_L, the inclination, phi_ref and f_ref)."""
     m = params[:, :2].abs()
     ones = torch.ones_like(params[:, 0])
     return [3e-7 * m[:, 0], 3e-7 * m[:, 1], 1e-6 * ones, 1e-6 * ones, None, None, None, None]
+
+
+def _directions(params: torch.Tensor, steps: list) -> dict:
+    """Per parameter j with a step, the stencil's direction per binary (B,):
+    0 central, +1 forward, -1 backward. A one-sided stencil keeps every point
+    on the parameters' side of a boundary the central one would cross: the
+    set-up's swap of the labels where m1 < m2, LAL's spin bound |chi| <= 1
+    and its mass-ratio bound q <= 1000 (each a branch or a NaN there, not
+    one smooth formula)."""
+    m1, m2, c1, c2 = params[:, 0], params[:, 1], params[:, 2], params[:, 3]
+    ordered = m1 >= m2
+    reach = [None if h is None else 4.0 * h for h in steps]
+    zero = torch.zeros_like(m1)
+    out = {}
+    # m1: the swap at m1 = m2; q = m1 / m2 grows with m1 when ordered
+    crosses = torch.where(ordered, m1 - reach[0] < m2, m1 + reach[0] >= m2)
+    d = torch.where(crosses, torch.where(ordered, zero + 1, zero - 1), zero)
+    q_bound = ordered & ((m1 + reach[0]) / m2 > _MAX_MASS_RATIO)
+    d = torch.where(q_bound, zero - 1, d)
+    q_bound = ~ordered & (m2 / (m1 - reach[0]) > _MAX_MASS_RATIO)
+    out[0] = torch.where(q_bound, zero + 1, d)
+    # m2 likewise
+    crosses = torch.where(ordered, m2 + reach[1] > m1, m2 - reach[1] <= m1)
+    d = torch.where(crosses, torch.where(ordered, zero - 1, zero + 1), zero)
+    q_bound = ordered & (m1 / (m2 - reach[1]) > _MAX_MASS_RATIO)
+    d = torch.where(q_bound, zero + 1, d)
+    q_bound = ~ordered & ((m2 + reach[1]) / m1 > _MAX_MASS_RATIO)
+    out[1] = torch.where(q_bound, zero - 1, d)
+    # the spins: |chi| <= 1
+    for j, chi in ((2, c1), (3, c2)):
+        d = torch.where(chi + reach[j] > 1.0, zero - 1, zero)
+        out[j] = torch.where(chi - reach[j] < -1.0, zero + 1, d)
+    return out
 
 
 def _factor_derivatives(inclination) -> Tuple:
@@ -262,6 +303,9 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_c
     set-up and evaluation with every region selection (bounds, branches)
     held at the parameters, the reference's jnp.where semantics, so that
     each difference is of one smooth formula; relative accuracy ~1e-8.
+    Where the central stencil would cross the set-up's swap of the labels
+    (m1 = m2), the spin bound |chi| = 1 or the mass-ratio bound q = 1000,
+    the stencil is one-sided (fourth order, on the parameters' side).
     d_L's, phi_ref's, the inclination's and f_ref's are closed forms (the
     projection with the modes' factors in the polarizations replaced: f_ref
     turns each mode h_{l,-m} by m/2 times the (2,2)'s constant phase); f_ref's
@@ -319,26 +363,36 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_c
 
     # the differences: every stencil point of every parameter as one batch of binaries (in slices of at
     # most 65,536), set up at once with the selections held at the parameters, projected in one launch
+    # each parameter's stencil per binary: central, or one-sided where the central one would cross a
+    # boundary (offsets and weights (B, 4), the base point's weight (B,))
     steps = _steps(params)
-    points = [(j, offset, weight) for j, step in enumerate(steps) if step is not None for offset, weight in _STENCIL]
+    stencils = {}
+    as_t = lambda values: torch.tensor(values, dtype=torch.float64, device=dev)  # noqa: E731
+    c_off, c_w = as_t([o for o, _ in _STENCIL]), as_t([w for _, w in _STENCIL])
+    f_off, f_w = as_t([o for o, _ in _FORWARD]), as_t([w for _, w in _FORWARD])
+    for j, d in _directions(params, steps).items():
+        side = (d != 0)[:, None]
+        stencils[j] = (torch.where(side, d[:, None] * f_off, c_off), torch.where(side, d[:, None] * f_w, c_w),
+                       d * _FORWARD_BASE)
+    points = [(j, s) for j in stencils for s in range(len(_STENCIL))]
     per_slice = max(1, 65536 // nb)
     acc = torch.zeros((8, nb), dtype=torch.float64, device=dev)
     for first in range(0, len(points), per_slice):
         chunk = points[first:first + per_slice]
         p = params.repeat(len(chunk), 1)
-        for s, (j, offset, _) in enumerate(chunk):
-            p[s * nb:(s + 1) * nb, j] += offset * steps[j]
+        for s, (j, k) in enumerate(chunk):
+            p[s * nb:(s + 1) * nb, j] += stencils[j][0][:, k] * steps[j]
         rec, head, modes_, _ = setup(p, freqs[:1], amp_scale, reference_mf)
         rec.view(len(chunk), nb, -1)[:, :, list(XAS_SELECTION_SLOTS)] = rec0[:, list(XAS_SELECTION_SLOTS)]
         modes_.view(len(chunk), nb, 4, -1)[:, :, :, list(MODE_SELECTION_SLOTS)] = \
             modes0[:, :, list(MODE_SELECTION_SLOTS)]
         sums = project(rec, head, modes_, freqs, cot_plus, cot_cross).view(len(chunk), nb)
-        for s, (j, _, weight) in enumerate(chunk):
-            acc[j] += weight * sums[s]
+        for s, (j, k) in enumerate(chunk):
+            acc[j] += stencils[j][1][:, k] * sums[s]
         del rec, head, modes_
-    for j, step in enumerate(steps):
-        if step is not None:
-            grads[:, j] = acc[j] / step
+    for j, (_, _, base) in stencils.items():
+        # the base point's own projection is the closed forms' first (the head unchanged)
+        grads[:, j] = (acc[j] + torch.where(base != 0, base * closed[0], torch.zeros_like(base))) / steps[j]
     return grads
 
 
