@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (20); a binding checks it before any call. */
+/* The version of this ABI (21); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -141,6 +141,14 @@ int wfx_xas_adjoint(const void* records, const double* jacobian, int64_t n_binar
  * Replaces the grid part of the reference's vjp (src/waveformx/torch.py:116). */
 int wfx_xas_grid_adjoint(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
                          int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* stream);
+
+/* wfx_xas_fast_atan: y[i] = the FP32 arctangent of the split and FP32
+ * evaluations (atan(x) = x P(x^2) on [0, 1], pi/2 - atan(1/x) beyond, P the
+ * float32 polynomial of waveformx_amd/csrc/gen_atan.py), for its accuracy
+ * test. It replaces jnp.arctan in those arithmetics' phases (the
+ * reference's src/waveformx/models/xas/core.py:441), which is no entry
+ * point of the reference. */
+int wfx_xas_fast_atan(const float* x, float* y, int64_t n, void* stream);
 
 /* Fused forward + adjoint of L_b = sum_f weights[f] (|h+ - d+|^2 +
  * |hx - dx|^2) for data d (as the cotangents' layout) and weights[N]

@@ -110,10 +110,48 @@ def test_split_binary_neutron_star(xas, cuda):
         assert compare(a[0], b[0])[1] < 1e-6
 
 
-def test_plain_fp32_runs(xas, cuda):
-    params = oracle.draw(16, seed=1)
-    hp, hc = run(xas, cuda, params, CONFIG0_GRID, "fp32")
+def test_plain_fp32_accuracy(xas, cuda):
+    """The plain FP32 arithmetic (every term, the leading inspiral phase
+    too, in FP32) on the config-1 distribution: finite, and within a
+    mismatch of 1e-6 of the oracle at the same float32 frequencies on every
+    binary and polarization (measured ~2e-7; its inspiral phase is good to
+    ~0.06 rad, so the bar is the reference's float32 mismatch bar, not a
+    pointwise one)."""
+    params = oracle.draw(64, seed=1)
+    f32 = CONFIG0_GRID.astype(np.float32).astype(np.float64)
+    hp, hc = run(xas, cuda, params, f32, "fp32")
     assert np.all(np.isfinite(hp)) and np.all(np.isfinite(hc))
+    op, oc = oracle.polarizations(params, f32)
+    worst = max(compare(a[b], r[b])[1] for a, r in ((hp, op), (hc, oc)) for b in range(len(params)))
+    print("plain fp32, worst mismatch", worst)
+    assert worst < 1e-6, worst
+
+
+def test_fast_atan_bound(cuda):
+    """fast_atan, the FP32 arctangent of the split and FP32 phases, through
+    wfx_xas_fast_atan: within 2e-7 of atan over [0, inf) (a dense grid of
+    [0, 4], a geometric one to 1e38, infinity), odd, NaN to NaN, and its
+    coefficients' recipe (csrc/gen_atan.py) bounds the same error in float32
+    arithmetic."""
+    import ctypes
+
+    import torch
+
+    from waveformx_amd._lib import call
+
+    x = np.concatenate([np.linspace(0.0, 4.0, 400001), np.geomspace(1e-38, 1e38, 200001), [np.inf]])
+    x32 = torch.as_tensor(np.concatenate([x, -x, [np.nan]]), dtype=torch.float32, device=cuda)
+    y = torch.empty_like(x32)
+    call("wfx_xas_fast_atan", ctypes.c_void_p(x32.data_ptr()), ctypes.c_void_p(y.data_ptr()), x32.numel(),
+         ctypes.c_void_p(torch.cuda.current_stream(cuda).cuda_stream))
+    got = y.cpu().numpy().astype(np.float64)
+    xs = x32.cpu().numpy().astype(np.float64)
+    n = len(x)
+    err = np.abs(got[:2 * n] - np.arctan(xs[:2 * n]))
+    print("fast_atan, worst |error|", err.max())
+    assert err.max() < 2e-7, (err.max(), xs[err.argmax()])
+    np.testing.assert_array_equal(got[n:2 * n], -got[:n])
+    assert np.isnan(got[-1])
 
 
 @pytest.mark.parametrize("precision", ["fp64", "split"])

@@ -325,3 +325,29 @@ def test_xhm_difference_stencils_stay_on_their_side():
         est = xhm._FORWARD_BASE * side * poly(x0) + sum(side * w * poly(x0 + side * o * h) for o, w in xhm._FORWARD)
         assert abs(est / h - exact) < 1e-9
     assert abs(sum(w * poly(x0 + o * h) for o, w in xhm._STENCIL) / h - exact) < 1e-9
+
+
+def test_fast_atan_coefficients_come_from_their_recipe():
+    """The kernel's arctangent coefficients (csrc/xas_point.h) are the
+    float32 output of csrc/gen_atan.py, literally; the recipe's fit bounds
+    |error| by 7e-8 in exact arithmetic on [0, 1] and by 2e-7 in float32
+    arithmetic over [0, 1e30]."""
+    import importlib.util
+    import re
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    spec = importlib.util.spec_from_file_location("gen_atan", root / "waveformx_amd" / "csrc" / "gen_atan.py")
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    c = gen.coefficients()
+    text = (root / "waveformx_amd" / "csrc" / "xas_point.h").read_text()
+    body = text[text.index("__device__ inline float fast_atan"):]
+    body = body[body.index("float p = "):body.index("p *= r;")]
+    found = re.findall(r"(-?\d\.\d+(?:e-?\d+)?f)", body)
+    assert found == gen.literals(c), found
+    x = np.linspace(0.0, 1.0, 100001)
+    exact = np.max(np.abs(x * np.polyval(c[::-1].astype(np.float64), x * x) - np.arctan(x)))
+    assert exact < 7e-8, exact
+    z = np.concatenate([x, np.geomspace(1.0, 1e30, 100001)])
+    assert np.max(np.abs(gen.fast_atan_f32(z, c).astype(np.float64) - np.arctan(z))) < 2e-7

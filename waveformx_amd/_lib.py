@@ -14,7 +14,7 @@ LIBRARY = Path(__file__).resolve().parent / "libwfx_xas.so"
 _lib = None
 
 _P, _I64, _I, _D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
-ABI_VERSION = 20
+ABI_VERSION = 21
 SIGNATURES = {
     "wfx_last_error": ([], ctypes.c_char_p),
     "wfx_abi_version": ([], _I),
@@ -25,6 +25,7 @@ SIGNATURES = {
     "wfx_xas_eval": ([_P, _I64, _P, _I, _I64, _I, _I, _I, _P, _P, _P], _I),
     "wfx_xas_reference_phase": ([_P, _I64, _P, _P, _P], _I),
     "wfx_xas_reference_phase_grad": ([_P, _I64, _P, _P, _P, _P], _I),
+    "wfx_xas_fast_atan": ([_P, _P, _I64, _P], _I),
     "wfx_xas_tile_prefix": ([_P, _I64, _I, _P, _P], _I),
     "wfx_xas_eval_ragged": ([_P, _I64, _P, _I, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P], _I),
     "wfx_xas_power_table_bytes": ([_I], _I),

@@ -603,6 +603,13 @@ __global__ __launch_bounds__(1024) void tile_prefix_kernel(const int64_t* __rest
     if (threadIdx.x == 0) tiles[0] = 0;
 }
 
+// fast_atan elementwise (its accuracy test's entry point)
+__global__ __launch_bounds__(kBlock) void fast_atan_kernel(const float* __restrict__ x, float* __restrict__ y,
+                                                           int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = fast_atan(x[i]);
+}
+
 // ---------------------------------------------------------------- adjoint
 
 // dL/dtheta of each binary: one block per binary walks its frequencies,
@@ -1750,7 +1757,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 20; }
+int wfx_abi_version(void) { return 21; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -1806,6 +1813,16 @@ int wfx_xas_reference_phase(const double* params, int64_t n_binaries, const doub
     if (blocks < 0) return -1;
     reference_phase_kernel<<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, mf, phase);
     return check_launch("wfx_xas_reference_phase");
+}
+
+int wfx_xas_fast_atan(const float* x, float* y, int64_t n, void* stream) {
+    if (n < 0) return fail("wfx_xas_fast_atan: negative size");
+    if (n == 0) return 0;
+    if (!x || !y) return fail("wfx_xas_fast_atan: null pointer");
+    const int blocks = grid_blocks((n + kBlock - 1) / kBlock, "wfx_xas_fast_atan: too large");
+    if (blocks < 0) return -1;
+    fast_atan_kernel<<<blocks, kBlock, 0, (hipStream_t)stream>>>(x, y, n);
+    return check_launch("wfx_xas_fast_atan");
 }
 
 int wfx_xas_reference_phase_grad(const double* params, int64_t n_binaries, const double* mf, double* phase,

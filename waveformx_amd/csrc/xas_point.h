@@ -69,18 +69,18 @@ struct PowF32 {
 };
 
 // arctangent in FP32: atan(x) = x P(x^2) on [0, 1] (P of degree 7 in x^2,
-// fitted here by Lawson's iteratively reweighted least squares towards the
-// minimax polynomial; |error| < 4e-8 in exact arithmetic, < 2e-7 in FP32),
-// and pi/2 - atan(1/x) beyond
+// the near-minimax polynomial that gen_atan.py fits by Lawson's iteratively
+// reweighted least squares and rounds to float32; |error| < 7e-8 in exact
+// arithmetic, < 2e-7 in FP32 over [0, inf)), and pi/2 - atan(1/x) beyond
 __device__ inline float fast_atan(float x) {
     const float a = fabsf(x);
     const bool big = a > 1.0f;
     const float r = big ? __builtin_amdgcn_rcpf(a) : a;
     const float r2 = r * r;
-    float p = -0.004054564982652664f;
-    p = fmaf(p, r2, 0.021862950176000595f);
-    p = fmaf(p, r2, -0.0559123158454895f);
-    p = fmaf(p, r2, 0.09642196446657181f);
+    float p = -0.004054555203765631f;
+    p = fmaf(p, r2, 0.02186291664838791f);
+    p = fmaf(p, r2, -0.05591227486729622f);
+    p = fmaf(p, r2, 0.09642194211483002f);
     p = fmaf(p, r2, -0.1390862911939621f);
     p = fmaf(p, r2, 0.19946566224098206f);
     p = fmaf(p, r2, -0.33329859375953674f);
