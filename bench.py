@@ -75,11 +75,23 @@ def draw(n, seed=1):
     return np.stack([m1, m1 * q, c1, c2, dl, inc, phi, np.full(n, 20.0)], axis=1)
 
 
+#: the least time the card runs a leg's work before its timed calls: after a
+#: change of load its clock dips and recovers over a few hundred
+#: milliseconds, which a timed region inside that window would average in
+WARM_SECONDS = 0.4
+
+
 def timed(torch, fn, steps, warmup):
-    """Average ms of ``fn`` over ``steps`` calls after ``warmup``, by events
-    on the current stream."""
-    for _ in range(warmup):
+    """Average ms of ``fn`` over ``steps`` calls after ``warmup`` calls and
+    at least WARM_SECONDS of them (at most 4000), by events on the current
+    stream."""
+    t0 = time.perf_counter()
+    calls = 0
+    while calls < warmup or (time.perf_counter() - t0 < WARM_SECONDS and calls < 4000):
         fn()
+        calls += 1
+        if calls % 8 == 0:
+            torch.cuda.synchronize()
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record()

@@ -561,16 +561,29 @@ __global__ __launch_bounds__(kBlock) void eval_f32_ragged(const float* __restric
         const float* rec = records + b * kSlots;
 #pragma unroll
         for (int k = 0; k < kSlots; ++k) rc[k] = rec[k];
+        // the tile's first bin and the number of its bins the binary has:
+        // the per-bin index arithmetic is then 32-bit, the outputs'
+        // addresses a wave-uniform base plus a 32-bit offset
+        const int64_t first = (w - tiles[b]) * (kBlock * BPT);
+        const int64_t left = stop - start - first;
+        const int count = left < kBlock * BPT ? (int)left : kBlock * BPT;
+        const double first_d = (double)first;
+        float2* __restrict__ out_p = hp + start + first;
+        float2* __restrict__ out_c = hc + start + first;
+        const F* __restrict__ tile_f = freqs ? freqs + start + first : nullptr;
+        const double g0 = freqs ? 0.0 : f0[b], gd = freqs ? 0.0 : df[b];
 #pragma unroll
         for (int j = 0; j < BPT; ++j) {
-            const int64_t k = (w - tiles[b]) * (kBlock * BPT) + j * kBlock + threadIdx.x;
-            const int64_t i = start + k;
-            if (i >= stop) break;
-            const typename M::Pow32 p = M::powers32((float)ragged_freq(freqs, f0, df, b, i, k));
+            const int k = j * kBlock + (int)threadIdx.x;
+            if (k >= count) break;
+            // f0 + k df as ragged_freq computes it (the bin's index first +
+            // k, exact in FP64)
+            const double f = tile_f ? (double)tile_f[k] : g0 + (first_d + (double)k) * gd;
+            const typename M::Pow32 p = M::powers32((float)f);
             float2 a, c;
             M::template point32<L>(rc, p, a, c);
-            hp[i] = a;
-            hc[i] = c;
+            out_p[k] = a;
+            out_c[k] = c;
         }
     }
 }
