@@ -317,8 +317,8 @@ def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
                                         "per_bin": per_split}
     # the gradient: the fused split loss and gradient (adjoint.loss_and_grad: the dual-number set-up, the
     # residuals, each mode's reverse-mode adjoint) of L = sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2), d at
-    # parameters perturbed by 1%, every binary; against the FP64 path's gradient of the same loss (xhm.vjp:
-    # differences with the region selections held) on the same float32 grid
+    # parameters perturbed by 1%, every binary; against the FP64 path's exact gradient of the same loss
+    # (xhm.vjp, the same adjoint in FP64) on the same float32 grid; the FP64 fused loss and gradient timed too
     from waveformx_amd import adjoint
 
     rng = np.random.default_rng(7)
@@ -334,8 +334,15 @@ def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
 
     ms_grad = timed(torch, fwd_grad, steps, warmup)
     loss32, g32 = state["g"]
-    hp, hc = xhm.polarizations(params, f64r)
     w64 = w32.to(torch.float64)
+
+    def fwd_grad64():
+        state["g64"] = adjoint.loss_and_grad(params, f64r, d_plus, d_cross, w64, precision="fp64",
+                                             approximant="IMRPhenomXHM")
+
+    ms_grad64 = timed(torch, fwd_grad64, max(2, steps // 4), 1)
+    del state["g64"]
+    hp, hc = xhm.polarizations(params, f64r)
     g64 = xhm.vjp(params, f64r, 2.0 * w64 * (hp - d_plus), 2.0 * w64 * (hc - d_cross))
     loss64 = ((hp - d_plus).abs() ** 2 + (hc - d_cross).abs() ** 2) @ w64
     del hp, hc
@@ -343,6 +350,7 @@ def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
     bar = 1e-2 * g64.abs() + 1e-3 * g64.abs().amax(0)
     result.update({"xhm_fwd_grad_binaries": nb, "xhm_fwd_grad_ms": ms_grad,
                    "xhm_fwd_grad_value": nb / (ms_grad * 1e-3), "xhm_fwd_grad_over_split_fwd": ms_grad / ms_split,
+                   "xhm_fp64_fwd_grad_ms": ms_grad64, "xhm_fp64_fwd_grad_over_fp64_fwd": ms_grad64 / ms,
                    "xhm_grad_finite": bool(torch.isfinite(g32[:, :7]).all()),
                    "xhm_grad_rel_err_split_vs_fp64": dict(zip(
                        ("mass_1", "mass_2", "chi_1", "chi_2", "luminosity_distance", "inclination", "phi_ref"),

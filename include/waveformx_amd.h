@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (21); a binding checks it before any call. */
+/* The version of this ABI (22); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -265,45 +265,57 @@ int wfx_xhm_eval(const void* records22, const void* head, const void* modes, int
                  int freqs_f64, int64_t n_freqs, int precision, void* hplus, void* hcross, void* mode_out,
                  void* stream);
 
-/* wfx_xhm_setup_jvp: wfx_xhm_setup's head and higher-mode records (same
- * arguments, same values) with their derivatives, forward mode through the
- * same FP64 set-up: dhead (B, wfx_xhm_head_slots()) the head's derivative in
- * the inclination, dmodes (B, 4 modes, wfx_xhm_record_slots(), 4 directions)
- * the records' derivatives in (mass_1, mass_2, chi_1z, chi_2z) (for
- * precision 2, of the split records' slots, float; the FP64 pairs' in their
- * first word). The (2,2) record and its derivatives come from
- * wfx_xas_setup_jvp (the same record). The forward half of the reference's
- * jax.vjp for IMRPhenomXHM (src/waveformx/torch.py:87-93; the MLX port's
- * tables_jvp, src/waveformx/models/xhm/mlx/_kernels.py:551-603). */
+/* wfx_xhm_setup_jvp: the head and the higher-mode records of the gradient
+ * (same arguments as wfx_xhm_setup) with their derivatives, forward mode
+ * through the same FP64 set-up: the records in the split evaluation's
+ * layout (each mode's leading inspiral term and inspiral constant as an
+ * FP64 slot, the later regions' constants reduced modulo 2 pi, quotients
+ * folded): float for precision 2 (wfx_xhm_setup's split records, the FP64
+ * slots as two floats), double for precision 0 (the same slots in FP64, an
+ * FP64 slot's second word zero; the FP64 evaluation's records are
+ * wfx_xhm_setup's, in LAL's form). dhead (B, wfx_xhm_head_slots()) the
+ * head's derivative in the inclination, dmodes (B, 4 modes,
+ * wfx_xhm_record_slots(), 4 directions) the records' derivatives in
+ * (mass_1, mass_2, chi_1z, chi_2z) (an FP64 slot's in its first word). The
+ * (2,2) record and its derivatives come from wfx_xas_setup_jvp (the same
+ * record). The forward half of the reference's jax.vjp for IMRPhenomXHM
+ * (src/waveformx/torch.py:87-93; the MLX port's tables_jvp,
+ * src/waveformx/models/xhm/mlx/_kernels.py:551-603). */
 int wfx_xhm_setup_jvp(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
                       double amp_scale, double reference_mf, int precision, void* head, void* dhead, void* modes,
                       void* dmodes, void* stream);
 
 /* wfx_xhm_adjoint: grads (B, 8) double, the gradient in (mass_1, mass_2,
  * chi_1z, chi_2z, luminosity_distance, inclination, phi_ref, f_ref) of the
- * split polarizations (precision 2) on the float grid freqs (N,), for the
- * binaries params (B, 8) set up by wfx_xas_setup_jvp (records22 (B, 64) the
- * (2,2)'s split record, jacobian22 (B, 64, 8) its FP64 derivatives) and
- * wfx_xhm_setup_jvp (head, dhead, modes, dmodes, precision 2). With weights
- * NULL, in_plus/in_cross are the cotangents (B, N) complex64 (dL/dRe +
- * i dL/dIm) and loss is not written; else they are the data, weights (N,)
- * float, and loss (B,) double receives L = sum_f weights (|h+ - d+|^2 +
- * |hx - dx|^2), the gradient being L's. Reverse mode: one wave per span of
- * frequencies and mode of a binary recomputes the mode, sums each bin's
- * dL/d(record slot) in registers and contracts the totals with the
+ * polarizations in the split layout's formula, FP32 (precision 2, freqs
+ * float) or FP64 (precision 0, freqs double), on the grid freqs (N,), for
+ * the binaries params (B, 8) set up by wfx_xas_setup_jvp (records22
+ * (B, 64) the (2,2)'s record of that precision, jacobian22 (B, 64, 8) its
+ * FP64 derivatives) and wfx_xhm_setup_jvp (head, dhead, modes, dmodes, the
+ * same precision). With weights NULL, in_plus/in_cross are the cotangents
+ * (B, N) complex64/complex128 (dL/dRe + i dL/dIm) and loss is not written;
+ * else they are the data, weights (N,) float/double, and loss (B,) double
+ * receives L = sum_f weights (|h+ - d+|^2 + |hx - dx|^2), the gradient being
+ * L's. dldf (N,) double, if not NULL, receives dL/df of each frequency (the
+ * grid's gradient, summed over the binaries). Reverse mode: one wave per
+ * span of frequencies and mode of a binary recomputes the mode, sums each
+ * bin's dL/d(record slot) in registers and contracts the totals with the
  * records' derivatives once; nothing per bin is kept but the (3,2)'s
- * cotangent of the (2,2) it recycles (and, for the loss, the residuals).
- * workspace: device memory of at least wfx_xhm_adjoint_workspace(B, N,
- * weights != NULL) bytes, 256-aligned (B x N x 8 bytes, 3x that for the
- * loss, and little more). Each sum runs in a fixed order: the result is
- * reproducible. The reverse half of the reference's jax.vjp for
- * IMRPhenomXHM (src/waveformx/torch.py:108-119; its GPU port's
- * cotangent_product, src/waveformx/models/xhm/mlx/_kernels.py:300-441). */
-int64_t wfx_xhm_adjoint_workspace(int64_t n_binaries, int64_t n_freqs, int loss);
+ * cotangent of the (2,2) it recycles (and, for the loss, the residuals; for
+ * dldf, each bin's dL/df per mode). workspace: device memory of at least
+ * wfx_xhm_adjoint_workspace(B, N, precision, weights != NULL, dldf != NULL)
+ * bytes, 256-aligned (B x N x 8 bytes per real's 4, 3x that for the loss,
+ * 4x more for dldf, and little more; -1 for an unknown precision). Each sum
+ * runs in a fixed order: the result is reproducible. The reverse half of
+ * the reference's jax.vjp for IMRPhenomXHM (src/waveformx/torch.py:108-119,
+ * the grid's gradient at :116; its GPU port's cotangent_product,
+ * src/waveformx/models/xhm/mlx/_kernels.py:300-441). */
+int64_t wfx_xhm_adjoint_workspace(int64_t n_binaries, int64_t n_freqs, int precision, int loss, int grid);
 int wfx_xhm_adjoint(const double* params, const void* records22, const double* jacobian22, const void* head,
                     const void* dhead, const void* modes, const void* dmodes, int64_t n_binaries, const void* freqs,
                     int64_t n_freqs, int precision, const void* in_plus, const void* in_cross, const void* weights,
-                    void* workspace, int64_t workspace_bytes, double* loss, double* grads, void* stream);
+                    void* workspace, int64_t workspace_bytes, double* loss, double* grads, double* dldf,
+                    void* stream);
 
 /* wfx_xhm_project: the FP64 polarizations of wfx_xhm_eval (FP64 records of
  * wfx_xhm_setup, float64 grid) contracted with cotangents without writing

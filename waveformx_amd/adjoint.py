@@ -12,10 +12,9 @@ and contracts with those derivatives. Nothing of the forward pass is stored.
 :func:`loss_and_grad` fuses the forward pass with a noise-weighted
 least-squares loss ``L = sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2)``: it reads
 the data and writes only each binary's loss and gradient. IMRPhenomXHM's
-split gradient is exact (:func:`waveformx_amd.xhm.adjoint_split`: the
-dual-number set-up and a forward-mode pass over the bins, the loss fused);
-its FP64 gradient is :func:`waveformx_amd.xhm.vjp`'s (differences of the
-FP64 path, each inner product fused into the evaluation).
+gradient, in either arithmetic, is :func:`waveformx_amd.xhm.adjoint`'s (the
+dual-number set-up and a reverse-mode pass per mode over the bins, the
+loss fused, the grid's gradient on request).
 
 Both models share the kernels' shape (one wave per binary, the same
 reduce-scatter and contraction; one thread per frequency for the grid's
@@ -108,14 +107,11 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, grad_plus: Optional[torch.Ten
         wrt_grid: bool = False, approximant: str = "IMRPhenomXAS") -> Tuple[torch.Tensor, Optional[torch.Tensor]]:
     """Gradients (B, 8) with respect to the packed parameters, and (N,)
     with respect to the grid if ``wrt_grid`` (else None). IMRPhenomXHM's is
-    :func:`waveformx_amd.xhm.adjoint_split` for ``"split"`` (exact, on the
-    float32 grid) and :func:`waveformx_amd.xhm.vjp` for ``"fp64"``; it has
-    no grid gradient."""
+    :func:`waveformx_amd.xhm.adjoint` (exact, FP64 on the float64 grid or
+    FP32 on the float32 grid)."""
     if approximant == "IMRPhenomXHM":
         from . import xhm
 
-        if wrt_grid:
-            raise NotImplementedError("IMRPhenomXHM has no gradient with respect to the grid")
         if precision not in xhm.PRECISIONS:
             raise ValueError(f"IMRPhenomXHM's arithmetic is one of {xhm.PRECISIONS}, got {precision!r}")
         nb, n = params.shape[0], freqs.shape[0]
@@ -123,9 +119,8 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, grad_plus: Optional[torch.Ten
         zeros = torch.zeros((nb, n), dtype=ctype, device=params.device)
         gp = zeros if grad_plus is None else grad_plus.reshape(nb, n)
         gc = zeros if grad_cross is None else grad_cross.reshape(nb, n)
-        if precision == "split":
-            return xhm.adjoint_split(params, freqs, gp, gc, None, amp_scale, reference_mf)[0], None
-        return xhm.vjp(params, freqs, gp, gc, amp_scale, reference_mf), None
+        grads, _, grid = xhm.adjoint(params, freqs, gp, gc, None, amp_scale, reference_mf, precision, wrt_grid)
+        return grads, grid
     _entries(approximant, precision)
     nb, n = params.shape[0], freqs.shape[0]
     ctype = torch.complex128 if precision == "fp64" else torch.complex64
@@ -170,16 +165,9 @@ def loss_and_grad(params: torch.Tensor, freqs: torch.Tensor, data_plus: torch.Te
         raise ValueError(f"data must be ({nb}, {n}) {ctype}")
     if weights.shape != (n,) or weights.dtype != rdtype:
         raise ValueError(f"weights must be ({n},) {rdtype}")
-    if xhm_model and precision == "split":
-        grads, loss = xhm.adjoint_split(params, freqs, data_plus, data_cross, weights, amp_scale, reference_mf)
-        return loss, grads
     if xhm_model:
-        hp, hc = xhm.polarizations(params, freqs, amp_scale, reference_mf, precision)
-        res_plus, res_cross = hp - data_plus, hc - data_cross
-        loss = ((res_plus.abs() ** 2 + res_cross.abs() ** 2) * weights).sum(1).to(torch.float64)
-        w = weights.to(torch.float64)
-        grads = xhm.vjp(params, freqs, 2.0 * w * res_plus.to(torch.complex128),
-                        2.0 * w * res_cross.to(torch.complex128), amp_scale, reference_mf)
+        grads, loss, _ = xhm.adjoint(params, freqs, data_plus, data_cross, weights, amp_scale, reference_mf,
+                                     precision)
         return loss, grads
     rec, jac = records if records is not None else _records(params, freqs, amp_scale, reference_mf, precision,
                                                             approximant)

@@ -19,16 +19,9 @@
 
 namespace wfx {
 
-// the FP32 adjoint uses the forward kernel's fast forms (|phi| <= pi, so
-// the hardware sine and cosine need no range reduction); FP64 the library's
-__device__ inline void m_sincos(float x, float* s, float* c) { __sincosf(x, s, c); }
-__device__ inline void m_sincos(double x, double* s, double* c) { sincos(x, s, c); }
-__device__ inline float d_atan(float x) { return fast_atan(x); }
-__device__ inline double d_atan(double x) { return atan(x); }
-__device__ inline float d_exp(float x) { return __expf(x); }
-__device__ inline double d_exp(double x) { return exp(x); }
-__device__ inline float d_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ inline double d_rcp(double x) { return 1.0 / x; }
+// (the FP32 adjoint uses the forward kernel's fast forms through m_sincos,
+// d_atan, d_exp, d_rcp: |phi| <= pi, so the hardware sine and cosine need
+// no range reduction; FP64 the library's)
 
 template <class T>
 struct BinPowers {
@@ -222,17 +215,18 @@ __device__ __forceinline__ void adjoint_point(const T* __restrict__ c, const Bin
 // dL/df of one bin (the frequency itself moving): dL/dA A'(f) + dL/dphi
 // phi'(f), with phi' the derivative of the region's phase (basis terms,
 // arctangent, leading term -5/3 T0 f^(-8/3)) and A' that of the region's
-// amplitude; zero above the cutoff.
+// amplitude; zero above the cutoff. The polarization factors kp, kc given
+// (grid_adjoint_point: the record's; IMRPhenomXHM's (2,2) takes kp = 1,
+// kc = 0, the cotangent of the mode itself).
 template <class T>
-__device__ inline T grid_adjoint_point(const T* __restrict__ c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re,
-                                       T gx_im) {
+__device__ inline T grid_adjoint_point_k(const T* __restrict__ c, const BinPowers<T>& p, T kp, T kc, T gp_re, T gp_im,
+                                         T gx_re, T gx_im) {
     bool past, ringdown;
     T z, amp;
     const T phi = phase_and_amp(c, p, past, ringdown, z, amp);
     if (!(p.f <= c[kFCut])) return T(0);
     T s, co;
     m_sincos(phi, &s, &co);
-    const T kp = c[kKp], kc = c[kKc];
     const T g_amp = kp * (gp_re * co + gp_im * s) + kc * (gx_im * co - gx_re * s);
     const T g_phi = -amp * (kp * (gp_re * s - gp_im * co) + kc * (gx_re * co + gx_im * s));
     const T f = p.f, t = p.t, m1 = p.m1, lf = p.lf;
@@ -279,6 +273,12 @@ __device__ inline T grid_adjoint_point(const T* __restrict__ c, const BinPowers<
         damp = p.m76 * m1 * poly;
     }
     return g_amp * damp + g_phi * dphi;
+}
+
+template <class T>
+__device__ inline T grid_adjoint_point(const T* __restrict__ c, const BinPowers<T>& p, T gp_re, T gp_im, T gx_re,
+                                       T gx_im) {
+    return grid_adjoint_point_k(c, p, c[kKp], c[kKc], gp_re, gp_im, gx_re, gx_im);
 }
 
 }  // namespace wfx

@@ -1242,9 +1242,10 @@ __global__ __launch_bounds__(64) void xhm_setup_kernel(const double* __restrict_
 // mode, direction), the (mode, direction) uniform over a block (blocks
 // [g P, (g + 1) P), g = 4 k + direction, P = ceil(nb / 64)); the 17th group
 // writes the head and its derivative in the inclination (one thread per
-// binary). Records as wfx_xhm_setup's (FP64, or split with the tangents of
-// the split slots); dmodes (B, 4 modes, kHmSlots, 4 directions), double or
-// float as the records. The (2,2) and its derivatives come from the
+// binary). The records in the split layout, the gradient's: float for the
+// split arithmetic, double for FP64, with the tangents of their slots;
+// dmodes (B, 4 modes, kHmSlots, 4 directions), double or float as the
+// records. The (2,2) and its derivatives come from the
 // IMRPhenomXAS derivative setup (the same record).
 __global__ __launch_bounds__(64) void xhm_setup_jvp_kernel(const double* __restrict__ params, int64_t nb,
                                                            const double* __restrict__ f_min, int64_t f_min_stride,
@@ -1303,11 +1304,11 @@ __global__ __launch_bounds__(64) void xhm_setup_jvp_kernel(const double* __restr
             for (int s = 0; s < kHmSlots; ++s) ((float*)modes)[at + s] = values[s];
         for (int s = 0; s < kHmSlots; ++s) ((float*)dmodes)[4 * (at + s) + dir] = tangents[s];
     } else {
-        D rec[kHmSlots];
-        write_mode_record(c, wf, rec);
+        double values[kHmSlots], tangents[kHmSlots];
+        write_mode_record_split(c, wf, values, tangents);
         if (dir == 0)
-            for (int s = 0; s < kHmSlots; ++s) ((double*)modes)[at + s] = rec[s].v;
-        for (int s = 0; s < kHmSlots; ++s) ((double*)dmodes)[4 * (at + s) + dir] = rec[s].d[0];
+            for (int s = 0; s < kHmSlots; ++s) ((double*)modes)[at + s] = values[s];
+        for (int s = 0; s < kHmSlots; ++s) ((double*)dmodes)[4 * (at + s) + dir] = tangents[s];
     }
 }
 
@@ -1410,11 +1411,12 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
     }
 }
 
-// The gradient of IMRPhenomXHM's split polarizations, reverse mode: each
-// binary's grid is cut into `chunks` spans of `span` frequencies (a multiple
-// of 64), and one wave takes one span of one mode of one binary, lane l the
-// span's frequencies l, l + 64, ...: the mode's forward values recomputed,
-// each bin's dL/d(record slot) summed per lane in registers (the (2,2): the
+// The gradient of IMRPhenomXHM's polarizations, reverse mode, in the split
+// layout's formula: FP32 (the split gradient) or FP64. Each binary's grid
+// is cut into `chunks` spans of `span` frequencies (a multiple of 64), and
+// one wave takes one span of one mode of one binary, lane l the span's
+// frequencies l, l + 64, ...: the mode's forward values recomputed, each
+// bin's dL/d(record slot) summed per lane in registers (the (2,2): the
 // IMRPhenomXAS adjoint's 64 slots; a higher mode: its 86 slots with a
 // derivative), one reduce-scatter over the wave, and the slots' totals
 // contracted once with the record's derivatives. The (3,2)'s ringdown
@@ -1424,22 +1426,28 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
 // 1..4 the higher modes): Re conj(cot) dh/d(mass_1, mass_2, chi_1z, chi_2z)
 // (0..3); Re conj(G) h (4); Re conj(cot) dF/d(iota) h (5); Re m conj(G) i h
 // (6), G = conj(F+) cot+ + conj(Fx) cotx the mode's cotangent; 0 (7).
+// With the grid's gradient, each higher mode's wave writes its bins' dL/df
+// (gfm[k][b][i]), the (2,2)'s wave adds them to its own (into gfm[0][b][i]),
+// and two passes sum them over the binaries in a fixed order.
 constexpr int kXhmGradSlots = 8;
 constexpr int kXhmGradRows = 5;
 // at least this many higher-mode waves per launch (the chip holds ~3,000
 // at the kernels' occupancy)
 constexpr int64_t kXhmGradWaves = 12288;
+// the grid gradient's first pass: each thread sums this many binaries
+constexpr int64_t kXhmGridGroup = 64;
 
 struct XhmGradPlan {
-    int64_t chunks, span, loss_chunks;
-    int64_t table, gh22, partial, cot_plus, cot_cross, loss_partial, bytes;
+    int64_t chunks, span, loss_chunks, grid_groups;
+    int64_t table, gh22, partial, cot_plus, cot_cross, loss_partial, gfm, grid_partial, bytes;
 };
 
 inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
 // the spans and the workspace's layout (each part 256-aligned) of B
-// binaries on N frequencies; bytes < 0 where the sizes overflow
-inline XhmGradPlan xhm_grad_plan(int64_t nb, int64_t n, bool loss) {
+// binaries on N frequencies in T (4- or 8-byte reals); bytes < 0 where the
+// sizes overflow
+inline XhmGradPlan xhm_grad_plan(int64_t nb, int64_t n, bool loss, bool grid, int64_t real) {
     XhmGradPlan g{};
     const int64_t max_chunks = (n + 63) / 64;
     int64_t chunks = (kXhmGradWaves + 4 * nb - 1) / (4 * nb);
@@ -1447,28 +1455,49 @@ inline XhmGradPlan xhm_grad_plan(int64_t nb, int64_t n, bool loss) {
     g.span = ((n + chunks - 1) / chunks + 63) / 64 * 64;
     g.chunks = (n + g.span - 1) / g.span;
     g.loss_chunks = (n + kXhmBlock - 1) / kXhmBlock;
-    if (nb > (INT64_MAX / 64) / n) {
+    g.grid_groups = (nb + kXhmGridGroup - 1) / kXhmGridGroup;
+    if (nb > (INT64_MAX / 128) / n) {
         g.bytes = -1;
         return g;
     }
+    const int64_t row = (real == 8 ? (int64_t)sizeof(PowerRow<double>) : (int64_t)sizeof(PowerRow<float>));
     int64_t off = 0;
     g.table = off;
-    off += align256(n * (int64_t)sizeof(PowerRow<float>));
+    off += align256(n * row);
     g.gh22 = off;
-    off += align256(nb * n * 8);
+    off += align256(nb * n * 2 * real);
     g.partial = off;
     off += align256(nb * kXhmGradRows * g.chunks * kXhmGradSlots * 8);
     if (loss) {
         g.cot_plus = off;
-        off += align256(nb * n * 8);
+        off += align256(nb * n * 2 * real);
         g.cot_cross = off;
-        off += align256(nb * n * 8);
+        off += align256(nb * n * 2 * real);
         g.loss_partial = off;
         off += align256(nb * g.loss_chunks * 8);
+    }
+    if (grid) {
+        g.gfm = off;
+        off += align256(kHmModes * nb * n * real);
+        g.grid_partial = off;
+        off += align256(g.grid_groups * n * 8);
     }
     g.bytes = off;
     return g;
 }
+
+template <class T>
+struct Vec2Of;
+template <>
+struct Vec2Of<float> {
+    using type = float2;
+};
+template <>
+struct Vec2Of<double> {
+    using type = double2;
+};
+template <class T>
+using vec2_t = typename Vec2Of<T>::type;
 
 // a double summed over the wave (every lane active), in every lane
 __device__ inline double wave_sum(double x) {
@@ -1477,7 +1506,8 @@ __device__ inline double wave_sum(double x) {
 }
 
 // the sums over the wave of d[0..3] and the closed forms, written by lane 0
-__device__ inline void xhm_grad_store(double* d, float v4, float v5, float v6, int lane, double* out) {
+template <class T>
+__device__ inline void xhm_grad_store(double* d, T v4, T v5, T v6, int lane, double* out) {
     double v[kXhmGradSlots] = {d[0], d[1], d[2], d[3], (double)v4, (double)v5, (double)v6, 0.0};
 #pragma unroll
     for (int j = 0; j < 7; ++j) v[j] = wave_sum(v[j]);
@@ -1486,42 +1516,61 @@ __device__ inline void xhm_grad_store(double* d, float v4, float v5, float v6, i
 }
 
 // mode h's cotangent from the polarizations' (F+, Fx the mode's factors)
-__device__ inline void mode_cotangent(const float* fac, float2 cp, float2 cc, float& g_re, float& g_im) {
+template <class T, class T2>
+__device__ inline void mode_cotangent(const T* fac, T2 cp, T2 cc, T& g_re, T& g_im) {
     g_re = fac[0] * cp.x + fac[1] * cp.y + fac[2] * cc.x + fac[3] * cc.y;
     g_im = fac[0] * cp.y - fac[1] * cp.x + fac[2] * cc.y - fac[3] * cc.x;
 }
 
 // the inclination's closed form: Re(conj(cot+) dF+ h + conj(cotx) dFx h)
-__device__ inline float inclination_term(const float* dfac, float2 cp, float2 cc, float h_re, float h_im) {
-    const float dp_re = dfac[0] * h_re - dfac[1] * h_im, dp_im = dfac[0] * h_im + dfac[1] * h_re;
-    const float dc_re = dfac[2] * h_re - dfac[3] * h_im, dc_im = dfac[2] * h_im + dfac[3] * h_re;
+template <class T, class T2>
+__device__ inline T inclination_term(const T* dfac, T2 cp, T2 cc, T h_re, T h_im) {
+    const T dp_re = dfac[0] * h_re - dfac[1] * h_im, dp_im = dfac[0] * h_im + dfac[1] * h_re;
+    const T dc_re = dfac[2] * h_re - dfac[3] * h_im, dc_im = dfac[2] * h_im + dfac[3] * h_re;
     return cp.x * dp_re + cp.y * dp_im + cc.x * dc_re + cc.y * dc_im;
+}
+
+// a frequency's powers, as the split layout's forward takes them
+__device__ inline PowF32 split_powers(const PowerRow<float>& row) {
+    PowF32 q;
+    q.f = row.f;
+    q.t = row.t;
+    q.m1 = row.m1;
+    q.lf = row.lf;
+    q.m76 = row.m76;
+    q.m53 = row.m53;
+    q.m53_hi = q.m53_lo = 0.0f;
+    return q;
+}
+__device__ inline FreqPowers<double> split_powers(const PowerRow<double>& row) {
+    return FreqPowers<double>{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
 }
 
 // the cotangents 2 w (h - d) of the loss and its value per block (the
 // evaluation's shape; loss_partial[b * chunks + c], summed in a fixed order)
-__global__ __launch_bounds__(kXhmBlock) void xhm_residual_split_kernel(
-    const float* __restrict__ rec22, const float* __restrict__ head, const float* __restrict__ modes, int64_t nb,
-    const float* __restrict__ freqs, int64_t n, int64_t chunks, const float2* __restrict__ data_plus,
-    const float2* __restrict__ data_cross, const float* __restrict__ weights, float2* __restrict__ cot_plus,
-    float2* __restrict__ cot_cross, double* __restrict__ loss_partial) {
+template <class T>
+__global__ __launch_bounds__(kXhmBlock) void xhm_residual_kernel(
+    const T* __restrict__ rec22, const T* __restrict__ head, const T* __restrict__ modes, int64_t nb,
+    const PowerRow<T>* __restrict__ table, int64_t n, int64_t chunks, const vec2_t<T>* __restrict__ data_plus,
+    const vec2_t<T>* __restrict__ data_cross, const T* __restrict__ weights, vec2_t<T>* __restrict__ cot_plus,
+    vec2_t<T>* __restrict__ cot_cross, double* __restrict__ loss_partial) {
     __shared__ double wave_sums[kXhmBlock / 64];
     const int64_t b = blockIdx.x / chunks;
     const int64_t c = blockIdx.x % chunks;
     const int64_t i = c * kXhmBlock + threadIdx.x;
     double l = 0.0;
     if (b < nb && i < n) {
-        float h[5][2];
-        xhm_modes_split(rec22 + b * kSlots, modes + b * kHmModes * kHmSlots, freqs[i], h);
-        float2 hp, hc;
-        xhm_sum_f32(head + b * kHeadSlots, h, hp, hc);
-        const float2 dp = data_plus[b * n + i], dc = data_cross[b * n + i];
-        const float w = weights[i];
-        const float rp_re = hp.x - dp.x, rp_im = hp.y - dp.y, rx_re = hc.x - dc.x, rx_im = hc.y - dc.y;
+        T h[5][2];
+        xhm_modes_split_at(rec22 + b * kSlots, modes + b * kHmModes * kHmSlots, split_powers(table[i]), h);
+        vec2_t<T> hp, hc;
+        xhm_sum_split(head + b * kHeadSlots, h, hp, hc);
+        const vec2_t<T> dp = data_plus[b * n + i], dc = data_cross[b * n + i];
+        const T w = weights[i];
+        const T rp_re = hp.x - dp.x, rp_im = hp.y - dp.y, rx_re = hc.x - dc.x, rx_im = hc.y - dc.y;
         l = (double)(w * (rp_re * rp_re + rp_im * rp_im + rx_re * rx_re + rx_im * rx_im));
-        const float two_w = 2.0f * w;
-        cot_plus[b * n + i] = make_float2(two_w * rp_re, two_w * rp_im);
-        cot_cross[b * n + i] = make_float2(two_w * rx_re, two_w * rx_im);
+        const T two_w = T(2) * w;
+        cot_plus[b * n + i] = vec2_t<T>{two_w * rp_re, two_w * rp_im};
+        cot_cross[b * n + i] = vec2_t<T>{two_w * rx_re, two_w * rx_im};
     }
     l = wave_sum(l);
     if ((threadIdx.x & 63) == 0) wave_sums[threadIdx.x >> 6] = l;
@@ -1533,13 +1582,14 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_residual_split_kernel(
     }
 }
 
-// one wave per (binary, span, higher mode k): the mode's adjoint
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void xhm_hm_adjoint_kernel(
-    const float* __restrict__ rec22, const float* __restrict__ head, const float* __restrict__ dhead,
-    const float* __restrict__ modes, const float* __restrict__ dmodes, int64_t nb,
-    const PowerRow<float>* __restrict__ table, int64_t n, int64_t chunks, int64_t span,
-    const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross, float2* __restrict__ gh22,
-    double* __restrict__ partial) {
+// one wave per (binary, span, higher mode k): the mode's adjoint (and,
+// with gfm, its bins' dL/df)
+template <class T, bool Grid>
+__device__ __forceinline__ void xhm_hm_adjoint_wave(
+    const T* __restrict__ rec22, const T* __restrict__ head, const T* __restrict__ dhead, const T* __restrict__ modes,
+    const T* __restrict__ dmodes, int64_t nb, const PowerRow<T>* __restrict__ table, int64_t n, int64_t chunks,
+    int64_t span, const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
+    vec2_t<T>* __restrict__ gh22, double* __restrict__ partial, T* __restrict__ gfm) {
     const int64_t x = blockIdx.x;
     const int k = (int)(x % kHmModes);
     const int64_t c = (x / kHmModes) % chunks;
@@ -1548,62 +1598,55 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void xh
     const int lane = threadIdx.x;
     // the records in LDS: the loop's many record reads then cost no scalar
     // loads and waits (its registers cannot hold them all)
-    __shared__ float s_mode[kHmSlots], s_22[kSlots];
+    __shared__ T s_mode[kHmSlots], s_22[kSlots];
     for (int j = lane; j < kHmSlots; j += 64) s_mode[j] = modes[(b * kHmModes + k) * kHmSlots + j];
     s_22[lane] = rec22[b * kSlots + lane];
     __syncthreads();
-    const float* r = s_mode;
-    const float* r22 = s_22;
-    const float* fac = head + b * kHeadSlots + kHeadFactors + 4 * (k + 1);
-    const float* dfac = dhead + b * kHeadSlots + kHeadFactors + 4 * (k + 1);
-    const float emm = (float)kHmEmm[k];
-    const bool on = r[kHmOn] != 0.0f;
-    const bool mixing = r[kHmMixing] != 0.0f;
+    const T* r = s_mode;
+    const T* r22 = s_22;
+    const T* fac = head + b * kHeadSlots + kHeadFactors + 4 * (k + 1);
+    const T* dfac = dhead + b * kHeadSlots + kHeadFactors + 4 * (k + 1);
+    const T emm = (T)kHmEmm[k];
+    const bool on = r[kHmOn] != T(0);
+    const bool mixing = r[kHmMixing] != T(0);
     const bool writes = k == kHmMixMode;
-    const float amp0 = r[kHmAmp0];
-    const float inv_amp0 = __builtin_amdgcn_rcpf(fabsf(amp0));
-    const float sign_inv = amp0 < 0.0f ? -inv_amp0 : inv_amp0;
+    const T amp0 = r[kHmAmp0];
+    const T inv_amp0 = d_rcp(fabs(amp0));
+    const T sign_inv = amp0 < T(0) ? -inv_amp0 : inv_amp0;
     const int64_t i0 = c * span;
     const int64_t i1 = i0 + span < n ? i0 + span : n;
-    float acc[kHmAcc];
+    T acc[kHmAcc];
 #pragma unroll
-    for (int j = 0; j < kHmAcc; ++j) acc[j] = 0.0f;
-    float v4 = 0.0f, v5 = 0.0f, v6 = 0.0f;
+    for (int j = 0; j < kHmAcc; ++j) acc[j] = T(0);
+    T v4 = T(0), v5 = T(0), v6 = T(0);
     for (int64_t i = i0 + lane; i < i1; i += 64) {
-        float gw_re = 0.0f, gw_im = 0.0f;
+        T gw_re = T(0), gw_im = T(0), gf = T(0);
         if (on) {
-            const PowerRow<float> row = table[i];
-            const BinPowers<float> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
-            const float2 cp = cot_plus[b * n + i], cc = cot_cross[b * n + i];
-            float g_re, g_im;
+            const PowerRow<T> row = table[i];
+            const BinPowers<T> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
+            const vec2_t<T> cp = cot_plus[b * n + i], cc = cot_cross[b * n + i];
+            T g_re, g_im;
             mode_cotangent(fac, cp, cc, g_re, g_im);
-            float w_re = 0.0f, w_im = 0.0f;
-            const float mf = r[kHmMsec] * p.f;
+            T w_re = T(0), w_im = T(0);
+            const T mf = r[kHmMsec] * p.f;
             if (mixing && (mf >= r[kHmFAmpIm] || mf >= r[kHmFPhaseIm])) {
-                // the (2,2) over |amp0|, as xhm_modes_split computes it
-                PowF32 q;
-                q.f = p.f;
-                q.t = p.t;
-                q.m1 = p.m1;
-                q.lf = p.lf;
-                q.m76 = p.m76;
-                q.m53 = p.m53;
-                q.m53_hi = q.m53_lo = 0.0f;
-                float a22, phi22, s22, c22;
-                amp_phase_f32<kLeadF64>(r22, q, a22, phi22);
-                __sincosf(phi22, &s22, &c22);
+                // the (2,2) over |amp0|, as the forward computes it
+                T a22, phi22, s22, c22;
+                mode22_split(r22, split_powers(row), a22, phi22);
+                m_sincos(phi22, &s22, &c22);
                 w_re = a22 * c22 * inv_amp0;
                 w_im = a22 * s22 * inv_amp0;
             }
-            float h_re, h_im;
-            hm_adjoint_split(r, p, w_re, w_im, g_re, g_im, acc, h_re, h_im, gw_re, gw_im);
+            T h_re, h_im;
+            hm_adjoint_split<Grid>(r, p, w_re, w_im, g_re, g_im, acc, h_re, h_im, gw_re, gw_im, gf);
             v4 += g_re * h_re + g_im * h_im;
             v6 += emm * (g_im * h_re - g_re * h_im);
             v5 += inclination_term(dfac, cp, cc, h_re, h_im);
             // w's dependence on amp0 (w = h_22 / |amp0|), for the mixing mode
             if (mixing) acc[aAmp0] -= (gw_re * w_re + gw_im * w_im) * sign_inv;
         }
-        if (writes) gh22[b * n + i] = make_float2(gw_re * inv_amp0, gw_im * inv_amp0);
+        if (writes) gh22[b * n + i] = vec2_t<T>{gw_re * inv_amp0, gw_im * inv_amp0};
+        if (Grid) gfm[(k * nb + b) * n + i] = gf;
     }
     if (!on) {
         // a mode that vanishes (the odd modes of equal masses): nothing,
@@ -1612,61 +1655,91 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void xh
         return;
     }
     // lane j: the totals of accumulators j and 64 + j
-    float hi[64];
+    T hi[64];
 #pragma unroll
-    for (int j = 0; j < 64; ++j) hi[j] = 64 + j < kHmAcc ? acc[64 + j] : 0.0f;
+    for (int j = 0; j < 64; ++j) hi[j] = 64 + j < kHmAcc ? acc[64 + j] : T(0);
     const double lo_total = (double)reduce_scatter64(acc, lane);
     const double hi_total = (double)reduce_scatter64(hi, lane);
-    const float* t = dmodes + (b * kHmModes + k) * kHmSlots * 4;
+    const T* t = dmodes + (b * kHmModes + k) * kHmSlots * 4;
     double d[4];
-    {
-        const float4 t0 = *reinterpret_cast<const float4*>(t + 4 * kHmAccSlot[lane]);
-        const float tl[4] = {t0.x, t0.y, t0.z, t0.w};
-        for (int j = 0; j < 4; ++j) d[j] = tl[j] == 0.0f ? 0.0 : lo_total * (double)tl[j];
+    for (int j = 0; j < 4; ++j) {
+        const T tl = t[4 * kHmAccSlot[lane] + j];
+        d[j] = tl == T(0) ? 0.0 : lo_total * (double)tl;
     }
     if (64 + lane < kHmAcc) {
-        const float4 t1 = *reinterpret_cast<const float4*>(t + 4 * kHmAccSlot[64 + lane]);
-        const float th[4] = {t1.x, t1.y, t1.z, t1.w};
-        for (int j = 0; j < 4; ++j) d[j] += th[j] == 0.0f ? 0.0 : hi_total * (double)th[j];
+        for (int j = 0; j < 4; ++j) {
+            const T th = t[4 * kHmAccSlot[64 + lane] + j];
+            d[j] += th == T(0) ? 0.0 : hi_total * (double)th;
+        }
     }
     xhm_grad_store(d, v4, v5, v6, lane, partial + ((b * kXhmGradRows + 1 + k) * chunks + c) * kXhmGradSlots);
 }
 
+// FP32: three waves per SIMD (168 registers; the 86 accumulators in
+// registers); FP64: the accumulators and the forward in doubles, one wave
+template <bool Grid>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void xhm_hm_adjoint_kernel(
+    const float* __restrict__ rec22, const float* __restrict__ head, const float* __restrict__ dhead,
+    const float* __restrict__ modes, const float* __restrict__ dmodes, int64_t nb,
+    const PowerRow<float>* __restrict__ table, int64_t n, int64_t chunks, int64_t span,
+    const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross, float2* __restrict__ gh22,
+    double* __restrict__ partial, float* __restrict__ gfm) {
+    xhm_hm_adjoint_wave<float, Grid>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, span, cot_plus,
+                                     cot_cross, gh22, partial, gfm);
+}
+template <bool Grid>
+__global__ __launch_bounds__(64) void xhm_hm_adjoint_f64_kernel(
+    const double* __restrict__ rec22, const double* __restrict__ head, const double* __restrict__ dhead,
+    const double* __restrict__ modes, const double* __restrict__ dmodes, int64_t nb,
+    const PowerRow<double>* __restrict__ table, int64_t n, int64_t chunks, int64_t span,
+    const double2* __restrict__ cot_plus, const double2* __restrict__ cot_cross, double2* __restrict__ gh22,
+    double* __restrict__ partial, double* __restrict__ gfm) {
+    xhm_hm_adjoint_wave<double, Grid>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, span, cot_plus,
+                                      cot_cross, gh22, partial, gfm);
+}
+
 // one wave per (binary, span): the (2,2)'s adjoint (the IMRPhenomXAS
-// record's, with the mode's own cotangent plus the one the (3,2) passes)
+// record's, with the mode's own cotangent plus the one the (3,2) passes);
+// with Grid, each bin's dL/df of the five modes into gfm[0][b][i]
+template <class T, bool Grid>
 __global__ __launch_bounds__(64) void xhm_22_adjoint_kernel(
-    const float* __restrict__ rec22, const double* __restrict__ jac22, const float* __restrict__ head,
-    const float* __restrict__ dhead, int64_t nb, const PowerRow<float>* __restrict__ table, int64_t n,
-    int64_t chunks, int64_t span, const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross,
-    const float2* __restrict__ gh22, double* __restrict__ partial) {
+    const T* __restrict__ rec22, const double* __restrict__ jac22, const T* __restrict__ head,
+    const T* __restrict__ dhead, int64_t nb, const PowerRow<T>* __restrict__ table, int64_t n, int64_t chunks,
+    int64_t span, const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
+    const vec2_t<T>* __restrict__ gh22, double* __restrict__ partial, T* __restrict__ gfm) {
     const int64_t b = blockIdx.x / chunks;
     const int64_t c = blockIdx.x % chunks;
     if (b >= nb) return;
     const int lane = threadIdx.x;
-    float rec[kSlots];
+    T rec[kSlots];
 #pragma unroll
     for (int k = 0; k < kSlots; ++k) rec[k] = rec22[b * kSlots + k];
-    float acc[kSlots];
+    T acc[kSlots];
 #pragma unroll
-    for (int k = 0; k < kSlots; ++k) acc[k] = 0.0f;
-    const float* fac = head + b * kHeadSlots + kHeadFactors;
-    const float* dfac = dhead + b * kHeadSlots + kHeadFactors;
+    for (int k = 0; k < kSlots; ++k) acc[k] = T(0);
+    const T* fac = head + b * kHeadSlots + kHeadFactors;
+    const T* dfac = dhead + b * kHeadSlots + kHeadFactors;
     const int64_t i0 = c * span;
     const int64_t i1 = i0 + span < n ? i0 + span : n;
-    float v4 = 0.0f, v5 = 0.0f, v6 = 0.0f;
+    T v4 = T(0), v5 = T(0), v6 = T(0);
     for (int64_t i = i0 + lane; i < i1; i += 64) {
-        const PowerRow<float> row = table[i];
-        const BinPowers<float> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
-        const float2 cp = cot_plus[b * n + i], cc = cot_cross[b * n + i];
-        float g_re, g_im;
+        const PowerRow<T> row = table[i];
+        const BinPowers<T> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
+        const vec2_t<T> cp = cot_plus[b * n + i], cc = cot_cross[b * n + i];
+        T g_re, g_im;
         mode_cotangent(fac, cp, cc, g_re, g_im);
-        const float2 extra = gh22[b * n + i];
-        float amp, s, co;
-        adjoint_point_k(rec, p, 1.0f, 0.0f, g_re + extra.x, g_im + extra.y, 0.0f, 0.0f, acc, amp, s, co);
-        const float h_re = amp * co, h_im = amp * s;
+        const vec2_t<T> extra = gh22[b * n + i];
+        T amp, s, co;
+        adjoint_point_k(rec, p, T(1), T(0), g_re + extra.x, g_im + extra.y, T(0), T(0), acc, amp, s, co);
+        const T h_re = amp * co, h_im = amp * s;
         v4 += g_re * h_re + g_im * h_im;
-        v6 += 2.0f * (g_im * h_re - g_re * h_im);
+        v6 += T(2) * (g_im * h_re - g_re * h_im);
         v5 += inclination_term(dfac, cp, cc, h_re, h_im);
+        if (Grid) {
+            T gf = grid_adjoint_point_k(rec, p, T(1), T(0), g_re + extra.x, g_im + extra.y, T(0), T(0));
+            for (int k = 0; k < kHmModes; ++k) gf += gfm[(k * nb + b) * n + i];
+            gfm[b * n + i] = gf;
+        }
     }
     double total = (double)reduce_scatter64(acc, lane);
     // slots without a derivative here: the factors (the mode's are the
@@ -1676,6 +1749,31 @@ __global__ __launch_bounds__(64) void xhm_22_adjoint_kernel(
     double d[4];
     for (int j = 0; j < 4; ++j) d[j] = jrow[j] == 0.0 ? 0.0 : total * jrow[j];
     xhm_grad_store(d, v4, v5, v6, lane, partial + (b * kXhmGradRows * chunks + c) * kXhmGradSlots);
+}
+
+// dL/df: partial[g][i] = the sum over binaries g * 64 .. g * 64 + 63 of
+// gf[b][i] in order, then dldf[i] = the sum over the groups in order (no
+// atomics: the same bits on every run)
+template <class T>
+__global__ __launch_bounds__(kBlock) void xhm_grid_sum_kernel(const T* __restrict__ gf, int64_t nb, int64_t n,
+                                                              int64_t groups, double* __restrict__ grid_partial) {
+    const int64_t tiles = (n + kBlock - 1) / kBlock;
+    const int64_t g = blockIdx.x / tiles;
+    const int64_t i = (blockIdx.x % tiles) * kBlock + threadIdx.x;
+    if (g >= groups || i >= n) return;
+    const int64_t b0 = g * kXhmGridGroup;
+    const int64_t b1 = b0 + kXhmGridGroup < nb ? b0 + kXhmGridGroup : nb;
+    double s = 0.0;
+    for (int64_t b = b0; b < b1; ++b) s += (double)gf[b * n + i];
+    grid_partial[g * n + i] = s;
+}
+__global__ __launch_bounds__(kBlock) void xhm_grid_total_kernel(const double* __restrict__ grid_partial,
+                                                                int64_t groups, int64_t n, double* __restrict__ dldf) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int64_t g = 0; g < groups; ++g) s += grid_partial[g * n + i];
+    dldf[i] = s;
 }
 
 // each binary's gradient (and loss) from the waves' sums, in a fixed order:
@@ -1770,7 +1868,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 21; }
+int wfx_abi_version(void) { return 22; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -2085,20 +2183,96 @@ int wfx_xhm_setup_jvp(const double* params, int64_t n_binaries, const double* f_
     return check_launch("wfx_xhm_setup_jvp");
 }
 
-int64_t wfx_xhm_adjoint_workspace(int64_t n_binaries, int64_t n_freqs, int loss) {
+int64_t wfx_xhm_adjoint_workspace(int64_t n_binaries, int64_t n_freqs, int precision, int loss, int grid) {
     if (n_binaries <= 0 || n_freqs <= 0) return 0;
-    return xhm_grad_plan(n_binaries, n_freqs, loss != 0).bytes;
+    if (precision != kPrecF64 && precision != kPrecSplit) return -1;
+    return xhm_grad_plan(n_binaries, n_freqs, loss != 0, grid != 0, precision == kPrecF64 ? 8 : 4).bytes;
 }
+
+}  // extern "C"
+
+namespace {
+
+// the launches of wfx_xhm_adjoint in T (float: split, double: FP64)
+template <class T>
+void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* records22, const double* jacobian22,
+                        const T* head, const T* dhead, const T* modes, const T* dmodes, int64_t nb, const T* freqs,
+                        int64_t n, const void* in_plus, const void* in_cross, const T* weights, char* ws,
+                        double* loss, double* grads, double* dldf, hipStream_t s, int table_blocks, int hm_blocks,
+                        int blocks22, int loss_blocks, int grid_blocks_1, int grid_blocks_2) {
+    using V = vec2_t<T>;
+    PowerRow<T>* table = (PowerRow<T>*)(ws + g.table);
+    V* gh22 = (V*)(ws + g.gh22);
+    double* partial = (double*)(ws + g.partial);
+    T* gfm = dldf ? (T*)(ws + g.gfm) : nullptr;
+    const V* cot_plus = (const V*)in_plus;
+    const V* cot_cross = (const V*)in_cross;
+    power_table_kernel<T, T><<<table_blocks, kBlock, 0, s>>>(freqs, n, table);
+    if (weights) {
+        V* res_plus = (V*)(ws + g.cot_plus);
+        V* res_cross = (V*)(ws + g.cot_cross);
+        xhm_residual_kernel<T><<<loss_blocks, kXhmBlock, 0, s>>>(records22, head, modes, nb, table, n, g.loss_chunks,
+                                                                 cot_plus, cot_cross, weights, res_plus, res_cross,
+                                                                 (double*)(ws + g.loss_partial));
+        cot_plus = res_plus;
+        cot_cross = res_cross;
+    }
+    if (std::is_same<T, float>::value) {
+        if (dldf)
+            xhm_hm_adjoint_kernel<true><<<hm_blocks, 64, 0, s>>>(
+                (const float*)records22, (const float*)head, (const float*)dhead, (const float*)modes,
+                (const float*)dmodes, nb, (const PowerRow<float>*)table, n, g.chunks, g.span, (const float2*)cot_plus,
+                (const float2*)cot_cross, (float2*)gh22, partial, (float*)gfm);
+        else
+            xhm_hm_adjoint_kernel<false><<<hm_blocks, 64, 0, s>>>(
+                (const float*)records22, (const float*)head, (const float*)dhead, (const float*)modes,
+                (const float*)dmodes, nb, (const PowerRow<float>*)table, n, g.chunks, g.span, (const float2*)cot_plus,
+                (const float2*)cot_cross, (float2*)gh22, partial, nullptr);
+    } else {
+        if (dldf)
+            xhm_hm_adjoint_f64_kernel<true><<<hm_blocks, 64, 0, s>>>(
+                (const double*)records22, (const double*)head, (const double*)dhead, (const double*)modes,
+                (const double*)dmodes, nb, (const PowerRow<double>*)table, n, g.chunks, g.span,
+                (const double2*)cot_plus, (const double2*)cot_cross, (double2*)gh22, partial, (double*)gfm);
+        else
+            xhm_hm_adjoint_f64_kernel<false><<<hm_blocks, 64, 0, s>>>(
+                (const double*)records22, (const double*)head, (const double*)dhead, (const double*)modes,
+                (const double*)dmodes, nb, (const PowerRow<double>*)table, n, g.chunks, g.span,
+                (const double2*)cot_plus, (const double2*)cot_cross, (double2*)gh22, partial, nullptr);
+    }
+    if (dldf)
+        xhm_22_adjoint_kernel<T, true><<<blocks22, 64, 0, s>>>(records22, jacobian22, head, dhead, nb, table, n,
+                                                               g.chunks, g.span, cot_plus, cot_cross, gh22, partial,
+                                                               gfm);
+    else
+        xhm_22_adjoint_kernel<T, false><<<blocks22, 64, 0, s>>>(records22, jacobian22, head, dhead, nb, table, n,
+                                                                g.chunks, g.span, cot_plus, cot_cross, gh22, partial,
+                                                                nullptr);
+    xhm_adjoint_finish_kernel<<<(int)((nb + 63) / 64), 64, 0, s>>>(
+        params, jacobian22, nb, partial, g.chunks, weights ? (const double*)(ws + g.loss_partial) : nullptr,
+        g.loss_chunks, weights ? loss : nullptr, grads);
+    if (dldf) {
+        double* grid_partial = (double*)(ws + g.grid_partial);
+        xhm_grid_sum_kernel<T><<<grid_blocks_1, kBlock, 0, s>>>(gfm, nb, n, g.grid_groups, grid_partial);
+        xhm_grid_total_kernel<<<grid_blocks_2, kBlock, 0, s>>>(grid_partial, g.grid_groups, n, dldf);
+    }
+}
+
+}  // namespace
+
+extern "C" {
 
 int wfx_xhm_adjoint(const double* params, const void* records22, const double* jacobian22, const void* head,
                     const void* dhead, const void* modes, const void* dmodes, int64_t n_binaries, const void* freqs,
                     int64_t n_freqs, int precision, const void* in_plus, const void* in_cross, const void* weights,
-                    void* workspace, int64_t workspace_bytes, double* loss, double* grads, void* stream) {
+                    void* workspace, int64_t workspace_bytes, double* loss, double* grads, double* dldf,
+                    void* stream) {
     if (n_binaries < 0 || n_freqs < 0) return fail("wfx_xhm_adjoint: negative size");
     if (n_binaries == 0) return 0;
     if (!params || !records22 || !jacobian22 || !head || !dhead || !modes || !dmodes || !grads)
         return fail("wfx_xhm_adjoint: null pointer");
-    if (precision != kPrecSplit) return fail("wfx_xhm_adjoint: precision must be 2 (split)");
+    if (precision != kPrecF64 && precision != kPrecSplit)
+        return fail("wfx_xhm_adjoint: precision must be 0 (FP64) or 2 (split)");
     if (weights && !loss) return fail("wfx_xhm_adjoint: the loss needs its output");
     hipStream_t s = (hipStream_t)stream;
     if (n_freqs == 0) {
@@ -2111,41 +2285,31 @@ int wfx_xhm_adjoint(const double* params, const void* records22, const double* j
         return check_launch("wfx_xhm_adjoint");
     }
     if (!freqs || !in_plus || !in_cross) return fail("wfx_xhm_adjoint: null pointer");
-    const XhmGradPlan g = xhm_grad_plan(n_binaries, n_freqs, weights != nullptr);
+    const bool f64 = precision == kPrecF64;
+    const XhmGradPlan g = xhm_grad_plan(n_binaries, n_freqs, weights != nullptr, dldf != nullptr, f64 ? 8 : 4);
     if (g.bytes < 0) return fail("wfx_xhm_adjoint: sizes too large");
     if (!workspace || workspace_bytes < g.bytes || (uintptr_t)workspace % 256)
         return fail("wfx_xhm_adjoint: the workspace must be 256-aligned and hold wfx_xhm_adjoint_workspace() bytes");
-    char* ws = (char*)workspace;
-    PowerRow<float>* table = (PowerRow<float>*)(ws + g.table);
-    float2* gh22 = (float2*)(ws + g.gh22);
-    double* partial = (double*)(ws + g.partial);
-    const float2* cot_plus = (const float2*)in_plus;
-    const float2* cot_cross = (const float2*)in_cross;
-    const int table_blocks = grid_blocks((n_freqs + kBlock - 1) / kBlock, "wfx_xhm_adjoint: grid too large");
+    const int64_t tiles = (n_freqs + kBlock - 1) / kBlock;
+    const int table_blocks = grid_blocks(tiles, "wfx_xhm_adjoint: grid too large");
     const int hm_blocks = grid_blocks(n_binaries * g.chunks * kHmModes, "wfx_xhm_adjoint: grid too large");
     const int blocks22 = grid_blocks(n_binaries * g.chunks, "wfx_xhm_adjoint: grid too large");
     const int loss_blocks = grid_blocks(n_binaries * g.loss_chunks, "wfx_xhm_adjoint: grid too large");
-    if (table_blocks < 0 || hm_blocks < 0 || blocks22 < 0 || loss_blocks < 0) return -1;
-    power_table_kernel<float, float><<<table_blocks, kBlock, 0, s>>>((const float*)freqs, n_freqs, table);
-    if (weights) {
-        float2* res_plus = (float2*)(ws + g.cot_plus);
-        float2* res_cross = (float2*)(ws + g.cot_cross);
-        xhm_residual_split_kernel<<<loss_blocks, kXhmBlock, 0, s>>>(
-            (const float*)records22, (const float*)head, (const float*)modes, n_binaries, (const float*)freqs,
-            n_freqs, g.loss_chunks, cot_plus, cot_cross, (const float*)weights, res_plus, res_cross,
-            (double*)(ws + g.loss_partial));
-        cot_plus = res_plus;
-        cot_cross = res_cross;
-    }
-    xhm_hm_adjoint_kernel<<<hm_blocks, 64, 0, s>>>((const float*)records22, (const float*)head, (const float*)dhead,
-                                                   (const float*)modes, (const float*)dmodes, n_binaries, table,
-                                                   n_freqs, g.chunks, g.span, cot_plus, cot_cross, gh22, partial);
-    xhm_22_adjoint_kernel<<<blocks22, 64, 0, s>>>((const float*)records22, jacobian22, (const float*)head,
-                                                  (const float*)dhead, n_binaries, table, n_freqs, g.chunks, g.span,
-                                                  cot_plus, cot_cross, gh22, partial);
-    xhm_adjoint_finish_kernel<<<(int)((n_binaries + 63) / 64), 64, 0, s>>>(
-        params, jacobian22, n_binaries, partial, g.chunks, weights ? (const double*)(ws + g.loss_partial) : nullptr,
-        g.loss_chunks, weights ? loss : nullptr, grads);
+    const int grid_1 = grid_blocks(g.grid_groups * tiles, "wfx_xhm_adjoint: grid too large");
+    if (table_blocks < 0 || hm_blocks < 0 || blocks22 < 0 || loss_blocks < 0 || grid_1 < 0) return -1;
+    char* ws = (char*)workspace;
+    if (f64)
+        xhm_adjoint_launch<double>(g, params, (const double*)records22, jacobian22, (const double*)head,
+                                   (const double*)dhead, (const double*)modes, (const double*)dmodes, n_binaries,
+                                   (const double*)freqs, n_freqs, in_plus, in_cross, (const double*)weights, ws, loss,
+                                   grads, dldf, s, table_blocks, hm_blocks, blocks22, loss_blocks, grid_1,
+                                   table_blocks);
+    else
+        xhm_adjoint_launch<float>(g, params, (const float*)records22, jacobian22, (const float*)head,
+                                  (const float*)dhead, (const float*)modes, (const float*)dmodes, n_binaries,
+                                  (const float*)freqs, n_freqs, in_plus, in_cross, (const float*)weights, ws, loss,
+                                  grads, dldf, s, table_blocks, hm_blocks, blocks22, loss_blocks, grid_1,
+                                  table_blocks);
     return check_launch("wfx_xhm_adjoint");
 }
 

@@ -174,6 +174,42 @@ __device__ inline float reduce_f32(float x) {
     return fmaf(-turns, kTwoPiLoF, fmaf(-turns, kTwoPiHiF, x));
 }
 
+// One template body for both arithmetics: the FP32 kernels' fast forms
+// (hardware reciprocal, exponential, sine and cosine on |x| <= pi, the
+// fitted arctangent) and their FP64 library counterparts.
+__device__ inline void m_sincos(float x, float* s, float* c) { __sincosf(x, s, c); }
+__device__ inline void m_sincos(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ inline float d_atan(float x) { return fast_atan(x); }
+__device__ inline double d_atan(double x) { return atan(x); }
+__device__ inline float d_atan2(float y, float x) { return atan2f(y, x); }
+__device__ inline double d_atan2(double y, double x) { return atan2(y, x); }
+__device__ inline float d_exp(float x) { return __expf(x); }
+__device__ inline double d_exp(double x) { return exp(x); }
+__device__ inline float d_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double d_rcp(double x) { return 1.0 / x; }
+__device__ inline float d_div(float a, float b) { return fast_div(a, b); }
+__device__ inline double d_div(double a, double b) { return a / b; }
+__device__ inline float d_sqrt(float x) { return __builtin_sqrtf(x); }
+__device__ inline double d_sqrt(double x) { return sqrt(x); }
+__device__ inline float d_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline double d_rsq(double x) { return 1.0 / sqrt(x); }
+__device__ inline float d_fma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ inline double d_fma(double a, double b, double c) { return fma(a, b, c); }
+// the smallest normal number: below it the FP32 forms' reciprocals flush
+__device__ inline float tiny_normal(float) { return 1.17549435e-38f; }
+__device__ inline double tiny_normal(double) { return 2.2250738585072014e-308; }
+// x reduced to [-pi, pi]: FP32 as reduce_f32, FP64 with 2 pi in two parts
+// (its low part 2 pi - fl(2 pi))
+__device__ inline float d_reduce(float x) { return reduce_f32(x); }
+__device__ inline double d_reduce(double x) {
+    const double turns = rint(x * kInvTwoPi);
+    return fma(-turns, 2.4492935982947064e-16, fma(-turns, kTwoPi, x));
+}
+// a record's FP64 slot: two floats (low word first) in a FP32 record, one
+// double in a FP64 record
+__device__ inline double wide_slot(const float* r, int slot) { return record_f64(r, slot); }
+__device__ inline double wide_slot(const double* r, int slot) { return r[slot]; }
+
 // the amplitude and the phase (reduced to [-pi, pi]) of the (2,2) mode in FP32
 template <int L>
 __device__ inline void amp_phase_f32(const float* __restrict__ c, const PowF32& p, float& amp, float& phi) {

@@ -1244,9 +1244,16 @@ WFX_HD inline R fold_two_pi(const R& x) {
 }
 WFX_HD inline void store_f64(float* r, int slot, double v) { __builtin_memcpy(r + slot, &v, sizeof(double)); }
 
-template <class R>
-WFX_HD inline void write_mode_record_split(const HmCoeffs<R>& c, const WaveformVars<R>& wf, float* r,
-                                           float* tangent = nullptr) {
+WFX_HD inline void store_f64(double* r, int slot, double v) {
+    r[slot] = v;
+    r[slot + 1] = 0.0;
+}
+
+// O = float: the split evaluation's record; O = double: the same layout in
+// FP64 (the FP64 gradient's), the FP64 slots as one double and a zero
+template <class R, class O>
+WFX_HD inline void write_mode_record_split(const HmCoeffs<R>& c, const WaveformVars<R>& wf, O* r,
+                                           O* tangent = nullptr) {
     R d[kHmSlots];
     write_mode_record(c, wf, d);
     d[kHmPhi] = R(0.0);  // (the leading term is the FP64 one)
@@ -1264,15 +1271,15 @@ WFX_HD inline void write_mode_record_split(const HmCoeffs<R>& c, const WaveformV
     const Powers<R> pm = useful_powers(wf.m_sec);
     const R t0 = c.phi[0] * pm.m_five_thirds;
     const R k_ins = c.c_insp + c.deltaphi;
-    for (int s = 0; s < kHmSlots; ++s) r[s] = (float)value_of(d[s]);
+    for (int s = 0; s < kHmSlots; ++s) r[s] = (O)value_of(d[s]);
     store_f64(r, kHmT0F64, value_of(t0));
     store_f64(r, kHmKInsF64, value_of(k_ins));
     if (tangent) {
-        for (int s = 0; s < kHmSlots; ++s) tangent[s] = (float)tangent_of(d[s]);
-        tangent[kHmT0F64] = (float)tangent_of(t0);
-        tangent[kHmT0F64 + 1] = 0.0f;
-        tangent[kHmKInsF64] = (float)tangent_of(k_ins);
-        tangent[kHmKInsF64 + 1] = 0.0f;
+        for (int s = 0; s < kHmSlots; ++s) tangent[s] = (O)tangent_of(d[s]);
+        tangent[kHmT0F64] = (O)tangent_of(t0);
+        tangent[kHmT0F64 + 1] = O(0);
+        tangent[kHmKInsF64] = (O)tangent_of(k_ins);
+        tangent[kHmKInsF64 + 1] = O(0);
     }
 }
 

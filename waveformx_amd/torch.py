@@ -17,8 +17,8 @@ keyword ``arithmetic`` ("fp64", "split", "split-df", "fp32"; IMRPhenomD has
 no "fp32": its table is FP64) overrides the choice. Tensors on the CPU are
 computed on the current HIP device and the results copied back.
 Approximants: IMRPhenomXAS, IMRPhenomD and IMRPhenomXHM (arithmetics
-"fp64" and "split"; its backward, to the binary's fields but not the grid,
-differentiates the FP64 path numerically: ``xhm.vjp``).
+"fp64" and "split"; its backward is the reverse-mode adjoint of
+``xhm.adjoint``, to the binary's fields and the grid).
 
 The backward pass (``torch.autograd.Function``, once differentiable, the
 cotangents conjugated as in ``torch.py:108-119``) runs the model's
@@ -112,9 +112,6 @@ class _ModelFunction(torch.autograd.Function):
         approximant, amp_scale, reference_mf, arithmetic = ctx.key
         grid, params = ctx.saved_tensors
         wrt_grid = ctx.needs_input_grad[1]
-        if approximant == "IMRPhenomXHM" and wrt_grid:
-            raise NotImplementedError("the gradient of IMRPhenomXHM with respect to the frequencies is not "
-                                      "implemented in waveformx_amd")
         grads, grad_grid = adjoint.vjp(params, grid.contiguous(), grad_hplus, grad_hcross, amp_scale,
                                        reference_mf, arithmetic, wrt_grid, approximant)
         dtype = grid.dtype

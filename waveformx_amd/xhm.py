@@ -31,7 +31,8 @@ import torch
 from ._lib import call, lib
 from .xas import RECORD_SLOTS, _f_min, _ptr, _require_device, _stream
 
-__all__ = ["MODES", "LAL_FLAGS", "PRECISIONS", "setup", "evaluate", "polarizations", "modes"]
+__all__ = ["MODES", "LAL_FLAGS", "PRECISIONS", "setup", "setup_jvp", "evaluate", "polarizations", "modes",
+           "adjoint", "vjp", "vjp_differences"]
 
 #: the modes (l, m) of the outputs, the (2,2) first (``xhm/reference.py:55-57``)
 MODES = ((2, 2), (2, 1), (3, 3), (3, 2), (4, 4))
@@ -76,14 +77,16 @@ def setup(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21, re
 
 def setup_jvp(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21,
               reference_mf: Optional[float] = None, precision: str = "fp64") -> dict:
-    """The set-up of :func:`setup` with its derivatives, in dual numbers
-    through the same FP64 code: ``records22``, ``head``, ``modes``, ``flags``
-    as :func:`setup`'s; ``jac22`` (B, 64, 8) float64, the (2,2) record's
+    """The gradient's set-up with its derivatives, in dual numbers through
+    the same FP64 code as :func:`setup`: ``records22``, ``head``, ``flags``
+    as :func:`setup`'s; ``modes`` the higher modes' records in the split
+    evaluation's layout (float32: :func:`setup`'s ``"split"`` records;
+    float64 for ``"fp64"``: the same slots in FP64, not the FP64
+    evaluation's LAL form); ``jac22`` (B, 64, 8) float64, the (2,2) record's
     derivatives in the 8 parameters (IMRPhenomXAS's derivative setup: the
-    same record); ``dhead`` (B, H), the head's derivative in the inclination;
-    ``dmodes`` (B, 4, S, 4), the higher modes' records' derivatives in
-    (m1, m2, chi1z, chi2z) (the last axis; of the split records' slots for
-    ``"split"``, in float32)."""
+    same record); ``dhead`` (B, H), the head's derivative in the
+    inclination; ``dmodes`` (B, 4, S, 4), the higher modes' records'
+    derivatives in (m1, m2, chi1z, chi2z) (the last axis)."""
     from . import xas
 
     if precision not in PRECISIONS:
@@ -294,11 +297,14 @@ def _factor_derivatives(inclination) -> Tuple:
     return dp, dc
 
 
-def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_cross: torch.Tensor,
-        amp_scale: float = 1e21, reference_mf: Optional[float] = None) -> torch.Tensor:
-    """dL/d(m1, m2, chi1z, chi2z, d_L, inclination, phi_ref, f_ref) (B, 8)
-    float64 of the FP64 polarizations, for the cotangents dL/dRe h + i dL/dIm h
-    (B, N) of h+ and hx: Re sum_f conj(cot) dh/dtheta. The masses' and
+def vjp_differences(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_cross: torch.Tensor,
+                    amp_scale: float = 1e21, reference_mf: Optional[float] = None) -> torch.Tensor:
+    """The FP64 gradient by differences, an independent check of
+    :func:`vjp`'s exact one (another arithmetic: the FP64 evaluation's LAL
+    form, differenced): dL/d(m1, m2, chi1z, chi2z, d_L, inclination, phi_ref,
+    f_ref) (B, 8) float64 of the FP64 polarizations, for the cotangents
+    dL/dRe h + i dL/dIm h (B, N) of h+ and hx: Re sum_f conj(cot) dh/dtheta.
+    The masses' and
     spins' derivatives are fourth-order central differences of the FP64
     set-up and evaluation with every region selection (bounds, branches)
     held at the parameters, the reference's jnp.where semantics, so that
@@ -398,36 +404,58 @@ def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_c
 
 # ------------------------------------------------ the exact gradient
 
-def adjoint_split(params: torch.Tensor, freqs: torch.Tensor, in_plus: torch.Tensor, in_cross: torch.Tensor,
-                  weights: Optional[torch.Tensor] = None, amp_scale: float = 1e21,
-                  reference_mf: Optional[float] = None) -> Tuple[torch.Tensor, Optional[torch.Tensor]]:
-    """The exact gradient (B, 8) of the split polarizations on the float32
-    grid ``freqs``: with ``weights`` None, ``in_plus``/``in_cross`` are the
-    cotangents (B, N) (dL/dRe + i dL/dIm) and the loss is None; else they are
-    the data and the loss (B,) is ``sum_f w (|h+ - d+|^2 + |hx - dx|^2)``.
+def adjoint(params: torch.Tensor, freqs: torch.Tensor, in_plus: torch.Tensor, in_cross: torch.Tensor,
+            weights: Optional[torch.Tensor] = None, amp_scale: float = 1e21, reference_mf: Optional[float] = None,
+            precision: str = "split", wrt_grid: bool = False
+            ) -> Tuple[torch.Tensor, Optional[torch.Tensor], Optional[torch.Tensor]]:
+    """The exact gradient ``(grads (B, 8), loss (B,) or None, dldf (N,) or
+    None)``, float64, of the polarizations on the grid ``freqs``: FP32 on
+    the float32 grid (``"split"``) or FP64 on the float64 grid (``"fp64"``),
+    in the split evaluation's layout (the same formula in either scalar).
+    With ``weights`` None, ``in_plus``/``in_cross`` are the cotangents
+    (B, N) (dL/dRe + i dL/dIm) and the loss is None; else they are the data
+    and the loss is ``sum_f w (|h+ - d+|^2 + |hx - dx|^2)``. ``wrt_grid``:
+    also dL/df of each frequency (the grid's gradient, summed over the
+    binaries; the set-up's f_min held, as the reference's stop_gradient).
     One dual-number set-up (the records and their derivatives) and
     ``wfx_xhm_adjoint``: reverse mode per mode and span of frequencies, each
     record's slots' totals contracted once with their derivatives."""
+    if precision not in PRECISIONS:
+        raise ValueError(f"IMRPhenomXHM's arithmetic is one of {PRECISIONS}, got {precision!r}")
     _require_device(params, "params")
     params = params.to(torch.float64).contiguous()
-    f32 = freqs.to(torch.float32).contiguous()
-    nb, n = params.shape[0], f32.shape[0]
-    in_plus = in_plus.to(torch.complex64).reshape(nb, n).contiguous()
-    in_cross = in_cross.to(torch.complex64).reshape(nb, n).contiguous()
+    rdtype = torch.float64 if precision == "fp64" else torch.float32
+    ctype = torch.complex128 if precision == "fp64" else torch.complex64
+    grid = freqs.to(rdtype).contiguous()
+    nb, n = params.shape[0], grid.shape[0]
+    in_plus = in_plus.to(ctype).reshape(nb, n).contiguous()
+    in_cross = in_cross.to(ctype).reshape(nb, n).contiguous()
+    dev = params.device
     if weights is not None:
-        weights = weights.to(torch.float32).contiguous()
+        weights = weights.to(rdtype).contiguous()
         if weights.shape != (n,):
             raise ValueError(f"weights must be ({n},)")
+    dldf = torch.zeros((n,), dtype=torch.float64, device=dev) if wrt_grid else None
     if nb == 0 or n == 0:
-        zeros = torch.zeros((nb, 8), dtype=torch.float64, device=params.device)
-        return zeros, (None if weights is None else torch.zeros((nb,), dtype=torch.float64, device=params.device))
-    j = setup_jvp(params, f32[:1], amp_scale, reference_mf, "split")
-    dev = params.device
-    workspace = torch.empty((lib().wfx_xhm_adjoint_workspace(nb, n, int(weights is not None)),), dtype=torch.uint8,
-                            device=dev)
+        zeros = torch.zeros((nb, 8), dtype=torch.float64, device=dev)
+        return zeros, (None if weights is None else torch.zeros((nb,), dtype=torch.float64, device=dev)), dldf
+    j = setup_jvp(params, grid[:1].to(torch.float64), amp_scale, reference_mf, precision)
+    code = _CODES[precision]
+    size = lib().wfx_xhm_adjoint_workspace(nb, n, code, int(weights is not None), int(wrt_grid))
+    workspace = torch.empty((size,), dtype=torch.uint8, device=dev)
     grads = torch.empty((nb, 8), dtype=torch.float64, device=dev)
     loss = None if weights is None else torch.empty((nb,), dtype=torch.float64, device=dev)
     call("wfx_xhm_adjoint", _ptr(params), _ptr(j["records22"]), _ptr(j["jac22"]), _ptr(j["head"]), _ptr(j["dhead"]),
-         _ptr(j["modes"]), _ptr(j["dmodes"]), nb, _ptr(f32), n, 2, _ptr(in_plus), _ptr(in_cross), _ptr(weights),
-         _ptr(workspace), workspace.numel(), _ptr(loss), _ptr(grads), _stream(dev))
-    return grads, (None if weights is None else loss)
+         _ptr(j["modes"]), _ptr(j["dmodes"]), nb, _ptr(grid), n, code, _ptr(in_plus), _ptr(in_cross), _ptr(weights),
+         _ptr(workspace), workspace.numel(), _ptr(loss), _ptr(grads), _ptr(dldf), _stream(dev))
+    return grads, loss, dldf
+
+
+def vjp(params: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Tensor, cot_cross: torch.Tensor,
+        amp_scale: float = 1e21, reference_mf: Optional[float] = None) -> torch.Tensor:
+    """dL/d(m1, m2, chi1z, chi2z, d_L, inclination, phi_ref, f_ref) (B, 8)
+    float64 of the FP64 polarizations on the float64 grid ``freqs``, for
+    the cotangents dL/dRe h + i dL/dIm h (B, N) of h+ and hx: the exact FP64
+    gradient (:func:`adjoint` in FP64). f_ref's derivative is zero where
+    f_ref is zero or with ``reference_mf``."""
+    return adjoint(params, freqs.to(torch.float64), cot_plus, cot_cross, None, amp_scale, reference_mf, "fp64")[0]
