@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (22); a binding checks it before any call. */
+/* The version of this ABI (23); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -265,25 +265,32 @@ int wfx_xhm_eval(const void* records22, const void* head, const void* modes, int
                  int freqs_f64, int64_t n_freqs, int precision, void* hplus, void* hcross, void* mode_out,
                  void* stream);
 
-/* wfx_xhm_setup_jvp: the head and the higher-mode records of the gradient
- * (same arguments as wfx_xhm_setup) with their derivatives, forward mode
- * through the same FP64 set-up: the records in the split evaluation's
- * layout (each mode's leading inspiral term and inspiral constant as an
- * FP64 slot, the later regions' constants reduced modulo 2 pi, quotients
- * folded): float for precision 2 (wfx_xhm_setup's split records, the FP64
- * slots as two floats), double for precision 0 (the same slots in FP64, an
- * FP64 slot's second word zero; the FP64 evaluation's records are
- * wfx_xhm_setup's, in LAL's form). dhead (B, wfx_xhm_head_slots()) the
- * head's derivative in the inclination, dmodes (B, 4 modes,
- * wfx_xhm_record_slots(), 4 directions) the records' derivatives in
- * (mass_1, mass_2, chi_1z, chi_2z) (an FP64 slot's in its first word). The
- * (2,2) record and its derivatives come from wfx_xas_setup_jvp (the same
- * record). The forward half of the reference's jax.vjp for IMRPhenomXHM
+/* wfx_xhm_setup_jvp: the gradient's set-up of IMRPhenomXHM (same arguments
+ * as wfx_xhm_setup) with its derivatives, forward mode through the same
+ * FP64 set-up: records22 (B, 64) the (2,2)'s IMRPhenomXAS record (double
+ * for precision 0, the split float record for 2), jacobian22 (B, 64, 8)
+ * double its derivatives and flags (B,) LAL's flag bits, as
+ * wfx_xas_setup_jvp writes them; the head and the higher-mode records in
+ * the split evaluation's layout (each mode's leading inspiral term and
+ * inspiral constant as an FP64 slot, the later regions' constants reduced
+ * modulo 2 pi, quotients folded): float for precision 2 (wfx_xhm_setup's
+ * split records, the FP64 slots as two floats), double for precision 0 (the
+ * same slots in FP64, an FP64 slot's second word zero; the FP64
+ * evaluation's records are wfx_xhm_setup's, in LAL's form). dhead (B,
+ * wfx_xhm_head_slots()) the head's derivative in the inclination, dmodes
+ * (B, 4 modes, wfx_xhm_record_slots(), 4 directions) the records'
+ * derivatives in (mass_1, mass_2, chi_1z, chi_2z) (an FP64 slot's in its
+ * first word). The higher modes take the (2,2)'s coefficients from its
+ * set-up through workspace: device memory of at least
+ * wfx_xhm_setup_jvp_workspace(B) bytes, 256-aligned (-1: too many
+ * binaries). The forward half of the reference's jax.vjp for IMRPhenomXHM
  * (src/waveformx/torch.py:87-93; the MLX port's tables_jvp,
  * src/waveformx/models/xhm/mlx/_kernels.py:551-603). */
+int64_t wfx_xhm_setup_jvp_workspace(int64_t n_binaries);
 int wfx_xhm_setup_jvp(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
-                      double amp_scale, double reference_mf, int precision, void* head, void* dhead, void* modes,
-                      void* dmodes, void* stream);
+                      double amp_scale, double reference_mf, int precision, void* records22, double* jacobian22,
+                      uint32_t* flags, void* head, void* dhead, void* modes, void* dmodes, void* workspace,
+                      int64_t workspace_bytes, void* stream);
 
 /* wfx_xhm_adjoint: grads (B, 8) double, the gradient in (mass_1, mass_2,
  * chi_1z, chi_2z, luminosity_distance, inclination, phi_ref, f_ref) of the

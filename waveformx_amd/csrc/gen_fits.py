@@ -12,7 +12,10 @@ with long literal expressions are transcribed the same way
 
 Every function is ``template <class R>`` over the setup's scalar: double
 (the FP64 and split-precision paths), float (plain FP32) or a forward-mode
-dual (gradients). Literal-only subexpressions are evaluated in double and
+dual (gradients); one that takes the amplitude's or the phase's coefficient
+struct is ``template <class R, class C = R>`` with the struct of ``C``, so
+that a Taylor scalar in the frequency (``R``) can take the struct of the
+setup's scalar (``C``) as constants, without a lifted copy. Literal-only subexpressions are evaluated in double and
 narrowed once to ``R``, as JAX does with Python floats (weak types) in a
 float32 evaluation. ``fma(a, b, c)`` is the reference's unfused ``a * b +
 c`` (``xpnr/_math.py:125-137``); the library is compiled with contraction
@@ -52,7 +55,17 @@ BLOCKS = [
      ["phi2_ns", "phi3_ns", "phi3_s", "phi4_ns", "phi4_s", "phi5l_ns", "phi5l_s", "phi6_ns", "phi6_s", "phi6l_ns",
       "phi7_ns", "phi7_s", "phi8_s", "phi8l_s"], {"phi8_common"}),
 ]
-STRUCTS = {"powers": "Powers<R>", "amp": "AmpCoeffs<R>", "phase": "PhaseCoeffs<R>", "wf": "WaveformVars<R>"}
+STRUCTS = {"powers": "Powers<R>", "amp": "AmpCoeffs<C>", "phase": "PhaseCoeffs<C>", "wf": "WaveformVars<R>"}
+#: the coefficient structs of their own scalar C
+COEFFICIENT_STRUCTS = ("amp", "phase")
+
+
+def template_head(args) -> str:
+    """``template <class R>``, or ``template <class R, class C = R>`` for a
+    function taking a coefficient struct."""
+    if any(a.arg in COEFFICIENT_STRUCTS for a in args):
+        return "template <class R, class C = R>"
+    return "template <class R>"
 CONSTANTS = {"PI": "kPi", "FRAC_PI_2": "(kPi / 2.0)", "FRAC_PI_4": "(kPi / 4.0)",
              "EGAMMA": "kEulerGamma", "LN_2": "kLn2"}
 MATH = {"cbrt": "m_cbrt", "sqrt": "m_sqrt", "exp": "m_exp", "abs": "m_abs", "arctan": "m_atan", "power": "m_pow",
@@ -157,7 +170,7 @@ class Writer:
         self.static = set()
         params = [f"const {STRUCTS[a.arg]}& {a.arg}" if a.arg in STRUCTS else f"R {a.arg}" for a in fn.args.args]
         lines = [f"// {module}.py:{fn.lineno}",
-                 f"template <class R> WFX_HD inline R {self.prefix}_{fn.name}({', '.join(params)}) {{"]
+                 f"{template_head(fn.args.args)} WFX_HD inline R {self.prefix}_{fn.name}({', '.join(params)}) {{"]
         declared = {a.arg for a in fn.args.args}
         for stmt in fn.body:
             self.statement(stmt, declared, lines)
@@ -311,7 +324,7 @@ class HmWriter(Writer):
         params = [f"const {STRUCTS[a.arg]}& {a.arg}" if a.arg in STRUCTS else
                   (f"int {a.arg}" if a.arg == "mode_tag" else f"R {a.arg}") for a in fn.args.args]
         lines = [f"// {module}.py:{fn.lineno}",
-                 f"template <class R> WFX_HD inline R {self.prefix}_{fn.name}({', '.join(params)}) {{"]
+                 f"{template_head(fn.args.args)} WFX_HD inline R {self.prefix}_{fn.name}({', '.join(params)}) {{"]
         declared = {a.arg for a in fn.args.args}
         for stmt in fn.body:
             self.statement(stmt, declared, lines)

@@ -175,9 +175,10 @@ __global__ __launch_bounds__(64) void setup_f32_kernel(const double* __restrict_
 struct XasRecord {
     static constexpr int kFoldA = kIntConst, kFoldB = kRdConst;
     template <class R>
-    __device__ static unsigned record(R* p, const R& f_min, double amp_scale, double reference_mf, R* rec) {
+    __device__ static unsigned record(R* p, const R& f_min, double amp_scale, double reference_mf, R* rec,
+                                      XasCoeffs22<R>* keep = nullptr) {
         with_reference_mf(p, reference_mf);
-        return xas_record<R>(p, f_min, amp_scale, rec);
+        return xas_record<R>(p, f_min, amp_scale, rec, keep);
     }
     // The record's derivative with respect to parameter dir (d_L,
     // inclination, phi_ref or f_ref) from its FP64 values v, into col[k *
@@ -266,13 +267,16 @@ struct PhenomDRecord {
 // whole setup in dual numbers; the thread of direction j also writes the
 // column of parameter j + 4, which enters the record only through a few
 // slots (S::closed_column). The direction-0 thread writes the record, FP64
-// (records) and/or FP32 split (split_records), either NULL.
-template <class S>
+// (records) and/or FP32 split (split_records), either NULL. With Keep
+// (IMRPhenomXAS), the thread of (b, dir) also writes its coefficient structs
+// in dual numbers to keep[4 b + dir], for IMRPhenomXHM's higher modes.
+template <class S, bool Keep = false>
 __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict__ params, int64_t nb,
                                                        const double* __restrict__ f_min, int64_t f_min_stride,
                                                        double amp_scale, double reference_mf,
                                                        double* __restrict__ records, float* __restrict__ split_records,
-                                                       double* __restrict__ jac, uint32_t* __restrict__ flags) {
+                                                       double* __restrict__ jac, uint32_t* __restrict__ flags,
+                                                       XasCoeffs22<Dual<1>>* __restrict__ keep = nullptr) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t b = gid >> 2;
     const int dir = (int)(gid & 3);
@@ -284,7 +288,14 @@ __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict_
         p[k].d[0] = k == dir ? 1.0 : 0.0;
     }
     D rec[kSlots];
-    const unsigned fl = S::template record<D>(p, D(f_min[b * f_min_stride]), amp_scale, reference_mf, rec);
+    unsigned fl;
+    if constexpr (Keep) {
+        XasCoeffs22<D> kept;
+        fl = S::template record<D>(p, D(f_min[b * f_min_stride]), amp_scale, reference_mf, rec, &kept);
+        keep[gid] = kept;
+    } else {
+        fl = S::template record<D>(p, D(f_min[b * f_min_stride]), amp_scale, reference_mf, rec);
+    }
     double* out = jac + b * kSlots * 8;
     for (int k = 0; k < kSlots; ++k) out[k * 8 + dir] = k < 60 ? rec[k].d[0] : 0.0;
     double v[kSlots];
@@ -1250,6 +1261,7 @@ __global__ __launch_bounds__(64) void xhm_setup_kernel(const double* __restrict_
 __global__ __launch_bounds__(64) void xhm_setup_jvp_kernel(const double* __restrict__ params, int64_t nb,
                                                            const double* __restrict__ f_min, int64_t f_min_stride,
                                                            double amp_scale, double reference_mf, int split,
+                                                           const XasCoeffs22<Dual<1>>* __restrict__ kept,
                                                            void* __restrict__ head, void* __restrict__ dhead,
                                                            void* __restrict__ modes, void* __restrict__ dmodes) {
     const int64_t per_group = (nb + 63) / 64;
@@ -1285,17 +1297,12 @@ __global__ __launch_bounds__(64) void xhm_setup_jvp_kernel(const double* __restr
     }
     const int k = g >> 2, dir = g & 3;
     using D = Dual<1>;
-    D p[kParams];
-    for (int i = 0; i < kParams; ++i) {
-        p[i] = D(params[b * kParams + i]);
-        p[i].d[0] = i == dir ? 1.0 : 0.0;
-    }
-    with_reference_mf(p, reference_mf);
-    const WaveformVars<D> wf = waveform_vars(p, D(f_min[b * f_min_stride]), amp_scale);
-    const AmpCoeffs<D> amp22 = amp_coeffs(wf);
-    const PhaseCoeffs<D> ph22 = phase_coeffs(wf);
+    // the (2,2)'s coefficients in this direction, from its derivative
+    // set-up (the same arithmetic)
+    const XasCoeffs22<D> c22 = kept[4 * b + dir];
+    const WaveformVars<D>& wf = c22.wf;
     HmCoeffs<D> c;
-    hm_setup(wf, amp22, ph22, k, c);
+    hm_setup(wf, c22.amp, c22.ph, k, c);
     const int64_t at = (b * kHmModes + k) * kHmSlots;
     if (split) {
         float values[kHmSlots], tangents[kHmSlots];
@@ -1868,7 +1875,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 22; }
+int wfx_abi_version(void) { return 23; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -2165,21 +2172,40 @@ int wfx_xhm_setup(const double* params, int64_t n_binaries, const double* f_min,
     return check_launch("wfx_xhm_setup");
 }
 
+int64_t wfx_xhm_setup_jvp_workspace(int64_t n_binaries) {
+    if (n_binaries <= 0) return 0;
+    if (n_binaries > INT64_MAX / (4 * (int64_t)sizeof(XasCoeffs22<Dual<1>>))) return -1;
+    return 4 * n_binaries * (int64_t)sizeof(XasCoeffs22<Dual<1>>);
+}
+
 int wfx_xhm_setup_jvp(const double* params, int64_t n_binaries, const double* f_min, int64_t f_min_stride,
-                      double amp_scale, double reference_mf, int precision, void* head, void* dhead, void* modes,
-                      void* dmodes, void* stream) {
+                      double amp_scale, double reference_mf, int precision, void* records22, double* jacobian22,
+                      uint32_t* flags, void* head, void* dhead, void* modes, void* dmodes, void* workspace,
+                      int64_t workspace_bytes, void* stream) {
     if (n_binaries < 0) return fail("wfx_xhm_setup_jvp: negative number of binaries");
     if (n_binaries == 0) return 0;
-    if (!params || !f_min || !head || !dhead || !modes || !dmodes) return fail("wfx_xhm_setup_jvp: null pointer");
+    if (!params || !f_min || !records22 || !jacobian22 || !head || !dhead || !modes || !dmodes)
+        return fail("wfx_xhm_setup_jvp: null pointer");
     if (f_min_stride != 0 && f_min_stride != 1) return fail("wfx_xhm_setup_jvp: f_min_stride must be 0 or 1");
     if (precision != kPrecF64 && precision != kPrecSplit)
         return fail("wfx_xhm_setup_jvp: precision must be 0 (FP64) or 2 (split)");
+    const int64_t need = wfx_xhm_setup_jvp_workspace(n_binaries);
+    if (need < 0) return fail("wfx_xhm_setup_jvp: too many binaries");
+    if (!workspace || workspace_bytes < need || (uintptr_t)workspace % 256)
+        return fail("wfx_xhm_setup_jvp: the workspace must be 256-aligned and hold wfx_xhm_setup_jvp_workspace() "
+                    "bytes");
+    const int blocks22 = grid_blocks((4 * n_binaries + 63) / 64, "wfx_xhm_setup_jvp: too many binaries");
     const int blocks =
         grid_blocks((4 * kHmModes + 1) * ((n_binaries + 63) / 64), "wfx_xhm_setup_jvp: too many binaries");
-    if (blocks < 0) return -1;
-    xhm_setup_jvp_kernel<<<blocks, 64, 0, (hipStream_t)stream>>>(params, n_binaries, f_min, f_min_stride, amp_scale,
-                                                                 reference_mf, precision == kPrecSplit, head, dhead,
-                                                                 modes, dmodes);
+    if (blocks22 < 0 || blocks < 0) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    XasCoeffs22<Dual<1>>* kept = (XasCoeffs22<Dual<1>>*)workspace;
+    const bool split = precision == kPrecSplit;
+    setup_jvp_kernel<XasRecord, true><<<blocks22, 64, 0, s>>>(
+        params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf, split ? nullptr : (double*)records22,
+        split ? (float*)records22 : nullptr, jacobian22, flags, kept);
+    xhm_setup_jvp_kernel<<<blocks, 64, 0, s>>>(params, n_binaries, f_min, f_min_stride, amp_scale, reference_mf,
+                                               split, kept, head, dhead, modes, dmodes);
     return check_launch("wfx_xhm_setup_jvp");
 }
 

@@ -633,8 +633,18 @@ WFX_HD inline void fold_term(R* rec, int slot, int log_slot, const R& c, const R
 // order of XasParam, f_min the first frequency of its grid; returns LAL's
 // error flags. T0 and the anchor A land in rec[kT0], rec[kAnchor] in R
 // (the FP32-split path runs this in double and splits them afterwards).
+// The (2,2)'s coefficient structs of a set-up, which IMRPhenomXHM's
+// higher-mode set-up takes from the (2,2)'s instead of recomputing them
 template <class R>
-WFX_HD inline unsigned xas_record(const R* params, const R& f_min, double amp_scale, R* rec) {
+struct XasCoeffs22 {
+    WaveformVars<R> wf;
+    AmpCoeffs<R> amp;
+    PhaseCoeffs<R> ph;
+};
+
+template <class R>
+WFX_HD inline unsigned xas_record(const R* params, const R& f_min, double amp_scale, R* rec,
+                                   XasCoeffs22<R>* keep = nullptr) {
     WaveformVars<R> wf = waveform_vars(params, f_min, amp_scale);
     R s = wf.m_sec;
     R one = R(1.0);
@@ -664,9 +674,14 @@ WFX_HD inline unsigned xas_record(const R* params, const R& f_min, double amp_sc
         rec[kAmpRdG] = amp.gamma_d2;
         rec[kFAmpIn] = amp.f_amp_match_in / s;
         rec[kFAmpRd] = amp.f_amp_rd_min / s;
+        if (keep) keep->amp = amp;
     }
 
     PhaseCoeffs<R> ph = phase_coeffs(wf);
+    if (keep) {
+        keep->wf = wf;
+        keep->ph = ph;
+    }
     R linb = time_shift(ph, wf);
     R inveta = R(1.0) / wf.eta;
     Powers<R> p = useful_powers(s);

@@ -66,20 +66,46 @@ WFX_HD inline Jet2<R> operator/(const Jet2<R>& a, const Jet2<R>& b) {
     const R dq = (a.d - q * b.d) / b.v;
     return Jet2<R>(q, dq, (a.dd - 2.0 * dq * b.d - q * b.dd) / b.v);
 }
-#define WFX_JET_MIXED(op)                                                                              \
-    template <class R, class S, typename std::enable_if<!IsJet<S>::value, int>::type = 0>              \
-    WFX_HD inline Jet2<R> operator op(const Jet2<R>& a, const S& b) {                                 \
-        return a op Jet2<R>(R(b));                                                                     \
-    }                                                                                                  \
-    template <class R, class S, typename std::enable_if<!IsJet<S>::value, int>::type = 0>              \
-    WFX_HD inline Jet2<R> operator op(const S& a, const Jet2<R>& b) {                                 \
-        return Jet2<R>(R(a)) op b;                                                                     \
-    }
-WFX_JET_MIXED(+)
-WFX_JET_MIXED(-)
-WFX_JET_MIXED(*)
-WFX_JET_MIXED(/)
-#undef WFX_JET_MIXED
+// a Jet2 and a constant (a scalar without Taylor parts): the operations of
+// the Jet2 forms above with the constant's zero parts left out (equal
+// values for finite operands), so that a struct of constants enters a Jet2
+// evaluation without a lifted copy
+template <class S>
+using IfNotJet = typename std::enable_if<!IsJet<S>::value, int>::type;
+template <class R, class S, IfNotJet<S> = 0>
+WFX_HD inline Jet2<R> operator+(const Jet2<R>& a, const S& b) {
+    return Jet2<R>(a.v + b, a.d, a.dd);
+}
+template <class R, class S, IfNotJet<S> = 0>
+WFX_HD inline Jet2<R> operator+(const S& a, const Jet2<R>& b) {
+    return Jet2<R>(a + b.v, b.d, b.dd);
+}
+template <class R, class S, IfNotJet<S> = 0>
+WFX_HD inline Jet2<R> operator-(const Jet2<R>& a, const S& b) {
+    return Jet2<R>(a.v - b, a.d, a.dd);
+}
+template <class R, class S, IfNotJet<S> = 0>
+WFX_HD inline Jet2<R> operator-(const S& a, const Jet2<R>& b) {
+    return Jet2<R>(a - b.v, -b.d, -b.dd);
+}
+template <class R, class S, IfNotJet<S> = 0>
+WFX_HD inline Jet2<R> operator*(const Jet2<R>& a, const S& b) {
+    return Jet2<R>(a.v * b, a.d * b, a.dd * b);
+}
+template <class R, class S, IfNotJet<S> = 0>
+WFX_HD inline Jet2<R> operator*(const S& a, const Jet2<R>& b) {
+    return Jet2<R>(a * b.v, a * b.d, a * b.dd);
+}
+template <class R, class S, IfNotJet<S> = 0>
+WFX_HD inline Jet2<R> operator/(const Jet2<R>& a, const S& b) {
+    return Jet2<R>(a.v / b, a.d / b, a.dd / b);
+}
+template <class R, class S, IfNotJet<S> = 0>
+WFX_HD inline Jet2<R> operator/(const S& a, const Jet2<R>& b) {
+    const R q = a / b.v;
+    const R dq = -(q * b.d) / b.v;
+    return Jet2<R>(q, dq, (-(2.0 * dq * b.d) - q * b.dd) / b.v);
+}
 #define WFX_JET_CMP(op)                                                                                \
     template <class R>                                                                                 \
     WFX_HD inline bool operator op(const Jet2<R>& a, const Jet2<R>& b) {                               \
@@ -195,19 +221,6 @@ WFX_HD inline R int_pow(const R& x, int n) {
     return acc;
 }
 
-// a coefficient struct of R as the same struct of Jet2<R> constants
-template <template <class> class S, class R>
-WFX_HD inline S<Jet2<R>> lift_jet(const S<R>& s) {
-    static_assert(sizeof(S<R>) % sizeof(R) == 0, "a struct of R");
-    constexpr int n = sizeof(S<R>) / sizeof(R);
-    static_assert(sizeof(S<Jet2<R>>) == n * sizeof(Jet2<R>), "a struct of Jet2<R>");
-    S<Jet2<R>> out;
-    const R* a = reinterpret_cast<const R*>(&s);
-    Jet2<R>* b = reinterpret_cast<Jet2<R>*>(&out);
-    for (int i = 0; i < n; ++i) b[i] = Jet2<R>(a[i]);
-    return out;
-}
-
 #include "xhm_fits.inc"
 
 // --------------------------------------------------------------- records
@@ -246,16 +259,16 @@ WFX_CONST int kHmEmm[kHmModes] = {1, 3, 2, 4};
 // ------------------------------------------------------------ the (2,2)
 
 // phase22.py:304-307 IMRPhenomX_Phase_22 (Mf >= fPhaseMatchIM: ringdown)
-template <class R>
-WFX_HD inline R phase_22(const R& mf, const Powers<R>& p, const PhaseCoeffs<R>& ph) {
+template <class R, class C = R>
+WFX_HD inline R phase_22(const R& mf, const Powers<R>& p, const PhaseCoeffs<C>& ph) {
     if (mf < ph.f_phase_match_in) return ins_phase_22_ansatz_int_exact(mf, p, ph);
     if (mf >= ph.f_phase_match_im)
         return m_fma(ph.c2_mrd, mf, rd_phase_22_ansatz_int_exact(mf, p, ph) + ph.c1_mrd);
     return m_fma(ph.c2_int, mf, itm_phase_22_ansatz_int_exact(mf, p, ph) + ph.c1_int);
 }
 // phase22.py:310-316 IMRPhenomX_dPhase_22
-template <class R>
-WFX_HD inline R dphase_22(const R& mf, const Powers<R>& p, const PhaseCoeffs<R>& ph) {
+template <class R, class C = R>
+WFX_HD inline R dphase_22(const R& mf, const Powers<R>& p, const PhaseCoeffs<C>& ph) {
     if (mf < ph.f_phase_match_in) return ins_phase_22_ansatz_exact(mf, p, ph);
     if (mf >= ph.f_phase_match_im) return rd_phase_22_ansatz_exact(mf, p, ph) + ph.c2_mrd;
     return itm_phase_22_ansatz_exact(mf, p, ph) + ph.c2_int;
@@ -579,9 +592,10 @@ WFX_HD inline void rotate(const R& mf, const Powers<R>& p, const HmCoeffs<C>& c,
     out_im = a_im + (b_re * sn + b_im * cs);
 }
 // reference.py:381-388 SpheroidalToSpherical with the (2,2) ringdown ansatz
+// (mf a Taylor scalar over the set-up's C, the (2,2)'s coefficients of C)
 template <class R, class C>
-WFX_HD inline void spheroidal_to_spherical(const R& mf, const HmCoeffs<C>& c, const AmpCoeffs<R>& amp22,
-                                           const PhaseCoeffs<R>& ph22, R& re, R& im) {
+WFX_HD inline void spheroidal_to_spherical(const R& mf, const HmCoeffs<C>& c, const AmpCoeffs<C>& amp22,
+                                           const PhaseCoeffs<C>& ph22, R& re, R& im) {
     const Powers<R> p = useful_powers(mf);
     const R a22 = rd_amp_22_ansatz_exact(mf, R(c.f_ring22), R(c.f_damp22), amp22) * c.amp_norm * p.m_seven_sixths;
     const R phi22 = m_fma(R(c.inveta), phase_22(mf, p, ph22), c.timeshift * mf) + c.phiref22;
@@ -733,8 +747,8 @@ WFX_HD inline void rd_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
 // reference.py:275-334 the intermediate amplitude (a polynomial in u times
 // f^(-7/6) through the collocation points)
 template <class R>
-WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<R>& wf, const AmpCoeffs<Jet2<R>>& amp22j,
-                                           const PhaseCoeffs<Jet2<R>>& ph22j, HmCoeffs<R>& c) {
+WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<R>& wf, const AmpCoeffs<R>& amp22,
+                                           const PhaseCoeffs<R>& ph22, HmCoeffs<R>& c) {
     // IMRPhenomXHMIntermediateAmpVersion (reference.py:59-60): 0 skipped,
     // 1 value, 2 value and derivative
     const int v21[6] = {1, 1, 0, 1, 0, 2}, vhm[6] = {2, 1, 1, 1, 1, 2};
@@ -768,7 +782,7 @@ WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<R>& wf, const AmpC
     R rd_value, rd_derivative;
     if (c.mixing) {
         Jet2<R> re, im;
-        spheroidal_to_spherical(Jet2<R>(f_rd, R(1.0), R(0.0)), c, amp22j, ph22j, re, im);
+        spheroidal_to_spherical(Jet2<R>(f_rd, R(1.0), R(0.0)), c, amp22, ph22, re, im);
         const Jet2<R> h = m_hypot(re, im);
         rd_value = h.v;
         rd_derivative = h.d;
@@ -929,7 +943,7 @@ WFX_HD inline R phase_fit(const WaveformVars<R>& wf, int tag, int i) {
 // derivatives of the rotated (3,2) ringdown phase)
 template <class R>
 WFX_HD inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<R>& ph22,
-                                   const AmpCoeffs<Jet2<R>>& amp22j, const PhaseCoeffs<Jet2<R>>& ph22j,
+                                   const AmpCoeffs<R>& amp22,
                                    HmCoeffs<R>& c) {
     const int emm = c.emm, tag = c.tag;
     const R eta = wf.eta, eta_m1 = 1.0 / eta;
@@ -1037,7 +1051,7 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<
     } else {
         // the rotated ringdown phase with its first and second derivatives
         Jet2<R> re, im;
-        spheroidal_to_spherical(Jet2<R>(fcut_rd, R(1.0), R(0.0)), c, amp22j, ph22j, re, im);
+        spheroidal_to_spherical(Jet2<R>(fcut_rd, R(1.0), R(0.0)), c, amp22, ph22, re, im);
         const Jet2<R> rotated = m_atan2(im, re);
         phi0_rd = rotated.v;
         dphi0_rd = rotated.d;
@@ -1147,10 +1161,8 @@ WFX_HD inline void hm_setup(const WaveformVars<R>& wf, const AmpCoeffs<R>& amp22
     pn_amp_coeffs(wf, c);
     inspiral_amp_coeffs(wf, c);
     rd_amp_coeffs(wf, c);
-    const AmpCoeffs<Jet2<R>> amp22j = lift_jet(amp22);
-    const PhaseCoeffs<Jet2<R>> ph22j = lift_jet(ph22);
-    intermediate_amp_coeffs(wf, amp22j, ph22j, c);
-    phase_coeffs_hm(wf, ph22, amp22j, ph22j, c);
+    intermediate_amp_coeffs(wf, amp22, ph22, c);
+    phase_coeffs_hm(wf, ph22, amp22, c);
 }
 
 // the record of a mode

@@ -86,9 +86,9 @@ def setup_jvp(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21
     derivatives in the 8 parameters (IMRPhenomXAS's derivative setup: the
     same record); ``dhead`` (B, H), the head's derivative in the
     inclination; ``dmodes`` (B, 4, S, 4), the higher modes' records'
-    derivatives in (m1, m2, chi1z, chi2z) (the last axis)."""
-    from . import xas
-
+    derivatives in (m1, m2, chi1z, chi2z) (the last axis). One call
+    (``wfx_xhm_setup_jvp``): the (2,2)'s derivative set-up hands its
+    coefficients to the higher modes' through a workspace."""
     if precision not in PRECISIONS:
         raise ValueError(f"IMRPhenomXHM's arithmetic is one of {PRECISIONS}, got {precision!r}")
     _require_device(params, "params")
@@ -96,20 +96,23 @@ def setup_jvp(params: torch.Tensor, f_min: torch.Tensor, amp_scale: float = 1e21
         raise ValueError("params must be a (B, 8) float64 tensor")
     params = params.contiguous()
     nb = params.shape[0]
-    records22, jac22, flags = xas.run_setup_jvp("wfx_xas_setup_jvp", params, f_min, amp_scale, reference_mf,
-                                                precision)
     f_min_t, stride = _f_min(f_min, nb)
     handle = lib()
     dev = params.device
     dtype = torch.float64 if precision == "fp64" else torch.float32
     slots = handle.wfx_xhm_record_slots()
+    records22 = torch.empty((nb, RECORD_SLOTS), dtype=dtype, device=dev)
+    jac22 = torch.empty((nb, RECORD_SLOTS, 8), dtype=torch.float64, device=dev)
+    flags = torch.empty((nb,), dtype=torch.int32, device=dev)
     head = torch.empty((nb, handle.wfx_xhm_head_slots()), dtype=dtype, device=dev)
     dhead = torch.empty_like(head)
     modes_ = torch.empty((nb, 4, slots), dtype=dtype, device=dev)
     dmodes = torch.empty((nb, 4, slots, 4), dtype=dtype, device=dev)
+    workspace = torch.empty((max(handle.wfx_xhm_setup_jvp_workspace(nb), 0),), dtype=torch.uint8, device=dev)
     call("wfx_xhm_setup_jvp", _ptr(params), nb, _ptr(f_min_t), stride, float(amp_scale),
-         float("nan") if reference_mf is None else float(reference_mf), _CODES[precision], _ptr(head), _ptr(dhead),
-         _ptr(modes_), _ptr(dmodes), _stream(dev))
+         float("nan") if reference_mf is None else float(reference_mf), _CODES[precision], _ptr(records22),
+         _ptr(jac22), _ptr(flags), _ptr(head), _ptr(dhead), _ptr(modes_), _ptr(dmodes), _ptr(workspace),
+         workspace.numel(), _stream(dev))
     return {"records22": records22, "jac22": jac22, "head": head, "dhead": dhead, "modes": modes_,
             "dmodes": dmodes, "flags": flags}
 
