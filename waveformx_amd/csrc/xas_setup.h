@@ -239,8 +239,9 @@ WFX_HD inline AmpCoeffs<R> amp_coeffs(const WaveformVars<R>& wf) {
 // the candidate rows, so that every index is a compile-time constant once
 // the loops unroll (an exchange through the runtime pivot index puts the
 // arrays in scratch memory). The arithmetic, and its result, is that of the
-// plain exchange, which dual numbers keep: their setup is bound by
-// register pressure, which the unrolled selects raise.
+// plain exchange. Dual numbers solve for their values the same way and for
+// their tangents from the values' factors (pivoted_solve_implicit); other
+// scalars (Taylor jets) keep the plain exchange.
 template <class R, int N>
 WFX_HD inline void pivoted_solve_exchange(R (&a)[N][N], R (&b)[N]) {
     for (int k = 0; k < N; ++k) {
@@ -320,10 +321,114 @@ WFX_HD inline void pivoted_solve_select(R (&a)[N][N], R (&b)[N]) {
     }
 }
 
+// Dual numbers: the values by pivoted_solve_select's arithmetic (so the
+// same bits as the plain exchange), the multipliers kept in place of the
+// eliminated entries and the row exchanges applied to whole rows (P A = L U),
+// then each direction's tangent by the implicit-function identity
+// A dx = db - dA x with the same factors: no elimination on the tangents,
+// and every index a compile-time constant (the arrays stay in registers).
+template <int M, int N>
+WFX_HD inline void pivoted_solve_implicit(Dual<M> (&a)[N][N], Dual<M> (&b)[N]) {
+    double lu[N][N], x[N];
+    int pivots[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        x[i] = b[i].v;
+#pragma unroll
+        for (int j = 0; j < N; ++j) lu[i][j] = a[i][j].v;
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        int piv = k;
+        double big = fabs(lu[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) {
+            const double m = fabs(lu[i][k]);
+            if (m > big) {
+                big = m;
+                piv = i;
+            }
+        }
+        pivots[k] = piv;
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) {
+            const bool swap = piv == i;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const double t = lu[k][j];
+                lu[k][j] = swap ? lu[i][j] : t;
+                lu[i][j] = swap ? t : lu[i][j];
+            }
+            const double t = x[k];
+            x[k] = swap ? x[i] : t;
+            x[i] = swap ? t : x[i];
+        }
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) {
+            const double l = lu[i][k] / lu[k][k];
+#pragma unroll
+            for (int j = k + 1; j < N; ++j) lu[i][j] = lu[i][j] - l * lu[k][j];
+            x[i] = x[i] - l * x[k];
+            lu[i][k] = l;
+        }
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        double s = x[i];
+#pragma unroll
+        for (int j = i + 1; j < N; ++j) s = s - lu[i][j] * x[j];
+        x[i] = s / lu[i][i];
+    }
+#pragma unroll
+    for (int d = 0; d < M; ++d) {
+        // r = db - dA x in the original row order, then P, L and U
+        double r[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double s = b[i].d[d];
+#pragma unroll
+            for (int j = 0; j < N; ++j) s = s - a[i][j].d[d] * x[j];
+            r[i] = s;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+#pragma unroll
+            for (int i = k + 1; i < N; ++i) {
+                const bool swap = pivots[k] == i;
+                const double t = r[k];
+                r[k] = swap ? r[i] : t;
+                r[i] = swap ? t : r[i];
+            }
+        }
+#pragma unroll
+        for (int i = 1; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < i; ++j) r[i] = r[i] - lu[i][j] * r[j];
+#pragma unroll
+        for (int i = N - 1; i >= 0; --i) {
+            double s = r[i];
+#pragma unroll
+            for (int j = i + 1; j < N; ++j) s = s - lu[i][j] * r[j];
+            r[i] = s / lu[i][i];
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) b[i].d[d] = r[i];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) b[i].v = x[i];
+}
+
+template <class R>
+struct IsDualNumber : std::false_type {};
+template <int M>
+struct IsDualNumber<Dual<M>> : std::true_type {};
+
 template <class R, int N>
 WFX_HD inline void pivoted_solve(R (&a)[N][N], R (&b)[N]) {
     if constexpr (std::is_floating_point<R>::value)
         pivoted_solve_select(a, b);
+    else if constexpr (IsDualNumber<R>::value)
+        pivoted_solve_implicit(a, b);
     else
         pivoted_solve_exchange(a, b);
 }
@@ -629,10 +734,6 @@ WFX_HD inline void fold_term(R* rec, int slot, int log_slot, const R& c, const R
     }
 }
 
-// The record of one binary (xas/core.py:257-379): params in the packed
-// order of XasParam, f_min the first frequency of its grid; returns LAL's
-// error flags. T0 and the anchor A land in rec[kT0], rec[kAnchor] in R
-// (the FP32-split path runs this in double and splits them afterwards).
 // The (2,2)'s coefficient structs of a set-up, which IMRPhenomXHM's
 // higher-mode set-up takes from the (2,2)'s instead of recomputing them
 template <class R>
@@ -642,6 +743,11 @@ struct XasCoeffs22 {
     PhaseCoeffs<R> ph;
 };
 
+// The record of one binary (xas/core.py:257-379): params in the packed
+// order of XasParam, f_min the first frequency of its grid; returns LAL's
+// error flags. T0 and the anchor A land in rec[kT0], rec[kAnchor] in R
+// (the FP32-split path runs this in double and splits them afterwards); with
+// keep, the coefficient structs are also kept there.
 template <class R>
 WFX_HD inline unsigned xas_record(const R* params, const R& f_min, double amp_scale, R* rec,
                                    XasCoeffs22<R>* keep = nullptr) {
