@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (23); a binding checks it before any call. */
+/* The version of this ABI (24); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -264,6 +264,22 @@ int wfx_xhm_setup(const double* params, int64_t n_binaries, const double* f_min,
 int wfx_xhm_eval(const void* records22, const void* head, const void* modes, int64_t n_binaries, const void* freqs,
                  int freqs_f64, int64_t n_freqs, int precision, void* hplus, void* hcross, void* mode_out,
                  void* stream);
+
+/* wfx_xhm_eval_ragged: the binaries on their own grids (the reference's
+ * per-binary frequencies (B, N), src/waveformx/models/xhm/mlx/_kernels.py:
+ * 565-576, generalised to a length per binary): binary b owns [offsets[b],
+ * offsets[b + 1]) of the concatenated grid freqs (float32 or, for FP64,
+ * float64 if freqs_f64) or, with freqs NULL, the uniform grid grid_f0[b] +
+ * k grid_df[b] of that length (computed in FP64); hplus, hcross (total,)
+ * and, if mode_out is not null, the modes (5, total), as wfx_xhm_eval
+ * writes them for one binary on its grid (the same bits). tiles (B + 1,)
+ * from wfx_xas_tile_prefix(offsets, B, 1, tiles) (tiles of 256 bins);
+ * blocks the persistent grid's size (0: 8 per CU). offsets must start at
+ * 0 and not decrease (checked on the host by the binding). */
+int wfx_xhm_eval_ragged(const void* records22, const void* head, const void* modes, int64_t n_binaries,
+                        const void* freqs, int freqs_f64, const double* grid_f0, const double* grid_df,
+                        const int64_t* offsets, const int64_t* tiles, int precision, int blocks, void* hplus,
+                        void* hcross, void* mode_out, void* stream);
 
 /* wfx_xhm_setup_jvp: the gradient's set-up of IMRPhenomXHM (same arguments
  * as wfx_xhm_setup) with its derivatives, forward mode through the same

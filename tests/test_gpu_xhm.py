@@ -684,3 +684,131 @@ def test_against_lalsimulation_aligned_xpnr(cuda):
         for got, ref in ((got_p, hp), (got_c, hc)):
             mism, err = mismatch_and_error(got.cpu().numpy()[0], ref)
             assert mism < 5e-6 and err < 5e-3, (index, mism, err)
+
+
+def own_grids(params, rng):
+    """Per-binary uniform grids as the ragged workload draws them: f_lower
+    ~ U[10, 30] Hz, df = 1/T with T ~ log-U[4, 16] s, up to min(2048 Hz,
+    0.3/M)."""
+    grids = []
+    for p in params:
+        f_lower = rng.uniform(10.0, 30.0)
+        df = 1.0 / np.exp(rng.uniform(np.log(4.0), np.log(16.0)))
+        f_max = min(2048.0, 0.3 / ((p[0] + p[1]) * MTSUN))
+        grids.append((f_lower, df, int(np.floor((f_max - f_lower) / df + 0.5)) + 1))
+    return grids
+
+
+def same_bits(a, b):
+    """Equal values and equal NaN positions (complex tensors)."""
+    import torch
+
+    ra, rb = torch.view_as_real(a), torch.view_as_real(b)
+    na, nb = torch.isnan(ra), torch.isnan(rb)
+    return bool(torch.equal(na, nb)) and bool(torch.equal(torch.where(na, 0.0, ra), torch.where(nb, 0.0, rb)))
+
+
+def test_own_grids(cuda):
+    """Binaries each on their own grid (xhm.evaluate_ragged: one launch over
+    256-bin tiles; the reference's per-binary (B, N) frequencies,
+    xhm/mlx/_kernels.py:565-576, with a length per binary): each binary's
+    slice, and its modes, equal the shared-grid evaluation of that binary
+    alone on its grid bit for bit, in FP64 and split; the uniform grids
+    generated on the device from (f0, df) equal the same grids given
+    explicitly; FP64 against the oracle within the polarization bar on
+    every binary; split against FP64 within the reference's float32
+    mismatch bar; ragged_polarizations routes IMRPhenomXHM here; a failing
+    binary's slice is NaN; empty grids are skipped."""
+    import torch
+
+    import waveformx_amd as wx
+    from waveformx_amd import xhm
+
+    rng = np.random.default_rng(41)
+    params = oxhm.draw(10, seed=42)
+    params[3, 2] = 1.5  # fails LAL's checks
+    grids = own_grids(params, rng)
+    grids[6] = (grids[6][0], grids[6][1], 0)  # an empty grid
+    lengths = np.array([g[2] for g in grids])
+    offsets = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    f0 = np.array([g[0] for g in grids])
+    df = np.array([g[1] for g in grids])
+    flat = np.concatenate([g[0] + np.arange(g[2]) * g[1] for g in grids])
+    p = torch.as_tensor(params, device=cuda)
+    off = torch.as_tensor(offsets, device=cuda)
+    results = {}
+    for precision, rdtype in (("fp64", torch.float64), ("split", torch.float32)):
+        print("own grids", precision, flush=True)
+        records, head, modes_, _ = xhm.setup(p, torch.as_tensor(f0, device=cuda), precision=precision)
+        fl = torch.as_tensor(flat, dtype=rdtype, device=cuda)
+        hp, hc, hm = xhm.evaluate_ragged(records, head, modes_, off, freqs=fl, with_modes=True)
+        gp, gc = xhm.evaluate_ragged(records, head, modes_, off, grid_f0=torch.as_tensor(f0, device=cuda),
+                                     grid_df=torch.as_tensor(df, device=cuda))
+        assert same_bits(hp, gp) and same_bits(hc, gc)
+        for b in range(len(grids)):
+            lo, hi = offsets[b], offsets[b + 1]
+            if hi == lo:
+                continue
+            sp, sc, sm = xhm.evaluate(records[b:b + 1], head[b:b + 1], modes_[b:b + 1], fl[lo:hi], with_modes=True)
+            assert same_bits(hp[lo:hi], sp[0]) and same_bits(hc[lo:hi], sc[0]), (precision, b)
+            assert same_bits(hm[:, lo:hi].contiguous(), sm[:, 0].contiguous()), (precision, b)
+        assert bool(torch.isnan(torch.view_as_real(hp[offsets[3]:offsets[4]])).all())
+        results[precision] = (hp.cpu().numpy(), hc.cpu().numpy())
+    # FP64 against the oracle, every binary that LAL accepts
+    for b in range(len(grids)):
+        lo, hi = offsets[b], offsets[b + 1]
+        if hi == lo or b == 3:
+            continue
+        op, oc = oxhm.polarizations(params[b:b + 1], flat[lo:hi], amp_scale=1e21)
+        bar = polarization_bar(params[b], flat[lo])
+        for got, ref in ((results["fp64"][0][lo:hi], op[0]), (results["fp64"][1][lo:hi], oc[0])):
+            assert relative_error(got, ref) <= bar, b
+        # split against FP64 (at the same float32-rounded frequencies: the FP64 path on the float32 grid)
+    records, head, modes_, _ = xhm.setup(p, torch.as_tensor(f0, device=cuda), precision="fp64")
+    ref_p, _ = xhm.evaluate_ragged(records, head, modes_, off,
+                                   freqs=torch.as_tensor(flat, dtype=torch.float32, device=cuda))
+    ref_p = ref_p.cpu().numpy()
+    for b in range(len(grids)):
+        lo, hi = offsets[b], offsets[b + 1]
+        if hi == lo or b == 3:
+            continue
+        a, r = results["split"][0][lo:hi].astype(np.complex128), ref_p[lo:hi]
+        mism = 1.0 - np.vdot(r, a).real / np.sqrt(np.vdot(a, a).real * np.vdot(r, r).real)
+        assert mism <= 1e-6, (b, mism)
+    # the public route: ragged_polarizations with descriptors and with explicit grids
+    spins = [np.zeros((len(params), 3)), np.zeros((len(params), 3))]
+    spins[0][:, 2], spins[1][:, 2] = params[:, 2], params[:, 3]
+    binaries = {"mass_1": params[:, 0], "mass_2": params[:, 1], "spin_1": spins[0], "spin_2": spins[1],
+                "luminosity_distance": params[:, 4], "inclination": params[:, 5], "phi_ref": params[:, 6],
+                "f_ref": params[:, 7]}
+    hp_d, hc_d, off_d = wx.ragged_polarizations("IMRPhenomXHM", grids, binaries, device=cuda)
+    np.testing.assert_array_equal(off_d.cpu().numpy(), offsets)
+    np.testing.assert_array_equal(np.nan_to_num(hp_d.cpu().numpy()), np.nan_to_num(results["fp64"][0]))
+    explicit = [g[0] + np.arange(g[2]) * g[1] for g in grids]
+    hp_e, _, _ = wx.ragged_polarizations("IMRPhenomXHM", explicit, binaries, device=cuda)
+    np.testing.assert_array_equal(np.nan_to_num(hp_e.cpu().numpy()), np.nan_to_num(results["fp64"][0]))
+
+
+def test_per_binary_grid_rows(cuda):
+    """Per-binary grids (B, N), the reference's batched frequencies
+    (xhm/mlx/_api.py:79-93): xhm.polarizations and xhm.modes equal, row by
+    row, the shared-grid evaluation of each binary on its own row (each set
+    up at its own first frequency), in FP64 and split."""
+    import torch
+
+    from waveformx_amd import xhm
+
+    params = oxhm.draw(5, seed=43)
+    rows = np.stack([np.linspace(f_lo, 900.0, 700) for f_lo in (12.0, 20.0, 25.5, 30.0, 17.25)])
+    p = torch.as_tensor(params, device=cuda)
+    for precision in ("fp64", "split"):
+        f = torch.as_tensor(rows, device=cuda)
+        hp, hc = xhm.polarizations(p, f, precision=precision)
+        modes = xhm.modes(p, f, precision=precision)
+        assert hp.shape == (5, 700) and modes.shape == (5, 5, 700)
+        for b in range(5):
+            one_p, one_c = xhm.polarizations(p[b:b + 1], f[b], precision=precision)
+            assert same_bits(hp[b], one_p[0]) and same_bits(hc[b], one_c[0]), (precision, b)
+            assert same_bits(modes[:, b].contiguous(), xhm.modes(p[b:b + 1], f[b], precision=precision)[:, 0].contiguous())
+    with pytest.raises(ValueError):
+        xhm.polarizations(p, torch.as_tensor(rows[:3], device=cuda))

@@ -244,7 +244,7 @@ def test_xhm_host_surface():
     with pytest.raises(wx.InvalidParameterError, match="delta_f"):
         wx.choose_fd_modes("IMRPhenomXHM", 0.0, 20.0, mass_1=30.0, mass_2=20.0)
     with pytest.raises(wx.UnsupportedError):
-        wx.ragged_polarizations("IMRPhenomXHM", [np.array([20.0, 30.0])], np.ones((1, 12)))
+        wx.ragged_polarizations("IMRPhenomXPNR", [np.array([20.0, 30.0])], np.ones((1, 12)))
     theta, phi = 0.8, 0.4
     c, s = np.cos(theta / 2), np.sin(theta / 2)
     y44 = 3 * np.sqrt(7 / np.pi) * c ** 6 * s ** 2 * np.exp(4j * phi)

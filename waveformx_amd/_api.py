@@ -241,8 +241,6 @@ def ragged_polarizations(approximant: str, grids: Sequence[Any], parameters: Any
     ``hplus``, ``hcross`` and the (B+1,) int64 ``offsets`` of each binary's
     bins. No parameter checks (as the reference's pure functions)."""
     model(approximant)
-    if approximant == "IMRPhenomXHM":
-        raise UnsupportedError("ragged grids are built for IMRPhenomXAS and IMRPhenomD")
     options = parse_options(options, option_keywords)
     table = parameter_table(parameters)
     dev = compute_device(device)
@@ -261,6 +259,12 @@ def ragged_polarizations(approximant: str, grids: Sequence[Any], parameters: Any
         f0_t = torch.as_tensor(f0, device=dev)
         df_t = torch.as_tensor(df, device=dev)
         f_min = f0_t if precision in ("fp64", "split", "split-df") else f0_t.to(torch.float32).to(torch.float64)
+        if approximant == "IMRPhenomXHM":
+            records, head, modes_, _ = module.setup(params, f_min, strain_scale(options), options.reference_mf,
+                                                    precision)
+            hp, hc = module.evaluate_ragged(records, head, modes_, off, grid_f0=f0_t, grid_df=df_t,
+                                            total=int(offsets[-1]))
+            return hp, hc, off
         records, _ = module.setup(params, f_min, strain_scale(options), options.reference_mf, precision)
         hp, hc = module.evaluate_ragged(records, off, grid_f0=f0_t, grid_df=df_t, precision=precision,
                                         total=int(offsets[-1]))
@@ -273,6 +277,11 @@ def ragged_polarizations(approximant: str, grids: Sequence[Any], parameters: Any
     starts = torch.as_tensor(offsets[:-1], device=dev).clamp(max=max(freqs.shape[0] - 1, 0))
     f_min = freqs[starts].to(torch.float64) if freqs.shape[0] else torch.zeros(len(table), device=dev,
                                                                                  dtype=torch.float64)
+    if approximant == "IMRPhenomXHM":
+        records, head, modes_, _ = module.setup(params, f_min, strain_scale(options), options.reference_mf,
+                                                precision)
+        hp, hc = module.evaluate_ragged(records, head, modes_, off, freqs=freqs)
+        return hp, hc, off
     records, _ = module.setup(params, f_min, strain_scale(options), options.reference_mf, precision)
     hp, hc = module.evaluate_ragged(records, off, freqs=freqs, precision=precision)
     return hp, hc, off

@@ -14,7 +14,7 @@ LIBRARY = Path(__file__).resolve().parent / "libwfx_xas.so"
 _lib = None
 
 _P, _I64, _I, _D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
-ABI_VERSION = 23
+ABI_VERSION = 24
 SIGNATURES = {
     "wfx_last_error": ([], ctypes.c_char_p),
     "wfx_abi_version": ([], _I),
@@ -46,6 +46,7 @@ SIGNATURES = {
     "wfx_xhm_record_slots": ([], _I),
     "wfx_xhm_head_slots": ([], _I),
     "wfx_xhm_setup": ([_P, _I64, _P, _I64, _D, _D, _I, _P, _P, _P, _P, _P], _I),
+    "wfx_xhm_eval_ragged": ([_P, _P, _P, _I64, _P, _I, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P], _I),
     "wfx_xhm_setup_jvp_workspace": ([_I64], _I64),
     "wfx_xhm_setup_jvp": ([_P, _I64, _P, _I64, _D, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P], _I),
     "wfx_xhm_adjoint_workspace": ([_I64, _I64, _I, _I, _I], _I64),

@@ -1349,6 +1349,78 @@ __global__ __launch_bounds__(kXhmBlock) __attribute__((amdgpu_waves_per_eu(3))) 
     }
 }
 
+// Binaries on their own grids: the concatenated grid (binary b owns
+// [offsets[b], offsets[b + 1]) of freqs) or the uniform grid f0[b] + k
+// df[b] of that length, in tiles of kXhmBlock consecutive bins of one
+// binary (tiles[b] from tile_prefix_kernel at kXhmBlock); a persistent
+// grid walks the tiles, each wave finding its tile's binary by the 64-ary
+// search; per bin the shared-grid kernels' arithmetic (the same bits for
+// the same frequency). mode_out, if not null, holds the five modes (5,
+// total).
+template <class F>
+__device__ inline double xhm_ragged_freq(const F* __restrict__ freqs, const double* __restrict__ f0,
+                                         const double* __restrict__ df, int64_t b, int64_t i, int64_t k) {
+    return freqs ? (double)freqs[i] : f0[b] + (double)k * df[b];
+}
+
+template <class F>
+__global__ __launch_bounds__(kXhmBlock) __attribute__((amdgpu_waves_per_eu(3))) void xhm_eval_ragged_kernel(
+    const double* __restrict__ rec22, const double* __restrict__ head, const double* __restrict__ modes, int64_t nb,
+    const F* __restrict__ freqs, const double* __restrict__ f0, const double* __restrict__ df,
+    const int64_t* __restrict__ offsets, const int64_t* __restrict__ tiles, double2* __restrict__ hp,
+    double2* __restrict__ hc, double2* __restrict__ mode_out) {
+    const int64_t total_tiles = tiles[nb];
+    const int64_t total = offsets[nb];
+    for (int64_t w = blockIdx.x; w < total_tiles; w += gridDim.x) {
+        const int64_t b = owner(tiles, nb, w);
+        const int64_t k = (w - tiles[b]) * kXhmBlock + threadIdx.x;
+        const int64_t i = offsets[b] + k;
+        if (i >= offsets[b + 1]) continue;
+        double h[5][2];
+        xhm_modes_f64(rec22 + b * kSlots, modes + b * kHmModes * kHmSlots, xhm_ragged_freq(freqs, f0, df, b, i, k),
+                      h);
+        double p_re, p_im, c_re, c_im;
+        xhm_sum(head + b * kHeadSlots, h, p_re, p_im, c_re, c_im);
+        hp[i] = make_double2(p_re, p_im);
+        hc[i] = make_double2(c_re, c_im);
+        if (mode_out) {
+            const double nan = __builtin_nan("");
+            const bool bad = head[b * kHeadSlots] != head[b * kHeadSlots];
+            for (int m = 0; m < 5; ++m)
+                mode_out[m * total + i] = bad ? make_double2(nan, nan) : make_double2(h[m][0], h[m][1]);
+        }
+    }
+}
+
+template <class F>
+__global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_ragged_kernel(
+    const float* __restrict__ rec22, const float* __restrict__ head, const float* __restrict__ modes, int64_t nb,
+    const F* __restrict__ freqs, const double* __restrict__ f0, const double* __restrict__ df,
+    const int64_t* __restrict__ offsets, const int64_t* __restrict__ tiles, float2* __restrict__ hp,
+    float2* __restrict__ hc, float2* __restrict__ mode_out) {
+    const int64_t total_tiles = tiles[nb];
+    const int64_t total = offsets[nb];
+    for (int64_t w = blockIdx.x; w < total_tiles; w += gridDim.x) {
+        const int64_t b = owner(tiles, nb, w);
+        const int64_t k = (w - tiles[b]) * kXhmBlock + threadIdx.x;
+        const int64_t i = offsets[b] + k;
+        if (i >= offsets[b + 1]) continue;
+        float h[5][2];
+        xhm_modes_split(rec22 + b * kSlots, modes + b * kHmModes * kHmSlots,
+                        (float)xhm_ragged_freq(freqs, f0, df, b, i, k), h);
+        float2 p, c;
+        xhm_sum_f32(head + b * kHeadSlots, h, p, c);
+        hp[i] = p;
+        hc[i] = c;
+        if (mode_out) {
+            const float nan = __builtin_nanf("");
+            const bool bad = head[b * kHeadSlots] != head[b * kHeadSlots];
+            for (int m = 0; m < 5; ++m)
+                mode_out[m * total + i] = bad ? make_float2(nan, nan) : make_float2(h[m][0], h[m][1]);
+        }
+    }
+}
+
 // the FP64 polarizations contracted with cotangents, nothing per bin
 // written (the gradient's differences and closed forms): the evaluation's
 // shape; for each of n_heads heads (head + j * nb * kHeadSlots: the modes'
@@ -1875,7 +1947,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 23; }
+int wfx_abi_version(void) { return 24; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -2369,6 +2441,37 @@ int wfx_xhm_eval(const void* records22, const void* head, const void* modes, int
                                                              n_freqs, chunks, (double2*)hplus, (double2*)hcross,
                                                              (double2*)mode_out);
     return check_launch("wfx_xhm_eval");
+}
+
+int wfx_xhm_eval_ragged(const void* records22, const void* head, const void* modes, int64_t n_binaries,
+                        const void* freqs, int freqs_f64, const double* grid_f0, const double* grid_df,
+                        const int64_t* offsets, const int64_t* tiles, int precision, int blocks, void* hplus,
+                        void* hcross, void* mode_out, void* stream) {
+    if (n_binaries < 0) return fail("wfx_xhm_eval_ragged: negative size");
+    if (n_binaries == 0) return 0;
+    if (!records22 || !head || !modes || !offsets || !tiles || !hplus || !hcross)
+        return fail("wfx_xhm_eval_ragged: null pointer");
+    if (!freqs && (!grid_f0 || !grid_df)) return fail("wfx_xhm_eval_ragged: give freqs or (grid_f0, grid_df)");
+    if (precision != kPrecF64 && precision != kPrecSplit)
+        return fail("wfx_xhm_eval_ragged: precision must be 0 (FP64) or 2 (split)");
+    if (precision == kPrecSplit && freqs && freqs_f64)
+        return fail("wfx_xhm_eval_ragged: the split evaluation takes a float32 grid");
+    if (blocks <= 0) blocks = 8 * cu_count();
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == kPrecSplit) {
+        xhm_eval_split_ragged_kernel<float><<<blocks, kXhmBlock, 0, s>>>(
+            (const float*)records22, (const float*)head, (const float*)modes, n_binaries, (const float*)freqs,
+            grid_f0, grid_df, offsets, tiles, (float2*)hplus, (float2*)hcross, (float2*)mode_out);
+    } else if (freqs && !freqs_f64) {
+        xhm_eval_ragged_kernel<float><<<blocks, kXhmBlock, 0, s>>>(
+            (const double*)records22, (const double*)head, (const double*)modes, n_binaries, (const float*)freqs,
+            grid_f0, grid_df, offsets, tiles, (double2*)hplus, (double2*)hcross, (double2*)mode_out);
+    } else {
+        xhm_eval_ragged_kernel<double><<<blocks, kXhmBlock, 0, s>>>(
+            (const double*)records22, (const double*)head, (const double*)modes, n_binaries, (const double*)freqs,
+            grid_f0, grid_df, offsets, tiles, (double2*)hplus, (double2*)hcross, (double2*)mode_out);
+    }
+    return check_launch("wfx_xhm_eval_ragged");
 }
 
 int wfx_xhm_project(const double* records22, const double* head, int n_heads, const double* modes,

@@ -31,8 +31,8 @@ import torch
 from ._lib import call, lib
 from .xas import RECORD_SLOTS, _f_min, _ptr, _require_device, _stream
 
-__all__ = ["MODES", "LAL_FLAGS", "PRECISIONS", "setup", "setup_jvp", "evaluate", "polarizations", "modes",
-           "adjoint", "vjp", "vjp_differences"]
+__all__ = ["MODES", "LAL_FLAGS", "PRECISIONS", "setup", "setup_jvp", "evaluate", "evaluate_ragged", "polarizations",
+           "modes", "adjoint", "vjp", "vjp_differences"]
 
 #: the modes (l, m) of the outputs, the (2,2) first (``xhm/reference.py:55-57``)
 MODES = ((2, 2), (2, 1), (3, 3), (3, 2), (4, 4))
@@ -152,6 +152,77 @@ def evaluate(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, fr
     return (hp, hc, mode_out) if with_modes else (hp, hc)
 
 
+def evaluate_ragged(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, offsets: torch.Tensor,
+                    freqs: Optional[torch.Tensor] = None, grid_f0: Optional[torch.Tensor] = None,
+                    grid_df: Optional[torch.Tensor] = None, with_modes: bool = False, tiles: Optional[torch.Tensor] = None,
+                    total: Optional[int] = None, blocks: int = 0):
+    """``(h+, hx)`` flat (total,) of binaries on their own grids: binary b
+    owns ``[offsets[b], offsets[b+1])`` of the concatenated ``freqs``
+    (float32 for split records; float32 or float64 for FP64 records), or the
+    uniform grid ``grid_f0[b] + k grid_df[b]`` of that length; with
+    ``with_modes`` also the modes (5, total). One persistent launch over
+    256-bin tiles; each bin has the bits :func:`evaluate` gives it on a
+    shared grid. ``tiles`` (:func:`waveformx_amd.xas.tile_prefix` of the
+    offsets) and ``total`` given, nothing synchronises with the host."""
+    from .xas import checked_total, tile_prefix
+
+    for t, name in ((records, "records"), (head, "head"), (modes_, "modes"), (offsets, "offsets")):
+        _require_device(t, name)
+    if offsets.dtype != torch.int64:
+        raise ValueError("offsets must be int64")
+    nb = records.shape[0]
+    if offsets.shape[0] != nb + 1 or head.shape[0] != nb or modes_.shape[0] != nb:
+        raise ValueError("records, head, modes and offsets (B + 1) must describe the same binaries")
+    if head.dtype != records.dtype or modes_.dtype != records.dtype:
+        raise ValueError("records, head and modes must come from one setup")
+    split = records.dtype == torch.float32
+    if freqs is None and (grid_f0 is None or grid_df is None):
+        raise ValueError("give freqs or (grid_f0, grid_df)")
+    if freqs is not None:
+        _require_device(freqs, "freqs")
+        if freqs.dtype not in (torch.float32, torch.float64) or (split and freqs.dtype != torch.float32):
+            raise ValueError("freqs must be float32 (split) or float32/float64 (FP64)")
+        freqs = freqs.contiguous()
+    if total is None:
+        total = checked_total(offsets)
+        if freqs is not None and freqs.shape[0] != total:
+            raise ValueError("offsets[-1] must equal the number of frequencies")
+    if freqs is not None and freqs.shape[0] < total:
+        raise ValueError("fewer frequencies than offsets[-1]")
+    if grid_f0 is not None:
+        grid_f0 = grid_f0.to(torch.float64).contiguous()
+        grid_df = grid_df.to(torch.float64).contiguous()
+    if tiles is None:
+        tiles = tile_prefix(offsets, 1)
+    dev = records.device
+    ctype = torch.complex64 if split else torch.complex128
+    hp = torch.empty((total,), dtype=ctype, device=dev)
+    hc = torch.empty((total,), dtype=ctype, device=dev)
+    mode_out = torch.empty((5, total), dtype=ctype, device=dev) if with_modes else None
+    if nb and total:
+        call("wfx_xhm_eval_ragged", _ptr(records), _ptr(head), _ptr(modes_), nb, _ptr(freqs),
+             int(freqs is not None and freqs.dtype == torch.float64), _ptr(grid_f0), _ptr(grid_df), _ptr(offsets),
+             _ptr(tiles), 2 if split else 0, int(blocks), _ptr(hp), _ptr(hc), _ptr(mode_out), _stream(dev))
+    return (hp, hc, mode_out) if with_modes else (hp, hc)
+
+
+def _per_binary(params: torch.Tensor, freqs: torch.Tensor, amp_scale, reference_mf, precision, with_modes):
+    """Per-binary grids (B, N), as the reference's (B, N) frequencies
+    (``xhm/mlx/_api.py:79-93``): each binary set up at its own first
+    frequency, the grid rows evaluated as one ragged batch."""
+    nb, n = freqs.shape
+    if nb != params.shape[0]:
+        raise ValueError(f"per-binary grids must be ({params.shape[0]}, N), got {tuple(freqs.shape)}")
+    freqs = freqs.contiguous()
+    records, head, modes_, _ = setup(params, freqs[:, 0].to(torch.float64), amp_scale, reference_mf, precision)
+    offsets = torch.arange(nb + 1, dtype=torch.int64, device=params.device) * n
+    out = evaluate_ragged(records, head, modes_, offsets, freqs=freqs.reshape(-1), with_modes=with_modes,
+                          total=nb * n)
+    if with_modes:
+        return out[2].reshape(5, nb, n)
+    return out[0].reshape(nb, n), out[1].reshape(nb, n)
+
+
 def project(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, freqs: torch.Tensor,
             cot_plus: torch.Tensor, cot_cross: torch.Tensor) -> torch.Tensor:
     """``Re sum_f (conj(cot+) h+ + conj(cotx) hx)`` (B,) float64 of the FP64
@@ -192,19 +263,25 @@ def project(records: torch.Tensor, head: torch.Tensor, modes_: torch.Tensor, fre
 
 def polarizations(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21,
                   reference_mf: Optional[float] = None, precision: str = "fp64") -> Tuple[torch.Tensor, torch.Tensor]:
-    """``(h+, hx)`` of a batch on a shared grid: the set-up at the grid's
-    first frequency, then the evaluation (``"split"`` on the float32 grid)."""
+    """``(h+, hx)`` of a batch on a shared grid (N,) or on per-binary grids
+    (B, N): the set-up at each grid's first frequency, then the evaluation
+    (``"split"`` on the float32 grid)."""
     if precision == "split":
         freqs = freqs.to(torch.float32)
+    if freqs.ndim == 2:
+        return _per_binary(params, freqs, amp_scale, reference_mf, precision, False)
     records, head, modes_, _ = setup(params, freqs[:1], amp_scale, reference_mf, precision)
     return evaluate(records, head, modes_, freqs)
 
 
 def modes(params: torch.Tensor, freqs: torch.Tensor, amp_scale: float = 1e21,
           reference_mf: Optional[float] = None, precision: str = "fp64") -> torch.Tensor:
-    """The five modes ``h_{l,-m}`` of :data:`MODES`, (5, B, N)."""
+    """The five modes ``h_{l,-m}`` of :data:`MODES`, (5, B, N), on a shared
+    grid (N,) or per-binary grids (B, N)."""
     if precision == "split":
         freqs = freqs.to(torch.float32)
+    if freqs.ndim == 2:
+        return _per_binary(params, freqs, amp_scale, reference_mf, precision, True)
     records, head, modes_, _ = setup(params, freqs[:1], amp_scale, reference_mf, precision)
     return evaluate(records, head, modes_, freqs, with_modes=True)[2]
 
