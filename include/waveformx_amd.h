@@ -42,7 +42,7 @@ extern "C" {
 /* The message of the last failed call on this host thread. */
 const char* wfx_last_error(void);
 
-/* The version of this ABI (24); a binding checks it before any call. */
+/* The version of this ABI (25); a binding checks it before any call. */
 int wfx_abi_version(void);
 
 /* Slots of one binary's record (64) and its size in bytes for a precision
@@ -134,13 +134,18 @@ int wfx_xas_adjoint(const void* records, const double* jacobian, int64_t n_binar
                     int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, const void* powers,
                     double* grads, void* stream);
 
-/* dldf[N] += dL/df, the gradient with respect to the shared grid's
- * frequencies (the first one's effect through f_min excluded, as the
+/* dldf[N] = dL/df (double), the gradient with respect to the shared
+ * grid's frequencies (the first one's effect through f_min excluded, as the
  * reference's stop_gradient, src/waveformx/models/xas/model.py:59), from
- * the cotangents as wfx_xas_adjoint; the caller zeroes dldf (double).
- * Replaces the grid part of the reference's vjp (src/waveformx/torch.py:116). */
+ * the cotangents as wfx_xas_adjoint. workspace: device memory of at least
+ * wfx_xas_grid_adjoint_workspace(B, N) bytes (a double per group of 64
+ * binaries and frequency), 256-aligned; the groups are summed in a fixed
+ * order, so the result is reproducible. Replaces the grid part of the
+ * reference's vjp (src/waveformx/torch.py:116). */
+int64_t wfx_xas_grid_adjoint_workspace(int64_t n_binaries, int64_t n_freqs);
 int wfx_xas_grid_adjoint(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
-                         int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* stream);
+                         int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* workspace,
+                         int64_t workspace_bytes, void* stream);
 
 /* wfx_xas_fast_atan: y[i] = the FP32 arctangent of the split and FP32
  * evaluations (atan(x) = x P(x^2) on [0, 1], pi/2 - atan(1/x) beyond, P the
@@ -224,7 +229,8 @@ int wfx_phd_adjoint(const void* records, const double* jacobian, int64_t n_binar
                     int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, const void* powers,
                     double* grads, void* stream);
 int wfx_phd_grid_adjoint(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
-                         int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* stream);
+                         int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* workspace,
+                         int64_t workspace_bytes, void* stream);
 int wfx_phd_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
                          int freqs_f64, int64_t n_freqs, int precision, const void* data_plus, const void* data_cross,
                          const void* weights, const void* powers, double* loss, double* grads, void* stream);

@@ -124,6 +124,13 @@ def test_grid_gradient(cuda, precision):
     scale = np.max(np.abs(expected))
     tol = 1e-6 if precision == "fp64" else 2e-3
     assert np.max(np.abs(grid - expected)) / scale < tol, np.max(np.abs(grid - expected)) / scale
+    # the sums over the binaries run in a fixed order: every call gives the same bits, also for a batch of
+    # 200 binaries (four groups of the kernel's partial sums)
+    many = torch.as_tensor(np.tile(params, (34, 1))[:200], device=cuda)
+    gpm, gcm = (torch.as_tensor(np.tile(x.cpu().numpy(), (34, 1))[:200], device=cuda) for x in (gp, gc))
+    runs = [adjoint.vjp(many, ft, gpm, gcm, 1e21, None, precision, wrt_grid=True)[1].cpu().numpy() for _ in range(3)]
+    for run in runs[1:]:
+        np.testing.assert_array_equal(run, runs[0])
 
 
 @pytest.mark.parametrize("precision", ["fp64", "split"])

@@ -14,7 +14,7 @@ LIBRARY = Path(__file__).resolve().parent / "libwfx_xas.so"
 _lib = None
 
 _P, _I64, _I, _D = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
-ABI_VERSION = 24
+ABI_VERSION = 25
 SIGNATURES = {
     "wfx_last_error": ([], ctypes.c_char_p),
     "wfx_abi_version": ([], _I),
@@ -31,7 +31,8 @@ SIGNATURES = {
     "wfx_xas_power_table_bytes": ([_I], _I),
     "wfx_xas_power_table": ([_P, _I, _I64, _I, _P, _P], _I),
     "wfx_xas_adjoint": ([_P, _P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P, _P], _I),
-    "wfx_xas_grid_adjoint": ([_P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P], _I),
+    "wfx_xas_grid_adjoint_workspace": ([_I64, _I64], _I64),
+    "wfx_xas_grid_adjoint": ([_P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P, _I64, _P], _I),
     "wfx_xas_loss_adjoint": ([_P, _P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P, _P, _P, _P], _I),
     "wfx_xas_loss_grad_static": ([_P, _I64, _P, _I64, _D, _D, _P, _I, _I64, _I, _P, _P, _P, _P, _P, _P], _I),
     "wfx_phd_setup": ([_P, _I64, _P, _I64, _D, _D, _I, _P, _P, _P], _I),
@@ -42,7 +43,7 @@ SIGNATURES = {
     "wfx_phd_setup_jvp": ([_P, _I64, _P, _I64, _D, _D, _P, _P, _P, _P, _P], _I),
     "wfx_phd_adjoint": ([_P, _P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P, _P], _I),
     "wfx_phd_loss_adjoint": ([_P, _P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P, _P, _P, _P], _I),
-    "wfx_phd_grid_adjoint": ([_P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P], _I),
+    "wfx_phd_grid_adjoint": ([_P, _I64, _P, _I, _I64, _I, _P, _P, _P, _P, _I64, _P], _I),
     "wfx_xhm_record_slots": ([], _I),
     "wfx_xhm_head_slots": ([], _I),
     "wfx_xhm_setup": ([_P, _I64, _P, _I64, _D, _D, _I, _P, _P, _P, _P, _P], _I),

@@ -28,7 +28,7 @@ from typing import Optional, Tuple
 import torch
 
 from . import phenomd, xas
-from ._lib import call
+from ._lib import call, lib
 from .xas import _code, _ptr, _require_device, _stream
 
 __all__ = ["vjp", "adjoint", "loss_and_grad", "PARAMETERS", "MODELS"]
@@ -96,9 +96,14 @@ def grid_adjoint(records: torch.Tensor, freqs: torch.Tensor, cot_plus: torch.Ten
     the reference's stop_gradient on f_min)."""
     entry = _entries(approximant, precision)[3]
     nb, n = records.shape[0], freqs.shape[0]
-    dldf = torch.zeros((n,), dtype=torch.float64, device=records.device)
+    dldf = torch.empty((n,), dtype=torch.float64, device=records.device)
+    size = lib().wfx_xas_grid_adjoint_workspace(nb, n)
+    if size < 0:
+        raise ValueError("the grid's gradient is too large")
+    workspace = torch.empty((size,), dtype=torch.uint8, device=records.device)
     call(entry, _ptr(records), nb, _ptr(freqs), int(freqs.dtype == torch.float64), n,
-         _code(precision), _ptr(cot_plus), _ptr(cot_cross), _ptr(dldf), _stream(records.device))
+         _code(precision), _ptr(cot_plus), _ptr(cot_cross), _ptr(dldf), _ptr(workspace), workspace.numel(),
+         _stream(records.device))
     return dldf
 
 

@@ -880,29 +880,49 @@ __global__ __launch_bounds__(kAdjointBlock) void adjoint_kernel(const T* __restr
 }
 
 // dL/df of each frequency of a shared grid: the evaluation's launch shape
-// (a thread per frequency, a loop over a group of binaries), the bins'
-// contributions summed in FP64 in a register, then added to dldf[i] (one
-// FP64 atomic per group and frequency)
+// (a thread per frequency, a loop over a group of kGridGroup binaries), the
+// bins' contributions summed in FP64 in a register in the binaries' order,
+// then partial[g][i] for group g; group_sum_kernel adds the groups in
+// order (no atomics: the same bits on every run)
+constexpr int kGridGroup = 64;
 template <class A, class T, class F>
 __global__ __launch_bounds__(kBlock) void grid_adjoint_kernel(const T* __restrict__ records, int64_t nb,
                                                               const F* __restrict__ freqs, int64_t n, int64_t ntiles,
-                                                              int group, const T* __restrict__ cot_plus,
+                                                              const T* __restrict__ cot_plus,
                                                               const T* __restrict__ cot_cross,
-                                                              double* __restrict__ dldf) {
+                                                              double* __restrict__ partial) {
     const int64_t w = blockIdx.x;
     const int64_t tile = w % ntiles;
-    const int64_t b0 = (w / ntiles) * group;
+    const int64_t g = w / ntiles;
+    const int64_t b0 = g * kGridGroup;
     const int64_t i = tile * kBlock + threadIdx.x;
     if (i >= n) return;
     const BinPowers<T> p = adjoint_powers<T>((double)freqs[i]);
-    const int64_t b1 = b0 + group < nb ? b0 + group : nb;
+    const int64_t b1 = b0 + kGridGroup < nb ? b0 + kGridGroup : nb;
     double acc = 0.0;
     for (int64_t b = b0; b < b1; ++b) {
         const int64_t o = 2 * (b * n + i);
         acc += (double)A::grid(records + b * kSlots, p, cot_plus[o], cot_plus[o + 1], cot_cross[o],
                                cot_cross[o + 1]);
     }
-    atomicAdd(dldf + i, acc);
+    partial[g * n + i] = acc;
+}
+
+// out[i] = the sum over g < groups of partial[g * n + i], in order
+__global__ __launch_bounds__(kBlock) void group_sum_kernel(const double* __restrict__ partial, int64_t groups,
+                                                           int64_t n, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int64_t g = 0; g < groups; ++g) s += partial[g * n + i];
+    out[i] = s;
+}
+
+// the grid adjoint's workspace: one double per (group of binaries, frequency)
+inline int64_t grid_adjoint_bytes(int64_t nb, int64_t n) {
+    const int64_t groups = (nb + kGridGroup - 1) / kGridGroup;
+    if (n > 0 && groups > INT64_MAX / 8 / n) return -1;
+    return groups * n * 8;
 }
 
 struct AdjointArgs {
@@ -1156,37 +1176,52 @@ int adjoint(const char* name, const void* records, const double* jacobian, int64
 template <class A>
 int grid_adjoint(const char* name, const void* records, int64_t n_binaries, const void* freqs, int freqs_f64,
                  int64_t n_freqs, int precision, const void* cot_plus, const void* cot_cross, double* dldf,
-                 void* stream) {
+                 void* workspace, int64_t workspace_bytes, void* stream) {
     const std::string who(name);
     if (n_binaries < 0 || n_freqs < 0) return fail((who + ": negative size").c_str());
-    if (n_binaries == 0 || n_freqs == 0) return 0;
-    if (!records || !freqs || !cot_plus || !cot_cross || !dldf) return fail((who + ": null pointer").c_str());
-    const int group = 64;
-    const int64_t ntiles = (n_freqs + kBlock - 1) / kBlock;
-    const int blocks = grid_blocks(ntiles * ((n_binaries + group - 1) / group), (who + ": too large").c_str());
-    if (blocks < 0) return -1;
+    if (n_freqs == 0) return 0;
+    if (!dldf) return fail((who + ": null pointer").c_str());
     hipStream_t s = (hipStream_t)stream;
+    const int64_t ntiles = (n_freqs + kBlock - 1) / kBlock;
+    const int out_blocks = grid_blocks(ntiles, (who + ": too large").c_str());
+    if (out_blocks < 0) return -1;
+    if (n_binaries == 0) {
+        // no binary: a zero gradient (a kernel, not a memset node)
+        group_sum_kernel<<<out_blocks, kBlock, 0, s>>>(nullptr, 0, n_freqs, dldf);
+        return check_launch(name);
+    }
+    if (!records || !freqs || !cot_plus || !cot_cross) return fail((who + ": null pointer").c_str());
+    const int64_t need = grid_adjoint_bytes(n_binaries, n_freqs);
+    if (need < 0) return fail((who + ": too large").c_str());
+    if (!workspace || workspace_bytes < need || (uintptr_t)workspace % 256)
+        return fail((who + ": the workspace must be 256-aligned and hold wfx_xas_grid_adjoint_workspace() bytes")
+                        .c_str());
+    const int64_t groups = (n_binaries + kGridGroup - 1) / kGridGroup;
+    const int blocks = grid_blocks(ntiles * groups, (who + ": too large").c_str());
+    if (blocks < 0) return -1;
+    double* partial = (double*)workspace;
     if (precision == kPrecF64) {
         if (freqs_f64)
             grid_adjoint_kernel<A, double, double><<<blocks, kBlock, 0, s>>>(
-                (const double*)records, n_binaries, (const double*)freqs, n_freqs, ntiles, group,
-                (const double*)cot_plus, (const double*)cot_cross, dldf);
+                (const double*)records, n_binaries, (const double*)freqs, n_freqs, ntiles, (const double*)cot_plus,
+                (const double*)cot_cross, partial);
         else
             grid_adjoint_kernel<A, double, float><<<blocks, kBlock, 0, s>>>(
-                (const double*)records, n_binaries, (const float*)freqs, n_freqs, ntiles, group,
-                (const double*)cot_plus, (const double*)cot_cross, dldf);
+                (const double*)records, n_binaries, (const float*)freqs, n_freqs, ntiles, (const double*)cot_plus,
+                (const double*)cot_cross, partial);
     } else if (precision == kPrecF32 || precision == kPrecSplit || precision == kPrecSplitDF) {
         if (freqs_f64)
             grid_adjoint_kernel<A, float, double><<<blocks, kBlock, 0, s>>>(
-                (const float*)records, n_binaries, (const double*)freqs, n_freqs, ntiles, group,
-                (const float*)cot_plus, (const float*)cot_cross, dldf);
+                (const float*)records, n_binaries, (const double*)freqs, n_freqs, ntiles, (const float*)cot_plus,
+                (const float*)cot_cross, partial);
         else
             grid_adjoint_kernel<A, float, float><<<blocks, kBlock, 0, s>>>(
-                (const float*)records, n_binaries, (const float*)freqs, n_freqs, ntiles, group,
-                (const float*)cot_plus, (const float*)cot_cross, dldf);
+                (const float*)records, n_binaries, (const float*)freqs, n_freqs, ntiles, (const float*)cot_plus,
+                (const float*)cot_cross, partial);
     } else {
         return fail((who + ": unknown precision").c_str());
     }
+    group_sum_kernel<<<out_blocks, kBlock, 0, s>>>(partial, groups, n_freqs, dldf);
     return check_launch(name);
 }
 
@@ -1846,14 +1881,6 @@ __global__ __launch_bounds__(kBlock) void xhm_grid_sum_kernel(const T* __restric
     for (int64_t b = b0; b < b1; ++b) s += (double)gf[b * n + i];
     grid_partial[g * n + i] = s;
 }
-__global__ __launch_bounds__(kBlock) void xhm_grid_total_kernel(const double* __restrict__ grid_partial,
-                                                                int64_t groups, int64_t n, double* __restrict__ dldf) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    double s = 0.0;
-    for (int64_t g = 0; g < groups; ++g) s += grid_partial[g * n + i];
-    dldf[i] = s;
-}
 
 // each binary's gradient (and loss) from the waves' sums, in a fixed order:
 // d_L's derivative -Re conj(G) h / d_L, f_ref's half phi_ref's times the
@@ -1947,7 +1974,7 @@ extern "C" {
 const char* wfx_last_error(void) { return g_error.c_str(); }
 
 // bumped whenever an entry point's signature changes
-int wfx_abi_version(void) { return 24; }
+int wfx_abi_version(void) { return 25; }
 
 int wfx_xas_record_slots(void) { return kSlots; }
 
@@ -2140,10 +2167,16 @@ int wfx_xas_adjoint(const void* records, const double* jacobian, int64_t n_binar
                                                 grads, stream);
 }
 
+int64_t wfx_xas_grid_adjoint_workspace(int64_t n_binaries, int64_t n_freqs) {
+    if (n_binaries <= 0 || n_freqs <= 0) return 0;
+    return grid_adjoint_bytes(n_binaries, n_freqs);
+}
+
 int wfx_xas_grid_adjoint(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
-                         int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* stream) {
+                         int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* workspace,
+                         int64_t workspace_bytes, void* stream) {
     return grid_adjoint<XasAdjoint>("wfx_xas_grid_adjoint", records, n_binaries, freqs, freqs_f64, n_freqs, precision,
-                                    cot_plus, cot_cross, dldf, stream);
+                                    cot_plus, cot_cross, dldf, workspace, workspace_bytes, stream);
 }
 
 int wfx_xas_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
@@ -2207,10 +2240,11 @@ int wfx_phd_adjoint(const void* records, const double* jacobian, int64_t n_binar
 }
 
 int wfx_phd_grid_adjoint(const void* records, int64_t n_binaries, const void* freqs, int freqs_f64, int64_t n_freqs,
-                         int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* stream) {
+                         int precision, const void* cot_plus, const void* cot_cross, double* dldf, void* workspace,
+                         int64_t workspace_bytes, void* stream) {
     if (precision == kPrecF32) return fail("wfx_phd_grid_adjoint: IMRPhenomD has no FP32-lead records");
     return grid_adjoint<PhenomDAdjoint>("wfx_phd_grid_adjoint", records, n_binaries, freqs, freqs_f64, n_freqs,
-                                        precision, cot_plus, cot_cross, dldf, stream);
+                                        precision, cot_plus, cot_cross, dldf, workspace, workspace_bytes, stream);
 }
 
 int wfx_phd_loss_adjoint(const void* records, const double* jacobian, int64_t n_binaries, const void* freqs,
@@ -2352,7 +2386,7 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
     if (dldf) {
         double* grid_partial = (double*)(ws + g.grid_partial);
         xhm_grid_sum_kernel<T><<<grid_blocks_1, kBlock, 0, s>>>(gfm, nb, n, g.grid_groups, grid_partial);
-        xhm_grid_total_kernel<<<grid_blocks_2, kBlock, 0, s>>>(grid_partial, g.grid_groups, n, dldf);
+        group_sum_kernel<<<grid_blocks_2, kBlock, 0, s>>>(grid_partial, g.grid_groups, n, dldf);
     }
 }
 
