@@ -44,6 +44,16 @@ METRIC = "XAS waveforms/s @4017 bins: FP32-split vs FP64 GPU (mismatch<=1e-6); f
 # under profiles/); FP64 FMA issues at the non-packed FP32 rate on this chip
 XHM_VALU_PER_BIN = {(4096, 8193): 1822.4}
 XHM_SPLIT_VALU_PER_BIN = {(4096, 8193): 629.9}
+# The algorithmic work per bin of the same workload (the oracle's op
+# counter, oracle_xhm_count, on its 4096 binaries: the branches each bin
+# takes, the M f powers from the frequency's; pinned by
+# tests/test_oracle_xhm.py::test_op_count): FP32 issue slots with the
+# transcendentals at the quarter rate (xhm.op_slots) for the split
+# evaluation; for FP64, its add, multiply, divide and compare operations
+# (267.4) beside 17.6 elementary-function calls, which FP64 runs in software
+XHM_SPLIT_SLOTS_PER_BIN = {(4096, 8193): 445.9}
+XHM_FP64_OPS_PER_BIN = {(4096, 8193): 267.4}
+XHM_FP64_CALLS_PER_BIN = {(4096, 8193): 17.6}
 
 
 def draw_xhm(n, seed=4):
@@ -309,12 +319,16 @@ def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
                    "xhm_split_roofline_hbm": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                                               "achieved": 16.0 * nb * n / (ms_split_eval * 1e-3) / 1e9,
                                               "frac": 16.0 * nb * n / (ms_split_eval * 1e-3) / 1e9 / HBM_PEAK_GBS}})
-    per_split = XHM_SPLIT_VALU_PER_BIN.get((nb, n))
+    per_split = XHM_SPLIT_SLOTS_PER_BIN.get((nb, n))
     if per_split is not None:
+        # algorithmic work (the formula's FP32 issue slots) per second against the VALU issue rate; the
+        # instructions the kernel issues beside it
         tops = per_split * nb * n / (ms_split_eval * 1e-3) / 1e12
+        issued = XHM_SPLIT_VALU_PER_BIN[(nb, n)] * nb * n / (ms_split_eval * 1e-3) / 1e12
         result["xhm_split_roofline"] = {"bound": "valu", "achieved": tops, "peak": VALU_PEAK_TOPS,
-                                        "unit": "T instructions/s", "frac": tops / VALU_PEAK_TOPS,
-                                        "per_bin": per_split}
+                                        "unit": "T op slots/s", "frac": tops / VALU_PEAK_TOPS,
+                                        "per_bin": per_split, "issued_per_bin": XHM_SPLIT_VALU_PER_BIN[(nb, n)],
+                                        "issue_frac": issued / VALU_PEAK_TOPS}
     # the gradient: the fused split loss and gradient (adjoint.loss_and_grad: the dual-number set-up, the
     # residuals, each mode's reverse-mode adjoint) of L = sum_f w(f) (|h+ - d+|^2 + |hx - dx|^2), d at
     # parameters perturbed by 1%, every binary; against the FP64 path's exact gradient of the same loss
@@ -358,11 +372,16 @@ def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
                    "xhm_grad_worst_fraction_of_float32_bar": float(((g32 - g64).abs() / bar)[:, :7].max()),
                    "xhm_loss_rel_err_split_vs_fp64": float(((loss32 - loss64).abs() / loss64).max())})
     del d_plus, d_cross
-    per_bin = XHM_VALU_PER_BIN.get((nb, n))
+    per_bin = XHM_FP64_OPS_PER_BIN.get((nb, n))
     if per_bin is not None:
+        # the formula's FP64 arithmetic (add, multiply, divide, compare; its elementary functions apart) per
+        # second against the FP64 VALU rate (the non-packed FP32 rate); the instructions issued beside it
         tops = per_bin * nb * n / (ms_eval * 1e-3) / 1e12
+        issued = XHM_VALU_PER_BIN[(nb, n)] * nb * n / (ms_eval * 1e-3) / 1e12
         result["xhm_roofline"] = {"bound": "valu", "achieved": tops, "peak": VALU_PEAK_TOPS,
-                                  "unit": "T instructions/s", "frac": tops / VALU_PEAK_TOPS, "per_bin": per_bin}
+                                  "unit": "T FP64 ops/s", "frac": tops / VALU_PEAK_TOPS, "per_bin": per_bin,
+                                  "calls_per_bin": XHM_FP64_CALLS_PER_BIN[(nb, n)],
+                                  "issued_per_bin": XHM_VALU_PER_BIN[(nb, n)], "issue_frac": issued / VALU_PEAK_TOPS}
     del out, state
     if cpu:
         from oracle import xhm as oxhm

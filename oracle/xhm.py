@@ -40,6 +40,7 @@ def lib():
         handle.oracle_xhm_mode_amplitude_phase.argtypes = [d, D, I, d, L, d, d, P]
         handle.oracle_swsh.argtypes = [D, D, I, I, d]
         handle.oracle_xhm_collocation.argtypes = [d, D, I, d]
+        handle.oracle_xhm_count.argtypes = [d, L, d, L, np.ctypeslib.ndpointer(np.int64, flags="C")]
         _ready = True
     return handle
 
@@ -146,3 +147,34 @@ def jacobian(params, freqs, amp_scale: float = 1.0, threads: int = 0):
         jp[:, :, j] = np.where(h[:, None] > 0, acc_p / safe, 0.0)
         jc[:, :, j] = np.where(h[:, None] > 0, acc_c / safe, 0.0)
     return jp, jc
+
+
+#: the op tally's kinds, in oracle_xhm_count's order
+OP_KINDS = ("add", "mul", "div", "sqrt", "exp", "log", "sincos", "atan", "cmp", "cbrt", "pow")
+
+
+def count_ops(params, freqs):
+    """Algorithmic op counts (per-frequency part, per-bin part) of the
+    polarizations of ``params`` (B, 8) on the shared grid ``freqs``: the
+    branches each bin takes, the powers of M f derived from the frequency's
+    (``oracle_xhm_count``): dicts of :data:`OP_KINDS`."""
+    p = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+    f = np.ascontiguousarray(freqs, dtype=np.float64)
+    out = np.zeros(22, dtype=np.int64)
+    lib().oracle_xhm_count(p, len(p), f, len(f), out)
+    return dict(zip(OP_KINDS, out[:11].tolist())), dict(zip(OP_KINDS, out[11:].tolist()))
+
+
+def op_slots(per_bin: dict, bins: int) -> float:
+    """Issue slots per bin of a tally in FP32 hardware terms: one per add,
+    multiplication, comparison; a division as a reciprocal and a
+    multiplication; the transcendentals (exp, log, sqrt, cbrt, a sine with
+    its cosine counted twice, an arctangent as its fitted polynomial's eight
+    operations and a reciprocal) at the quarter rate of the hardware's
+    transcendental unit, four slots each."""
+    quarter = 4.0
+    slots = per_bin["add"] + per_bin["mul"] + per_bin["cmp"] + per_bin["div"] * (1 + quarter)
+    slots += quarter * (per_bin["exp"] + per_bin["log"] + per_bin["sqrt"] + per_bin["cbrt"] + 2 * per_bin["sincos"])
+    slots += per_bin["atan"] * (8 + quarter)
+    return slots / bins
+

@@ -803,13 +803,14 @@ static double intermediate_phase_ansatz(double ff, const Powers<double>& p, cons
     if (hm.tag == 32) dphase = o_fma(ph.c3, p.m_three, dphase);
     return dphase + lorentzian;
 }
-static double intermediate_phase_ansatz_int(double ff, const Powers<double>& p, const HmVars& hm, const HmPhase& ph) {
+template <class R>
+static R intermediate_phase_ansatz_int(const R& ff, const Powers<R>& p, const HmVars& hm, const HmPhase& ph) {
     double fda = hm.f_damp, frd = hm.f_ring;
-    double phase_ir = o_fma(ph.c0, ff, ph.c1 * p.log);
+    R phase_ir = o_fma(ph.c0, ff, ph.c1 * p.log);
     phase_ir = o_fma(-ph.c2, p.m_one, phase_ir);
     phase_ir = o_fma(-(1.0 / 3.0 * ph.c4), p.m_three, phase_ir);
     if (hm.tag == 32) phase_ir = o_fma(-(0.5 * ph.c3), p.m_two, phase_ir);
-    return o_fma(ph.c_l, std::atan((ff - frd) / fda), phase_ir);
+    return o_fma(ph.c_l, o_atan((ff - frd) / fda), phase_ir);
 }
 
 // phase.py:199-202 GetfcutInsp
@@ -1236,6 +1237,68 @@ static void xhm_evaluate(const XhmSetup& s, double inclination, const double* fr
     }
 }
 
+// ------------------------------------------------------------- op count
+
+// The algorithmic work of one bin, tallied on the Counted scalar: each
+// higher mode's amplitude and phase in the region the bin falls in (the
+// branches the device takes, where the reference's jnp.where evaluates
+// every region), the (2,2) by IMRPhenomXAS's table form (the device's
+// (2,2) is that record), each mode's sine and cosine, and the sum with the
+// factors. The powers of M f come from the frequency's (counted once per
+// frequency, as the device shares them over binaries) through the
+// binary's M: M f, (M f)^(1/3), 1/(M f), log(M f) and (M f)^(-7/6), five
+// operations per bin; the other powers the formula reads are given.
+static inline Counted o_hypot(const Counted& x, const Counted& y) {
+    g_tally.mul += 2;
+    g_tally.add += 1;
+    g_tally.sqrt_ += 1;
+    return Counted(o_hypot(x.v, y.v));
+}
+static inline Counted o_atan2(const Counted& y, const Counted& x) {
+    g_tally.atan_ += 1;
+    g_tally.div += 1;
+    return Counted(std::atan2(y.v, x.v));
+}
+
+template <template <class> class S>
+static S<Counted> counted(const S<double>& s) {
+    static_assert(sizeof(S<Counted>) == sizeof(S<double>), "a struct of doubles");
+    S<Counted> out;
+    std::memcpy(static_cast<void*>(&out), static_cast<const void*>(&s), sizeof(out));
+    return out;
+}
+
+template <class R>
+static void mode_point_counted(const R& mf, const Powers<R>& p, const HmMode& c, const R& h22_re, const R& h22_im,
+                               R& a, R& phi) {
+    const HmVars& hm = c.hm;
+    const HmAmp& amp = c.amp;
+    const HmPhase& ph = c.ph;
+    const bool amp_ins = mf < amp.f_amp_match_in, amp_rd = !amp_ins && mf >= amp.f_amp_match_im;
+    const bool ph_ins = mf < ph.f_phase_match_in, ph_rd = !ph_ins && mf >= ph.f_phase_match_im;
+    R rot_re = R(0.0), rot_im = R(0.0);
+    if (hm.mixing && (amp_rd || ph_rd)) {
+        double amp0 = std::fabs(c.amp0);
+        rotate(mf, p, c, h22_re / amp0, h22_im / amp0, rot_re, rot_im);
+    }
+    if (amp_ins)
+        a = inspiral_amp_ansatz(p, amp);
+    else if (amp_rd)
+        a = hm.mixing ? o_hypot(rot_re, rot_im) : rd_amp_ansatz(mf, hm, amp);
+    else
+        a = inter_amp_ansatz(mf, p, amp);
+    if (ph_ins)
+        phi = o_fma(ph.c1_insp, mf, inspiral_phase_ansatz_int(mf, p, ph)) + ph.c_insp;
+    else if (ph_rd)
+        phi = o_fma(ph.c1_rd, mf,
+                    hm.mixing ? o_atan2(rot_im, rot_re)
+                              : ringdown_phase_ansatz_int(mf, p, hm.f_ring, hm.f_damp, false, ph, c.sph)) +
+              ph.c_rd;
+    else
+        phi = intermediate_phase_ansatz_int(mf, p, hm, ph);
+    phi = phi + ph.deltaphi;
+}
+
 extern "C" {
 
 // The polarizations (and, with modes non-null, the five modes h_{l,-m}
@@ -1306,6 +1369,72 @@ void oracle_xhm_collocation(const double* params, double f_min, int k, double* o
     for (int i = 0; i < c.n_colloc_phase; ++i) out[1 + i] = c.colloc_phase_f[i], out[7 + i] = c.colloc_phase_v[i];
     out[13] = c.n_colloc_amp;
     for (int i = 0; i < c.n_colloc_amp; ++i) out[14 + i] = c.colloc_amp_f[i], out[20 + i] = c.colloc_amp_v[i];
+}
+
+// The algorithmic work of the polarizations of nb binaries on a shared grid
+// (mode_point_counted): out[0..10] = {add, mul, div, sqrt, exp, log,
+// sincos, atan, cmp, cbrt, pow} of the per-frequency powers, out[11..21]
+// those of the bins.
+void oracle_xhm_count(const double* params, long nb, const double* freqs, long n, long long* out) {
+    std::memset(&g_tally, 0, sizeof g_tally);
+    std::vector<FreqPowers<Counted>> fps(n);
+    for (long i = 0; i < n; ++i) fps[i] = frequency_powers(Counted(freqs[i]));
+    OpTally freq = g_tally;
+    std::memset(&g_tally, 0, sizeof g_tally);
+    for (long b = 0; b < nb; ++b) {
+        XhmSetup s;
+        xhm_setup(params + 8 * b, freqs[0], 1e21, s);
+        double pb[8];
+        for (int k = 0; k < 8; ++k) pb[k] = params[8 * b + k];
+        std::vector<double> tab(TABLE_SIZE);
+        table_of<double>(pb, freqs[0], 1e21, tab.data());
+        std::complex<double> fp[5], fc[5];
+        mode_factors(params[8 * b + 5], 2, 2, fp[0], fc[0]);
+        for (int k = 0; k < 4; ++k) mode_factors(params[8 * b + 5], HM_ELL[k], HM_EMM[k], fp[k + 1], fc[k + 1]);
+        for (long i = 0; i < n; ++i) {
+            // the (2,2): IMRPhenomXAS's table form with kp = 1, kc = 0 (its polarization factors' products
+            // counted below with the other modes')
+            Counted hp22[2], hc22[2];
+            OpTally before = g_tally;
+            evaluate_point(tab.data(), fps[i], hp22, hc22);
+            // evaluate_point's products by kp and kc (four multiplications) are not the mode's
+            g_tally.mul = before.mul + (g_tally.mul - before.mul) - 4;
+            // the powers of M f from the frequency's and the binary's M (five operations)
+            const double mf_v = s.wf.m_sec * freqs[i];
+            Counted mf = Counted(s.wf.m_sec) * Counted(freqs[i]);
+            g_tally.mul += 3;
+            g_tally.add += 1;
+            g_tally.sqrt_ += 1;
+            const Powers<double> pd = powers(mf_v);
+            const Powers<Counted> p = counted(pd);
+            Counted h[5][2];
+            h[0][0] = hp22[0];
+            h[0][1] = hp22[1];
+            for (int k = 0; k < 4; ++k) {
+                const HmMode& c = s.modes[k];
+                Counted a, phi;
+                mode_point_counted(mf, p, c, h[0][0], h[0][1], a, phi);
+                Counted r = Counted(c.amp0) * a;
+                h[k + 1][0] = r * o_cos(phi);
+                h[k + 1][1] = r * o_sin(phi);
+            }
+            // (h+, hx) = sum_k (F+, Fx)_k h_k: four complex multiply-adds per mode
+            Counted pr, pi, cr, ci;
+            for (int k = 0; k < 5; ++k) {
+                pr = pr + (Counted(fp[k].real()) * h[k][0] - Counted(fp[k].imag()) * h[k][1]);
+                pi = pi + (Counted(fp[k].real()) * h[k][1] + Counted(fp[k].imag()) * h[k][0]);
+                cr = cr + (Counted(fc[k].real()) * h[k][0] - Counted(fc[k].imag()) * h[k][1]);
+                ci = ci + (Counted(fc[k].real()) * h[k][1] + Counted(fc[k].imag()) * h[k][0]);
+            }
+        }
+    }
+    OpTally bins = g_tally;
+    long long* f = reinterpret_cast<long long*>(&freq);
+    long long* g = reinterpret_cast<long long*>(&bins);
+    for (int k = 0; k < 11; ++k) {
+        out[k] = f[k];
+        out[11 + k] = g[k];
+    }
 }
 
 // {}_{-2}Y_{lm}(theta, phi) as (re, im)

@@ -215,3 +215,35 @@ def test_collocation_conditions(case):
         for f, v in zip(c["amp_f"], c["amp_v"]):
             a, _ = xhm.mode_amplitude_phase(p, case[7], k, np.array([f]))
             assert a[0] == pytest.approx(v, rel=1e-10), (mode, f)
+
+
+def test_op_count():
+    """The algorithmic work of IMRPhenomXHM per bin (oracle_xhm_count: the
+    branches each bin takes, the M f powers from the frequency's): one
+    sine-cosine per mode (five) below every ringdown, one more where the
+    (3,2)'s ringdown rotates the (2,2); the frequency's powers counted once
+    per frequency; the add/multiply count frozen within a range on the
+    multimode workload."""
+    params = xhm.draw(16, seed=4)
+    m_total = (params[:, 0] + params[:, 1]).max()
+    low = np.geomspace(20.0, 0.002 / (m_total * MTSUN), 64)  # every mode in its inspiral
+    per_freq, per_bin = xhm.count_ops(params, low)
+    bins = 16 * len(low)
+    assert per_bin["sincos"] == 5 * bins
+    assert per_freq["exp"] == 2 * len(low) and per_freq["log"] == 3 * len(low) and per_freq["sincos"] == 0
+    f = np.linspace(20.0, 2048.0, 8193)
+    per_freq, per_bin = xhm.count_ops(params, f)
+    bins = 16 * len(f)
+    assert 5 * bins < per_bin["sincos"] <= 6 * bins
+    flops = (per_bin["add"] + per_bin["mul"]) / bins
+    assert 180 < flops < 280, flops
+    assert 350 < xhm.op_slots(per_bin, bins) < 550
+    # bench.py prices the IMRPhenomXHM rooflines with these counts on its workload (4096 binaries,
+    # XHM_SPLIT_SLOTS_PER_BIN 445.9, XHM_FP64_OPS_PER_BIN 267.4): its first 16 binaries agree within 5%
+    import bench
+
+    p16 = bench.draw_xhm(4096)[:16]
+    _, pb = xhm.count_ops(p16, f)
+    assert abs(xhm.op_slots(pb, bins) / bench.XHM_SPLIT_SLOTS_PER_BIN[(4096, 8193)] - 1) < 0.05
+    ops = (pb["add"] + pb["mul"] + pb["div"] + pb["cmp"]) / bins
+    assert abs(ops / bench.XHM_FP64_OPS_PER_BIN[(4096, 8193)] - 1) < 0.05
