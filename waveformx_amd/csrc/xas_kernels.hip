@@ -1536,15 +1536,20 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
 // contracted once with the record's derivatives. The (3,2)'s ringdown
 // recycles the (2,2): its wave writes the cotangent it passes to the (2,2)
 // per bin (gh22), which the (2,2)'s wave, launched after, adds to its own.
-// Per wave, partial[((b * 5 + row) * chunks + c) * 8 + j] (row 0 the (2,2),
-// 1..4 the higher modes): Re conj(cot) dh/d(mass_1, mass_2, chi_1z, chi_2z)
+// A higher mode's bins in the ringdown of both its amplitude and its phase
+// (most of them on a linear grid) go to waves of their own with compact
+// accumulators (xhm_hm_rd_adjoint_*: 16 slots for a plain mode, 26 for the
+// mixing (3,2)), so that more waves fit a SIMD; the general waves skip them.
+// Per wave, partial[((b * 9 + row) * chunks + c) * 8 + j] (row 0 the (2,2),
+// 1..4 the higher modes' general waves, 5..8 their ringdown waves):
+// Re conj(cot) dh/d(mass_1, mass_2, chi_1z, chi_2z)
 // (0..3); Re conj(G) h (4); Re conj(cot) dF/d(iota) h (5); Re m conj(G) i h
 // (6), G = conj(F+) cot+ + conj(Fx) cotx the mode's cotangent; 0 (7).
 // With the grid's gradient, each higher mode's wave writes its bins' dL/df
 // (gfm[k][b][i]), the (2,2)'s wave adds them to its own (into gfm[0][b][i]),
 // and two passes sum them over the binaries in a fixed order.
 constexpr int kXhmGradSlots = 8;
-constexpr int kXhmGradRows = 5;
+constexpr int kXhmGradRows = 9;
 // at least this many higher-mode waves per launch (the chip holds ~3,000
 // at the kernels' occupancy)
 constexpr int64_t kXhmGradWaves = 12288;
@@ -1738,11 +1743,12 @@ __device__ __forceinline__ void xhm_hm_adjoint_wave(
         if (on) {
             const PowerRow<T> row = table[i];
             const BinPowers<T> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
+            const T mf = r[kHmMsec] * p.f;
+            if (hm_ringdown_bin(r, mf)) continue;  // the ringdown waves' bin
             const vec2_t<T> cp = cot_plus[b * n + i], cc = cot_cross[b * n + i];
             T g_re, g_im;
             mode_cotangent(fac, cp, cc, g_re, g_im);
             T w_re = T(0), w_im = T(0);
-            const T mf = r[kHmMsec] * p.f;
             if (mixing && (mf >= r[kHmFAmpIm] || mf >= r[kHmFPhaseIm])) {
                 // the (2,2) over |amp0|, as the forward computes it
                 T a22, phi22, s22, c22;
@@ -1810,6 +1816,111 @@ __global__ __launch_bounds__(64) void xhm_hm_adjoint_f64_kernel(
     double* __restrict__ partial, double* __restrict__ gfm) {
     xhm_hm_adjoint_wave<double, Grid>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, span, cot_plus,
                                       cot_cross, gh22, partial, gfm);
+}
+
+// one wave per (binary, span, higher mode) on the mode's ringdown bins
+// (hm_ringdown_bin): the plain modes (2,1), (3,3), (4,4) with 16 compact
+// accumulators (Mixing false; blocks (b, c, 3 modes)), or the mixing (3,2)
+// with 26 (Mixing true; blocks (b, c)), its (2,2) recomputed and the
+// cotangent of the (2,2) written per bin as the general wave does
+template <class T, bool Grid, bool Mixing>
+__device__ __forceinline__ void xhm_hm_rd_adjoint_wave(
+    const T* __restrict__ rec22, const T* __restrict__ head, const T* __restrict__ dhead, const T* __restrict__ modes,
+    const T* __restrict__ dmodes, int64_t nb, const PowerRow<T>* __restrict__ table, int64_t n, int64_t chunks,
+    int64_t span, const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
+    vec2_t<T>* __restrict__ gh22, double* __restrict__ partial, T* __restrict__ gfm) {
+    constexpr int kModes = Mixing ? 1 : 3;
+    constexpr int kAcc = Mixing ? (int)kMixAcc : (int)kRdAcc;
+    const int64_t x = blockIdx.x;
+    const int km = (int)(x % kModes);
+    const int k = Mixing ? kHmMixMode : (km < 2 ? km : 3);
+    const int64_t c = (x / kModes) % chunks;
+    const int64_t b = x / kModes / chunks;
+    if (b >= nb) return;
+    const int lane = threadIdx.x;
+    __shared__ T s_mode[kHmSlots], s_22[kSlots];
+    for (int j = lane; j < kHmSlots; j += 64) s_mode[j] = modes[(b * kHmModes + k) * kHmSlots + j];
+    if (Mixing) s_22[lane] = rec22[b * kSlots + lane];
+    __syncthreads();
+    const T* r = s_mode;
+    double* out = partial + ((b * kXhmGradRows + 5 + k) * chunks + c) * kXhmGradSlots;
+    if (r[kHmOn] == T(0)) {
+        // the mode vanishes: the general wave writes its every bin
+        if (lane < kXhmGradSlots) out[lane] = 0.0;
+        return;
+    }
+    const T* fac = head + b * kHeadSlots + kHeadFactors + 4 * (k + 1);
+    const T* dfac = dhead + b * kHeadSlots + kHeadFactors + 4 * (k + 1);
+    const T emm = (T)kHmEmm[k];
+    const T amp0 = r[kHmAmp0];
+    const T inv_amp0 = d_rcp(fabs(amp0));
+    const T sign_inv = amp0 < T(0) ? -inv_amp0 : inv_amp0;
+    const int64_t i0 = c * span;
+    const int64_t i1 = i0 + span < n ? i0 + span : n;
+    T acc[kAcc];
+#pragma unroll
+    for (int j = 0; j < kAcc; ++j) acc[j] = T(0);
+    T v4 = T(0), v5 = T(0), v6 = T(0);
+    for (int64_t i = i0 + lane; i < i1; i += 64) {
+        const PowerRow<T> row = table[i];
+        const BinPowers<T> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
+        if (!hm_ringdown_bin(r, r[kHmMsec] * p.f)) continue;  // the general wave's bin
+        const vec2_t<T> cp = cot_plus[b * n + i], cc = cot_cross[b * n + i];
+        T g_re, g_im;
+        mode_cotangent(fac, cp, cc, g_re, g_im);
+        T h_re, h_im, gf = T(0);
+        if constexpr (Mixing) {
+            T a22, phi22, s22, c22;
+            mode22_split(s_22, split_powers(row), a22, phi22);
+            m_sincos(phi22, &s22, &c22);
+            const T w_re = a22 * c22 * inv_amp0, w_im = a22 * s22 * inv_amp0;
+            T gw_re, gw_im;
+            hm_adjoint_ringdown_mixing<Grid>(r, p, w_re, w_im, g_re, g_im, acc, h_re, h_im, gw_re, gw_im, gf);
+            // w's dependence on amp0 (w = h_22 / |amp0|)
+            acc[xAmp0] -= (gw_re * w_re + gw_im * w_im) * sign_inv;
+            gh22[b * n + i] = vec2_t<T>{gw_re * inv_amp0, gw_im * inv_amp0};
+        } else {
+            hm_adjoint_ringdown<Grid>(r, p, g_re, g_im, acc, h_re, h_im, gf);
+        }
+        v4 += g_re * h_re + g_im * h_im;
+        v6 += emm * (g_im * h_re - g_re * h_im);
+        v5 += inclination_term(dfac, cp, cc, h_re, h_im);
+        if (Grid) gfm[(k * nb + b) * n + i] = gf;
+    }
+    // lane j: the total of accumulator j, contracted with its slot's
+    // derivatives
+    T full[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) full[j] = j < kAcc ? acc[j] : T(0);
+    const double total = (double)reduce_scatter64(full, lane);
+    double d[4] = {0.0, 0.0, 0.0, 0.0};
+    if (lane < kAcc) {
+        const int slot = Mixing ? kMixAccSlot[lane] : kRdAccSlot[lane];
+        const T* t = dmodes + ((b * kHmModes + k) * kHmSlots + slot) * 4;
+        for (int j = 0; j < 4; ++j) d[j] = t[j] == T(0) ? 0.0 : total * (double)t[j];
+    }
+    xhm_grad_store(d, v4, v5, v6, lane, out);
+}
+
+template <bool Grid, bool Mixing>
+__global__ __launch_bounds__(64) void xhm_hm_rd_adjoint_kernel(
+    const float* __restrict__ rec22, const float* __restrict__ head, const float* __restrict__ dhead,
+    const float* __restrict__ modes, const float* __restrict__ dmodes, int64_t nb,
+    const PowerRow<float>* __restrict__ table, int64_t n, int64_t chunks, int64_t span,
+    const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross, float2* __restrict__ gh22,
+    double* __restrict__ partial, float* __restrict__ gfm) {
+    xhm_hm_rd_adjoint_wave<float, Grid, Mixing>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, span,
+                                                cot_plus, cot_cross, gh22, partial, gfm);
+}
+template <bool Grid, bool Mixing>
+__global__ __launch_bounds__(64) void xhm_hm_rd_adjoint_f64_kernel(
+    const double* __restrict__ rec22, const double* __restrict__ head, const double* __restrict__ dhead,
+    const double* __restrict__ modes, const double* __restrict__ dmodes, int64_t nb,
+    const PowerRow<double>* __restrict__ table, int64_t n, int64_t chunks, int64_t span,
+    const double2* __restrict__ cot_plus, const double2* __restrict__ cot_cross, double2* __restrict__ gh22,
+    double* __restrict__ partial, double* __restrict__ gfm) {
+    xhm_hm_rd_adjoint_wave<double, Grid, Mixing>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, span,
+                                                 cot_plus, cot_cross, gh22, partial, gfm);
 }
 
 // one wave per (binary, span): the (2,2)'s adjoint (the IMRPhenomXAS
@@ -2371,6 +2482,42 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
                 (const double*)records22, (const double*)head, (const double*)dhead, (const double*)modes,
                 (const double*)dmodes, nb, (const PowerRow<double>*)table, n, g.chunks, g.span,
                 (const double2*)cot_plus, (const double2*)cot_cross, (double2*)gh22, partial, nullptr);
+    }
+    // the ringdown waves: the plain modes, then the mixing (3,2) (its
+    // cotangent of the (2,2) complete before the (2,2)'s waves)
+    const int rd_blocks = 3 * blocks22, mix_blocks = blocks22;
+    if (std::is_same<T, float>::value) {
+        using V2 = float2;
+        const float *r22 = (const float*)records22, *hd = (const float*)head, *dh = (const float*)dhead,
+                    *md = (const float*)modes, *dm = (const float*)dmodes;
+        const PowerRow<float>* tb = (const PowerRow<float>*)table;
+        if (dldf) {
+            xhm_hm_rd_adjoint_kernel<true, false><<<rd_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n, g.chunks,
+                g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (float*)gfm);
+            xhm_hm_rd_adjoint_kernel<true, true><<<mix_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n, g.chunks,
+                g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (float*)gfm);
+        } else {
+            xhm_hm_rd_adjoint_kernel<false, false><<<rd_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
+                g.chunks, g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
+            xhm_hm_rd_adjoint_kernel<false, true><<<mix_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
+                g.chunks, g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
+        }
+    } else {
+        using V2 = double2;
+        const double *r22 = (const double*)records22, *hd = (const double*)head, *dh = (const double*)dhead,
+                     *md = (const double*)modes, *dm = (const double*)dmodes;
+        const PowerRow<double>* tb = (const PowerRow<double>*)table;
+        if (dldf) {
+            xhm_hm_rd_adjoint_f64_kernel<true, false><<<rd_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
+                g.chunks, g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (double*)gfm);
+            xhm_hm_rd_adjoint_f64_kernel<true, true><<<mix_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
+                g.chunks, g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (double*)gfm);
+        } else {
+            xhm_hm_rd_adjoint_f64_kernel<false, false><<<rd_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
+                g.chunks, g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
+            xhm_hm_rd_adjoint_f64_kernel<false, true><<<mix_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
+                g.chunks, g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
+        }
     }
     if (dldf)
         xhm_22_adjoint_kernel<T, true><<<blocks22, 64, 0, s>>>(records22, jacobian22, head, dhead, nb, table, n,

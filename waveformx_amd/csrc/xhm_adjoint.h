@@ -75,40 +75,56 @@ struct MfGrad {
     T mf, t, m1, lg, m76;
 };
 
+// where the helpers below add: the accumulator indices of the slots they
+// touch (HmIx: the full set of HmAcc; the ringdown kernels' compact sets
+// define their own)
+struct HmIx {
+    static constexpr int FRing = aFRing, FDamp = aFDamp, Rd = aRd, FalloffAt = aFalloffAt;
+};
+
 // z = (M f - f_ring) / f_damp's adjoint (the arctangents of the
 // intermediate and ringdown phases), for its cotangent gz
-template <class T>
+template <class Ix = HmIx, class T>
 __device__ __forceinline__ void atan_z_adjoint(T gz, T z, T inv_damp, T* acc, MfGrad<T>& g) {
     g.mf += gz * inv_damp;
-    acc[aFRing] -= gz * inv_damp;
-    acc[aFDamp] -= gz * z * inv_damp;
+    acc[Ix::FRing] -= gz * inv_damp;
+    acc[Ix::FDamp] -= gz * z * inv_damp;
 }
 
 // a Lorentzian ringdown amplitude rd0 / (exp(rd1 d) (d^2 + rd2)), d = M f -
 // f_ring (its value a), for its cotangent ga
-template <class T>
+template <class Ix = HmIx, class T>
 __device__ __forceinline__ void lorentzian_adjoint(const T* __restrict__ r, T mf, T a, T ga, T* acc, MfGrad<T>& g) {
     const T dfr = mf - r[kHmFRing];
     const T e = d_exp(r[kHmRd + 1] * dfr);
     const T q = d_fma(dfr, dfr, r[kHmRd + 2]);
     const T inv_q = d_rcp(q);
-    acc[aRd] += ga * d_rcp(e * q);
-    acc[aRd + 1] -= ga * a * dfr;
-    acc[aRd + 2] -= ga * a * inv_q;
+    acc[Ix::Rd] += ga * d_rcp(e * q);
+    acc[Ix::Rd + 1] -= ga * a * dfr;
+    acc[Ix::Rd + 2] -= ga * a * inv_q;
     const T gd = -ga * a * d_fma(T(2) * dfr, inv_q, r[kHmRd + 1]);
     g.mf += gd;
-    acc[aFRing] -= gd;
+    acc[Ix::FRing] -= gd;
 }
 
 // the fall-off rd3 exp(-rd4 (M f - f_at)) (its value a), cotangent ga
-template <class T>
+template <class Ix = HmIx, class T>
 __device__ __forceinline__ void falloff_adjoint(const T* __restrict__ r, T mf, T a, T ga, T* acc, MfGrad<T>& g) {
     const T x = mf - r[kHmFalloffAt];
-    acc[aRd + 3] += ga * d_exp(-r[kHmRd + 4] * x);
-    acc[aRd + 4] -= ga * a * x;
+    acc[Ix::Rd + 3] += ga * d_exp(-r[kHmRd + 4] * x);
+    acc[Ix::Rd + 4] -= ga * a * x;
     const T gx = -ga * a * r[kHmRd + 4];
     g.mf += gx;
-    acc[aFalloffAt] -= gx;
+    acc[Ix::FalloffAt] -= gx;
+}
+
+// A bin in the ringdown of both the amplitude and the phase (most of a
+// mode's bins on a linear grid): the region predicate that the general
+// adjoint and the ringdown adjoints below split the bins by, with the same
+// arithmetic on both sides, so that every bin is taken exactly once
+template <class T>
+__device__ __forceinline__ bool hm_ringdown_bin(const T* __restrict__ r, T mf) {
+    return !(mf < r[kHmFAmpIn]) && mf >= r[kHmFAmpIm] && !(mf < r[kHmFPhaseIn]) && mf >= r[kHmFPhaseIm];
 }
 
 // One bin of higher mode r (its kHmOn set) at the powers p of f, in the
@@ -455,6 +471,204 @@ __device__ __forceinline__ void hm_adjoint_split(const T* __restrict__ r, const 
         if (ph_ins) lead = (T)(-5.0 / 3.0 * wide_slot(r, kHmT0F64) * p.m53) * g_phi;
         gf = d_fma(g.mf, r[kHmMsec], (d_fma(g.t, t * (T(1) / T(3)), g.lg - g.m1 * m1) + lead) * p.m1);
     }
+}
+
+// --------------------------------------------------------- the ringdown
+
+// The ringdown adjoints' compact accumulators. A plain mode's ringdown (the
+// Lorentzian or the fall-off, the phase with its arctangent) touches 16
+// slots; the mixing mode's (the (3,2): its rotation of the (2,2), amplitude
+// |rot| and phase arg rot) 26. M f's powers enter through M f and 1/(M f)
+// only there.
+enum RdAcc : int {
+    rAmp0 = 0, rMsec = 1, rInvM = 2, rFRing = 3, rFDamp = 4, rFalloffAt = 5,
+    rRd = 6,  // 5
+    rC1Rd = 11, rCRd = 12, rAlpha0 = 13, rAlpha2 = 14, rAlphaL = 15,
+    kRdAcc = 16
+};
+WFX_CONST int kRdAccSlot[kRdAcc] = {kHmAmp0, kHmMsec, kHmInvM, kHmFRing, kHmFDamp, kHmFalloffAt,
+                                    kHmRd, kHmRd + 1, kHmRd + 2, kHmRd + 3, kHmRd + 4,
+                                    kHmC1Rd, kHmCRd, kHmAlpha0, kHmAlpha2, kHmAlphaL};
+struct RdIx {
+    static constexpr int FRing = rFRing, FDamp = rFDamp, Rd = rRd, FalloffAt = rFalloffAt;
+};
+enum MixAcc : int {
+    xAmp0 = 0, xMsec = 1, xInvM = 2, xFRing = 3, xFDamp = 4, xFalloffAt = 5,
+    xRd = 6,      // 5
+    xRdAux = 11,  // 4
+    xC1Rd = 15, xCRd = 16,
+    xSph = 17,    // 5
+    xMix = 22,    // 4
+    kMixAcc = 26
+};
+WFX_CONST int kMixAccSlot[kMixAcc] = {kHmAmp0, kHmMsec, kHmInvM, kHmFRing, kHmFDamp, kHmFalloffAt,
+                                      kHmRd, kHmRd + 1, kHmRd + 2, kHmRd + 3, kHmRd + 4,
+                                      kHmRdAux, kHmRdAux + 1, kHmRdAux + 2, kHmRdAux + 3,
+                                      kHmC1Rd, kHmCRd,
+                                      kHmSph, kHmSph + 1, kHmSph + 2, kHmSph + 3, kHmSph + 4,
+                                      kHmMix, kHmMix + 1, kHmMix + 2, kHmMix + 3};
+struct MixIx {
+    static constexpr int FRing = xFRing, FDamp = xFDamp, Rd = xRd, FalloffAt = xFalloffAt;
+};
+
+// M f's cotangents back to the record's M slots and, with Grid, to f
+template <bool Grid, int Msec, int InvM, class T>
+__device__ __forceinline__ void ringdown_mf_adjoint(const T* __restrict__ r, const BinPowers<T>& p, T m1,
+                                                    const MfGrad<T>& g, T* acc, T& gf) {
+    acc[Msec] += g.mf * p.f;
+    acc[InvM] += g.m1 * p.m1;
+    if (Grid) gf = d_fma(g.mf, r[kHmMsec], (-(g.m1 * m1)) * p.m1);
+}
+
+// hm_adjoint_split for a plain mode's bin in the ringdown of both its
+// amplitude and its phase (hm_ringdown_bin), the same operations on the
+// 16 compact accumulators
+template <bool Grid, class T>
+__device__ __forceinline__ void hm_adjoint_ringdown(const T* __restrict__ r, const BinPowers<T>& p, T g_re, T g_im,
+                                                    T* acc, T& h_re, T& h_im, T& gf) {
+    const T mf = r[kHmMsec] * p.f;
+    const T m1 = p.m1 * r[kHmInvM];
+    const T f_ring = r[kHmFRing], f_damp = r[kHmFDamp];
+    const T z = d_div(mf - f_ring, f_damp);
+    const T atan_z = d_atan(z);
+    const T inv_damp = d_rcp(f_damp);
+    const T datan_z = d_rcp(d_fma(z, z, T(1)));
+    const bool falloff = mf >= r[kHmFalloff];
+    T a;
+    if (falloff) {
+        a = r[kHmRd + 3] * d_exp(-r[kHmRd + 4] * (mf - r[kHmFalloffAt]));
+    } else {
+        const T dfr = mf - f_ring;
+        a = d_div(r[kHmRd], d_exp(r[kHmRd + 1] * dfr) * d_fma(dfr, dfr, r[kHmRd + 2]));
+    }
+    const bool clamped = a < T(0);
+    if (clamped) a = T(1.0e-15);
+    T rest = d_fma(r[kHmC1Rd] + r[kHmAlpha0], mf, -(f_ring * f_ring * r[kHmAlpha2]) * m1);
+    rest = d_fma(r[kHmAlphaL], atan_z, rest);
+    const T phi = d_reduce(rest + r[kHmCRd]);
+    T sn, cs;
+    m_sincos(phi, &sn, &cs);
+    const T amp0 = r[kHmAmp0];
+    const T val = amp0 * a;
+    h_re = val * cs;
+    h_im = val * sn;
+
+    const T g_val = d_fma(g_re, cs, g_im * sn);
+    const T g_phi = val * d_fma(g_im, cs, -g_re * sn);
+    acc[rAmp0] += g_val * a;
+    const T g_a = clamped ? T(0) : g_val * amp0;
+    MfGrad<T> g{T(0), T(0), T(0), T(0), T(0)};
+    acc[rC1Rd] += g_phi * mf;
+    acc[rCRd] += g_phi;
+    acc[rAlpha0] += g_phi * mf;
+    g.mf += g_phi * (r[kHmC1Rd] + r[kHmAlpha0]);
+    const T a2 = r[kHmAlpha2];
+    acc[rAlpha2] -= g_phi * f_ring * f_ring * m1;
+    acc[rFRing] -= g_phi * T(2) * f_ring * a2 * m1;
+    g.m1 -= g_phi * f_ring * f_ring * a2;
+    acc[rAlphaL] += g_phi * atan_z;
+    atan_z_adjoint<RdIx>(g_phi * r[kHmAlphaL] * datan_z, z, inv_damp, acc, g);
+    if (falloff)
+        falloff_adjoint<RdIx>(r, mf, a, g_a, acc, g);
+    else
+        lorentzian_adjoint<RdIx>(r, mf, a, g_a, acc, g);
+    ringdown_mf_adjoint<Grid, rMsec, rInvM>(r, p, m1, g, acc, gf);
+}
+
+// hm_adjoint_split for the mixing mode's bin in the ringdown of both its
+// amplitude and its phase: h = amp0 rot e^{i (c1_rd M f + c_rd)}, rot the
+// rotation of w = h_22 / |amp0|, on the 26 compact accumulators; (gw_re,
+// gw_im) the cotangent of w
+template <bool Grid, class T>
+__device__ __forceinline__ void hm_adjoint_ringdown_mixing(const T* __restrict__ r, const BinPowers<T>& p, T w_re,
+                                                           T w_im, T g_re, T g_im, T* acc, T& h_re, T& h_im,
+                                                           T& gw_re, T& gw_im, T& gf) {
+    const T mf = r[kHmMsec] * p.f;
+    const T m1 = p.m1 * r[kHmInvM];
+    const T f_ring = r[kHmFRing], f_damp = r[kHmFDamp];
+    const T z = d_div(mf - f_ring, f_damp);
+    const T atan_z = d_atan(z);
+    const T inv_damp = d_rcp(f_damp);
+    const T datan_z = d_rcp(d_fma(z, z, T(1)));
+    const bool aux = mf < r[kHmFRdAux];
+    const bool falloff = !aux && mf >= r[kHmFalloff];
+    T amplm;
+    if (aux) {
+        const T* x = r + kHmRdAux;
+        amplm = d_fma(d_fma(d_fma(x[3], mf, x[2]), mf, x[1]), mf, x[0]);
+    } else if (falloff) {
+        amplm = r[kHmRd + 3] * d_exp(-r[kHmRd + 4] * (mf - r[kHmFalloffAt]));
+    } else {
+        const T dfr = mf - f_ring;
+        amplm = d_div(r[kHmRd], d_exp(r[kHmRd + 1] * dfr) * d_fma(dfr, dfr, r[kHmRd + 2]));
+    }
+    const T* sp = r + kHmSph;
+    T philm = d_fma(sp[0], mf, sp[4]);
+    philm = d_fma(-sp[2], m1, philm);
+    philm = d_fma(-(T(1) / T(3)) * sp[3], m1 * m1 * m1, philm);
+    philm = d_fma(sp[1], atan_z, philm);
+    T sn_s, cs_s;
+    m_sincos(d_reduce(philm), &sn_s, &cs_s);
+    const T m2r = r[kHmMix], m2i = r[kHmMix + 1], m3r = r[kHmMix + 2], m3i = r[kHmMix + 3];
+    const T b_re = amplm * m3r, b_im = -(m3i * amplm);
+    const T rot_re = d_fma(w_re, m2r, w_im * m2i) + d_fma(b_re, cs_s, -b_im * sn_s);
+    const T rot_im = d_fma(w_im, m2r, -w_re * m2i) + d_fma(b_re, sn_s, b_im * cs_s);
+    T a = d_sqrt(d_fma(rot_re, rot_re, rot_im * rot_im));
+    if (a < T(0)) a = T(1.0e-15);
+    const T phi = d_reduce(d_fma(r[kHmC1Rd], mf, d_atan2(rot_im, rot_re)) + r[kHmCRd]);
+    T sn, cs;
+    m_sincos(phi, &sn, &cs);
+    const T amp0 = r[kHmAmp0];
+    const T val = amp0 * a;
+    h_re = val * cs;
+    h_im = val * sn;
+
+    const T g_val = d_fma(g_re, cs, g_im * sn);
+    const T g_phi = val * d_fma(g_im, cs, -g_re * sn);
+    acc[xAmp0] += g_val * a;
+    MfGrad<T> g{T(0), T(0), T(0), T(0), T(0)};
+    // h = amp0 rot e^{i (c1_rd M f + c_rd)} exactly: rot's cotangent needs no
+    // division by |rot|
+    T e_s, e_c;
+    m_sincos(d_reduce(d_fma(r[kHmC1Rd], mf, r[kHmCRd])), &e_s, &e_c);
+    const T g_rot_re = amp0 * d_fma(g_re, e_c, g_im * e_s);
+    const T g_rot_im = amp0 * d_fma(g_im, e_c, -g_re * e_s);
+    acc[xC1Rd] += g_phi * mf;
+    acc[xCRd] += g_phi;
+    g.mf += g_phi * r[kHmC1Rd];
+    // the rotation, back to w, the spheroidal phase and amplitude
+    acc[xMix] += d_fma(g_rot_re, w_re, g_rot_im * w_im);
+    acc[xMix + 1] += d_fma(g_rot_re, w_im, -g_rot_im * w_re);
+    gw_re = d_fma(g_rot_re, m2r, -g_rot_im * m2i);
+    gw_im = d_fma(g_rot_re, m2i, g_rot_im * m2r);
+    const T g_bre = d_fma(g_rot_re, cs_s, g_rot_im * sn_s);
+    const T g_bim = d_fma(-g_rot_re, sn_s, g_rot_im * cs_s);
+    const T g_phl = g_rot_re * (-b_re * sn_s - b_im * cs_s) + g_rot_im * d_fma(b_re, cs_s, -b_im * sn_s);
+    acc[xMix + 2] += g_bre * amplm;
+    acc[xMix + 3] -= g_bim * amplm;
+    const T g_aml = d_fma(g_bre, m3r, -g_bim * m3i);
+    acc[xSph] += g_phl * mf;
+    g.mf += g_phl * sp[0];
+    acc[xSph + 4] += g_phl;
+    acc[xSph + 2] -= g_phl * m1;
+    g.m1 -= g_phl * sp[2];
+    acc[xSph + 3] -= g_phl * m1 * m1 * m1 * (T(1) / T(3));
+    g.m1 -= g_phl * sp[3] * m1 * m1;
+    acc[xSph + 1] += g_phl * atan_z;
+    atan_z_adjoint<MixIx>(g_phl * sp[1] * datan_z, z, inv_damp, acc, g);
+    if (aux) {
+        const T* x = r + kHmRdAux;
+        acc[xRdAux] += g_aml;
+        acc[xRdAux + 1] += g_aml * mf;
+        acc[xRdAux + 2] += g_aml * mf * mf;
+        acc[xRdAux + 3] += g_aml * mf * mf * mf;
+        g.mf += g_aml * d_fma(d_fma(T(3) * x[3], mf, T(2) * x[2]), mf, x[1]);
+    } else if (falloff) {
+        falloff_adjoint<MixIx>(r, mf, amplm, g_aml, acc, g);
+    } else {
+        lorentzian_adjoint<MixIx>(r, mf, amplm, g_aml, acc, g);
+    }
+    ringdown_mf_adjoint<Grid, xMsec, xInvM>(r, p, m1, g, acc, gf);
 }
 
 }  // namespace wfx
