@@ -492,18 +492,22 @@ WFX_HD inline unsigned pd_record(const R* params, const R& f_min, double amp_sca
     // -Psi and the time shift t0 M f in powers of f (core.py:293-333):
     // rows[r][basis index of phenomd/core.py:50-53]
     R rows[3][14];
+#pragma unroll
     for (int r = 0; r < 3; ++r)
+#pragma unroll
         for (int k = 0; k < 14; ++k) rows[r][k] = R(0.0);
     const R log_pis = m_log(R(kPi) * s);
     const R u = m_cbrt(R(kPi) * s);
     R u_pow[10];
     u_pow[0] = R(1.0);
     u_pow[1] = u;
+#pragma unroll
     for (int k = 2; k < 10; ++k) u_pow[k] = u_pow[k - 1] * u;
     const R u_inv = R(1.0) / u;
     const R u_pow_inv[4] = {R(1.0), u_inv, u_inv * u_inv, u_inv * u_inv * u_inv};
     // the basis index of f^((k-5)/3) and, for k = 5, 6, of its log term
     const int index[12] = {-1, -1, 0, 1, 2, 4, 5, 7, 8, 9, 10, 11};
+#pragma unroll
     for (int k = 2; k < 12; ++k) {
         const R scale = k >= 5 ? u_pow[k - 5] : u_pow_inv[5 - k];
         R coefficient = c.v[k];
@@ -525,10 +529,13 @@ WFX_HD inline unsigned pd_record(const R* params, const R& f_min, double amp_sca
     rows[2][0] = rows[2][0] + a2 / eta / s;
     rows[2][13] = rows[2][13] + R(-4.0 / 3.0) * a3 / eta * m_sqrt(s) * m_sqrt(m_sqrt(s));
     rows[2][4] = rows[2][4] + -c.c1_mrd;
+#pragma unroll
     for (int r = 0; r < 3; ++r) rows[r][8] = rows[r][8] + c.t0 * s;
 
+#pragma unroll
     for (int k = 0; k < kSlots; ++k) rec[k] = R(0.0);
     const int ins_basis[12] = {4, 0, 1, 2, 3, 5, 6, 7, 8, 9, 10, 11};
+#pragma unroll
     for (int k = 0; k < 12; ++k) rec[kPdInsConst + k] = rows[0][ins_basis[k]];
     rec[kPdIntConst] = rows[1][4];
     rec[kPdIntP1] = rows[1][8];
@@ -569,7 +576,9 @@ WFX_HD inline unsigned pd_record(const R* params, const R& f_min, double amp_sca
 
     // the amplitude
     const R prefactor = R(kPdKAmp) * R(amp_scale) * m_sqrt(m1 * m2) * m_exp(-m_log(total) / R(6.0)) / distance;
+#pragma unroll
     for (int k = 0; k < 10; ++k) rec[kPdAmpIns + k] = prefactor * c.rho[k] * u_pow[k];
+#pragma unroll
     for (int j = 0; j < 5; ++j) rec[kPdAmpInt + j] = prefactor * c.delta_int[j];
     const R g1 = c.gamma[0], g2 = c.gamma[1], g3 = c.gamma[2];
     const R width = g3 * c.f_damp;

@@ -40,9 +40,9 @@ template <int N>
 struct Dual {
     double v;
     double d[N];
-    WFX_HD Dual() : v(0.0) {
-        for (int i = 0; i < N; ++i) d[i] = 0.0;
-    }
+    // trivial, as a double's: an array of duals then needs no constructor
+    // loop, which would keep it in scratch memory (Dual() is zero)
+    Dual() = default;
     WFX_HD Dual(double x) : v(x) {
         for (int i = 0; i < N; ++i) d[i] = 0.0;
     }

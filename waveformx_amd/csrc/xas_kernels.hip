@@ -170,15 +170,17 @@ __global__ __launch_bounds__(64) void setup_f32_kernel(const double* __restrict_
 }
 
 // The models' per-binary setups, for the derivative setup: the record of a
-// binary in any scalar (with reference_mf applied as the model does) and
-// the slots of the FP32 record that take the anchor (write_f32_record).
+// binary in any scalar (with reference_mf applied as the model does), each
+// slot handed to out(k, value) once, and the slots of the FP32 record that
+// take the anchor (write_f32_record).
 struct XasRecord {
     static constexpr int kFoldA = kIntConst, kFoldB = kRdConst;
-    template <class R>
-    __device__ static unsigned record(R* p, const R& f_min, double amp_scale, double reference_mf, R* rec,
-                                      XasCoeffs22<R>* keep = nullptr) {
+    template <class R, class Out>
+    __device__ __forceinline__ static unsigned record_to(R* p, const R& f_min, double amp_scale,
+                                                         double reference_mf, Out& out,
+                                                         XasCoeffs22<R>* keep = nullptr) {
         with_reference_mf(p, reference_mf);
-        return xas_record<R>(p, f_min, amp_scale, rec, keep);
+        return xas_record_to<R>(p, f_min, amp_scale, out, keep);
     }
     // The record's derivative with respect to parameter dir (d_L,
     // inclination, phi_ref or f_ref) from its FP64 values v, into col[k *
@@ -222,9 +224,14 @@ struct XasRecord {
 
 struct PhenomDRecord {
     static constexpr int kFoldA = kPdIntConst, kFoldB = kPdMrdConst;
-    template <class R>
-    __device__ static unsigned record(R* p, const R& f_min, double amp_scale, double reference_mf, R* rec) {
-        return pd_record<R>(p, f_min, amp_scale, reference_mf, rec);
+    template <class R, class Out>
+    __device__ __forceinline__ static unsigned record_to(R* p, const R& f_min, double amp_scale,
+                                                         double reference_mf, Out& out) {
+        R rec[kSlots];
+        const unsigned fl = pd_record<R>(p, f_min, amp_scale, reference_mf, rec);
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) out(k, rec[k]);
+        return fl;
     }
     // as XasRecord::closed_column, for IMRPhenomD's record
     // (phenomd/core.py:274-377)
@@ -270,6 +277,19 @@ struct PhenomDRecord {
 // (records) and/or FP32 split (split_records), either NULL. With Keep
 // (IMRPhenomXAS), the thread of (b, dir) also writes its coefficient structs
 // in dual numbers to keep[4 b + dir], for IMRPhenomXHM's higher modes.
+// Each slot leaves the set-up as soon as it is final: its derivative to the
+// thread's Jacobian column, its value (direction 0's) to the wave's LDS, so
+// that the set-up's registers never hold the whole record in dual numbers.
+struct JvpSink {
+    double* col;   // the thread's column: slot k at col[8 k]
+    double* vals;  // the binary's values, slot k at vals[k]
+    bool lead;     // direction 0: writes the values
+    __device__ void operator()(int k, const Dual<1>& x) const {
+        col[k * 8] = k < 60 ? x.d[0] : 0.0;
+        if (lead) vals[k] = x.v;
+    }
+};
+
 template <class S, bool Keep = false>
 __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict__ params, int64_t nb,
                                                        const double* __restrict__ f_min, int64_t f_min_stride,
@@ -280,31 +300,34 @@ __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict_
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t b = gid >> 2;
     const int dir = (int)(gid & 3);
-    if (b >= nb) return;
+    const bool live = b < nb;
+    // the 16 binaries' values of the block (one wave)
+    __shared__ double s_vals[16][kSlots];
+    double* vals = s_vals[threadIdx.x >> 2];
     using D = Dual<1>;
-    D p[kParams];
-    for (int k = 0; k < kParams; ++k) {
-        p[k] = D(params[b * kParams + k]);
-        p[k].d[0] = k == dir ? 1.0 : 0.0;
+    unsigned fl = 0;
+    if (live) {
+        D p[kParams];
+        for (int k = 0; k < kParams; ++k) {
+            p[k] = D(params[b * kParams + k]);
+            p[k].d[0] = k == dir ? 1.0 : 0.0;
+        }
+        JvpSink out{jac + b * kSlots * 8 + dir, vals, dir == 0};
+        if constexpr (Keep)
+            fl = S::template record_to<D>(p, D(f_min[b * f_min_stride]), amp_scale, reference_mf, out, keep + gid);
+        else
+            fl = S::template record_to<D>(p, D(f_min[b * f_min_stride]), amp_scale, reference_mf, out);
     }
-    D rec[kSlots];
-    unsigned fl;
-    if constexpr (Keep) {
-        XasCoeffs22<D> kept;
-        fl = S::template record<D>(p, D(f_min[b * f_min_stride]), amp_scale, reference_mf, rec, &kept);
-        keep[gid] = kept;
-    } else {
-        fl = S::template record<D>(p, D(f_min[b * f_min_stride]), amp_scale, reference_mf, rec);
-    }
+    __syncthreads();
+    if (!live) return;
     double* out = jac + b * kSlots * 8;
-    for (int k = 0; k < kSlots; ++k) out[k * 8 + dir] = k < 60 ? rec[k].d[0] : 0.0;
-    double v[kSlots];
-    for (int k = 0; k < kSlots; ++k) v[k] = rec[k].v;
     double pv[kParams];
     for (int k = 0; k < kParams; ++k) pv[k] = params[b * kParams + k];
     for (int k = 0; k < kSlots; ++k) out[k * 8 + dir + 4] = 0.0;
-    S::closed_column(dir + 4, pv, reference_mf, fl, v, out + dir + 4, 8);
+    S::closed_column(dir + 4, pv, reference_mf, fl, vals, out + dir + 4, 8);
     if (dir == 0) {
+        double v[kSlots];
+        for (int k = 0; k < kSlots; ++k) v[k] = vals[k];
         v[kT0Lo] = 0.0;
         v[kAnchorLo] = 0.0;
         if (records)
@@ -1333,10 +1356,10 @@ __global__ __launch_bounds__(64) void xhm_setup_jvp_kernel(const double* __restr
     const int k = g >> 2, dir = g & 3;
     using D = Dual<1>;
     // the (2,2)'s coefficients in this direction, from its derivative
-    // set-up (the same arithmetic)
-    const XasCoeffs22<D> c22 = kept[4 * b + dir];
+    // set-up (the same arithmetic), read where the set-up uses them
+    const XasCoeffs22<D>& c22 = kept[4 * b + dir];
     const WaveformVars<D>& wf = c22.wf;
-    HmCoeffs<D> c;
+    HmCoeffs<D> c{};
     hm_setup(wf, c22.amp, c22.ph, k, c);
     const int64_t at = (b * kHmModes + k) * kHmSlots;
     if (split) {
@@ -1527,9 +1550,10 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
 
 // The gradient of IMRPhenomXHM's polarizations, reverse mode, in the split
 // layout's formula: FP32 (the split gradient) or FP64. Each binary's grid
-// is cut into `chunks` spans of `span` frequencies (a multiple of 64), and
-// one wave takes one span of one mode of one binary, lane l the span's
-// frequencies l, l + 64, ...: the mode's forward values recomputed, each
+// is dealt to `chunks` waves per mode in blocks of 64 frequencies, block j
+// to wave j mod chunks (so that the waves of one binary and mode share its
+// regions evenly, whatever part of the grid each region holds), lane l
+// taking frequency l of each block: the mode's forward values recomputed, each
 // bin's dL/d(record slot) summed per lane in registers (the (2,2): the
 // IMRPhenomXAS adjoint's 64 slots; a higher mode: its 86 slots with a
 // derivative), one reduce-scatter over the wave, and the slots' totals
@@ -1551,28 +1575,31 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
 constexpr int kXhmGradSlots = 8;
 constexpr int kXhmGradRows = 9;
 // at least this many higher-mode waves per launch (the chip holds ~3,000
-// at the kernels' occupancy)
+// at the kernels' occupancy), and at most this many blocks of 64
+// frequencies per wave: a wave's bins may all fall in one region, and the
+// longest waves, started last, end the launch
 constexpr int64_t kXhmGradWaves = 12288;
+constexpr int64_t kXhmGradBlocksPerWave = 64;
 // the grid gradient's first pass: each thread sums this many binaries
 constexpr int64_t kXhmGridGroup = 64;
 
 struct XhmGradPlan {
-    int64_t chunks, span, loss_chunks, grid_groups;
+    int64_t chunks, loss_chunks, grid_groups;
     int64_t table, gh22, partial, cot_plus, cot_cross, loss_partial, gfm, grid_partial, bytes;
 };
 
 inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
-// the spans and the workspace's layout (each part 256-aligned) of B
+// the chunks and the workspace's layout (each part 256-aligned) of B
 // binaries on N frequencies in T (4- or 8-byte reals); bytes < 0 where the
 // sizes overflow
 inline XhmGradPlan xhm_grad_plan(int64_t nb, int64_t n, bool loss, bool grid, int64_t real) {
     XhmGradPlan g{};
-    const int64_t max_chunks = (n + 63) / 64;
+    const int64_t blocks = (n + 63) / 64;
     int64_t chunks = (kXhmGradWaves + 4 * nb - 1) / (4 * nb);
-    chunks = chunks < 1 ? 1 : chunks > max_chunks ? max_chunks : chunks;
-    g.span = ((n + chunks - 1) / chunks + 63) / 64 * 64;
-    g.chunks = (n + g.span - 1) / g.span;
+    const int64_t least = (blocks + kXhmGradBlocksPerWave - 1) / kXhmGradBlocksPerWave;
+    chunks = chunks < least ? least : chunks;
+    g.chunks = chunks > blocks ? blocks : chunks;
     g.loss_chunks = (n + kXhmBlock - 1) / kXhmBlock;
     g.grid_groups = (nb + kXhmGridGroup - 1) / kXhmGridGroup;
     if (nb > (INT64_MAX / 128) / n) {
@@ -1701,13 +1728,13 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_residual_kernel(
     }
 }
 
-// one wave per (binary, span, higher mode k): the mode's adjoint (and,
+// one wave per (binary, chunk, higher mode k): the mode's adjoint (and,
 // with gfm, its bins' dL/df)
 template <class T, bool Grid>
 __device__ __forceinline__ void xhm_hm_adjoint_wave(
     const T* __restrict__ rec22, const T* __restrict__ head, const T* __restrict__ dhead, const T* __restrict__ modes,
     const T* __restrict__ dmodes, int64_t nb, const PowerRow<T>* __restrict__ table, int64_t n, int64_t chunks,
-    int64_t span, const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
+    const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
     vec2_t<T>* __restrict__ gh22, double* __restrict__ partial, T* __restrict__ gfm) {
     const int64_t x = blockIdx.x;
     const int k = (int)(x % kHmModes);
@@ -1732,13 +1759,11 @@ __device__ __forceinline__ void xhm_hm_adjoint_wave(
     const T amp0 = r[kHmAmp0];
     const T inv_amp0 = d_rcp(fabs(amp0));
     const T sign_inv = amp0 < T(0) ? -inv_amp0 : inv_amp0;
-    const int64_t i0 = c * span;
-    const int64_t i1 = i0 + span < n ? i0 + span : n;
     T acc[kHmAcc];
 #pragma unroll
     for (int j = 0; j < kHmAcc; ++j) acc[j] = T(0);
     T v4 = T(0), v5 = T(0), v6 = T(0);
-    for (int64_t i = i0 + lane; i < i1; i += 64) {
+    for (int64_t i = c * 64 + lane; i < n; i += chunks * 64) {
         T gw_re = T(0), gw_im = T(0), gf = T(0);
         if (on) {
             const PowerRow<T> row = table[i];
@@ -1801,24 +1826,24 @@ template <bool Grid>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void xhm_hm_adjoint_kernel(
     const float* __restrict__ rec22, const float* __restrict__ head, const float* __restrict__ dhead,
     const float* __restrict__ modes, const float* __restrict__ dmodes, int64_t nb,
-    const PowerRow<float>* __restrict__ table, int64_t n, int64_t chunks, int64_t span,
+    const PowerRow<float>* __restrict__ table, int64_t n, int64_t chunks,
     const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross, float2* __restrict__ gh22,
     double* __restrict__ partial, float* __restrict__ gfm) {
-    xhm_hm_adjoint_wave<float, Grid>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, span, cot_plus,
+    xhm_hm_adjoint_wave<float, Grid>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, cot_plus,
                                      cot_cross, gh22, partial, gfm);
 }
 template <bool Grid>
 __global__ __launch_bounds__(64) void xhm_hm_adjoint_f64_kernel(
     const double* __restrict__ rec22, const double* __restrict__ head, const double* __restrict__ dhead,
     const double* __restrict__ modes, const double* __restrict__ dmodes, int64_t nb,
-    const PowerRow<double>* __restrict__ table, int64_t n, int64_t chunks, int64_t span,
+    const PowerRow<double>* __restrict__ table, int64_t n, int64_t chunks,
     const double2* __restrict__ cot_plus, const double2* __restrict__ cot_cross, double2* __restrict__ gh22,
     double* __restrict__ partial, double* __restrict__ gfm) {
-    xhm_hm_adjoint_wave<double, Grid>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, span, cot_plus,
+    xhm_hm_adjoint_wave<double, Grid>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, cot_plus,
                                       cot_cross, gh22, partial, gfm);
 }
 
-// one wave per (binary, span, higher mode) on the mode's ringdown bins
+// one wave per (binary, chunk, higher mode) on the mode's ringdown bins
 // (hm_ringdown_bin): the plain modes (2,1), (3,3), (4,4) with 16 compact
 // accumulators (Mixing false; blocks (b, c, 3 modes)), or the mixing (3,2)
 // with 26 (Mixing true; blocks (b, c)), its (2,2) recomputed and the
@@ -1827,7 +1852,7 @@ template <class T, bool Grid, bool Mixing>
 __device__ __forceinline__ void xhm_hm_rd_adjoint_wave(
     const T* __restrict__ rec22, const T* __restrict__ head, const T* __restrict__ dhead, const T* __restrict__ modes,
     const T* __restrict__ dmodes, int64_t nb, const PowerRow<T>* __restrict__ table, int64_t n, int64_t chunks,
-    int64_t span, const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
+    const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
     vec2_t<T>* __restrict__ gh22, double* __restrict__ partial, T* __restrict__ gfm) {
     constexpr int kModes = Mixing ? 1 : 3;
     constexpr int kAcc = Mixing ? (int)kMixAcc : (int)kRdAcc;
@@ -1855,13 +1880,11 @@ __device__ __forceinline__ void xhm_hm_rd_adjoint_wave(
     const T amp0 = r[kHmAmp0];
     const T inv_amp0 = d_rcp(fabs(amp0));
     const T sign_inv = amp0 < T(0) ? -inv_amp0 : inv_amp0;
-    const int64_t i0 = c * span;
-    const int64_t i1 = i0 + span < n ? i0 + span : n;
     T acc[kAcc];
 #pragma unroll
     for (int j = 0; j < kAcc; ++j) acc[j] = T(0);
     T v4 = T(0), v5 = T(0), v6 = T(0);
-    for (int64_t i = i0 + lane; i < i1; i += 64) {
+    for (int64_t i = c * 64 + lane; i < n; i += chunks * 64) {
         const PowerRow<T> row = table[i];
         const BinPowers<T> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
         if (!hm_ringdown_bin(r, r[kHmMsec] * p.f)) continue;  // the general wave's bin
@@ -1906,31 +1929,31 @@ template <bool Grid, bool Mixing>
 __global__ __launch_bounds__(64) void xhm_hm_rd_adjoint_kernel(
     const float* __restrict__ rec22, const float* __restrict__ head, const float* __restrict__ dhead,
     const float* __restrict__ modes, const float* __restrict__ dmodes, int64_t nb,
-    const PowerRow<float>* __restrict__ table, int64_t n, int64_t chunks, int64_t span,
+    const PowerRow<float>* __restrict__ table, int64_t n, int64_t chunks,
     const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross, float2* __restrict__ gh22,
     double* __restrict__ partial, float* __restrict__ gfm) {
-    xhm_hm_rd_adjoint_wave<float, Grid, Mixing>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, span,
+    xhm_hm_rd_adjoint_wave<float, Grid, Mixing>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks,
                                                 cot_plus, cot_cross, gh22, partial, gfm);
 }
 template <bool Grid, bool Mixing>
 __global__ __launch_bounds__(64) void xhm_hm_rd_adjoint_f64_kernel(
     const double* __restrict__ rec22, const double* __restrict__ head, const double* __restrict__ dhead,
     const double* __restrict__ modes, const double* __restrict__ dmodes, int64_t nb,
-    const PowerRow<double>* __restrict__ table, int64_t n, int64_t chunks, int64_t span,
+    const PowerRow<double>* __restrict__ table, int64_t n, int64_t chunks,
     const double2* __restrict__ cot_plus, const double2* __restrict__ cot_cross, double2* __restrict__ gh22,
     double* __restrict__ partial, double* __restrict__ gfm) {
-    xhm_hm_rd_adjoint_wave<double, Grid, Mixing>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, span,
+    xhm_hm_rd_adjoint_wave<double, Grid, Mixing>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks,
                                                  cot_plus, cot_cross, gh22, partial, gfm);
 }
 
-// one wave per (binary, span): the (2,2)'s adjoint (the IMRPhenomXAS
+// one wave per (binary, chunk): the (2,2)'s adjoint (the IMRPhenomXAS
 // record's, with the mode's own cotangent plus the one the (3,2) passes);
 // with Grid, each bin's dL/df of the five modes into gfm[0][b][i]
 template <class T, bool Grid>
 __global__ __launch_bounds__(64) void xhm_22_adjoint_kernel(
     const T* __restrict__ rec22, const double* __restrict__ jac22, const T* __restrict__ head,
     const T* __restrict__ dhead, int64_t nb, const PowerRow<T>* __restrict__ table, int64_t n, int64_t chunks,
-    int64_t span, const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
+    const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
     const vec2_t<T>* __restrict__ gh22, double* __restrict__ partial, T* __restrict__ gfm) {
     const int64_t b = blockIdx.x / chunks;
     const int64_t c = blockIdx.x % chunks;
@@ -1944,10 +1967,8 @@ __global__ __launch_bounds__(64) void xhm_22_adjoint_kernel(
     for (int k = 0; k < kSlots; ++k) acc[k] = T(0);
     const T* fac = head + b * kHeadSlots + kHeadFactors;
     const T* dfac = dhead + b * kHeadSlots + kHeadFactors;
-    const int64_t i0 = c * span;
-    const int64_t i1 = i0 + span < n ? i0 + span : n;
     T v4 = T(0), v5 = T(0), v6 = T(0);
-    for (int64_t i = i0 + lane; i < i1; i += 64) {
+    for (int64_t i = c * 64 + lane; i < n; i += chunks * 64) {
         const PowerRow<T> row = table[i];
         const BinPowers<T> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
         const vec2_t<T> cp = cot_plus[b * n + i], cc = cot_cross[b * n + i];
@@ -2464,23 +2485,23 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
         if (dldf)
             xhm_hm_adjoint_kernel<true><<<hm_blocks, 64, 0, s>>>(
                 (const float*)records22, (const float*)head, (const float*)dhead, (const float*)modes,
-                (const float*)dmodes, nb, (const PowerRow<float>*)table, n, g.chunks, g.span, (const float2*)cot_plus,
+                (const float*)dmodes, nb, (const PowerRow<float>*)table, n, g.chunks, (const float2*)cot_plus,
                 (const float2*)cot_cross, (float2*)gh22, partial, (float*)gfm);
         else
             xhm_hm_adjoint_kernel<false><<<hm_blocks, 64, 0, s>>>(
                 (const float*)records22, (const float*)head, (const float*)dhead, (const float*)modes,
-                (const float*)dmodes, nb, (const PowerRow<float>*)table, n, g.chunks, g.span, (const float2*)cot_plus,
+                (const float*)dmodes, nb, (const PowerRow<float>*)table, n, g.chunks, (const float2*)cot_plus,
                 (const float2*)cot_cross, (float2*)gh22, partial, nullptr);
     } else {
         if (dldf)
             xhm_hm_adjoint_f64_kernel<true><<<hm_blocks, 64, 0, s>>>(
                 (const double*)records22, (const double*)head, (const double*)dhead, (const double*)modes,
-                (const double*)dmodes, nb, (const PowerRow<double>*)table, n, g.chunks, g.span,
+                (const double*)dmodes, nb, (const PowerRow<double>*)table, n, g.chunks,
                 (const double2*)cot_plus, (const double2*)cot_cross, (double2*)gh22, partial, (double*)gfm);
         else
             xhm_hm_adjoint_f64_kernel<false><<<hm_blocks, 64, 0, s>>>(
                 (const double*)records22, (const double*)head, (const double*)dhead, (const double*)modes,
-                (const double*)dmodes, nb, (const PowerRow<double>*)table, n, g.chunks, g.span,
+                (const double*)dmodes, nb, (const PowerRow<double>*)table, n, g.chunks,
                 (const double2*)cot_plus, (const double2*)cot_cross, (double2*)gh22, partial, nullptr);
     }
     // the ringdown waves: the plain modes, then the mixing (3,2) (its
@@ -2493,14 +2514,14 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
         const PowerRow<float>* tb = (const PowerRow<float>*)table;
         if (dldf) {
             xhm_hm_rd_adjoint_kernel<true, false><<<rd_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n, g.chunks,
-                g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (float*)gfm);
+                (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (float*)gfm);
             xhm_hm_rd_adjoint_kernel<true, true><<<mix_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n, g.chunks,
-                g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (float*)gfm);
+                (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (float*)gfm);
         } else {
             xhm_hm_rd_adjoint_kernel<false, false><<<rd_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
-                g.chunks, g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
+                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
             xhm_hm_rd_adjoint_kernel<false, true><<<mix_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
-                g.chunks, g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
+                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
         }
     } else {
         using V2 = double2;
@@ -2509,23 +2530,23 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
         const PowerRow<double>* tb = (const PowerRow<double>*)table;
         if (dldf) {
             xhm_hm_rd_adjoint_f64_kernel<true, false><<<rd_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
-                g.chunks, g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (double*)gfm);
+                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (double*)gfm);
             xhm_hm_rd_adjoint_f64_kernel<true, true><<<mix_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
-                g.chunks, g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (double*)gfm);
+                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (double*)gfm);
         } else {
             xhm_hm_rd_adjoint_f64_kernel<false, false><<<rd_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
-                g.chunks, g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
+                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
             xhm_hm_rd_adjoint_f64_kernel<false, true><<<mix_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
-                g.chunks, g.span, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
+                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
         }
     }
     if (dldf)
         xhm_22_adjoint_kernel<T, true><<<blocks22, 64, 0, s>>>(records22, jacobian22, head, dhead, nb, table, n,
-                                                               g.chunks, g.span, cot_plus, cot_cross, gh22, partial,
+                                                               g.chunks, cot_plus, cot_cross, gh22, partial,
                                                                gfm);
     else
         xhm_22_adjoint_kernel<T, false><<<blocks22, 64, 0, s>>>(records22, jacobian22, head, dhead, nb, table, n,
-                                                                g.chunks, g.span, cot_plus, cot_cross, gh22, partial,
+                                                                g.chunks, cot_plus, cot_cross, gh22, partial,
                                                                 nullptr);
     xhm_adjoint_finish_kernel<<<(int)((nb + 63) / 64), 64, 0, s>>>(
         params, jacobian22, nb, partial, g.chunks, weights ? (const double*)(ws + g.loss_partial) : nullptr,

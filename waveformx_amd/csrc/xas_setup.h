@@ -715,8 +715,11 @@ WFX_HD inline R phase_rest(const C* c, const FreqPowers<R>& fp, bool& past) {
         total = total + c[kInsP2] * (f * f);
     }
     if (past) {
-        const int a = ringdown ? kAtanA2 : kAtanA1;
-        total = total + c[a] * m_atan(c[a + 1] * (f - c[kShift]));
+        // (the region's two slots selected by value: an index chosen at run
+        // time would keep a record in registers in memory instead)
+        const R a = ringdown ? R(c[kAtanA2]) : R(c[kAtanA1]);
+        const R w = ringdown ? R(c[kAtanW2]) : R(c[kAtanW1]);
+        total = total + a * m_atan(w * (f - c[kShift]));
     }
     return total;
 }
@@ -743,17 +746,36 @@ struct XasCoeffs22 {
     PhaseCoeffs<R> ph;
 };
 
+// the amplitude's slots of the record (the rest are the phase's, the
+// cutoff, the polarization factors and the leading term's)
+constexpr bool is_amp_slot(int k) { return (k >= kAmpIns && k <= kAmpRdG) || k == kFAmpIn || k == kFAmpRd; }
+
+// a record sink that fills an array
+template <class R>
+struct RecordArray {
+    R* rec;
+    WFX_HD void operator()(int k, const R& x) const { rec[k] = x; }
+};
+
 // The record of one binary (xas/core.py:257-379): params in the packed
 // order of XasParam, f_min the first frequency of its grid; returns LAL's
-// error flags. T0 and the anchor A land in rec[kT0], rec[kAnchor] in R
-// (the FP32-split path runs this in double and splits them afterwards); with
-// keep, the coefficient structs are also kept there.
-template <class R>
-WFX_HD inline unsigned xas_record(const R* params, const R& f_min, double amp_scale, R* rec,
-                                   XasCoeffs22<R>* keep = nullptr) {
+// error flags. Each of the kSlots slots goes to out(k, value) once, when it
+// is final: the amplitude's first, so that a sink that stores them (the
+// set-up in dual numbers) holds none of them through the phase's solves.
+// T0 and the anchor A are slots kT0, kAnchor in R (the FP32-split path runs
+// this in double and splits them afterwards); with keep, the coefficient
+// structs are also stored there, each as soon as it is final.
+template <class R, class Out>
+WFX_HD __attribute__((always_inline)) inline unsigned xas_record_to(const R* params, const R& f_min, double amp_scale, Out& out,
+                                      XasCoeffs22<R>* keep = nullptr) {
     WaveformVars<R> wf = waveform_vars(params, f_min, amp_scale);
+    if (keep) keep->wf = wf;
     R s = wf.m_sec;
     R one = R(1.0);
+    // the phase's slots (and the rest), summed here; the amplitude's
+    // entries stay unused
+    R rec[kSlots];
+#pragma unroll
     for (int k = 0; k < kSlots; ++k) rec[k] = R(0.0);
 
     // the amplitude first, so that its coefficients are dead before the
@@ -767,27 +789,25 @@ WFX_HD inline unsigned xas_record(const R* params, const R& f_min, double amp_sc
                              p.two * p.one_third, p.two * p.two_thirds, p.three};
         const R ins[10] = {one, R(0.0), amp.pn_two_thirds, amp.pn_three_thirds, amp.pn_four_thirds,
                            amp.pn_five_thirds, amp.pn_six_thirds, amp.rho1, amp.rho2, amp.rho3};
-        for (int k = 0; k < 10; ++k) rec[kAmpIns + k] = scale * ins[k] * third[k];
-        rec[kAmpIntN] = prefactor;
-        rec[kAmpIntD + 0] = amp.delta0 * one;
-        rec[kAmpIntD + 1] = amp.delta1 * s;
-        rec[kAmpIntD + 2] = amp.delta2 * p.two;
-        rec[kAmpIntD + 3] = amp.delta3 * p.three;
-        rec[kAmpIntD + 4] = amp.delta4 * p.four;
-        rec[kAmpRdN] = scale * amp.gamma_d13;
-        rec[kAmpRdGr] = amp.gamma_r;
-        rec[kAmpRdS] = s;
-        rec[kAmpRdG] = amp.gamma_d2;
-        rec[kFAmpIn] = amp.f_amp_match_in / s;
-        rec[kFAmpRd] = amp.f_amp_rd_min / s;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) out(kAmpIns + k, scale * ins[k] * third[k]);
+        out(kAmpIntN, prefactor);
+        out(kAmpIntD + 0, amp.delta0 * one);
+        out(kAmpIntD + 1, amp.delta1 * s);
+        out(kAmpIntD + 2, amp.delta2 * p.two);
+        out(kAmpIntD + 3, amp.delta3 * p.three);
+        out(kAmpIntD + 4, amp.delta4 * p.four);
+        out(kAmpRdN, scale * amp.gamma_d13);
+        out(kAmpRdGr, amp.gamma_r);
+        out(kAmpRdS, s);
+        out(kAmpRdG, amp.gamma_d2);
+        out(kFAmpIn, amp.f_amp_match_in / s);
+        out(kFAmpRd, amp.f_amp_rd_min / s);
         if (keep) keep->amp = amp;
     }
 
     PhaseCoeffs<R> ph = phase_coeffs(wf);
-    if (keep) {
-        keep->wf = wf;
-        keep->ph = ph;
-    }
+    if (keep) keep->ph = ph;
     R linb = time_shift(ph, wf);
     R inveta = R(1.0) / wf.eta;
     Powers<R> p = useful_powers(s);
@@ -852,7 +872,18 @@ WFX_HD inline unsigned xas_record(const R* params, const R& f_min, double amp_sc
     R t0 = R(K0) * lead_cbrt(params[kMass1] + params[kMass2]) / (params[kMass1] * params[kMass2]);
     rec[kT0] = t0;
     rec[kAnchor] = past ? R(0.0) : t0 * lead_exp(R(-5.0 / 3.0) * lead_log(wf.f_ref));
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k)
+        if (!is_amp_slot(k)) out(k, rec[k]);
     return wf.flags;
+}
+
+// xas_record_to into rec[0 .. kSlots)
+template <class R>
+WFX_HD inline unsigned xas_record(const R* params, const R& f_min, double amp_scale, R* rec,
+                                   XasCoeffs22<R>* keep = nullptr) {
+    RecordArray<R> out{rec};
+    return xas_record_to(params, f_min, amp_scale, out, keep);
 }
 
 }  // namespace wfx

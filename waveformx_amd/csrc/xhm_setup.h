@@ -28,7 +28,7 @@ namespace wfx {
 template <class R>
 struct Jet2 {
     R v, d, dd;
-    WFX_HD Jet2() : v(0.0), d(0.0), dd(0.0) {}
+    Jet2() = default;  // trivial, as Dual's (Jet2() is zero)
     WFX_HD Jet2(const R& x) : v(x), d(0.0), dd(0.0) {}
     WFX_HD Jet2(const R& x, const R& dx, const R& ddx) : v(x), d(dx), dd(ddx) {}
 };
