@@ -16,6 +16,9 @@
 #include "wfx_libm.h"
 
 #define WFX_HD __host__ __device__
+// inlined whatever its size: the set-ups' stages, so that a mode index known
+// at the call site reaches every branch and array index inside as a constant
+#define WFX_HDI __host__ __device__ __attribute__((always_inline))
 #define WFX_CONST __device__ __constant__ static const
 
 namespace wfx {
@@ -45,6 +48,18 @@ struct Dual {
     Dual() = default;
     WFX_HD Dual(double x) : v(x) {
         for (int i = 0; i < N; ++i) d[i] = 0.0;
+    }
+    // copies member by member, not as a block: a copy of one of two duals
+    // chosen at run time (c ? a : b) then reads through the chosen address,
+    // which the compiler turns into a select of values (a block copy from
+    // it would keep both duals in scratch memory)
+    WFX_HD Dual(const Dual& o) : v(o.v) {
+        for (int i = 0; i < N; ++i) d[i] = o.d[i];
+    }
+    WFX_HD Dual& operator=(const Dual& o) {
+        v = o.v;
+        for (int i = 0; i < N; ++i) d[i] = o.d[i];
+        return *this;
     }
 };
 

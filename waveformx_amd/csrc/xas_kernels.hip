@@ -1250,6 +1250,19 @@ int grid_adjoint(const char* name, const void* records, int64_t n_binaries, cons
 
 // ------------------------------------------------------------ IMRPhenomXHM
 
+// f(std::integral_constant<int, k>) for the higher mode k (0..3): the mode's
+// set-up inlined into f then sees k, and every branch and array index that
+// follows from it, as a constant
+template <class F>
+__device__ __forceinline__ void with_mode(int k, F f) {
+    switch (k) {
+        case 0: f(std::integral_constant<int, 0>{}); break;
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        default: f(std::integral_constant<int, 3>{}); break;
+    }
+}
+
 // one FP64 thread per (binary, group), five groups: the four higher modes'
 // records (group k < 4) and the (2,2)'s IMRPhenomXAS record with the flags
 // and the binary's head (the factors of the five modes in the
@@ -1359,22 +1372,28 @@ __global__ __launch_bounds__(64) void xhm_setup_jvp_kernel(const double* __restr
     // set-up (the same arithmetic), read where the set-up uses them
     const XasCoeffs22<D>& c22 = kept[4 * b + dir];
     const WaveformVars<D>& wf = c22.wf;
-    HmCoeffs<D> c{};
-    hm_setup(wf, c22.amp, c22.ph, k, c);
     const int64_t at = (b * kHmModes + k) * kHmSlots;
-    if (split) {
-        float values[kHmSlots], tangents[kHmSlots];
-        write_mode_record_split(c, wf, values, tangents);
-        if (dir == 0)
-            for (int s = 0; s < kHmSlots; ++s) ((float*)modes)[at + s] = values[s];
-        for (int s = 0; s < kHmSlots; ++s) ((float*)dmodes)[4 * (at + s) + dir] = tangents[s];
-    } else {
-        double values[kHmSlots], tangents[kHmSlots];
-        write_mode_record_split(c, wf, values, tangents);
-        if (dir == 0)
-            for (int s = 0; s < kHmSlots; ++s) ((double*)modes)[at + s] = values[s];
-        for (int s = 0; s < kHmSlots; ++s) ((double*)dmodes)[4 * (at + s) + dir] = tangents[s];
-    }
+    with_mode(k, [&](auto mode) __attribute__((always_inline)) {
+        HmCoeffs<D> c{};
+        hm_setup(wf, c22.amp, c22.ph, decltype(mode)::value, c);
+        if (split) {
+            float values[kHmSlots], tangents[kHmSlots];
+            write_mode_record_split(c, wf, values, tangents);
+#pragma unroll
+            for (int s = 0; s < kHmSlots; ++s) {
+                if (dir == 0) ((float*)modes)[at + s] = values[s];
+                ((float*)dmodes)[4 * (at + s) + dir] = tangents[s];
+            }
+        } else {
+            double values[kHmSlots], tangents[kHmSlots];
+            write_mode_record_split(c, wf, values, tangents);
+#pragma unroll
+            for (int s = 0; s < kHmSlots; ++s) {
+                if (dir == 0) ((double*)modes)[at + s] = values[s];
+                ((double*)dmodes)[4 * (at + s) + dir] = tangents[s];
+            }
+        }
+    });
 }
 
 // one thread per (binary, frequency), 256-thread blocks, a block on 256

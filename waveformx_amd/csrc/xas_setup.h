@@ -617,7 +617,7 @@ WFX_HD inline PhaseCoeffs<R> phase_coeffs(const WaveformVars<R>& wf) {
 
 // phase22.py:319-326 (the derivative by region: phase22.py:310-316)
 template <class R>
-WFX_HD inline R time_shift(const PhaseCoeffs<R>& ph, const WaveformVars<R>& wf) {
+WFX_HDI inline R time_shift(const PhaseCoeffs<R>& ph, const WaveformVars<R>& wf) {
     R linb = ut_linb(wf.eta, wf.s_tot_r, wf.dchi, wf.delta);
     R x = wf.f_ring - wf.f_damp;
     Powers<R> p = useful_powers(x);

@@ -305,7 +305,7 @@ WFX_HD inline R hm_horner(const double (&c)[N], const R& u) {
 // variables.py:118-169 with reference.py:102-122 (QNM in 1 - a_f, the
 // 122022 flags)
 template <class R>
-WFX_HD inline void mode_vars(const WaveformVars<R>& wf, int k, HmCoeffs<R>& c) {
+WFX_HDI inline void mode_vars(const WaveformVars<R>& wf, int k, HmCoeffs<R>& c) {
     c.ell = kHmEll[k];
     c.emm = kHmEmm[k];
     c.tag = 10 * c.ell + c.emm;
@@ -339,7 +339,7 @@ WFX_HD inline void mode_vars(const WaveformVars<R>& wf, int k, HmCoeffs<R>& c) {
 // inspiral.py:49-215 IMRPhenomXHM_GetPNAmplitudeCoefficients (LAL's
 // complex PN coefficients as (re, im) pairs, the reference's order)
 template <class R>
-WFX_HD inline void pn_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
+WFX_HDI inline void pn_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
     const R chi_a = c.chi_a, chi_s = c.chi_s, eta = wf.eta, delta = wf.delta;
     const double pi = lalpi_itself;
     const double sqrt_5_7 = sqrt(5.0 / 7.0);
@@ -354,6 +354,7 @@ WFX_HD inline void pn_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
     const R zero = 0.0 * eta;
     R* re = c.pn_re;
     R* im = c.pn_im;
+#pragma unroll
     for (int k = 0; k < 7; ++k) re[k] = im[k] = zero;
     if (c.tag == 21) {
         const Powers<double> p2 = useful_powers(2.0);
@@ -493,6 +494,7 @@ WFX_HD inline R pn_amp(const Powers<R>& p, const C* pn_re, const C* pn_im, const
                        const typename NoDeduce<C>::type& amp_norm) {
     const R terms[6] = {p.one_third, p.two_thirds, p.itself, p.four_thirds, p.five_thirds, p.two};
     R re = R(pn_re[0]), im = R(pn_im[0]);
+#pragma unroll
     for (int k = 0; k < 6; ++k) {
         re = re + terms[k] * pn_re[k + 1];
         im = im + terms[k] * pn_im[k + 1];
@@ -500,7 +502,7 @@ WFX_HD inline R pn_amp(const Powers<R>& p, const C* pn_re, const C* pn_im, const
     return m_hypot(re, im) * (pn_global * p.m_seven_sixths * amp_norm);
 }
 template <class R, class C>
-WFX_HD inline R inspiral_amp(const Powers<R>& p, const C* pn_re, const C* pn_im,
+WFX_HDI inline R inspiral_amp(const Powers<R>& p, const C* pn_re, const C* pn_im,
                              const typename NoDeduce<C>::type& pn_global, const typename NoDeduce<C>::type& amp_norm,
                              const typename NoDeduce<C>::type& pn_dominant, const typename NoDeduce<C>::type& fcut73,
                              const typename NoDeduce<C>::type& fcut83, const typename NoDeduce<C>::type& fcut3,
@@ -594,7 +596,7 @@ WFX_HD inline void rotate(const R& mf, const Powers<R>& p, const HmCoeffs<C>& c,
 // reference.py:381-388 SpheroidalToSpherical with the (2,2) ringdown ansatz
 // (mf a Taylor scalar over the set-up's C, the (2,2)'s coefficients of C)
 template <class R, class C>
-WFX_HD inline void spheroidal_to_spherical(const R& mf, const HmCoeffs<C>& c, const AmpCoeffs<C>& amp22,
+WFX_HDI inline void spheroidal_to_spherical(const R& mf, const HmCoeffs<C>& c, const AmpCoeffs<C>& amp22,
                                            const PhaseCoeffs<C>& ph22, R& re, R& im) {
     const Powers<R> p = useful_powers(mf);
     const R a22 = rd_amp_22_ansatz_exact(mf, R(c.f_ring22), R(c.f_damp22), amp22) * c.amp_norm * p.m_seven_sixths;
@@ -605,11 +607,12 @@ WFX_HD inline void spheroidal_to_spherical(const R& mf, const HmCoeffs<C>& c, co
 // phase.py:155-193 GetSpheroidalCoefficients (and the (2,2) time shift and
 // phase offset of the (3,2))
 template <class R>
-WFX_HD inline void spheroidal_coeffs(const WaveformVars<R>& wf, const PhaseCoeffs<R>& ph22, HmCoeffs<R>& c) {
+WFX_HDI inline void spheroidal_coeffs(const WaveformVars<R>& wf, const PhaseCoeffs<R>& ph22, HmCoeffs<R>& c) {
     const R fringlm = c.f_ring, fdamplm = c.f_damp;
     const R freqs[4] = {wf.f_ring, m_fma(-1.5, fdamplm, fringlm), m_fma(-0.5, fdamplm, fringlm),
                         m_fma(0.5, fdamplm, fringlm)};
     R a[4][4], x[4] = {hpr_phase_32_p1(wf), hpr_phase_32_p2(wf), hpr_phase_32_p3(wf), hpr_phase_32_p4(wf)};
+#pragma unroll
     for (int i = 0; i < 4; ++i) {
         const R ff = freqs[i];
         const R ffm1 = 1.0 / ff, ffm2 = ffm1 * ffm1;
@@ -648,10 +651,11 @@ WFX_HD inline void spheroidal_coeffs(const WaveformVars<R>& wf, const PhaseCoeff
 
 // reference.py:179-197 the pseudo-PN coefficients (three collocation points)
 template <class R>
-WFX_HD inline void inspiral_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
+WFX_HDI inline void inspiral_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
     const R fcut = c.f_amp_match_in;
     const R freqs[3] = {0.5 * fcut, 0.75 * fcut, fcut};
     R values[3];
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
         R v;
         switch (c.tag * 10 + i) {
@@ -675,13 +679,16 @@ WFX_HD inline void inspiral_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c
     c.fcut83 = finsp.eight_thirds;
     c.fcut3 = finsp.three;
     R v[3];
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
         const Powers<R> f = useful_powers(freqs[i]);
         const R pn = pn_amp(f, c.pn_re, c.pn_im, c.pn_global, c.amp_norm);
         v[i] = (values[i] - pn) / c.pn_dominant / f.m_seven_sixths;
     }
+#pragma unroll
     for (int j = 0; j < 3; ++j) {
         R s = R(0.0);
+#pragma unroll
         for (int i = 0; i < 3; ++i) s = s + kHmInspiralInverse[j][i] * v[i];
         c.ins[j] = s;
     }
@@ -690,7 +697,7 @@ WFX_HD inline void inspiral_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c
 // reference.py:239-272 the ringdown amplitude coefficients (and the (3,2)'s
 // auxiliary cubic)
 template <class R>
-WFX_HD inline void rd_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
+WFX_HDI inline void rd_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
     const R fdamp = c.f_damp;
     R rd[3];
     if (c.tag == 32) {
@@ -726,6 +733,7 @@ WFX_HD inline void rd_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
     c.rd[2] = rd[2];
     c.rd[3] = value;
     c.rd[4] = -1.0 * hm_dlorentzian(falloff, c.f_ring, c.f_damp, rd) / value;
+#pragma unroll
     for (int j = 0; j < 4; ++j) c.rd_aux[j] = R(0.0);
     if (c.mixing) {
         R values[4] = {m_abs(har_amp_32_rdaux1(wf)), m_abs(har_amp_32_rdaux2(wf)),
@@ -733,6 +741,7 @@ WFX_HD inline void rd_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
                        hm_dlorentzian(c.f_rd_aux, c.f_ring, c.f_damp, c.rd)};
         const R freqs[3] = {c.f_amp_match_im, 0.5 * (c.f_amp_match_im + c.f_rd_aux), c.f_rd_aux};
         R a[4][4];
+#pragma unroll
         for (int i = 0; i < 3; ++i) {
             const R f = freqs[i];
             a[i][0] = 1.0, a[i][1] = f, a[i][2] = f * f, a[i][3] = f * f * f;
@@ -740,6 +749,7 @@ WFX_HD inline void rd_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
         const R f = c.f_rd_aux;
         a[3][0] = 0.0, a[3][1] = 1.0, a[3][2] = 2.0 * f, a[3][3] = 3.0 * f * f;
         pivoted_solve<R, 4>(a, values);
+#pragma unroll
         for (int j = 0; j < 4; ++j) c.rd_aux[j] = values[j];
     }
 }
@@ -747,7 +757,7 @@ WFX_HD inline void rd_amp_coeffs(const WaveformVars<R>& wf, HmCoeffs<R>& c) {
 // reference.py:275-334 the intermediate amplitude (a polynomial in u times
 // f^(-7/6) through the collocation points)
 template <class R>
-WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<R>& wf, const AmpCoeffs<R>& amp22,
+WFX_HDI inline void intermediate_amp_coeffs(const WaveformVars<R>& wf, const AmpCoeffs<R>& amp22,
                                            const PhaseCoeffs<R>& ph22, HmCoeffs<R>& c) {
     // IMRPhenomXHMIntermediateAmpVersion (reference.py:59-60): 0 skipped,
     // 1 value, 2 value and derivative
@@ -766,11 +776,13 @@ WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<R>& wf, const AmpC
     auto value_row = [&](const R& f) {
         const R m76 = m_pow(f, -seven_sixths);
         const R u = (f - center) / half;
+#pragma unroll
         for (int j = 0; j < n; ++j) a[row][j] = m76 * int_pow(u, j);
     };
     auto derivative_row = [&](const R& f) {
         const R m76 = m_pow(f, -seven_sixths);
         const R u = (f - center) / half;
+#pragma unroll
         for (int j = 0; j < n; ++j)
             a[row][j] =
                 m76 * (-seven_sixths / f * int_pow(u, j) + (j ? R((double)j * int_pow(u, j - 1) / half) : R(0.0)));
@@ -791,6 +803,7 @@ WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<R>& wf, const AmpC
                           c.f_amp_rd_falloff);
         rd_derivative = hm_dlorentzian(f_rd, c.f_ring, c.f_damp, c.rd);
     }
+#pragma unroll
     for (int i = 0; i < n_colloc; ++i) {
         const int digit = version[i];
         const R f = m_fma(deltaf, (double)i, f_in);
@@ -835,17 +848,22 @@ WFX_HD inline void intermediate_amp_coeffs(const WaveformVars<R>& wf, const AmpC
     }
     if (n == 5) {
         R a5[5][5], b5[5];
+#pragma unroll
         for (int i = 0; i < 5; ++i) {
             b5[i] = values[i];
+#pragma unroll
             for (int j = 0; j < 5; ++j) a5[i][j] = a[i][j];
         }
         pivoted_solve<R, 5>(a5, b5);
+#pragma unroll
         for (int j = 0; j < 5; ++j) c.inter[j] = b5[j];
     } else {
         pivoted_solve<R, 8>(a, values);
+#pragma unroll
         for (int j = 0; j < 8; ++j) c.inter[j] = values[j];
     }
     c.n_inter = n;
+#pragma unroll
     for (int j = n; j < 8; ++j) c.inter[j] = R(0.0);
 }
 
@@ -860,6 +878,7 @@ WFX_HD inline R inspiral_phase_int(const R& mf, const Powers<R>& p, const C* phi
     const R freqs[13] = {p.m_five_thirds, p.m_four_thirds, p.m_one, p.m_two_thirds, p.m_one_third, R(1.0),
                          p.one_third, p.two_thirds, mf, p.four_thirds, p.five_thirds, p.two, p.seven_thirds};
     R philm = R(0.0);
+#pragma unroll
     for (int i = 0; i < 13; ++i) philm = m_fma(m_fma(phi_l[i], p.log, phi[i]), freqs[i], philm);
     return philm;
 }
@@ -869,6 +888,7 @@ WFX_HD inline R inspiral_phase(const R& mf, const Powers<R>& p, const C* phi, co
                          p.m_one, p.m_two_thirds, p.m_one_third, R(1.0), p.one_third, p.two_thirds, mf,
                          p.four_thirds};
     R dphilm = R(0.0);
+#pragma unroll
     for (int i = 0; i < 13; ++i) {
         const R coefficient = m_fma(phi_l[i], p.log, phi[i]);
         dphilm = m_fma(m_fma(coefficient, kHmInspiralExponents[i], phi_l[i]), freqs[i], dphilm);
@@ -942,7 +962,7 @@ WFX_HD inline R phase_fit(const WaveformVars<R>& wf, int tag, int i) {
 // reference.py:397-563 IMRPhenomXHM_GetPhaseCoefficients (122022, exact
 // derivatives of the rotated (3,2) ringdown phase)
 template <class R>
-WFX_HD inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<R>& ph22,
+WFX_HDI inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<R>& ph22,
                                    const AmpCoeffs<R>& amp22,
                                    HmCoeffs<R>& c) {
     const int emm = c.emm, tag = c.tag;
@@ -971,6 +991,7 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<
     }
     const R f_phase_match_in = c.f_meco_lm;
     R values[6];
+#pragma unroll
     for (int i = 0; i < 6; ++i) values[i] = phase_fit(wf, tag, i) + c.delta_t;
     R lambda_fit;
     switch (tag) {
@@ -993,6 +1014,7 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<
     const double m_over_2 = emm * 0.5, two_over_m = 1.0 / m_over_2;
     const R fact = ph22.phi_norm / eta;
     const double log_m_over_2 = kHmLogMOver2[c.mode_int];
+#pragma unroll
     for (int i = 0; i < 13; ++i) {
         const R nl = non_log[i] * fact, lg = log_terms[i] * fact, ps = pseudo_pn[i] * fact;
         const double scale = kHmInspiralScale[c.mode_int][i];
@@ -1008,6 +1030,7 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<
         PhaseCoeffs<R> unconnected = ph22;
         unconnected.c1_int = unconnected.c2_int = unconnected.c1_mrd = unconnected.c2_mrd = R(0.0);
         R insp[3];
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
             const R ff = two_over_m * freqs[k];
             insp[k] = 1.0 / eta * dphase_22(ff, useful_powers(ff), unconnected);
@@ -1035,6 +1058,7 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<
         dphi0_rd = rd_phase(fcut_rd, p_rd, fring, fdamp, false, c.alpha0, c.alpha2, c.alpha_l, R(0.0), R(0.0), R(0.0),
                             R(0.0));
         R a[5][5], b[5];
+#pragma unroll
         for (int i = 0; i < 5; ++i) {
             const R ff = freqs[chosen[i]];
             const R ffm1 = 1.0 / ff, ffm2 = ffm1 * ffm1;
@@ -1046,6 +1070,7 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<
             b[i] = values[chosen[i]];
         }
         pivoted_solve<R, 5>(a, b);
+#pragma unroll
         for (int j = 0; j < 5; ++j) x[j] = b[j];
         x[5] = R(0.0);
     } else {
@@ -1064,6 +1089,7 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<
         const R g_c = 0.5 * (1.0 / fcut_insp + 1.0 / fcut_rd);
         const R g_h = 0.5 * (1.0 / fcut_insp - 1.0 / fcut_rd);
         R a[6][6], b[6];
+#pragma unroll
         for (int i = 0; i < 5; ++i) {
             const R ff = freqs[i];
             const R v = (1.0 / ff - g_c) / g_h;
@@ -1091,8 +1117,10 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<
         const double binom[5][5] = {{1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0}, {1, 3, 3, 1, 0},
                                     {1, 4, 6, 4, 1}};
         R pc[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
+#pragma unroll
         for (int k = 0; k < 5; ++k) {
             const R scale = quartic[k] / int_pow(g_h, k);
+#pragma unroll
             for (int j = 0; j <= k; ++j) pc[j] = pc[j] + scale * binom[k][j] * int_pow(-g_c, k - j);
         }
         x[0] = pc[0], x[1] = b[1], x[2] = pc[1], x[3] = pc[2], x[4] = pc[4], x[5] = pc[3];
@@ -1127,7 +1155,7 @@ WFX_HD inline void phase_coeffs_hm(const WaveformVars<R>& wf, const PhaseCoeffs<
 
 // reference.py:580-608 mode_setup: the coefficients of higher mode k
 template <class R>
-WFX_HD inline void hm_setup(const WaveformVars<R>& wf, const AmpCoeffs<R>& amp22, const PhaseCoeffs<R>& ph22,
+WFX_HDI inline void hm_setup(const WaveformVars<R>& wf, const AmpCoeffs<R>& amp22, const PhaseCoeffs<R>& ph22,
                             int k, HmCoeffs<R>& c) {
     mode_vars(wf, k, c);
     c.f_ring22 = wf.f_ring;
@@ -1172,7 +1200,8 @@ WFX_HD inline void hm_setup(const WaveformVars<R>& wf, const AmpCoeffs<R>& amp22
 enum HmMassSlot : int { kHmT0F64 = 112, kHmKInsF64 = 114, kHmLogM = 116, kHmCbrtM = 117, kHmInvM = 118 };
 
 template <class R>
-WFX_HD inline void write_mode_record(const HmCoeffs<R>& c, const WaveformVars<R>& wf, R* r) {
+WFX_HDI inline void write_mode_record(const HmCoeffs<R>& c, const WaveformVars<R>& wf, R* r) {
+#pragma unroll
     for (int s = 0; s < kHmSlots; ++s) r[s] = R(0.0);
     r[kHmTag] = R((double)c.tag);
     r[kHmOn] = R(c.amp_zero ? 0.0 : 1.0);
@@ -1191,6 +1220,7 @@ WFX_HD inline void write_mode_record(const HmCoeffs<R>& c, const WaveformVars<R>
     r[kHmFAmpIm] = c.f_amp_match_im;
     r[kHmFPhaseIn] = c.f_phase_match_in;
     r[kHmFPhaseIm] = c.f_phase_match_im;
+#pragma unroll
     for (int k = 0; k < 7; ++k) r[kHmPnRe + k] = c.pn_re[k], r[kHmPnIm + k] = c.pn_im[k];
     r[kHmPnGlobal] = c.pn_global;
     r[kHmAmpNorm] = c.amp_norm;
@@ -1198,14 +1228,19 @@ WFX_HD inline void write_mode_record(const HmCoeffs<R>& c, const WaveformVars<R>
     r[kHmFcut73] = c.fcut73;
     r[kHmFcut83] = c.fcut83;
     r[kHmFcut3] = c.fcut3;
+#pragma unroll
     for (int k = 0; k < 3; ++k) r[kHmIns + k] = c.ins[k];
+#pragma unroll
     for (int k = 0; k < 5; ++k) r[kHmRd + k] = c.rd[k];
+#pragma unroll
     for (int k = 0; k < 4; ++k) r[kHmRdAux + k] = c.rd_aux[k];
     r[kHmFRdAux] = c.f_rd_aux;
     r[kHmInterC] = c.inter_center;
     r[kHmInterH] = c.inter_half;
+#pragma unroll
     for (int k = 0; k < 8; ++k) r[kHmInter + k] = c.inter[k];
     r[kHmNInter] = R((double)c.n_inter);
+#pragma unroll
     for (int k = 0; k < 13; ++k) r[kHmPhi + k] = c.phi[k], r[kHmPhiL + k] = c.phi_l[k];
     r[kHmAlpha0] = c.alpha0;
     r[kHmAlpha2] = c.alpha2;
@@ -1264,7 +1299,7 @@ WFX_HD inline void store_f64(double* r, int slot, double v) {
 // O = float: the split evaluation's record; O = double: the same layout in
 // FP64 (the FP64 gradient's), the FP64 slots as one double and a zero
 template <class R, class O>
-WFX_HD inline void write_mode_record_split(const HmCoeffs<R>& c, const WaveformVars<R>& wf, O* r,
+WFX_HDI inline void write_mode_record_split(const HmCoeffs<R>& c, const WaveformVars<R>& wf, O* r,
                                            O* tangent = nullptr) {
     R d[kHmSlots];
     write_mode_record(c, wf, d);
@@ -1283,10 +1318,12 @@ WFX_HD inline void write_mode_record_split(const HmCoeffs<R>& c, const WaveformV
     const Powers<R> pm = useful_powers(wf.m_sec);
     const R t0 = c.phi[0] * pm.m_five_thirds;
     const R k_ins = c.c_insp + c.deltaphi;
+#pragma unroll
     for (int s = 0; s < kHmSlots; ++s) r[s] = (O)value_of(d[s]);
     store_f64(r, kHmT0F64, value_of(t0));
     store_f64(r, kHmKInsF64, value_of(k_ins));
     if (tangent) {
+#pragma unroll
         for (int s = 0; s < kHmSlots; ++s) tangent[s] = (O)tangent_of(d[s]);
         tangent[kHmT0F64] = (O)tangent_of(t0);
         tangent[kHmT0F64 + 1] = O(0);
