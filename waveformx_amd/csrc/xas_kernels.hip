@@ -20,7 +20,9 @@
 #include "xhm_point.h"
 
 #include <atomic>
+#include <climits>
 #include <cstdio>
+#include <limits>
 #include <mutex>
 #include <string>
 
@@ -1604,7 +1606,7 @@ constexpr int64_t kXhmGridGroup = 64;
 
 struct XhmGradPlan {
     int64_t chunks, loss_chunks, grid_groups;
-    int64_t table, gh22, partial, cot_plus, cot_cross, loss_partial, gfm, grid_partial, bytes;
+    int64_t table, cuts, gh22, partial, cot_plus, cot_cross, loss_partial, gfm, grid_partial, bytes;
 };
 
 inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
@@ -1629,6 +1631,8 @@ inline XhmGradPlan xhm_grad_plan(int64_t nb, int64_t n, bool loss, bool grid, in
     int64_t off = 0;
     g.table = off;
     off += align256(n * row);
+    g.cuts = off;
+    off += align256(nb * kHmModes * (int64_t)sizeof(int));
     g.gh22 = off;
     off += align256(nb * n * 2 * real);
     g.partial = off;
@@ -1747,6 +1751,51 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_residual_kernel(
     }
 }
 
+// Each higher mode's first ringdown bin (hm_ringdown_bin), cuts[b * 4 + k],
+// where the grid ascends: the mode's M f then grows with the bin, and so
+// does the predicate (each of its comparisons turns true once, a NaN bound
+// leaves it constant), so that its general bins are those before the cut
+// and its ringdown bins those from it. The general waves stop at the cut
+// and the ringdown waves start at its block, instead of both visiting every
+// bin; -1 (every wave visits every bin, testing each) where the grid does
+// not ascend or M is not a positive finite number. One thread per (binary,
+// mode); every block first checks the grid's order (a read of the table's
+// frequencies, from L2).
+template <class T>
+__global__ __launch_bounds__(kBlock) void xhm_cuts_kernel(const T* __restrict__ modes, int64_t nb,
+                                                          const PowerRow<T>* __restrict__ table, int64_t n,
+                                                          int* __restrict__ cuts) {
+    int descends = n > INT_MAX;
+    for (int64_t i = threadIdx.x; i + 1 < n && !descends; i += kBlock)
+        descends = !(table[i].f <= table[i + 1].f);
+    descends = __syncthreads_or(descends);
+    const int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (x >= nb * kHmModes) return;
+    const T* r = modes + x * kHmSlots;
+    const T msec = r[kHmMsec];
+    int cut = -1;
+    if (!descends && msec > T(0) && msec <= std::numeric_limits<T>::max()) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            if (hm_ringdown_bin(r, msec * table[mid].f))
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        cut = (int)lo;
+    }
+    cuts[x] = cut;
+}
+
+// the first bin of wave (c, lane) at or past the block of bin `from`: the
+// wave takes blocks c, c + chunks, ... of 64 bins
+__device__ inline int64_t first_bin_from(int64_t from, int64_t c, int64_t chunks, int lane) {
+    const int64_t block = from / 64;
+    const int64_t skip = block > c ? (block - c + chunks - 1) / chunks : 0;
+    return (c + skip * chunks) * 64 + lane;
+}
+
 // one wave per (binary, chunk, higher mode k): the mode's adjoint (and,
 // with gfm, its bins' dL/df)
 template <class T, bool Grid>
@@ -1754,7 +1803,7 @@ __device__ __forceinline__ void xhm_hm_adjoint_wave(
     const T* __restrict__ rec22, const T* __restrict__ head, const T* __restrict__ dhead, const T* __restrict__ modes,
     const T* __restrict__ dmodes, int64_t nb, const PowerRow<T>* __restrict__ table, int64_t n, int64_t chunks,
     const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
-    vec2_t<T>* __restrict__ gh22, double* __restrict__ partial, T* __restrict__ gfm) {
+    vec2_t<T>* __restrict__ gh22, double* __restrict__ partial, T* __restrict__ gfm, const int* __restrict__ cuts) {
     const int64_t x = blockIdx.x;
     const int k = (int)(x % kHmModes);
     const int64_t c = (x / kHmModes) % chunks;
@@ -1782,7 +1831,11 @@ __device__ __forceinline__ void xhm_hm_adjoint_wave(
 #pragma unroll
     for (int j = 0; j < kHmAcc; ++j) acc[j] = T(0);
     T v4 = T(0), v5 = T(0), v6 = T(0);
-    for (int64_t i = c * 64 + lane; i < n; i += chunks * 64) {
+    // up to the mode's first ringdown bin (all bins where the mode vanishes:
+    // this wave then writes each bin's zero cotangents)
+    const int cut = cuts[b * kHmModes + k];
+    const int64_t end = (!on || cut < 0) ? n : cut;
+    for (int64_t i = c * 64 + lane; i < end; i += chunks * 64) {
         T gw_re = T(0), gw_im = T(0), gf = T(0);
         if (on) {
             const PowerRow<T> row = table[i];
@@ -1847,9 +1900,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void xh
     const float* __restrict__ modes, const float* __restrict__ dmodes, int64_t nb,
     const PowerRow<float>* __restrict__ table, int64_t n, int64_t chunks,
     const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross, float2* __restrict__ gh22,
-    double* __restrict__ partial, float* __restrict__ gfm) {
+    double* __restrict__ partial, float* __restrict__ gfm, const int* __restrict__ cuts) {
     xhm_hm_adjoint_wave<float, Grid>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, cot_plus,
-                                     cot_cross, gh22, partial, gfm);
+                                     cot_cross, gh22, partial, gfm, cuts);
 }
 template <bool Grid>
 __global__ __launch_bounds__(64) void xhm_hm_adjoint_f64_kernel(
@@ -1857,9 +1910,9 @@ __global__ __launch_bounds__(64) void xhm_hm_adjoint_f64_kernel(
     const double* __restrict__ modes, const double* __restrict__ dmodes, int64_t nb,
     const PowerRow<double>* __restrict__ table, int64_t n, int64_t chunks,
     const double2* __restrict__ cot_plus, const double2* __restrict__ cot_cross, double2* __restrict__ gh22,
-    double* __restrict__ partial, double* __restrict__ gfm) {
+    double* __restrict__ partial, double* __restrict__ gfm, const int* __restrict__ cuts) {
     xhm_hm_adjoint_wave<double, Grid>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks, cot_plus,
-                                      cot_cross, gh22, partial, gfm);
+                                      cot_cross, gh22, partial, gfm, cuts);
 }
 
 // one wave per (binary, chunk, higher mode) on the mode's ringdown bins
@@ -1872,7 +1925,7 @@ __device__ __forceinline__ void xhm_hm_rd_adjoint_wave(
     const T* __restrict__ rec22, const T* __restrict__ head, const T* __restrict__ dhead, const T* __restrict__ modes,
     const T* __restrict__ dmodes, int64_t nb, const PowerRow<T>* __restrict__ table, int64_t n, int64_t chunks,
     const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
-    vec2_t<T>* __restrict__ gh22, double* __restrict__ partial, T* __restrict__ gfm) {
+    vec2_t<T>* __restrict__ gh22, double* __restrict__ partial, T* __restrict__ gfm, const int* __restrict__ cuts) {
     constexpr int kModes = Mixing ? 1 : 3;
     constexpr int kAcc = Mixing ? (int)kMixAcc : (int)kRdAcc;
     const int64_t x = blockIdx.x;
@@ -1903,7 +1956,9 @@ __device__ __forceinline__ void xhm_hm_rd_adjoint_wave(
 #pragma unroll
     for (int j = 0; j < kAcc; ++j) acc[j] = T(0);
     T v4 = T(0), v5 = T(0), v6 = T(0);
-    for (int64_t i = c * 64 + lane; i < n; i += chunks * 64) {
+    // from the block of the mode's first ringdown bin
+    const int cut = cuts[b * kHmModes + k];
+    for (int64_t i = first_bin_from(cut < 0 ? 0 : cut, c, chunks, lane); i < n; i += chunks * 64) {
         const PowerRow<T> row = table[i];
         const BinPowers<T> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
         if (!hm_ringdown_bin(r, r[kHmMsec] * p.f)) continue;  // the general wave's bin
@@ -1950,9 +2005,9 @@ __global__ __launch_bounds__(64) void xhm_hm_rd_adjoint_kernel(
     const float* __restrict__ modes, const float* __restrict__ dmodes, int64_t nb,
     const PowerRow<float>* __restrict__ table, int64_t n, int64_t chunks,
     const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross, float2* __restrict__ gh22,
-    double* __restrict__ partial, float* __restrict__ gfm) {
+    double* __restrict__ partial, float* __restrict__ gfm, const int* __restrict__ cuts) {
     xhm_hm_rd_adjoint_wave<float, Grid, Mixing>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks,
-                                                cot_plus, cot_cross, gh22, partial, gfm);
+                                                cot_plus, cot_cross, gh22, partial, gfm, cuts);
 }
 template <bool Grid, bool Mixing>
 __global__ __launch_bounds__(64) void xhm_hm_rd_adjoint_f64_kernel(
@@ -1960,9 +2015,9 @@ __global__ __launch_bounds__(64) void xhm_hm_rd_adjoint_f64_kernel(
     const double* __restrict__ modes, const double* __restrict__ dmodes, int64_t nb,
     const PowerRow<double>* __restrict__ table, int64_t n, int64_t chunks,
     const double2* __restrict__ cot_plus, const double2* __restrict__ cot_cross, double2* __restrict__ gh22,
-    double* __restrict__ partial, double* __restrict__ gfm) {
+    double* __restrict__ partial, double* __restrict__ gfm, const int* __restrict__ cuts) {
     xhm_hm_rd_adjoint_wave<double, Grid, Mixing>(rec22, head, dhead, modes, dmodes, nb, table, n, chunks,
-                                                 cot_plus, cot_cross, gh22, partial, gfm);
+                                                 cot_plus, cot_cross, gh22, partial, gfm, cuts);
 }
 
 // one wave per (binary, chunk): the (2,2)'s adjoint (the IMRPhenomXAS
@@ -2036,31 +2091,40 @@ __global__ __launch_bounds__(kBlock) void xhm_grid_sum_kernel(const T* __restric
 // each binary's gradient (and loss) from the waves' sums, in a fixed order:
 // d_L's derivative -Re conj(G) h / d_L, f_ref's half phi_ref's times the
 // (2,2)'s phase turn per unit f_ref (its record's inspiral constant's
-// derivative less its anchor's: zero where f_ref is zero or mass-scaled)
+// derivative less its anchor's: zero where f_ref is zero or mass-scaled).
+// Eight threads per binary, thread j summing column j of its rows (and the
+// one of column 7, which is zero, the loss), each sum in row order.
+constexpr int64_t kXhmFinishPerBlock = 64 / kXhmGradSlots;
 __global__ __launch_bounds__(64) void xhm_adjoint_finish_kernel(const double* __restrict__ params,
                                                                 const double* __restrict__ jac22, int64_t nb,
                                                                 const double* __restrict__ partial, int64_t chunks,
                                                                 const double* __restrict__ loss_partial,
                                                                 int64_t loss_chunks, double* __restrict__ loss,
                                                                 double* __restrict__ grads) {
-    const int64_t b = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (b >= nb) return;
-    double s[kXhmGradSlots];
-    for (int j = 0; j < kXhmGradSlots; ++j) s[j] = 0.0;
-    for (int64_t rc = 0; rc < kXhmGradRows * chunks; ++rc)
-        for (int j = 0; j < kXhmGradSlots; ++j) s[j] += partial[(b * kXhmGradRows * chunks + rc) * kXhmGradSlots + j];
-    double* g = grads + b * 8;
-    for (int j = 0; j < 4; ++j) g[j] = s[j];
-    g[4] = -s[4] / params[b * kParams + kDistance];
-    g[5] = s[5];
-    g[6] = s[6];
-    const double turn = jac22[(b * kSlots + kInsConst) * 8 + kFRef] - jac22[(b * kSlots + kAnchor) * 8 + kFRef];
-    g[7] = 0.5 * s[6] * turn;
-    if (loss) {
-        double l = 0.0;
-        for (int64_t c = 0; c < loss_chunks; ++c) l += loss_partial[b * loss_chunks + c];
-        loss[b] = l;
+    const int64_t b = (int64_t)blockIdx.x * kXhmFinishPerBlock + threadIdx.x / kXhmGradSlots;
+    const int j = (int)(threadIdx.x % kXhmGradSlots);
+    const bool live = b < nb;
+    double s = 0.0, l = 0.0;
+    if (live) {
+        for (int64_t rc = 0; rc < kXhmGradRows * chunks; ++rc)
+            s += partial[(b * kXhmGradRows * chunks + rc) * kXhmGradSlots + j];
+        if (loss && j == kXhmGradSlots - 1)
+            for (int64_t c = 0; c < loss_chunks; ++c) l += loss_partial[b * loss_chunks + c];
     }
+    // the binary's sums of columns 4 and 6, to every one of its threads
+    const int first = (int)(threadIdx.x & ~(kXhmGradSlots - 1));
+    const double s4 = __shfl(s, first + 4, 64), s6 = __shfl(s, first + 6, 64);
+    if (!live) return;
+    double v = s;
+    if (j == 4) {
+        v = -s4 / params[b * kParams + kDistance];
+    } else if (j == 7) {
+        const double turn =
+            jac22[(b * kSlots + kInsConst) * 8 + kFRef] - jac22[(b * kSlots + kAnchor) * 8 + kFRef];
+        v = 0.5 * s6 * turn;
+        if (loss) loss[b] = l;
+    }
+    grads[b * 8 + j] = v;
 }
 
 // the reference phase with its derivatives, for model Model behind `name`
@@ -2491,6 +2555,8 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
     const V* cot_plus = (const V*)in_plus;
     const V* cot_cross = (const V*)in_cross;
     power_table_kernel<T, T><<<table_blocks, kBlock, 0, s>>>(freqs, n, table);
+    int* cuts = (int*)(ws + g.cuts);
+    xhm_cuts_kernel<T><<<(int)((nb * kHmModes + kBlock - 1) / kBlock), kBlock, 0, s>>>(modes, nb, table, n, cuts);
     if (weights) {
         V* res_plus = (V*)(ws + g.cot_plus);
         V* res_cross = (V*)(ws + g.cot_cross);
@@ -2505,23 +2571,23 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
             xhm_hm_adjoint_kernel<true><<<hm_blocks, 64, 0, s>>>(
                 (const float*)records22, (const float*)head, (const float*)dhead, (const float*)modes,
                 (const float*)dmodes, nb, (const PowerRow<float>*)table, n, g.chunks, (const float2*)cot_plus,
-                (const float2*)cot_cross, (float2*)gh22, partial, (float*)gfm);
+                (const float2*)cot_cross, (float2*)gh22, partial, (float*)gfm, cuts);
         else
             xhm_hm_adjoint_kernel<false><<<hm_blocks, 64, 0, s>>>(
                 (const float*)records22, (const float*)head, (const float*)dhead, (const float*)modes,
                 (const float*)dmodes, nb, (const PowerRow<float>*)table, n, g.chunks, (const float2*)cot_plus,
-                (const float2*)cot_cross, (float2*)gh22, partial, nullptr);
+                (const float2*)cot_cross, (float2*)gh22, partial, nullptr, cuts);
     } else {
         if (dldf)
             xhm_hm_adjoint_f64_kernel<true><<<hm_blocks, 64, 0, s>>>(
                 (const double*)records22, (const double*)head, (const double*)dhead, (const double*)modes,
                 (const double*)dmodes, nb, (const PowerRow<double>*)table, n, g.chunks,
-                (const double2*)cot_plus, (const double2*)cot_cross, (double2*)gh22, partial, (double*)gfm);
+                (const double2*)cot_plus, (const double2*)cot_cross, (double2*)gh22, partial, (double*)gfm, cuts);
         else
             xhm_hm_adjoint_f64_kernel<false><<<hm_blocks, 64, 0, s>>>(
                 (const double*)records22, (const double*)head, (const double*)dhead, (const double*)modes,
                 (const double*)dmodes, nb, (const PowerRow<double>*)table, n, g.chunks,
-                (const double2*)cot_plus, (const double2*)cot_cross, (double2*)gh22, partial, nullptr);
+                (const double2*)cot_plus, (const double2*)cot_cross, (double2*)gh22, partial, nullptr, cuts);
     }
     // the ringdown waves: the plain modes, then the mixing (3,2) (its
     // cotangent of the (2,2) complete before the (2,2)'s waves)
@@ -2533,14 +2599,14 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
         const PowerRow<float>* tb = (const PowerRow<float>*)table;
         if (dldf) {
             xhm_hm_rd_adjoint_kernel<true, false><<<rd_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n, g.chunks,
-                (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (float*)gfm);
+                (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (float*)gfm, cuts);
             xhm_hm_rd_adjoint_kernel<true, true><<<mix_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n, g.chunks,
-                (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (float*)gfm);
+                (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (float*)gfm, cuts);
         } else {
             xhm_hm_rd_adjoint_kernel<false, false><<<rd_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
-                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
+                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr, cuts);
             xhm_hm_rd_adjoint_kernel<false, true><<<mix_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
-                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
+                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr, cuts);
         }
     } else {
         using V2 = double2;
@@ -2549,14 +2615,14 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
         const PowerRow<double>* tb = (const PowerRow<double>*)table;
         if (dldf) {
             xhm_hm_rd_adjoint_f64_kernel<true, false><<<rd_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
-                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (double*)gfm);
+                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (double*)gfm, cuts);
             xhm_hm_rd_adjoint_f64_kernel<true, true><<<mix_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
-                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (double*)gfm);
+                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, (double*)gfm, cuts);
         } else {
             xhm_hm_rd_adjoint_f64_kernel<false, false><<<rd_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
-                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
+                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr, cuts);
             xhm_hm_rd_adjoint_f64_kernel<false, true><<<mix_blocks, 64, 0, s>>>(r22, hd, dh, md, dm, nb, tb, n,
-                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr);
+                g.chunks, (const V2*)cot_plus, (const V2*)cot_cross, (V2*)gh22, partial, nullptr, cuts);
         }
     }
     if (dldf)
@@ -2567,7 +2633,7 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
         xhm_22_adjoint_kernel<T, false><<<blocks22, 64, 0, s>>>(records22, jacobian22, head, dhead, nb, table, n,
                                                                 g.chunks, cot_plus, cot_cross, gh22, partial,
                                                                 nullptr);
-    xhm_adjoint_finish_kernel<<<(int)((nb + 63) / 64), 64, 0, s>>>(
+    xhm_adjoint_finish_kernel<<<(int)((nb + kXhmFinishPerBlock - 1) / kXhmFinishPerBlock), 64, 0, s>>>(
         params, jacobian22, nb, partial, g.chunks, weights ? (const double*)(ws + g.loss_partial) : nullptr,
         g.loss_chunks, weights ? loss : nullptr, grads);
     if (dldf) {
@@ -2597,7 +2663,8 @@ int wfx_xhm_adjoint(const double* params, const void* records22, const double* j
     if (n_freqs == 0) {
         // no frequency: a zero gradient and loss (a kernel, not a memset
         // node, should the caller capture the call in a graph)
-        const int blocks = grid_blocks((n_binaries + 63) / 64, "wfx_xhm_adjoint: grid too large");
+        const int blocks = grid_blocks((n_binaries + kXhmFinishPerBlock - 1) / kXhmFinishPerBlock,
+                                       "wfx_xhm_adjoint: grid too large");
         if (blocks < 0) return -1;
         xhm_adjoint_finish_kernel<<<blocks, 64, 0, s>>>(params, jacobian22, n_binaries, nullptr, 0, nullptr, 0,
                                                         weights ? loss : nullptr, grads);
