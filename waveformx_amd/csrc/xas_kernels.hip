@@ -279,15 +279,19 @@ struct PhenomDRecord {
 // (records) and/or FP32 split (split_records), either NULL. With Keep
 // (IMRPhenomXAS), the thread of (b, dir) also writes its coefficient structs
 // in dual numbers to keep[4 b + dir], for IMRPhenomXHM's higher modes.
-// Each slot leaves the set-up as soon as it is final: its derivative to the
-// thread's Jacobian column, its value (direction 0's) to the wave's LDS, so
-// that the set-up's registers never hold the whole record in dual numbers.
+// Each slot leaves the set-up as soon as it is final: its derivative and its
+// value (direction 0's) to the wave's LDS, so that the set-up's registers
+// never hold the whole record in dual numbers. After the set-up the wave
+// stores its binaries' Jacobian rows whole (derivatives and the closed-form
+// half's zeros, contiguous 64-byte rows), then the closed-form entries over
+// them: rows stored as 32-byte halves (each thread its column) moved 58 MB
+// for 36 MB at 8192 binaries, whole rows 41 MB.
 struct JvpSink {
-    double* col;   // the thread's column: slot k at col[8 k]
+    double* col;   // the thread's derivatives (LDS): slot k at col[4 k]
     double* vals;  // the binary's values, slot k at vals[k]
     bool lead;     // direction 0: writes the values
     __device__ void operator()(int k, const Dual<1>& x) const {
-        col[k * 8] = k < 60 ? x.d[0] : 0.0;
+        col[k * 4] = k < 60 ? x.d[0] : 0.0;
         if (lead) vals[k] = x.v;
     }
 };
@@ -303,8 +307,10 @@ __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict_
     const int64_t b = gid >> 2;
     const int dir = (int)(gid & 3);
     const bool live = b < nb;
-    // the 16 binaries' values of the block (one wave)
+    // the 16 binaries' values of the block (one wave), and their Jacobian
+    // rows' derivative halves (after those are stored, their FP32 records)
     __shared__ double s_vals[16][kSlots];
+    __shared__ double s_dual[16][kSlots][4];
     double* vals = s_vals[threadIdx.x >> 2];
     using D = Dual<1>;
     unsigned fl = 0;
@@ -314,28 +320,46 @@ __global__ __launch_bounds__(64) void setup_jvp_kernel(const double* __restrict_
             p[k] = D(params[b * kParams + k]);
             p[k].d[0] = k == dir ? 1.0 : 0.0;
         }
-        JvpSink out{jac + b * kSlots * 8 + dir, vals, dir == 0};
+        JvpSink out{&s_dual[threadIdx.x >> 2][0][dir], vals, dir == 0};
         if constexpr (Keep)
             fl = S::template record_to<D>(p, D(f_min[b * f_min_stride]), amp_scale, reference_mf, out, keep + gid);
         else
             fl = S::template record_to<D>(p, D(f_min[b * f_min_stride]), amp_scale, reference_mf, out);
     }
     __syncthreads();
-    if (!live) return;
-    double* out = jac + b * kSlots * 8;
-    double pv[kParams];
-    for (int k = 0; k < kParams; ++k) pv[k] = params[b * kParams + k];
-    for (int k = 0; k < kSlots; ++k) out[k * 8 + dir + 4] = 0.0;
-    S::closed_column(dir + 4, pv, reference_mf, fl, vals, out + dir + 4, 8);
-    if (dir == 0) {
-        double v[kSlots];
-        for (int k = 0; k < kSlots; ++k) v[k] = vals[k];
-        v[kT0Lo] = 0.0;
-        v[kAnchorLo] = 0.0;
+    // the block's binaries' rows (contiguous), element e of row e / 8:
+    // the derivatives in columns 0..3, zeros in the closed-form columns
+    const int64_t first = (int64_t)blockIdx.x * (blockDim.x >> 2);
+    const int64_t count = (nb - first < 16 ? nb - first : 16) * kSlots;
+    double* rows = jac + first * kSlots * 8;
+    for (int64_t e = threadIdx.x; e < count * 8; e += blockDim.x) {
+        const int64_t row = e >> 3;
+        const int c = (int)(e & 7);
+        rows[e] = c < 4 ? s_dual[row / kSlots][row % kSlots][c] : 0.0;
+    }
+    // the barrier orders those zeros before the closed-form entries that
+    // other threads of the block store over them
+    __syncthreads();
+    float(*s_split)[kSlots] = reinterpret_cast<float(*)[kSlots]>(&s_dual[0][0][0]);
+    if (live) {
+        double pv[kParams];
+        for (int k = 0; k < kParams; ++k) pv[k] = params[b * kParams + k];
+        S::closed_column(dir + 4, pv, reference_mf, fl, vals, jac + b * kSlots * 8 + dir + 4, 8);
+        if (dir == 0) {
+            double v[kSlots];
+            for (int k = 0; k < kSlots; ++k) v[k] = vals[k];
+            v[kT0Lo] = 0.0;
+            v[kAnchorLo] = 0.0;
+            if (split_records) write_f32_record(v, s_split[threadIdx.x >> 2], S::kFoldA, S::kFoldB);
+            if (flags) flags[b] = fl;
+        }
+    }
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < count; e += blockDim.x) {
+        const int slot = (int)(e % kSlots);
         if (records)
-            for (int k = 0; k < kSlots; ++k) records[b * kSlots + k] = v[k];
-        if (split_records) write_f32_record(v, split_records + b * kSlots, S::kFoldA, S::kFoldB);
-        if (flags) flags[b] = fl;
+            records[first * kSlots + e] = (slot == kT0Lo || slot == kAnchorLo) ? 0.0 : s_vals[e / kSlots][slot];
+        if (split_records) split_records[first * kSlots + e] = s_split[e / kSlots][slot];
     }
 }
 
@@ -1632,7 +1656,7 @@ inline XhmGradPlan xhm_grad_plan(int64_t nb, int64_t n, bool loss, bool grid, in
     g.table = off;
     off += align256(n * row);
     g.cuts = off;
-    off += align256(nb * kHmModes * (int64_t)sizeof(int));
+    off += align256((nb * kHmModes + 1) * (int64_t)sizeof(int));
     g.gh22 = off;
     off += align256(nb * n * 2 * real);
     g.partial = off;
@@ -1758,34 +1782,53 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_residual_kernel(
 // and its ringdown bins those from it. The general waves stop at the cut
 // and the ringdown waves start at its block, instead of both visiting every
 // bin; -1 (every wave visits every bin, testing each) where the grid does
-// not ascend or M is not a positive finite number. One thread per (binary,
-// mode); every block first checks the grid's order (a read of the table's
-// frequencies, from L2).
+// not ascend or M is not a positive finite number.
+
+// *ascends = 1 if the table's frequencies never decrease (and hold no NaN)
+// and the bins' indices fit an int, else 0: one block of kOrderBlock threads
+constexpr int kOrderBlock = 1024;
 template <class T>
-__global__ __launch_bounds__(kBlock) void xhm_cuts_kernel(const T* __restrict__ modes, int64_t nb,
-                                                          const PowerRow<T>* __restrict__ table, int64_t n,
-                                                          int* __restrict__ cuts) {
+__global__ __launch_bounds__(kOrderBlock) void grid_order_kernel(const PowerRow<T>* __restrict__ table, int64_t n,
+                                                                 int* __restrict__ ascends) {
     int descends = n > INT_MAX;
-    for (int64_t i = threadIdx.x; i + 1 < n && !descends; i += kBlock)
-        descends = !(table[i].f <= table[i + 1].f);
+#pragma unroll 8
+    for (int64_t i = threadIdx.x; i + 1 < n; i += kOrderBlock) descends |= !(table[i].f <= table[i + 1].f);
     descends = __syncthreads_or(descends);
-    const int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (threadIdx.x == 0) *ascends = !descends;
+}
+
+// one wave per (binary, mode): a 64-ary search of the predicate's first
+// true bin (lane l probes lo + (l + 1) step - 1; three rounds for 8193 bins)
+template <class T>
+__global__ __launch_bounds__(64) void xhm_cuts_kernel(const T* __restrict__ modes, int64_t nb,
+                                                      const PowerRow<T>* __restrict__ table, int64_t n,
+                                                      const int* __restrict__ ascends, int* __restrict__ cuts) {
+    const int64_t x = blockIdx.x;
+    const int lane = threadIdx.x;
     if (x >= nb * kHmModes) return;
     const T* r = modes + x * kHmSlots;
     const T msec = r[kHmMsec];
-    int cut = -1;
-    if (!descends && msec > T(0) && msec <= std::numeric_limits<T>::max()) {
-        int64_t lo = 0, hi = n;
-        while (lo < hi) {
-            const int64_t mid = lo + (hi - lo) / 2;
-            if (hm_ringdown_bin(r, msec * table[mid].f))
-                hi = mid;
-            else
-                lo = mid + 1;
-        }
-        cut = (int)lo;
+    if (!*ascends || !(msec > T(0) && msec <= std::numeric_limits<T>::max())) {
+        if (lane == 0) cuts[x] = -1;
+        return;
     }
-    cuts[x] = cut;
+    // the predicate is false below lo and true from hi
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t step = (hi - lo + 63) / 64;
+        const int64_t p = lo + (lane + 1) * step - 1;
+        const bool probe = p < hi;
+        const uint64_t found = __ballot(probe && hm_ringdown_bin(r, msec * table[p].f));
+        const uint64_t probed = __ballot(probe);
+        if (found) {
+            const int64_t first = __ffsll((unsigned long long)found) - 1;
+            hi = lo + (first + 1) * step - 1;
+            lo += first * step;
+        } else {
+            lo += (64 - __clzll((unsigned long long)probed)) * step;
+        }
+    }
+    if (lane == 0) cuts[x] = (int)lo;
 }
 
 // the first bin of wave (c, lane) at or past the block of bin `from`: the
@@ -2106,10 +2149,14 @@ __global__ __launch_bounds__(64) void xhm_adjoint_finish_kernel(const double* __
     const bool live = b < nb;
     double s = 0.0, l = 0.0;
     if (live) {
+        // unrolled so that the loads of a batch are in flight together
+#pragma unroll 9
         for (int64_t rc = 0; rc < kXhmGradRows * chunks; ++rc)
             s += partial[(b * kXhmGradRows * chunks + rc) * kXhmGradSlots + j];
-        if (loss && j == kXhmGradSlots - 1)
+        if (loss && j == kXhmGradSlots - 1) {
+#pragma unroll 8
             for (int64_t c = 0; c < loss_chunks; ++c) l += loss_partial[b * loss_chunks + c];
+        }
     }
     // the binary's sums of columns 4 and 6, to every one of its threads
     const int first = (int)(threadIdx.x & ~(kXhmGradSlots - 1));
@@ -2555,8 +2602,11 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
     const V* cot_plus = (const V*)in_plus;
     const V* cot_cross = (const V*)in_cross;
     power_table_kernel<T, T><<<table_blocks, kBlock, 0, s>>>(freqs, n, table);
+    // the cuts, then the grid's order flag after them
     int* cuts = (int*)(ws + g.cuts);
-    xhm_cuts_kernel<T><<<(int)((nb * kHmModes + kBlock - 1) / kBlock), kBlock, 0, s>>>(modes, nb, table, n, cuts);
+    int* ascends = cuts + nb * kHmModes;
+    grid_order_kernel<T><<<1, kOrderBlock, 0, s>>>(table, n, ascends);
+    xhm_cuts_kernel<T><<<(int)(nb * kHmModes), 64, 0, s>>>(modes, nb, table, n, ascends, cuts);
     if (weights) {
         V* res_plus = (V*)(ws + g.cot_plus);
         V* res_cross = (V*)(ws + g.cot_cross);
