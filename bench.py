@@ -246,6 +246,43 @@ def bench_ragged(torch, xas, dev, nb, steps, warmup):
             "ragged_padded_bins": nb * longest, "ragged_shape": [plan.bins_per_thread, plan.blocks]}
 
 
+XHM_MODE_NAMES = ("22", "21", "33", "32", "44")
+
+
+def xhm_mode_gradients(torch, xhm, params, perturbed, f32, f64, w32, w64, rows):
+    """Per mode, on the first ``rows`` binaries: the loss of that mode alone
+    (the other modes' factors in the polarizations zeroed in the set-up's
+    head, and in the data's), its split gradient against its FP64 one (both
+    the exact adjoint; the (3,2)'s ringdown still recycles the (2,2)): the
+    largest error per parameter over the binaries, relative to the
+    parameter's largest gradient (max over the 7 parameters), and the worst
+    fraction of the reference's float32 bar."""
+    params = params[:rows]
+    pert = torch.as_tensor(perturbed[:rows], device=params.device)
+    out = {}
+    for k, name in enumerate(XHM_MODE_NAMES):
+        def only(head):
+            kept = head[:, 4 * k:4 * k + 4].clone()
+            head = head.clone()
+            head[:, :20] = 0
+            head[:, 4 * k:4 * k + 4] = kept
+            return head
+
+        records, head, modes_, _ = xhm.setup(pert, f64[:1])
+        d_plus, d_cross = xhm.evaluate(records, only(head), modes_, f64)
+        grads = {}
+        for precision, grid, w, ctype in (("split", f32, w32, torch.complex64), ("fp64", f64, w64, torch.complex128)):
+            j = xhm.setup_jvp(params, grid[:1].to(torch.float64), precision=precision)
+            j["head"], j["dhead"] = only(j["head"]), only(j["dhead"])
+            grads[precision] = xhm.adjoint_with(j, params, grid, d_plus.to(ctype), d_cross.to(ctype), w,
+                                                precision=precision)[0][:, :7]
+        g32, g64 = grads["split"], grads["fp64"]
+        bar = 1e-2 * g64.abs() + 1e-3 * g64.abs().amax(0)
+        out[name] = {"max_rel_err": float(((g32 - g64).abs().amax(0) / g64.abs().amax(0)).max()),
+                     "worst_fraction_of_float32_bar": float(((g32 - g64).abs() / bar).max())}
+    return out
+
+
 def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
     """IMRPhenomXHM, the multimode workload: nb binaries on linspace(20,
     2048, n) Hz, the five modes summed into h+, hx, in FP64 (complex128) and
@@ -372,6 +409,8 @@ def bench_xhm(torch, dev, nb, n, steps, warmup, cpu=True):
                    "xhm_grad_worst_fraction_of_float32_bar": float(((g32 - g64).abs() / bar)[:, :7].max()),
                    "xhm_loss_rel_err_split_vs_fp64": float(((loss32 - loss64).abs() / loss64).max())})
     del d_plus, d_cross
+    result["xhm_grad_split_vs_fp64_per_mode"] = xhm_mode_gradients(torch, xhm, params, perturbed, f32, f64r, w32,
+                                                                   w64, min(nb, 512))
     per_bin = XHM_FP64_OPS_PER_BIN.get((nb, n))
     if per_bin is not None:
         # the formula's FP64 arithmetic (add, multiply, divide, compare; its elementary functions apart) per

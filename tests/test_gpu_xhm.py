@@ -588,6 +588,78 @@ def test_loss_and_grad(cuda):
         adjoint.loss_and_grad(p, f, dp, dc, w, precision="split", powers=w, approximant="IMRPhenomXHM")
 
 
+def test_single_mode_gradient(cuda):
+    """xhm.adjoint_with: from an unchanged set-up, adjoint's bits; with all
+    but one mode's factors zeroed in the set-up's head (and in the data),
+    the exact gradient of that mode's own loss: against central differences
+    of that loss (the FP64 evaluation with the same head, steps of 1e-6
+    times max(|x|, 0.1) in the masses and spins) to 1e-5 of each
+    parameter's largest, and
+    the split gradient within the reference's float32 bar of the FP64 one
+    (every mode; the (3,2)'s ringdown still recycles the (2,2))."""
+    import torch
+
+    from waveformx_amd import xhm
+
+    params = oxhm.draw(3, seed=21)
+    p = torch.as_tensor(params, device=cuda)
+    perturbed = params * (1.0 + 1e-2 * np.random.default_rng(5).normal(size=params.shape))
+    perturbed[:, 7] = params[:, 7]
+    freqs = np.linspace(20.0, 600.0, 700)
+    psd = (freqs / 100.0) ** -4 + 1.0 + (freqs / 100.0) ** 2
+    f64 = torch.as_tensor(freqs, device=cuda)
+    f32 = f64.float()
+    w64 = torch.as_tensor(1.0 / psd, device=cuda)
+
+    def only(head, k):
+        out = torch.zeros_like(head)
+        out[:, 20:] = head[:, 20:]
+        out[:, 4 * k:4 * k + 4] = head[:, 4 * k:4 * k + 4]
+        return out
+
+    def forward(theta, k):
+        records, head, modes_, _ = xhm.setup(torch.as_tensor(theta, device=cuda), f64[:1])
+        return xhm.evaluate(records, only(head, k), modes_, f64)
+
+    cot = torch.as_tensor(np.random.default_rng(6).normal(size=(3, 700)) * (1 + 1j), device=cuda)
+    j = xhm.setup_jvp(p, f64[:1], precision="fp64")
+    np.testing.assert_array_equal(xhm.adjoint_with(j, p, f64, cot, cot, precision="fp64")[0].cpu().numpy(),
+                                  xhm.adjoint(p, f64, cot, cot, precision="fp64")[0].cpu().numpy())
+    for k in range(5):
+        dp, dc = forward(perturbed, k)
+        j = xhm.setup_jvp(p, f64[:1], precision="fp64")
+        j["head"], j["dhead"] = only(j["head"], k), only(j["dhead"], k)
+        grads, loss, _ = xhm.adjoint_with(j, p, f64, dp, dc, w64, precision="fp64")
+        grads = grads.cpu().numpy()
+        hp, hc = forward(params, k)
+        np.testing.assert_allclose(loss.cpu().numpy(),
+                                   (((hp - dp).abs() ** 2 + (hc - dc).abs() ** 2) @ w64).cpu().numpy(), rtol=1e-9)
+        diff = np.zeros((3, 4))
+        for d in range(4):
+            step = 1e-6 * np.maximum(np.abs(params[:, d]), 0.1)
+            losses = []
+            for sign in (1.0, -1.0):
+                theta = params.copy()
+                theta[:, d] += sign * step
+                hp, hc = forward(theta, k)
+                losses.append((((hp - dp).abs() ** 2 + (hc - dc).abs() ** 2) @ w64).cpu().numpy())
+            diff[:, d] = (losses[0] - losses[1]) / (2.0 * step)
+        err = np.abs(grads[:, :4] - diff) / np.abs(diff).max(axis=0)
+        print("mode", oxhm.MODES[k], "gradient vs central differences", err.max(axis=0), flush=True)
+        assert np.all(err <= 1e-5), (k, err)
+        j32 = xhm.setup_jvp(p, f32[:1].double(), precision="split")
+        j32["head"], j32["dhead"] = only(j32["head"], k), only(j32["dhead"], k)
+        dp64, dc64 = forward(perturbed, k)
+        g32 = xhm.adjoint_with(j32, p, f32, dp64.to(torch.complex64), dc64.to(torch.complex64), w64.float(),
+                               precision="split")[0].cpu().numpy()
+        j64 = xhm.setup_jvp(p, f32[:1].double(), precision="fp64")
+        j64["head"], j64["dhead"] = only(j64["head"], k), only(j64["dhead"], k)
+        g64 = xhm.adjoint_with(j64, p, f32.double(), dp64, dc64, w64, precision="fp64")[0].cpu().numpy()
+        excess = np.abs(g32 - g64)[:, :7] / float32_bar(g64[:, :7])
+        print("mode", oxhm.MODES[k], "split gradient, worst fraction of the float32 bar", excess.max(), flush=True)
+        assert np.all(excess <= 1.0), (k, excess)
+
+
 def test_split_gradient(cuda):
     """The exact split gradient (xhm.adjoint: the dual-number set-up and
     the reverse-mode adjoint of the split evaluation) for random cotangents

@@ -505,6 +505,29 @@ def adjoint(params: torch.Tensor, freqs: torch.Tensor, in_plus: torch.Tensor, in
     _require_device(params, "params")
     params = params.to(torch.float64).contiguous()
     rdtype = torch.float64 if precision == "fp64" else torch.float32
+    nb, n = params.shape[0], freqs.shape[0]
+    j = None
+    if nb > 0 and n > 0:
+        j = setup_jvp(params, freqs[:1].to(rdtype).to(torch.float64), amp_scale, reference_mf, precision)
+    return adjoint_with(j, params, freqs, in_plus, in_cross, weights, precision, wrt_grid)
+
+
+def adjoint_with(setup: Optional[dict], params: torch.Tensor, freqs: torch.Tensor, in_plus: torch.Tensor,
+                 in_cross: torch.Tensor, weights: Optional[torch.Tensor] = None, precision: str = "split",
+                 wrt_grid: bool = False
+                 ) -> Tuple[torch.Tensor, Optional[torch.Tensor], Optional[torch.Tensor]]:
+    """:func:`adjoint` from a derivative set-up already made
+    (:func:`setup_jvp`'s dict, in ``precision``, of these ``params`` and
+    the grid's first frequency; None only for an empty batch or grid): the
+    same launch of ``wfx_xhm_adjoint``. The set-up's head holds each mode's
+    factors in the polarizations; a caller may change them (zeroing all but
+    one mode's factors and their inclination derivatives gives that mode's
+    loss and its exact gradient)."""
+    if precision not in PRECISIONS:
+        raise ValueError(f"IMRPhenomXHM's arithmetic is one of {PRECISIONS}, got {precision!r}")
+    _require_device(params, "params")
+    params = params.to(torch.float64).contiguous()
+    rdtype = torch.float64 if precision == "fp64" else torch.float32
     ctype = torch.complex128 if precision == "fp64" else torch.complex64
     grid = freqs.to(rdtype).contiguous()
     nb, n = params.shape[0], grid.shape[0]
@@ -519,7 +542,9 @@ def adjoint(params: torch.Tensor, freqs: torch.Tensor, in_plus: torch.Tensor, in
     if nb == 0 or n == 0:
         zeros = torch.zeros((nb, 8), dtype=torch.float64, device=dev)
         return zeros, (None if weights is None else torch.zeros((nb,), dtype=torch.float64, device=dev)), dldf
-    j = setup_jvp(params, grid[:1].to(torch.float64), amp_scale, reference_mf, precision)
+    j = setup
+    if j is None or j["records22"].shape[0] != nb or j["head"].dtype != rdtype:
+        raise ValueError(f"the set-up must be setup_jvp's of these {nb} binaries in {precision!r}")
     code = _CODES[precision]
     size = lib().wfx_xhm_adjoint_workspace(nb, n, code, int(weights is not None), int(wrt_grid))
     workspace = torch.empty((size,), dtype=torch.uint8, device=dev)
