@@ -473,6 +473,114 @@ __device__ __forceinline__ void hm_adjoint_split(const T* __restrict__ r, const 
     }
 }
 
+// ----------------------------------------------------- the intermediate
+
+// A plain mode's bins in the intermediate region of both its amplitude and
+// its phase (most of a plain mode's bins before its ringdown on a linear
+// grid): the amplitude's polynomial in u and the phase's closed form,
+// through M f, log(M f), (M f)^-1 and (M f)^(-7/6); 23 accumulators.
+enum InterAcc : int {
+    iAmp0 = 0, iMsec = 1, iCbrtM = 2, iInvM = 3, iLogM = 4, iFRing = 5, iFDamp = 6, iInterC = 7, iInterH = 8,
+    iInter = 9,  // 8
+    iC0 = 17, iC1 = 18, iC2 = 19, iC4 = 20, iCL = 21, iDeltaPhi = 22,
+    kInterAcc = 23
+};
+WFX_CONST int kInterAccSlot[kInterAcc] = {kHmAmp0, kHmMsec, kHmCbrtM, kHmInvM, kHmLogM, kHmFRing, kHmFDamp,
+                                          kHmInterC, kHmInterH,
+                                          kHmInter, kHmInter + 1, kHmInter + 2, kHmInter + 3, kHmInter + 4,
+                                          kHmInter + 5, kHmInter + 6, kHmInter + 7,
+                                          kHmC0, kHmC1, kHmC2, kHmC4, kHmCL, kHmDeltaPhi};
+struct InterIx {
+    static constexpr int FRing = iFRing, FDamp = iFDamp, Rd = -1, FalloffAt = -1;
+};
+
+// the predicate of those bins (the regions' tests of hm_adjoint_split)
+template <class T>
+__device__ __forceinline__ bool hm_intermediate_bin(const T* __restrict__ r, T mf) {
+    return !(mf < r[kHmFAmpIn]) && !(mf >= r[kHmFAmpIm]) && !(mf < r[kHmFPhaseIn]) && !(mf >= r[kHmFPhaseIm]);
+}
+
+// hm_adjoint_split for a plain mode's intermediate bin (hm_intermediate_bin,
+// kHmMixing unset): the same operations, in the same order, on the 23
+// compact accumulators
+template <bool Grid, class T>
+__device__ __forceinline__ void hm_adjoint_intermediate(const T* __restrict__ r, const BinPowers<T>& p, T g_re,
+                                                        T g_im, T* acc, T& h_re, T& h_im, T& gf) {
+    const T mf = r[kHmMsec] * p.f;
+    const T t = p.t * r[kHmCbrtM];
+    const T m1 = p.m1 * r[kHmInvM];
+    const T lg = p.lf + r[kHmLogM];
+    const T rs = d_rsq(t);
+    const T m76 = m1 * rs;
+    const T f_ring = r[kHmFRing], f_damp = r[kHmFDamp];
+    const T z = d_div(mf - f_ring, f_damp);
+    const T atan_z = d_atan(z);
+    const T inv_damp = d_rcp(f_damp);
+    const T datan_z = d_rcp(d_fma(z, z, T(1)));
+    const int n_inter = (int)r[kHmNInter];
+    const T u = (mf - r[kHmInterC]) * r[kHmInterH];
+    T inter = T(0);
+#pragma unroll
+    for (int j = 7; j >= 0; --j)
+        if (j < n_inter) inter = d_fma(inter, u, r[kHmInter + j]);
+    T a = inter * m76;
+    const bool clamped = a < T(0);
+    if (clamped) a = T(1.0e-15);
+    T rest = d_fma(r[kHmC0], mf, r[kHmC1] * lg);
+    rest = d_fma(-r[kHmC2], m1, rest);
+    const T m12 = m1 * m1;
+    rest = d_fma(-(T(1) / T(3)) * r[kHmC4], m12 * m1, rest);
+    rest = d_fma(r[kHmCL], atan_z, rest);
+    const T phi = d_reduce(rest + r[kHmDeltaPhi]);
+    T sn, cs;
+    m_sincos(phi, &sn, &cs);
+    const T amp0 = r[kHmAmp0];
+    const T val = amp0 * a;
+    h_re = val * cs;
+    h_im = val * sn;
+
+    const T g_val = d_fma(g_re, cs, g_im * sn);
+    const T g_phi = val * d_fma(g_im, cs, -g_re * sn);
+    acc[iAmp0] += g_val * a;
+    const T g_a = clamped ? T(0) : g_val * amp0;
+    MfGrad<T> g{T(0), T(0), T(0), T(0), T(0)};
+    acc[iC0] += g_phi * mf;
+    g.mf += g_phi * r[kHmC0];
+    acc[iC1] += g_phi * lg;
+    g.lg += g_phi * r[kHmC1];
+    acc[iC2] -= g_phi * m1;
+    g.m1 -= g_phi * r[kHmC2];
+    acc[iC4] -= g_phi * m12 * m1 * (T(1) / T(3));
+    g.m1 -= g_phi * r[kHmC4] * m12;
+    acc[iCL] += g_phi * atan_z;
+    atan_z_adjoint<InterIx>(g_phi * r[kHmCL] * datan_z, z, inv_damp, acc, g);
+    acc[iDeltaPhi] += g_phi;
+    g.m76 += g_a * inter;
+    const T g_res = g_a * m76;
+    T pw = T(1), d_u = T(0);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (j < n_inter) acc[iInter + j] += g_res * pw;
+        if (j + 1 < n_inter) d_u = d_fma((T)(j + 1) * r[kHmInter + j + 1], pw, d_u);
+        pw *= u;
+    }
+    const T g_u = g_res * d_u;
+    acc[iInterH] += g_u * (mf - r[kHmInterC]);
+    const T g_x = g_u * r[kHmInterH];
+    g.mf += g_x;
+    acc[iInterC] -= g_x;
+    g.m1 += g.m76 * rs;
+    g.t -= T(0.5) * g.m76 * m76 * d_div(T(1), t);
+    acc[iMsec] += g.mf * p.f;
+    acc[iCbrtM] += g.t * p.t;
+    acc[iInvM] += g.m1 * p.m1;
+    acc[iLogM] += g.lg;
+    if (Grid) {
+        const T lead = T(0);
+        gf = d_fma(g.mf, r[kHmMsec], (d_fma(g.t, t * (T(1) / T(3)), g.lg - g.m1 * m1) + lead) * p.m1);
+    }
+}
+
 // --------------------------------------------------------- the ringdown
 
 // The ringdown adjoints' compact accumulators. A plain mode's ringdown (the
