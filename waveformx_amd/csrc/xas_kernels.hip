@@ -1609,11 +1609,11 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
 // (most of them on a linear grid) go to waves of their own with compact
 // accumulators (xhm_hm_rd_adjoint_*: 16 slots for a plain mode, 26 for the
 // mixing (3,2)), so that more waves fit a SIMD; the general waves skip them.
-// So do a plain mode's bins in the intermediate region of both
-// (xhm_hm_inter_adjoint_*: 23 slots).
-// Per wave, partial[((b * 12 + row) * chunks + c) * 8 + j] (row 0 the (2,2),
-// 1..4 the higher modes' general waves, 5..8 their ringdown waves, 9..11
-// the intermediate waves of the (2,1), (3,3) and (4,4)):
+// So do a mode's bins in the intermediate region of both
+// (xhm_hm_inter_adjoint_*: 24 slots).
+// Per wave, partial[((b * 13 + row) * chunks + c) * 8 + j] (row 0 the (2,2),
+// 1..4 the higher modes' general waves, 5..8 their ringdown waves, 9..12
+// their intermediate waves):
 // Re conj(cot) dh/d(mass_1, mass_2, chi_1z, chi_2z)
 // (0..3); Re conj(G) h (4); Re conj(cot) dF/d(iota) h (5); Re m conj(G) i h
 // (6), G = conj(F+) cot+ + conj(Fx) cotx the mode's cotangent; 0 (7).
@@ -1621,7 +1621,7 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
 // (gfm[k][b][i]), the (2,2)'s wave adds them to its own (into gfm[0][b][i]),
 // and two passes sum them over the binaries in a fixed order.
 constexpr int kXhmGradSlots = 8;
-constexpr int kXhmGradRows = 12;
+constexpr int kXhmGradRows = 13;
 // at least this many higher-mode waves per launch (the chip holds ~3,000
 // at the kernels' occupancy), and at most this many blocks of 64
 // frequencies per wave: a wave's bins may all fall in one region, and the
@@ -1631,12 +1631,15 @@ constexpr int64_t kXhmGradBlocksPerWave = 64;
 // the grid gradient's first pass: each thread sums this many binaries
 constexpr int64_t kXhmGridGroup = 64;
 
-// the waves per binary and higher mode on its general bins (before its
-// ringdown), at most the chunks
-constexpr int64_t kXhmGenChunks = 3;
+// the waves per binary and higher mode on its general bins (its
+// inspiral, the short part of its grid before the intermediate bins) and
+// on its intermediate bins, at most the chunks (1, 2 or 3 of each within
+// 3% of one another at 4096 x 8193, two the fastest)
+constexpr int64_t kXhmGenChunks = 2;
+constexpr int64_t kXhmInterChunks = 2;
 
 struct XhmGradPlan {
-    int64_t chunks, gen_chunks, loss_chunks, grid_groups;
+    int64_t chunks, gen_chunks, inter_chunks, loss_chunks, grid_groups;
     int64_t table, cuts, gh22, partial, cot_plus, cot_cross, loss_partial, gfm, grid_partial, bytes;
 };
 
@@ -1653,6 +1656,7 @@ inline XhmGradPlan xhm_grad_plan(int64_t nb, int64_t n, bool loss, bool grid, in
     chunks = chunks < least ? least : chunks;
     g.chunks = chunks > blocks ? blocks : chunks;
     g.gen_chunks = g.chunks < kXhmGenChunks ? g.chunks : kXhmGenChunks;
+    g.inter_chunks = g.chunks < kXhmInterChunks ? g.chunks : kXhmInterChunks;
     g.loss_chunks = (n + kXhmBlock - 1) / kXhmBlock;
     g.grid_groups = (nb + kXhmGridGroup - 1) / kXhmGridGroup;
     if (nb > (INT64_MAX / 128) / n) {
@@ -1790,7 +1794,7 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_residual_kernel(
 // ringdown bins those from it. The general waves stop at the cut and the
 // ringdown waves start at its block, instead of both visiting every bin;
 // -1 (every wave visits every bin, testing each) where the grid does not
-// ascend or M is not a positive finite number. Likewise a plain mode's
+// ascend or M is not a positive finite number. Likewise a mode's
 // intermediate bins, a range of their own waves that the general waves
 // jump over.
 
@@ -1832,10 +1836,10 @@ __device__ inline int64_t wave_first_true(int64_t n, int lane, Pred pred) {
 }
 
 // One wave per (binary, mode): cuts[3 x] the first ringdown bin (-1: none
-// known, every wave tests every bin), and for a plain mode [cuts[3 x + 1],
-// cuts[3 x + 2]) its intermediate bins (hm_intermediate_bin: from the first
-// bin past both inspiral bounds to the first bin at either ringdown bound;
-// an empty range for the mixing mode or where the cut is unknown)
+// known, every wave tests every bin), and [cuts[3 x + 1], cuts[3 x + 2])
+// its intermediate bins (hm_intermediate_bin: from the first bin past both
+// inspiral bounds to the first bin at either ringdown bound; an empty range
+// where the cut is unknown)
 template <class T>
 __global__ __launch_bounds__(64) void xhm_cuts_kernel(const T* __restrict__ modes, int64_t nb,
                                                       const PowerRow<T>* __restrict__ table, int64_t n,
@@ -1848,17 +1852,15 @@ __global__ __launch_bounds__(64) void xhm_cuts_kernel(const T* __restrict__ mode
     int rd = -1, lo = 0, hi = 0;
     if (*ascends && msec > T(0) && msec <= std::numeric_limits<T>::max()) {
         rd = (int)wave_first_true(n, lane, [&](int64_t i) { return hm_ringdown_bin(r, msec * table[i].f); });
-        if (r[kHmMixing] == T(0)) {
-            lo = (int)wave_first_true(n, lane, [&](int64_t i) {
-                const T mf = msec * table[i].f;
-                return !(mf < r[kHmFAmpIn]) && !(mf < r[kHmFPhaseIn]);
-            });
-            hi = (int)wave_first_true(n, lane, [&](int64_t i) {
-                const T mf = msec * table[i].f;
-                return mf >= r[kHmFAmpIm] || mf >= r[kHmFPhaseIm];
-            });
-            hi = hi < lo ? lo : hi;
-        }
+        lo = (int)wave_first_true(n, lane, [&](int64_t i) {
+            const T mf = msec * table[i].f;
+            return !(mf < r[kHmFAmpIn]) && !(mf < r[kHmFPhaseIn]);
+        });
+        hi = (int)wave_first_true(n, lane, [&](int64_t i) {
+            const T mf = msec * table[i].f;
+            return mf >= r[kHmFAmpIm] || mf >= r[kHmFPhaseIm];
+        });
+        hi = hi < lo ? lo : hi;
     }
     if (lane == 0) {
         cuts[3 * x] = rd;
@@ -1917,7 +1919,7 @@ __device__ __forceinline__ void xhm_hm_adjoint_wave(
     T v4 = T(0), v5 = T(0), v6 = T(0);
     // the blocks of 64 bins c, c + gen, ... up to the mode's first ringdown
     // bin (all bins where the mode vanishes: this wave then writes each
-    // bin's zero cotangents); the blocks wholly inside a plain mode's
+    // bin's zero cotangents); the blocks wholly inside the mode's
     // intermediate range [lo, hi) (its own waves') jumped over, the bins at
     // its edges tested
     const int* cut = cuts + 3 * (b * kHmModes + k);
@@ -2145,28 +2147,34 @@ __global__ __launch_bounds__(64) void xhm_hm_rd_adjoint_f64_kernel(
                                                  cot_plus, cot_cross, gh22, partial, gfm, cuts);
 }
 
-// one wave per (binary, chunk, plain higher mode) on the mode's
-// intermediate bins (hm_intermediate_bin in the cuts' range [lo, hi), the
-// bins that the general waves skip): the (2,1), (3,3), (4,4) (blocks (b,
-// c, 3 modes)) with 23 compact accumulators
+// one wave per (binary, chunk, higher mode) on the mode's intermediate bins
+// (hm_intermediate_bin in the cuts' range [lo, hi), the bins that the
+// general waves skip; blocks (b, c, 4 modes)) with 24 compact accumulators;
+// the (3,2)'s wave writes those bins' cotangent of the (2,2), zero (no
+// rotation there), as the general wave does for its bins
 template <class T, bool Grid>
 __device__ __forceinline__ void xhm_hm_inter_adjoint_wave(
     const T* __restrict__ head, const T* __restrict__ dhead, const T* __restrict__ modes,
     const T* __restrict__ dmodes, int64_t nb, const PowerRow<T>* __restrict__ table, int64_t n, int64_t chunks,
-    const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross, double* __restrict__ partial,
-    T* __restrict__ gfm, const int* __restrict__ cuts) {
+    int64_t waves, const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
+    vec2_t<T>* __restrict__ gh22, double* __restrict__ partial, T* __restrict__ gfm, const int* __restrict__ cuts) {
+    // waves (<= chunks) per binary and mode; chunk 0's zeroes the mode's
+    // partial sums of chunks waves .. chunks - 1
     const int64_t x = blockIdx.x;
-    const int km = (int)(x % 3);
-    const int k = km < 2 ? km : 3;
-    const int64_t c = (x / 3) % chunks;
-    const int64_t b = x / 3 / chunks;
+    const int k = (int)(x % kHmModes);
+    const int64_t c = (x / kHmModes) % waves;
+    const int64_t b = x / kHmModes / waves;
     if (b >= nb) return;
     const int lane = threadIdx.x;
+    if (c == 0 && lane < kXhmGradSlots)
+        for (int64_t rest = waves; rest < chunks; ++rest)
+            partial[((b * kXhmGradRows + 9 + k) * chunks + rest) * kXhmGradSlots + lane] = 0.0;
     __shared__ T s_mode[kHmSlots];
     for (int j = lane; j < kHmSlots; j += 64) s_mode[j] = modes[(b * kHmModes + k) * kHmSlots + j];
     __syncthreads();
     const T* r = s_mode;
-    double* out = partial + ((b * kXhmGradRows + 9 + km) * chunks + c) * kXhmGradSlots;
+    const bool writes = k == kHmMixMode;
+    double* out = partial + ((b * kXhmGradRows + 9 + k) * chunks + c) * kXhmGradSlots;
     if (r[kHmOn] == T(0)) {
         // the mode vanishes: the general wave writes its every bin
         if (lane < kXhmGradSlots) out[lane] = 0.0;
@@ -2183,8 +2191,8 @@ __device__ __forceinline__ void xhm_hm_inter_adjoint_wave(
     const int64_t lo = cut[1], hi = cut[2];
     // each iteration loads the next one's powers and cotangents before its
     // own arithmetic, so that their latency overlaps it
-    const int64_t step = chunks * 64;
-    int64_t i = first_bin_from(lo, c, chunks, lane);
+    const int64_t step = waves * 64;
+    int64_t i = first_bin_from(lo, c, waves, lane);
     PowerRow<T> next_row{};
     vec2_t<T> next_cp{}, next_cc{};
     if (i < hi) {
@@ -2210,6 +2218,7 @@ __device__ __forceinline__ void xhm_hm_inter_adjoint_wave(
         v4 += g_re * h_re + g_im * h_im;
         v6 += emm * (g_im * h_re - g_re * h_im);
         v5 += inclination_term(dfac, cp, cc, h_re, h_im);
+        if (writes) gh22[b * n + i] = vec2_t<T>{T(0), T(0)};
         if (Grid) gfm[(k * nb + b) * n + i] = gf;
     }
     // lane j: the total of accumulator j, contracted with its slot's
@@ -2230,19 +2239,21 @@ template <bool Grid>
 __global__ __launch_bounds__(64) void xhm_hm_inter_adjoint_kernel(
     const float* __restrict__ head, const float* __restrict__ dhead, const float* __restrict__ modes,
     const float* __restrict__ dmodes, int64_t nb, const PowerRow<float>* __restrict__ table, int64_t n,
-    int64_t chunks, const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross,
-    double* __restrict__ partial, float* __restrict__ gfm, const int* __restrict__ cuts) {
-    xhm_hm_inter_adjoint_wave<float, Grid>(head, dhead, modes, dmodes, nb, table, n, chunks, cot_plus, cot_cross,
-                                           partial, gfm, cuts);
+    int64_t chunks, int64_t waves, const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross,
+    float2* __restrict__ gh22, double* __restrict__ partial, float* __restrict__ gfm, const int* __restrict__ cuts) {
+    xhm_hm_inter_adjoint_wave<float, Grid>(head, dhead, modes, dmodes, nb, table, n, chunks, waves, cot_plus,
+                                           cot_cross, gh22, partial, gfm, cuts);
 }
 template <bool Grid>
 __global__ __launch_bounds__(64) void xhm_hm_inter_adjoint_f64_kernel(
     const double* __restrict__ head, const double* __restrict__ dhead, const double* __restrict__ modes,
     const double* __restrict__ dmodes, int64_t nb, const PowerRow<double>* __restrict__ table, int64_t n,
-    int64_t chunks, const double2* __restrict__ cot_plus, const double2* __restrict__ cot_cross,
-    double* __restrict__ partial, double* __restrict__ gfm, const int* __restrict__ cuts) {
-    xhm_hm_inter_adjoint_wave<double, Grid>(head, dhead, modes, dmodes, nb, table, n, chunks, cot_plus, cot_cross,
-                                            partial, gfm, cuts);
+    int64_t chunks, int64_t waves, const double2* __restrict__ cot_plus, const double2* __restrict__ cot_cross,
+    double2* __restrict__ gh22, double* __restrict__ partial, double* __restrict__ gfm,
+    const int* __restrict__ cuts) {
+    xhm_hm_inter_adjoint_wave<double, Grid>(head, dhead, modes, dmodes, nb, table, n, chunks, waves, cot_plus,
+                                            cot_cross,
+                                            gh22, partial, gfm, cuts);
 }
 
 // one wave per (binary, chunk): the (2,2)'s adjoint (the IMRPhenomXAS
@@ -2840,32 +2851,32 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
                 (const double*)dmodes, nb, (const PowerRow<double>*)table, n, g.chunks, g.gen_chunks,
                 (const double2*)cot_plus, (const double2*)cot_cross, (double2*)gh22, partial, nullptr, cuts);
     }
-    // the plain modes' intermediate waves
-    const int inter_blocks = 3 * blocks22;
+    // the intermediate waves
+    const int inter_blocks = (int)(kHmModes * nb * g.inter_chunks);
     if (std::is_same<T, float>::value) {
         const float *hd = (const float*)head, *dh = (const float*)dhead, *md = (const float*)modes,
                     *dm = (const float*)dmodes;
         const PowerRow<float>* tb = (const PowerRow<float>*)table;
         if (dldf)
             xhm_hm_inter_adjoint_kernel<true><<<inter_blocks, 64, 0, s>>>(
-                hd, dh, md, dm, nb, tb, n, g.chunks, (const float2*)cot_plus, (const float2*)cot_cross, partial,
-                (float*)gfm, cuts);
+                hd, dh, md, dm, nb, tb, n, g.chunks, g.inter_chunks, (const float2*)cot_plus,
+                (const float2*)cot_cross, (float2*)gh22, partial, (float*)gfm, cuts);
         else
             xhm_hm_inter_adjoint_kernel<false><<<inter_blocks, 64, 0, s>>>(
-                hd, dh, md, dm, nb, tb, n, g.chunks, (const float2*)cot_plus, (const float2*)cot_cross, partial,
-                nullptr, cuts);
+                hd, dh, md, dm, nb, tb, n, g.chunks, g.inter_chunks, (const float2*)cot_plus,
+                (const float2*)cot_cross, (float2*)gh22, partial, nullptr, cuts);
     } else {
         const double *hd = (const double*)head, *dh = (const double*)dhead, *md = (const double*)modes,
                      *dm = (const double*)dmodes;
         const PowerRow<double>* tb = (const PowerRow<double>*)table;
         if (dldf)
             xhm_hm_inter_adjoint_f64_kernel<true><<<inter_blocks, 64, 0, s>>>(
-                hd, dh, md, dm, nb, tb, n, g.chunks, (const double2*)cot_plus, (const double2*)cot_cross, partial,
-                (double*)gfm, cuts);
+                hd, dh, md, dm, nb, tb, n, g.chunks, g.inter_chunks, (const double2*)cot_plus,
+                (const double2*)cot_cross, (double2*)gh22, partial, (double*)gfm, cuts);
         else
             xhm_hm_inter_adjoint_f64_kernel<false><<<inter_blocks, 64, 0, s>>>(
-                hd, dh, md, dm, nb, tb, n, g.chunks, (const double2*)cot_plus, (const double2*)cot_cross, partial,
-                nullptr, cuts);
+                hd, dh, md, dm, nb, tb, n, g.chunks, g.inter_chunks, (const double2*)cot_plus,
+                (const double2*)cot_cross, (double2*)gh22, partial, nullptr, cuts);
     }
     // the ringdown waves: the plain modes, then the mixing (3,2) (its
     // cotangent of the (2,2) complete before the (2,2)'s waves)

@@ -475,21 +475,23 @@ __device__ __forceinline__ void hm_adjoint_split(const T* __restrict__ r, const 
 
 // ----------------------------------------------------- the intermediate
 
-// A plain mode's bins in the intermediate region of both its amplitude and
-// its phase (most of a plain mode's bins before its ringdown on a linear
-// grid): the amplitude's polynomial in u and the phase's closed form,
-// through M f, log(M f), (M f)^-1 and (M f)^(-7/6); 23 accumulators.
+// A mode's bins in the intermediate region of both its amplitude and its
+// phase (most of a mode's bins before its ringdown on a linear grid): the
+// amplitude's polynomial in u and the phase's closed form, through M f,
+// log(M f), (M f)^-1 and (M f)^(-7/6); 24 accumulators. Such a bin is below
+// both ringdown bounds, so the mixing mode does not rotate there: its
+// phase only adds the c3 term.
 enum InterAcc : int {
     iAmp0 = 0, iMsec = 1, iCbrtM = 2, iInvM = 3, iLogM = 4, iFRing = 5, iFDamp = 6, iInterC = 7, iInterH = 8,
     iInter = 9,  // 8
-    iC0 = 17, iC1 = 18, iC2 = 19, iC4 = 20, iCL = 21, iDeltaPhi = 22,
-    kInterAcc = 23
+    iC0 = 17, iC1 = 18, iC2 = 19, iC4 = 20, iCL = 21, iDeltaPhi = 22, iC3 = 23,
+    kInterAcc = 24
 };
 WFX_CONST int kInterAccSlot[kInterAcc] = {kHmAmp0, kHmMsec, kHmCbrtM, kHmInvM, kHmLogM, kHmFRing, kHmFDamp,
                                           kHmInterC, kHmInterH,
                                           kHmInter, kHmInter + 1, kHmInter + 2, kHmInter + 3, kHmInter + 4,
                                           kHmInter + 5, kHmInter + 6, kHmInter + 7,
-                                          kHmC0, kHmC1, kHmC2, kHmC4, kHmCL, kHmDeltaPhi};
+                                          kHmC0, kHmC1, kHmC2, kHmC4, kHmCL, kHmDeltaPhi, kHmC3};
 struct InterIx {
     static constexpr int FRing = iFRing, FDamp = iFDamp, Rd = -1, FalloffAt = -1;
 };
@@ -500,9 +502,8 @@ __device__ __forceinline__ bool hm_intermediate_bin(const T* __restrict__ r, T m
     return !(mf < r[kHmFAmpIn]) && !(mf >= r[kHmFAmpIm]) && !(mf < r[kHmFPhaseIn]) && !(mf >= r[kHmFPhaseIm]);
 }
 
-// hm_adjoint_split for a plain mode's intermediate bin (hm_intermediate_bin,
-// kHmMixing unset): the same operations, in the same order, on the 23
-// compact accumulators
+// hm_adjoint_split for a mode's intermediate bin (hm_intermediate_bin):
+// the same operations, in the same order, on the 24 compact accumulators
 template <bool Grid, class T>
 __device__ __forceinline__ void hm_adjoint_intermediate(const T* __restrict__ r, const BinPowers<T>& p, T g_re,
                                                         T g_im, T* acc, T& h_re, T& h_im, T& gf) {
@@ -526,10 +527,12 @@ __device__ __forceinline__ void hm_adjoint_intermediate(const T* __restrict__ r,
     T a = inter * m76;
     const bool clamped = a < T(0);
     if (clamped) a = T(1.0e-15);
+    const bool mixing = r[kHmMixing] != T(0);
     T rest = d_fma(r[kHmC0], mf, r[kHmC1] * lg);
     rest = d_fma(-r[kHmC2], m1, rest);
     const T m12 = m1 * m1;
     rest = d_fma(-(T(1) / T(3)) * r[kHmC4], m12 * m1, rest);
+    if (mixing) rest = d_fma(-T(0.5) * r[kHmC3], m12, rest);
     rest = d_fma(r[kHmCL], atan_z, rest);
     const T phi = d_reduce(rest + r[kHmDeltaPhi]);
     T sn, cs;
@@ -552,6 +555,10 @@ __device__ __forceinline__ void hm_adjoint_intermediate(const T* __restrict__ r,
     g.m1 -= g_phi * r[kHmC2];
     acc[iC4] -= g_phi * m12 * m1 * (T(1) / T(3));
     g.m1 -= g_phi * r[kHmC4] * m12;
+    if (mixing) {
+        acc[iC3] -= T(0.5) * g_phi * m12;
+        g.m1 -= g_phi * r[kHmC3] * m1;
+    }
     acc[iCL] += g_phi * atan_z;
     atan_z_adjoint<InterIx>(g_phi * r[kHmCL] * datan_z, z, inv_damp, acc, g);
     acc[iDeltaPhi] += g_phi;
