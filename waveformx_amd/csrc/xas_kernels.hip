@@ -1610,10 +1610,12 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
 // accumulators (xhm_hm_rd_adjoint_*: 16 slots for a plain mode, 26 for the
 // mixing (3,2)), so that more waves fit a SIMD; the general waves skip them.
 // So do a mode's bins in the intermediate region of both
-// (xhm_hm_inter_adjoint_*: 24 slots).
-// Per wave, partial[((b * 13 + row) * chunks + c) * 8 + j] (row 0 the (2,2),
+// (xhm_hm_inter_adjoint_*: 24 slots) and in the inspiral of both
+// (xhm_hm_ins_adjoint_*: 43 slots); the general waves keep the bins whose
+// amplitude and phase regions differ.
+// Per wave, partial[((b * 17 + row) * chunks + c) * 8 + j] (row 0 the (2,2),
 // 1..4 the higher modes' general waves, 5..8 their ringdown waves, 9..12
-// their intermediate waves):
+// their intermediate waves, 13..16 their inspiral waves):
 // Re conj(cot) dh/d(mass_1, mass_2, chi_1z, chi_2z)
 // (0..3); Re conj(G) h (4); Re conj(cot) dF/d(iota) h (5); Re m conj(G) i h
 // (6), G = conj(F+) cot+ + conj(Fx) cotx the mode's cotangent; 0 (7).
@@ -1621,7 +1623,9 @@ __global__ __launch_bounds__(kXhmBlock) void xhm_eval_split_kernel(const float* 
 // (gfm[k][b][i]), the (2,2)'s wave adds them to its own (into gfm[0][b][i]),
 // and two passes sum them over the binaries in a fixed order.
 constexpr int kXhmGradSlots = 8;
-constexpr int kXhmGradRows = 13;
+constexpr int kXhmGradRows = 17;
+// the ints per binary and mode of xhm_cuts_kernel
+constexpr int kCutsPer = 4;
 // at least this many higher-mode waves per launch (the chip holds ~3,000
 // at the kernels' occupancy), and at most this many blocks of 64
 // frequencies per wave: a wave's bins may all fall in one region, and the
@@ -1668,7 +1672,7 @@ inline XhmGradPlan xhm_grad_plan(int64_t nb, int64_t n, bool loss, bool grid, in
     g.table = off;
     off += align256(n * row);
     g.cuts = off;
-    off += align256((3 * nb * kHmModes + 1) * (int64_t)sizeof(int));
+    off += align256((kCutsPer * nb * kHmModes + 1) * (int64_t)sizeof(int));
     g.gh22 = off;
     off += align256(nb * n * 2 * real);
     g.partial = off;
@@ -1835,11 +1839,12 @@ __device__ inline int64_t wave_first_true(int64_t n, int lane, Pred pred) {
     return lo;
 }
 
-// One wave per (binary, mode): cuts[3 x] the first ringdown bin (-1: none
-// known, every wave tests every bin), and [cuts[3 x + 1], cuts[3 x + 2])
-// its intermediate bins (hm_intermediate_bin: from the first bin past both
-// inspiral bounds to the first bin at either ringdown bound; an empty range
-// where the cut is unknown)
+// One wave per (binary, mode), kCutsPer ints at cuts[kCutsPer x]: the first
+// ringdown bin (-1: none known, every wave tests every bin); [lo, hi) its
+// intermediate bins (hm_intermediate_bin: from the first bin past both
+// inspiral bounds to the first bin at either ringdown bound); and the end of
+// its inspiral bins (hm_inspiral_bin: those before the first bin past either
+// inspiral bound). Empty ranges where the cut is unknown.
 template <class T>
 __global__ __launch_bounds__(64) void xhm_cuts_kernel(const T* __restrict__ modes, int64_t nb,
                                                       const PowerRow<T>* __restrict__ table, int64_t n,
@@ -1849,8 +1854,12 @@ __global__ __launch_bounds__(64) void xhm_cuts_kernel(const T* __restrict__ mode
     if (x >= nb * kHmModes) return;
     const T* r = modes + x * kHmSlots;
     const T msec = r[kHmMsec];
-    int rd = -1, lo = 0, hi = 0;
+    int rd = -1, lo = 0, hi = 0, ins = 0;
     if (*ascends && msec > T(0) && msec <= std::numeric_limits<T>::max()) {
+        ins = (int)wave_first_true(n, lane, [&](int64_t i) {
+            const T mf = msec * table[i].f;
+            return !(mf < r[kHmFAmpIn] && mf < r[kHmFPhaseIn]);
+        });
         rd = (int)wave_first_true(n, lane, [&](int64_t i) { return hm_ringdown_bin(r, msec * table[i].f); });
         lo = (int)wave_first_true(n, lane, [&](int64_t i) {
             const T mf = msec * table[i].f;
@@ -1863,9 +1872,10 @@ __global__ __launch_bounds__(64) void xhm_cuts_kernel(const T* __restrict__ mode
         hi = hi < lo ? lo : hi;
     }
     if (lane == 0) {
-        cuts[3 * x] = rd;
-        cuts[3 * x + 1] = lo;
-        cuts[3 * x + 2] = hi;
+        cuts[kCutsPer * x] = rd;
+        cuts[kCutsPer * x + 1] = lo;
+        cuts[kCutsPer * x + 2] = hi;
+        cuts[kCutsPer * x + 3] = ins;
     }
 }
 
@@ -1922,11 +1932,14 @@ __device__ __forceinline__ void xhm_hm_adjoint_wave(
     // bin's zero cotangents); the blocks wholly inside the mode's
     // intermediate range [lo, hi) (its own waves') jumped over, the bins at
     // its edges tested
-    const int* cut = cuts + 3 * (b * kHmModes + k);
+    const int* cut = cuts + kCutsPer * (b * kHmModes + k);
     const int64_t end = (!on || cut[0] < 0) ? n : cut[0];
     const int64_t inter_lo = cut[1], inter_hi = cut[2];
     const int64_t skip_from = (inter_lo + 63) / 64, skip_to = inter_hi / 64;
-    for (int64_t blk = c; blk * 64 < end; blk += gen) {
+    // from the block of the inspiral's end (the inspiral waves' bins before it)
+    const int64_t ins_end = on ? cut[3] : 0;
+    bool took = false;
+    for (int64_t blk = first_bin_from(ins_end, c, gen, 0) / 64; blk * 64 < end; blk += gen) {
         if (on && blk >= skip_from && blk < skip_to) {
             blk += (skip_to - blk + gen - 1) / gen * gen;
             if (blk * 64 >= end) break;
@@ -1939,7 +1952,9 @@ __device__ __forceinline__ void xhm_hm_adjoint_wave(
             const BinPowers<T> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
             const T mf = r[kHmMsec] * p.f;
             if (hm_ringdown_bin(r, mf)) continue;  // the ringdown waves' bin
+            if (i < ins_end && hm_inspiral_bin(r, mf)) continue;  // the inspiral waves' bin
             if (i >= inter_lo && i < inter_hi && hm_intermediate_bin(r, mf)) continue;  // the intermediate waves' bin
+            took = true;
             const vec2_t<T> cp = cot_plus[b * n + i], cc = cot_cross[b * n + i];
             T g_re, g_im;
             mode_cotangent(fac, cp, cc, g_re, g_im);
@@ -1963,9 +1978,11 @@ __device__ __forceinline__ void xhm_hm_adjoint_wave(
         if (writes) gh22[b * n + i] = vec2_t<T>{gw_re * inv_amp0, gw_im * inv_amp0};
         if (Grid) gfm[(k * nb + b) * n + i] = gf;
     }
-    if (!on) {
+    if (!on || __ballot(took) == 0) {
         // a mode that vanishes (the odd modes of equal masses): nothing,
-        // whatever its record's derivatives (which may be NaN there)
+        // whatever its record's derivatives (which may be NaN there); a wave
+        // that took no bin (on an ascending grid the other kinds of wave
+        // usually take them all): zero sums, without the reductions
         if (lane < kXhmGradSlots) partial[((b * kXhmGradRows + 1 + k) * chunks + c) * kXhmGradSlots + lane] = 0.0;
         return;
     }
@@ -2055,7 +2072,7 @@ __device__ __forceinline__ void xhm_hm_rd_adjoint_wave(
     for (int j = 0; j < kAcc; ++j) acc[j] = T(0);
     T v4 = T(0), v5 = T(0), v6 = T(0);
     // from the block of the mode's first ringdown bin
-    const int cut = cuts[3 * (b * kHmModes + k)];
+    const int cut = cuts[kCutsPer * (b * kHmModes + k)];
     // a plain mode's iteration loads the next one's powers and cotangents
     // before its own arithmetic, so that their latency overlaps it (the
     // mixing mode's registers have no room for them: 167 -> 184, two waves
@@ -2187,7 +2204,7 @@ __device__ __forceinline__ void xhm_hm_inter_adjoint_wave(
 #pragma unroll
     for (int j = 0; j < kInterAcc; ++j) acc[j] = T(0);
     T v4 = T(0), v5 = T(0), v6 = T(0);
-    const int* cut = cuts + 3 * (b * kHmModes + k);
+    const int* cut = cuts + kCutsPer * (b * kHmModes + k);
     const int64_t lo = cut[1], hi = cut[2];
     // each iteration loads the next one's powers and cotangents before its
     // own arithmetic, so that their latency overlaps it
@@ -2254,6 +2271,95 @@ __global__ __launch_bounds__(64) void xhm_hm_inter_adjoint_f64_kernel(
     xhm_hm_inter_adjoint_wave<double, Grid>(head, dhead, modes, dmodes, nb, table, n, chunks, waves, cot_plus,
                                             cot_cross,
                                             gh22, partial, gfm, cuts);
+}
+
+// one wave per (binary, chunk, higher mode) on the mode's inspiral bins
+// (hm_inspiral_bin before the cuts' inspiral end, the bins that the general
+// waves skip; blocks (b, c, 4 modes)) with 43 compact accumulators; waves (<=
+// chunks) per binary and mode, chunk 0's zeroing the rest of its partial
+// sums; the (3,2)'s wave writes those bins' zero cotangent of the (2,2)
+template <class T, bool Grid>
+__device__ __forceinline__ void xhm_hm_ins_adjoint_wave(
+    const T* __restrict__ head, const T* __restrict__ dhead, const T* __restrict__ modes,
+    const T* __restrict__ dmodes, int64_t nb, const PowerRow<T>* __restrict__ table, int64_t n, int64_t chunks,
+    int64_t waves, const vec2_t<T>* __restrict__ cot_plus, const vec2_t<T>* __restrict__ cot_cross,
+    vec2_t<T>* __restrict__ gh22, double* __restrict__ partial, T* __restrict__ gfm, const int* __restrict__ cuts) {
+    const int64_t x = blockIdx.x;
+    const int k = (int)(x % kHmModes);
+    const int64_t c = (x / kHmModes) % waves;
+    const int64_t b = x / kHmModes / waves;
+    if (b >= nb) return;
+    const int lane = threadIdx.x;
+    if (c == 0 && lane < kXhmGradSlots)
+        for (int64_t rest = waves; rest < chunks; ++rest)
+            partial[((b * kXhmGradRows + 13 + k) * chunks + rest) * kXhmGradSlots + lane] = 0.0;
+    __shared__ T s_mode[kHmSlots];
+    for (int j = lane; j < kHmSlots; j += 64) s_mode[j] = modes[(b * kHmModes + k) * kHmSlots + j];
+    __syncthreads();
+    const T* r = s_mode;
+    const bool writes = k == kHmMixMode;
+    double* out = partial + ((b * kXhmGradRows + 13 + k) * chunks + c) * kXhmGradSlots;
+    const int64_t end = cuts[kCutsPer * (b * kHmModes + k) + 3];
+    if (r[kHmOn] == T(0) || c * 64 >= end) {
+        // the mode vanishes (the general wave writes its every bin), or no
+        // inspiral bin falls to this wave
+        if (lane < kXhmGradSlots) out[lane] = 0.0;
+        return;
+    }
+    const T* fac = head + b * kHeadSlots + kHeadFactors + 4 * (k + 1);
+    const T* dfac = dhead + b * kHeadSlots + kHeadFactors + 4 * (k + 1);
+    const T emm = (T)kHmEmm[k];
+    T acc[kInsAcc];
+#pragma unroll
+    for (int j = 0; j < kInsAcc; ++j) acc[j] = T(0);
+    T v4 = T(0), v5 = T(0), v6 = T(0);
+    for (int64_t i = c * 64 + lane; i < end; i += waves * 64) {
+        const PowerRow<T> row = table[i];
+        const BinPowers<T> p{row.f, row.t, row.m1, row.lf, row.m76, row.m53};
+        if (!hm_inspiral_bin(r, r[kHmMsec] * p.f)) continue;  // the general wave's bin
+        const vec2_t<T> cp = cot_plus[b * n + i], cc = cot_cross[b * n + i];
+        T g_re, g_im;
+        mode_cotangent(fac, cp, cc, g_re, g_im);
+        T h_re, h_im, gf = T(0);
+        hm_adjoint_inspiral<Grid>(r, p, g_re, g_im, acc, h_re, h_im, gf);
+        v4 += g_re * h_re + g_im * h_im;
+        v6 += emm * (g_im * h_re - g_re * h_im);
+        v5 += inclination_term(dfac, cp, cc, h_re, h_im);
+        if (writes) gh22[b * n + i] = vec2_t<T>{T(0), T(0)};
+        if (Grid) gfm[(k * nb + b) * n + i] = gf;
+    }
+    // lane j: the total of accumulator j, contracted with its slot's
+    // derivatives
+    T full[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) full[j] = j < kInsAcc ? acc[j] : T(0);
+    const double total = (double)reduce_scatter64(full, lane);
+    double d[4] = {0.0, 0.0, 0.0, 0.0};
+    if (lane < kInsAcc) {
+        const T* t = dmodes + ((b * kHmModes + k) * kHmSlots + kInsAccSlot[lane]) * 4;
+        for (int j = 0; j < 4; ++j) d[j] = t[j] == T(0) ? 0.0 : total * (double)t[j];
+    }
+    xhm_grad_store(d, v4, v5, v6, lane, out);
+}
+
+template <bool Grid>
+__global__ __launch_bounds__(64) void xhm_hm_ins_adjoint_kernel(
+    const float* __restrict__ head, const float* __restrict__ dhead, const float* __restrict__ modes,
+    const float* __restrict__ dmodes, int64_t nb, const PowerRow<float>* __restrict__ table, int64_t n,
+    int64_t chunks, int64_t waves, const float2* __restrict__ cot_plus, const float2* __restrict__ cot_cross,
+    float2* __restrict__ gh22, double* __restrict__ partial, float* __restrict__ gfm, const int* __restrict__ cuts) {
+    xhm_hm_ins_adjoint_wave<float, Grid>(head, dhead, modes, dmodes, nb, table, n, chunks, waves, cot_plus,
+                                         cot_cross, gh22, partial, gfm, cuts);
+}
+template <bool Grid>
+__global__ __launch_bounds__(64) void xhm_hm_ins_adjoint_f64_kernel(
+    const double* __restrict__ head, const double* __restrict__ dhead, const double* __restrict__ modes,
+    const double* __restrict__ dmodes, int64_t nb, const PowerRow<double>* __restrict__ table, int64_t n,
+    int64_t chunks, int64_t waves, const double2* __restrict__ cot_plus, const double2* __restrict__ cot_cross,
+    double2* __restrict__ gh22, double* __restrict__ partial, double* __restrict__ gfm,
+    const int* __restrict__ cuts) {
+    xhm_hm_ins_adjoint_wave<double, Grid>(head, dhead, modes, dmodes, nb, table, n, chunks, waves, cot_plus,
+                                          cot_cross, gh22, partial, gfm, cuts);
 }
 
 // one wave per (binary, chunk): the (2,2)'s adjoint (the IMRPhenomXAS
@@ -2814,7 +2920,7 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
     power_table_kernel<T, T><<<table_blocks, kBlock, 0, s>>>(freqs, n, table);
     // the cuts, then the grid's order flag after them
     int* cuts = (int*)(ws + g.cuts);
-    int* ascends = cuts + 3 * nb * kHmModes;
+    int* ascends = cuts + kCutsPer * nb * kHmModes;
     grid_order_kernel<T><<<1, kOrderBlock, 0, s>>>(table, n, ascends);
     xhm_cuts_kernel<T><<<(int)(nb * kHmModes), 64, 0, s>>>(modes, nb, table, n, ascends, cuts);
     if (weights) {
@@ -2851,7 +2957,7 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
                 (const double*)dmodes, nb, (const PowerRow<double>*)table, n, g.chunks, g.gen_chunks,
                 (const double2*)cot_plus, (const double2*)cot_cross, (double2*)gh22, partial, nullptr, cuts);
     }
-    // the intermediate waves
+    // the intermediate and the inspiral waves
     const int inter_blocks = (int)(kHmModes * nb * g.inter_chunks);
     if (std::is_same<T, float>::value) {
         const float *hd = (const float*)head, *dh = (const float*)dhead, *md = (const float*)modes,
@@ -2865,6 +2971,15 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
             xhm_hm_inter_adjoint_kernel<false><<<inter_blocks, 64, 0, s>>>(
                 hd, dh, md, dm, nb, tb, n, g.chunks, g.inter_chunks, (const float2*)cot_plus,
                 (const float2*)cot_cross, (float2*)gh22, partial, nullptr, cuts);
+        // the inspiral waves
+        if (dldf)
+            xhm_hm_ins_adjoint_kernel<true><<<inter_blocks, 64, 0, s>>>(
+                hd, dh, md, dm, nb, tb, n, g.chunks, g.inter_chunks, (const float2*)cot_plus,
+                (const float2*)cot_cross, (float2*)gh22, partial, (float*)gfm, cuts);
+        else
+            xhm_hm_ins_adjoint_kernel<false><<<inter_blocks, 64, 0, s>>>(
+                hd, dh, md, dm, nb, tb, n, g.chunks, g.inter_chunks, (const float2*)cot_plus,
+                (const float2*)cot_cross, (float2*)gh22, partial, nullptr, cuts);
     } else {
         const double *hd = (const double*)head, *dh = (const double*)dhead, *md = (const double*)modes,
                      *dm = (const double*)dmodes;
@@ -2875,6 +2990,14 @@ void xhm_adjoint_launch(const XhmGradPlan& g, const double* params, const T* rec
                 (const double2*)cot_cross, (double2*)gh22, partial, (double*)gfm, cuts);
         else
             xhm_hm_inter_adjoint_f64_kernel<false><<<inter_blocks, 64, 0, s>>>(
+                hd, dh, md, dm, nb, tb, n, g.chunks, g.inter_chunks, (const double2*)cot_plus,
+                (const double2*)cot_cross, (double2*)gh22, partial, nullptr, cuts);
+        if (dldf)
+            xhm_hm_ins_adjoint_f64_kernel<true><<<inter_blocks, 64, 0, s>>>(
+                hd, dh, md, dm, nb, tb, n, g.chunks, g.inter_chunks, (const double2*)cot_plus,
+                (const double2*)cot_cross, (double2*)gh22, partial, (double*)gfm, cuts);
+        else
+            xhm_hm_ins_adjoint_f64_kernel<false><<<inter_blocks, 64, 0, s>>>(
                 hd, dh, md, dm, nb, tb, n, g.chunks, g.inter_chunks, (const double2*)cot_plus,
                 (const double2*)cot_cross, (double2*)gh22, partial, nullptr, cuts);
     }

@@ -588,6 +588,150 @@ __device__ __forceinline__ void hm_adjoint_intermediate(const T* __restrict__ r,
     }
 }
 
+// ---------------------------------------------------------- the inspiral
+
+// A mode's bins in the inspiral of both its amplitude and its phase (below
+// both inspiral bounds, and for the mixing mode below both ringdown bounds,
+// so that it does not rotate): the PN and pseudo-PN amplitude and the phase's
+// polynomial in (M f)^(1/3) with its logarithms and leading term; 43
+// accumulators.
+enum InsAcc : int {
+    nAmp0 = 0, nMsec = 1, nCbrtM = 2, nInvM = 3, nLogM = 4,
+    nT0 = 5, nKIns = 6, nC1Insp = 7, nPhiL5 = 8, nPhiL6 = 9, nPhiL8 = 10,
+    nPhi = 11,         // 12: phi_1 .. phi_12
+    nPnGlobal = 23, nAmpNorm = 24, nPnDominant = 25,
+    nPnRe = 26,        // 7
+    nPnIm = 33,        // 7
+    nIns = 40,         // 3
+    kInsAcc = 43
+};
+WFX_CONST int kInsAccSlot[kInsAcc] = {
+    kHmAmp0, kHmMsec, kHmCbrtM, kHmInvM, kHmLogM,
+    kHmT0F64, kHmKInsF64, kHmC1Insp, kHmPhiL + 5, kHmPhiL + 6, kHmPhiL + 8,
+    kHmPhi + 1, kHmPhi + 2, kHmPhi + 3, kHmPhi + 4, kHmPhi + 5, kHmPhi + 6, kHmPhi + 7, kHmPhi + 8, kHmPhi + 9,
+    kHmPhi + 10, kHmPhi + 11, kHmPhi + 12,
+    kHmPnGlobal, kHmAmpNorm, kHmPnDominant,
+    kHmPnRe, kHmPnRe + 1, kHmPnRe + 2, kHmPnRe + 3, kHmPnRe + 4, kHmPnRe + 5, kHmPnRe + 6,
+    kHmPnIm, kHmPnIm + 1, kHmPnIm + 2, kHmPnIm + 3, kHmPnIm + 4, kHmPnIm + 5, kHmPnIm + 6,
+    kHmIns, kHmIns + 1, kHmIns + 2,
+};
+
+// the predicate of those bins (the regions' tests of hm_adjoint_split)
+template <class T>
+__device__ __forceinline__ bool hm_inspiral_bin(const T* __restrict__ r, T mf) {
+    const bool rotating = r[kHmMixing] != T(0) && (mf >= r[kHmFAmpIm] || mf >= r[kHmFPhaseIm]);
+    return mf < r[kHmFAmpIn] && mf < r[kHmFPhaseIn] && !rotating;
+}
+
+// hm_adjoint_split for a mode's inspiral bin (hm_inspiral_bin): the same
+// operations, in the same order, on the 43 compact accumulators
+template <bool Grid, class T>
+__device__ __forceinline__ void hm_adjoint_inspiral(const T* __restrict__ r, const BinPowers<T>& p, T g_re, T g_im,
+                                                    T* acc, T& h_re, T& h_im, T& gf) {
+    const T mf = r[kHmMsec] * p.f;
+    const T t = p.t * r[kHmCbrtM];
+    const T m1 = p.m1 * r[kHmInvM];
+    const T lg = p.lf + r[kHmLogM];
+    const T rs = d_rsq(t);
+    const T m76 = m1 * rs;
+    T pn_re = r[kHmPnRe + 6], pn_im = r[kHmPnIm + 6];
+#pragma unroll
+    for (int k = 5; k >= 0; --k) {
+        pn_re = d_fma(pn_re, t, r[kHmPnRe + k]);
+        pn_im = d_fma(pn_im, t, r[kHmPnIm + k]);
+    }
+    const T t2 = t * t;
+    const T t7 = t2 * t2 * t2 * t;
+    const T ins_poly = d_fma(d_fma(r[kHmIns + 2], t, r[kHmIns + 1]), t, r[kHmIns]);
+    const T pseudo = t7 * ins_poly;
+    const T mag = d_sqrt(d_fma(pn_re, pn_re, pn_im * pn_im));
+    T a = m76 * d_fma(mag, r[kHmPnGlobal] * r[kHmAmpNorm], pseudo * r[kHmPnDominant]);
+    const bool clamped = a < T(0);
+    if (clamped) a = T(1.0e-15);
+    T poly = r[kHmPhi + 12];
+#pragma unroll
+    for (int i = 11; i >= 1; --i) poly = d_fma(poly, t, r[kHmPhi + i]);
+    const T inv_t = d_div(T(1), t);
+    const T m43 = m1 * inv_t;
+    T rest = poly * m43;
+    const T logs = d_fma(d_fma(r[kHmPhiL + 8], t * t, T(0)), t, d_fma(r[kHmPhiL + 6], t, r[kHmPhiL + 5]));
+    rest = d_fma(lg, logs, rest);
+    rest = d_fma(r[kHmC1Insp], mf, rest);
+    const double full = fma(wide_slot(r, kHmT0F64), p.m53, wide_slot(r, kHmKInsF64));
+    const T phi = d_reduce(lead_turned(T(0), full) + rest);
+    T sn, cs;
+    m_sincos(phi, &sn, &cs);
+    const T amp0 = r[kHmAmp0];
+    const T val = amp0 * a;
+    h_re = val * cs;
+    h_im = val * sn;
+
+    const T g_val = d_fma(g_re, cs, g_im * sn);
+    const T g_phi = val * d_fma(g_im, cs, -g_re * sn);
+    acc[nAmp0] += g_val * a;
+    const T g_a = clamped ? T(0) : g_val * amp0;
+    MfGrad<T> g{T(0), T(0), T(0), T(0), T(0)};
+    acc[nT0] += g_phi * (T)p.m53;
+    acc[nKIns] += g_phi;
+    acc[nC1Insp] += g_phi * mf;
+    g.mf += g_phi * r[kHmC1Insp];
+    g.lg += g_phi * logs;
+    const T g_logs = g_phi * lg;
+    acc[nPhiL5] += g_logs;
+    acc[nPhiL6] += g_logs * t;
+    acc[nPhiL8] += g_logs * t2 * t;
+    g.t += g_logs * d_fma(T(3) * r[kHmPhiL + 8], t2, r[kHmPhiL + 6]);
+    const T g_poly = g_phi * m43;
+    const T g_m43 = g_phi * poly;
+    g.m1 += g_m43 * inv_t;
+    g.t -= g_m43 * m43 * inv_t;
+    T pw = T(1), dpoly = T(0);
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+        acc[nPhi + j] += g_poly * pw;
+        if (j < 11) dpoly = d_fma((T)(j + 1) * r[kHmPhi + j + 2], pw, dpoly);
+        pw *= t;
+    }
+    g.t += g_poly * dpoly;
+    const T glob = r[kHmPnGlobal] * r[kHmAmpNorm], dom = r[kHmPnDominant];
+    g.m76 += g_a * d_fma(mag, glob, pseudo * dom);
+    const T g_in = g_a * m76;
+    acc[nPnGlobal] += g_in * mag * r[kHmAmpNorm];
+    acc[nAmpNorm] += g_in * mag * r[kHmPnGlobal];
+    acc[nPnDominant] += g_in * pseudo;
+    const T g_mag = g_in * glob;
+    const T inv_mag = mag > T(0) ? d_rcp(mag) : T(0);
+    const T g_pre = g_mag * pn_re * inv_mag, g_pim = g_mag * pn_im * inv_mag;
+    T qw = T(1), d_re = T(0), d_im = T(0);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        acc[nPnRe + k] += g_pre * qw;
+        acc[nPnIm + k] += g_pim * qw;
+        if (k < 6) {
+            d_re = d_fma((T)(k + 1) * r[kHmPnRe + k + 1], qw, d_re);
+            d_im = d_fma((T)(k + 1) * r[kHmPnIm + k + 1], qw, d_im);
+        }
+        qw *= t;
+    }
+    g.t += d_fma(g_pre, d_re, g_pim * d_im);
+    const T g_ps = g_in * dom;
+    acc[nIns] += g_ps * t7;
+    acc[nIns + 1] += g_ps * t7 * t;
+    acc[nIns + 2] += g_ps * t7 * t * t;
+    const T t6 = t7 * d_div(T(1), t);
+    g.t += g_ps * d_fma(T(7) * t6, ins_poly, t7 * d_fma(T(2) * r[kHmIns + 2], t, r[kHmIns + 1]));
+    g.m1 += g.m76 * rs;
+    g.t -= T(0.5) * g.m76 * m76 * d_div(T(1), t);
+    acc[nMsec] += g.mf * p.f;
+    acc[nCbrtM] += g.t * p.t;
+    acc[nInvM] += g.m1 * p.m1;
+    acc[nLogM] += g.lg;
+    if (Grid) {
+        const T lead = (T)(-5.0 / 3.0 * wide_slot(r, kHmT0F64) * p.m53) * g_phi;
+        gf = d_fma(g.mf, r[kHmMsec], (d_fma(g.t, t * (T(1) / T(3)), g.lg - g.m1 * m1) + lead) * p.m1);
+    }
+}
+
 // --------------------------------------------------------- the ringdown
 
 // The ringdown adjoints' compact accumulators. A plain mode's ringdown (the
