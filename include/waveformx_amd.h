@@ -327,9 +327,13 @@ int wfx_xhm_setup_jvp(const double* params, int64_t n_binaries, const double* f_
  * receives L = sum_f weights (|h+ - d+|^2 + |hx - dx|^2), the gradient being
  * L's. dldf (N,) double, if not NULL, receives dL/df of each frequency (the
  * grid's gradient, summed over the binaries). Reverse mode: one wave per
- * span of frequencies and mode of a binary recomputes the mode, sums each
- * bin's dL/d(record slot) in registers and contracts the totals with the
- * records' derivatives once; nothing per bin is kept but the (3,2)'s
+ * span of frequencies, mode and region (inspiral, intermediate, ringdown,
+ * the rest: on an ascending grid each takes only its own blocks) of a
+ * binary recomputes the mode, sums each bin's dL/d(record slot) in
+ * registers and contracts the totals with the records' derivatives once
+ * (on a grid in any other order every wave tests every bin: the same
+ * gradient up to the sums' grouping); nothing per bin is kept but the
+ * (3,2)'s
  * cotangent of the (2,2) it recycles (and, for the loss, the residuals; for
  * dldf, each bin's dL/df per mode). workspace: device memory of at least
  * wfx_xhm_adjoint_workspace(B, N, precision, weights != NULL, dldf != NULL)

@@ -660,6 +660,47 @@ def test_single_mode_gradient(cuda):
         assert np.all(excess <= 1.0), (k, excess)
 
 
+def test_gradient_on_a_permuted_grid(cuda):
+    """The adjoint's region waves on an ascending grid (each kind of wave
+    takes only its own blocks, between the cuts found by search) against
+    the same bins in a shuffled order (no cuts: every wave tests every bin):
+    the same loss and gradient up to the sums' grouping, in both arithmetics,
+    and dL/df permuted with the bins; a grid that ascends but for one pair of
+    bins takes the shuffled path too."""
+    import torch
+
+    from waveformx_amd import xhm
+
+    params = oxhm.draw(6, seed=31)
+    p = torch.as_tensor(params, device=cuda)
+    perturbed = params * (1.0 + 1e-2 * np.random.default_rng(9).normal(size=params.shape))
+    perturbed[:, 7] = params[:, 7]
+    freqs = np.linspace(20.0, 1024.0, 1500)
+    w = 1.0 / ((freqs / 100.0) ** -4 + 1.0 + (freqs / 100.0) ** 2)
+    order = np.random.default_rng(10).permutation(freqs.size)
+    swapped = np.arange(freqs.size)
+    swapped[[700, 701]] = swapped[[701, 700]]
+    for precision, rdtype, ctype, tol in (("fp64", torch.float64, torch.complex128, 1e-11),
+                                          ("split", torch.float32, torch.complex64, 1e-5)):
+        f = torch.as_tensor(freqs, dtype=rdtype, device=cuda)
+        wt = torch.as_tensor(w, dtype=rdtype, device=cuda)
+        dp, dc = xhm.polarizations(torch.as_tensor(perturbed, device=cuda), f.double())
+        dp, dc = dp.to(ctype), dc.to(ctype)
+        grads, loss, dldf = xhm.adjoint(p, f, dp, dc, wt, precision=precision, wrt_grid=True)
+        for perm in (order, swapped):
+            idx = torch.as_tensor(perm, device=cuda)
+            # the shuffled grid's first frequency is not f_min: the set-up takes the sorted grid's
+            j = xhm.setup_jvp(p, f[:1].double(), precision=precision)
+            g2, l2, d2 = xhm.adjoint_with(j, p, f[idx], dp[:, idx], dc[:, idx], wt[idx], precision=precision,
+                                          wrt_grid=True)
+            err = (g2 - grads).abs().amax(0) / grads.abs().amax(0)
+            print(precision, "permuted grid: gradient", err.max().item(), flush=True)
+            assert bool((err[:7] <= tol).all()), err
+            np.testing.assert_allclose(l2.cpu().numpy(), loss.cpu().numpy(), rtol=tol)
+            np.testing.assert_allclose(d2.cpu().numpy(), dldf[idx].cpu().numpy(), rtol=0,
+                                       atol=tol * dldf.abs().max().item())
+
+
 def test_split_gradient(cuda):
     """The exact split gradient (xhm.adjoint: the dual-number set-up and
     the reverse-mode adjoint of the split evaluation) for random cotangents
