@@ -1843,8 +1843,8 @@ __device__ inline int64_t wave_first_true(int64_t n, int lane, Pred pred) {
 // ringdown bin (-1: none known, every wave tests every bin); [lo, hi) its
 // intermediate bins (hm_intermediate_bin: from the first bin past both
 // inspiral bounds to the first bin at either ringdown bound); and the end of
-// its inspiral bins (hm_inspiral_bin: those before the first bin past either
-// inspiral bound). Empty ranges where the cut is unknown.
+// its inspiral bins (hm_inspiral_bin holds on the bins before it and on no
+// bin from it). Empty ranges where the cut is unknown.
 template <class T>
 __global__ __launch_bounds__(64) void xhm_cuts_kernel(const T* __restrict__ modes, int64_t nb,
                                                       const PowerRow<T>* __restrict__ table, int64_t n,
@@ -1856,10 +1856,9 @@ __global__ __launch_bounds__(64) void xhm_cuts_kernel(const T* __restrict__ mode
     const T msec = r[kHmMsec];
     int rd = -1, lo = 0, hi = 0, ins = 0;
     if (*ascends && msec > T(0) && msec <= std::numeric_limits<T>::max()) {
-        ins = (int)wave_first_true(n, lane, [&](int64_t i) {
-            const T mf = msec * table[i].f;
-            return !(mf < r[kHmFAmpIn] && mf < r[kHmFPhaseIn]);
-        });
+        // the inspiral predicate is a conjunction of tests "M f below a
+        // bound": true on the bins before some bin, false from it
+        ins = (int)wave_first_true(n, lane, [&](int64_t i) { return !hm_inspiral_bin(r, msec * table[i].f); });
         rd = (int)wave_first_true(n, lane, [&](int64_t i) { return hm_ringdown_bin(r, msec * table[i].f); });
         lo = (int)wave_first_true(n, lane, [&](int64_t i) {
             const T mf = msec * table[i].f;
