@@ -897,16 +897,54 @@ __global__ __launch_bounds__(kAdjointBlock) void adjoint_kernel(const T* __restr
     T acc[kSlots];
 #pragma unroll
     for (int k = 0; k < kSlots; ++k) acc[k] = T(0);
+    // IMRPhenomXAS's fused loss in FP32: each iteration loads the next one's
+    // powers, inputs and weight before its own arithmetic, so that their
+    // latency overlaps it (0.326 -> 0.307 ms for the loss and gradient at
+    // 8192 x 4017, the same bits; the other variants lose a wave per SIMD to
+    // the registers it takes: IMRPhenomD's loss 2% slower, FP64 has none to
+    // spare)
+    constexpr bool kAhead =
+        std::is_same<T, float>::value && std::is_same<A, XasAdjoint>::value && Mode == kLoss;
+    PowerRow<T> next_row{};
+    T next_in[4] = {T(0), T(0), T(0), T(0)};
+    T next_w = T(0);
+    auto fetch = [&](int64_t j) __attribute__((always_inline)) {
+        if (table) next_row = table[j];
+        const int64_t o = 2 * (b * n + j);
+        next_in[0] = in_plus[o];
+        next_in[1] = in_plus[o + 1];
+        next_in[2] = in_cross[o];
+        next_in[3] = in_cross[o + 1];
+        if (Mode == kLoss) next_w = weights[j];
+    };
+    if (kAhead && lane < n) fetch(lane);
     for (int64_t i = lane; i < n; i += kAdjointBlock) {
-        const BinPowers<T> p = powers_at<T>(freqs, table, i);
-        const int64_t o = 2 * (b * n + i);
-        T gp_re = in_plus[o], gp_im = in_plus[o + 1], gx_re = in_cross[o], gx_im = in_cross[o + 1];
+        BinPowers<T> p;
+        T gp_re, gp_im, gx_re, gx_im, w_i = T(0);
+        if constexpr (kAhead) {
+            p = table ? BinPowers<T>{next_row.f, next_row.t, next_row.m1, next_row.lf, next_row.m76, next_row.m53}
+                      : adjoint_powers<T>((double)freqs[i]);
+            gp_re = next_in[0];
+            gp_im = next_in[1];
+            gx_re = next_in[2];
+            gx_im = next_in[3];
+            w_i = next_w;
+            if (i + kAdjointBlock < n) fetch(i + kAdjointBlock);
+        } else {
+            p = powers_at<T>(freqs, table, i);
+            const int64_t o = 2 * (b * n + i);
+            gp_re = in_plus[o];
+            gp_im = in_plus[o + 1];
+            gx_re = in_cross[o];
+            gx_im = in_cross[o + 1];
+            if (Mode == kLoss) w_i = weights[i];
+        }
         if (Mode == kLoss) {
             T amp;
             const T phi = A::forward(c, p, amp);
             T s, co;
             m_sincos(phi, &s, &co);
-            const T w = weights[i];
+            const T w = w_i;
             const T ap = c[A::kKpSlot] * amp, ac = c[A::kKcSlot] * amp;
             const T rp_re = ap * co - gp_re, rp_im = ap * s - gp_im;
             const T rx_re = -ac * s - gx_re, rx_im = ac * co - gx_im;
